@@ -1,0 +1,2290 @@
+// SPDX-License-Identifier: GPL-3.0-or-later
+//
+// CPU FP64 oracle for the PM7 NDDO SCF single point behind pm7-rs `run_pm7`.
+//
+// TEST INFRASTRUCTURE ONLY: this is the checker, never the product. Only tests/, the smoke entry
+// and bench.py's cpu_baseline leg load it. It restates, function by function, the reference's
+// algorithm; every routine cites the pm7-rs source it follows (paths relative to the pm7-rs
+// tree, e.g. src/scf.rs:1189). The reference itself is Rust and cannot be built in this image,
+// so this restatement is pinned instead by the reference's golden vectors
+// (tools/oracle/baselines/reference.json), by the reference's unit-test constants and by closed
+// forms; see tests/test_oracle_*.py.
+//
+// Scope of this restatement: the sp Dewar-Sabelli-Klopman path (every element whose valence
+// shell is s or sp with principal quantum number <= 3: H He Li Be B C N O F Ne Na Mg), RHF and
+// UHF, A-DIIS -> CDIIS, the level-shift stall guard, core-core repulsion, D2 dispersion, the EH+
+// hydrogen-bond term, the MM corrections, Mulliken charges, the dipole breakdown and the heat of
+// formation. Molecules holding a d-bearing element (Al..Cl and heavier) are reported as
+// unsupported (status 3): the MNDO/d two-centre kernel is not restated here.
+
+#include "pm7_oracle.h"
+
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <set>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+namespace {
+
+// ---------------------------------------------------------------------------------------------
+// Constants (src/constants.rs:14-27): CODATA-2018 values MOPAC v23.2.5 uses.
+// ---------------------------------------------------------------------------------------------
+constexpr double EV = 27.211386245988;          // eV per Hartree
+constexpr double A0 = 0.529177210903;           // Bohr radius, Angstrom
+constexpr double EV_TO_KCAL = 23.06054783061903;
+constexpr double KCAL_TO_EV = 1.0 / EV_TO_KCAL;
+constexpr double AU_DIPOLE_TO_DEBYE = 2.541746473;
+constexpr double PI = 3.14159265358979323846;
+
+// Atomic masses (u) for the centre-of-mass dipole origin (src/data_tables.rs MASS, first 36).
+const double MASS[37] = {0.0,     1.0079,  4.0026,  6.94,    9.01218, 10.81,   12.011,  14.0067,
+                         15.9994, 18.9984, 20.179,  22.98977, 24.305, 26.98154, 28.0855, 30.97376,
+                         32.06,   35.453,  39.948,  39.098,  40.078,  44.956,  47.867,  50.942,
+                         51.996,  54.938,  55.845,  58.933,  58.693,  63.546,  65.38,   69.723,
+                         72.63,   74.922,  78.971,  79.904,  83.798};
+
+// ---------------------------------------------------------------------------------------------
+// Tiny deterministic parallel-for: static contiguous chunks, each output written by one thread.
+// ---------------------------------------------------------------------------------------------
+int g_threads = 1;
+
+template <class F>
+void parallel_for(long n, F&& body) {
+  int t = std::max(1, std::min<int>(g_threads, (int)std::max<long>(1, n / 8)));
+  if (t <= 1 || n < 64) {
+    for (long i = 0; i < n; ++i) body(i);
+    return;
+  }
+  std::vector<std::thread> pool;
+  long chunk = (n + t - 1) / t;
+  for (int k = 0; k < t; ++k) {
+    long lo = k * chunk, hi = std::min(n, lo + chunk);
+    if (lo >= hi) break;
+    pool.emplace_back([lo, hi, &body]() {
+      for (long i = lo; i < hi; ++i) body(i);
+    });
+  }
+  for (auto& th : pool) th.join();
+}
+
+// ---------------------------------------------------------------------------------------------
+// Parameters (src/params.rs:205-432, 604-652)
+// ---------------------------------------------------------------------------------------------
+struct Element {
+  int z = 0, n = 0, ios = 0, iop = 0, iod = 0, n_orb = 0;
+  double core_charge = 0, eheat_ev = 0, u_ss = 0, u_pp = 0, u_dd = 0;
+  double zeta_s = 0, zeta_p = 0, zeta_d = 0, beta_s = 0, beta_p = 0, beta_d = 0;
+  double g_ss = 0, g_sp = 0, g_pp = 0, g_p2 = 0, h_sp = 0;
+  double zsn = 0, zpn = 0, zdn = 0, f0sd = 0, g2sd = 0, poc = 0;
+  std::vector<std::array<double, 3>> gauss;  // (K, L, M)
+  double e_isol = 0, dd = 0, qq = 0, rho0 = 0, rho1 = 0, rho2 = 0;
+  bool has_p() const { return n_orb >= 4; }
+  double po9() const { return poc > 1.0e-5 ? poc : rho0; }  // src/params.rs:80-87
+};
+
+struct Params {
+  int method = 0;
+  std::map<int, Element> el;
+  std::map<std::pair<int, int>, std::pair<double, double>> pairs;  // (hi,lo) -> (alpb,xfac)
+  double vpar[61] = {0};
+  const Element* get(int z) const {
+    auto it = el.find(z);
+    return it == el.end() ? nullptr : &it->second;
+  }
+  // src/params.rs:172-197: explicit pair, else average of the two diagonal pairs.
+  std::pair<double, double> pair(int zi, int zj) const {
+    auto key = std::make_pair(std::max(zi, zj), std::min(zi, zj));
+    auto it = pairs.find(key);
+    if (it != pairs.end()) return it->second;
+    auto ii = pairs.find({zi, zi});
+    auto jj = pairs.find({zj, zj});
+    std::pair<double, double> a = ii == pairs.end() ? std::make_pair(0.0, 0.0) : ii->second;
+    std::pair<double, double> b = jj == pairs.end() ? std::make_pair(0.0, 0.0) : jj->second;
+    return {0.5 * (a.first + b.first), 0.5 * (a.second + b.second)};
+  }
+};
+
+// MOPAC ddpo.F90 charge separations (src/params.rs:602-612).
+double dd_charge_sep(int n, double zs, double zp) {
+  double nn = n;
+  return (2.0 * nn + 1.0) * std::pow(4.0 * zs * zp, nn + 0.5) / std::pow(zs + zp, 2.0 * nn + 2.0) /
+         std::sqrt(3.0);
+}
+double qq_charge_sep(int n, double zp) {
+  double nn = n;
+  return std::sqrt((4.0 * nn * nn + 6.0 * nn + 2.0) / 20.0) / zp;
+}
+// MOPAC calpar.F90 fixed 5-step secant (src/params.rs:621-636).
+template <class F>
+double calpar_secant(double target, double guess, F f) {
+  double d1 = guess, d2 = guess + 0.04;
+  for (int it = 0; it < 5; ++it) {
+    double f1 = f(d1), f2 = f(d2);
+    if (std::fabs(f2 - f1) < 1.0e-25) break;
+    double d3 = d1 + (d2 - d1) * (target - f1) / (f2 - f1);
+    d1 = d2;
+    d2 = d3;
+  }
+  return d2;
+}
+double signum(double x) { return std::signbit(x) ? -1.0 : 1.0; }
+double additive_rho1(double hsp_ev, double dd) {  // src/params.rs:638-643
+  double hsp = hsp_ev / EV;
+  auto f = [dd](double d) { return 0.5 * d - 0.5 / std::sqrt(4.0 * dd * dd + 1.0 / (d * d)); };
+  double guess = std::pow(std::fabs(hsp) / (dd * dd), 1.0 / 3.0) * signum(hsp);
+  return 0.5 / calpar_secant(hsp, guess, f);
+}
+double additive_rho2(double hpp_ev, double qq) {  // src/params.rs:645-652
+  double hpp = hpp_ev / EV;
+  auto f = [qq](double q) {
+    return 0.25 * q - 0.5 / std::sqrt(4.0 * qq * qq + 1.0 / (q * q)) +
+           0.25 / std::sqrt(8.0 * qq * qq + 1.0 / (q * q));
+  };
+  double guess = std::pow(std::fabs(hpp) / (3.0 * std::pow(qq, 4)), 0.2) * signum(hpp);
+  return 0.5 / calpar_secant(hpp, guess, f);
+}
+
+std::vector<std::vector<std::string>> read_tsv(const std::string& path) {
+  std::vector<std::vector<std::string>> rows;
+  FILE* fp = std::fopen(path.c_str(), "r");
+  if (!fp) return rows;
+  char line[4096];
+  while (std::fgets(line, sizeof line, fp)) {
+    if (line[0] == '#' || line[0] == '\n') continue;
+    std::vector<std::string> fields;
+    char* save = nullptr;
+    for (char* tok = strtok_r(line, "\t\n", &save); tok; tok = strtok_r(nullptr, "\t\n", &save))
+      fields.emplace_back(tok);
+    rows.push_back(fields);
+  }
+  std::fclose(fp);
+  return rows;
+}
+
+std::map<std::string, Params> g_params_cache;
+
+const Params* load_params(const char* data_dir, int method) {
+  std::string key = std::string(data_dir ? data_dir : ".") + "#" + std::to_string(method);
+  auto it = g_params_cache.find(key);
+  if (it != g_params_cache.end()) return &it->second;
+  std::string prefix = std::string(data_dir ? data_dir : ".") + (method == 1 ? "/pm7ts" : "/pm7");
+  auto er = read_tsv(prefix + "_elements.tsv");
+  if (er.size() < 2) return nullptr;
+  Params P;
+  P.method = method;
+  const auto& hdr = er[0];
+  auto col = [&](const char* name) {
+    for (size_t i = 0; i < hdr.size(); ++i)
+      if (hdr[i] == name) return (int)i;
+    return -1;
+  };
+  for (size_t r = 1; r < er.size(); ++r) {
+    auto v = [&](const char* name) { return std::strtod(er[r][col(name)].c_str(), nullptr); };
+    Element e;
+    e.z = (int)v("z");
+    e.n = (int)v("n");
+    e.ios = (int)v("ios");
+    e.iop = (int)v("iop");
+    e.iod = (int)v("iod");
+    e.core_charge = v("core_charge");
+    e.eheat_ev = v("eheat_kcal") * KCAL_TO_EV;
+    e.u_ss = v("u_ss");
+    e.u_pp = v("u_pp");
+    e.u_dd = v("u_dd");
+    e.zeta_s = v("zeta_s");
+    e.zeta_p = v("zeta_p");
+    e.zeta_d = v("zeta_d");
+    e.beta_s = v("beta_s");
+    e.beta_p = v("beta_p");
+    e.beta_d = v("beta_d");
+    e.g_ss = v("g_ss");
+    e.g_sp = v("g_sp");
+    e.g_pp = v("g_pp");
+    e.g_p2 = v("g_p2");
+    e.h_sp = v("h_sp");
+    e.zsn = v("zsn");
+    e.zpn = v("zpn");
+    e.zdn = v("zdn");
+    e.f0sd = v("f0sd");
+    e.g2sd = v("g2sd");
+    e.poc = v("poc");
+    e.n_orb = e.zeta_d > 1.0e-20 ? 9 : (e.zeta_p > 1.0e-20 ? 4 : 1);
+    if (e.n_orb >= 4) {
+      e.dd = dd_charge_sep(e.n, e.zeta_s, std::max(e.zeta_p, 0.3));
+      e.qq = qq_charge_sep(e.n, std::max(e.zeta_p, 0.3));
+      e.rho1 = additive_rho1(std::max(e.h_sp, 1.0e-7), e.dd);
+      e.rho2 = additive_rho2(std::max(0.5 * (e.g_pp - e.g_p2), 0.1), e.qq);
+    }
+    for (int s = 1; s <= 4; ++s) {
+      char a[8], b[8], c[8];
+      std::snprintf(a, sizeof a, "g1_%d", s);
+      std::snprintf(b, sizeof b, "g2_%d", s);
+      std::snprintf(c, sizeof c, "g3_%d", s);
+      double k = v(a), l = v(b), m = v(c);
+      if (k != 0.0 || l != 0.0 || m != 0.0) e.gauss.push_back({k, l, m});
+    }
+    // Isolated-atom energy (src/params.rs:268-286). The d-shell `eiscor` term is zero for every
+    // element this restatement accepts.
+    double gssc = std::max(e.ios - 1, 0);
+    double p = e.iop;
+    double l = std::min(p, 6.0 - p);
+    double gspc = e.ios * p;
+    double gp2c = p * (p - 1.0) / 2.0 + 0.5 * l * (l - 1.0) / 2.0;
+    double gppc = -0.5 * l * (l - 1.0) / 2.0;
+    double hspc = -p * e.ios * 0.5;
+    e.e_isol = e.u_ss * e.ios + e.u_pp * p + e.u_dd * e.iod + e.g_ss * gssc + e.g_pp * gppc +
+               e.g_sp * gspc + e.g_p2 * gp2c + e.h_sp * hspc;
+    e.rho0 = e.g_ss != 0.0 ? EV / (2.0 * e.g_ss) : 0.0;
+    P.el[e.z] = e;
+  }
+  for (auto& row : read_tsv(prefix + "_pairs.tsv")) {
+    if (row.size() < 4 || row[0] == "z_hi") continue;
+    int hi = std::atoi(row[0].c_str()), lo = std::atoi(row[1].c_str());
+    P.pairs[{std::max(hi, lo), std::min(hi, lo)}] = {std::strtod(row[2].c_str(), nullptr),
+                                                     std::strtod(row[3].c_str(), nullptr)};
+  }
+  for (auto& row : read_tsv(prefix + "_vpar.tsv")) {
+    if (row.size() < 2 || row[0] == "index") continue;
+    int idx = std::atoi(row[0].c_str());
+    if (idx >= 1 && idx <= 60) P.vpar[idx] = std::strtod(row[1].c_str(), nullptr);
+  }
+  auto res = g_params_cache.emplace(key, P);
+  return &res.first->second;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Geometry and basis (src/basis.rs:37-62, src/pbc/images.rs:180-199)
+// ---------------------------------------------------------------------------------------------
+struct V3 {
+  double x, y, z;
+};
+inline V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline double norm(V3 a) { return std::sqrt(dot(a, a)); }
+
+struct Mol {
+  std::vector<int> z;
+  std::vector<V3> pos;  // Bohr
+  std::vector<int> off, norb;
+  std::vector<const Element*> el;
+  int nao = 0;
+  int n() const { return (int)z.size(); }
+};
+
+// Matrix helpers, row-major n x n.
+using Mat = std::vector<double>;
+
+inline int pack(int a, int b) {
+  int h = std::max(a, b), l = std::min(a, b);
+  return h * (h + 1) / 2 + l;
+}
+
+// ---------------------------------------------------------------------------------------------
+// sp two-centre integrals (src/integrals.rs:16-538)
+// ---------------------------------------------------------------------------------------------
+// PM7 feathering toward the point charge (src/integrals.rs:31-41).
+void feather_to_point(double r, double& cfrac, double& point) {
+  point = EV / r;
+  double r_ang = r * A0;
+  cfrac = r_ang < 7.0 ? 1.0 - std::exp(-(r_ang - 7.0) * (r_ang - 7.0) * 0.22) : 0.0;
+}
+void feather_ri(double* ri, int n, double r, const int* mono, int nmono) {  // :47-56
+  double cfrac, point;
+  feather_to_point(r, cfrac, point);
+  double add = point * (1.0 - cfrac);
+  for (int i = 0; i < n; ++i) ri[i] *= cfrac;
+  for (int k = 0; k < nmono; ++k) ri[mono[k]] += add;
+}
+const int XX_MONO[10] = {0, 2, 3, 10, 11, 15, 16, 17, 18, 20};  // :61
+const int XH_MONO[3] = {0, 2, 3};                               // :63
+
+// Shortest-arc quaternion rotation taking v onto +x (src/integrals.rs:74-110).
+void rotation_to_x(double vx, double vy, double vz, double R[3][3]) {
+  double qx = 0.0, qy = vz, qz = -vy, qw = vx + 1.0;
+  if (std::fabs(qw) < 1.0e-13) {
+    qx = 0.0;
+    qy = 0.0;
+    qz = 1.0;
+    qw = 0.0;
+  }
+  double inv = 1.0 / std::sqrt(qx * qx + qy * qy + qz * qz + qw * qw);
+  qy *= inv;
+  qz *= inv;
+  qw *= inv;
+  R[0][0] = (qy * qy + qz * qz) * -2.0 + 1.0;
+  R[0][1] = qz * qw * -2.0;
+  R[0][2] = qy * qw * 2.0;
+  R[1][0] = qz * qw * 2.0;
+  R[1][1] = qz * qz * -2.0 + 1.0;
+  R[1][2] = qy * qz * 2.0;
+  R[2][0] = qy * qw * -2.0;
+  R[2][1] = qy * qz * 2.0;
+  R[2][2] = qy * qy * -2.0 + 1.0;
+}
+
+inline double isq(double x, double c) { return 1.0 / std::sqrt(x + c); }  // 1/sqrt(x + c)
+inline double isq2(double a, double c) { return 1.0 / std::sqrt(a * a + c); }
+
+void local_xh(const Element& ei, const Element& ej, double r, double ri[4]) {  // :273-294
+  const double ev1 = EV / 2.0, ev2 = EV / 4.0;
+  double da = ei.dd, qa = ei.qq * 2.0;
+  double aee = (ei.rho0 + ej.rho0) * (ei.rho0 + ej.rho0);
+  double ade = (ei.rho1 + ej.rho0) * (ei.rho1 + ej.rho0);
+  double aqe = (ei.rho2 + ej.rho0) * (ei.rho2 + ej.rho0);
+  double ee = EV / std::sqrt(r * r + aee);
+  double ev1dsqr6 = ev1 / std::sqrt(r * r + aqe);
+  ri[0] = ee;
+  ri[1] = ev1 * isq2(r + da, ade) - ev1 * isq2(r - da, ade);
+  ri[2] = ee + ev2 * isq2(r + qa, aqe) + ev2 * isq2(r - qa, aqe) - ev1dsqr6;
+  ri[3] = ee + ev1 / std::sqrt(r * r + qa * qa + aqe) - ev1dsqr6;
+  feather_ri(ri, 4, r, XH_MONO, 3);
+}
+
+void local_xx(const Element& ei, const Element& ej, double r, double ri[22]) {  // :296-450
+  const double ev1 = EV / 2.0, ev2 = EV / 4.0, ev3 = EV / 8.0, ev4 = EV / 16.0;
+  double da = ei.dd, db = ej.dd, qa = ei.qq * 2.0, qb = ej.qq * 2.0, qa1 = ei.qq, qb1 = ej.qq;
+  auto sq = [](double x) { return x * x; };
+  double aee = sq(ei.rho0 + ej.rho0), ade = sq(ei.rho1 + ej.rho0), aqe = sq(ei.rho2 + ej.rho0);
+  double aed = sq(ei.rho0 + ej.rho1), aeq = sq(ei.rho0 + ej.rho2), axx = sq(ei.rho1 + ej.rho1);
+  double adq = sq(ei.rho1 + ej.rho2), aqd = sq(ei.rho2 + ej.rho1), aqq = sq(ei.rho2 + ej.rho2);
+  double r2 = r * r;
+  double ee = EV / std::sqrt(r2 + aee);
+  double dze = ev1 * isq2(r - da, ade) - ev1 * isq2(r + da, ade);
+  double ev1dsqr6 = ev1 / std::sqrt(r2 + aqe);
+  double qzze = ev2 * isq2(r - qa, aqe) + ev2 * isq2(r + qa, aqe) - ev1dsqr6;
+  double qxxe = ev1 * isq(r2 + qa * qa, aqe) - ev1dsqr6;
+  double edz = ev1 * isq2(r + db, aed) - ev1 * isq2(r - db, aed);
+  double ev1dsqr12 = ev1 / std::sqrt(r2 + aeq);
+  double eqzz = ev2 * isq2(r - qb, aeq) + ev2 * isq2(r + qb, aeq) - ev1dsqr12;
+  double eqxx = ev1 * isq(r2 + qb * qb, aeq) - ev1dsqr12;
+  double e20 = ev2 * isq2(r + da, adq), e22 = ev2 * isq2(r - da, adq);
+  double e24 = ev2 * isq2(r - db, aqd), e26 = ev2 * isq2(r + db, aqd);
+  double e36 = ev2 / std::sqrt(r2 + aqq);
+  double e39 = ev2 * isq(r2 + qa * qa, aqq), e40 = ev2 * isq(r2 + qb * qb, aqq);
+  double e42 = ev3 * isq2(r - qb, aqq), e44 = ev3 * isq2(r + qb, aqq);
+  double e46 = ev3 * isq2(r + qa, aqq), e48 = ev3 * isq2(r - qa, aqq);
+  ri[0] = ee;
+  ri[1] = -dze;
+  ri[2] = ee + qzze;
+  ri[3] = ee + qxxe;
+  ri[4] = -edz;
+  ri[5] = ev2 * isq2(r + da - db, axx) + ev2 * isq2(r - da + db, axx) -
+          ev2 * isq2(r - da - db, axx) - ev2 * isq2(r + da + db, axx);
+  ri[6] = ev1 * isq(r2 + sq(da - db), axx) - ev1 * isq(r2 + sq(da + db), axx);
+  ri[7] = -edz + ev3 * isq2(r + qa - db, aqd) - ev3 * isq2(r + qa + db, aqd) +
+          ev3 * isq2(r - qa - db, aqd) - ev3 * isq2(r - qa + db, aqd) - e24 + e26;
+  ri[8] = -edz - e24 + ev2 * isq(sq(r - db) + qa * qa, aqd) + e26 -
+          ev2 * isq(sq(r + db) + qa * qa, aqd);
+  ri[9] = ev2 * isq(sq(qa1 - db) + sq(r + qa1), aqd) - ev2 * isq(sq(qa1 - db) + sq(r - qa1), aqd) -
+          ev2 * isq(sq(qa1 + db) + sq(r + qa1), aqd) + ev2 * isq(sq(qa1 + db) + sq(r - qa1), aqd);
+  ri[10] = ee + eqzz;
+  ri[11] = ee + eqxx;
+  ri[12] = -dze + ev3 * isq2(r + da - qb, adq) - ev3 * isq2(r - da - qb, adq) +
+           ev3 * isq2(r + da + qb, adq) - ev3 * isq2(r - da + qb, adq) + e22 - e20;
+  ri[13] = -dze - e20 + ev2 * isq(sq(r + da) + qb * qb, adq) + e22 -
+           ev2 * isq(sq(r - da) + qb * qb, adq);
+  ri[14] = ev2 * isq(sq(da - qb1) + sq(r - qb1), adq) - ev2 * isq(sq(da - qb1) + sq(r + qb1), adq) -
+           ev2 * isq(sq(da + qb1) + sq(r - qb1), adq) + ev2 * isq(sq(da + qb1) + sq(r + qb1), adq);
+  ri[15] = ee + eqzz + qzze + ev4 * isq2(r + qa - qb, aqq) + ev4 * isq2(r + qa + qb, aqq) +
+           ev4 * isq2(r - qa - qb, aqq) + ev4 * isq2(r - qa + qb, aqq) - e48 - e46 - e42 - e44 +
+           e36;
+  ri[16] = ee + eqzz + qxxe + ev3 * isq(sq(r - qb) + qa * qa, aqq) +
+           ev3 * isq(sq(r + qb) + qa * qa, aqq) - e42 - e44 - e39 + e36;
+  ri[17] = ee + eqxx + qzze + ev3 * isq(sq(r + qa) + qb * qb, aqq) +
+           ev3 * isq(sq(r - qa) + qb * qb, aqq) - e46 - e48 - e40 + e36;
+  double qxxqxx = ev3 * isq(r2 + sq(qa - qb), aqq) + ev3 * isq(r2 + sq(qa + qb), aqq) - e39 - e40 + e36;
+  ri[18] = ee + eqxx + qxxe + qxxqxx;
+  double m2 = sq(qa1 - qb1), p2 = sq(qa1 + qb1);
+  ri[19] = ev3 * isq(sq(r + qa1 - qb1) + m2, aqq) - ev3 * isq(sq(r + qa1 + qb1) + m2, aqq) -
+           ev3 * isq(sq(r - qa1 - qb1) + m2, aqq) + ev3 * isq(sq(r - qa1 + qb1) + m2, aqq) -
+           ev3 * isq(sq(r + qa1 - qb1) + p2, aqq) + ev3 * isq(sq(r + qa1 + qb1) + p2, aqq) +
+           ev3 * isq(sq(r - qa1 - qb1) + p2, aqq) - ev3 * isq(sq(r - qa1 + qb1) + p2, aqq);
+  double qxxqyy = ev2 * isq(r2 + qa * qa + qb * qb, aqq) - e39 - e40 + e36;
+  ri[20] = ee + eqxx + qxxe + qxxqyy;
+  ri[21] = (qxxqxx - qxxqyy) * 0.5;
+  feather_ri(ri, 22, r, XX_MONO, 10);
+}
+
+// src/integrals.rs:452-464
+void build_wxh(const double ri[4], const double* r0, const double* r1, const double* r2,
+               double w[10]) {
+  w[pack(0, 0)] = ri[0];
+  for (int k = 0; k < 3; ++k) w[pack(k + 1, 0)] = ri[1] * r0[k];
+  for (int k = 0; k < 3; ++k)
+    for (int l = 0; l <= k; ++l)
+      w[pack(k + 1, l + 1)] = ri[2] * (r0[k] * r0[l]) + ri[3] * (r1[k] * r1[l] + r2[k] * r2[l]);
+}
+
+// src/integrals.rs:466-538: the 10x10 block (row = pack on atom i, col = pack on atom j).
+void rotate_xx(const double ri[22], const double* r0, const double* r1, const double* r2,
+               double w[100]) {
+  int idx = 0;
+  for (int kk = 0; kk < 4; ++kk)
+    for (int ll = 0; ll <= kk; ++ll)
+      for (int mm = 0; mm < 4; ++mm)
+        for (int nn = 0; nn <= mm; ++nn) {
+          int k = kk - 1, l = ll - 1, m = mm - 1, n = nn - 1;
+          double val;
+          if (kk == 0) {
+            if (mm == 0)
+              val = ri[0];
+            else if (nn == 0)
+              val = ri[4] * r0[m];
+            else
+              val = ri[10] * (r0[m] * r0[n]) + ri[11] * (r1[m] * r1[n] + r2[m] * r2[n]);
+          } else if (ll == 0) {
+            if (mm == 0)
+              val = ri[1] * r0[k];
+            else if (nn == 0)
+              val = ri[5] * (r0[k] * r0[m]) + ri[6] * (r1[k] * r1[m] + r2[k] * r2[m]);
+            else {
+              double t0 = r0[k] * r0[m] * r0[n];
+              double t1 = (r1[m] * r1[n] + r2[m] * r2[n]) * r0[k];
+              double mix = r1[k] * (r1[n] * r0[m] + r1[m] * r0[n]) + r2[k] * (r2[m] * r0[n] + r2[n] * r0[m]);
+              val = ri[12] * t0 + ri[13] * t1 + ri[14] * mix;
+            }
+          } else if (mm == 0) {
+            val = ri[2] * (r0[k] * r0[l]) + ri[3] * (r1[k] * r1[l] + r2[k] * r2[l]);
+          } else if (nn == 0) {
+            double t0 = r0[k] * r0[l] * r0[m];
+            double t1 = (r1[k] * r1[l] + r2[k] * r2[l]) * r0[m];
+            double t2 = r1[l] * r1[m] + r2[l] * r2[m];
+            val = ri[7] * t0 + ri[8] * t1 + ri[9] * (r0[k] * t2 + r0[l] * (r1[k] * r1[m] + r2[k] * r2[m]));
+          } else {
+            double t0 = r0[k] * r0[l] * r0[m] * r0[n];
+            double v = ri[15] * t0;
+            v += ri[16] * ((r1[k] * r1[l] + r2[k] * r2[l]) * r0[m] * r0[n]);
+            v += ri[17] * ((r1[m] * r1[n] + r2[m] * r2[n]) * (r0[k] * r0[l]));
+            v += ri[18] * (r1[k] * r1[l] * r1[m] * r1[n] + r2[k] * r2[l] * r2[m] * r2[n]);
+            double mix1 = r0[m] * (r1[l] * r1[n] + r2[l] * r2[n]);
+            double mix2 = r0[n] * (r1[l] * r1[m] + r2[l] * r2[m]);
+            double val5 = r0[k] * (mix1 + mix2) +
+                          r0[l] * (r0[m] * (r1[k] * r1[n] + r2[k] * r2[n]) + r0[n] * (r1[k] * r1[m] + r2[k] * r2[m]));
+            v += ri[19] * val5;
+            v += ri[20] * (r1[k] * r1[l] * r2[m] * r2[n] + r2[k] * r2[l] * r1[m] * r1[n]);
+            double cross = (r1[k] * r2[l] + r2[k] * r1[l]) * (r1[m] * r2[n] + r2[m] * r1[n]);
+            v += ri[21] * cross;
+            val = v;
+          }
+          w[idx++] = val;
+        }
+}
+
+struct PairBlock {
+  int a = 0, b = 0;      // ordered: a has at least as many AOs as b
+  int npa = 1, npb = 1;  // packed pair counts (1 or 10)
+  std::vector<double> w;  // npa x npb
+  double e1b[4][4] = {{0}}, e2a[4][4] = {{0}}, s[4][4] = {{0}};
+};
+
+// src/integrals.rs:186-271 for the sp path.
+void pair_two_electron(const Element& ei, const Element& ej, V3 d, PairBlock& out) {
+  double r = std::sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+  double inv = 1.0 / r;
+  double x[3] = {d.x * inv, d.y * inv, d.z * inv};
+  bool hi = ei.has_p(), hj = ej.has_p();
+  double R[3][3];
+  rotation_to_x(-x[0], -x[1], -x[2], R);
+  if (!hi && !hj) {
+    double aee = (ei.rho0 + ej.rho0) * (ei.rho0 + ej.rho0);
+    double ee_nddo = EV / std::sqrt(r * r + aee);
+    double cfrac, point;
+    feather_to_point(r, cfrac, point);
+    double ee = ee_nddo * cfrac + point * (1.0 - cfrac);
+    out.npa = out.npb = 1;
+    out.w.assign(1, ee);
+    out.e1b[0][0] = ee * -ej.core_charge;
+    out.e2a[0][0] = ee * -ei.core_charge;
+    return;
+  }
+  if (hi && !hj) {
+    double ri[4];
+    local_xh(ei, ej, r, ri);
+    double w[10];
+    build_wxh(ri, R[0], R[1], R[2], w);
+    out.npa = 10;
+    out.npb = 1;
+    out.w.assign(w, w + 10);
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) out.e1b[a][b] = w[pack(a, b)] * -ej.core_charge;
+    out.e2a[0][0] = w[0] * -ei.core_charge;
+    return;
+  }
+  double ri[22];
+  local_xx(ei, ej, r, ri);
+  double w[100];
+  rotate_xx(ri, R[0], R[1], R[2], w);
+  out.npa = out.npb = 10;
+  out.w.assign(w, w + 100);
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) {
+      out.e1b[a][b] = w[pack(a, b) * 10] * -ej.core_charge;
+      out.e2a[a][b] = w[pack(a, b)] * -ei.core_charge;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Slater overlaps (src/overlap.rs:16-40, 120-328, 403-462)
+// ---------------------------------------------------------------------------------------------
+void aintgs(double x, double a[13]) {
+  for (int i = 0; i < 13; ++i) a[i] = 0.0;
+  if (std::fabs(x) < 1.0e-12) return;
+  double inv = 1.0 / x;
+  a[0] = std::exp(-x) * inv;
+  for (int i = 1; i < 13; ++i) a[i] = a[0] + a[i - 1] * inv * i;
+}
+void bintgs(double x, double b[13]) {
+  double absx = std::fabs(x);
+  if (absx > 0.5) {
+    double inv = 1.0 / x;
+    double tx = std::exp(x) * inv;
+    double tmx = std::exp(-x) * inv * -1.0;
+    b[0] = tx + tmx;
+    for (int i = 1; i < 13; ++i) {
+      double sign = (i % 2 == 0) ? 1.0 : -1.0;
+      b[i] = tx * sign + tmx + b[i - 1] * inv * i;
+    }
+  } else if (absx > 1.0e-6) {
+    double x2 = x * x, x3 = x2 * x, x4 = x2 * x2, x5 = x4 * x, x6 = x4 * x2;
+    b[0] = x2 * (1.0 / 3.0) + x4 * (1.0 / 60.0) + x6 * (1.0 / 2520.0) + 2.0;
+    b[2] = x2 * (1.0 / 5.0) + x4 * (1.0 / 84.0) + x6 * (1.0 / 3240.0) + 2.0 / 3.0;
+    b[4] = x2 * (1.0 / 7.0) + x4 * (1.0 / 108.0) + x6 * (1.0 / 3960.0) + 2.0 / 5.0;
+    b[6] = x2 * (1.0 / 9.0) + x4 * (1.0 / 132.0) + x6 * (1.0 / 4680.0) + 2.0 / 7.0;
+    b[8] = x2 * (1.0 / 11.0) + x4 * (1.0 / 156.0) + x6 * (1.0 / 5400.0) + 2.0 / 9.0;
+    b[10] = x2 * (1.0 / 13.0) + x4 * (1.0 / 180.0) + x6 * (1.0 / 6120.0) + 2.0 / 11.0;
+    b[12] = x2 * (1.0 / 15.0) + x4 * (1.0 / 204.0) + x6 * (1.0 / 6840.0) + 2.0 / 13.0;
+    b[1] = x * (-2.0 / 3.0) - x3 * (1.0 / 15.0) - x5 * (1.0 / 420.0);
+    b[3] = x * (-2.0 / 5.0) - x3 * (1.0 / 21.0) - x5 * (1.0 / 540.0);
+    b[5] = x * (-2.0 / 7.0) - x3 * (1.0 / 27.0) - x5 * (1.0 / 660.0);
+    b[7] = x * (-2.0 / 9.0) - x3 * (1.0 / 33.0) - x5 * (1.0 / 780.0);
+    b[9] = x * (-2.0 / 11.0) - x3 * (1.0 / 39.0) - x5 * (1.0 / 900.0);
+    b[11] = x * (-2.0 / 13.0) - x3 * (1.0 / 45.0) - x5 * (1.0 / 1020.0);
+  } else {
+    for (int i = 0; i < 13; ++i) b[i] = (i % 2 == 0) ? 2.0 / (i + 1.0) : 0.0;
+  }
+}
+
+// Local-frame sigma/pi overlaps s111 s211 s121 s221 s222 for ea.n >= eb.n (n <= 3).
+bool overlap_locals(const Element& ea, const Element& eb, double r, double s[5]) {
+  int jcall;
+  if (ea.n == 1 && eb.n == 1) jcall = 2;
+  else if (ea.n == 2 && eb.n == 1) jcall = 3;
+  else if (ea.n == 2 && eb.n == 2) jcall = 4;
+  else if (ea.n == 3 && eb.n == 1) jcall = 431;
+  else if (ea.n == 3 && eb.n == 2) jcall = 5;
+  else if (ea.n == 3 && eb.n == 3) jcall = 6;
+  else return false;  // n >= 4 needs the numerical quadrature (not restated here)
+  double za[2] = {ea.zeta_s, ea.zeta_p}, zb[2] = {eb.zeta_s, eb.zeta_p};
+  double a111[13], b111[13], a211[13], b211[13], a121[13], b121[13], a22[13], b22[13];
+  aintgs(r * (0.5 * (za[0] + zb[0])), a111);
+  bintgs(r * (0.5 * (za[0] - zb[0])), b111);
+  aintgs(r * (0.5 * (za[1] + zb[0])), a211);
+  bintgs(r * (0.5 * (za[1] - zb[0])), b211);
+  aintgs(r * (0.5 * (za[0] + zb[1])), a121);
+  bintgs(r * (0.5 * (za[0] - zb[1])), b121);
+  aintgs(r * (0.5 * (za[1] + zb[1])), a22);
+  bintgs(r * (0.5 * (za[1] - zb[1])), b22);
+  double s111 = 0, s211 = 0, s121 = 0, s221 = 0, s222 = 0;
+  const double sq3 = std::sqrt(3.0);
+  auto p = [](double x, double e) { return std::pow(x, e); };
+  auto pi = [](double x, int k) {
+    double v = 1.0;
+    for (int i = 0; i < k; ++i) v *= x;
+    return v;
+  };
+  switch (jcall) {
+    case 2:
+      s111 = p(r * r * (za[0] * zb[0]), 1.5) * (a111[2] * b111[0] - b111[2] * a111[0]) / 4.0;
+      break;
+    case 3:
+      s111 = pi(r, 4) * (p(zb[0], 1.5) * p(za[0], 2.5)) *
+             (a111[3] * b111[0] - b111[3] * a111[0] + a111[2] * b111[1] - b111[2] * a111[1]) / (sq3 * 8.0);
+      s211 = pi(r, 4) * (p(zb[0], 1.5) * p(za[1], 2.5)) *
+             (a211[2] * b211[0] - b211[2] * a211[0] + a211[3] * b211[1] - b211[3] * a211[1]) / 8.0;
+      break;
+    case 4: {
+      s111 = pi(r, 5) * p(zb[0] * za[0], 2.5) * (a111[4] * b111[0] + b111[4] * a111[0] - a111[2] * b111[2] * 2.0) / 48.0;
+      s211 = pi(r, 5) * p(zb[0] * za[1], 2.5) *
+             (a211[3] * (b211[0] - b211[2]) - a211[1] * (b211[2] - b211[4]) + b211[3] * (a211[0] - a211[2]) -
+              b211[1] * (a211[2] - a211[4])) / (16.0 * sq3);
+      s121 = pi(r, 5) * p(zb[1] * za[0], 2.5) *
+             (a121[3] * (b121[0] - b121[2]) - a121[1] * (b121[2] - b121[4]) - b121[3] * (a121[0] - a121[2]) +
+              b121[1] * (a121[2] - a121[4])) / (16.0 * sq3);
+      double w = pi(r, 5) * p(zb[1] * za[1], 2.5) / 16.0;
+      s221 = -(w * (b22[2] * (a22[4] + a22[0]) - a22[2] * (b22[4] + b22[0])));
+      s222 = w * (a22[4] * (b22[0] - b22[2]) - b22[4] * (a22[0] - a22[2]) - a22[2] * b22[0] + b22[2] * a22[0]) * 0.5;
+      break;
+    }
+    case 431:
+      s111 = pi(r, 5) * (p(zb[0], 1.5) * p(za[0], 3.5)) *
+             (a111[4] * b111[0] + b111[1] * a111[3] * 2.0 - a111[1] * b111[3] * 2.0 - b111[4] * a111[0]) /
+             (std::sqrt(10.0) * 24.0);
+      s211 = pi(r, 5) * (p(zb[0], 1.5) * p(za[1], 3.5)) *
+             (a211[3] * (b211[0] + b211[2]) - a211[1] * (b211[4] + b211[2]) + b211[1] * (a211[2] + a211[4]) -
+              b211[3] * (a211[2] + a211[0])) / (8.0 * std::sqrt(30.0));
+      break;
+    case 5:
+      s111 = pi(r, 6) * (p(zb[0], 2.5) * p(za[0], 3.5)) *
+             (a111[5] * b111[0] + b111[1] * a111[4] - b111[2] * a111[3] * 2.0 - a111[2] * b111[3] * 2.0 +
+              b111[4] * a111[1] + b111[5] * a111[0]) / (std::sqrt(30.0) * 48.0);
+      s211 = pi(r, 6) * (p(zb[0], 2.5) * p(za[1], 3.5)) *
+             (a211[4] * b211[0] + b211[1] * a211[5] - b211[3] * a211[3] * 2.0 - a211[2] * b211[2] * 2.0 +
+              a211[1] * b211[5] + a211[0] * b211[4]) / (48.0 * std::sqrt(10.0));
+      s121 = pi(r, 6) * (p(zb[1], 2.5) * p(za[0], 3.5)) *
+             ((a121[4] * b121[0] - a121[5] * b121[1]) + (a121[3] * b121[1] - a121[4] * b121[2]) * 2.0 -
+              (a121[1] * b121[3] - a121[2] * b121[4]) * 2.0 - (a121[0] * b121[4] - a121[1] * b121[5])) /
+             (48.0 * std::sqrt(10.0));
+      s221 = pi(r, 6) * (p(zb[1], 2.5) * p(za[1], 3.5)) *
+             ((a22[3] * b22[0] - a22[5] * b22[2]) + (a22[2] * b22[1] - a22[4] * b22[3]) -
+              (a22[1] * b22[2] - a22[3] * b22[4]) - (a22[0] * b22[3] - a22[2] * b22[5])) / (16.0 * std::sqrt(30.0));
+      s222 = pi(r, 6) * (p(zb[1], 2.5) * p(za[1], 3.5)) *
+             ((a22[5] - a22[3]) * (b22[0] - b22[2]) + (a22[4] - a22[2]) * (b22[1] - b22[3]) -
+              (a22[3] - a22[1]) * (b22[2] - b22[4]) - (a22[2] - a22[0]) * (b22[3] - b22[5])) /
+             (32.0 * std::sqrt(30.0));
+      break;
+    case 6:
+      s111 = pi(r, 7) * p(zb[0] * za[0], 3.5) *
+             (a111[6] * b111[0] - b111[2] * a111[4] * 3.0 + a111[2] * b111[4] * 3.0 - a111[0] * b111[6]) / 1440.0;
+      s211 = pi(r, 7) * p(zb[0] * za[1], 3.5) *
+             ((a211[5] * b211[0] + a211[6] * b211[1]) + (-a211[4] * b211[1] - a211[5] * b211[2]) -
+              (a211[3] * b211[2] + a211[4] * b211[3]) * 2.0 - (-a211[2] * b211[3] - a211[3] * b211[4]) * 2.0 +
+              (a211[1] * b211[4] + a211[2] * b211[5]) + (-a211[0] * b211[5] - a211[1] * b211[6])) /
+             (480.0 * sq3);
+      s121 = pi(r, 7) * p(zb[1] * za[0], 3.5) *
+             ((a121[5] * b121[0] - a121[6] * b121[1]) + (a121[4] * b121[1] - a121[5] * b121[2]) -
+              (a121[3] * b121[2] - a121[4] * b121[3]) * 2.0 - (a121[2] * b121[3] - a121[3] * b121[4]) * 2.0 +
+              (a121[1] * b121[4] - a121[2] * b121[5]) + (a121[0] * b121[5] - a121[1] * b121[6])) /
+             (480.0 * sq3);
+      s221 = pi(r, 7) * (p(zb[1], 3.5) * p(za[1], 3.5)) *
+             ((a22[4] * b22[0] - a22[6] * b22[2]) - (a22[2] * b22[2] - a22[4] * b22[4]) * 2.0 +
+              (a22[0] * b22[4] - a22[2] * b22[6])) / 480.0;
+      s222 = pi(r, 7) * (p(zb[1], 3.5) * p(za[1], 3.5)) *
+             ((a22[6] - a22[4]) * (b22[0] - b22[2]) - (a22[4] - a22[2]) * (b22[2] - b22[4]) * 2.0 +
+              (a22[2] - a22[0]) * (b22[4] - b22[6])) / 960.0;
+      break;
+  }
+  s[0] = s111;
+  s[1] = s211;
+  s[2] = s121;
+  s[3] = s221;
+  s[4] = s222;
+  return true;
+}
+
+// src/overlap.rs:290-328
+void build_di(const double s[5], const double dir[3], double di[4][4]) {
+  double rr[3][3];
+  rotation_to_x(dir[0], dir[1], dir[2], rr);
+  double rot[3][3];
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) rot[a][b] = rr[b][a];
+  di[0][0] = s[0];
+  for (int a = 0; a < 3; ++a) {
+    di[1 + a][0] = s[1] * rot[a][0];
+    di[0][1 + a] = -s[2] * rot[a][0];
+  }
+  double mm[3] = {-s[3], s[4], s[4]};
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      double v = 0.0;
+      for (int k = 0; k < 3; ++k) v += rot[a][k] * mm[k] * rot[b][k];
+      di[1 + a][1 + b] = v;
+    }
+}
+
+// src/overlap.rs:16-40 with pos_i at the origin and pos_j = d.
+bool diatom_overlap(const Element& ei, const Element& ej, V3 d, double out[4][4]) {
+  double r = norm(d);
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) out[a][b] = (a == b) ? 1.0 : 0.0;
+  if (r < 1.0e-8) return true;
+  double x[3] = {d.x / r, d.y / r, d.z / r};
+  double s[5];
+  double di[4][4];
+  if (ei.n >= ej.n) {
+    if (!overlap_locals(ei, ej, r, s)) return false;
+    build_di(s, x, di);
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) out[a][b] = di[a][b];
+  } else {
+    if (!overlap_locals(ej, ei, r, s)) return false;
+    double mx[3] = {-x[0], -x[1], -x[2]};
+    build_di(s, mx, di);
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) out[a][b] = di[b][a];
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Core Hamiltonian (src/hamiltonian.rs:310-638, molecular branch)
+// ---------------------------------------------------------------------------------------------
+double beta_of(const Element& e, int orb) { return orb == 0 ? e.beta_s : (orb <= 3 ? e.beta_p : e.beta_d); }
+
+struct Core {
+  Mat h;
+  std::vector<PairBlock> pairs;
+};
+
+int build_mol(const Params& P, int n, const int* z, const double* xyz, Mol& m, std::string& err) {
+  m.z.assign(z, z + n);
+  m.pos.resize(n);
+  m.off.resize(n);
+  m.norb.resize(n);
+  m.el.resize(n);
+  int nao = 0;
+  for (int i = 0; i < n; ++i) {
+    m.pos[i] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+    const Element* e = P.get(z[i]);
+    if (!e) {
+      err = "missing element " + std::to_string(z[i]);
+      return 2;
+    }
+    if (e->n_orb == 9 || e->n > 3) {
+      err = "element " + std::to_string(z[i]) + " needs the MNDO/d or numeric-overlap kernels";
+      return 3;
+    }
+    m.el[i] = e;
+    m.off[i] = nao;
+    m.norb[i] = e->n_orb;
+    nao += e->n_orb;
+  }
+  m.nao = nao;
+  return 0;
+}
+
+Core build_core(const Mol& m) {
+  int nao = m.nao, n = m.n();
+  Core c;
+  c.h.assign((size_t)nao * nao, 0.0);
+  for (int i = 0; i < n; ++i) {
+    const Element& e = *m.el[i];
+    for (int k = 0; k < m.norb[i]; ++k)
+      c.h[(size_t)(m.off[i] + k) * nao + m.off[i] + k] = k == 0 ? e.u_ss : (k <= 3 ? e.u_pp : e.u_dd);
+  }
+  // Molecular pair list u < v, heavier (more AOs) first (src/hamiltonian.rs:386-392).
+  std::vector<std::pair<int, int>> list;
+  for (int u = 0; u < n; ++u)
+    for (int v = u + 1; v < n; ++v) list.push_back({u, v});
+  c.pairs.resize(list.size());
+  parallel_for((long)list.size(), [&](long p) {
+    int u = list[p].first, v = list[p].second;
+    V3 d = sub(m.pos[v], m.pos[u]);
+    int a = u, b = v;
+    if (m.el[u]->n_orb < m.el[v]->n_orb) {
+      a = v;
+      b = u;
+      d = {-d.x, -d.y, -d.z};
+    }
+    PairBlock& pb = c.pairs[p];
+    pb.a = a;
+    pb.b = b;
+    double r = norm(d);
+    V3 dn = {d.x / r * r, d.y / r * r, d.z / r * r};  // pair_two_electron(ea, eb, d/r, r)
+    pair_two_electron(*m.el[a], *m.el[b], dn, pb);
+    diatom_overlap(*m.el[a], *m.el[b], d, pb.s);
+  });
+  for (const auto& pb : c.pairs) {
+    const Element& ea = *m.el[pb.a];
+    const Element& eb = *m.el[pb.b];
+    int oa = m.off[pb.a], ob = m.off[pb.b], na = m.norb[pb.a], nb = m.norb[pb.b];
+    for (int i = 0; i < na; ++i)
+      for (int j = 0; j < na; ++j) c.h[(size_t)(oa + i) * nao + oa + j] += pb.e1b[i][j];
+    for (int i = 0; i < nb; ++i)
+      for (int j = 0; j < nb; ++j) c.h[(size_t)(ob + i) * nao + ob + j] += pb.e2a[i][j];
+    for (int i = 0; i < na; ++i) {
+      double bi = beta_of(ea, i);
+      for (int j = 0; j < nb; ++j) {
+        double bj = beta_of(eb, j);
+        double v = 0.5 * (bi + bj) * pb.s[i][j];
+        c.h[(size_t)(oa + i) * nao + ob + j] += v;
+        c.h[(size_t)(ob + j) * nao + oa + i] += v;
+      }
+    }
+  }
+  return c;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Fock build (src/fock.rs:69-107 one-centre integrals, 149-338 build_fock_spin_x)
+// ---------------------------------------------------------------------------------------------
+double oc_two_electron(int a, int b, int c, int d, const Element& e) {
+  if (a == b && c == d) {
+    if (a == 0 && c == 0) return e.g_ss;
+    if (a == 0 || c == 0) return e.g_sp;
+    return a == c ? e.g_pp : e.g_p2;
+  }
+  int ba = std::min(a, b), bb = std::max(a, b), kc = std::min(c, d), kd = std::max(c, d);
+  if (ba == 0 && bb != 0 && kc == 0 && kd != 0 && bb == kd) return e.h_sp;
+  if (ba != 0 && bb != 0 && ba != bb && ba == kc && bb == kd) return 0.5 * (e.g_pp - e.g_p2);
+  return 0.0;
+}
+
+Mat build_fock_spin(const Mol& m, const Core& c, const Mat& pt, const Mat& ps) {
+  int nao = m.nao, n = m.n();
+  Mat f = c.h;
+  auto P = [&](const Mat& x, int i, int j) { return x[(size_t)i * nao + j]; };
+  // One-centre blocks.
+  std::vector<std::vector<double>> oc(n);
+  parallel_for(n, [&](long ia) {
+    int off = m.off[ia], nn = m.norb[ia];
+    const Element& e = *m.el[ia];
+    oc[ia].assign(nn * nn, 0.0);
+    for (int mu = 0; mu < nn; ++mu)
+      for (int nu = 0; nu < nn; ++nu) {
+        double acc = 0.0;
+        for (int la = 0; la < nn; ++la)
+          for (int si = 0; si < nn; ++si) {
+            acc += P(pt, off + la, off + si) * oc_two_electron(mu, nu, la, si, e);
+            acc -= P(ps, off + la, off + si) * oc_two_electron(mu, la, nu, si, e);
+          }
+        oc[ia][mu * nn + nu] = acc;
+      }
+  });
+  for (int ia = 0; ia < n; ++ia) {
+    int off = m.off[ia], nn = m.norb[ia];
+    for (int mu = 0; mu < nn; ++mu)
+      for (int nu = 0; nu < nn; ++nu) f[(size_t)(off + mu) * nao + off + nu] += oc[ia][mu * nn + nu];
+  }
+  // Two-centre blocks.
+  struct TC {
+    double jaa[16], jbb[16], kab[16];
+  };
+  std::vector<TC> tc(c.pairs.size());
+  parallel_for((long)c.pairs.size(), [&](long p) {
+    const PairBlock& pb = c.pairs[p];
+    int oa = m.off[pb.a], ob = m.off[pb.b], na = m.norb[pb.a], nb = m.norb[pb.b];
+    int stride = pb.npb;
+    const double* w = pb.w.data();
+    TC& t = tc[p];
+    for (int mu = 0; mu < na; ++mu)
+      for (int nu = 0; nu < na; ++nu) {
+        int row = pack(mu, nu) * stride;
+        double acc = 0.0;
+        for (int la = 0; la < nb; ++la)
+          for (int si = 0; si < nb; ++si) acc += P(pt, ob + la, ob + si) * w[row + pack(la, si)];
+        t.jaa[mu * na + nu] = acc;
+      }
+    for (int la = 0; la < nb; ++la)
+      for (int si = 0; si < nb; ++si) {
+        int col = pack(la, si);
+        double acc = 0.0;
+        for (int mu = 0; mu < na; ++mu)
+          for (int nu = 0; nu < na; ++nu) acc += P(pt, oa + mu, oa + nu) * w[pack(mu, nu) * stride + col];
+        t.jbb[la * nb + si] = acc;
+      }
+    for (int mu = 0; mu < na; ++mu)
+      for (int la = 0; la < nb; ++la) {
+        double acc = 0.0;
+        for (int nu = 0; nu < na; ++nu) {
+          int row = pack(mu, nu) * stride;
+          for (int si = 0; si < nb; ++si) acc += P(ps, oa + nu, ob + si) * w[row + pack(la, si)];
+        }
+        t.kab[mu * nb + la] = -acc;
+      }
+  });
+  for (size_t p = 0; p < c.pairs.size(); ++p) {
+    const PairBlock& pb = c.pairs[p];
+    int oa = m.off[pb.a], ob = m.off[pb.b], na = m.norb[pb.a], nb = m.norb[pb.b];
+    const TC& t = tc[p];
+    for (int mu = 0; mu < na; ++mu)
+      for (int nu = 0; nu < na; ++nu) f[(size_t)(oa + mu) * nao + oa + nu] += t.jaa[mu * na + nu];
+    for (int la = 0; la < nb; ++la)
+      for (int si = 0; si < nb; ++si) f[(size_t)(ob + la) * nao + ob + si] += t.jbb[la * nb + si];
+    for (int mu = 0; mu < na; ++mu)
+      for (int la = 0; la < nb; ++la) {
+        double v = t.kab[mu * nb + la];
+        f[(size_t)(oa + mu) * nao + ob + la] += v;
+        f[(size_t)(ob + la) * nao + oa + mu] += v;
+      }
+  }
+  return f;
+}
+
+Mat build_fock_rhf(const Mol& m, const Core& c, const Mat& p) {  // src/fock.rs:648-686
+  Mat half = p;
+  for (auto& v : half) v *= 0.5;
+  return build_fock_spin(m, c, p, half);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Dense linear algebra
+// ---------------------------------------------------------------------------------------------
+double frob(const Mat& a, const Mat& b) {  // src/linalg.rs:138-141 (in-order fold)
+  double s = 0.0;
+  for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+  return s;
+}
+
+// Symmetric eigensolve: Householder tridiagonalisation with accumulated transforms followed by
+// the implicit-shift QL iteration (the EISPACK tred2/tql2 pair; Golub & Van Loan 8.3). Stands
+// in for faer's self_adjoint_eigen (src/linalg.rs:231-278): ascending eigenvalues, eigenvectors
+// as columns. Eigenvector signs are not pinned by the reference, only P and eps are.
+void eigh(int n, const Mat& a, std::vector<double>& w, Mat& vout) {
+  std::vector<double> V(a), d(n), e(n);
+  auto v = [&](int i, int j) -> double& { return V[(size_t)i * n + j]; };
+  if (n == 1) {
+    w.assign(1, a[0]);
+    vout.assign(1, 1.0);
+    return;
+  }
+  for (int j = 0; j < n; ++j) d[j] = v(n - 1, j);
+  for (int i = n - 1; i > 0; --i) {
+    double scale = 0.0, h = 0.0;
+    for (int k = 0; k < i; ++k) scale += std::fabs(d[k]);
+    if (scale == 0.0) {
+      e[i] = d[i - 1];
+      for (int j = 0; j < i; ++j) {
+        d[j] = v(i - 1, j);
+        v(i, j) = 0.0;
+        v(j, i) = 0.0;
+      }
+    } else {
+      for (int k = 0; k < i; ++k) {
+        d[k] /= scale;
+        h += d[k] * d[k];
+      }
+      double f = d[i - 1];
+      double g = std::sqrt(h);
+      if (f > 0) g = -g;
+      e[i] = scale * g;
+      h -= f * g;
+      d[i - 1] = f - g;
+      for (int j = 0; j < i; ++j) e[j] = 0.0;
+      for (int j = 0; j < i; ++j) {
+        f = d[j];
+        v(j, i) = f;
+        g = e[j] + v(j, j) * f;
+        for (int k = j + 1; k <= i - 1; ++k) {
+          g += v(k, j) * d[k];
+          e[k] += v(k, j) * f;
+        }
+        e[j] = g;
+      }
+      f = 0.0;
+      for (int j = 0; j < i; ++j) {
+        e[j] /= h;
+        f += e[j] * d[j];
+      }
+      double hh = f / (h + h);
+      for (int j = 0; j < i; ++j) e[j] -= hh * d[j];
+      for (int j = 0; j < i; ++j) {
+        f = d[j];
+        g = e[j];
+        for (int k = j; k <= i - 1; ++k) v(k, j) -= (f * e[k] + g * d[k]);
+        d[j] = v(i - 1, j);
+        v(i, j) = 0.0;
+      }
+    }
+    d[i] = h;
+  }
+  for (int i = 0; i < n - 1; ++i) {
+    v(n - 1, i) = v(i, i);
+    v(i, i) = 1.0;
+    double h = d[i + 1];
+    if (h != 0.0) {
+      for (int k = 0; k <= i; ++k) d[k] = v(k, i + 1) / h;
+      for (int j = 0; j <= i; ++j) {
+        double g = 0.0;
+        for (int k = 0; k <= i; ++k) g += v(k, i + 1) * v(k, j);
+        for (int k = 0; k <= i; ++k) v(k, j) -= g * d[k];
+      }
+    }
+    for (int k = 0; k <= i; ++k) v(k, i + 1) = 0.0;
+  }
+  for (int j = 0; j < n; ++j) {
+    d[j] = v(n - 1, j);
+    v(n - 1, j) = 0.0;
+  }
+  v(n - 1, n - 1) = 1.0;
+  e[0] = 0.0;
+  // tql2
+  for (int i = 1; i < n; ++i) e[i - 1] = e[i];
+  e[n - 1] = 0.0;
+  double f = 0.0, tst1 = 0.0;
+  const double eps = std::ldexp(1.0, -52);
+  for (int l = 0; l < n; ++l) {
+    tst1 = std::max(tst1, std::fabs(d[l]) + std::fabs(e[l]));
+    int mm = l;
+    while (mm < n) {
+      if (std::fabs(e[mm]) <= eps * tst1) break;
+      ++mm;
+    }
+    if (mm >= n) mm = n - 1;
+    if (mm > l) {
+      int iter = 0;
+      do {
+        ++iter;
+        double g = d[l];
+        double p = (d[l + 1] - g) / (2.0 * e[l]);
+        double r = std::hypot(p, 1.0);
+        if (p < 0) r = -r;
+        d[l] = e[l] / (p + r);
+        d[l + 1] = e[l] * (p + r);
+        double dl1 = d[l + 1];
+        double h = g - d[l];
+        for (int i = l + 2; i < n; ++i) d[i] -= h;
+        f += h;
+        p = d[mm];
+        double c = 1.0, c2 = c, c3 = c, el1 = e[l + 1], s = 0.0, s2 = 0.0;
+        for (int i = mm - 1; i >= l; --i) {
+          c3 = c2;
+          c2 = c;
+          s2 = s;
+          g = c * e[i];
+          h = c * p;
+          r = std::hypot(p, e[i]);
+          e[i + 1] = s * r;
+          s = e[i] / r;
+          c = p / r;
+          p = c * d[i] - s * g;
+          d[i + 1] = h + s * (c * g + s * d[i]);
+          for (int k = 0; k < n; ++k) {
+            h = v(k, i + 1);
+            v(k, i + 1) = s * v(k, i) + c * h;
+            v(k, i) = c * v(k, i) - s * h;
+          }
+        }
+        p = -s * s2 * c3 * el1 * e[l] / dl1;
+        e[l] = s * p;
+        d[l] = c * p;
+      } while (std::fabs(e[l]) > eps * tst1 && iter < 200);
+    }
+    d[l] += f;
+    e[l] = 0.0;
+  }
+  // ascending sort with vectors
+  std::vector<int> idx(n);
+  for (int i = 0; i < n; ++i) idx[i] = i;
+  std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return d[x] < d[y]; });
+  w.resize(n);
+  vout.assign((size_t)n * n, 0.0);
+  for (int j = 0; j < n; ++j) {
+    w[j] = d[idx[j]];
+    for (int i = 0; i < n; ++i) vout[(size_t)i * n + j] = v(i, idx[j]);
+  }
+}
+
+// P = w * C_occ C_occ^T (src/scf.rs:1121-1141)
+Mat density_from_coeff(const Mat& c, int n, int nocc, double weight) {
+  Mat p((size_t)n * n, 0.0);
+  parallel_for(n, [&](long mu) {
+    for (int nu = 0; nu < n; ++nu) {
+      double dotv = 0.0;
+      for (int k = 0; k < nocc; ++k) dotv += c[(size_t)mu * n + k] * c[(size_t)nu * n + k];
+      p[(size_t)mu * n + nu] = weight * dotv;
+    }
+  });
+  return p;
+}
+
+// [F,P] = FP - (FP)^T (src/scf.rs:1148-1156)
+void commutator(const Mat& f, const Mat& p, int n, Mat& out) {
+  Mat fp((size_t)n * n, 0.0);
+  parallel_for(n, [&](long i) {
+    for (int k = 0; k < n; ++k) {
+      double a = f[(size_t)i * n + k];
+      for (int j = 0; j < n; ++j) fp[(size_t)i * n + j] += a * p[(size_t)k * n + j];
+    }
+  });
+  out.resize((size_t)n * n);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) out[(size_t)i * n + j] = fp[(size_t)i * n + j] - fp[(size_t)j * n + i];
+}
+
+double rms_diff(const Mat& a, const Mat& b) {  // src/scf.rs:1178-1187
+  double s = 0.0;
+  for (size_t i = 0; i < a.size(); ++i) s += (a[i] - b[i]) * (a[i] - b[i]);
+  return std::sqrt(s / (double)std::max<size_t>(1, a.size()));
+}
+
+// ---------------------------------------------------------------------------------------------
+// SCF accelerators (src/scf.rs:1534-1916)
+// ---------------------------------------------------------------------------------------------
+const double MAX_LEVEL_SHIFT = 64.0;
+const int STALL_WINDOW = 5;
+const double ENERGY_PROGRESS_EV = 1.0e-9;
+const double DIIS_MAX_WEIGHT = 25.0;
+
+void apply_level_shift(Mat& f, const Mat& p, int n, double occ, double shift) {
+  if (shift <= 0.0) return;
+  double inv = 1.0 / occ;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      double q = p[(size_t)i * n + j] * inv;
+      f[(size_t)i * n + j] += shift * ((i == j) ? 1.0 - q : -q);
+    }
+}
+
+struct StallGuard {  // src/scf.rs:1581-1656
+  double floor_, shift;
+  std::vector<double> steps, energies;
+  explicit StallGuard(double fl) : floor_(fl), shift(fl) {}
+  void update(double step, double energy) {
+    steps.push_back(step);
+    energies.push_back(energy);
+    size_t n = steps.size();
+    if (n < 2 * STALL_WINDOW) return;
+    auto smallest = [](const std::vector<double>& v, size_t lo, size_t hi) {
+      double m = INFINITY;
+      for (size_t i = lo; i < hi; ++i) m = std::min(m, v[i]);
+      return m;
+    };
+    double step_now = smallest(steps, n - STALL_WINDOW, n);
+    double step_was = smallest(steps, n - 2 * STALL_WINDOW, n - STALL_WINDOW);
+    double e_now = smallest(energies, n - STALL_WINDOW, n);
+    double e_was = smallest(energies, n - 2 * STALL_WINDOW, n - STALL_WINDOW);
+    bool moved;
+    if (step_now <= step_was * 0.1) {
+      if (shift <= floor_) {
+        moved = false;
+      } else {
+        double relaxed = std::max(shift * 0.5, floor_);
+        shift = relaxed < 0.25 ? floor_ : relaxed;
+        moved = true;
+      }
+    } else if (step_now > step_was * 0.9 && e_now > e_was - ENERGY_PROGRESS_EV && shift < MAX_LEVEL_SHIFT) {
+      shift = shift < 1.0 ? 1.0 : std::min(shift * 2.0, MAX_LEVEL_SHIFT);
+      moved = true;
+    } else {
+      moved = false;
+    }
+    if (moved) {
+      steps.clear();
+      energies.clear();
+    } else {
+      steps.erase(steps.begin());
+      energies.erase(energies.begin());
+    }
+  }
+};
+
+bool solve_bordered_small(std::vector<double> m, std::vector<double> rhs, int n, std::vector<double>& x) {
+  for (int col = 0; col < n; ++col) {
+    int piv = col;
+    double best = std::fabs(m[col * n + col]);
+    for (int row = col + 1; row < n; ++row) {
+      double v = std::fabs(m[row * n + col]);
+      if (v > best) {
+        best = v;
+        piv = row;
+      }
+    }
+    if (best < 1.0e-12) return false;
+    if (piv != col) {
+      for (int j = 0; j < n; ++j) std::swap(m[col * n + j], m[piv * n + j]);
+      std::swap(rhs[col], rhs[piv]);
+    }
+    for (int row = col + 1; row < n; ++row) {
+      double factor = m[row * n + col] / m[col * n + col];
+      if (factor == 0.0) continue;
+      for (int j = col; j < n; ++j) m[row * n + j] -= factor * m[col * n + j];
+      rhs[row] -= factor * rhs[col];
+    }
+  }
+  x.assign(n, 0.0);
+  for (int col = n - 1; col >= 0; --col) {
+    double sum = rhs[col];
+    for (int j = col + 1; j < n; ++j) sum -= m[col * n + j] * x[j];
+    x[col] = sum / m[col * n + col];
+  }
+  return true;
+}
+
+bool diis_window_coeffs(const std::vector<double>& gram, int n, int skip, std::vector<double>& c) {
+  int m = n - skip;
+  if (m < 2) return false;
+  double scale = 0.0;
+  for (int i = 0; i < m; ++i) scale = std::max(scale, std::fabs(gram[(skip + i) * n + skip + i]));
+  if (!std::isfinite(scale) || scale <= 0.0) return false;
+  int dim = m + 1;
+  std::vector<double> b((size_t)dim * dim, 0.0), rhs(dim, 0.0);
+  for (int i = 0; i < m; ++i) {
+    for (int j = 0; j < m; ++j) b[i * dim + j] = gram[(skip + i) * n + skip + j] / scale;
+    b[i * dim + m] = -1.0;
+    b[m * dim + i] = -1.0;
+  }
+  rhs[m] = -1.0;
+  if (!solve_bordered_small(b, rhs, dim, c)) return false;
+  for (double v : c)
+    if (!std::isfinite(v)) return false;
+  return true;
+}
+
+bool diis_coeffs(const std::vector<Mat>& es, int& skip_out, std::vector<double>& coeffs) {
+  int n = (int)es.size();
+  if (n < 2) return false;
+  std::vector<double> gram((size_t)n * n);
+  for (int i = 0; i < n; ++i)
+    for (int j = i; j < n; ++j) gram[i * n + j] = gram[j * n + i] = frob(es[i], es[j]);
+  for (int skip = 0; skip < n - 1; ++skip) {
+    std::vector<double> c;
+    if (diis_window_coeffs(gram, n, skip, c)) {
+      double sum = 0.0;
+      for (int i = 0; i < n - skip; ++i) sum += std::fabs(c[i]);
+      if (sum <= DIIS_MAX_WEIGHT) {
+        skip_out = skip;
+        coeffs = c;
+        return true;
+      }
+    }
+  }
+  return false;
+}
+
+Mat combine(const std::vector<Mat>& fs, int from, const std::vector<double>& coeffs) {
+  Mat out(fs[from].size(), 0.0);
+  for (size_t i = from; i < fs.size(); ++i) {
+    double ci = coeffs[i - from];
+    for (size_t k = 0; k < out.size(); ++k) out[k] += ci * fs[i][k];
+  }
+  return out;
+}
+
+std::vector<double> simplex_project(const std::vector<double>& v) {
+  std::vector<double> u(v);
+  std::sort(u.begin(), u.end(), [](double a, double b) { return a > b; });
+  double css = 0.0, theta = 0.0;
+  for (size_t j = 0; j < u.size(); ++j) {
+    css += u[j];
+    double t = (css - 1.0) / (double)(j + 1);
+    if (u[j] - t > 0.0) theta = t;
+  }
+  std::vector<double> out(v.size());
+  for (size_t i = 0; i < v.size(); ++i) out[i] = std::max(v[i] - theta, 0.0);
+  return out;
+}
+
+std::vector<double> solve_adiis_simplex(const std::vector<double>& d, const std::vector<std::vector<double>>& s) {
+  int n = (int)d.size();
+  double l = 1.0e-12;
+  for (int i = 0; i < n; ++i) {
+    double row = 0.0;
+    for (int j = 0; j < n; ++j) row += std::fabs(s[i][j] + s[j][i]);
+    l = std::max(l, row);
+  }
+  double lr = 1.0 / l;
+  std::vector<double> c(n, 0.0);
+  c[n - 1] = 1.0;
+  for (int it = 0; it < 400; ++it) {
+    std::vector<double> g(n), trial(n);
+    for (int k = 0; k < n; ++k) {
+      double acc = 2.0 * d[k];
+      for (int j = 0; j < n; ++j) acc += (s[k][j] + s[j][k]) * c[j];
+      g[k] = acc;
+    }
+    for (int i = 0; i < n; ++i) trial[i] = c[i] - lr * g[i];
+    auto proj = simplex_project(trial);
+    double delta = 0.0;
+    for (int i = 0; i < n; ++i) delta += std::fabs(proj[i] - c[i]);
+    c = proj;
+    if (delta < 1.0e-12) break;
+  }
+  return c;
+}
+
+bool adiis_extrapolate(const std::vector<Mat>& D, const std::vector<Mat>& F, Mat& out) {
+  int n = (int)D.size();
+  if (n < 2) return false;
+  const Mat& dn = D[n - 1];
+  const Mat& fn = F[n - 1];
+  std::vector<double> d(n);
+  std::vector<std::vector<double>> s(n, std::vector<double>(n));
+  for (int i = 0; i < n; ++i) {
+    double acc = 0.0;
+    for (size_t k = 0; k < dn.size(); ++k) acc += (D[i][k] - dn[k]) * fn[k];
+    d[i] = acc;
+    for (int j = 0; j < n; ++j) {
+      double a2 = 0.0;
+      for (size_t k = 0; k < dn.size(); ++k) a2 += (D[i][k] - dn[k]) * (F[j][k] - fn[k]);
+      s[i][j] = a2;
+    }
+  }
+  auto c = solve_adiis_simplex(d, s);
+  out = combine(F, 0, c);
+  return true;
+}
+
+void push_hist(std::vector<Mat>& h, const Mat& m, size_t max) {
+  if (h.size() >= max) h.erase(h.begin());
+  h.push_back(m);
+}
+
+// MOPAC diagonal guess (src/scf.rs:1050-1114), sp atoms only.
+Mat sad_density(const Mol& m, double n_electrons) {
+  int nao = m.nao;
+  Mat p((size_t)nao * nao, 0.0);
+  double tore_sum = 0.0;
+  for (int i = 0; i < m.n(); ++i) tore_sum += m.el[i]->core_charge;
+  double yy = nao > 0 ? (tore_sum - n_electrons) / nao : 0.0;
+  for (int i = 0; i < m.n(); ++i) {
+    int off = m.off[i];
+    double tore = m.el[i]->core_charge;
+    if (m.norb[i] == 1) {
+      p[(size_t)off * nao + off] = tore - yy;
+    } else if (m.norb[i] == 4) {
+      double w = tore * 0.25 - yy;
+      for (int k = 0; k < 4; ++k) p[(size_t)(off + k) * nao + off + k] = w;
+    }
+  }
+  return p;
+}
+
+struct ScfState {
+  Mat density, spin;
+  std::vector<double> eps, eps_b;
+  int n_occ = 0, n_occ_b = 0;
+  double e_el = 0, dp_err = INFINITY;
+  bool converged = false, uhf = false;
+  int iterations = 0;
+};
+
+ScfState rhf_loop(const Mol& m, const Core& core, int n_occ, const oracle_options& o) {  // :1189-1322
+  int nao = m.nao;
+  ScfState st;
+  st.n_occ = n_occ;
+  Mat density = sad_density(m, 2.0 * n_occ);
+  double e_old = 0.0;
+  std::vector<Mat> hf, he, hd;
+  int accel = o.use_diis ? o.accelerator : 0;
+  StallGuard guard(o.level_shift);
+  Mat mo;
+  std::vector<double> eps;
+  for (int iter = 0; iter < o.max_scf; ++iter) {
+    st.iterations = iter + 1;
+    Mat f = build_fock_rhf(m, core, density);
+    double e = 0.5 * (frob(density, core.h) + frob(density, f));
+    push_hist(hf, f, 8);
+    Mat err;
+    commutator(f, density, nao, err);
+    push_hist(he, err, 8);
+    push_hist(hd, density, 8);
+    double err_norm = std::sqrt(frob(err, err));
+    Mat fuse;
+    bool ok = false;
+    if (accel == 1 || (accel == 2 && err_norm <= o.adiis_switch)) {
+      int skip;
+      std::vector<double> c;
+      if (diis_coeffs(he, skip, c)) {
+        fuse = combine(hf, skip, c);
+        ok = true;
+      }
+    } else if (accel == 2) {
+      ok = adiis_extrapolate(hd, hf, fuse);
+    }
+    if (!ok) fuse = f;
+    apply_level_shift(fuse, density, nao, 2.0, guard.shift);
+    Mat c;
+    eigh(nao, fuse, eps, c);
+    Mat pn = density_from_coeff(c, nao, n_occ, 2.0);
+    double dp = rms_diff(pn, density);
+    st.dp_err = dp;
+    double de = std::fabs(e - e_old);
+    density = pn;
+    mo = c;
+    e_old = e;
+    bool residual_ok = guard.shift == 0.0 && err_norm < 1.0e-7;
+    if (iter > 0 && de < o.e_tol && (dp < o.p_tol || residual_ok)) {
+      st.converged = true;
+      break;
+    }
+    guard.update(dp, e);
+  }
+  Mat ff = build_fock_rhf(m, core, density);
+  st.e_el = 0.5 * (frob(density, core.h) + frob(density, ff));
+  if (guard.shift > 0.0) eigh(nao, ff, eps, mo);
+  st.density = density;
+  st.eps = eps;
+  return st;
+}
+
+ScfState uhf_loop(const Mol& m, const Core& core, int na, int nb, const oracle_options& o) {  // :1325-1532
+  int nao = m.nao;
+  ScfState st;
+  st.uhf = true;
+  st.n_occ = na;
+  st.n_occ_b = nb;
+  Mat total = sad_density(m, (double)(na + nb));
+  double ntot = std::max(na + nb, 1);
+  double fa = na / ntot, fb = nb / ntot;
+  Mat pa = total, pb = total;
+  for (auto& v : pa) v *= fa;
+  for (auto& v : pb) v *= fb;
+  double e_old = 0.0;
+  StallGuard guard(o.level_shift);
+  std::vector<double> eps_a;
+  Mat ca;
+  for (int iter = 0; iter < o.max_scf; ++iter) {
+    st.iterations = iter + 1;
+    Mat pt(pa);
+    for (size_t k = 0; k < pt.size(); ++k) pt[k] += pb[k];
+    Mat Fa = build_fock_spin(m, core, pt, pa);
+    Mat Fb = build_fock_spin(m, core, pt, pb);
+    double e = 0.5 * (frob(pt, core.h) + frob(pa, Fa) + frob(pb, Fb));
+    Mat ea, eb;
+    commutator(Fa, pa, nao, ea);
+    commutator(Fb, pb, nao, eb);
+    double s = 0.0;
+    for (double v : ea) s += v * v;
+    for (double v : eb) s += v * v;
+    double err_norm = std::sqrt(s);
+    apply_level_shift(Fa, pa, nao, 1.0, guard.shift);
+    apply_level_shift(Fb, pb, nao, 1.0, guard.shift);
+    std::vector<double> wa, wb;
+    Mat cA, cB;
+    eigh(nao, Fa, wa, cA);
+    eigh(nao, Fb, wb, cB);
+    Mat pan = density_from_coeff(cA, nao, na, 1.0);
+    Mat pbn = density_from_coeff(cB, nao, nb, 1.0);
+    double dp = rms_diff(pan, pa) + rms_diff(pbn, pb);
+    st.dp_err = dp;
+    double de = std::fabs(e - e_old);
+    eps_a = wa;
+    ca = cA;
+    pa = pan;
+    pb = pbn;
+    e_old = e;
+    bool residual_ok = guard.shift == 0.0 && err_norm < 1.0e-7;
+    if (iter > 0 && de < o.e_tol && (dp < o.p_tol || residual_ok)) {
+      st.converged = true;
+      break;
+    }
+    guard.update(dp, e);
+  }
+  Mat dens(pa), spin(pa);
+  for (size_t k = 0; k < dens.size(); ++k) {
+    dens[k] += pb[k];
+    spin[k] -= pb[k];
+  }
+  Mat Fa = build_fock_spin(m, core, dens, pa);
+  Mat Fb = build_fock_spin(m, core, dens, pb);
+  st.e_el = 0.5 * (frob(dens, core.h) + frob(pa, Fa) + frob(pb, Fb));
+  if (guard.shift > 0.0) eigh(nao, Fa, eps_a, ca);
+  Mat cb;
+  eigh(nao, Fb, st.eps_b, cb);
+  st.density = dens;
+  st.spin = spin;
+  st.eps = eps_a;
+  return st;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Core-core repulsion (src/repulsion.rs:16-36, 164-263)
+// ---------------------------------------------------------------------------------------------
+double core_gaussian_slot(const Element& src, double zz, double r_ang, size_t slot) {
+  if (slot >= src.gauss.size()) return 0.0;
+  double k = src.gauss[slot][0], l = src.gauss[slot][1], mm = src.gauss[slot][2];
+  double ax = (r_ang - mm) * (r_ang - mm) * l;
+  if (ax >= 25.0) return 0.0;
+  return std::exp(-ax) * (zz * k) / r_ang;
+}
+
+double pair_core_energy(const Params& P, const Element& ei, const Element& ej, double r) {
+  int zi = ei.z, zj = ej.z;
+  double r_ang = r * A0;
+  double zz = ei.core_charge * ej.core_charge;
+  double gab_nddo = EV / std::sqrt(r * r + (ei.po9() + ej.po9()) * (ei.po9() + ej.po9()));
+  double cfrac, point;
+  feather_to_point(r, cfrac, point);
+  double gab = gab_nddo * cfrac + point * (1.0 - cfrac);
+  double enuc = gab * zz;
+  auto pr = P.pair(zi, zj);
+  double alpb0 = pr.first, xfac = pr.second;
+  bool defined = std::fabs(xfac) > 1.0e-5;
+  if (defined) {
+    double alpb = alpb0 < 1.0e-6 ? 1.2 : alpb0;
+    double scale = 1.0 + std::exp(-(r_ang + std::pow(r_ang, 6) * 0.0003) * alpb) * (2.0 * xfac);
+    int hi = std::max(zi, zj), lo = std::min(zi, zj);
+    if (lo == 1 && (hi == 6 || hi == 7)) {
+      scale = 1.0 + std::exp(-(r_ang * r_ang) * alpb) * (2.0 * xfac);
+    } else if (lo == 1 && hi == 8) {
+      scale = 1.0 + std::exp(-(r_ang * r_ang) * alpb) * (2.0 * xfac) -
+              std::exp(r_ang * (-2.0 * P.vpar[4])) * P.vpar[3];
+    } else if (lo == 6 && hi == 6) {
+      scale = scale + std::exp(r_ang * (-P.vpar[2])) * P.vpar[1];
+    } else if (lo == 8 && hi == 14) {
+      scale = scale - std::exp(-(r_ang - 2.9) * (r_ang - 2.9)) * 0.0007;
+    }
+    enuc *= scale;
+  } else {
+    bool sparkle = (zi >= 57 && zi <= 71) || (zj >= 57 && zj <= 71);
+    double ex = sparkle ? -3.0 : -2.18;
+    enuc *= 1.0 + std::exp(r_ang * ex) * 10.0;
+  }
+  double corr = core_gaussian_slot(ei, zz, r_ang, 0);
+  corr += core_gaussian_slot(ej, zz, r_ang, 0);
+  if (!defined)
+    for (size_t s = 0; s < 4; ++s) {
+      corr += core_gaussian_slot(ei, zz, r_ang, s);
+      corr += core_gaussian_slot(ej, zz, r_ang, s);
+    }
+  double reduced = r_ang / (std::pow((double)zi, 0.3333) + std::pow((double)zj, 0.3333));
+  if (reduced < 3.0) {
+    double guard = 1.0e-8 / std::pow(reduced, 12);
+    corr += guard < 1.0e5 ? guard : 1.0e5;
+  }
+  return enuc + corr;
+}
+
+double core_core_energy(const Params& P, const Mol& m) {
+  double sum = 0.0;
+  for (int i = 0; i < m.n(); ++i)
+    for (int j = i + 1; j < m.n(); ++j) sum += pair_core_energy(P, *m.el[i], *m.el[j], norm(sub(m.pos[j], m.pos[i])));
+  return sum;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Post-SCF geometric corrections (kcal/mol)
+// ---------------------------------------------------------------------------------------------
+// Covalent radii (src/constants.rs covalent_radius_angstrom), Z = 0..20.
+double covalent_radius(int z) {
+  static const double R[21] = {0.0,  0.31, 0.28, 1.28, 0.96, 0.84, 0.76, 0.71, 0.66, 0.57, 0.58,
+                               1.66, 1.41, 1.21, 1.11, 1.07, 1.05, 1.02, 1.06, 2.03, 1.76};
+  return (z >= 0 && z <= 20) ? R[z] : 1.5;
+}
+
+// D2 dispersion (src/dispersion.rs:19-264).
+double dispersion_energy(const std::vector<int>& z, const std::vector<V3>& pos) {
+  static const double ALPHA = 15.450118, S = 1.226593, CSCALE = 2.286419;
+  auto c6tab = [](int zz) {
+    switch (zz) {
+      case 1: return 0.16; case 2: return 0.084; case 5: return 5.79; case 6: return 1.65;
+      case 7: return 1.11; case 8: return 0.70; case 9: return 0.57; case 10: return 0.45;
+      case 15: return 3.25; case 16: return 5.79; case 17: return 5.97; case 18: return 3.71;
+      case 27: return 0.04; case 35: return 11.60; case 36: return 4.47; case 53: return 25.80;
+      case 54: return 16.50; default: return 0.0;
+    }
+  };
+  auto r0tab = [](int zz) {
+    switch (zz) {
+      case 1: return 156.0; case 2: return 140.0; case 5: return 180.0; case 6: return 170.0;
+      case 7: return 155.0; case 8: return 152.0; case 9: return 147.0; case 10: return 154.0;
+      case 15: return 180.0; case 16: return 180.0; case 17: return 175.0; case 18: return 188.0;
+      case 28: return 140.0; case 35: return 185.0; case 36: return 202.0; case 53: return 198.0;
+      case 54: return 216.0; default: return 0.0;
+    }
+  };
+  auto nefftab = [](int zz) {
+    switch (zz) {
+      case 1: return 0.80; case 2: return 1.42; case 5: return 2.16; case 6: return 2.50;
+      case 7: return 2.82; case 8: return 3.15; case 9: return 3.48; case 10: return 3.81;
+      case 15: return 4.50; case 16: return 4.80; case 17: return 5.10; case 18: return 5.40;
+      case 27: return 2.90; case 35: return 6.00; case 36: return 6.30; case 53: return 6.95;
+      case 54: return 7.25; default: return 0.0;
+    }
+  };
+  int n = (int)z.size();
+  std::vector<int> nb(n, 0);
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      double d = norm(sub(pos[j], pos[i]));
+      if (d * A0 < 1.3 * (covalent_radius(z[i]) + covalent_radius(z[j]))) {
+        nb[i]++;
+        nb[j]++;
+      }
+    }
+  double sum = 0.0;
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      int zi = z[i], zj = z[j];
+      bool ok = c6tab(zi) != 0 && r0tab(zi) != 0 && nefftab(zi) != 0 && c6tab(zj) != 0 &&
+                r0tab(zj) != 0 && nefftab(zj) != 0;
+      if (!ok) continue;
+      double ni = nefftab(zi), nj = nefftab(zj);
+      double c6i = zi == 6 ? (nb[i] == 4 ? 0.95 : 1.65) : c6tab(zi);
+      double c6j = zj == 6 ? (nb[j] == 4 ? 0.95 : 1.65) : c6tab(zj);
+      double c6 = 2.0 * std::pow(c6i * c6i * c6j * c6j * ni * nj, 1.0 / 3.0) /
+                  (std::pow(c6i * nj * nj, 1.0 / 3.0) + std::pow(c6j * ni * ni, 1.0 / 3.0));
+      double ri = r0tab(zi), rj = r0tab(zj);
+      double r0 = (ri * ri * ri + rj * rj * rj) / (ri * ri + rj * rj) / 1000.0 * 2.0;
+      double r = norm(sub(pos[j], pos[i]));
+      double rij_nm = r * (A0 * 0.1);
+      double damp_denom = std::exp((rij_nm * (1.0 / (S * r0)) - 1.0) * (-ALPHA)) + 1.0;
+      double e = (1.0 / std::pow(rij_nm, 6)) * (c6 / (1000.0 * 4.184)) / damp_denom;
+      sum += e * (-CSCALE);
+    }
+  return sum;
+}
+
+// EH+ hydrogen-bond correction (src/hbond.rs:53-626, 1294-1322), brute-force scans in ascending
+// index order (the reference's spatial grid only pre-filters the same scans).
+namespace hb {
+const double MOPAC_TWO_PI = 6.28318530717959;
+const double SHORTCUT = 2.4, LONGCUT = 7.0, COVCUT = 1.2;
+double deg(double d) { return PI / (180.0 / d); }
+double dist(const std::vector<V3>& c, int a, int b) { return norm(sub(c[a], c[b])); }
+double bangle(const std::vector<V3>& c, int i, int j, int k) {
+  V3 a = sub(c[i], c[j]), b = sub(c[j], c[k]), d = sub(c[i], c[k]);
+  double d2ij = dot(a, a), d2jk = dot(b, b), d2ik = dot(d, d);
+  double xy = std::sqrt(d2ij * d2jk);
+  if (xy < 1.0e-20) return 0.0;
+  double t = std::min(1.0, std::max(-1.0, 0.5 * (d2ij + d2jk - d2ik) / xy));
+  return std::acos(t);
+}
+double dang(double a1, double a2, double b1, double b2) {
+  const double zero = 1.0e-6;
+  if ((std::fabs(a1) >= zero || std::fabs(a2) >= zero) && (std::fabs(b1) >= zero || std::fabs(b2) >= zero)) {
+    double an = 1.0 / std::sqrt(a1 * a1 + a2 * a2), bn = 1.0 / std::sqrt(b1 * b1 + b2 * b2);
+    a1 *= an;
+    a2 *= an;
+    b1 *= bn;
+    b2 *= bn;
+    double sinth = a1 * b2 - a2 * b1;
+    double costh = std::min(1.0, std::max(-1.0, a1 * b1 + a2 * b2));
+    double rcos = std::acos(costh);
+    if (std::fabs(rcos) >= 4.0e-5) {
+      if (sinth > 0.0) rcos = MOPAC_TWO_PI - rcos;
+      return -rcos;
+    }
+  }
+  return 0.0;
+}
+double dihed(const std::vector<V3>& c, int i, int j, int k, int l) {
+  double xi1 = c[i].x - c[k].x, xj1 = c[j].x - c[k].x, xl1 = c[l].x - c[k].x;
+  double yi1 = c[i].y - c[k].y, yj1 = c[j].y - c[k].y, yl1 = c[l].y - c[k].y;
+  double zi1 = c[i].z - c[k].z, zj1 = c[j].z - c[k].z, zl1 = c[l].z - c[k].z;
+  double dd = std::sqrt(xj1 * xj1 + yj1 * yj1 + zj1 * zj1);
+  double cosa = dd > 0.0 ? std::min(1.0, std::max(-1.0, zj1 / dd)) : 0.0;
+  double ddd = 1.0 - cosa * cosa;
+  double xi2, xl2, yi2, yl2, costh, sinth;
+  double yxdist = ddd > 0.0 ? dd * std::sqrt(ddd) : 0.0;
+  if (ddd > 0.0 && yxdist > 1.0e-6) {
+    double cosph = yj1 / yxdist, sinph = xj1 / yxdist;
+    xi2 = xi1 * cosph - yi1 * sinph;
+    xl2 = xl1 * cosph - yl1 * sinph;
+    yi2 = xi1 * sinph + yi1 * cosph;
+    double yj2 = xj1 * sinph + yj1 * cosph;
+    yl2 = xl1 * sinph + yl1 * cosph;
+    costh = cosa;
+    sinth = yj2 / dd;
+  } else {
+    xi2 = xi1;
+    xl2 = xl1;
+    yi2 = yi1;
+    yl2 = yl1;
+    costh = cosa;
+    sinth = 0.0;
+  }
+  double yi3 = yi2 * costh - zi1 * sinth;
+  double yl3 = yl2 * costh - zl1 * sinth;
+  double angle = dang(xl2, yl3, xi2, yi3);
+  if (angle < 0.0) angle += 2.0 * PI;
+  if (angle >= MOPAC_TWO_PI) angle = 0.0;
+  return angle;
+}
+double covrad_scaled(int z) {
+  static const double COVRAD[94] = {
+      0.32, 0.46, 1.20, 0.94, 0.77, 0.75, 0.71, 0.63, 0.64, 0.67, 1.40, 1.25, 1.13, 1.04, 1.10,
+      1.02, 0.99, 0.96, 1.76, 1.54, 1.33, 1.22, 1.21, 1.10, 1.07, 1.04, 1.00, 0.99, 1.01, 1.09,
+      1.12, 1.09, 1.15, 1.10, 1.14, 1.17, 1.89, 1.67, 1.47, 1.39, 1.32, 1.24, 1.15, 1.13, 1.13,
+      1.08, 1.15, 1.23, 1.28, 1.26, 1.26, 1.23, 1.32, 1.31, 2.09, 1.76, 1.62, 1.47, 1.58, 1.57,
+      1.56, 1.55, 1.51, 1.52, 1.51, 1.50, 1.49, 1.49, 1.48, 1.53, 1.46, 1.37, 1.31, 1.23, 1.18,
+      1.16, 1.11, 1.12, 1.13, 1.32, 1.30, 1.30, 1.36, 1.31, 1.38, 1.42, 2.01, 1.81, 1.67, 1.58,
+      1.52, 1.53, 1.54, 1.55};
+  if (z <= 0 || z > 94) return 1.5 * 4.0 / 3.0;
+  return COVRAD[z - 1] * 4.0 / 3.0;
+}
+double bonding(const std::vector<int>& nat, int x, int y) { return covrad_scaled(nat[x]) + covrad_scaled(nat[y]); }
+
+struct HBond {
+  int s[9];
+  int nra, nrb;
+  bool disabled;
+};
+
+std::vector<int> neighbours_capped(const std::vector<V3>& c, const std::vector<int>& nat, int atom) {
+  std::vector<int> list;
+  for (int j = 0; j < (int)c.size(); ++j) {
+    if (j != atom && dist(c, j, atom) < bonding(nat, j, atom)) {
+      list.push_back(j);
+      if (list.size() == 5) {
+        int worst = 0;
+        double wd = -1.0;
+        for (int k = 0; k < 5; ++k) {
+          double d = dist(c, atom, list[k]);
+          if (d > wd) {
+            wd = d;
+            worst = k;
+          }
+        }
+        list.erase(list.begin() + worst);
+      }
+    }
+  }
+  return list;
+}
+std::vector<int> neighbours_all(const std::vector<V3>& c, const std::vector<int>& nat, int atom) {
+  std::vector<int> list;
+  for (int k = 0; k < (int)c.size(); ++k)
+    if (k != atom && dist(c, k, atom) < bonding(nat, k, atom)) list.push_back(k);
+  return list;
+}
+void assign_refs(const std::vector<V3>& c, const std::vector<int>& nat, const std::vector<int>& nb, int x, int h,
+                 int* s, int base) {
+  int i2 = base, i3 = base + 1, i4 = base + 2;
+  auto dh = [&](int a) { return dist(c, a, h); };
+  int n = (int)nb.size();
+  if (n == 3 || n == 4) {
+    std::vector<int> order(nb);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return dh(b) < dh(a); });
+    s[i2] = order[0];
+    s[i3] = order[1];
+    s[i4] = order[2];
+  } else if (n == 2) {
+    int far, other;
+    if (dh(nb[0]) >= dh(nb[1])) {
+      far = nb[0];
+      other = nb[1];
+    } else {
+      far = nb[1];
+      other = nb[0];
+    }
+    s[i2] = far;
+    s[i3] = other;
+    s[i4] = dist(c, x, h) < bonding(nat, x, h) ? h : x;
+  } else if (n == 1) {
+    s[i2] = nb[0];
+    auto nc = neighbours_all(c, nat, nb[0]);
+    int best = nb[0];
+    double bd = -1.0;
+    for (int k : nc) {
+      double d = dh(k);
+      if (d > bd) {
+        bd = d;
+        best = k;
+      }
+    }
+    s[i3] = best;
+    s[i4] = dist(c, x, h) < bonding(nat, x, h) ? h : x;
+  } else {
+    int fill = dist(c, x, h) < bonding(nat, x, h) ? h : x;
+    s[i2] = s[i3] = s[i4] = fill;
+  }
+}
+std::vector<HBond> build_hbonds(const std::vector<V3>& c, const std::vector<int>& nat) {
+  int numat = (int)c.size();
+  bool any = false;
+  for (int zz : nat) any |= (zz == 7 || zz == 8);
+  if (!any) return {};
+  const double rah2 = 1.4 * 1.4;
+  std::vector<int> acceptors, bonding_h;
+  std::vector<char> used(numat, 0);
+  for (int i = 0; i < numat; ++i) {
+    if (nat[i] == 7 || nat[i] == 8) {
+      acceptors.push_back(i);
+      for (int j = 0; j < numat; ++j) {
+        if (nat[j] != 1) continue;
+        V3 d = sub(c[i], c[j]);
+        if (!used[j] && dot(d, d) < rah2) {
+          bonding_h.push_back(j);
+          used[j] = 1;
+        }
+      }
+    }
+  }
+  const double cutoff2 = 49.0;
+  std::vector<std::tuple<int, int, int>> pairs;
+  std::set<std::tuple<int, int, int>> seen;
+  for (int i : acceptors) {
+    for (int j : bonding_h) {
+      V3 dij = sub(c[i], c[j]);
+      if (dot(dij, dij) >= rah2) continue;
+      for (int k : acceptors) {
+        if (k == i) continue;
+        V3 dki = sub(c[k], c[i]);
+        if (dot(dki, dki) >= cutoff2) continue;
+        if (bangle(c, k, j, i) <= PI * 0.5) continue;
+        auto key = std::make_tuple(j, std::min(i, k), std::max(i, k));
+        if (seen.count(key)) continue;
+        seen.insert(key);
+        pairs.push_back({i, j, k});
+      }
+    }
+  }
+  std::vector<HBond> out;
+  for (auto& t : pairs) {
+    int a1 = std::get<0>(t), h = std::get<1>(t), a5 = std::get<2>(t);
+    HBond b;
+    for (int k = 0; k < 9; ++k) b.s[k] = 0;
+    b.s[0] = a1;
+    b.s[4] = a5;
+    b.s[8] = h;
+    auto na = neighbours_capped(c, nat, a1);
+    auto nb = neighbours_capped(c, nat, a5);
+    b.nra = (int)na.size();
+    b.nrb = (int)nb.size();
+    bool dis = false;
+    for (int j : na) {
+      if (j == a5) dis = true;
+      for (int kk : nb)
+        if (j == kk && kk != h) dis = true;
+    }
+    b.disabled = dis;
+    if (!dis) {
+      assign_refs(c, nat, na, a1, h, b.s, 1);
+      assign_refs(c, nat, nb, a5, h, b.s, 5);
+    }
+    out.push_back(b);
+  }
+  return out;
+}
+double wrap_pi(double v) {
+  if (v <= -PI) v += 2.0 * PI;
+  if (v > PI) v -= 2.0 * PI;
+  return v;
+}
+bool side_terms(const std::vector<V3>& c, const std::vector<int>& nat, int nrbonds, int heavy, int r1, int r2, int r3,
+                int h, double& a2c, double& tcos) {
+  double angle2_shift = 0.0, angle2_shift_2 = 0.0, torsion_shift = 0.0;
+  bool tcs = false, tcs2 = false;
+  if (nat[heavy] == 8) {
+    if (nrbonds == 1) {
+      angle2_shift = PI;
+      angle2_shift_2 = deg(120.0);
+      torsion_shift = 0.0;
+      tcs2 = true;
+    } else {
+      angle2_shift = deg(109.48);
+      angle2_shift_2 = angle2_shift;
+      torsion_shift = deg(54.74);
+    }
+  } else if (nat[heavy] == 7) {
+    if (nrbonds == 2) {
+      angle2_shift = deg(120.0);
+      angle2_shift_2 = angle2_shift;
+      torsion_shift = 0.0;
+    } else {
+      angle2_shift = deg(109.48);
+      angle2_shift_2 = angle2_shift;
+      torsion_shift = deg(54.74);
+      tcs = true;
+    }
+  }
+  double torsion_check = 0.0;
+  if (tcs) {
+    double tc = dihed(c, r2, r1, heavy, r3);
+    if (tc <= -PI) tc += 2.0 * PI;
+    if (tc > PI) tc -= 2.0 * PI;
+    tc = tc < 0.0 ? -PI - tc : PI - tc;
+    double bac = tc;
+    double tcp = tc < 0.0 ? -tc : tc;
+    tcp *= 180.0 / PI;
+    torsion_shift += deg((54.74 - tcp) / 54.74 * 35.26);
+    angle2_shift -= deg((54.74 - tcp) / 54.74 * 19.48);
+    angle2_shift_2 = angle2_shift;
+    torsion_check = bac;
+  }
+  double angle2 = bangle(c, r1, heavy, h);
+  double ac = std::cos(angle2_shift - angle2);
+  double ac2 = std::cos(angle2_shift_2 - angle2);
+  if (ac2 > ac) ac = ac2;
+  if (ac <= 0.0) return false;
+  double tcorr = dihed(c, r2, r1, heavy, h);
+  if (tcorr <= -PI) tcorr += 2.0 * PI;
+  if (tcorr > PI) tcorr -= 2.0 * PI;
+  if (!tcs2 || std::fabs(tcorr * 180.0 / PI) > 90.0) tcorr = tcorr < 0.0 ? -PI - tcorr : PI - tcorr;
+  double tc;
+  if (torsion_check < 0.0) {
+    tc = std::cos(wrap_pi(torsion_shift - tcorr));
+  } else if (torsion_check > 0.0) {
+    tc = std::cos(wrap_pi(-torsion_shift - tcorr));
+  } else {
+    double c1 = std::cos(wrap_pi(torsion_shift - tcorr));
+    double c2 = std::cos(wrap_pi(-torsion_shift - tcorr));
+    tc = c2 > c1 ? c2 : c1;
+  }
+  if (dist(c, h, heavy) > dist(c, h, r1) && tcs2) tc = 0.0;
+  if (r2 == r3 || h == r3) tc = 1.0;
+  a2c = ac;
+  tcos = tc;
+  return true;
+}
+double eh_plus(const std::vector<V3>& c, const HBond& b, const std::vector<int>& nat) {
+  if (b.disabled) return 0.0;
+  double a0 = A0;
+  double scale_nsp3 = -0.171271 * a0 * a0, scale_osp3 = -0.098822 * a0 * a0;
+  double scale_nsp2 = -0.171271 * a0 * a0, scale_osp2 = scale_osp3;
+  double hartree2kcal = EV * EV_TO_KCAL;
+  int s1 = b.s[0], s2 = b.s[1], s3 = b.s[2], s4 = b.s[3], s5 = b.s[4], s6 = b.s[5], s7 = b.s[6], s8 = b.s[7],
+      h = b.s[8];
+  double angle_cos = -std::cos(bangle(c, s1, h, s5));
+  if (angle_cos <= 0.0) return 0.0;
+  double a2c, tcos, a2cn, tcn;
+  if (!side_terms(c, nat, b.nra, s1, s2, s3, s4, h, a2c, tcos)) return 0.0;
+  if (!side_terms(c, nat, b.nrb, s5, s6, s7, s8, h, a2cn, tcn)) return 0.0;
+  tcn = std::fabs(tcn);
+  double sa = nat[s1] == 7 ? (b.nra >= 3 ? scale_nsp3 : scale_nsp2) : (b.nra >= 2 ? scale_osp3 : scale_osp2);
+  double sb = nat[s5] == 7 ? (b.nrb >= 3 ? scale_nsp3 : scale_nsp2) : (b.nrb >= 2 ? scale_osp3 : scale_osp2);
+  double sc = (sa + sb) / 2.0;
+  double had = dist(c, h, s1), hbd = dist(c, h, s5);
+  double xc_min = std::min(had, hbd);
+  double gap = std::max(had, hbd) - xc_min;
+  double damping = gap > 0.5 ? 1.0 - 1.0 / (1.0 + std::exp(-60.0 * (xc_min / COVCUT - 1.0))) : 1.0;
+  double oo = dist(c, s1, s5);
+  damping /= 1.0 + std::exp(-100.0 * (oo / SHORTCUT - 1.0));
+  damping *= 1.0 - 1.0 / (1.0 + std::exp(-10.0 * (oo / LONGCUT - 1.0)));
+  double xy = dist(c, s1, s5);
+  double inner = 1.0 - a2c * tcos * a2cn * tcn;
+  double e = sc / std::pow(xy, 2.0) * (angle_cos * angle_cos) * (1.0 - inner * inner) * hartree2kcal * damping;
+  if (nat[s1] == 8 && nat[s5] == 8) {
+    double t = std::max(xy - 2.67, 0.0);
+    e += -2.5 * std::exp(-80.0 * t * t) * std::pow(angle_cos, 4);
+  }
+  return e;
+}
+}  // namespace hb
+
+double hbond_energy(const std::vector<int>& z, const std::vector<V3>& pos) {
+  std::vector<V3> c(pos.size());
+  for (size_t i = 0; i < pos.size(); ++i) c[i] = {pos[i].x * A0, pos[i].y * A0, pos[i].z * A0};
+  double sum = 0.0;
+  for (auto& b : hb::build_hbonds(c, z)) sum += hb::eh_plus(c, b, z);
+  return sum;
+}
+
+// MM corrections (src/mm_corrections.rs:46-240).
+double c_triple_bond_energy(const std::vector<int>& z, const std::vector<V3>& pos) {
+  double sum = 0.0;
+  for (size_t a = 0; a < z.size(); ++a)
+    for (size_t b = a + 1; b < z.size(); ++b) {
+      if (z[a] != 6 || z[b] != 6) continue;
+      double r = norm(sub(pos[b], pos[a])) * A0;
+      if (r >= 1.33) continue;
+      double w;
+      if (r < 1.21) {
+        w = 1.0;
+      } else {
+        double x = (r - 1.21) / (1.33 - 1.21);
+        double x3 = x * x * x, x4 = std::pow(x, 4), x5 = std::pow(x, 5), x6 = std::pow(x, 6);
+        double smooth = 1.0 - 10.0 * x3 + 15.0 * x4 - 6.0 * x5;
+        double tail = x3 - 3.0 * x4 + 3.0 * x5 - x6;
+        w = smooth + (-5.0 + 25.0 * x) * tail;
+      }
+      sum += w;
+    }
+  return sum * 12.0;
+}
+double si_o_h_energy(const std::vector<int>& z, const std::vector<V3>& pos) {
+  int n = (int)z.size();
+  double sum = 0.0;
+  for (int o = 0; o < n; ++o) {
+    if (z[o] != 8) continue;
+    auto nearest = [&](int zz, double cut) {
+      int best = -1;
+      double bd = 0.0;
+      for (int k = 0; k < n; ++k) {
+        if (k == o || z[k] != zz) continue;
+        double r = norm(sub(pos[k], pos[o])) * A0;
+        if (r < cut && (best < 0 || r < bd)) {
+          best = k;
+          bd = r;
+        }
+      }
+      return best;
+    };
+    int si = nearest(14, 2.4), h = nearest(1, 1.4);
+    if (si < 0 || h < 0) continue;
+    double rsio = norm(sub(pos[o], pos[si])) * A0, roh = norm(sub(pos[o], pos[h])) * A0;
+    double damp = std::exp(-33.0 * std::max(rsio * rsio - 1.7 * 1.7, 0.0)) * std::exp(-68.0 * std::max(roh * roh - 1.0, 0.0));
+    double ref = 125.0 * PI / 180.0;
+    V3 u = sub(pos[si], pos[o]), v = sub(pos[h], pos[o]);
+    double cs = std::min(1.0, std::max(-1.0, dot(u, v) / (norm(u) * norm(v))));
+    double ang = std::acos(cs);
+    sum += 15.0 * damp * (ang - ref) * (ang - ref);
+  }
+  return sum;
+}
+
+void set_msg(oracle_result* out, const std::string& s) {
+  if (!out) return;
+  std::snprintf(out->message, sizeof out->message, "%s", s.c_str());
+}
+
+}  // namespace
+
+// =============================================================================================
+// C API
+// =============================================================================================
+extern "C" {
+
+void oracle_default_options(oracle_options* o) {
+  o->method = 0;
+  o->multiplicity = 1;
+  o->max_scf = 200;
+  o->use_diis = 1;
+  o->accelerator = 2;
+  o->reference = 0;
+  o->threads = 0;
+  o->charge = 0.0;
+  o->e_tol = 1.0e-8;
+  o->p_tol = 1.0e-7;
+  o->adiis_switch = 0.1;
+  o->level_shift = 0.0;
+}
+
+int oracle_nao(const char* data_dir, int n_atoms, const int* z) {
+  const Params* P = load_params(data_dir, 0);
+  if (!P) return -1;
+  int nao = 0;
+  for (int i = 0; i < n_atoms; ++i) {
+    const Element* e = P->get(z[i]);
+    if (!e) return -1;
+    nao += e->n_orb;
+  }
+  return nao;
+}
+
+int oracle_element(const char* data_dir, int method, int z, double* out9) {
+  const Params* P = load_params(data_dir, method);
+  if (!P) return 1;
+  const Element* e = P->get(z);
+  if (!e) return 2;
+  double v[9] = {(double)e->n_orb, e->dd, e->qq, e->rho0, e->rho1, e->rho2, e->e_isol, e->eheat_ev, e->po9()};
+  std::memcpy(out9, v, sizeof v);
+  return 0;
+}
+
+int oracle_pair_block(const char* data_dir, int za, int zb, const double* d, double* w, double* e1b,
+                      double* e2a, double* s) {
+  const Params* P = load_params(data_dir, 0);
+  if (!P) return 1;
+  const Element* ea = P->get(za);
+  const Element* eb = P->get(zb);
+  if (!ea || !eb) return 2;
+  if (ea->n_orb == 9 || eb->n_orb == 9 || ea->n > 3 || eb->n > 3) return 3;
+  PairBlock pb;
+  V3 dv = {d[0], d[1], d[2]};
+  pair_two_electron(*ea, *eb, dv, pb);
+  double ss[4][4];
+  diatom_overlap(*ea, *eb, dv, ss);
+  for (size_t i = 0; i < pb.w.size(); ++i) w[i] = pb.w[i];
+  for (int i = 0; i < 81; ++i) e1b[i] = e2a[i] = s[i] = 0.0;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      e1b[i * 9 + j] = pb.e1b[i][j];
+      e2a[i * 9 + j] = pb.e2a[i][j];
+      s[i * 9 + j] = ss[i][j];
+    }
+  return 0;
+}
+
+int oracle_hcore(const char* data_dir, int n_atoms, const int* z, const double* xyz, double* h) {
+  const Params* P = load_params(data_dir, 0);
+  if (!P) return 1;
+  Mol m;
+  std::string err;
+  int st = build_mol(*P, n_atoms, z, xyz, m, err);
+  if (st) return st;
+  g_threads = (int)std::max(1u, std::thread::hardware_concurrency());
+  Core c = build_core(m);
+  std::memcpy(h, c.h.data(), c.h.size() * sizeof(double));
+  return 0;
+}
+
+int oracle_fock(const char* data_dir, int n_atoms, const int* z, const double* xyz, const double* pt,
+                const double* ps, double* f) {
+  const Params* P = load_params(data_dir, 0);
+  if (!P) return 1;
+  Mol m;
+  std::string err;
+  int st = build_mol(*P, n_atoms, z, xyz, m, err);
+  if (st) return st;
+  g_threads = (int)std::max(1u, std::thread::hardware_concurrency());
+  Core c = build_core(m);
+  size_t nn = (size_t)m.nao * m.nao;
+  Mat PT(pt, pt + nn), PS(ps, ps + nn);
+  Mat F = build_fock_spin(m, c, PT, PS);
+  std::memcpy(f, F.data(), nn * sizeof(double));
+  return 0;
+}
+
+double oracle_core_core(const char* data_dir, int n_atoms, const int* z, const double* xyz) {
+  const Params* P = load_params(data_dir, 0);
+  if (!P) return NAN;
+  Mol m;
+  std::string err;
+  if (build_mol(*P, n_atoms, z, xyz, m, err)) return NAN;
+  return core_core_energy(*P, m);
+}
+
+void oracle_corrections(int n_atoms, const int* z, const double* xyz, double* out4) {
+  std::vector<int> zz(z, z + n_atoms);
+  std::vector<V3> pos(n_atoms);
+  for (int i = 0; i < n_atoms; ++i) pos[i] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+  out4[0] = dispersion_energy(zz, pos);
+  out4[1] = hbond_energy(zz, pos);
+  out4[2] = c_triple_bond_energy(zz, pos);
+  out4[3] = si_o_h_energy(zz, pos);
+}
+
+int oracle_eigh(int n, const double* a, double* w, double* v) {
+  Mat A(a, a + (size_t)n * n), V;
+  std::vector<double> W;
+  eigh(n, A, W, V);
+  std::memcpy(w, W.data(), n * sizeof(double));
+  std::memcpy(v, V.data(), (size_t)n * n * sizeof(double));
+  return 0;
+}
+
+// run_pm7 / run_pm7_once, molecular branch (src/scf.rs:512-867).
+int oracle_run(const char* data_dir, int n_atoms, const int* z, const double* xyz,
+               const oracle_options* opts, oracle_result* out, double* density, double* spin_density,
+               double* eps, double* charges) {
+  oracle_options o;
+  if (opts)
+    o = *opts;
+  else
+    oracle_default_options(&o);
+  std::memset(out, 0, sizeof *out);
+  g_threads = o.threads > 0 ? o.threads : (int)std::max(1u, std::thread::hardware_concurrency());
+  if (n_atoms <= 0 || o.max_scf <= 0 || !(o.e_tol > 0) || !(o.p_tol > 0) || o.multiplicity < 1 ||
+      !std::isfinite(o.charge) || o.level_shift < 0 || o.adiis_switch < 0) {
+    set_msg(out, "invalid options or empty molecule");
+    return out->status = 1;
+  }
+  for (int i = 0; i < n_atoms; ++i)
+    for (int k = 0; k < 3; ++k)
+      if (!std::isfinite(xyz[3 * i + k])) {
+        set_msg(out, "non-finite coordinates");
+        return out->status = 1;
+      }
+  for (int i = 0; i < n_atoms; ++i)
+    for (int j = 0; j < i; ++j) {
+      double dx = xyz[3 * i] - xyz[3 * j], dy = xyz[3 * i + 1] - xyz[3 * j + 1], dz = xyz[3 * i + 2] - xyz[3 * j + 2];
+      if (dx * dx + dy * dy + dz * dz < 1.0e-20) {
+        set_msg(out, "coincident atoms");
+        return out->status = 1;
+      }
+    }
+  const Params* P = load_params(data_dir, o.method == 1 ? 1 : 0);
+  if (!P) {
+    set_msg(out, "parameter tables not found");
+    return out->status = 1;
+  }
+  Mol m;
+  std::string err;
+  int st = build_mol(*P, n_atoms, z, xyz, m, err);
+  if (st) {
+    set_msg(out, err);
+    return out->status = st;
+  }
+  out->nao = m.nao;
+  out->n_atoms = n_atoms;
+  double n_elec = 0.0;
+  for (int i = 0; i < n_atoms; ++i) n_elec += m.el[i]->core_charge;
+  n_elec -= o.charge;
+  long ne = std::lround(n_elec);
+  if (std::fabs(n_elec - ne) > 1.0e-6 || ne < 0) {
+    set_msg(out, "invalid electron count");
+    return out->status = 1;
+  }
+  long unp = o.multiplicity - 1;
+  if (ne - unp < 0 || (ne - unp) % 2 != 0) {
+    set_msg(out, "electron count incompatible with multiplicity");
+    return out->status = 1;
+  }
+  int na = (int)((ne + unp) / 2), nb = (int)((ne - unp) / 2);
+  bool uhf = o.reference == 0 ? (na != nb) : (o.reference == 2);
+  if (o.reference == 1 && na != nb) {
+    set_msg(out, "restricted reference needs a closed shell");
+    return out->status = 1;
+  }
+  Core core = build_core(m);
+  ScfState s = uhf ? uhf_loop(m, core, na, nb, o) : rhf_loop(m, core, na, o);
+  double core_ev = core_core_energy(*P, m);
+  double corr = 0.0;
+  if (o.method != 3) {
+    corr += dispersion_energy(m.z, m.pos);
+    corr += hbond_energy(m.z, m.pos);
+    corr += c_triple_bond_energy(m.z, m.pos);
+    corr += si_o_h_energy(m.z, m.pos);
+  }
+  double e_isol = 0.0, eheat = 0.0;
+  for (int i = 0; i < n_atoms; ++i) {
+    e_isol += m.el[i]->e_isol;
+    eheat += m.el[i]->eheat_ev;
+  }
+  out->electronic_ev = s.e_el;
+  out->core_ev = core_ev;
+  out->correction_kcal = corr;
+  out->total_ev = s.e_el + core_ev + corr * KCAL_TO_EV;
+  out->hof_kcal = (s.e_el + core_ev - e_isol + eheat) * EV_TO_KCAL + corr;
+  out->iterations = s.iterations;
+  out->converged = s.converged;
+  out->unrestricted = s.uhf;
+  out->density_error = s.dp_err;
+  out->n_occ = s.n_occ;
+  out->n_occ_beta = s.uhf ? s.n_occ_b : -1;
+  int nao = m.nao;
+  // Charges (src/scf.rs:777-787) and dipole (src/dipole.rs:282-348, centre-of-mass origin).
+  std::vector<double> q(n_atoms);
+  for (int i = 0; i < n_atoms; ++i) {
+    double pop = 0.0;
+    for (int k = 0; k < m.norb[i]; ++k) pop += s.density[(size_t)(m.off[i] + k) * nao + m.off[i] + k];
+    q[i] = m.el[i]->core_charge - pop;
+  }
+  double net = 0.0;
+  for (double v : q) net += v;
+  V3 origin = {0, 0, 0};
+  if (std::fabs(net) > 0.5) {
+    double tot = 0.0;
+    for (int i = 0; i < n_atoms; ++i) {
+      double mass = (z[i] >= 0 && z[i] <= 36) ? MASS[z[i]] : 0.0;
+      origin.x += m.pos[i].x * mass;
+      origin.y += m.pos[i].y * mass;
+      origin.z += m.pos[i].z * mass;
+      tot += mass;
+    }
+    if (tot > 0) origin = {origin.x / tot, origin.y / tot, origin.z / tot};
+  }
+  double pc[3] = {0, 0, 0}, sp[3] = {0, 0, 0};
+  for (int i = 0; i < n_atoms; ++i) {
+    pc[0] += (m.pos[i].x - origin.x) * q[i];
+    pc[1] += (m.pos[i].y - origin.y) * q[i];
+    pc[2] += (m.pos[i].z - origin.z) * q[i];
+    if (m.el[i]->has_p()) {
+      int off = m.off[i];
+      double hyb = -2.0 * m.el[i]->dd;
+      for (int k = 0; k < 3; ++k) sp[k] += hyb * s.density[(size_t)off * nao + off + 1 + k];
+    }
+  }
+  for (int k = 0; k < 3; ++k) {
+    out->dipole_point[k] = pc[k] * AU_DIPOLE_TO_DEBYE;
+    out->dipole_sp[k] = sp[k] * AU_DIPOLE_TO_DEBYE;
+    out->dipole_pd[k] = 0.0;
+    out->dipole[k] = out->dipole_point[k] + out->dipole_sp[k];
+  }
+  out->homo_ev = s.n_occ >= 1 ? s.eps[s.n_occ - 1] : NAN;
+  out->lumo_ev = s.n_occ < nao ? s.eps[s.n_occ] : NAN;
+  out->homo_ev_beta = NAN;
+  out->lumo_ev_beta = NAN;
+  if (s.uhf) {
+    out->homo_ev_beta = s.n_occ_b >= 1 ? s.eps_b[s.n_occ_b - 1] : NAN;
+    out->lumo_ev_beta = s.n_occ_b < nao ? s.eps_b[s.n_occ_b] : NAN;
+  }
+  if (density) std::memcpy(density, s.density.data(), (size_t)nao * nao * sizeof(double));
+  if (spin_density) {
+    if (s.uhf)
+      std::memcpy(spin_density, s.spin.data(), (size_t)nao * nao * sizeof(double));
+    else
+      std::memset(spin_density, 0, (size_t)nao * nao * sizeof(double));
+  }
+  if (eps) std::memcpy(eps, s.eps.data(), nao * sizeof(double));
+  if (charges) std::memcpy(charges, q.data(), n_atoms * sizeof(double));
+  if (!s.converged) {
+    set_msg(out, "SCF not converged");
+    return out->status = 5;
+  }
+  return out->status = 0;
+}
+
+}  // extern "C"

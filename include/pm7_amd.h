@@ -1,0 +1,149 @@
+/* SPDX-License-Identifier: GPL-3.0-or-later
+ *
+ * pm7_amd: PM7 NDDO SCF single points on one MI355X (gfx950), behind a plain C ABI.
+ *
+ * The boundary replaces pm7-rs's
+ *     pub fn run_pm7(molecule: &Molecule, params: &Pm7Parameters, options: &Pm7Options)
+ *         -> Result<Pm7Result>                                     (pm7-rs src/scf.rs:512-516)
+ * Units follow the reference: coordinates in Bohr (src/system.rs:23), energies in eV, heats of
+ * formation in kcal/mol, matrices row-major with eigenvectors as COLUMNS in ascending eigenvalue
+ * order (src/linalg.rs:229-230). Status codes map one-to-one onto pm7-rs's Pm7Error variants
+ * (src/error.rs:13-48) plus PM7_ERR_DEVICE.
+ *
+ * Threading: a context owns one device and its streams; calls on one context are serialised
+ * internally. Separate contexts may be used from separate threads.
+ */
+#ifndef PM7_AMD_H
+#define PM7_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PM7_ABI_VERSION 1
+
+typedef enum {
+  PM7_OK = 0,
+  PM7_ERR_INVALID_INPUT = 1,      /* Pm7Error::InvalidInput */
+  PM7_ERR_MISSING_ELEMENT = 2,    /* Pm7Error::MissingElement */
+  PM7_ERR_MISSING_PARAMETER = 3,  /* Pm7Error::MissingParameter */
+  PM7_ERR_UNSUPPORTED = 4,        /* Pm7Error::UnsupportedFeature */
+  PM7_ERR_LINEAR_ALGEBRA = 5,     /* Pm7Error::LinearAlgebra */
+  PM7_ERR_SCF_NOT_CONVERGED = 6,  /* Pm7Error::ScfNotConverged (energies are still filled) */
+  PM7_ERR_INSUFFICIENT_MEMORY = 7,/* Pm7Error::InsufficientMemory */
+  PM7_ERR_DEVICE = 8              /* HIP runtime failure */
+} pm7_status;
+
+typedef enum { PM7_METHOD_PM7 = 0, PM7_METHOD_PM7_TS = 1, PM7_METHOD_PM7_MINUS = 3 } pm7_method;
+typedef enum { PM7_ACCEL_NONE = 0, PM7_ACCEL_CDIIS = 1, PM7_ACCEL_ADIIS_CDIIS = 2 } pm7_accelerator;
+typedef enum { PM7_REF_AUTO = 0, PM7_REF_RHF = 1, PM7_REF_UHF = 2 } pm7_reference;
+typedef enum { PM7_PRECISION_FP64 = 0, PM7_PRECISION_MIXED = 1 } pm7_precision;
+
+typedef struct pm7_ctx pm7_ctx;
+
+/* pm7-rs Molecule (src/system.rs:20-39): atomic numbers and Bohr coordinates, xyz[3*i + k]. */
+typedef struct {
+  int32_t n_atoms;
+  const int32_t* z;
+  const double* xyz_bohr;
+} pm7_molecule_v1;
+
+/* pm7-rs Pm7Options (src/scf.rs:73-234), defaults from pm7_default_options (src/scf.rs:207-234). */
+typedef struct {
+  int32_t method;        /* pm7_method; must match the loaded tables (src/scf.rs:530-535) */
+  double charge;         /* options.charge is the charge used (src/scf.rs:592) */
+  int32_t multiplicity;  /* >= 1 */
+  int32_t max_scf;       /* default 200 */
+  double e_tol;          /* eV, default 1e-8 */
+  double p_tol;          /* RMS density change, default 1e-7 */
+  int32_t use_diis;      /* default 1; 0 forces plain iteration */
+  int32_t accelerator;   /* pm7_accelerator, default ADIIS->CDIIS */
+  double adiis_switch;   /* commutator norm of the A-DIIS -> CDIIS switch, default 0.1 */
+  double level_shift;    /* static level-shift floor in eV, default 0 */
+  int32_t reference;     /* pm7_reference, default AUTO */
+  int32_t precision;     /* pm7_precision, default FP64 (engine extension) */
+  double mixed_switch;   /* FP32 -> FP64 hand-over density change for PRECISION_MIXED (1e-5) */
+  int64_t max_memory_mb; /* device-memory budget for this call; <= 0 means 80% of free HBM */
+} pm7_options_v1;
+
+/* pm7-rs Pm7Result (src/scf.rs:262-342), molecular fields. Arrays are allocated by the library
+ * and released by pm7_result_free. */
+typedef struct {
+  int32_t status;
+  int32_t iterations;
+  int32_t converged;
+  int32_t unrestricted;
+  int32_t nao;
+  int32_t n_atoms;
+  int32_t n_occ;
+  int32_t n_occ_beta;        /* -1 for RHF */
+  int32_t fp32_iterations;   /* iterations run in FP32 under PRECISION_MIXED */
+  int32_t jacobi_sweeps;     /* total eigensolver sweeps (diagnostic) */
+  double electronic_ev;
+  double core_ev;
+  double total_ev;
+  double heat_of_formation_kcal;
+  double correction_kcal;    /* dispersion + EH+ + MM terms included in total/HOF */
+  double density_error;      /* last RMS density change */
+  double homo_ev, lumo_ev, homo_ev_beta, lumo_ev_beta; /* NaN when absent */
+  double dipole_debye[3];
+  double dipole_point_charge_debye[3];
+  double dipole_sp_hybrid_debye[3];
+  double dipole_pd_hybrid_debye[3];
+  double dipole_magnitude;
+  double* density;           /* nao*nao, total density */
+  double* spin_density;      /* nao*nao, P_alpha - P_beta, UHF only (else NULL) */
+  double* mo_energies;       /* nao */
+  double* mo_coeff;          /* nao*nao, columns = MOs */
+  double* mo_energies_beta;  /* UHF only */
+  double* mo_coeff_beta;     /* UHF only */
+  double* charges;           /* n_atoms Mulliken net charges */
+  char message[256];
+} pm7_result_v1;
+
+/* Context: device + streams + parameter tables + pooled device memory.
+ * data_dir: directory of the bundled parameter tables (NULL: next to the library). */
+pm7_status pm7_ctx_create(int32_t device, const char* data_dir, pm7_ctx** out);
+void pm7_ctx_destroy(pm7_ctx* ctx);
+const char* pm7_last_error(const pm7_ctx* ctx);
+int32_t pm7_abi_version(void);
+
+/* Replaces Pm7Parameters::method (src/params.rs:130-155): (re)load the bundled tables. */
+pm7_status pm7_params_load(pm7_ctx* ctx, int32_t method);
+
+void pm7_default_options(pm7_options_v1* opts);
+
+/* Replaces run_pm7 (src/scf.rs:512-867) for one molecule. */
+pm7_status pm7_run(pm7_ctx* ctx, const pm7_molecule_v1* mol, const pm7_options_v1* opts,
+                   pm7_result_v1* out);
+
+/* Many molecules of uneven size in one persistent launch per size class; results[i] is
+ * exactly what pm7_run would return for mols[i] (fixed-order reductions, no atomics).
+ * opts may hold one entry (shared) or n entries. Returns PM7_OK when every call was dispatched;
+ * per-molecule outcomes are in results[i].status. */
+pm7_status pm7_run_batch(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols,
+                         const pm7_options_v1* opts, int32_t n_opts, pm7_result_v1* results);
+
+void pm7_result_free(pm7_result_v1* res);
+
+/* Replaces build_core (src/hamiltonian.rs:224-231): the core Hamiltonian H (nao*nao). */
+pm7_status pm7_core_hamiltonian(pm7_ctx* ctx, const pm7_molecule_v1* mol, double* h_out,
+                                int32_t* nao_out);
+
+/* Replaces build_fock_spin (src/fock.rs:133-147): F = H + J(p_tot) - K(p_spin). */
+pm7_status pm7_fock_spin(pm7_ctx* ctx, const pm7_molecule_v1* mol, const double* p_tot,
+                         const double* p_spin, double* f_out);
+
+/* Replaces core_core_energy (src/repulsion.rs:16-36), eV. */
+pm7_status pm7_core_core_energy(pm7_ctx* ctx, const pm7_molecule_v1* mol, double* e_out);
+
+/* Device timing of the last pm7_run / pm7_run_batch call, milliseconds:
+ * [0] integrals+H_core, [1] SCF megakernels, [2] output gather, [3] whole call (host wall). */
+void pm7_last_timings(const pm7_ctx* ctx, double* ms4);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,790 @@
+// SPDX-License-Identifier: GPL-3.0-or-later
+//
+// Host driver and C ABI of the PM7 engine (drop-in for pm7-rs run_pm7, src/scf.rs:512-867).
+//
+// A call (single molecule or batch) is: validate on the host -> lay the batch out in HBM (atoms,
+// atom pairs, integral stream, per-molecule matrix slab) -> one integral launch for every pair of
+// every molecule -> one SCF megakernel launch per size class (molecules run concurrently, each
+// in its own workgroup, each stopping when it converges) -> one gather launch -> one copy back.
+// The geometry-only corrections (dispersion, EH+, MM) run on the host while the device works.
+#include <dlfcn.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/pm7_amd.h"
+#include "host.h"
+#include "kernels.h"
+
+using namespace pm7;
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap && p) return hipSuccess;
+    if (p) {
+      hipError_t e = hipFree(p);
+      p = nullptr;
+      cap = 0;
+      if (e != hipSuccess) return e;
+    }
+    size_t want = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+double nan_() { return std::nan(""); }
+
+const double MASS[37] = {0.0,     1.0079,  4.0026,  6.94,     9.01218, 10.81,    12.011,  14.0067, 15.9994, 18.9984,
+                         20.179,  22.98977, 24.305, 26.98154, 28.0855, 30.97376, 32.06,   35.453,  39.948,  39.098,
+                         40.078,  44.956,  47.867,  50.942,   51.996,  54.938,   55.845,  58.933,  58.693,  63.546,
+                         65.38,   69.723,  72.63,   74.922,   78.971,  79.904,   83.798};
+
+std::string default_data_dir() {
+  Dl_info info;
+  if (dladdr((void*)&default_data_dir, &info) && info.dli_fname) {
+    std::string so(info.dli_fname);
+    auto slash = so.rfind('/');
+    std::string dir = slash == std::string::npos ? "." : so.substr(0, slash);
+    return dir + "/../data";
+  }
+  return "data";
+}
+
+// One validated molecule of a call.
+struct Job {
+  int idx = 0;
+  pm7_options_v1 o{};
+  std::vector<int> z;
+  std::vector<double> xyz;
+  int nao = 0, ld = 0, n_pp = 0, na = 0, nb = 0, uhf = 0, variant = 0;
+  double sad_yy = 0.0;
+  int64_t pair0 = 0, mat0 = 0, pp0 = 0, vec0 = 0, gath0 = 0, w0 = 0;
+  int atom0 = 0;
+  double corr_kcal = 0.0;
+};
+
+}  // namespace
+
+struct pm7_ctx {
+  int device = 0;
+  std::string data_dir;
+  std::string err;
+  ParamSet params;
+  int method = -1;
+  std::vector<ElemDev> elem_host;
+  hipStream_t st[SCF_VARIANTS] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev[8] = {};
+  DevBuf d_elem, d_pairpar, d_mols, d_list, d_atom_z, d_atom_ao, d_atom_pp, d_xyz, d_pairs, d_items, d_gath_off;
+  DevBuf d_w, d_slab, d_vecs, d_out, d_ecore, d_ecore_mol, d_gath;
+  double timings[4] = {0, 0, 0, 0};
+  std::mutex mu;
+};
+
+namespace {
+
+pm7_status fail(pm7_ctx* ctx, pm7_status st, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return st;
+}
+
+void set_msg(pm7_result_v1* r, const std::string& m) { std::snprintf(r->message, sizeof r->message, "%s", m.c_str()); }
+
+#define HIPCHK(expr)                                                                         \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) return fail(ctx, PM7_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// Input validation (pm7-rs validate_input src/scf.rs:937-992, run_pm7_once 530-627).
+pm7_status validate(pm7_ctx* ctx, const pm7_molecule_v1& m, const pm7_options_v1& o, Job& j, std::string& msg) {
+  if (m.n_atoms <= 0 || !m.z || !m.xyz_bohr) {
+    msg = "a PM7 calculation needs at least one atom";
+    return PM7_ERR_INVALID_INPUT;
+  }
+  if (!std::isfinite(o.charge)) return msg = "charge must be finite", PM7_ERR_INVALID_INPUT;
+  if (o.max_scf <= 0) return msg = "max_scf must be greater than zero", PM7_ERR_INVALID_INPUT;
+  if (!(std::isfinite(o.e_tol) && o.e_tol > 0)) return msg = "e_tol must be finite and positive", PM7_ERR_INVALID_INPUT;
+  if (!(std::isfinite(o.p_tol) && o.p_tol > 0)) return msg = "p_tol must be finite and positive", PM7_ERR_INVALID_INPUT;
+  if (!(std::isfinite(o.level_shift) && o.level_shift >= 0))
+    return msg = "level_shift must be finite and non-negative", PM7_ERR_INVALID_INPUT;
+  if (!(std::isfinite(o.adiis_switch) && o.adiis_switch >= 0))
+    return msg = "adiis_switch must be finite and non-negative", PM7_ERR_INVALID_INPUT;
+  if (o.multiplicity < 1) return msg = "multiplicity must be >= 1", PM7_ERR_INVALID_INPUT;
+  if ((o.method == PM7_METHOD_PM7_TS) != (ctx->method == PM7_METHOD_PM7_TS))
+    return msg = "options method does not match the loaded parameter method", PM7_ERR_INVALID_INPUT;
+  const int n = m.n_atoms;
+  j.z.assign(m.z, m.z + n);
+  j.xyz.assign(m.xyz_bohr, m.xyz_bohr + 3 * n);
+  for (int i = 0; i < 3 * n; ++i)
+    if (!std::isfinite(j.xyz[i])) {
+      msg = "atom " + std::to_string(i / 3 + 1) + " has non-finite coordinates";
+      return PM7_ERR_INVALID_INPUT;
+    }
+  // Coincident atoms, O(N) bucket grid (src/scf.rs:1001-1040).
+  {
+    const double h = 1.0e-6;
+    std::unordered_map<long long, std::vector<int>> cells;
+    auto key = [](long long a, long long b, long long c) { return (a * 73856093LL) ^ (b * 19349663LL) ^ (c * 83492791LL); };
+    for (int i = 0; i < n; ++i) {
+      long long cx = (long long)std::floor(j.xyz[3 * i] / h), cy = (long long)std::floor(j.xyz[3 * i + 1] / h),
+                cz = (long long)std::floor(j.xyz[3 * i + 2] / h);
+      for (int a = -1; a <= 1; ++a)
+        for (int b = -1; b <= 1; ++b)
+          for (int c = -1; c <= 1; ++c) {
+            auto it = cells.find(key(cx + a, cy + b, cz + c));
+            if (it == cells.end()) continue;
+            for (int k : it->second) {
+              double dx = j.xyz[3 * i] - j.xyz[3 * k], dy = j.xyz[3 * i + 1] - j.xyz[3 * k + 1],
+                     dz = j.xyz[3 * i + 2] - j.xyz[3 * k + 2];
+              if (dx * dx + dy * dy + dz * dz < 1.0e-20) {
+                msg = "atoms " + std::to_string(std::min(i, k) + 1) + " and " + std::to_string(std::max(i, k) + 1) +
+                      " occupy the same position";
+                return PM7_ERR_INVALID_INPUT;
+              }
+            }
+          }
+      cells[key(cx, cy, cz)].push_back(i);
+    }
+  }
+  double n_elec = 0.0, tore = 0.0;
+  int nao = 0, npp = 0;
+  for (int i = 0; i < n; ++i) {
+    int z = j.z[i];
+    if (z <= 0 || z >= MAX_Z || !ctx->params.el[z].present) {
+      msg = "element " + std::to_string(z) + " has no PM7 parameters";
+      return PM7_ERR_MISSING_ELEMENT;
+    }
+    const HostElement& e = ctx->params.el[z];
+    if (e.n_orb == 9 || e.n > 3) {
+      msg = "element " + std::to_string(z) + " needs the MNDO/d two-centre kernel, not built yet";
+      return PM7_ERR_UNSUPPORTED;
+    }
+    tore += e.core_charge;
+    nao += e.n_orb;
+    npp += npair_of(e.n_orb);
+  }
+  n_elec = tore - o.charge;
+  long ne = std::lround(n_elec);
+  if (std::fabs(n_elec - (double)ne) > 1.0e-6 || ne < 0) {
+    msg = "invalid electron count " + std::to_string(n_elec);
+    return PM7_ERR_INVALID_INPUT;
+  }
+  long unp = o.multiplicity - 1;
+  if (ne - unp < 0 || (ne - unp) % 2 != 0) {
+    msg = "electron count " + std::to_string(ne) + " is incompatible with multiplicity " + std::to_string(o.multiplicity);
+    return PM7_ERR_INVALID_INPUT;
+  }
+  j.na = (int)((ne + unp) / 2);
+  j.nb = (int)((ne - unp) / 2);
+  if (o.reference == PM7_REF_RHF && j.na != j.nb) {
+    msg = "restricted (RHF) needs a closed shell; use UHF";
+    return PM7_ERR_INVALID_INPUT;
+  }
+  j.uhf = o.reference == PM7_REF_AUTO ? (j.na != j.nb) : (o.reference == PM7_REF_UHF);
+  j.nao = nao;
+  j.ld = nao + (nao & 1);
+  j.n_pp = npp;
+  j.sad_yy = (tore - (double)(j.na + j.nb)) / nao;
+  j.variant = -1;
+  for (int v = 0; v < SCF_VARIANTS; ++v)
+    if (nao <= VARIANT_MAX_NAO[v]) {
+      j.variant = v;
+      break;
+    }
+  if (j.variant < 0) {
+    msg = "molecules above 1024 basis functions need the multi-workgroup path, not built yet";
+    return PM7_ERR_UNSUPPORTED;
+  }
+  j.o = o;
+  return PM7_OK;
+}
+
+struct Prepared {
+  std::vector<Job> jobs;
+  std::vector<MolDev> mols;
+  int64_t n_pairs = 0, w_words = 0, slab_words = 0, vec_words = 0, gath_words = 0;
+  int n_items = 0;
+  int list_off[SCF_VARIANTS + 1] = {0, 0, 0, 0};
+  BatchDev b;
+};
+
+size_t bytes_needed(const Prepared& P) {
+  return (size_t)(P.w_words + P.slab_words + P.vec_words + P.gath_words) * 8 + (size_t)P.n_pairs * (sizeof(PairDev) + 8) +
+         P.jobs.size() * (sizeof(MolDev) + sizeof(MolOut) + 64);
+}
+
+// Lay out, upload and run the once-per-geometry kernels.
+pm7_status prepare(pm7_ctx* ctx, Prepared& P, int64_t max_memory_mb) {
+  const int nm = (int)P.jobs.size();
+  std::vector<int> atom_z, atom_ao, atom_pp;
+  std::vector<double> xyz;
+  std::vector<PairDev> pairs;
+  std::vector<int2> items;
+  std::vector<int64_t> gath_off(nm);
+  int64_t w_off = 0, mat = 0, vec = 0, gath = 0, pair0 = 0;
+  int atom0 = 0;
+  P.mols.resize(nm);
+  std::vector<int> lists[SCF_VARIANTS];
+  for (int mi = 0; mi < nm; ++mi) {
+    Job& j = P.jobs[mi];
+    const int n = (int)j.z.size();
+    j.atom0 = atom0;
+    j.pair0 = pair0;
+    int ao = 0, pp = 0;
+    for (int i = 0; i < n; ++i) {
+      const HostElement& e = ctx->params.el[j.z[i]];
+      atom_z.push_back(j.z[i]);
+      atom_ao.push_back(ao);
+      atom_pp.push_back(pp);
+      for (int k = 0; k < 3; ++k) xyz.push_back(j.xyz[3 * i + k]);
+      for (int el = 0; el < npair_of(e.n_orb); ++el) items.push_back(make_int2(mi, (atom0 + i) | (el << 24)));
+      ao += e.n_orb;
+      pp += npair_of(e.n_orb);
+    }
+    for (int u = 0; u < n; ++u)
+      for (int v = u + 1; v < n; ++v) {
+        int nu = ctx->params.el[j.z[u]].n_orb, nv = ctx->params.el[j.z[v]].n_orb;
+        PairDev pd;
+        pd.mol = mi;
+        pd.pad = 0;
+        if (nu >= nv) {
+          pd.f = atom0 + u;
+          pd.s = atom0 + v;
+        } else {
+          pd.f = atom0 + v;
+          pd.s = atom0 + u;
+        }
+        pd.w_off = w_off;
+        w_off += (int64_t)npair_of(std::max(nu, nv)) * npair_of(std::min(nu, nv));
+        pairs.push_back(pd);
+      }
+    j.mat0 = mat;
+    j.pp0 = mat + (int64_t)NSLOT * j.ld * j.ld;
+    mat = j.pp0 + ((j.n_pp + 7) / 8) * 8;
+    j.vec0 = vec;
+    vec += 2 * j.ld;
+    j.gath0 = gath;
+    gath_off[mi] = gath;
+    gath += (int64_t)(j.uhf ? 4 : 2) * j.nao * j.nao;
+    MolDev& M = P.mols[mi];
+    std::memset(&M, 0, sizeof M);
+    M.n_atoms = n;
+    M.nao = j.nao;
+    M.ld = j.ld;
+    M.n_occ = j.na;
+    M.n_occ_b = j.nb;
+    M.uhf = j.uhf;
+    M.atom0 = atom0;
+    M.max_scf = j.o.max_scf;
+    M.accel = j.o.use_diis ? j.o.accelerator : 0;
+    M.use_diis = j.o.use_diis;
+    M.precision = j.o.precision;
+    M.n_pp = j.n_pp;
+    M.pair0 = pair0;
+    M.mat0 = j.mat0;
+    M.pp0 = j.pp0;
+    M.vec0 = j.vec0;
+    M.e_tol = j.o.e_tol;
+    M.p_tol = j.o.p_tol;
+    M.adiis_switch = j.o.adiis_switch;
+    M.shift_floor = j.o.level_shift;
+    M.sad_yy = j.sad_yy;
+    M.mixed_switch = j.o.mixed_switch;
+    lists[j.variant].push_back(mi);
+    atom0 += n;
+    pair0 += (int64_t)n * (n - 1) / 2;
+  }
+  std::vector<int> list;
+  for (int v = 0; v < SCF_VARIANTS; ++v) {
+    P.list_off[v] = (int)list.size();
+    // Largest molecules first so the longest SCFs start first.
+    std::stable_sort(lists[v].begin(), lists[v].end(),
+                     [&](int a, int b) { return P.jobs[a].nao > P.jobs[b].nao; });
+    list.insert(list.end(), lists[v].begin(), lists[v].end());
+  }
+  P.list_off[SCF_VARIANTS] = (int)list.size();
+  P.n_pairs = (int64_t)pairs.size();
+  P.w_words = w_off;
+  P.slab_words = mat;
+  P.vec_words = vec;
+  P.gath_words = gath;
+  P.n_items = (int)items.size();
+  // HBM budget (pm7-rs memory::guard, src/memory.rs:95-118, but for device memory).
+  size_t need = bytes_needed(P);
+  size_t free_b = 0, total_b = 0;
+  HIPCHK(hipMemGetInfo(&free_b, &total_b));
+  size_t held = ctx->d_w.cap + ctx->d_slab.cap + ctx->d_gath.cap + ctx->d_pairs.cap + ctx->d_ecore.cap;
+  size_t budget = max_memory_mb > 0 ? (size_t)max_memory_mb * 1024 * 1024 : (size_t)(0.8 * (double)(free_b + held));
+  if (need > budget)
+    return fail(ctx, PM7_ERR_INSUFFICIENT_MEMORY,
+                "needs " + std::to_string(need >> 20) + " MiB of device memory, budget " + std::to_string(budget >> 20) +
+                    " MiB (integrals + SCF working set)");
+  auto up = [&](DevBuf& d, const void* src, size_t bytes) -> hipError_t {
+    hipError_t e = d.ensure(bytes);
+    if (e != hipSuccess) return e;
+    if (bytes) return hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, ctx->st[0]);
+    return hipSuccess;
+  };
+  HIPCHK(up(ctx->d_mols, P.mols.data(), P.mols.size() * sizeof(MolDev)));
+  HIPCHK(up(ctx->d_list, list.data(), list.size() * sizeof(int)));
+  HIPCHK(up(ctx->d_atom_z, atom_z.data(), atom_z.size() * sizeof(int)));
+  HIPCHK(up(ctx->d_atom_ao, atom_ao.data(), atom_ao.size() * sizeof(int)));
+  HIPCHK(up(ctx->d_atom_pp, atom_pp.data(), atom_pp.size() * sizeof(int)));
+  HIPCHK(up(ctx->d_xyz, xyz.data(), xyz.size() * sizeof(double)));
+  HIPCHK(up(ctx->d_pairs, pairs.data(), pairs.size() * sizeof(PairDev)));
+  HIPCHK(up(ctx->d_items, items.data(), items.size() * sizeof(int2)));
+  HIPCHK(up(ctx->d_gath_off, gath_off.data(), gath_off.size() * sizeof(int64_t)));
+  HIPCHK(ctx->d_w.ensure((size_t)P.w_words * 8));
+  HIPCHK(ctx->d_slab.ensure((size_t)P.slab_words * 8));
+  HIPCHK(ctx->d_vecs.ensure((size_t)P.vec_words * 8));
+  HIPCHK(ctx->d_out.ensure((size_t)nm * sizeof(MolOut)));
+  HIPCHK(ctx->d_ecore.ensure((size_t)P.n_pairs * 8));
+  HIPCHK(ctx->d_ecore_mol.ensure((size_t)nm * 8));
+  HIPCHK(ctx->d_gath.ensure((size_t)P.gath_words * 8));
+  // Zero the slab: pad rows/columns of every matrix must be exact zeros.
+  HIPCHK(hipMemsetAsync(ctx->d_slab.p, 0, (size_t)P.slab_words * 8, ctx->st[0]));
+  BatchDev& b = P.b;
+  b.mols = ctx->d_mols.as<MolDev>();
+  b.mol_list = ctx->d_list.as<int>();
+  b.atom_z = ctx->d_atom_z.as<int>();
+  b.atom_ao = ctx->d_atom_ao.as<int>();
+  b.atom_pp = ctx->d_atom_pp.as<int>();
+  b.xyz = ctx->d_xyz.as<double>();
+  b.pairs = ctx->d_pairs.as<PairDev>();
+  b.elem = ctx->d_elem.as<ElemDev>();
+  b.pairpar = ctx->d_pairpar.as<double2>();
+  b.hc_items = ctx->d_items.as<int2>();
+  b.gath_off = ctx->d_gath_off.as<int64_t>();
+  b.w = ctx->d_w.as<double>();
+  b.slab = ctx->d_slab.as<double>();
+  b.vecs = ctx->d_vecs.as<double>();
+  b.out = ctx->d_out.as<MolOut>();
+  b.e_core = ctx->d_ecore.as<double>();
+  b.e_core_mol = ctx->d_ecore_mol.as<double>();
+  b.gath = ctx->d_gath.as<double>();
+  b.vpar = make_double4(ctx->params.vpar[1], ctx->params.vpar[2], ctx->params.vpar[3], ctx->params.vpar[4]);
+  HIPCHK(hipEventRecord(ctx->ev[0], ctx->st[0]));
+  HIPCHK(launch_integrals(b, P.n_pairs, P.n_items, nm, ctx->st[0]));
+  HIPCHK(hipEventRecord(ctx->ev[1], ctx->st[0]));
+  return PM7_OK;
+}
+
+void finish_result(pm7_ctx* ctx, const Job& j, const MolOut& R, double e_core, const double* eps_ab,
+                   const double* gath, pm7_result_v1* out) {
+  const int n = j.nao, nat = (int)j.z.size();
+  std::memset(out, 0, sizeof *out);
+  out->nao = n;
+  out->n_atoms = nat;
+  out->iterations = R.iterations;
+  out->converged = R.converged;
+  out->unrestricted = j.uhf;
+  out->n_occ = j.na;
+  out->n_occ_beta = j.uhf ? j.nb : -1;
+  out->fp32_iterations = R.fp32_iters;
+  out->jacobi_sweeps = R.sweeps;
+  out->electronic_ev = R.e_el;
+  out->core_ev = e_core;
+  out->correction_kcal = j.corr_kcal;
+  out->total_ev = R.e_el + e_core + j.corr_kcal * KCAL_TO_EV;
+  double e_isol = 0.0, eheat = 0.0;
+  for (int z : j.z) {
+    e_isol += ctx->params.el[z].e_isol;
+    eheat += ctx->params.el[z].eheat_ev;
+  }
+  out->heat_of_formation_kcal = (R.e_el + e_core - e_isol + eheat) * EV_TO_KCAL + j.corr_kcal;
+  out->density_error = R.dp_err;
+  const size_t nn = (size_t)n * n;
+  out->density = (double*)std::malloc(nn * sizeof(double));
+  out->mo_coeff = (double*)std::malloc(nn * sizeof(double));
+  out->mo_energies = (double*)std::malloc(n * sizeof(double));
+  out->charges = (double*)std::malloc(nat * sizeof(double));
+  std::memcpy(out->mo_energies, eps_ab, n * sizeof(double));
+  std::memcpy(out->mo_coeff, gath + nn, nn * sizeof(double));
+  if (!j.uhf) {
+    std::memcpy(out->density, gath, nn * sizeof(double));
+  } else {
+    const double* pa = gath;
+    const double* pb = gath + 2 * nn;
+    out->spin_density = (double*)std::malloc(nn * sizeof(double));
+    for (size_t k = 0; k < nn; ++k) {
+      out->density[k] = pa[k] + pb[k];
+      out->spin_density[k] = pa[k] - pb[k];
+    }
+    out->mo_energies_beta = (double*)std::malloc(n * sizeof(double));
+    out->mo_coeff_beta = (double*)std::malloc(nn * sizeof(double));
+    std::memcpy(out->mo_energies_beta, eps_ab + j.ld, n * sizeof(double));
+    std::memcpy(out->mo_coeff_beta, gath + 3 * nn, nn * sizeof(double));
+  }
+  // Mulliken charges (src/scf.rs:777-787) and the dipole breakdown (src/dipole.rs:282-348).
+  double net = 0.0;
+  int ao = 0;
+  std::vector<int> off(nat);
+  for (int i = 0; i < nat; ++i) {
+    const HostElement& e = ctx->params.el[j.z[i]];
+    off[i] = ao;
+    double pop = 0.0;
+    for (int k = 0; k < e.n_orb; ++k) pop += out->density[(size_t)(ao + k) * n + ao + k];
+    out->charges[i] = e.core_charge - pop;
+    net += out->charges[i];
+    ao += e.n_orb;
+  }
+  double org[3] = {0, 0, 0};
+  if (std::fabs(net) > 0.5) {
+    double tot = 0.0;
+    for (int i = 0; i < nat; ++i) {
+      double m = j.z[i] <= 36 ? MASS[j.z[i]] : 0.0;
+      for (int k = 0; k < 3; ++k) org[k] += j.xyz[3 * i + k] * m;
+      tot += m;
+    }
+    if (tot > 0)
+      for (int k = 0; k < 3; ++k) org[k] /= tot;
+  }
+  const double D = 2.541746473;
+  for (int k = 0; k < 3; ++k) {
+    double pc = 0.0, sp = 0.0;
+    for (int i = 0; i < nat; ++i) {
+      pc += (j.xyz[3 * i + k] - org[k]) * out->charges[i];
+      const HostElement& e = ctx->params.el[j.z[i]];
+      if (e.n_orb >= 4) sp += -2.0 * e.dd * out->density[(size_t)off[i] * n + off[i] + 1 + k];
+    }
+    out->dipole_point_charge_debye[k] = pc * D;
+    out->dipole_sp_hybrid_debye[k] = sp * D;
+    out->dipole_pd_hybrid_debye[k] = 0.0;
+    out->dipole_debye[k] = pc * D + sp * D;
+  }
+  out->dipole_magnitude = std::sqrt(out->dipole_debye[0] * out->dipole_debye[0] +
+                                    out->dipole_debye[1] * out->dipole_debye[1] +
+                                    out->dipole_debye[2] * out->dipole_debye[2]);
+  out->homo_ev = j.na >= 1 ? eps_ab[j.na - 1] : nan_();
+  out->lumo_ev = j.na < n ? eps_ab[j.na] : nan_();
+  out->homo_ev_beta = nan_();
+  out->lumo_ev_beta = nan_();
+  if (j.uhf) {
+    out->homo_ev_beta = j.nb >= 1 ? eps_ab[j.ld + j.nb - 1] : nan_();
+    out->lumo_ev_beta = j.nb < n ? eps_ab[j.ld + j.nb] : nan_();
+  }
+  if (R.converged) {
+    out->status = PM7_OK;
+  } else {
+    out->status = PM7_ERR_SCF_NOT_CONVERGED;
+    set_msg(out, "SCF not converged after " + std::to_string(R.iterations) + " iterations (error " +
+                     std::to_string(R.dp_err) + ")");
+  }
+}
+
+pm7_status ensure_params(pm7_ctx* ctx, int method) {
+  if (ctx->method == method) return PM7_OK;
+  std::string err;
+  if (!load_params(ctx->data_dir, method, ctx->params, err)) return fail(ctx, PM7_ERR_MISSING_PARAMETER, err);
+  to_device_table(ctx->params, ctx->elem_host);
+  HIPCHK(ctx->d_elem.ensure(ctx->elem_host.size() * sizeof(ElemDev)));
+  HIPCHK(hipMemcpy(ctx->d_elem.p, ctx->elem_host.data(), ctx->elem_host.size() * sizeof(ElemDev), hipMemcpyHostToDevice));
+  std::vector<double2> pp(MAX_Z * MAX_Z);
+  for (int i = 0; i < MAX_Z * MAX_Z; ++i) pp[i] = make_double2(ctx->params.pair[i].first, ctx->params.pair[i].second);
+  HIPCHK(ctx->d_pairpar.ensure(pp.size() * sizeof(double2)));
+  HIPCHK(hipMemcpy(ctx->d_pairpar.p, pp.data(), pp.size() * sizeof(double2), hipMemcpyHostToDevice));
+  ctx->method = method;
+  return PM7_OK;
+}
+
+pm7_status run_jobs(pm7_ctx* ctx, std::vector<Job>& jobs, pm7_result_v1* results, int64_t max_memory_mb) {
+  auto t0 = std::chrono::steady_clock::now();
+  Prepared P;
+  P.jobs = std::move(jobs);
+  pm7_status st = prepare(ctx, P, max_memory_mb);
+  if (st != PM7_OK) {
+    for (auto& j : P.jobs) {
+      results[j.idx].status = st;
+      set_msg(&results[j.idx], ctx->err);
+    }
+    return st;
+  }
+  const int nm = (int)P.jobs.size();
+  // SCF megakernels: one stream per size class, all after the integrals.
+  for (int v = 1; v < SCF_VARIANTS; ++v) HIPCHK(hipStreamWaitEvent(ctx->st[v], ctx->ev[1], 0));
+  for (int v = 0; v < SCF_VARIANTS; ++v)
+    HIPCHK(launch_scf(v, P.b, P.list_off[v], P.list_off[v + 1] - P.list_off[v], ctx->st[v]));
+  for (int v = 1; v < SCF_VARIANTS; ++v) {
+    HIPCHK(hipEventRecord(ctx->ev[2 + v], ctx->st[v]));
+    HIPCHK(hipStreamWaitEvent(ctx->st[0], ctx->ev[2 + v], 0));
+  }
+  HIPCHK(hipEventRecord(ctx->ev[2], ctx->st[0]));
+  HIPCHK(launch_gather(P.b, nm, ctx->st[0]));
+  HIPCHK(hipEventRecord(ctx->ev[5], ctx->st[0]));
+  // Host-side geometric corrections overlap the device work.
+  for (auto& j : P.jobs) {
+    j.corr_kcal = 0.0;
+    if (j.o.method != PM7_METHOD_PM7_MINUS) {
+      j.corr_kcal += dispersion_kcal(j.z, j.xyz);
+      j.corr_kcal += hbond_kcal(j.z, j.xyz);
+      j.corr_kcal += c_triple_kcal(j.z, j.xyz);
+      j.corr_kcal += si_o_h_kcal(j.z, j.xyz);
+    }
+  }
+  std::vector<MolOut> outs(nm);
+  std::vector<double> ecore(nm), vecs(P.vec_words), gath(P.gath_words);
+  HIPCHK(hipMemcpyAsync(outs.data(), P.b.out, nm * sizeof(MolOut), hipMemcpyDeviceToHost, ctx->st[0]));
+  HIPCHK(hipMemcpyAsync(ecore.data(), P.b.e_core_mol, nm * sizeof(double), hipMemcpyDeviceToHost, ctx->st[0]));
+  HIPCHK(hipMemcpyAsync(vecs.data(), P.b.vecs, P.vec_words * sizeof(double), hipMemcpyDeviceToHost, ctx->st[0]));
+  HIPCHK(hipMemcpyAsync(gath.data(), P.b.gath, P.gath_words * sizeof(double), hipMemcpyDeviceToHost, ctx->st[0]));
+  HIPCHK(hipStreamSynchronize(ctx->st[0]));
+  float ms = 0.0f;
+  if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) == hipSuccess) ctx->timings[0] = ms;
+  if (hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]) == hipSuccess) ctx->timings[1] = ms;
+  if (hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[5]) == hipSuccess) ctx->timings[2] = ms;
+  for (int mi = 0; mi < nm; ++mi) {
+    const Job& j = P.jobs[mi];
+    finish_result(ctx, j, outs[mi], ecore[mi], vecs.data() + j.vec0, gath.data() + j.gath0, &results[j.idx]);
+  }
+  ctx->timings[3] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return PM7_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t pm7_abi_version(void) { return PM7_ABI_VERSION; }
+
+void pm7_default_options(pm7_options_v1* o) {
+  std::memset(o, 0, sizeof *o);
+  o->method = PM7_METHOD_PM7;
+  o->charge = 0.0;
+  o->multiplicity = 1;
+  o->max_scf = 200;
+  o->e_tol = 1.0e-8;
+  o->p_tol = 1.0e-7;
+  o->use_diis = 1;
+  o->accelerator = PM7_ACCEL_ADIIS_CDIIS;
+  o->adiis_switch = 0.1;
+  o->level_shift = 0.0;
+  o->reference = PM7_REF_AUTO;
+  o->precision = PM7_PRECISION_FP64;
+  o->mixed_switch = 1.0e-5;
+  o->max_memory_mb = 0;
+}
+
+pm7_status pm7_ctx_create(int32_t device, const char* data_dir, pm7_ctx** out) {
+  if (!out) return PM7_ERR_INVALID_INPUT;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return PM7_ERR_DEVICE;
+  if (device < 0 || device >= count) return PM7_ERR_INVALID_INPUT;
+  if (hipSetDevice(device) != hipSuccess) return PM7_ERR_DEVICE;
+  pm7_ctx* ctx = new pm7_ctx();
+  ctx->device = device;
+  ctx->data_dir = data_dir ? data_dir : default_data_dir();
+  for (int v = 0; v < SCF_VARIANTS; ++v)
+    if (hipStreamCreateWithFlags(&ctx->st[v], hipStreamNonBlocking) != hipSuccess) {
+      delete ctx;
+      return PM7_ERR_DEVICE;
+    }
+  for (auto& e : ctx->ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete ctx;
+      return PM7_ERR_DEVICE;
+    }
+  pm7_status st = ensure_params(ctx, PM7_METHOD_PM7);
+  if (st != PM7_OK) {
+    pm7_ctx_destroy(ctx);
+    return st;
+  }
+  *out = ctx;
+  return PM7_OK;
+}
+
+void pm7_ctx_destroy(pm7_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  for (auto& s : ctx->st)
+    if (s) (void)hipStreamSynchronize(s);
+  DevBuf* bufs[] = {&ctx->d_elem, &ctx->d_pairpar, &ctx->d_mols, &ctx->d_list,  &ctx->d_atom_z, &ctx->d_atom_ao,
+                    &ctx->d_atom_pp, &ctx->d_xyz,   &ctx->d_pairs, &ctx->d_items, &ctx->d_gath_off, &ctx->d_w,
+                    &ctx->d_slab, &ctx->d_vecs,    &ctx->d_out,   &ctx->d_ecore, &ctx->d_ecore_mol, &ctx->d_gath};
+  for (DevBuf* b : bufs) b->release();
+  for (auto& s : ctx->st)
+    if (s) (void)hipStreamDestroy(s);
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  delete ctx;
+}
+
+const char* pm7_last_error(const pm7_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+pm7_status pm7_params_load(pm7_ctx* ctx, int32_t method) {
+  if (!ctx) return PM7_ERR_INVALID_INPUT;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  return ensure_params(ctx, method == PM7_METHOD_PM7_TS ? PM7_METHOD_PM7_TS : PM7_METHOD_PM7);
+}
+
+pm7_status pm7_run_batch(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols, const pm7_options_v1* opts,
+                         int32_t n_opts, pm7_result_v1* results) {
+  if (!ctx || n < 0 || (n > 0 && (!mols || !results))) return PM7_ERR_INVALID_INPUT;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  pm7_options_v1 def;
+  pm7_default_options(&def);
+  std::vector<Job> jobs;
+  int64_t budget = 0;
+  for (int i = 0; i < n; ++i) {
+    std::memset(&results[i], 0, sizeof results[i]);
+    const pm7_options_v1& o = (opts && n_opts > 0) ? opts[n_opts == 1 ? 0 : i] : def;
+    Job j;
+    j.idx = i;
+    std::string msg;
+    pm7_status st = validate(ctx, mols[i], o, j, msg);
+    if (st != PM7_OK) {
+      results[i].status = st;
+      results[i].n_atoms = mols[i].n_atoms;
+      set_msg(&results[i], msg);
+      continue;
+    }
+    if (o.max_memory_mb > 0) budget = budget > 0 ? std::min(budget, o.max_memory_mb) : o.max_memory_mb;
+    jobs.push_back(std::move(j));
+  }
+  if (jobs.empty()) return PM7_OK;
+  pm7_status st = run_jobs(ctx, jobs, results, budget);
+  return st == PM7_ERR_INSUFFICIENT_MEMORY ? PM7_OK : st;
+}
+
+pm7_status pm7_run(pm7_ctx* ctx, const pm7_molecule_v1* mol, const pm7_options_v1* opts, pm7_result_v1* out) {
+  if (!ctx || !mol || !out) return PM7_ERR_INVALID_INPUT;
+  pm7_status st = pm7_run_batch(ctx, 1, mol, opts, opts ? 1 : 0, out);
+  if (st != PM7_OK) return st;
+  return (pm7_status)out->status;
+}
+
+void pm7_result_free(pm7_result_v1* r) {
+  if (!r) return;
+  double** arrays[] = {&r->density, &r->spin_density, &r->mo_energies, &r->mo_coeff,
+                       &r->mo_energies_beta, &r->mo_coeff_beta, &r->charges};
+  for (double** a : arrays) {
+    std::free(*a);
+    *a = nullptr;
+  }
+}
+
+pm7_status pm7_core_hamiltonian(pm7_ctx* ctx, const pm7_molecule_v1* mol, double* h_out, int32_t* nao_out) {
+  if (!ctx || !mol) return PM7_ERR_INVALID_INPUT;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  pm7_options_v1 o;
+  pm7_default_options(&o);
+  o.method = ctx->method;
+  o.reference = PM7_REF_UHF;  // any electron count is fine for the core Hamiltonian
+  o.multiplicity = 1;
+  Job j;
+  std::string msg;
+  pm7_status st = validate(ctx, *mol, o, j, msg);
+  if (st != PM7_OK && st != PM7_ERR_INVALID_INPUT) return fail(ctx, st, msg);
+  if (st == PM7_ERR_INVALID_INPUT) {  // retry as a doublet when the electron count is odd
+    o.multiplicity = 2;
+    st = validate(ctx, *mol, o, j, msg);
+    if (st != PM7_OK) return fail(ctx, st, msg);
+  }
+  if (nao_out) *nao_out = j.nao;
+  if (!h_out) return PM7_OK;
+  Prepared P;
+  P.jobs.push_back(j);
+  st = prepare(ctx, P, 0);
+  if (st != PM7_OK) return st;
+  const Job& J = P.jobs[0];
+  HIPCHK(hipMemcpy2DAsync(h_out, J.nao * sizeof(double), P.b.slab + J.mat0, J.ld * sizeof(double),
+                          J.nao * sizeof(double), J.nao, hipMemcpyDeviceToHost, ctx->st[0]));
+  HIPCHK(hipStreamSynchronize(ctx->st[0]));
+  return PM7_OK;
+}
+
+pm7_status pm7_fock_spin(pm7_ctx* ctx, const pm7_molecule_v1* mol, const double* p_tot, const double* p_spin,
+                         double* f_out) {
+  if (!ctx || !mol || !p_tot || !p_spin || !f_out) return PM7_ERR_INVALID_INPUT;
+  int32_t nao = 0;
+  pm7_status st = pm7_core_hamiltonian(ctx, mol, nullptr, &nao);
+  if (st != PM7_OK) return st;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  pm7_options_v1 o;
+  pm7_default_options(&o);
+  o.method = ctx->method;
+  o.reference = PM7_REF_UHF;
+  Job j;
+  std::string msg;
+  st = validate(ctx, *mol, o, j, msg);
+  if (st != PM7_OK) {
+    o.multiplicity = 2;
+    st = validate(ctx, *mol, o, j, msg);
+    if (st != PM7_OK) return fail(ctx, st, msg);
+  }
+  Prepared P;
+  P.jobs.push_back(j);
+  st = prepare(ctx, P, 0);
+  if (st != PM7_OK) return st;
+  const Job& J = P.jobs[0];
+  const int64_t sq = (int64_t)J.ld * J.ld;
+  double* base = P.b.slab + J.mat0;
+  HIPCHK(hipMemcpy2DAsync(base + 3 * sq, J.ld * sizeof(double), p_tot, J.nao * sizeof(double), J.nao * sizeof(double),
+                          J.nao, hipMemcpyHostToDevice, ctx->st[0]));
+  HIPCHK(hipMemcpy2DAsync(base + 4 * sq, J.ld * sizeof(double), p_spin, J.nao * sizeof(double), J.nao * sizeof(double),
+                          J.nao, hipMemcpyHostToDevice, ctx->st[0]));
+  HIPCHK(launch_fock_once(P.b, 0, 3, 4, 1, ctx->st[0]));
+  HIPCHK(hipMemcpy2DAsync(f_out, J.nao * sizeof(double), base + 1 * sq, J.ld * sizeof(double), J.nao * sizeof(double),
+                          J.nao, hipMemcpyDeviceToHost, ctx->st[0]));
+  HIPCHK(hipStreamSynchronize(ctx->st[0]));
+  return PM7_OK;
+}
+
+pm7_status pm7_core_core_energy(pm7_ctx* ctx, const pm7_molecule_v1* mol, double* e_out) {
+  if (!ctx || !mol || !e_out) return PM7_ERR_INVALID_INPUT;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  pm7_options_v1 o;
+  pm7_default_options(&o);
+  o.method = ctx->method;
+  o.reference = PM7_REF_UHF;
+  Job j;
+  std::string msg;
+  pm7_status st = validate(ctx, *mol, o, j, msg);
+  if (st != PM7_OK) {
+    o.multiplicity = 2;
+    st = validate(ctx, *mol, o, j, msg);
+    if (st != PM7_OK) return fail(ctx, st, msg);
+  }
+  Prepared P;
+  P.jobs.push_back(j);
+  st = prepare(ctx, P, 0);
+  if (st != PM7_OK) return st;
+  HIPCHK(hipMemcpyAsync(e_out, P.b.e_core_mol, sizeof(double), hipMemcpyDeviceToHost, ctx->st[0]));
+  HIPCHK(hipStreamSynchronize(ctx->st[0]));
+  return PM7_OK;
+}
+
+void pm7_last_timings(const pm7_ctx* ctx, double* ms4) {
+  if (!ctx || !ms4) return;
+  for (int k = 0; k < 4; ++k) ms4[k] = ctx->timings[k];
+}
+
+}  // extern "C"

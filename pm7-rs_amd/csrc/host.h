@@ -1,0 +1,42 @@
+// SPDX-License-Identifier: GPL-3.0-or-later
+//
+// Host-side pieces of the engine (parameters, geometric post-SCF corrections, batch layout).
+#pragma once
+
+#include <array>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "pm7_device.h"
+
+namespace pm7 {
+
+struct HostElement {
+  bool present = false;
+  int z = 0, n = 0, ios = 0, iop = 0, iod = 0, n_orb = 0;
+  double core_charge = 0, eheat_ev = 0, e_isol = 0;
+  double u_ss = 0, u_pp = 0, u_dd = 0, zeta_s = 0, zeta_p = 0, zeta_d = 0;
+  double beta_s = 0, beta_p = 0, beta_d = 0;
+  double g_ss = 0, g_sp = 0, g_pp = 0, g_p2 = 0, h_sp = 0, poc = 0;
+  double dd = 0, qq = 0, rho0 = 0, rho1 = 0, rho2 = 0;
+  std::vector<std::array<double, 3>> gauss;
+};
+
+struct ParamSet {
+  int method = 0;
+  HostElement el[MAX_Z];
+  std::vector<std::pair<double, double>> pair;  // dense MAX_Z x MAX_Z (alpb, xfac), fallbacks resolved
+  double vpar[61] = {0};
+};
+
+bool load_params(const std::string& dir, int method, ParamSet& P, std::string& err);
+void to_device_table(const ParamSet& P, std::vector<ElemDev>& out);
+
+// Geometric post-SCF corrections in kcal/mol (coordinates in Bohr).
+double dispersion_kcal(const std::vector<int>& z, const std::vector<double>& xyz);
+double hbond_kcal(const std::vector<int>& z, const std::vector<double>& xyz);
+double c_triple_kcal(const std::vector<int>& z, const std::vector<double>& xyz);
+double si_o_h_kcal(const std::vector<int>& z, const std::vector<double>& xyz);
+
+}  // namespace pm7

@@ -1,0 +1,45 @@
+// SPDX-License-Identifier: GPL-3.0-or-later
+//
+// Host-callable launchers for the engine's HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pm7_device.h"
+
+namespace pm7 {
+
+// Device pointers of one prepared batch.
+struct BatchDev {
+  const MolDev* mols = nullptr;
+  const int* mol_list = nullptr;   // molecule indices grouped by SCF kernel variant
+  const int* atom_z = nullptr;
+  const int* atom_ao = nullptr;    // molecule-relative AO offset
+  const int* atom_pp = nullptr;    // molecule-relative packed-density offset
+  const double* xyz = nullptr;     // Bohr
+  const PairDev* pairs = nullptr;
+  const ElemDev* elem = nullptr;
+  const double2* pairpar = nullptr;
+  const int2* hc_items = nullptr;  // (molecule, atom | element << 24)
+  const int64_t* gath_off = nullptr;
+  double* w = nullptr;
+  double* slab = nullptr;
+  double* vecs = nullptr;
+  MolOut* out = nullptr;
+  double* e_core = nullptr;        // per pair
+  double* e_core_mol = nullptr;    // per molecule
+  double* gath = nullptr;          // packed outputs
+  double4 vpar;
+};
+
+// SCF kernel variants: 0 = <=64 AOs (256 threads, LDS Jacobi), 1 = <=140 AOs (512 threads, LDS
+// Jacobi), 2 = <=1024 AOs (512 threads, L2-resident Jacobi).
+constexpr int SCF_VARIANTS = 3;
+constexpr int VARIANT_MAX_NAO[SCF_VARIANTS] = {64, 140, 1024};
+
+hipError_t launch_integrals(const BatchDev& b, int64_t n_pairs, int n_items, int n_mols, hipStream_t s);
+hipError_t launch_scf(int variant, const BatchDev& b, int list_off, int count, hipStream_t s);
+hipError_t launch_gather(const BatchDev& b, int n_mols, hipStream_t s);
+hipError_t launch_fock_once(const BatchDev& b, int mol, int pt_slot, int ps_slot, int f_slot, hipStream_t s);
+
+}  // namespace pm7

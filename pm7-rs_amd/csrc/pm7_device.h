@@ -1,0 +1,78 @@
+// SPDX-License-Identifier: GPL-3.0-or-later
+//
+// Device-side data layout of the PM7 engine (shared by the HIP kernels and the host driver).
+//
+// HBM layout of one batch (every array is one allocation, molecules concatenated):
+//   atoms  : z, AO offset, packed-pair offset, xyz (Bohr)              -- per atom
+//   pairs  : PairDev {mol, first, second, w_off}, triangular u<v order  -- per atom pair
+//   w      : rotated two-centre blocks, row-major npair_first x npair_second, FP64
+//   slab   : per molecule NSLOT square matrices of leading dimension ld (= nao rounded up to
+//            even) holding H, Fock, densities, eigenvectors and the DIIS history, then the packed
+//            per-atom densities used by the Coulomb contraction
+//   vecs   : per molecule 2*ld eigenvalues (alpha, beta)
+#pragma once
+
+#include <stdint.h>
+
+namespace pm7 {
+
+constexpr double EV = 27.211386245988;        // eV per Hartree (CODATA 2018, MOPAC v23.2.5)
+constexpr double A0 = 0.529177210903;         // Bohr radius in Angstrom
+constexpr double EV_TO_KCAL = 23.06054783061903;
+constexpr double KCAL_TO_EV = 1.0 / EV_TO_KCAL;
+constexpr int MAX_Z = 87;
+constexpr int NSLOT = 32;       // square matrices per molecule in the slab
+constexpr int HIST = 8;         // DIIS history depth
+
+// Slab slots, RHF.
+enum RhfSlot { S_H = 0, S_F = 1, S_W = 2, S_V = 5, S_A = 6, S_T = 7, S_HF = 8, S_HE = 16, S_HD = 24 };
+// Slab slots, UHF (no DIIS history, so the history region is reused).
+enum UhfSlot {
+  U_H = 0, U_FA = 1, U_FB = 2, U_PA = 3, U_PB = 4, U_PAN = 5, U_PBN = 6, U_VA = 7, U_VB = 8,
+  U_A = 9, U_T = 10, U_WA = 11, U_WB = 12, U_PT = 13, U_CA = 14, U_CB = 15
+};
+
+struct ElemDev {
+  double u_ss, u_pp, beta_s, beta_p, core_charge;
+  double g_ss, g_sp, g_pp, g_p2, h_sp;
+  double zeta_s, zeta_p, dd, qq, rho0, rho1, rho2, po9;
+  double gk[4], gl[4], gm[4];
+  int n, n_orb, n_gauss, present;
+};
+
+struct PairDev {
+  int mol;       // batch-local molecule index
+  int f, s;      // batch-global atom indices: `f` carries at least as many AOs as `s`
+  int pad;
+  int64_t w_off; // offset of the npair(f) x npair(s) block in the integral array
+};
+
+struct MolDev {
+  int n_atoms, nao, ld, n_occ, n_occ_b, uhf;
+  int atom0;            // batch-global index of the first atom
+  int max_scf, accel, use_diis, precision, n_pp;
+  int64_t pair0;        // first pair of the molecule
+  int64_t mat0;         // slab offset (doubles)
+  int64_t pp0;          // slab offset of the packed densities
+  int64_t vec0;         // offset into the eigenvalue array
+  double e_tol, p_tol, adiis_switch, shift_floor, sad_yy, mixed_switch;
+};
+
+struct MolOut {
+  double e_el, dp_err, err_norm, shift, e_fp32_last;
+  int iterations, converged, sweeps, p_slot, c_slot, cb_slot, pb_slot, fp32_iters;
+};
+
+__host__ __device__ inline int npair_of(int norb) { return norb * (norb + 1) / 2; }
+
+__host__ __device__ inline int pack_idx(int a, int b) {
+  int h = a > b ? a : b, l = a > b ? b : a;
+  return h * (h + 1) / 2 + l;
+}
+
+// Index of the pair (u, v), u < v, inside a molecule of n atoms (triangular, row-major).
+__host__ __device__ inline int64_t tri_index(int u, int v, int n) {
+  return (int64_t)u * (2 * n - u - 1) / 2 + (v - u - 1);
+}
+
+}  // namespace pm7

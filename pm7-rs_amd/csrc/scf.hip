@@ -1,0 +1,949 @@
+// SPDX-License-Identifier: GPL-3.0-or-later
+//
+// The SCF megakernel: one workgroup owns one molecule for its whole SCF, so a batch of
+// molecules of uneven size is ONE launch and every molecule converges (and stops) on its own.
+// Per iteration (pm7-rs src/scf.rs:1231-1295 RHF, 1380-1470 UHF):
+//
+//   F = H + G(P)              J as an owner-computes gather per (atom, packed orbital pair),
+//                             K per atom pair, one-centre J-K on the atom's own block
+//   E = 1/2 <P, H + F>        fixed-order tree reduction
+//   e = FP - (FP)^T, |e|      2x2 register-tiled product, written into the DIIS ring
+//   A-DIIS / CDIIS            history dots in one fused pass, the <= 9x9 bordered solve or the
+//                             simplex-projected A-DIIS minimisation on thread 0
+//   F' = sum c_i F_i (+ shift (1 - P/2) when the stall guard raised one)
+//   (eps, C) = eig(F')        parallel cyclic Jacobi (round-robin pairing, 2x2 block updates),
+//                             matrix resident in LDS when it fits, eigenvectors in L2
+//   P' = 2 C_occ C_occ^T, dP  2x2 register-tiled product with the RMS change fused in
+//
+// No atomics anywhere: every output element has one owner and every reduction has a fixed
+// order, so a molecule's result does not depend on which other molecules share the launch.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "pm7_device.h"
+
+namespace pm7 {
+
+template <int NT>
+struct Wg {
+  static constexpr int NW = NT / 64;
+};
+
+// Sum NV values over the workgroup (deterministic: fixed shuffle tree, then wave order).
+template <int NT, int NV>
+__device__ inline void wg_sum_n(double (&v)[NV], double* red) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double x = v[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane == 0) red[wave * NV + k] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double s = 0.0;
+    for (int w = 0; w < Wg<NT>::NW; ++w) s += red[w * NV + threadIdx.x];
+    red[Wg<NT>::NW * NV + threadIdx.x] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = red[Wg<NT>::NW * NV + k];
+  __syncthreads();
+}
+
+template <int NT>
+__device__ inline double wg_sum(double x, double* red) {
+  double v[1] = {x};
+  wg_sum_n<NT, 1>(v, red);
+  return v[0];
+}
+
+__device__ inline void unpack_e(int e, int& mu, int& nu) {
+  mu = (e >= 6) ? 3 : (e >= 3) ? 2 : (e >= 1) ? 1 : 0;
+  nu = e - mu * (mu + 1) / 2;
+}
+
+// One-centre two-electron integral (mu nu | la si) of an sp atom (pm7-rs src/fock.rs:69-107).
+__device__ inline double oc_int(int a, int b, int c, int d, const ElemDev& e) {
+  if (a == b && c == d) {
+    if (a == 0 && c == 0) return e.g_ss;
+    if (a == 0 || c == 0) return e.g_sp;
+    return a == c ? e.g_pp : e.g_p2;
+  }
+  int ba = min(a, b), bb = max(a, b), kc = min(c, d), kd = max(c, d);
+  if (ba == 0 && bb != 0 && kc == 0 && kd != 0 && bb == kd) return e.h_sp;
+  if (ba != 0 && bb != 0 && ba != bb && ba == kc && bb == kd) return 0.5 * (e.g_pp - e.g_p2);
+  return 0.0;
+}
+
+struct MolCtx {
+  MolDev M;
+  const int* atom_z;
+  const int* atom_ao;
+  const int* atom_pp;
+  const PairDev* pairs;
+  const ElemDev* elem;
+  const double* w;
+  double* base;  // molecule slab
+  double* pp;    // packed densities
+  int ld;
+  __device__ double* slot(int s) const { return base + (int64_t)s * ld * ld; }
+};
+
+__device__ inline int find_atom(const int* atom_pp, int a0, int na, int t) {
+  int lo = 0, hi = na - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (atom_pp[a0 + mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// F^a = H + J(Pt) - xa K(Psa) (and F^b with Psb when given).
+template <int NT>
+__device__ void fock_build(const MolCtx& c, const double* __restrict__ Pt, const double* __restrict__ Psa, double xa,
+                           const double* __restrict__ Psb, double xb, double* __restrict__ Fa, double* __restrict__ Fb) {
+  const MolDev& M = c.M;
+  const int ld = c.ld;
+  const double* H = c.slot(S_H);
+  // Packed per-atom densities: pp[kl] = P[la][si] * (la == si ? 1 : 2).
+  for (int t = threadIdx.x; t < M.n_pp; t += NT) {
+    int la = find_atom(c.atom_pp, M.atom0, M.n_atoms, t);
+    int a = M.atom0 + la;
+    int e = t - c.atom_pp[a];
+    int mu, nu;
+    unpack_e(e, mu, nu);
+    int o = c.atom_ao[a];
+    c.pp[t] = Pt[(int64_t)(o + mu) * ld + o + nu] * (mu == nu ? 1.0 : 2.0);
+  }
+  __syncthreads();
+  // Diagonal blocks: two-centre Coulomb gather + one-centre J - K.
+  for (int t = threadIdx.x; t < M.n_pp; t += NT) {
+    int la = find_atom(c.atom_pp, M.atom0, M.n_atoms, t);
+    int a = M.atom0 + la;
+    int e = t - c.atom_pp[a];
+    int mu, nu;
+    unpack_e(e, mu, nu);
+    const ElemDev ea = c.elem[c.atom_z[a]];
+    const int npa = npair_of(ea.n_orb);
+    double acc = 0.0;
+    for (int lb = 0; lb < M.n_atoms; ++lb) {
+      if (lb == la) continue;
+      int u = la < lb ? la : lb, v = la < lb ? lb : la;
+      const PairDev P = c.pairs[M.pair0 + tri_index(u, v, M.n_atoms)];
+      int b = M.atom0 + lb;
+      int ppb = c.atom_pp[b];
+      int npb = (lb + 1 < M.n_atoms ? c.atom_pp[b + 1] : M.n_pp) - ppb;
+      const double* wp = c.w + P.w_off;
+      const double* pb = c.pp + ppb;
+      if (P.f == a) {
+        const double* row = wp + (int64_t)e * npb;
+        for (int kl = 0; kl < npb; ++kl) acc += row[kl] * pb[kl];
+      } else {
+        for (int kl = 0; kl < npb; ++kl) acc += wp[(int64_t)kl * npa + e] * pb[kl];
+      }
+    }
+    const int o = c.atom_ao[a];
+    const int n = ea.n_orb;
+    double ja = 0.0, ka = 0.0, kb = 0.0;
+    for (int l = 0; l < n; ++l)
+      for (int s = 0; s < n; ++s) {
+        int64_t ix = (int64_t)(o + l) * ld + o + s;
+        ja += Pt[ix] * oc_int(mu, nu, l, s, ea);
+        double gx = oc_int(mu, l, nu, s, ea);
+        ka += Psa[ix] * gx;
+        if (Fb) kb += Psb[ix] * gx;
+      }
+    const int64_t i1 = (int64_t)(o + mu) * ld + o + nu, i2 = (int64_t)(o + nu) * ld + o + mu;
+    double fa = H[i1] + acc + ja - xa * ka;
+    Fa[i1] = fa;
+    Fa[i2] = fa;
+    if (Fb) {
+      double fb = H[i1] + acc + ja - xb * kb;
+      Fb[i1] = fb;
+      Fb[i2] = fb;
+    }
+  }
+  // Off-diagonal blocks: exchange, one owner per atom pair.
+  const int npairs = M.n_atoms * (M.n_atoms - 1) / 2;
+  for (int p = threadIdx.x; p < npairs; p += NT) {
+    const PairDev P = c.pairs[M.pair0 + p];
+    const ElemDev ef = c.elem[c.atom_z[P.f]];
+    const ElemDev es = c.elem[c.atom_z[P.s]];
+    const int nf = ef.n_orb, ns = es.n_orb, nps = npair_of(ns);
+    const int of = c.atom_ao[P.f], os = c.atom_ao[P.s];
+    const double* wp = c.w + P.w_off;
+    double Xa[4][4], Xb[4][4];
+    for (int i = 0; i < nf; ++i)
+      for (int j = 0; j < ns; ++j) {
+        Xa[i][j] = Psa[(int64_t)(of + i) * ld + os + j];
+        Xb[i][j] = Fb ? Psb[(int64_t)(of + i) * ld + os + j] : 0.0;
+      }
+    for (int mu = 0; mu < nf; ++mu)
+      for (int lam = 0; lam < ns; ++lam) {
+        double acca = 0.0, accb = 0.0;
+        for (int nu = 0; nu < nf; ++nu) {
+          const double* row = wp + (int64_t)pack_idx(mu, nu) * nps;
+          for (int sg = 0; sg < ns; ++sg) {
+            double wv = row[pack_idx(lam, sg)];
+            acca += Xa[nu][sg] * wv;
+            accb += Xb[nu][sg] * wv;
+          }
+        }
+        const int64_t i1 = (int64_t)(of + mu) * ld + os + lam, i2 = (int64_t)(os + lam) * ld + of + mu;
+        double fa = H[i1] - xa * acca;
+        Fa[i1] = fa;
+        Fa[i2] = fa;
+        if (Fb) {
+          double fb = H[i1] - xb * accb;
+          Fb[i1] = fb;
+          Fb[i2] = fb;
+        }
+      }
+  }
+  __syncthreads();
+}
+
+// sum_ij X_ij Y_ij over the n x n block (pad entries are zero).
+template <int NT>
+__device__ double frob(int n, int ld, const double* X, const double* Y, double* red) {
+  double acc = 0.0;
+  for (int idx = threadIdx.x; idx < n * n; idx += NT) {
+    int i = idx / n, j = idx - i * n;
+    acc += X[(int64_t)i * ld + j] * Y[(int64_t)i * ld + j];
+  }
+  return wg_sum<NT>(acc, red);
+}
+
+// ||FP - (FP)^T||_F^2; the commutator itself is stored into E when E != nullptr.
+template <int NT>
+__device__ double commutator(int n, int ld, const double* __restrict__ F, const double* __restrict__ P,
+                             double* __restrict__ E, double* red) {
+  const int nt = ld / 2;
+  double nrm = 0.0;
+  for (int t = threadIdx.x; t < nt * nt; t += NT) {
+    int I = t / nt, J = t - I * nt;
+    if (I > J) continue;
+    const int i0 = 2 * I, j0 = 2 * J;
+    double tij[2][2] = {{0, 0}, {0, 0}}, tji[2][2] = {{0, 0}, {0, 0}};
+    const double* F0 = F + (int64_t)i0 * ld;
+    const double* F1 = F0 + ld;
+    const double* G0 = F + (int64_t)j0 * ld;
+    const double* G1 = G0 + ld;
+    const double* P0 = P + (int64_t)i0 * ld;
+    const double* P1 = P0 + ld;
+    const double* Q0 = P + (int64_t)j0 * ld;
+    const double* Q1 = Q0 + ld;
+    for (int k = 0; k < n; ++k) {
+      double f0 = F0[k], f1 = F1[k], g0 = G0[k], g1 = G1[k];
+      double p0 = P0[k], p1 = P1[k], q0 = Q0[k], q1 = Q1[k];
+      tij[0][0] += f0 * q0;
+      tij[0][1] += f0 * q1;
+      tij[1][0] += f1 * q0;
+      tij[1][1] += f1 * q1;
+      tji[0][0] += g0 * p0;
+      tji[0][1] += g0 * p1;
+      tji[1][0] += g1 * p0;
+      tji[1][1] += g1 * p1;
+    }
+    double part = 0.0;
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b) {
+        double e = tij[a][b] - tji[b][a];
+        part += e * e;
+        if (E) {
+          E[(int64_t)(i0 + a) * ld + j0 + b] = e;
+          E[(int64_t)(j0 + b) * ld + i0 + a] = -e;
+        }
+      }
+    nrm += (I == J) ? part : 2.0 * part;
+  }
+  return wg_sum<NT>(nrm, red);
+}
+
+// P' = w C_occ C_occ^T (C columns sorted ascending); returns sum (P' - P)^2 over n x n.
+template <int NT>
+__device__ double density(int n, int ld, int nocc, double wgt, const double* __restrict__ C,
+                          const double* __restrict__ Pold, double* __restrict__ Pn, double* red) {
+  const int nt = ld / 2;
+  double dsq = 0.0;
+  for (int t = threadIdx.x; t < nt * nt; t += NT) {
+    int I = t / nt, J = t - I * nt;
+    if (I > J) continue;
+    const int i0 = 2 * I, j0 = 2 * J;
+    const double* A0 = C + (int64_t)i0 * ld;
+    const double* A1 = A0 + ld;
+    const double* B0 = C + (int64_t)j0 * ld;
+    const double* B1 = B0 + ld;
+    double s[2][2] = {{0, 0}, {0, 0}};
+    for (int k = 0; k < nocc; ++k) {
+      double a0 = A0[k], a1 = A1[k], b0 = B0[k], b1 = B1[k];
+      s[0][0] += a0 * b0;
+      s[0][1] += a0 * b1;
+      s[1][0] += a1 * b0;
+      s[1][1] += a1 * b1;
+    }
+    double part = 0.0;
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b) {
+        const int i = i0 + a, j = j0 + b;
+        double v = (i < n && j < n) ? wgt * s[a][b] : 0.0;
+        Pn[(int64_t)i * ld + j] = v;
+        Pn[(int64_t)j * ld + i] = v;
+        double d = v - Pold[(int64_t)i * ld + j];
+        part += d * d;
+      }
+
+    dsq += (I == J) ? part : 2.0 * part;
+  }
+  return wg_sum<NT>(dsq, red);
+}
+
+__device__ inline int rr_player(int pos, int r, int n2) { return pos == 0 ? 0 : 1 + (pos - 1 + r) % (n2 - 1); }
+
+// Parallel cyclic Jacobi eigensolve of the symmetric ld x ld matrix Win (pad rows/cols zero).
+// Writes ascending eigenvalues eps[0..n) and the matching eigenvectors as the columns of C.
+// A lives in LDS when LDSN > 0, otherwise in the global workspace Ag. Returns the sweep count.
+template <int NT, int LDSN, int MAXN>
+__device__ int jacobi(int n, int ld, const double* __restrict__ Win, double* Ag, double* __restrict__ V,
+                      double* __restrict__ C, double* __restrict__ eps, double* s_a, double* s_c, double* s_s,
+                      double* s_t, int* s_perm, double* red) {
+  const int n2 = ld, m = ld / 2;
+  double* A = (LDSN > 0) ? s_a : Ag;
+  for (int idx = threadIdx.x; idx < n2 * n2; idx += NT) {
+    int i = idx / n2, j = idx - i * n2;
+    A[idx] = Win[idx];
+    V[idx] = (i == j) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  int sweep = 0;
+  for (; sweep < 40; ++sweep) {
+    double sums[2] = {0.0, 0.0};
+    for (int idx = threadIdx.x; idx < n2 * n2; idx += NT) {
+      int i = idx / n2, j = idx - i * n2;
+      double v = A[idx];
+      if (i == j) sums[1] += v * v;
+      else sums[0] += v * v;
+    }
+    wg_sum_n<NT, 2>(sums, red);
+    if (sums[0] <= 1.0e-26 * (sums[0] + sums[1])) break;
+    for (int r = 0; r < n2 - 1; ++r) {
+      if (threadIdx.x < m) {
+        const int k = threadIdx.x;
+        const int p = rr_player(k, r, n2), q = rr_player(n2 - 1 - k, r, n2);
+        const double apq = A[p * n2 + q];
+        double cc = 1.0, ss = 0.0, tt = 0.0;
+        if (apq != 0.0) {
+          double tau = (A[q * n2 + q] - A[p * n2 + p]) / (2.0 * apq);
+          if (fabs(tau) > 1.0e150) tt = 0.5 / tau;
+          else tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+          cc = 1.0 / sqrt(1.0 + tt * tt);
+          ss = tt * cc;
+        }
+        s_c[k] = cc;
+        s_s[k] = ss;
+        s_t[k] = tt;
+      }
+      __syncthreads();
+      for (int b = threadIdx.x; b < m * m; b += NT) {
+        const int k = b / m, l = b - k * m;
+        const int pk = rr_player(k, r, n2), qk = rr_player(n2 - 1 - k, r, n2);
+        if (k == l) {
+          const double apq = A[pk * n2 + qk], t = s_t[k];
+          A[pk * n2 + pk] -= t * apq;
+          A[qk * n2 + qk] += t * apq;
+          A[pk * n2 + qk] = 0.0;
+          A[qk * n2 + pk] = 0.0;
+          continue;
+        }
+        const int pl = rr_player(l, r, n2), ql = rr_player(n2 - 1 - l, r, n2);
+        const double ck = s_c[k], sk = s_s[k], cl = s_c[l], sl = s_s[l];
+        const double x00 = A[pk * n2 + pl], x01 = A[pk * n2 + ql], x10 = A[qk * n2 + pl], x11 = A[qk * n2 + ql];
+        const double y00 = ck * x00 - sk * x10, y01 = ck * x01 - sk * x11;
+        const double y10 = sk * x00 + ck * x10, y11 = sk * x01 + ck * x11;
+        A[pk * n2 + pl] = cl * y00 - sl * y01;
+        A[pk * n2 + ql] = sl * y00 + cl * y01;
+        A[qk * n2 + pl] = cl * y10 - sl * y11;
+        A[qk * n2 + ql] = sl * y10 + cl * y11;
+      }
+      for (int it = threadIdx.x; it < n2 * m; it += NT) {
+        const int i = it / m, k = it - i * m;
+        const double ss = s_s[k];
+        if (ss == 0.0) continue;
+        const int pk = rr_player(k, r, n2), qk = rr_player(n2 - 1 - k, r, n2);
+        const double cc = s_c[k];
+        const double vp = V[(int64_t)i * n2 + pk], vq = V[(int64_t)i * n2 + qk];
+        V[(int64_t)i * n2 + pk] = cc * vp - ss * vq;
+        V[(int64_t)i * n2 + qk] = ss * vp + cc * vq;
+      }
+      __syncthreads();
+    }
+  }
+  // Ascending order of the real eigenvalues (ties broken by index).
+  for (int i = threadIdx.x; i < n; i += NT) {
+    const double di = A[i * n2 + i];
+    int rank = 0;
+    for (int j = 0; j < n; ++j) {
+      const double dj = A[j * n2 + j];
+      rank += (dj < di || (dj == di && j < i)) ? 1 : 0;
+    }
+    s_perm[rank] = i;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < n; k += NT) eps[k] = A[s_perm[k] * n2 + s_perm[k]];
+  for (int idx = threadIdx.x; idx < n2 * n2; idx += NT) {
+    const int i = idx / n2, k = idx - i * n2;
+    C[idx] = (i < n && k < n) ? V[(int64_t)i * n2 + s_perm[k]] : 0.0;
+  }
+  __syncthreads();
+  return sweep;
+}
+
+// ---- small serial solvers (thread 0) -------------------------------------------------------
+
+__device__ bool solve_small(double* m, double* rhs, int n, double* x) {
+  for (int col = 0; col < n; ++col) {
+    int piv = col;
+    double best = fabs(m[col * n + col]);
+    for (int row = col + 1; row < n; ++row) {
+      double v = fabs(m[row * n + col]);
+      if (v > best) {
+        best = v;
+        piv = row;
+      }
+    }
+    if (best < 1.0e-12) return false;
+    if (piv != col) {
+      for (int j = 0; j < n; ++j) {
+        double t = m[col * n + j];
+        m[col * n + j] = m[piv * n + j];
+        m[piv * n + j] = t;
+      }
+      double t = rhs[col];
+      rhs[col] = rhs[piv];
+      rhs[piv] = t;
+    }
+    for (int row = col + 1; row < n; ++row) {
+      double factor = m[row * n + col] / m[col * n + col];
+      if (factor == 0.0) continue;
+      for (int j = col; j < n; ++j) m[row * n + j] -= factor * m[col * n + j];
+      rhs[row] -= factor * rhs[col];
+    }
+  }
+  for (int col = n - 1; col >= 0; --col) {
+    double sum = rhs[col];
+    for (int j = col + 1; j < n; ++j) sum -= m[col * n + j] * x[j];
+    x[col] = sum / m[col * n + col];
+  }
+  return true;
+}
+
+// CDIIS over the chronological window (pm7-rs src/scf.rs:1684-1771): returns the number of
+// coefficients (0 on failure); coefficient i belongs to chronological entry skip + i.
+__device__ int cdiis(int cnt, const int* slot, const double* G, double* coef, int* skip_out) {
+  if (cnt < 2) return 0;
+  for (int skip = 0; skip < cnt - 1; ++skip) {
+    const int m = cnt - skip, dim = m + 1;
+    double scale = 0.0;
+    for (int i = 0; i < m; ++i) scale = fmax(scale, fabs(G[slot[skip + i] * HIST + slot[skip + i]]));
+    if (!isfinite(scale) || scale <= 0.0) continue;
+    double b[(HIST + 1) * (HIST + 1)], rhs[HIST + 1], x[HIST + 1];
+    for (int i = 0; i < dim * dim; ++i) b[i] = 0.0;
+    for (int i = 0; i < m; ++i) {
+      for (int j = 0; j < m; ++j) b[i * dim + j] = G[slot[skip + i] * HIST + slot[skip + j]] / scale;
+      b[i * dim + m] = -1.0;
+      b[m * dim + i] = -1.0;
+      rhs[i] = 0.0;
+    }
+    rhs[m] = -1.0;
+    if (!solve_small(b, rhs, dim, x)) continue;
+    bool finite = true;
+    for (int i = 0; i < dim; ++i) finite = finite && isfinite(x[i]);
+    if (!finite) continue;
+    double sum = 0.0;
+    for (int i = 0; i < m; ++i) sum += fabs(x[i]);
+    if (sum <= 25.0) {
+      for (int i = 0; i < m; ++i) coef[i] = x[i];
+      *skip_out = skip;
+      return m;
+    }
+  }
+  return 0;
+}
+
+__device__ void simplex_project(const double* v, int n, double* out) {
+  double u[HIST];
+  for (int i = 0; i < n; ++i) u[i] = v[i];
+  for (int i = 1; i < n; ++i) {  // insertion sort, descending
+    double key = u[i];
+    int j = i - 1;
+    while (j >= 0 && u[j] < key) {
+      u[j + 1] = u[j];
+      --j;
+    }
+    u[j + 1] = key;
+  }
+  double css = 0.0, theta = 0.0;
+  for (int j = 0; j < n; ++j) {
+    css += u[j];
+    double t = (css - 1.0) / (double)(j + 1);
+    if (u[j] - t > 0.0) theta = t;
+  }
+  for (int i = 0; i < n; ++i) out[i] = fmax(v[i] - theta, 0.0);
+}
+
+// A-DIIS on the simplex (pm7-rs src/scf.rs:1812-1916) from M[s][t] = <D_s, F_t>.
+__device__ int adiis(int cnt, const int* slot, const double* Mm, double* coef) {
+  if (cnt < 2) return 0;
+  const int n = cnt, sn = slot[n - 1];
+  double d[HIST], s[HIST][HIST];
+  const double mnn = Mm[sn * HIST + sn];
+  for (int i = 0; i < n; ++i) {
+    const int si = slot[i];
+    d[i] = Mm[si * HIST + sn] - mnn;
+    for (int j = 0; j < n; ++j) {
+      const int sj = slot[j];
+      s[i][j] = Mm[si * HIST + sj] - Mm[si * HIST + sn] - Mm[sn * HIST + sj] + mnn;
+    }
+  }
+  double l = 1.0e-12;
+  for (int i = 0; i < n; ++i) {
+    double row = 0.0;
+    for (int j = 0; j < n; ++j) row += fabs(s[i][j] + s[j][i]);
+    l = fmax(l, row);
+  }
+  const double lr = 1.0 / l;
+  double c[HIST], g[HIST], trial[HIST], proj[HIST];
+  for (int i = 0; i < n; ++i) c[i] = (i == n - 1) ? 1.0 : 0.0;
+  for (int it = 0; it < 400; ++it) {
+    for (int k = 0; k < n; ++k) {
+      double acc = 2.0 * d[k];
+      for (int j = 0; j < n; ++j) acc += (s[k][j] + s[j][k]) * c[j];
+      g[k] = acc;
+    }
+    for (int i = 0; i < n; ++i) trial[i] = c[i] - lr * g[i];
+    simplex_project(trial, n, proj);
+    double delta = 0.0;
+    for (int i = 0; i < n; ++i) {
+      delta += fabs(proj[i] - c[i]);
+      c[i] = proj[i];
+    }
+    if (delta < 1.0e-12) break;
+  }
+  for (int i = 0; i < n; ++i) coef[i] = c[i];
+  return n;
+}
+
+// Level-shift stall guard (pm7-rs src/scf.rs:1581-1656); state lives in LDS.
+struct Guard {
+  double floor_, shift;
+  double steps[10], energies[10];
+  int n;
+};
+
+__device__ void guard_update(Guard& g, double step, double energy) {
+  g.steps[g.n] = step;
+  g.energies[g.n] = energy;
+  g.n++;
+  if (g.n < 10) return;
+  double sn = INFINITY, sw = INFINITY, en = INFINITY, ew = INFINITY;
+  for (int i = 0; i < 5; ++i) {
+    sw = fmin(sw, g.steps[i]);
+    ew = fmin(ew, g.energies[i]);
+    sn = fmin(sn, g.steps[5 + i]);
+    en = fmin(en, g.energies[5 + i]);
+  }
+  bool moved = false;
+  if (sn <= sw * 0.1) {
+    if (g.shift > g.floor_) {
+      double relaxed = fmax(g.shift * 0.5, g.floor_);
+      g.shift = relaxed < 0.25 ? g.floor_ : relaxed;
+      moved = true;
+    }
+  } else if (sn > sw * 0.9 && en > ew - 1.0e-9 && g.shift < 64.0) {
+    g.shift = g.shift < 1.0 ? 1.0 : fmin(g.shift * 2.0, 64.0);
+    moved = true;
+  }
+  if (moved) {
+    g.n = 0;
+  } else {
+    for (int i = 0; i < 9; ++i) {
+      g.steps[i] = g.steps[i + 1];
+      g.energies[i] = g.energies[i + 1];
+    }
+    g.n = 9;
+  }
+}
+
+template <int NT>
+__device__ void level_shift(int n, int ld, double* W, const double* P, double occ, double shift) {
+  if (shift <= 0.0) return;
+  const double inv = 1.0 / occ;
+  for (int idx = threadIdx.x; idx < n * n; idx += NT) {
+    int i = idx / n, j = idx - i * n;
+    double q = P[(int64_t)i * ld + j] * inv;
+    W[(int64_t)i * ld + j] += shift * ((i == j) ? 1.0 - q : -q);
+  }
+  __syncthreads();
+}
+
+// MOPAC diagonal guess (pm7-rs src/scf.rs:1050-1114) for sp atoms, scaled by `scale`.
+template <int NT>
+__device__ void sad_guess(const MolCtx& c, double* P, double scale) {
+  const int ld = c.ld;
+  for (int idx = threadIdx.x; idx < ld * ld; idx += NT) P[idx] = 0.0;
+  __syncthreads();
+  for (int la = threadIdx.x; la < c.M.n_atoms; la += NT) {
+    int a = c.M.atom0 + la;
+    const ElemDev e = c.elem[c.atom_z[a]];
+    int o = c.atom_ao[a];
+    double v = e.n_orb == 1 ? e.core_charge - c.M.sad_yy : e.core_charge * 0.25 - c.M.sad_yy;
+    for (int k = 0; k < e.n_orb; ++k) P[(int64_t)(o + k) * ld + o + k] = v * scale;
+  }
+  __syncthreads();
+}
+
+template <int NT>
+__device__ void copy_mat(int ld, const double* src, double* dst) {
+  for (int idx = threadIdx.x; idx < ld * ld; idx += NT) dst[idx] = src[idx];
+  __syncthreads();
+}
+
+template <int NT, int LDSN, int MAXN>
+__global__ void __launch_bounds__(NT)
+    k_scf(const int* __restrict__ mol_list, const MolDev* __restrict__ mols, const int* __restrict__ atom_z,
+          const int* __restrict__ atom_ao, const int* __restrict__ atom_pp, const PairDev* __restrict__ pairs,
+          const ElemDev* __restrict__ elem, const double* __restrict__ w, double* __restrict__ slab,
+          double* __restrict__ vecs, MolOut* __restrict__ out) {
+  __shared__ double s_a[LDSN > 0 ? LDSN * LDSN : 1];
+  __shared__ double s_red[(Wg<NT>::NW + 1) * 3 * HIST];
+  __shared__ double s_c[MAXN / 2], s_s[MAXN / 2], s_t[MAXN / 2];
+  __shared__ int s_perm[MAXN];
+  __shared__ double s_G[HIST * HIST], s_M[HIST * HIST], s_coef[HIST + 1];
+  __shared__ int s_slot[HIST], s_cslot[HIST];
+  __shared__ int s_int[8];     // 0 head, 1 cnt, 2 ncoef, 3 done
+  __shared__ double s_dbl[8];  // 0 e_old, 1 err, 2 dp
+  __shared__ Guard s_guard;
+
+  const int mi = mol_list[blockIdx.x];
+  MolCtx c;
+  c.M = mols[mi];
+  c.atom_z = atom_z;
+  c.atom_ao = atom_ao;
+  c.atom_pp = atom_pp;
+  c.pairs = pairs;
+  c.elem = elem;
+  c.w = w;
+  c.ld = c.M.ld;
+  c.base = slab + c.M.mat0;
+  c.pp = slab + c.M.pp0;
+  const int n = c.M.nao, ld = c.ld;
+  double* eps_a = vecs + c.M.vec0;
+  double* eps_b = eps_a + ld;
+  int sweeps = 0;
+  if (threadIdx.x == 0) {
+    s_guard.floor_ = c.M.shift_floor;
+    s_guard.shift = c.M.shift_floor;
+    s_guard.n = 0;
+    s_int[0] = 0;
+    s_int[1] = 0;
+    s_int[3] = 0;
+    s_dbl[0] = 0.0;
+  }
+  __syncthreads();
+  double* H = c.slot(S_H);
+  MolOut res;
+  res.converged = 0;
+  res.iterations = 0;
+  res.dp_err = INFINITY;
+  res.err_norm = 0.0;
+  res.fp32_iters = 0;
+  res.e_fp32_last = 0.0;
+
+  if (!c.M.uhf) {
+    // ------------------------------------------------------------------ RHF
+    sad_guess<NT>(c, c.slot(S_HD), 1.0);
+    const int nocc = c.M.n_occ;
+    int iter = 0;
+    for (; iter < c.M.max_scf; ++iter) {
+      const int h = s_int[0];
+      double* P = c.slot(S_HD + h);
+      double* F = c.slot(S_HF + h);
+      double* E = c.slot(S_HE + h);
+      fock_build<NT>(c, P, P, 0.5, nullptr, 0.0, F, nullptr);
+      double eh[2] = {0.0, 0.0};
+      for (int idx = threadIdx.x; idx < n * n; idx += NT) {
+        int i = idx / n, j = idx - i * n;
+        int64_t x = (int64_t)i * ld + j;
+        eh[0] += P[x] * H[x];
+        eh[1] += P[x] * F[x];
+      }
+      wg_sum_n<NT, 2>(eh, s_red);
+      const double e = 0.5 * (eh[0] + eh[1]);
+      const double err = sqrt(commutator<NT>(n, ld, F, P, E, s_red));
+      if (threadIdx.x == 0) {
+        if (s_int[1] < HIST) s_int[1]++;
+        s_int[0] = (h + 1) % HIST;
+        const int cnt = s_int[1];
+        for (int i = 0; i < cnt; ++i) s_slot[i] = (s_int[0] - cnt + i + 2 * HIST) % HIST;
+      }
+      __syncthreads();
+      const int cnt = s_int[1];
+      // History dots: <E_h,E_t>, <D_h,F_t>, <D_t,F_h> for every live slot t.
+      {
+        double acc[3 * HIST];
+        for (int k = 0; k < 3 * HIST; ++k) acc[k] = 0.0;
+        const double* Fh = F;
+        for (int idx = threadIdx.x; idx < ld * ld; idx += NT) {
+          const double eh0 = E[idx], dh = P[idx], fh = Fh[idx];
+#pragma unroll
+          for (int q = 0; q < HIST; ++q) {
+            if (q < cnt) {
+              const int t = s_slot[q];
+              const int64_t off = (int64_t)t * ld * ld + idx;
+              acc[3 * q] += eh0 * c.slot(S_HE)[off];
+              acc[3 * q + 1] += dh * c.slot(S_HF)[off];
+              acc[3 * q + 2] += c.slot(S_HD)[off] * fh;
+            }
+          }
+        }
+        wg_sum_n<NT, 3 * HIST>(acc, s_red);
+        if (threadIdx.x == 0) {
+          for (int q = 0; q < cnt; ++q) {
+            const int t = s_slot[q];
+            s_G[h * HIST + t] = acc[3 * q];
+            s_G[t * HIST + h] = acc[3 * q];
+            s_M[h * HIST + t] = acc[3 * q + 1];
+            s_M[t * HIST + h] = acc[3 * q + 2];
+          }
+          int ncoef = 0, skip = 0;
+          if (c.M.use_diis && c.M.accel != 0) {
+            if (c.M.accel == 1 || err <= c.M.adiis_switch) {
+              ncoef = cdiis(cnt, s_slot, s_G, s_coef, &skip);
+            } else {
+              ncoef = adiis(cnt, s_slot, s_M, s_coef);
+              skip = 0;
+            }
+          }
+          for (int i = 0; i < ncoef; ++i) s_cslot[i] = s_slot[skip + i];
+          s_int[2] = ncoef;
+          s_dbl[1] = err;
+        }
+        __syncthreads();
+      }
+      // Extrapolated Fock.
+      double* W = c.slot(S_W);
+      const int ncoef = s_int[2];
+      for (int idx = threadIdx.x; idx < ld * ld; idx += NT) {
+        double v;
+        if (ncoef == 0) {
+          v = F[idx];
+        } else {
+          v = 0.0;
+          for (int i = 0; i < ncoef; ++i) v += s_coef[i] * c.slot(S_HF)[(int64_t)s_cslot[i] * ld * ld + idx];
+        }
+        W[idx] = v;
+      }
+      __syncthreads();
+      level_shift<NT>(n, ld, W, P, 2.0, s_guard.shift);
+      sweeps += jacobi<NT, LDSN, MAXN>(n, ld, W, c.slot(S_A), c.slot(S_T), c.slot(S_V), eps_a, s_a, s_c, s_s, s_t,
+                                       s_perm, s_red);
+      double* Pn = c.slot(S_HD + s_int[0]);
+      const double dsq = density<NT>(n, ld, nocc, 2.0, c.slot(S_V), P, Pn, s_red);
+      if (threadIdx.x == 0) {
+        const double dp = sqrt(dsq / (double)((int64_t)n * n));
+        const double de = fabs(e - s_dbl[0]);
+        s_dbl[0] = e;
+        s_dbl[2] = dp;
+        const bool resid_ok = s_guard.shift == 0.0 && err < 1.0e-7;
+        if (iter > 0 && de < c.M.e_tol && (dp < c.M.p_tol || resid_ok)) {
+          s_int[3] = 1;
+        } else {
+          guard_update(s_guard, dp, e);
+        }
+      }
+      __syncthreads();
+      if (s_int[3]) {
+        ++iter;
+        break;
+      }
+    }
+    res.iterations = iter > c.M.max_scf ? c.M.max_scf : iter;
+    res.converged = s_int[3];
+    res.dp_err = s_dbl[2];
+    res.err_norm = s_dbl[1];
+    // Final energy from the converged density.
+    const int pslot = S_HD + s_int[0];
+    double* P = c.slot(pslot);
+    double* F = c.slot(S_F);
+    fock_build<NT>(c, P, P, 0.5, nullptr, 0.0, F, nullptr);
+    double eh[2] = {0.0, 0.0};
+    for (int idx = threadIdx.x; idx < n * n; idx += NT) {
+      int i = idx / n, j = idx - i * n;
+      int64_t x = (int64_t)i * ld + j;
+      eh[0] += P[x] * H[x];
+      eh[1] += P[x] * F[x];
+    }
+    wg_sum_n<NT, 2>(eh, s_red);
+    res.e_el = 0.5 * (eh[0] + eh[1]);
+    if (s_guard.shift > 0.0)
+      sweeps += jacobi<NT, LDSN, MAXN>(n, ld, F, c.slot(S_A), c.slot(S_T), c.slot(S_V), eps_a, s_a, s_c, s_s, s_t,
+                                       s_perm, s_red);
+    res.p_slot = pslot;
+    res.c_slot = S_V;
+    res.pb_slot = -1;
+    res.cb_slot = -1;
+  } else {
+    // ------------------------------------------------------------------ UHF (no DIIS)
+    const int na = c.M.n_occ, nb = c.M.n_occ_b;
+    const double ntot = (double)max(na + nb, 1);
+    double* Pa = c.slot(U_PA);
+    double* Pb = c.slot(U_PB);
+    double* Pan = c.slot(U_PAN);
+    double* Pbn = c.slot(U_PBN);
+    double* Pt = c.slot(U_PT);
+    double* Fa = c.slot(U_FA);
+    double* Fb = c.slot(U_FB);
+    sad_guess<NT>(c, Pa, na / ntot);
+    sad_guess<NT>(c, Pb, nb / ntot);
+    int iter = 0;
+    for (; iter < c.M.max_scf; ++iter) {
+      for (int idx = threadIdx.x; idx < ld * ld; idx += NT) Pt[idx] = Pa[idx] + Pb[idx];
+      __syncthreads();
+      fock_build<NT>(c, Pt, Pa, 1.0, Pb, 1.0, Fa, Fb);
+      double eh[3] = {0.0, 0.0, 0.0};
+      for (int idx = threadIdx.x; idx < n * n; idx += NT) {
+        int i = idx / n, j = idx - i * n;
+        int64_t x = (int64_t)i * ld + j;
+        eh[0] += Pt[x] * H[x];
+        eh[1] += Pa[x] * Fa[x];
+        eh[2] += Pb[x] * Fb[x];
+      }
+      wg_sum_n<NT, 3>(eh, s_red);
+      const double e = 0.5 * (eh[0] + eh[1] + eh[2]);
+      const double ea2 = commutator<NT>(n, ld, Fa, Pa, nullptr, s_red);
+      const double eb2 = commutator<NT>(n, ld, Fb, Pb, nullptr, s_red);
+      const double err = sqrt(ea2 + eb2);
+      const double shift = s_guard.shift;
+      level_shift<NT>(n, ld, Fa, Pa, 1.0, shift);
+      level_shift<NT>(n, ld, Fb, Pb, 1.0, shift);
+      sweeps += jacobi<NT, LDSN, MAXN>(n, ld, Fa, c.slot(U_A), c.slot(U_T), c.slot(U_CA), eps_a, s_a, s_c, s_s, s_t,
+                                       s_perm, s_red);
+      sweeps += jacobi<NT, LDSN, MAXN>(n, ld, Fb, c.slot(U_A), c.slot(U_T), c.slot(U_CB), eps_b, s_a, s_c, s_s, s_t,
+                                       s_perm, s_red);
+      const double da = density<NT>(n, ld, na, 1.0, c.slot(U_CA), Pa, Pan, s_red);
+      const double db = density<NT>(n, ld, nb, 1.0, c.slot(U_CB), Pb, Pbn, s_red);
+      if (threadIdx.x == 0) {
+        const double nn = (double)((int64_t)n * n);
+        const double dp = sqrt(da / nn) + sqrt(db / nn);
+        const double de = fabs(e - s_dbl[0]);
+        s_dbl[0] = e;
+        s_dbl[2] = dp;
+        s_dbl[1] = err;
+        const bool resid_ok = s_guard.shift == 0.0 && err < 1.0e-7;
+        if (iter > 0 && de < c.M.e_tol && (dp < c.M.p_tol || resid_ok)) s_int[3] = 1;
+        else guard_update(s_guard, dp, e);
+      }
+      __syncthreads();
+      double* tmp = Pa;
+      Pa = Pan;
+      Pan = tmp;
+      tmp = Pb;
+      Pb = Pbn;
+      Pbn = tmp;
+      if (s_int[3]) {
+        ++iter;
+        break;
+      }
+    }
+    res.iterations = iter > c.M.max_scf ? c.M.max_scf : iter;
+    res.converged = s_int[3];
+    res.dp_err = s_dbl[2];
+    res.err_norm = s_dbl[1];
+    for (int idx = threadIdx.x; idx < ld * ld; idx += NT) Pt[idx] = Pa[idx] + Pb[idx];
+    __syncthreads();
+    fock_build<NT>(c, Pt, Pa, 1.0, Pb, 1.0, Fa, Fb);
+    double eh[3] = {0.0, 0.0, 0.0};
+    for (int idx = threadIdx.x; idx < n * n; idx += NT) {
+      int i = idx / n, j = idx - i * n;
+      int64_t x = (int64_t)i * ld + j;
+      eh[0] += Pt[x] * H[x];
+      eh[1] += Pa[x] * Fa[x];
+      eh[2] += Pb[x] * Fb[x];
+    }
+    wg_sum_n<NT, 3>(eh, s_red);
+    res.e_el = 0.5 * (eh[0] + eh[1] + eh[2]);
+    if (s_guard.shift > 0.0)
+      sweeps += jacobi<NT, LDSN, MAXN>(n, ld, Fa, c.slot(U_A), c.slot(U_T), c.slot(U_CA), eps_a, s_a, s_c, s_s, s_t,
+                                       s_perm, s_red);
+    sweeps += jacobi<NT, LDSN, MAXN>(n, ld, Fb, c.slot(U_A), c.slot(U_T), c.slot(U_CB), eps_b, s_a, s_c, s_s, s_t,
+                                     s_perm, s_red);
+    res.p_slot = (Pa == c.slot(U_PA)) ? U_PA : U_PAN;
+    res.pb_slot = (Pb == c.slot(U_PB)) ? U_PB : U_PBN;
+    res.c_slot = U_CA;
+    res.cb_slot = U_CB;
+  }
+  if (threadIdx.x == 0) {
+    res.shift = s_guard.shift;
+    res.sweeps = sweeps;
+    out[mi] = res;
+  }
+}
+
+// Instances: LDS-resident Jacobi for molecules up to 140 AOs, L2-resident above that.
+template __global__ void k_scf<512, 140, 140>(const int*, const MolDev*, const int*, const int*, const int*,
+                                              const PairDev*, const ElemDev*, const double*, double*, double*, MolOut*);
+template __global__ void k_scf<256, 64, 64>(const int*, const MolDev*, const int*, const int*, const int*,
+                                            const PairDev*, const ElemDev*, const double*, double*, double*, MolOut*);
+template __global__ void k_scf<512, 0, 1024>(const int*, const MolDev*, const int*, const int*, const int*,
+                                             const PairDev*, const ElemDev*, const double*, double*, double*, MolOut*);
+
+// One Fock build on a molecule's slab (the build_fock_spin entry point): F = H + J(Pt) - K(Ps).
+__global__ void __launch_bounds__(256) k_fock_once(int mi, const MolDev* __restrict__ mols, const int* __restrict__ atom_z,
+                                                    const int* __restrict__ atom_ao, const int* __restrict__ atom_pp,
+                                                    const PairDev* __restrict__ pairs, const ElemDev* __restrict__ elem,
+                                                    const double* __restrict__ w, double* __restrict__ slab, int pt_slot,
+                                                    int ps_slot, int f_slot) {
+  MolCtx c;
+  c.M = mols[mi];
+  c.atom_z = atom_z;
+  c.atom_ao = atom_ao;
+  c.atom_pp = atom_pp;
+  c.pairs = pairs;
+  c.elem = elem;
+  c.w = w;
+  c.ld = c.M.ld;
+  c.base = slab + c.M.mat0;
+  c.pp = slab + c.M.pp0;
+  fock_build<256>(c, c.slot(pt_slot), c.slot(ps_slot), 1.0, nullptr, 0.0, c.slot(f_slot), nullptr);
+}
+
+hipError_t launch_scf(int variant, const BatchDev& b, int list_off, int count, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  const int* list = b.mol_list + list_off;
+  switch (variant) {
+    case 0:
+      hipLaunchKernelGGL((k_scf<256, 64, 64>), dim3(count), dim3(256), 0, s, list, b.mols, b.atom_z, b.atom_ao,
+                         b.atom_pp, b.pairs, b.elem, b.w, b.slab, b.vecs, b.out);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_scf<512, 140, 140>), dim3(count), dim3(512), 0, s, list, b.mols, b.atom_z, b.atom_ao,
+                         b.atom_pp, b.pairs, b.elem, b.w, b.slab, b.vecs, b.out);
+      break;
+    default:
+      hipLaunchKernelGGL((k_scf<512, 0, 1024>), dim3(count), dim3(512), 0, s, list, b.mols, b.atom_z, b.atom_ao,
+                         b.atom_pp, b.pairs, b.elem, b.w, b.slab, b.vecs, b.out);
+      break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_fock_once(const BatchDev& b, int mol, int pt_slot, int ps_slot, int f_slot, hipStream_t s) {
+  hipLaunchKernelGGL(k_fock_once, dim3(1), dim3(256), 0, s, mol, b.mols, b.atom_z, b.atom_ao, b.atom_pp, b.pairs,
+                     b.elem, b.w, b.slab, pt_slot, ps_slot, f_slot);
+  return hipGetLastError();
+}
+
+}  // namespace pm7

@@ -1,0 +1,454 @@
+# SPDX-License-Identifier: GPL-3.0-or-later
+"""pm7amd: the PM7 NDDO SCF single point on one MI355X, mirroring pm7-rs's API.
+
+The Python surface follows pm7-rs names and semantics (``Molecule``, ``Pm7Parameters``,
+``Pm7Options``, ``run_pm7`` -> ``Pm7Result``, ``Pm7Error``; src/scf.rs:73-342, 512-867) and calls
+the HIP engine through the C ABI declared in ``include/pm7_amd.h``. There is no CPU fallback: if
+the library or a HIP device is missing, every call raises ``Pm7Error``.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from pathlib import Path
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libpm7amd.so"
+DATA_DIR = _HERE.parent / "data"
+
+ANGSTROM_TO_BOHR = 1.0 / 0.529177210903
+EV_TO_KCAL = 23.06054783061903
+
+STATUS_NAMES = {
+    0: "Ok", 1: "InvalidInput", 2: "MissingElement", 3: "MissingParameter", 4: "UnsupportedFeature",
+    5: "LinearAlgebra", 6: "ScfNotConverged", 7: "InsufficientMemory", 8: "Device",
+}
+
+METHODS = {"pm7": 0, "pm7-ts": 1, "pm7ts": 1, "pm7-minus": 3, "pm7-": 3, "pm7minus": 3}
+ACCELERATORS = {"none": 0, "cdiis": 1, "adiis-cdiis": 2, "adiis_cdiis": 2, "hybrid": 2}
+REFERENCES = {"auto": 0, "rhf": 1, "restricted": 1, "r": 1, "uhf": 2, "unrestricted": 2, "u": 2}
+PRECISIONS = {"fp64": 0, "mixed": 1}
+
+SYMBOLS = ["X", "H", "He", "Li", "Be", "B", "C", "N", "O", "F", "Ne", "Na", "Mg", "Al", "Si", "P", "S",
+           "Cl", "Ar", "K", "Ca", "Sc", "Ti", "V", "Cr", "Mn", "Fe", "Co", "Ni", "Cu", "Zn", "Ga", "Ge",
+           "As", "Se", "Br", "Kr"]
+Z_OF = {s.lower(): i for i, s in enumerate(SYMBOLS)}
+
+
+class Pm7Error(RuntimeError):
+    """Mirror of pm7-rs ``Pm7Error`` (src/error.rs:13-48): ``kind`` is the variant name."""
+
+    def __init__(self, status: int, message: str, result: Optional["Pm7Result"] = None):
+        self.status = int(status)
+        self.kind = STATUS_NAMES.get(self.status, f"Status{status}")
+        self.result = result
+        super().__init__(f"{self.kind}: {message}")
+
+
+# ---- C ABI structs (include/pm7_amd.h) -----------------------------------------------------
+
+class _Molecule(C.Structure):
+    _fields_ = [("n_atoms", C.c_int32), ("z", C.POINTER(C.c_int32)), ("xyz_bohr", C.POINTER(C.c_double))]
+
+
+class _Options(C.Structure):
+    _fields_ = [("method", C.c_int32), ("charge", C.c_double), ("multiplicity", C.c_int32),
+                ("max_scf", C.c_int32), ("e_tol", C.c_double), ("p_tol", C.c_double),
+                ("use_diis", C.c_int32), ("accelerator", C.c_int32), ("adiis_switch", C.c_double),
+                ("level_shift", C.c_double), ("reference", C.c_int32), ("precision", C.c_int32),
+                ("mixed_switch", C.c_double), ("max_memory_mb", C.c_int64)]
+
+
+class _Result(C.Structure):
+    _fields_ = [("status", C.c_int32), ("iterations", C.c_int32), ("converged", C.c_int32),
+                ("unrestricted", C.c_int32), ("nao", C.c_int32), ("n_atoms", C.c_int32),
+                ("n_occ", C.c_int32), ("n_occ_beta", C.c_int32), ("fp32_iterations", C.c_int32),
+                ("jacobi_sweeps", C.c_int32), ("electronic_ev", C.c_double), ("core_ev", C.c_double),
+                ("total_ev", C.c_double), ("heat_of_formation_kcal", C.c_double),
+                ("correction_kcal", C.c_double), ("density_error", C.c_double),
+                ("homo_ev", C.c_double), ("lumo_ev", C.c_double), ("homo_ev_beta", C.c_double),
+                ("lumo_ev_beta", C.c_double), ("dipole_debye", C.c_double * 3),
+                ("dipole_point_charge_debye", C.c_double * 3), ("dipole_sp_hybrid_debye", C.c_double * 3),
+                ("dipole_pd_hybrid_debye", C.c_double * 3), ("dipole_magnitude", C.c_double),
+                ("density", C.POINTER(C.c_double)), ("spin_density", C.POINTER(C.c_double)),
+                ("mo_energies", C.POINTER(C.c_double)), ("mo_coeff", C.POINTER(C.c_double)),
+                ("mo_energies_beta", C.POINTER(C.c_double)), ("mo_coeff_beta", C.POINTER(C.c_double)),
+                ("charges", C.POINTER(C.c_double)), ("message", C.c_char * 256)]
+
+
+ABI_FUNCTIONS = [
+    "pm7_ctx_create", "pm7_ctx_destroy", "pm7_last_error", "pm7_abi_version", "pm7_params_load",
+    "pm7_default_options", "pm7_run", "pm7_run_batch", "pm7_result_free", "pm7_core_hamiltonian",
+    "pm7_fock_spin", "pm7_core_core_energy", "pm7_last_timings",
+]
+
+_lib = None
+
+
+def library() -> C.CDLL:
+    """Load the engine library (raises Pm7Error when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise Pm7Error(8, f"engine library missing: build it with `make -C pm7-rs_amd` ({LIB_PATH})")
+    lib = C.CDLL(str(LIB_PATH))
+    vp = C.c_void_p
+    lib.pm7_ctx_create.argtypes = [C.c_int32, C.c_char_p, C.POINTER(vp)]
+    lib.pm7_ctx_destroy.argtypes = [vp]
+    lib.pm7_last_error.argtypes = [vp]
+    lib.pm7_last_error.restype = C.c_char_p
+    lib.pm7_params_load.argtypes = [vp, C.c_int32]
+    lib.pm7_default_options.argtypes = [C.POINTER(_Options)]
+    lib.pm7_run.argtypes = [vp, C.POINTER(_Molecule), C.POINTER(_Options), C.POINTER(_Result)]
+    lib.pm7_run_batch.argtypes = [vp, C.c_int32, C.POINTER(_Molecule), C.POINTER(_Options), C.c_int32,
+                                  C.POINTER(_Result)]
+    lib.pm7_result_free.argtypes = [C.POINTER(_Result)]
+    lib.pm7_core_hamiltonian.argtypes = [vp, C.POINTER(_Molecule), C.POINTER(C.c_double), C.POINTER(C.c_int32)]
+    lib.pm7_fock_spin.argtypes = [vp, C.POINTER(_Molecule), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double)]
+    lib.pm7_core_core_energy.argtypes = [vp, C.POINTER(_Molecule), C.POINTER(C.c_double)]
+    lib.pm7_last_timings.argtypes = [vp, C.POINTER(C.c_double)]
+    _lib = lib
+    return lib
+
+
+# ---- reference-shaped Python types ---------------------------------------------------------
+
+@dataclass
+class Atom:
+    z: int
+    position: tuple  # Bohr
+
+
+@dataclass
+class Molecule:
+    """pm7-rs ``Molecule`` (src/system.rs:20-39): atoms with Bohr positions."""
+
+    atoms: list
+    charge: float = 0.0
+    multiplicity: int = 1
+
+    @classmethod
+    def from_symbols_angstrom(cls, atoms: Sequence, charge: float = 0.0, multiplicity: int = 1) -> "Molecule":
+        out = []
+        for sym, x, y, z in atoms:
+            zz = Z_OF[sym.lower()] if isinstance(sym, str) else int(sym)
+            out.append(Atom(zz, (x * ANGSTROM_TO_BOHR, y * ANGSTROM_TO_BOHR, z * ANGSTROM_TO_BOHR)))
+        return cls(out, charge, multiplicity)
+
+    @classmethod
+    def from_xyz_str(cls, text: str, charge: float = 0.0) -> "Molecule":
+        """Plain XYZ (Angstrom) as pm7-rs ``Molecule::from_xyz_str`` reads it (src/system.rs:149-196)."""
+        text = text.lstrip("﻿")
+        lines = text.splitlines()
+        if not lines:
+            raise Pm7Error(1, "empty XYZ")
+        try:
+            n = int(lines[0].strip())
+        except ValueError:
+            raise Pm7Error(1, f"invalid XYZ atom count: {lines[0]}") from None
+        atoms = []
+        for k in range(n):
+            if 2 + k >= len(lines):
+                raise Pm7Error(1, f"XYZ ended before atom {k}")
+            parts = lines[2 + k].split()
+            if len(parts) < 4:
+                raise Pm7Error(1, f"XYZ line {k + 3} has fewer than 4 fields")
+            sym = parts[0].lower()
+            if sym not in Z_OF:
+                raise Pm7Error(1, f"unknown element on line {k + 3}: {parts[0]}")
+            atoms.append((sym, float(parts[1]), float(parts[2]), float(parts[3])))
+        return cls.from_symbols_angstrom(atoms, charge)
+
+    def arrays(self):
+        z = np.ascontiguousarray([a.z for a in self.atoms], dtype=np.int32)
+        xyz = np.ascontiguousarray([a.position for a in self.atoms], dtype=np.float64).reshape(-1, 3)
+        return z, xyz
+
+
+@dataclass
+class Pm7Parameters:
+    """Selects the bundled parameter tables (pm7-rs ``Pm7Parameters::method``)."""
+
+    method: str = "pm7"
+
+    @classmethod
+    def standard(cls) -> "Pm7Parameters":
+        return cls("pm7")
+
+
+@dataclass
+class Pm7Options:
+    """pm7-rs ``Pm7Options`` (src/scf.rs:73-234) plus the engine's precision switch."""
+
+    method: str = "pm7"
+    charge: float = 0.0
+    multiplicity: int = 1
+    max_scf: int = 200
+    e_tol: float = 1.0e-8
+    p_tol: float = 1.0e-7
+    use_diis: bool = True
+    accelerator: str = "adiis-cdiis"
+    adiis_switch: float = 0.1
+    level_shift: float = 0.0
+    reference: str = "auto"
+    precision: str = "fp64"
+    mixed_switch: float = 1.0e-5
+    max_memory_mb: Optional[int] = None
+
+    def to_c(self) -> _Options:
+        o = _Options()
+        library().pm7_default_options(C.byref(o))
+        o.method = METHODS[self.method.lower()]
+        o.charge = float(self.charge)
+        o.multiplicity = int(self.multiplicity)
+        o.max_scf = int(self.max_scf)
+        o.e_tol = float(self.e_tol)
+        o.p_tol = float(self.p_tol)
+        o.use_diis = 1 if self.use_diis else 0
+        o.accelerator = ACCELERATORS[self.accelerator.lower()]
+        o.adiis_switch = float(self.adiis_switch)
+        o.level_shift = float(self.level_shift)
+        o.reference = REFERENCES[self.reference.lower()]
+        o.precision = PRECISIONS[self.precision.lower()]
+        o.mixed_switch = float(self.mixed_switch)
+        o.max_memory_mb = int(self.max_memory_mb or 0)
+        return o
+
+
+@dataclass
+class Pm7Result:
+    """pm7-rs ``Pm7Result`` (src/scf.rs:262-342), molecular fields."""
+
+    status: int
+    density: np.ndarray
+    spin_density: Optional[np.ndarray]
+    mo_energies: np.ndarray
+    mo_coeff: np.ndarray
+    n_occ: int
+    mo_energies_beta: Optional[np.ndarray]
+    mo_coeff_beta: Optional[np.ndarray]
+    n_occ_beta: Optional[int]
+    electronic_ev: float
+    core_ev: float
+    total_ev: float
+    heat_of_formation_kcal: float
+    correction_kcal: float
+    charges: np.ndarray
+    dipole_debye: np.ndarray
+    dipole_magnitude: float
+    dipole_point_charge_debye: np.ndarray
+    dipole_sp_hybrid_debye: np.ndarray
+    dipole_pd_hybrid_debye: np.ndarray
+    homo_ev: Optional[float]
+    lumo_ev: Optional[float]
+    homo_ev_beta: Optional[float]
+    lumo_ev_beta: Optional[float]
+    iterations: int
+    converged: bool
+    unrestricted: bool
+    density_error: float
+    fp32_iterations: int = 0
+    jacobi_sweeps: int = 0
+    message: str = ""
+
+    @property
+    def gap_ev(self) -> Optional[float]:
+        if self.homo_ev is None or self.lumo_ev is None:
+            return None
+        return self.lumo_ev - self.homo_ev
+
+
+def _opt(x: float) -> Optional[float]:
+    return None if math.isnan(x) else float(x)
+
+
+def _convert(r: _Result) -> Pm7Result:
+    n = r.nao
+    na = r.n_atoms
+
+    def mat(ptr, rows, cols):
+        if not ptr:
+            return None
+        return np.ctypeslib.as_array(ptr, shape=(rows * cols,)).reshape(rows, cols).copy()
+
+    def vec(ptr, k):
+        if not ptr:
+            return None
+        return np.ctypeslib.as_array(ptr, shape=(k,)).copy()
+
+    if not r.density:
+        empty = np.zeros((0, 0))
+        dens = empty
+    else:
+        dens = mat(r.density, n, n)
+    return Pm7Result(
+        status=r.status, density=dens, spin_density=mat(r.spin_density, n, n),
+        mo_energies=vec(r.mo_energies, n) if r.mo_energies else np.zeros(0),
+        mo_coeff=mat(r.mo_coeff, n, n) if r.mo_coeff else np.zeros((0, 0)),
+        n_occ=r.n_occ, mo_energies_beta=vec(r.mo_energies_beta, n), mo_coeff_beta=mat(r.mo_coeff_beta, n, n),
+        n_occ_beta=r.n_occ_beta if r.n_occ_beta >= 0 else None,
+        electronic_ev=r.electronic_ev, core_ev=r.core_ev, total_ev=r.total_ev,
+        heat_of_formation_kcal=r.heat_of_formation_kcal, correction_kcal=r.correction_kcal,
+        charges=vec(r.charges, na) if r.charges else np.zeros(0),
+        dipole_debye=np.array(r.dipole_debye[:]), dipole_magnitude=r.dipole_magnitude,
+        dipole_point_charge_debye=np.array(r.dipole_point_charge_debye[:]),
+        dipole_sp_hybrid_debye=np.array(r.dipole_sp_hybrid_debye[:]),
+        dipole_pd_hybrid_debye=np.array(r.dipole_pd_hybrid_debye[:]),
+        homo_ev=_opt(r.homo_ev), lumo_ev=_opt(r.lumo_ev), homo_ev_beta=_opt(r.homo_ev_beta),
+        lumo_ev_beta=_opt(r.lumo_ev_beta), iterations=r.iterations, converged=bool(r.converged),
+        unrestricted=bool(r.unrestricted), density_error=r.density_error, fp32_iterations=r.fp32_iterations,
+        jacobi_sweeps=r.jacobi_sweeps, message=r.message.decode(errors="replace"),
+    )
+
+
+class Engine:
+    """A device context (pm7_ctx): device, streams, parameter tables, pooled HBM."""
+
+    def __init__(self, device: int = 0, data_dir: Optional[str] = None, method: str = "pm7"):
+        lib = library()
+        h = C.c_void_p()
+        st = lib.pm7_ctx_create(int(device), str(data_dir or DATA_DIR).encode(), C.byref(h))
+        if st != 0:
+            raise Pm7Error(st, f"pm7_ctx_create failed on device {device}")
+        self._h = h
+        self._lib = lib
+        if METHODS[method.lower()] == 1:
+            self.load(method)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pm7_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover - interpreter teardown order varies
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def last_error(self) -> str:
+        return self._lib.pm7_last_error(self._h).decode(errors="replace")
+
+    def load(self, method: str):
+        st = self._lib.pm7_params_load(self._h, METHODS[method.lower()])
+        if st != 0:
+            raise Pm7Error(st, self.last_error())
+
+    @staticmethod
+    def _mol(m: Molecule):
+        z, xyz = m.arrays()
+        cm = _Molecule(len(z), z.ctypes.data_as(C.POINTER(C.c_int32)), xyz.ctypes.data_as(C.POINTER(C.c_double)))
+        return cm, (z, xyz)
+
+    def run(self, molecule: Molecule, options: Optional[Pm7Options] = None, raise_on_error: bool = True) -> Pm7Result:
+        return self.run_batch([molecule], options, raise_on_error=raise_on_error)[0]
+
+    def run_batch(self, molecules: Sequence[Molecule], options=None, raise_on_error: bool = False):
+        """Many molecules in one device pass; ``options`` is one Pm7Options or one per molecule."""
+        n = len(molecules)
+        if n == 0:
+            return []
+        if options is None:
+            options = Pm7Options()
+        opts_list = options if isinstance(options, (list, tuple)) else [options]
+        c_mols = (_Molecule * n)()
+        keep = []
+        for i, m in enumerate(molecules):
+            cm, k = self._mol(m)
+            c_mols[i] = cm
+            keep.append(k)
+        c_opts = (_Options * len(opts_list))(*[o.to_c() for o in opts_list])
+        c_res = (_Result * n)()
+        st = self._lib.pm7_run_batch(self._h, n, c_mols, c_opts, len(opts_list), c_res)
+        if st != 0:
+            raise Pm7Error(st, self.last_error())
+        out = []
+        for i in range(n):
+            r = _convert(c_res[i])
+            self._lib.pm7_result_free(C.byref(c_res[i]))
+            if raise_on_error and r.status != 0:
+                raise Pm7Error(r.status, r.message, r)
+            out.append(r)
+        return out
+
+    def core_hamiltonian(self, molecule: Molecule) -> np.ndarray:
+        cm, keep = self._mol(molecule)
+        nao = C.c_int32(0)
+        st = self._lib.pm7_core_hamiltonian(self._h, C.byref(cm), None, C.byref(nao))
+        if st != 0:
+            raise Pm7Error(st, self.last_error())
+        h = np.zeros((nao.value, nao.value))
+        st = self._lib.pm7_core_hamiltonian(self._h, C.byref(cm), h.ctypes.data_as(C.POINTER(C.c_double)),
+                                            C.byref(nao))
+        if st != 0:
+            raise Pm7Error(st, self.last_error())
+        return h
+
+    def fock_spin(self, molecule: Molecule, p_tot: np.ndarray, p_spin: np.ndarray) -> np.ndarray:
+        cm, keep = self._mol(molecule)
+        pt = np.ascontiguousarray(p_tot, dtype=np.float64)
+        ps = np.ascontiguousarray(p_spin, dtype=np.float64)
+        f = np.zeros_like(pt)
+        st = self._lib.pm7_fock_spin(self._h, C.byref(cm), pt.ctypes.data_as(C.POINTER(C.c_double)),
+                                     ps.ctypes.data_as(C.POINTER(C.c_double)),
+                                     f.ctypes.data_as(C.POINTER(C.c_double)))
+        if st != 0:
+            raise Pm7Error(st, self.last_error())
+        return f
+
+    def core_core_energy(self, molecule: Molecule) -> float:
+        cm, keep = self._mol(molecule)
+        e = C.c_double(0.0)
+        st = self._lib.pm7_core_core_energy(self._h, C.byref(cm), C.byref(e))
+        if st != 0:
+            raise Pm7Error(st, self.last_error())
+        return e.value
+
+    def last_timings_ms(self):
+        buf = (C.c_double * 4)()
+        self._lib.pm7_last_timings(self._h, buf)
+        return list(buf)
+
+
+_default_engine: Optional[Engine] = None
+
+
+def default_engine() -> Engine:
+    global _default_engine
+    if _default_engine is None:
+        _default_engine = Engine(0)
+    return _default_engine
+
+
+def run_pm7(molecule: Molecule, params: Optional[Pm7Parameters] = None,
+            options: Optional[Pm7Options] = None) -> Pm7Result:
+    """Drop-in for pm7-rs ``run_pm7`` (src/scf.rs:512-516): raises Pm7Error like ``Err(..)``."""
+    options = options or Pm7Options(charge=molecule.charge, multiplicity=molecule.multiplicity)
+    if params is not None and METHODS[params.method.lower()] != METHODS[options.method.lower()]:
+        raise Pm7Error(1, f"options method `{options.method}` does not match parameter method `{params.method}`")
+    eng = default_engine()
+    return eng.run(molecule, options, raise_on_error=True)
+
+
+def run_pm7_batch(molecules: Sequence[Molecule], options=None) -> list:
+    """Batched single points; per-molecule failures are reported in ``result.status``."""
+    return default_engine().run_batch(molecules, options)
+
+
+__all__ = [
+    "Atom", "Molecule", "Pm7Parameters", "Pm7Options", "Pm7Result", "Pm7Error", "Engine", "run_pm7",
+    "run_pm7_batch", "default_engine", "library", "ABI_FUNCTIONS", "LIB_PATH", "DATA_DIR",
+]

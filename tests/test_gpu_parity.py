@@ -1,0 +1,169 @@
+# SPDX-License-Identifier: GPL-3.0-or-later
+"""GPU parity: the HIP engine against the CPU FP64 oracle and pm7-rs's golden vectors.
+
+Bars (north star): |dE| <= 1e-6 eV, |dHf| <= 1e-4 kcal/mol, max |dP| <= 1e-7; stage-level
+pieces (H_core, one Fock build, core-core) to 1e-9 or better since they involve no iteration.
+"""
+
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle_binding as ob
+import pm7amd
+from pm7amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+CASES = json.loads((Path(__file__).parent / "golden" / "reference_cases.json").read_text())["cases"]
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = pm7amd.Engine(0)
+    yield e
+    e.close()
+
+
+def mol_of(atoms, charge=0.0, mult=1):
+    return pm7amd.Molecule.from_symbols_angstrom(atoms, charge, mult)
+
+
+def oracle_of(atoms, charge=0.0, mult=1):
+    z, xyz = ob.atoms_to_arrays(atoms)
+    return ob.run(z, xyz, charge, mult)
+
+
+@pytest.mark.parametrize("name", ["water", "formaldehyde", "ammonium", "hf_chain_8", "alkane_c20", "methyl_radical"])
+def test_core_hamiltonian_matches_oracle(eng, name):
+    say(f"hcore {name}")
+    c = CASES[name]
+    h = eng.core_hamiltonian(mol_of(c["atoms"], c["charge"], c["multiplicity"]))
+    z, xyz = ob.atoms_to_arrays(c["atoms"])
+    ho = ob.hcore(z, xyz)
+    assert h.shape == ho.shape
+    assert np.max(np.abs(h - ho)) <= 1e-9
+
+
+@pytest.mark.parametrize("name", ["water", "formic_acid_dimer", "alkane_c20", "water_wire_8"])
+def test_fock_build_matches_oracle(eng, name):
+    say(f"fock {name}")
+    c = CASES[name]
+    z, xyz = ob.atoms_to_arrays(c["atoms"])
+    n = ob.nao(z)
+    rng = np.random.default_rng(7)
+    a = rng.normal(size=(n, n)) * 0.3
+    pt = a + a.T
+    b = rng.normal(size=(n, n)) * 0.2
+    ps = b + b.T
+    f = eng.fock_spin(mol_of(c["atoms"]), pt, ps)
+    fo = ob.fock(z, xyz, pt, ps)
+    assert np.max(np.abs(f - fo)) <= 1e-9
+
+
+@pytest.mark.parametrize("name", ["water", "ammonium", "alkane_c30", "boron_trifluoride"])
+def test_core_core_matches_oracle(eng, name):
+    say(f"core-core {name}")
+    c = CASES[name]
+    z, xyz = ob.atoms_to_arrays(c["atoms"])
+    e = eng.core_core_energy(mol_of(c["atoms"]))
+    assert abs(e - ob.core_core(z, xyz)) <= 1e-9
+
+
+def test_golden_cases_in_one_batch(eng):
+    names = sorted(CASES)
+    say(f"golden batch of {len(names)}")
+    mols = [mol_of(CASES[k]["atoms"], CASES[k]["charge"], CASES[k]["multiplicity"]) for k in names]
+    opts = [pm7amd.Pm7Options(charge=CASES[k]["charge"], multiplicity=CASES[k]["multiplicity"]) for k in names]
+    res = eng.run_batch(mols, opts)
+    bad = []
+    for k, r in zip(names, res):
+        c = CASES[k]
+        de = r.total_ev - c["energy_ev"]
+        dh = r.heat_of_formation_kcal - c["heat_of_formation_kcal"]
+        dq = float(np.max(np.abs(r.charges - np.array(c["charges"]))))
+        if r.status != 0 or abs(de) > 1e-6 or abs(dh) > 1e-4 or dq > 1e-6:
+            bad.append((k, r.status, de, dh, dq, r.message))
+    assert not bad, bad
+
+
+@pytest.mark.parametrize("name", ["water", "benzene", "glycine", "alkane_c30", "water_wire_12", "nitrogen_dioxide",
+                                  "allyl_radical", "carbonate"])
+def test_engine_matches_oracle_density(eng, name):
+    say(f"scf {name}")
+    c = CASES[name]
+    r = eng.run(mol_of(c["atoms"], c["charge"], c["multiplicity"]),
+                pm7amd.Pm7Options(charge=c["charge"], multiplicity=c["multiplicity"]))
+    res, dens, spin, eps, q = oracle_of(c["atoms"], c["charge"], c["multiplicity"])
+    assert abs(r.total_ev - res.total_ev) <= 1e-6
+    assert abs(r.heat_of_formation_kcal - res.hof_kcal) <= 1e-4
+    assert np.max(np.abs(r.density - dens)) <= 1e-7
+    assert np.max(np.abs(r.mo_energies - eps)) <= 1e-5
+    if r.unrestricted:
+        assert np.max(np.abs(r.spin_density - spin)) <= 1e-7
+
+
+def test_config1_water(eng):
+    say("config 1 water")
+    atoms = W.water()
+    r = eng.run(mol_of(atoms))
+    res, dens, *_ = oracle_of(atoms)
+    assert r.converged and res.status == 0
+    assert abs(r.total_ev - res.total_ev) <= 1e-6
+    assert abs(r.heat_of_formation_kcal - res.hof_kcal) <= 1e-4
+    assert np.max(np.abs(r.density - dens)) <= 1e-7
+
+
+def test_config2_alkane_c32(eng):
+    say("config 2 C32H66")
+    atoms = W.alkane_chain(32, 2)
+    r = eng.run(mol_of(atoms))
+    assert r.density.shape == (194, 194)
+    res, dens, *_ = oracle_of(atoms)
+    assert abs(r.total_ev - res.total_ev) <= 1e-6
+    assert abs(r.heat_of_formation_kcal - res.hof_kcal) <= 1e-4
+    assert np.max(np.abs(r.density - dens)) <= 1e-7
+
+
+def test_batch_result_is_the_same_alone_and_in_the_mix(eng):
+    mols_atoms = W.organic_batch(24, 11)
+    say("batch vs alone, 24 molecules")
+    mols = [mol_of(a) for a in mols_atoms]
+    together = eng.run_batch(mols)
+    for i in (0, 5, 17, 23):
+        alone = eng.run(mols[i])
+        assert alone.total_ev == together[i].total_ev
+        assert np.array_equal(alone.density, together[i].density)
+    for i, a in enumerate(mols_atoms[:6]):
+        res, dens, *_ = oracle_of(a)
+        assert abs(together[i].total_ev - res.total_ev) <= 1e-6
+        assert np.max(np.abs(together[i].density - dens)) <= 1e-7
+
+
+def test_error_paths_are_reported_per_molecule(eng):
+    say("error paths")
+    water = mol_of(W.water())
+    coincident = mol_of([["O", 0, 0, 0], ["H", 0, 0, 0], ["H", 0.96, 0, 0]])
+    sulfur = mol_of([["S", 0, 0, 0], ["H", 1.34, 0, 0], ["H", -0.35, 1.29, 0]])
+    odd = mol_of([["O", 0, 0, 0], ["H", 0.97, 0, 0]])
+    res = eng.run_batch([water, coincident, sulfur, odd, water],
+                        [pm7amd.Pm7Options(), pm7amd.Pm7Options(), pm7amd.Pm7Options(), pm7amd.Pm7Options(),
+                         pm7amd.Pm7Options(max_scf=1)])
+    assert [r.status for r in res] == [0, 1, 4, 1, 6]
+    assert np.isfinite(res[4].total_ev) and res[4].iterations == 1
+    assert res[0].total_ev == pytest.approx(oracle_of(W.water())[0].total_ev, abs=1e-6)
+
+
+def test_insufficient_memory_is_refused_before_allocation(eng):
+    say("memory budget")
+    big = mol_of(W.alkane_chain(32, 2))
+    r = eng.run(big, pm7amd.Pm7Options(max_memory_mb=1), raise_on_error=False)
+    assert r.status == 7

@@ -1,0 +1,191 @@
+#!/usr/bin/env python3
+# SPDX-License-Identifier: GPL-3.0-or-later
+"""Benchmark: batched PM7 SCF single points on one MI355X.
+
+Workload: 256 closed-shell H/C/N/O molecules of uneven size (3-60 atoms, heavy-atom count uniform
+in 1-25, seeded random walk, seed 3). One step = one batched solve of all 256 molecules, inputs
+uploaded and integrals built inside the step (a step is the whole request as a caller sees it).
+
+Prints one JSON line: solves/s (value), per-step latency p50/p99, the one-molecule-at-a-time FP64
+GPU path for comparison, parity of a sample against the CPU FP64 oracle, the roofline of the
+dominant kernel (HIP events on that kernel's own stream) and the CPU baseline (the oracle timed on
+this host's cores on a bounded sample).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "pm7-rs_amd"))
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+
+
+def pct(xs, q):
+    s = sorted(xs)
+    if not s:
+        return None
+    k = (len(s) - 1) * q
+    lo = int(k)
+    hi = min(lo + 1, len(s) - 1)
+    return s[lo] + (s[hi] - s[lo]) * (k - lo)
+
+
+def scf_flops(results, classes):
+    """Algorithmic FP64 flops per size class: per SCF iteration 2n^3 (commutator FP),
+    2 n^2 n_occ (density), 9 n^3 (dense symmetric eigensolve with vectors, Golub & Van Loan's
+    symmetric-QR count), doubled for UHF."""
+    out = [0.0, 0.0, 0.0, 0.0]
+    for r in results:
+        n = r.density.shape[0]
+        if n == 0:
+            continue
+        c = 0 if n <= 64 else (1 if n <= 112 else (2 if n <= 196 else 3))
+        spins = 2 if r.unrestricted else 1
+        per = 11.0 * n ** 3 + 2.0 * n * n * r.n_occ
+        out[c] += spins * per * r.iterations
+    return out
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--molecules", type=int, default=256)
+    ap.add_argument("--single", type=int, default=32, help="molecules timed one at a time")
+    ap.add_argument("--cpu-sample", type=int, default=24, help="molecules timed on the CPU oracle")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: every box here has a single MI355X; the batched path "
+              "shards by molecule with no collective, so N GPUs = N independent copies.", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    import pm7amd
+    from pm7amd import workloads as W
+
+    t0 = time.time()
+    batch = W.organic_batch(args.molecules, 3)
+    mols = [pm7amd.Molecule.from_symbols_angstrom(a) for a in batch]
+    print(f"[bench] workload built: {len(mols)} molecules in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    eng = pm7amd.Engine(0)
+
+    def run_mode(precision):
+        opts = pm7amd.Pm7Options(precision=precision)
+        for _ in range(args.warmup):
+            eng.run_batch(mols, opts)
+        torch.cuda.synchronize()
+        lat, kern = [], []
+        res = None
+        start = time.perf_counter()
+        for _ in range(args.steps):
+            s = time.perf_counter()
+            res = eng.run_batch(mols, opts)
+            lat.append((time.perf_counter() - s) * 1e3)
+            kern.append(eng.last_timings_ms())
+        torch.cuda.synchronize()
+        total = time.perf_counter() - start
+        return res, lat, kern, total
+
+    res64, lat64, kern64, total64 = run_mode("fp64")
+    print(f"[bench] fp64 batch: {np.mean(lat64):.1f} ms/step", file=sys.stderr, flush=True)
+    bad = [r.status for r in res64 if r.status != 0]
+    solves64 = args.molecules * args.steps / total64
+
+    # One molecule at a time (the plain FP64 GPU path the batched path is compared with).
+    single = mols[: args.single]
+    for m in single[:2]:
+        eng.run(m)
+    s = time.perf_counter()
+    for m in single:
+        eng.run(m)
+    single_rate = len(single) / (time.perf_counter() - s)
+
+    # Roofline of the dominant kernel: the SCF megakernel size class with the largest time.
+    cls_ms = [float(np.mean([k[4 + c] for k in kern64])) for c in range(4)]
+    dom = int(np.argmax(cls_ms))
+    flops = scf_flops(res64, None)
+    achieved_tf = flops[dom] / (cls_ms[dom] * 1e-3) / 1e12 if cls_ms[dom] > 0 else 0.0
+    roofline = {"bound": "mfma", "kernel": ["k_scf<256,64,LDS,LDS>", "k_scf<512,112,LDS,LDS>",
+                                            "k_scf<512,196,LDS,L2>", "k_scf<512,1024,L2,L2>"][dom],
+                "achieved": round(achieved_tf, 6), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved_tf / FP64_PEAK_TFLOPS, 8), "traffic": None,
+                "avg_launch_ms": round(cls_ms[dom], 4), "class_ms": [round(x, 4) for x in cls_ms]}
+
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle_binding as ob
+
+        threads = min(16, os.cpu_count() or 1)
+        sample = batch[: args.cpu_sample]
+        de = dh = dp = 0.0
+        s = time.perf_counter()
+        for i, a in enumerate(sample):
+            z, xyz = ob.atoms_to_arrays(a)
+            ref, dens, *_ = ob.run(z, xyz, threads=threads)
+            de = max(de, abs(ref.total_ev - res64[i].total_ev))
+            dh = max(dh, abs(ref.hof_kcal - res64[i].heat_of_formation_kcal))
+            dp = max(dp, float(np.max(np.abs(dens - res64[i].density))))
+        cpu_t = time.perf_counter() - s
+        cpu = {"value": round(len(sample) / cpu_t, 3), "unit": "solves/s", "cores": threads, "kind": "port",
+               "sample": f"first {len(sample)} molecules of the 256-molecule batch, CPU FP64 oracle, "
+                         f"{threads} threads per solve, {cpu_t:.1f}s"}
+        parity = {"sample": len(sample), "max_abs_dE_ev": de, "max_abs_dHf_kcal": dh, "max_abs_dP": dp}
+
+    ms_step = float(np.mean(lat64))
+    line = {
+        "metric": "batched SCF solves/s + p99 latency, mixed vs FP64 GPU at |ΔHf|≤1e-4 kcal/mol",
+        "value": round(solves64, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "256 closed-shell H/C/N/O molecules, 3-60 atoms, seeded random walk (seed 3), "
+                               "one batched solve per step", "n_molecules": args.molecules, "precision": "fp64"},
+        "latency_ms": {"p50": round(pct(lat64, 0.5), 3), "p99": round(pct(lat64, 0.99), 3)},
+        "fp64": {"solves_per_s": round(solves64, 3), "failed": len(bad)},
+        "mixed": None,
+        "single_molecule_fp64": {"solves_per_s": round(single_rate, 3), "molecules": len(single),
+                                 "batched_speedup": round(solves64 / single_rate, 3)},
+        "kernel_ms": {"integrals": round(float(np.mean([k[0] for k in kern64])), 4),
+                      "scf": round(float(np.mean([k[1] for k in kern64])), 4),
+                      "gather": round(float(np.mean([k[2] for k in kern64])), 4)},
+        "iterations": {"mean": round(float(np.mean([r.iterations for r in res64])), 2),
+                       "max": int(max(r.iterations for r in res64))},
+        "phase_ms_slowest": dict(zip(["fock", "energy_commutator", "diis", "eigensolve", "density", "total"],
+                                     [round(x, 3) for x in max(res64, key=lambda r: r.device_phase_ms[5]).device_phase_ms])),
+        "phase_ms_sum": dict(zip(["fock", "energy_commutator", "diis", "eigensolve", "density", "total"],
+                                 [round(sum(r.device_phase_ms[k] for r in res64), 2) for k in range(6)])),
+        "eig_split_ms_slowest": dict(zip(["warm_start_in", "sweeps", "sort_out"],
+                                         [round(x, 3) for x in max(res64, key=lambda r: r.device_phase_ms[5]).eig_phase_ms])),
+        "jacobi_sweeps_per_solve": round(float(np.mean([r.jacobi_sweeps / max(1, r.iterations) for r in res64])), 2),
+        "parity_vs_oracle": parity,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    eng.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

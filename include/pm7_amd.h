@@ -93,6 +93,10 @@ typedef struct {
   double dipole_sp_hybrid_debye[3];
   double dipole_pd_hybrid_debye[3];
   double dipole_magnitude;
+  double device_phase_ms[6]; /* time inside this molecule's SCF workgroup: Fock build, energy +
+                                commutator, DIIS extrapolation, eigensolve, density, whole SCF */
+  double eig_phase_ms[3];    /* eigensolve split: warm-start transform, Jacobi sweeps, sort and
+                                back-transform */
   double* density;           /* nao*nao, total density */
   double* spin_density;      /* nao*nao, P_alpha - P_beta, UHF only (else NULL) */
   double* mo_energies;       /* nao */
@@ -139,9 +143,11 @@ pm7_status pm7_fock_spin(pm7_ctx* ctx, const pm7_molecule_v1* mol, const double*
 /* Replaces core_core_energy (src/repulsion.rs:16-36), eV. */
 pm7_status pm7_core_core_energy(pm7_ctx* ctx, const pm7_molecule_v1* mol, double* e_out);
 
-/* Device timing of the last pm7_run / pm7_run_batch call, milliseconds:
- * [0] integrals+H_core, [1] SCF megakernels, [2] output gather, [3] whole call (host wall). */
-void pm7_last_timings(const pm7_ctx* ctx, double* ms4);
+/* Timing of the last pm7_run / pm7_run_batch call in milliseconds, measured with HIP events on
+ * the stream each kernel runs on: [0] integrals + H_core, [1] all SCF megakernels, [2] output
+ * gather, [3] whole call (host wall clock), [4..7] SCF megakernel of size class 0/1/2/3 (<= 64,
+ * <= 112, <= 196, <= 1024 basis functions; 0 when the class was empty). Writes min(n, 8) values. */
+void pm7_last_timings(const pm7_ctx* ctx, double* ms, int32_t n);
 
 #ifdef __cplusplus
 }

@@ -929,7 +929,7 @@ double frob(const Mat& a, const Mat& b) {  // src/linalg.rs:138-141 (in-order fo
 // the implicit-shift QL iteration (the EISPACK tred2/tql2 pair; Golub & Van Loan 8.3). Stands
 // in for faer's self_adjoint_eigen (src/linalg.rs:231-278): ascending eigenvalues, eigenvectors
 // as columns. Eigenvector signs are not pinned by the reference, only P and eps are.
-void eigh(int n, const Mat& a, std::vector<double>& w, Mat& vout) {
+void eigh_tql(int n, const Mat& a, std::vector<double>& w, Mat& vout) {
   std::vector<double> V(a), d(n), e(n);
   auto v = [&](int i, int j) -> double& { return V[(size_t)i * n + j]; };
   if (n == 1) {
@@ -1072,6 +1072,90 @@ void eigh(int n, const Mat& a, std::vector<double>& w, Mat& vout) {
     w[j] = d[idx[j]];
     for (int i = 0; i < n; ++i) vout[(size_t)i * n + j] = v(i, idx[j]);
   }
+}
+
+
+// Parallel-order cyclic Jacobi (round-robin pairing), the same rotation schedule as the engine's
+// device eigensolver. Selected with PM7_ORACLE_EIGEN=jacobi to study eigenvector-basis effects.
+void eigh_jacobi(int n, const Mat& a, std::vector<double>& w, Mat& vout) {
+  int n2 = n + (n & 1), m = n2 / 2;
+  std::vector<double> A((size_t)n2 * n2, 0.0), V((size_t)n2 * n2, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) A[(size_t)i * n2 + j] = a[(size_t)i * n + j];
+  for (int i = 0; i < n2; ++i) V[(size_t)i * n2 + i] = 1.0;
+  auto player = [n2](int pos, int r) { return pos == 0 ? 0 : 1 + (pos - 1 + r) % (n2 - 1); };
+  for (int sweep = 0; sweep < 40; ++sweep) {
+    double off = 0.0, dg = 0.0;
+    for (int i = 0; i < n2; ++i)
+      for (int j = 0; j < n2; ++j) (i == j ? dg : off) += A[(size_t)i * n2 + j] * A[(size_t)i * n2 + j];
+    if (off <= 1.0e-26 * (off + dg)) break;
+    for (int r = 0; r < n2 - 1; ++r) {
+      std::vector<double> c(m), s(m), t(m);
+      for (int k = 0; k < m; ++k) {
+        int p = player(k, r), q = player(n2 - 1 - k, r);
+        double apq = A[(size_t)p * n2 + q];
+        c[k] = 1.0; s[k] = 0.0; t[k] = 0.0;
+        if (apq != 0.0) {
+          double tau = (A[(size_t)q * n2 + q] - A[(size_t)p * n2 + p]) / (2.0 * apq);
+          t[k] = std::fabs(tau) > 1e150 ? 0.5 / tau : (tau >= 0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+          c[k] = 1.0 / std::sqrt(1.0 + t[k] * t[k]);
+          s[k] = t[k] * c[k];
+        }
+      }
+      std::vector<double> B(A);
+      for (int k = 0; k < m; ++k)
+        for (int l = 0; l < m; ++l) {
+          int pk = player(k, r), qk = player(n2 - 1 - k, r), pl = player(l, r), ql = player(n2 - 1 - l, r);
+          if (k == l) {
+            double apq = A[(size_t)pk * n2 + qk];
+            B[(size_t)pk * n2 + pk] = A[(size_t)pk * n2 + pk] - t[k] * apq;
+            B[(size_t)qk * n2 + qk] = A[(size_t)qk * n2 + qk] + t[k] * apq;
+            B[(size_t)pk * n2 + qk] = B[(size_t)qk * n2 + pk] = 0.0;
+            continue;
+          }
+          double x00 = A[(size_t)pk * n2 + pl], x01 = A[(size_t)pk * n2 + ql], x10 = A[(size_t)qk * n2 + pl], x11 = A[(size_t)qk * n2 + ql];
+          double y00 = c[k] * x00 - s[k] * x10, y01 = c[k] * x01 - s[k] * x11, y10 = s[k] * x00 + c[k] * x10, y11 = s[k] * x01 + c[k] * x11;
+          B[(size_t)pk * n2 + pl] = c[l] * y00 - s[l] * y01;
+          B[(size_t)pk * n2 + ql] = s[l] * y00 + c[l] * y01;
+          B[(size_t)qk * n2 + pl] = c[l] * y10 - s[l] * y11;
+          B[(size_t)qk * n2 + ql] = s[l] * y10 + c[l] * y11;
+        }
+      A.swap(B);
+      for (int i = 0; i < n2; ++i)
+        for (int k = 0; k < m; ++k) {
+          if (s[k] == 0.0) continue;
+          int pk = player(k, r), qk = player(n2 - 1 - k, r);
+          double vp = V[(size_t)i * n2 + pk], vq = V[(size_t)i * n2 + qk];
+          V[(size_t)i * n2 + pk] = c[k] * vp - s[k] * vq;
+          V[(size_t)i * n2 + qk] = s[k] * vp + c[k] * vq;
+        }
+    }
+  }
+  std::vector<int> perm(n);
+  for (int i = 0; i < n; ++i) {
+    int rank = 0;
+    double di = A[(size_t)i * n2 + i];
+    for (int j = 0; j < n; ++j) {
+      double dj = A[(size_t)j * n2 + j];
+      rank += (dj < di || (dj == di && j < i)) ? 1 : 0;
+    }
+    perm[rank] = i;
+  }
+  w.resize(n);
+  vout.assign((size_t)n * n, 0.0);
+  for (int k = 0; k < n; ++k) {
+    w[k] = A[(size_t)perm[k] * n2 + perm[k]];
+    for (int i = 0; i < n; ++i) vout[(size_t)i * n + k] = V[(size_t)i * n2 + perm[k]];
+  }
+}
+
+void eigh(int n, const Mat& a, std::vector<double>& w, Mat& vout) {
+  static const bool use_jacobi = [] {
+    const char* e = std::getenv("PM7_ORACLE_EIGEN");
+    return e && std::strcmp(e, "jacobi") == 0;
+  }();
+  if (use_jacobi) eigh_jacobi(n, a, w, vout);
+  else eigh_tql(n, a, w, vout);
 }
 
 // P = w * C_occ C_occ^T (src/scf.rs:1121-1141)

@@ -96,11 +96,12 @@ struct pm7_ctx {
   ParamSet params;
   int method = -1;
   std::vector<ElemDev> elem_host;
-  hipStream_t st[SCF_VARIANTS] = {nullptr, nullptr, nullptr};
-  hipEvent_t ev[8] = {};
+  hipStream_t st[SCF_VARIANTS] = {};
+  hipEvent_t ev[12] = {};
+  hipEvent_t ev_var[2 * SCF_VARIANTS] = {};
   DevBuf d_elem, d_pairpar, d_mols, d_list, d_atom_z, d_atom_ao, d_atom_pp, d_xyz, d_pairs, d_items, d_gath_off;
   DevBuf d_w, d_slab, d_vecs, d_out, d_ecore, d_ecore_mol, d_gath;
-  double timings[4] = {0, 0, 0, 0};
+  double timings[8] = {};
   std::mutex mu;
 };
 
@@ -228,7 +229,7 @@ struct Prepared {
   std::vector<MolDev> mols;
   int64_t n_pairs = 0, w_words = 0, slab_words = 0, vec_words = 0, gath_words = 0;
   int n_items = 0;
-  int list_off[SCF_VARIANTS + 1] = {0, 0, 0, 0};
+  int list_off[SCF_VARIANTS + 1] = {};
   BatchDev b;
 };
 
@@ -417,6 +418,8 @@ void finish_result(pm7_ctx* ctx, const Job& j, const MolOut& R, double e_core, c
   }
   out->heat_of_formation_kcal = (R.e_el + e_core - e_isol + eheat) * EV_TO_KCAL + j.corr_kcal;
   out->density_error = R.dp_err;
+  for (int k = 0; k < 6; ++k) out->device_phase_ms[k] = (double)R.ticks[k] * 1.0e-5;  // 100 MHz clock
+  for (int k = 0; k < 3; ++k) out->eig_phase_ms[k] = (double)R.ticks[PH_EIG_IN + k] * 1.0e-5;
   const size_t nn = (size_t)n * n;
   out->density = (double*)std::malloc(nn * sizeof(double));
   out->mo_coeff = (double*)std::malloc(nn * sizeof(double));
@@ -526,11 +529,16 @@ pm7_status run_jobs(pm7_ctx* ctx, std::vector<Job>& jobs, pm7_result_v1* results
   const int nm = (int)P.jobs.size();
   // SCF megakernels: one stream per size class, all after the integrals.
   for (int v = 1; v < SCF_VARIANTS; ++v) HIPCHK(hipStreamWaitEvent(ctx->st[v], ctx->ev[1], 0));
-  for (int v = 0; v < SCF_VARIANTS; ++v)
-    HIPCHK(launch_scf(v, P.b, P.list_off[v], P.list_off[v + 1] - P.list_off[v], ctx->st[v]));
+  int count[SCF_VARIANTS];
+  for (int v = 0; v < SCF_VARIANTS; ++v) {
+    count[v] = P.list_off[v + 1] - P.list_off[v];
+    HIPCHK(hipEventRecord(ctx->ev_var[2 * v], ctx->st[v]));
+    HIPCHK(launch_scf(v, P.b, P.list_off[v], count[v], ctx->st[v]));
+    HIPCHK(hipEventRecord(ctx->ev_var[2 * v + 1], ctx->st[v]));
+  }
   for (int v = 1; v < SCF_VARIANTS; ++v) {
-    HIPCHK(hipEventRecord(ctx->ev[2 + v], ctx->st[v]));
-    HIPCHK(hipStreamWaitEvent(ctx->st[0], ctx->ev[2 + v], 0));
+    HIPCHK(hipEventRecord(ctx->ev[6 + v], ctx->st[v]));
+    HIPCHK(hipStreamWaitEvent(ctx->st[0], ctx->ev[6 + v], 0));
   }
   HIPCHK(hipEventRecord(ctx->ev[2], ctx->st[0]));
   HIPCHK(launch_gather(P.b, nm, ctx->st[0]));
@@ -556,6 +564,11 @@ pm7_status run_jobs(pm7_ctx* ctx, std::vector<Job>& jobs, pm7_result_v1* results
   if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) == hipSuccess) ctx->timings[0] = ms;
   if (hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]) == hipSuccess) ctx->timings[1] = ms;
   if (hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[5]) == hipSuccess) ctx->timings[2] = ms;
+  for (int v = 0; v < SCF_VARIANTS; ++v) {
+    ctx->timings[4 + v] = 0.0;
+    if (count[v] > 0 && hipEventElapsedTime(&ms, ctx->ev_var[2 * v], ctx->ev_var[2 * v + 1]) == hipSuccess)
+      ctx->timings[4 + v] = ms;
+  }
   for (int mi = 0; mi < nm; ++mi) {
     const Job& j = P.jobs[mi];
     finish_result(ctx, j, outs[mi], ecore[mi], vecs.data() + j.vec0, gath.data() + j.gath0, &results[j.idx]);
@@ -608,6 +621,11 @@ pm7_status pm7_ctx_create(int32_t device, const char* data_dir, pm7_ctx** out) {
       delete ctx;
       return PM7_ERR_DEVICE;
     }
+  for (auto& e : ctx->ev_var)
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete ctx;
+      return PM7_ERR_DEVICE;
+    }
   pm7_status st = ensure_params(ctx, PM7_METHOD_PM7);
   if (st != PM7_OK) {
     pm7_ctx_destroy(ctx);
@@ -629,6 +647,8 @@ void pm7_ctx_destroy(pm7_ctx* ctx) {
   for (auto& s : ctx->st)
     if (s) (void)hipStreamDestroy(s);
   for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : ctx->ev_var)
     if (e) (void)hipEventDestroy(e);
   delete ctx;
 }
@@ -782,9 +802,9 @@ pm7_status pm7_core_core_energy(pm7_ctx* ctx, const pm7_molecule_v1* mol, double
   return PM7_OK;
 }
 
-void pm7_last_timings(const pm7_ctx* ctx, double* ms4) {
-  if (!ctx || !ms4) return;
-  for (int k = 0; k < 4; ++k) ms4[k] = ctx->timings[k];
+void pm7_last_timings(const pm7_ctx* ctx, double* ms, int32_t n) {
+  if (!ctx || !ms) return;
+  for (int k = 0; k < n && k < 8; ++k) ms[k] = ctx->timings[k];
 }
 
 }  // extern "C"

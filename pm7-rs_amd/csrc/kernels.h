@@ -32,10 +32,11 @@ struct BatchDev {
   double4 vpar;
 };
 
-// SCF kernel variants: 0 = <=64 AOs (256 threads, LDS Jacobi), 1 = <=140 AOs (512 threads, LDS
-// Jacobi), 2 = <=1024 AOs (512 threads, L2-resident Jacobi).
-constexpr int SCF_VARIANTS = 3;
-constexpr int VARIANT_MAX_NAO[SCF_VARIANTS] = {64, 140, 1024};
+// SCF kernel size classes (padded basis size): 0 <= 64 (256 threads, eigensolver fully in LDS),
+// 1 <= 112 (512 threads, fully in LDS), 2 <= 196 (packed matrix in LDS, rotations in L2),
+// 3 <= 1024 (eigensolver in L2/HBM).
+constexpr int SCF_VARIANTS = 4;
+constexpr int VARIANT_MAX_NAO[SCF_VARIANTS] = {64, 112, 196, 1024};
 
 hipError_t launch_integrals(const BatchDev& b, int64_t n_pairs, int n_items, int n_mols, hipStream_t s);
 hipError_t launch_scf(int variant, const BatchDev& b, int list_off, int count, hipStream_t s);

@@ -25,11 +25,13 @@ constexpr int NSLOT = 32;       // square matrices per molecule in the slab
 constexpr int HIST = 8;         // DIIS history depth
 
 // Slab slots, RHF.
-enum RhfSlot { S_H = 0, S_F = 1, S_W = 2, S_V = 5, S_A = 6, S_T = 7, S_HF = 8, S_HE = 16, S_HD = 24 };
+// C0/C1 double-buffer the sorted eigenvectors so each eigensolve can start from the last one;
+// VG and AG hold the Jacobi rotation accumulator and the packed matrix when they do not fit LDS.
+enum RhfSlot { S_H = 0, S_F = 1, S_W = 2, S_C0 = 3, S_C1 = 4, S_VG = 5, S_AG = 6, S_T = 7, S_HF = 8, S_HE = 16, S_HD = 24 };
 // Slab slots, UHF (no DIIS history, so the history region is reused).
 enum UhfSlot {
-  U_H = 0, U_FA = 1, U_FB = 2, U_PA = 3, U_PB = 4, U_PAN = 5, U_PBN = 6, U_VA = 7, U_VB = 8,
-  U_A = 9, U_T = 10, U_WA = 11, U_WB = 12, U_PT = 13, U_CA = 14, U_CB = 15
+  U_H = 0, U_FA = 1, U_FB = 2, U_PA = 3, U_PB = 4, U_PAN = 5, U_PBN = 6, U_CA0 = 7, U_CB0 = 8,
+  U_AG = 9, U_T = 10, U_CA1 = 11, U_CB1 = 12, U_PT = 13, U_VG = 14
 };
 
 struct ElemDev {
@@ -58,9 +60,18 @@ struct MolDev {
   double e_tol, p_tol, adiis_switch, shift_floor, sad_yy, mixed_switch;
 };
 
+// Phases of an SCF iteration timed inside the megakernel (device wall clock, 100 MHz ticks).
+// PH_EIG is split into its warm-start transform (EIG_IN), the Jacobi sweeps (EIG_SWEEP) and the
+// sort + back-transform (EIG_OUT).
+enum Phase {
+  PH_FOCK = 0, PH_COMM = 1, PH_DIIS = 2, PH_EIG = 3, PH_DENS = 4, PH_TOTAL = 5,
+  PH_EIG_IN = 6, PH_EIG_SWEEP = 7, PH_EIG_OUT = 8, NPHASE = 9
+};
+
 struct MolOut {
   double e_el, dp_err, err_norm, shift, e_fp32_last;
   int iterations, converged, sweeps, p_slot, c_slot, cb_slot, pb_slot, fp32_iters;
+  unsigned long long ticks[NPHASE];
 };
 
 __host__ __device__ inline int npair_of(int norb) { return norb * (norb + 1) / 2; }

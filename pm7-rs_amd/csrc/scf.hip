@@ -175,29 +175,40 @@ __device__ void fock_build(const MolCtx& c, const double* __restrict__ Pt, const
     const int nf = ef.n_orb, ns = es.n_orb, nps = npair_of(ns);
     const int of = c.atom_ao[P.f], os = c.atom_ao[P.s];
     const double* wp = c.w + P.w_off;
+    // Same-spin density block held in registers: every index below is a compile-time constant.
     double Xa[4][4], Xb[4][4];
-    for (int i = 0; i < nf; ++i)
-      for (int j = 0; j < ns; ++j) {
-        Xa[i][j] = Psa[(int64_t)(of + i) * ld + os + j];
-        Xb[i][j] = Fb ? Psb[(int64_t)(of + i) * ld + os + j] : 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const bool in = i < nf && j < ns;
+        Xa[i][j] = in ? Psa[(int64_t)(of + i) * ld + os + j] : 0.0;
+        Xb[i][j] = (in && Fb) ? Psb[(int64_t)(of + i) * ld + os + j] : 0.0;
       }
-    for (int mu = 0; mu < nf; ++mu)
-      for (int lam = 0; lam < ns; ++lam) {
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu)
+#pragma unroll
+      for (int lam = 0; lam < 4; ++lam) {
+        if (mu >= nf || lam >= ns) continue;
         double acca = 0.0, accb = 0.0;
-        for (int nu = 0; nu < nf; ++nu) {
+#pragma unroll
+        for (int nu = 0; nu < 4; ++nu) {
+          if (nu >= nf) continue;
           const double* row = wp + (int64_t)pack_idx(mu, nu) * nps;
-          for (int sg = 0; sg < ns; ++sg) {
-            double wv = row[pack_idx(lam, sg)];
+#pragma unroll
+          for (int sg = 0; sg < 4; ++sg) {
+            if (sg >= ns) continue;
+            const double wv = row[pack_idx(lam, sg)];
             acca += Xa[nu][sg] * wv;
             accb += Xb[nu][sg] * wv;
           }
         }
         const int64_t i1 = (int64_t)(of + mu) * ld + os + lam, i2 = (int64_t)(os + lam) * ld + of + mu;
-        double fa = H[i1] - xa * acca;
+        const double fa = H[i1] - xa * acca;
         Fa[i1] = fa;
         Fa[i2] = fa;
         if (Fb) {
-          double fb = H[i1] - xb * accb;
+          const double fb = H[i1] - xb * accb;
           Fb[i1] = fb;
           Fb[i2] = fb;
         }
@@ -303,29 +314,40 @@ __device__ double density(int n, int ld, int nocc, double wgt, const double* __r
 
 __device__ inline int rr_player(int pos, int r, int n2) { return pos == 0 ? 0 : 1 + (pos - 1 + r) % (n2 - 1); }
 
-// Parallel cyclic Jacobi eigensolve of the symmetric ld x ld matrix Win (pad rows/cols zero).
-// Writes ascending eigenvalues eps[0..n) and the matching eigenvectors as the columns of C.
-// A lives in LDS when LDSN > 0, otherwise in the global workspace Ag. Returns the sweep count.
-template <int NT, int LDSN, int MAXN>
-__device__ int jacobi(int n, int ld, const double* __restrict__ Win, double* Ag, double* __restrict__ V,
-                      double* __restrict__ C, double* __restrict__ eps, double* s_a, double* s_c, double* s_s,
-                      double* s_t, int* s_perm, double* red) {
-  const int n2 = ld, m = ld / 2;
-  double* A = (LDSN > 0) ? s_a : Ag;
-  for (int idx = threadIdx.x; idx < n2 * n2; idx += NT) {
-    int i = idx / n2, j = idx - i * n2;
-    A[idx] = Win[idx];
-    V[idx] = (i == j) ? 1.0 : 0.0;
-  }
-  __syncthreads();
+// Packed upper-triangular index of the symmetric element (i, j).
+__device__ inline int pidx(int i, int j) {
+  const int a = i < j ? i : j, b = i < j ? j : i;
+  return b * (b + 1) / 2 + a;
+}
+
+// Row of the packed upper-triangular position t (t = j(j+1)/2 + i, i <= j).
+__device__ inline int tri_row(int t) {
+  int j = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+  while (j * (j + 1) / 2 > t) --j;
+  while ((j + 1) * (j + 2) / 2 <= t) ++j;
+  return j;
+}
+
+// Parallel cyclic Jacobi on the packed symmetric n2 x n2 matrix A, accumulating the rotations
+// into V (row-major, stride n2). Round-robin pairing gives n2/2 disjoint rotations per round;
+// the 2x2 block (pair k rows, pair l columns) takes both its row and column rotation in one
+// update, so a round is one pass over the packed matrix. Returns the number of sweeps.
+template <int NT>
+__device__ int jacobi_core(int n2, double* A, double* V, double* s_c, double* s_s, double* s_t, int* s_p, int* s_q,
+                           double* red) {
+  constexpr int NX = 32, NY = NT / NX;  // 2-D thread grid: compile-time divisors only
+  const int tx = threadIdx.x % NX, ty = threadIdx.x / NX;
+  const int m = n2 / 2;
   int sweep = 0;
   for (; sweep < 40; ++sweep) {
     double sums[2] = {0.0, 0.0};
-    for (int idx = threadIdx.x; idx < n2 * n2; idx += NT) {
-      int i = idx / n2, j = idx - i * n2;
-      double v = A[idx];
-      if (i == j) sums[1] += v * v;
-      else sums[0] += v * v;
+    for (int j = ty; j < n2; j += NY) {
+      const double* rowj = A + j * (j + 1) / 2;
+      for (int i = tx; i <= j; i += NX) {
+        const double v = rowj[i];
+        if (i == j) sums[1] += v * v;
+        else sums[0] += 2.0 * v * v;
+      }
     }
     wg_sum_n<NT, 2>(sums, red);
     if (sums[0] <= 1.0e-26 * (sums[0] + sums[1])) break;
@@ -333,72 +355,193 @@ __device__ int jacobi(int n, int ld, const double* __restrict__ Win, double* Ag,
       if (threadIdx.x < m) {
         const int k = threadIdx.x;
         const int p = rr_player(k, r, n2), q = rr_player(n2 - 1 - k, r, n2);
-        const double apq = A[p * n2 + q];
+        const double apq = A[pidx(p, q)];
         double cc = 1.0, ss = 0.0, tt = 0.0;
         if (apq != 0.0) {
-          double tau = (A[q * n2 + q] - A[p * n2 + p]) / (2.0 * apq);
+          const double tau = (A[pidx(q, q)] - A[pidx(p, p)]) / (2.0 * apq);
           if (fabs(tau) > 1.0e150) tt = 0.5 / tau;
           else tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
           cc = 1.0 / sqrt(1.0 + tt * tt);
           ss = tt * cc;
         }
+        s_p[k] = p;
+        s_q[k] = q;
         s_c[k] = cc;
         s_s[k] = ss;
         s_t[k] = tt;
       }
       __syncthreads();
-      for (int b = threadIdx.x; b < m * m; b += NT) {
-        const int k = b / m, l = b - k * m;
-        const int pk = rr_player(k, r, n2), qk = rr_player(n2 - 1 - k, r, n2);
-        if (k == l) {
-          const double apq = A[pk * n2 + qk], t = s_t[k];
-          A[pk * n2 + pk] -= t * apq;
-          A[qk * n2 + qk] += t * apq;
-          A[pk * n2 + qk] = 0.0;
-          A[qk * n2 + pk] = 0.0;
-          continue;
+      // 2x2 blocks (pair k rows, pair l columns), k <= l, each unordered block once.
+      for (int k = ty; k < m; k += NY) {
+        const int pk = s_p[k], qk = s_q[k];
+        const double ck = s_c[k], sk = s_s[k];
+        for (int l = k + tx; l < m; l += NX) {
+          if (l == k) {
+            const double tk = s_t[k];
+            if (tk == 0.0) continue;
+            const int ipq = pidx(pk, qk);
+            const double apq = A[ipq];
+            A[pidx(pk, pk)] -= tk * apq;
+            A[pidx(qk, qk)] += tk * apq;
+            A[ipq] = 0.0;
+            continue;
+          }
+          const double cl = s_c[l], sl = s_s[l];
+          if (sk == 0.0 && sl == 0.0) continue;
+          const int pl = s_p[l], ql = s_q[l];
+          const int i00 = pidx(pk, pl), i01 = pidx(pk, ql), i10 = pidx(qk, pl), i11 = pidx(qk, ql);
+          const double x00 = A[i00], x01 = A[i01], x10 = A[i10], x11 = A[i11];
+          const double y00 = ck * x00 - sk * x10, y01 = ck * x01 - sk * x11;
+          const double y10 = sk * x00 + ck * x10, y11 = sk * x01 + ck * x11;
+          A[i00] = cl * y00 - sl * y01;
+          A[i01] = sl * y00 + cl * y01;
+          A[i10] = cl * y10 - sl * y11;
+          A[i11] = sl * y10 + cl * y11;
         }
-        const int pl = rr_player(l, r, n2), ql = rr_player(n2 - 1 - l, r, n2);
-        const double ck = s_c[k], sk = s_s[k], cl = s_c[l], sl = s_s[l];
-        const double x00 = A[pk * n2 + pl], x01 = A[pk * n2 + ql], x10 = A[qk * n2 + pl], x11 = A[qk * n2 + ql];
-        const double y00 = ck * x00 - sk * x10, y01 = ck * x01 - sk * x11;
-        const double y10 = sk * x00 + ck * x10, y11 = sk * x01 + ck * x11;
-        A[pk * n2 + pl] = cl * y00 - sl * y01;
-        A[pk * n2 + ql] = sl * y00 + cl * y01;
-        A[qk * n2 + pl] = cl * y10 - sl * y11;
-        A[qk * n2 + ql] = sl * y10 + cl * y11;
       }
-      for (int it = threadIdx.x; it < n2 * m; it += NT) {
-        const int i = it / m, k = it - i * m;
+      // Rotation accumulator: rows over ty, column pairs over tx. Each thread issues the loads of
+      // four rows before any store so the memory requests overlap instead of chaining.
+      for (int k = tx; k < m; k += NX) {
         const double ss = s_s[k];
         if (ss == 0.0) continue;
-        const int pk = rr_player(k, r, n2), qk = rr_player(n2 - 1 - k, r, n2);
+        const int pk = s_p[k], qk = s_q[k];
         const double cc = s_c[k];
-        const double vp = V[(int64_t)i * n2 + pk], vq = V[(int64_t)i * n2 + qk];
-        V[(int64_t)i * n2 + pk] = cc * vp - ss * vq;
-        V[(int64_t)i * n2 + qk] = ss * vp + cc * vq;
+        for (int i0 = ty; i0 < n2; i0 += 4 * NY) {
+          double vp[4], vq[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * NY;
+            vp[u] = i < n2 ? V[i * n2 + pk] : 0.0;
+            vq[u] = i < n2 ? V[i * n2 + qk] : 0.0;
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int i = i0 + u * NY;
+            if (i < n2) {
+              V[i * n2 + pk] = cc * vp[u] - ss * vq[u];
+              V[i * n2 + qk] = ss * vp[u] + cc * vq[u];
+            }
+          }
+        }
       }
       __syncthreads();
     }
   }
-  // Ascending order of the real eigenvalues (ties broken by index).
+  return sweep;
+}
+
+// Eigenpairs of the symmetric ld x ld matrix W (pad rows/columns zero): ascending eigenvalues
+// eps[0..n) and eigenvectors as the columns of Cout. With Cprev (the previous SCF iteration's
+// eigenvectors) the solve is warm-started: Jacobi diagonalises Cprev^T W Cprev, which is nearly
+// diagonal once the SCF settles, and Cout = Cprev V. A and V sit in LDS when the variant allows.
+template <int NT, bool A_LDS, bool V_LDS>
+__device__ int eigensolve(int n, int ld, const double* __restrict__ W, const double* __restrict__ Cprev,
+                          double* __restrict__ Cout, double* __restrict__ eps, double* Ag, double* Vg,
+                          double* __restrict__ T, double* s_a, double* s_v, double* s_c, double* s_s, double* s_t,
+                          int* s_perm, int* s_p, int* s_q, double* red, unsigned long long* tsub) {
+  const int n2 = ld, nt = ld / 2;
+  unsigned long long t0 = wall_clock64();
+  double* A = A_LDS ? s_a : Ag;
+  double* V = V_LDS ? s_v : Vg;
+  if (Cprev) {
+    // T = W Cprev
+    for (int t = threadIdx.x; t < nt * nt; t += NT) {
+      const int I = t / nt, J = t - I * nt, i0 = 2 * I, j0 = 2 * J;
+      const double* w0 = W + (int64_t)i0 * ld;
+      const double* w1 = w0 + ld;
+      double s00 = 0.0, s01 = 0.0, s10 = 0.0, s11 = 0.0;
+      for (int k = 0; k < n; ++k) {
+        const double a0 = w0[k], a1 = w1[k];
+        const double b0 = Cprev[(int64_t)k * ld + j0], b1 = Cprev[(int64_t)k * ld + j0 + 1];
+        s00 += a0 * b0;
+        s01 += a0 * b1;
+        s10 += a1 * b0;
+        s11 += a1 * b1;
+      }
+      T[(int64_t)i0 * ld + j0] = s00;
+      T[(int64_t)i0 * ld + j0 + 1] = s01;
+      T[(int64_t)(i0 + 1) * ld + j0] = s10;
+      T[(int64_t)(i0 + 1) * ld + j0 + 1] = s11;
+    }
+    __syncthreads();
+    // A = Cprev^T T, upper triangle only.
+    for (int t = threadIdx.x; t < nt * nt; t += NT) {
+      const int I = t / nt, J = t - I * nt;
+      if (I > J) continue;
+      const int i0 = 2 * I, j0 = 2 * J;
+      double s00 = 0.0, s01 = 0.0, s10 = 0.0, s11 = 0.0;
+      for (int k = 0; k < n; ++k) {
+        const double* c = Cprev + (int64_t)k * ld;
+        const double* tr = T + (int64_t)k * ld;
+        const double a0 = c[i0], a1 = c[i0 + 1], b0 = tr[j0], b1 = tr[j0 + 1];
+        s00 += a0 * b0;
+        s01 += a0 * b1;
+        s10 += a1 * b0;
+        s11 += a1 * b1;
+      }
+      A[pidx(i0, j0)] = s00;
+      A[pidx(i0, j0 + 1)] = s01;
+      if (I != J) A[pidx(i0 + 1, j0)] = s10;
+      A[pidx(i0 + 1, j0 + 1)] = s11;
+    }
+  } else {
+    constexpr int NX = 32, NY = NT / NX;
+    for (int j = threadIdx.x / NX; j < n2; j += NY)
+      for (int i = threadIdx.x % NX; i <= j; i += NX) A[j * (j + 1) / 2 + i] = W[(int64_t)i * ld + j];
+  }
+  for (int idx = threadIdx.x; idx < n2 * n2; idx += NT) {
+    const int i = idx / n2, j = idx - i * n2;
+    V[idx] = (i == j) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  unsigned long long t1 = wall_clock64();
+  const int sweeps = jacobi_core<NT>(n2, A, V, s_c, s_s, s_t, s_p, s_q, red);
+  unsigned long long t2 = wall_clock64();
   for (int i = threadIdx.x; i < n; i += NT) {
-    const double di = A[i * n2 + i];
+    const double di = A[pidx(i, i)];
     int rank = 0;
     for (int j = 0; j < n; ++j) {
-      const double dj = A[j * n2 + j];
+      const double dj = A[pidx(j, j)];
       rank += (dj < di || (dj == di && j < i)) ? 1 : 0;
     }
     s_perm[rank] = i;
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < n; k += NT) eps[k] = A[s_perm[k] * n2 + s_perm[k]];
-  for (int idx = threadIdx.x; idx < n2 * n2; idx += NT) {
-    const int i = idx / n2, k = idx - i * n2;
-    C[idx] = (i < n && k < n) ? V[(int64_t)i * n2 + s_perm[k]] : 0.0;
+  for (int k = threadIdx.x; k < n; k += NT) eps[k] = A[pidx(s_perm[k], s_perm[k])];
+  if (Cprev) {
+    // Cout = Cprev V(:, perm)
+    for (int t = threadIdx.x; t < nt * nt; t += NT) {
+      const int I = t / nt, J = t - I * nt, i0 = 2 * I, k0 = 2 * J;
+      const int c0 = k0 < n ? s_perm[k0] : -1, c1 = k0 + 1 < n ? s_perm[k0 + 1] : -1;
+      const double* a0p = Cprev + (int64_t)i0 * ld;
+      const double* a1p = a0p + ld;
+      double s00 = 0.0, s01 = 0.0, s10 = 0.0, s11 = 0.0;
+      for (int j = 0; j < n; ++j) {
+        const double a0 = a0p[j], a1 = a1p[j];
+        const double b0 = c0 >= 0 ? V[j * n2 + c0] : 0.0, b1 = c1 >= 0 ? V[j * n2 + c1] : 0.0;
+        s00 += a0 * b0;
+        s01 += a0 * b1;
+        s10 += a1 * b0;
+        s11 += a1 * b1;
+      }
+      Cout[(int64_t)i0 * ld + k0] = s00;
+      Cout[(int64_t)i0 * ld + k0 + 1] = s01;
+      Cout[(int64_t)(i0 + 1) * ld + k0] = s10;
+      Cout[(int64_t)(i0 + 1) * ld + k0 + 1] = s11;
+    }
+  } else {
+    for (int idx = threadIdx.x; idx < n2 * n2; idx += NT) {
+      const int i = idx / n2, k = idx - i * n2;
+      Cout[idx] = (i < n && k < n) ? V[i * n2 + s_perm[k]] : 0.0;
+    }
   }
   __syncthreads();
-  return sweep;
+  if (threadIdx.x == 0) {
+    tsub[0] += t1 - t0;
+    tsub[1] += t2 - t1;
+    tsub[2] += wall_clock64() - t2;
+  }
+  return sweeps;
 }
 
 // ---- small serial solvers (thread 0) -------------------------------------------------------
@@ -440,16 +583,19 @@ __device__ bool solve_small(double* m, double* rhs, int n, double* x) {
   return true;
 }
 
-// CDIIS over the chronological window (pm7-rs src/scf.rs:1684-1771): returns the number of
+// CDIIS over the chronological window (pm7-rs src/scf.rs:1684-1771), run by one thread with its
+// <= 9x9 bordered system in the LDS scratch `wk` (b: 81, rhs: 9, x: 9). Returns the number of
 // coefficients (0 on failure); coefficient i belongs to chronological entry skip + i.
-__device__ int cdiis(int cnt, const int* slot, const double* G, double* coef, int* skip_out) {
+__device__ int cdiis(int cnt, const int* slot, const double* G, double* coef, int* skip_out, double* wk) {
   if (cnt < 2) return 0;
+  double* b = wk;
+  double* rhs = wk + (HIST + 1) * (HIST + 1);
+  double* x = rhs + HIST + 1;
   for (int skip = 0; skip < cnt - 1; ++skip) {
     const int m = cnt - skip, dim = m + 1;
     double scale = 0.0;
     for (int i = 0; i < m; ++i) scale = fmax(scale, fabs(G[slot[skip + i] * HIST + slot[skip + i]]));
     if (!isfinite(scale) || scale <= 0.0) continue;
-    double b[(HIST + 1) * (HIST + 1)], rhs[HIST + 1], x[HIST + 1];
     for (int i = 0; i < dim * dim; ++i) b[i] = 0.0;
     for (int i = 0; i < m; ++i) {
       for (int j = 0; j < m; ++j) b[i * dim + j] = G[slot[skip + i] * HIST + slot[skip + j]] / scale;
@@ -473,67 +619,75 @@ __device__ int cdiis(int cnt, const int* slot, const double* G, double* coef, in
   return 0;
 }
 
-__device__ void simplex_project(const double* v, int n, double* out) {
-  double u[HIST];
-  for (int i = 0; i < n; ++i) u[i] = v[i];
-  for (int i = 1; i < n; ++i) {  // insertion sort, descending
-    double key = u[i];
-    int j = i - 1;
-    while (j >= 0 && u[j] < key) {
-      u[j + 1] = u[j];
-      --j;
-    }
-    u[j + 1] = key;
-  }
-  double css = 0.0, theta = 0.0;
-  for (int j = 0; j < n; ++j) {
-    css += u[j];
-    double t = (css - 1.0) / (double)(j + 1);
-    if (u[j] - t > 0.0) theta = t;
-  }
-  for (int i = 0; i < n; ++i) out[i] = fmax(v[i] - theta, 0.0);
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
 }
 
-// A-DIIS on the simplex (pm7-rs src/scf.rs:1812-1916) from M[s][t] = <D_s, F_t>.
-__device__ int adiis(int cnt, const int* slot, const double* Mm, double* coef) {
-  if (cnt < 2) return 0;
-  const int n = cnt, sn = slot[n - 1];
-  double d[HIST], s[HIST][HIST];
+// A-DIIS on the simplex (pm7-rs src/scf.rs:1812-1916) from M[s][t] = <D_s, F_t>, run by one full
+// wavefront: lane i owns coefficient i, its gradient row and its rank in the simplex projection,
+// so the 400-step projected-gradient loop needs no memory traffic. Writes `cnt` coefficients.
+__device__ void adiis_wave(int cnt, const int* slot, const double* Mm, double* coef) {
+  const int lane = threadIdx.x & 63;
+  const int n = cnt;
+  const bool act = lane < n;
+  const int sn = slot[n - 1];
   const double mnn = Mm[sn * HIST + sn];
-  for (int i = 0; i < n; ++i) {
-    const int si = slot[i];
-    d[i] = Mm[si * HIST + sn] - mnn;
-    for (int j = 0; j < n; ++j) {
-      const int sj = slot[j];
-      s[i][j] = Mm[si * HIST + sj] - Mm[si * HIST + sn] - Mm[sn * HIST + sj] + mnn;
+  double d = 0.0;
+  double srow[HIST];  // s[lane][j] + s[j][lane]
+#pragma unroll
+  for (int j = 0; j < HIST; ++j) srow[j] = 0.0;
+  if (act) {
+    const int si = slot[lane];
+    d = Mm[si * HIST + sn] - mnn;
+#pragma unroll
+    for (int j = 0; j < HIST; ++j) {
+      if (j < n) {
+        const int sj = slot[j];
+        const double sij = Mm[si * HIST + sj] - Mm[si * HIST + sn] - Mm[sn * HIST + sj] + mnn;
+        const double sji = Mm[sj * HIST + si] - Mm[sj * HIST + sn] - Mm[sn * HIST + si] + mnn;
+        srow[j] = sij + sji;
+      }
     }
   }
-  double l = 1.0e-12;
-  for (int i = 0; i < n; ++i) {
-    double row = 0.0;
-    for (int j = 0; j < n; ++j) row += fabs(s[i][j] + s[j][i]);
-    l = fmax(l, row);
-  }
-  const double lr = 1.0 / l;
-  double c[HIST], g[HIST], trial[HIST], proj[HIST];
-  for (int i = 0; i < n; ++i) c[i] = (i == n - 1) ? 1.0 : 0.0;
+  double rowabs = 0.0;
+#pragma unroll
+  for (int j = 0; j < HIST; ++j) rowabs += fabs(srow[j]);
+  const double lr = 1.0 / fmax(1.0e-12, wave_max(act ? rowabs : 0.0));
+  double c = (act && lane == n - 1) ? 1.0 : 0.0;
   for (int it = 0; it < 400; ++it) {
-    for (int k = 0; k < n; ++k) {
-      double acc = 2.0 * d[k];
-      for (int j = 0; j < n; ++j) acc += (s[k][j] + s[j][k]) * c[j];
-      g[k] = acc;
+    double g = 2.0 * d;
+#pragma unroll
+    for (int j = 0; j < HIST; ++j) g += srow[j] * __shfl(c, j, 64);
+    const double trial = act ? c - lr * g : -INFINITY;
+    // Descending rank among the active lanes (ties by lane) and the inclusive prefix sum.
+    int rank = 0;
+    double css = trial;
+#pragma unroll
+    for (int j = 0; j < HIST; ++j) {
+      const double tj = __shfl(trial, j, 64);
+      const bool before = (j < n) && (tj > trial || (tj == trial && j < lane));
+      rank += before ? 1 : 0;
+      css += before ? tj : 0.0;
     }
-    for (int i = 0; i < n; ++i) trial[i] = c[i] - lr * g[i];
-    simplex_project(trial, n, proj);
-    double delta = 0.0;
-    for (int i = 0; i < n; ++i) {
-      delta += fabs(proj[i] - c[i]);
-      c[i] = proj[i];
-    }
+    const double t = (css - 1.0) / (double)(rank + 1);
+    const bool cond = act && (trial - t > 0.0);
+    const double best = wave_max(cond ? (double)rank : -1.0);
+    const unsigned long long win = __ballot(cond && (double)rank == best);
+    const int wl = win ? (__ffsll((long long)win) - 1) : 0;
+    const double theta = win ? __shfl(t, wl, 64) : 0.0;
+    const double proj = act ? fmax(trial - theta, 0.0) : 0.0;
+    const double delta = wave_sum(act ? fabs(proj - c) : 0.0);
+    c = proj;
     if (delta < 1.0e-12) break;
   }
-  for (int i = 0; i < n; ++i) coef[i] = c[i];
-  return n;
+  if (act) coef[lane] = c;
 }
 
 // Level-shift stall guard (pm7-rs src/scf.rs:1581-1656); state lives in LDS.
@@ -611,18 +765,23 @@ __device__ void copy_mat(int ld, const double* src, double* dst) {
   __syncthreads();
 }
 
-template <int NT, int LDSN, int MAXN>
+// MAXN: largest padded basis of the size class; A_LDS / V_LDS: packed Fock matrix / rotation
+// accumulator of the eigensolver resident in LDS (otherwise in the molecule's slab).
+template <int NT, int MAXN, bool A_LDS, bool V_LDS>
 __global__ void __launch_bounds__(NT)
     k_scf(const int* __restrict__ mol_list, const MolDev* __restrict__ mols, const int* __restrict__ atom_z,
           const int* __restrict__ atom_ao, const int* __restrict__ atom_pp, const PairDev* __restrict__ pairs,
           const ElemDev* __restrict__ elem, const double* __restrict__ w, double* __restrict__ slab,
           double* __restrict__ vecs, MolOut* __restrict__ out) {
-  __shared__ double s_a[LDSN > 0 ? LDSN * LDSN : 1];
+  __shared__ double s_a[A_LDS ? MAXN * (MAXN + 1) / 2 : 1];
+  __shared__ double s_v[V_LDS ? MAXN * MAXN : 1];
   __shared__ double s_red[(Wg<NT>::NW + 1) * 3 * HIST];
   __shared__ double s_c[MAXN / 2], s_s[MAXN / 2], s_t[MAXN / 2];
   __shared__ int s_perm[MAXN];
+  __shared__ int s_pq[MAXN];  // this round's rotation pairs (p then q)
   __shared__ double s_G[HIST * HIST], s_M[HIST * HIST], s_coef[HIST + 1];
   __shared__ int s_slot[HIST], s_cslot[HIST];
+  __shared__ double s_wk[(HIST + 1) * (HIST + 3)];  // bordered DIIS system
   __shared__ int s_int[8];     // 0 head, 1 cnt, 2 ncoef, 3 done
   __shared__ double s_dbl[8];  // 0 e_old, 1 err, 2 dp
   __shared__ Guard s_guard;
@@ -655,6 +814,15 @@ __global__ void __launch_bounds__(NT)
   __syncthreads();
   double* H = c.slot(S_H);
   MolOut res;
+  unsigned long long tk[NPHASE] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  unsigned long long t_mark = wall_clock64();
+  const unsigned long long t_start = t_mark;
+  // Attribute the time since the last mark to phase `ph` (thread 0 keeps the books).
+  auto lap = [&](int ph) {
+    const unsigned long long now = wall_clock64();
+    tk[ph] += now - t_mark;
+    t_mark = now;
+  };
   res.converged = 0;
   res.iterations = 0;
   res.dp_err = INFINITY;
@@ -666,6 +834,7 @@ __global__ void __launch_bounds__(NT)
     // ------------------------------------------------------------------ RHF
     sad_guess<NT>(c, c.slot(S_HD), 1.0);
     const int nocc = c.M.n_occ;
+    int cbuf = 0;  // eigenvector buffer the next solve writes
     int iter = 0;
     for (; iter < c.M.max_scf; ++iter) {
       const int h = s_int[0];
@@ -673,6 +842,7 @@ __global__ void __launch_bounds__(NT)
       double* F = c.slot(S_HF + h);
       double* E = c.slot(S_HE + h);
       fock_build<NT>(c, P, P, 0.5, nullptr, 0.0, F, nullptr);
+      lap(PH_FOCK);
       double eh[2] = {0.0, 0.0};
       for (int idx = threadIdx.x; idx < n * n; idx += NT) {
         int i = idx / n, j = idx - i * n;
@@ -683,6 +853,7 @@ __global__ void __launch_bounds__(NT)
       wg_sum_n<NT, 2>(eh, s_red);
       const double e = 0.5 * (eh[0] + eh[1]);
       const double err = sqrt(commutator<NT>(n, ld, F, P, E, s_red));
+      lap(PH_COMM);
       if (threadIdx.x == 0) {
         if (s_int[1] < HIST) s_int[1]++;
         s_int[0] = (h + 1) % HIST;
@@ -719,17 +890,20 @@ __global__ void __launch_bounds__(NT)
             s_M[t * HIST + h] = acc[3 * q + 2];
           }
           int ncoef = 0, skip = 0;
-          if (c.M.use_diis && c.M.accel != 0) {
-            if (c.M.accel == 1 || err <= c.M.adiis_switch) {
-              ncoef = cdiis(cnt, s_slot, s_G, s_coef, &skip);
-            } else {
-              ncoef = adiis(cnt, s_slot, s_M, s_coef);
-              skip = 0;
-            }
-          }
+          if (c.M.use_diis && c.M.accel != 0 && (c.M.accel == 1 || err <= c.M.adiis_switch))
+            ncoef = cdiis(cnt, s_slot, s_G, s_coef, &skip, s_wk);
           for (int i = 0; i < ncoef; ++i) s_cslot[i] = s_slot[skip + i];
           s_int[2] = ncoef;
           s_dbl[1] = err;
+        }
+        __syncthreads();
+        const bool use_adiis = c.M.use_diis && c.M.accel == 2 && err > c.M.adiis_switch && cnt >= 2;
+        if (use_adiis && threadIdx.x < 64) {
+          adiis_wave(cnt, s_slot, s_M, s_coef);
+          if (threadIdx.x == 0) {
+            for (int i = 0; i < cnt; ++i) s_cslot[i] = s_slot[i];
+            s_int[2] = cnt;
+          }
         }
         __syncthreads();
       }
@@ -748,10 +922,16 @@ __global__ void __launch_bounds__(NT)
       }
       __syncthreads();
       level_shift<NT>(n, ld, W, P, 2.0, s_guard.shift);
-      sweeps += jacobi<NT, LDSN, MAXN>(n, ld, W, c.slot(S_A), c.slot(S_T), c.slot(S_V), eps_a, s_a, s_c, s_s, s_t,
-                                       s_perm, s_red);
+      lap(PH_DIIS);
+      double* Cnew = c.slot(cbuf ? S_C1 : S_C0);
+      sweeps += eigensolve<NT, A_LDS, V_LDS>(n, ld, W, iter > 0 ? c.slot(cbuf ? S_C0 : S_C1) : nullptr, Cnew, eps_a,
+                                             c.slot(S_AG), c.slot(S_VG), c.slot(S_T), s_a, s_v, s_c, s_s, s_t, s_perm,
+                                             s_pq, s_pq + MAXN / 2, s_red, tk + PH_EIG_IN);
+      lap(PH_EIG);
       double* Pn = c.slot(S_HD + s_int[0]);
-      const double dsq = density<NT>(n, ld, nocc, 2.0, c.slot(S_V), P, Pn, s_red);
+      const double dsq = density<NT>(n, ld, nocc, 2.0, Cnew, P, Pn, s_red);
+      lap(PH_DENS);
+      cbuf ^= 1;
       if (threadIdx.x == 0) {
         const double dp = sqrt(dsq / (double)((int64_t)n * n));
         const double de = fabs(e - s_dbl[0]);
@@ -788,11 +968,14 @@ __global__ void __launch_bounds__(NT)
     }
     wg_sum_n<NT, 2>(eh, s_red);
     res.e_el = 0.5 * (eh[0] + eh[1]);
-    if (s_guard.shift > 0.0)
-      sweeps += jacobi<NT, LDSN, MAXN>(n, ld, F, c.slot(S_A), c.slot(S_T), c.slot(S_V), eps_a, s_a, s_c, s_s, s_t,
-                                       s_perm, s_red);
+    if (s_guard.shift > 0.0) {
+      sweeps += eigensolve<NT, A_LDS, V_LDS>(n, ld, F, c.slot(cbuf ? S_C0 : S_C1), c.slot(cbuf ? S_C1 : S_C0), eps_a,
+                                             c.slot(S_AG), c.slot(S_VG), c.slot(S_T), s_a, s_v, s_c, s_s, s_t, s_perm,
+                                             s_pq, s_pq + MAXN / 2, s_red, tk + PH_EIG_IN);
+      cbuf ^= 1;
+    }
     res.p_slot = pslot;
-    res.c_slot = S_V;
+    res.c_slot = cbuf ? S_C0 : S_C1;
     res.pb_slot = -1;
     res.cb_slot = -1;
   } else {
@@ -808,11 +991,13 @@ __global__ void __launch_bounds__(NT)
     double* Fb = c.slot(U_FB);
     sad_guess<NT>(c, Pa, na / ntot);
     sad_guess<NT>(c, Pb, nb / ntot);
+    int cbuf = 0;
     int iter = 0;
     for (; iter < c.M.max_scf; ++iter) {
       for (int idx = threadIdx.x; idx < ld * ld; idx += NT) Pt[idx] = Pa[idx] + Pb[idx];
       __syncthreads();
       fock_build<NT>(c, Pt, Pa, 1.0, Pb, 1.0, Fa, Fb);
+      lap(PH_FOCK);
       double eh[3] = {0.0, 0.0, 0.0};
       for (int idx = threadIdx.x; idx < n * n; idx += NT) {
         int i = idx / n, j = idx - i * n;
@@ -826,15 +1011,23 @@ __global__ void __launch_bounds__(NT)
       const double ea2 = commutator<NT>(n, ld, Fa, Pa, nullptr, s_red);
       const double eb2 = commutator<NT>(n, ld, Fb, Pb, nullptr, s_red);
       const double err = sqrt(ea2 + eb2);
+      lap(PH_COMM);
       const double shift = s_guard.shift;
       level_shift<NT>(n, ld, Fa, Pa, 1.0, shift);
       level_shift<NT>(n, ld, Fb, Pb, 1.0, shift);
-      sweeps += jacobi<NT, LDSN, MAXN>(n, ld, Fa, c.slot(U_A), c.slot(U_T), c.slot(U_CA), eps_a, s_a, s_c, s_s, s_t,
-                                       s_perm, s_red);
-      sweeps += jacobi<NT, LDSN, MAXN>(n, ld, Fb, c.slot(U_A), c.slot(U_T), c.slot(U_CB), eps_b, s_a, s_c, s_s, s_t,
-                                       s_perm, s_red);
-      const double da = density<NT>(n, ld, na, 1.0, c.slot(U_CA), Pa, Pan, s_red);
-      const double db = density<NT>(n, ld, nb, 1.0, c.slot(U_CB), Pb, Pbn, s_red);
+      double* Ca = c.slot(cbuf ? U_CA1 : U_CA0);
+      double* Cb = c.slot(cbuf ? U_CB1 : U_CB0);
+      sweeps += eigensolve<NT, A_LDS, V_LDS>(n, ld, Fa, iter > 0 ? c.slot(cbuf ? U_CA0 : U_CA1) : nullptr, Ca, eps_a,
+                                             c.slot(U_AG), c.slot(U_VG), c.slot(U_T), s_a, s_v, s_c, s_s, s_t, s_perm,
+                                             s_pq, s_pq + MAXN / 2, s_red, tk + PH_EIG_IN);
+      sweeps += eigensolve<NT, A_LDS, V_LDS>(n, ld, Fb, iter > 0 ? c.slot(cbuf ? U_CB0 : U_CB1) : nullptr, Cb, eps_b,
+                                             c.slot(U_AG), c.slot(U_VG), c.slot(U_T), s_a, s_v, s_c, s_s, s_t, s_perm,
+                                             s_pq, s_pq + MAXN / 2, s_red, tk + PH_EIG_IN);
+      lap(PH_EIG);
+      const double da = density<NT>(n, ld, na, 1.0, Ca, Pa, Pan, s_red);
+      const double db = density<NT>(n, ld, nb, 1.0, Cb, Pb, Pbn, s_red);
+      lap(PH_DENS);
+      cbuf ^= 1;
       if (threadIdx.x == 0) {
         const double nn = (double)((int64_t)n * n);
         const double dp = sqrt(da / nn) + sqrt(db / nn);
@@ -875,30 +1068,46 @@ __global__ void __launch_bounds__(NT)
     }
     wg_sum_n<NT, 3>(eh, s_red);
     res.e_el = 0.5 * (eh[0] + eh[1] + eh[2]);
-    if (s_guard.shift > 0.0)
-      sweeps += jacobi<NT, LDSN, MAXN>(n, ld, Fa, c.slot(U_A), c.slot(U_T), c.slot(U_CA), eps_a, s_a, s_c, s_s, s_t,
-                                       s_perm, s_red);
-    sweeps += jacobi<NT, LDSN, MAXN>(n, ld, Fb, c.slot(U_A), c.slot(U_T), c.slot(U_CB), eps_b, s_a, s_c, s_s, s_t,
-                                     s_perm, s_red);
+    // The last loop solve left alpha/beta vectors in buffer (cbuf ^ 1).
+    int ca_last = cbuf ? U_CA0 : U_CA1, cb_last = cbuf ? U_CB0 : U_CB1;
+    if (s_guard.shift > 0.0) {
+      const int dst = (ca_last == U_CA0) ? U_CA1 : U_CA0;
+      sweeps += eigensolve<NT, A_LDS, V_LDS>(n, ld, Fa, c.slot(ca_last), c.slot(dst), eps_a, c.slot(U_AG), c.slot(U_VG),
+                                             c.slot(U_T), s_a, s_v, s_c, s_s, s_t, s_perm, s_pq,
+                                             s_pq + MAXN / 2, s_red, tk + PH_EIG_IN);
+      ca_last = dst;
+    }
+    {
+      const int dst = (cb_last == U_CB0) ? U_CB1 : U_CB0;
+      sweeps += eigensolve<NT, A_LDS, V_LDS>(n, ld, Fb, c.slot(cb_last), c.slot(dst), eps_b, c.slot(U_AG), c.slot(U_VG),
+                                             c.slot(U_T), s_a, s_v, s_c, s_s, s_t, s_perm, s_pq,
+                                             s_pq + MAXN / 2, s_red, tk + PH_EIG_IN);
+      cb_last = dst;
+    }
     res.p_slot = (Pa == c.slot(U_PA)) ? U_PA : U_PAN;
     res.pb_slot = (Pb == c.slot(U_PB)) ? U_PB : U_PBN;
-    res.c_slot = U_CA;
-    res.cb_slot = U_CB;
+    res.c_slot = ca_last;
+    res.cb_slot = cb_last;
   }
   if (threadIdx.x == 0) {
+    tk[PH_TOTAL] = wall_clock64() - t_start;
+    for (int k = 0; k < NPHASE; ++k) res.ticks[k] = tk[k];
     res.shift = s_guard.shift;
     res.sweeps = sweeps;
     out[mi] = res;
   }
 }
 
-// Instances: LDS-resident Jacobi for molecules up to 140 AOs, L2-resident above that.
-template __global__ void k_scf<512, 140, 140>(const int*, const MolDev*, const int*, const int*, const int*,
-                                              const PairDev*, const ElemDev*, const double*, double*, double*, MolOut*);
-template __global__ void k_scf<256, 64, 64>(const int*, const MolDev*, const int*, const int*, const int*,
-                                            const PairDev*, const ElemDev*, const double*, double*, double*, MolOut*);
-template __global__ void k_scf<512, 0, 1024>(const int*, const MolDev*, const int*, const int*, const int*,
-                                             const PairDev*, const ElemDev*, const double*, double*, double*, MolOut*);
+// Size classes: <= 64 and <= 112 basis functions keep the packed matrix and the rotation
+// accumulator in LDS; <= 196 keeps the packed matrix in LDS; larger molecules run from L2/HBM.
+#define PM7_SCF_INSTANCE(NT, MAXN, AL, VL)                                                                        \
+  template __global__ void k_scf<NT, MAXN, AL, VL>(const int*, const MolDev*, const int*, const int*, const int*, \
+                                                   const PairDev*, const ElemDev*, const double*, double*, double*, \
+                                                   MolOut*);
+PM7_SCF_INSTANCE(256, 64, true, true)
+PM7_SCF_INSTANCE(512, 112, true, true)
+PM7_SCF_INSTANCE(512, 196, true, false)
+PM7_SCF_INSTANCE(512, 1024, false, false)
 
 // One Fock build on a molecule's slab (the build_fock_spin entry point): F = H + J(Pt) - K(Ps).
 __global__ void __launch_bounds__(256) k_fock_once(int mi, const MolDev* __restrict__ mols, const int* __restrict__ atom_z,
@@ -925,16 +1134,20 @@ hipError_t launch_scf(int variant, const BatchDev& b, int list_off, int count, h
   const int* list = b.mol_list + list_off;
   switch (variant) {
     case 0:
-      hipLaunchKernelGGL((k_scf<256, 64, 64>), dim3(count), dim3(256), 0, s, list, b.mols, b.atom_z, b.atom_ao,
-                         b.atom_pp, b.pairs, b.elem, b.w, b.slab, b.vecs, b.out);
+      hipLaunchKernelGGL((k_scf<256, 64, true, true>), dim3(count), dim3(256), 0, s, list, b.mols, b.atom_z,
+                         b.atom_ao, b.atom_pp, b.pairs, b.elem, b.w, b.slab, b.vecs, b.out);
       break;
     case 1:
-      hipLaunchKernelGGL((k_scf<512, 140, 140>), dim3(count), dim3(512), 0, s, list, b.mols, b.atom_z, b.atom_ao,
-                         b.atom_pp, b.pairs, b.elem, b.w, b.slab, b.vecs, b.out);
+      hipLaunchKernelGGL((k_scf<512, 112, true, true>), dim3(count), dim3(512), 0, s, list, b.mols, b.atom_z,
+                         b.atom_ao, b.atom_pp, b.pairs, b.elem, b.w, b.slab, b.vecs, b.out);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_scf<512, 196, true, false>), dim3(count), dim3(512), 0, s, list, b.mols, b.atom_z,
+                         b.atom_ao, b.atom_pp, b.pairs, b.elem, b.w, b.slab, b.vecs, b.out);
       break;
     default:
-      hipLaunchKernelGGL((k_scf<512, 0, 1024>), dim3(count), dim3(512), 0, s, list, b.mols, b.atom_z, b.atom_ao,
-                         b.atom_pp, b.pairs, b.elem, b.w, b.slab, b.vecs, b.out);
+      hipLaunchKernelGGL((k_scf<512, 1024, false, false>), dim3(count), dim3(512), 0, s, list, b.mols, b.atom_z,
+                         b.atom_ao, b.atom_pp, b.pairs, b.elem, b.w, b.slab, b.vecs, b.out);
       break;
   }
   return hipGetLastError();

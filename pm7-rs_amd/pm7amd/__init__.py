@@ -75,6 +75,7 @@ class _Result(C.Structure):
                 ("lumo_ev_beta", C.c_double), ("dipole_debye", C.c_double * 3),
                 ("dipole_point_charge_debye", C.c_double * 3), ("dipole_sp_hybrid_debye", C.c_double * 3),
                 ("dipole_pd_hybrid_debye", C.c_double * 3), ("dipole_magnitude", C.c_double),
+                ("device_phase_ms", C.c_double * 6), ("eig_phase_ms", C.c_double * 3),
                 ("density", C.POINTER(C.c_double)), ("spin_density", C.POINTER(C.c_double)),
                 ("mo_energies", C.POINTER(C.c_double)), ("mo_coeff", C.POINTER(C.c_double)),
                 ("mo_energies_beta", C.POINTER(C.c_double)), ("mo_coeff_beta", C.POINTER(C.c_double)),
@@ -113,7 +114,7 @@ def library() -> C.CDLL:
     lib.pm7_fock_spin.argtypes = [vp, C.POINTER(_Molecule), C.POINTER(C.c_double), C.POINTER(C.c_double),
                                   C.POINTER(C.c_double)]
     lib.pm7_core_core_energy.argtypes = [vp, C.POINTER(_Molecule), C.POINTER(C.c_double)]
-    lib.pm7_last_timings.argtypes = [vp, C.POINTER(C.c_double)]
+    lib.pm7_last_timings.argtypes = [vp, C.POINTER(C.c_double), C.c_int32]
     _lib = lib
     return lib
 
@@ -256,6 +257,8 @@ class Pm7Result:
     density_error: float
     fp32_iterations: int = 0
     jacobi_sweeps: int = 0
+    device_phase_ms: tuple = ()
+    eig_phase_ms: tuple = ()
     message: str = ""
 
     @property
@@ -304,7 +307,9 @@ def _convert(r: _Result) -> Pm7Result:
         homo_ev=_opt(r.homo_ev), lumo_ev=_opt(r.lumo_ev), homo_ev_beta=_opt(r.homo_ev_beta),
         lumo_ev_beta=_opt(r.lumo_ev_beta), iterations=r.iterations, converged=bool(r.converged),
         unrestricted=bool(r.unrestricted), density_error=r.density_error, fp32_iterations=r.fp32_iterations,
-        jacobi_sweeps=r.jacobi_sweeps, message=r.message.decode(errors="replace"),
+        jacobi_sweeps=r.jacobi_sweeps, device_phase_ms=tuple(r.device_phase_ms[:]),
+        eig_phase_ms=tuple(r.eig_phase_ms[:]),
+        message=r.message.decode(errors="replace"),
     )
 
 
@@ -418,8 +423,9 @@ class Engine:
         return e.value
 
     def last_timings_ms(self):
-        buf = (C.c_double * 4)()
-        self._lib.pm7_last_timings(self._h, buf)
+        """[integrals, scf (all classes), gather, wall, scf class 0, 1, 2, 3] in ms."""
+        buf = (C.c_double * 8)()
+        self._lib.pm7_last_timings(self._h, buf, 8)
         return list(buf)
 
 

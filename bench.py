@@ -28,6 +28,22 @@ sys.path.insert(0, str(ROOT / "pm7-rs_amd"))
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 
 
+SCF_KERNELS = ["pm7::k_scf<256, 64, true, true>", "pm7::k_scf<512, 112, true, true>",
+               "pm7::k_scf<512, 196, true, false>", "pm7::k_scf<512, 1024, false, false>"]
+TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_organic256_fp64.json"
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the FETCH_SIZE / WRITE_SIZE counter runs (scaled by a
+    plain copy's readings) kept beside the profiles, or None when that kernel was not profiled."""
+    try:
+        per = json.loads(TRAFFIC_FILE.read_text())["per_launch"]
+    except (OSError, ValueError, KeyError):
+        return None
+    rec = per.get(kernel)
+    return None if rec is None else round(rec["bytes"])
+
+
 def pct(xs, q):
     s = sorted(xs)
     if not s:
@@ -54,6 +70,16 @@ def scf_flops(results, classes):
     return out
 
 
+def slowest_info(results):
+    """Basis size, iterations and Jacobi cost of the molecule with the longest device SCF time."""
+    r = max(results, key=lambda x: x.device_phase_ms[5])
+    n = r.density.shape[0]
+    n2 = n + (n & 1)
+    rounds = r.jacobi_sweeps * max(1, n2 - 1)
+    return {"nao": n, "iterations": r.iterations, "jacobi_sweeps": r.jacobi_sweeps,
+            "us_per_jacobi_round": round(r.eig_phase_ms[1] * 1e3 / rounds, 3) if rounds else None}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,6 +89,8 @@ def main() -> int:
     ap.add_argument("--single", type=int, default=32, help="molecules timed one at a time")
     ap.add_argument("--cpu-sample", type=int, default=24, help="molecules timed on the CPU oracle")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: every box here has a single MI355X; the batched path "
@@ -71,6 +99,17 @@ def main() -> int:
 
     import numpy as np
     import torch
+
+    if args.copy_probe_mb > 0:
+        # A plain copy of known size: its FETCH_SIZE / WRITE_SIZE readings per dispatch give the
+        # factor between those counters and bytes moved.
+        x = torch.ones(args.copy_probe_mb << 17, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": x.numel() * 8, "dispatches": 3}), flush=True)
+        return 0
 
     import pm7amd
     from pm7amd import workloads as W
@@ -117,10 +156,10 @@ def main() -> int:
     dom = int(np.argmax(cls_ms))
     flops = scf_flops(res64, None)
     achieved_tf = flops[dom] / (cls_ms[dom] * 1e-3) / 1e12 if cls_ms[dom] > 0 else 0.0
-    roofline = {"bound": "mfma", "kernel": ["k_scf<256,64,LDS,LDS>", "k_scf<512,112,LDS,LDS>",
-                                            "k_scf<512,196,LDS,L2>", "k_scf<512,1024,L2,L2>"][dom],
+    kname = SCF_KERNELS[dom]
+    roofline = {"bound": "mfma", "kernel": kname,
                 "achieved": round(achieved_tf, 6), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved_tf / FP64_PEAK_TFLOPS, 8), "traffic": None,
+                "frac": round(achieved_tf / FP64_PEAK_TFLOPS, 8), "traffic": measured_traffic(kname),
                 "avg_launch_ms": round(cls_ms[dom], 4), "class_ms": [round(x, 4) for x in cls_ms]}
 
     cpu = None
@@ -168,7 +207,12 @@ def main() -> int:
                                  "batched_speedup": round(solves64 / single_rate, 3)},
         "kernel_ms": {"integrals": round(float(np.mean([k[0] for k in kern64])), 4),
                       "scf": round(float(np.mean([k[1] for k in kern64])), 4),
-                      "gather": round(float(np.mean([k[2] for k in kern64])), 4)},
+                      "gather": round(float(np.mean([k[2] for k in kern64])), 4),
+                      "call_wall": round(float(np.mean([k[3] for k in kern64])), 4),
+                      "host_prepare": round(float(np.mean([k[8] for k in kern64])), 4),
+                      "host_launch": round(float(np.mean([k[9] for k in kern64])), 4),
+                      "device_wait": round(float(np.mean([k[10] for k in kern64])), 4),
+                      "host_results": round(float(np.mean([k[11] for k in kern64])), 4)},
         "iterations": {"mean": round(float(np.mean([r.iterations for r in res64])), 2),
                        "max": int(max(r.iterations for r in res64))},
         "phase_ms_slowest": dict(zip(["fock", "energy_commutator", "diis", "eigensolve", "density", "total"],
@@ -177,6 +221,7 @@ def main() -> int:
                                  [round(sum(r.device_phase_ms[k] for r in res64), 2) for k in range(6)])),
         "eig_split_ms_slowest": dict(zip(["warm_start_in", "sweeps", "sort_out"],
                                          [round(x, 3) for x in max(res64, key=lambda r: r.device_phase_ms[5]).eig_phase_ms])),
+        "slowest_molecule": slowest_info(res64),
         "jacobi_sweeps_per_solve": round(float(np.mean([r.jacobi_sweeps / max(1, r.iterations) for r in res64])), 2),
         "parity_vs_oracle": parity,
         "roofline": roofline,

@@ -1096,9 +1096,11 @@ void eigh_jacobi(int n, const Mat& a, std::vector<double>& w, Mat& vout) {
         double apq = A[(size_t)p * n2 + q];
         c[k] = 1.0; s[k] = 0.0; t[k] = 0.0;
         if (apq != 0.0) {
-          double tau = (A[(size_t)q * n2 + q] - A[(size_t)p * n2 + p]) / (2.0 * apq);
-          t[k] = std::fabs(tau) > 1e150 ? 0.5 / tau : (tau >= 0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
-          c[k] = 1.0 / std::sqrt(1.0 + t[k] * t[k]);
+          // t = sgn(d) x / (|d| + r), c = sqrt((|d| + r) / 2r): the smaller root of t^2 + 2 t d/x - 1 = 0
+          double d = A[(size_t)q * n2 + q] - A[(size_t)p * n2 + p], x = 2.0 * apq;
+          double r = std::sqrt(d * d + x * x), den = std::fabs(d) + r;
+          t[k] = (d >= 0.0 ? x : -x) / den;
+          c[k] = std::sqrt(den / (2.0 * r));
           s[k] = t[k] * c[k];
         }
       }

@@ -101,7 +101,7 @@ struct pm7_ctx {
   hipEvent_t ev_var[2 * SCF_VARIANTS] = {};
   DevBuf d_elem, d_pairpar, d_mols, d_list, d_atom_z, d_atom_ao, d_atom_pp, d_xyz, d_pairs, d_items, d_gath_off;
   DevBuf d_w, d_slab, d_vecs, d_out, d_ecore, d_ecore_mol, d_gath;
-  double timings[8] = {};
+  double timings[PM7_TIMING_SLOTS] = {};
   std::mutex mu;
 };
 
@@ -519,6 +519,7 @@ pm7_status run_jobs(pm7_ctx* ctx, std::vector<Job>& jobs, pm7_result_v1* results
   Prepared P;
   P.jobs = std::move(jobs);
   pm7_status st = prepare(ctx, P, max_memory_mb);
+  const auto t_prep = std::chrono::steady_clock::now();
   if (st != PM7_OK) {
     for (auto& j : P.jobs) {
       results[j.idx].status = st;
@@ -559,7 +560,9 @@ pm7_status run_jobs(pm7_ctx* ctx, std::vector<Job>& jobs, pm7_result_v1* results
   HIPCHK(hipMemcpyAsync(ecore.data(), P.b.e_core_mol, nm * sizeof(double), hipMemcpyDeviceToHost, ctx->st[0]));
   HIPCHK(hipMemcpyAsync(vecs.data(), P.b.vecs, P.vec_words * sizeof(double), hipMemcpyDeviceToHost, ctx->st[0]));
   HIPCHK(hipMemcpyAsync(gath.data(), P.b.gath, P.gath_words * sizeof(double), hipMemcpyDeviceToHost, ctx->st[0]));
+  const auto t_launched = std::chrono::steady_clock::now();
   HIPCHK(hipStreamSynchronize(ctx->st[0]));
+  const auto t_synced = std::chrono::steady_clock::now();
   float ms = 0.0f;
   if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) == hipSuccess) ctx->timings[0] = ms;
   if (hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]) == hipSuccess) ctx->timings[1] = ms;
@@ -573,7 +576,13 @@ pm7_status run_jobs(pm7_ctx* ctx, std::vector<Job>& jobs, pm7_result_v1* results
     const Job& j = P.jobs[mi];
     finish_result(ctx, j, outs[mi], ecore[mi], vecs.data() + j.vec0, gath.data() + j.gath0, &results[j.idx]);
   }
-  ctx->timings[3] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  const auto t_end = std::chrono::steady_clock::now();
+  using msd = std::chrono::duration<double, std::milli>;
+  ctx->timings[3] = msd(t_end - t0).count();
+  ctx->timings[8] = msd(t_prep - t0).count();
+  ctx->timings[9] = msd(t_launched - t_prep).count();
+  ctx->timings[10] = msd(t_synced - t_launched).count();
+  ctx->timings[11] = msd(t_end - t_synced).count();
   return PM7_OK;
 }
 
@@ -804,7 +813,7 @@ pm7_status pm7_core_core_energy(pm7_ctx* ctx, const pm7_molecule_v1* mol, double
 
 void pm7_last_timings(const pm7_ctx* ctx, double* ms, int32_t n) {
   if (!ctx || !ms) return;
-  for (int k = 0; k < n && k < 8; ++k) ms[k] = ctx->timings[k];
+  for (int k = 0; k < n && k < PM7_TIMING_SLOTS; ++k) ms[k] = ctx->timings[k];
 }
 
 }  // extern "C"

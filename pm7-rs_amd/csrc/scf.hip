@@ -19,45 +19,13 @@
 // order, so a molecule's result does not depend on which other molecules share the launch.
 #include <hip/hip_runtime.h>
 
+#include "eigen.h"
 #include "kernels.h"
 #include "pm7_device.h"
+#include "wg.h"
 
 namespace pm7 {
 
-template <int NT>
-struct Wg {
-  static constexpr int NW = NT / 64;
-};
-
-// Sum NV values over the workgroup (deterministic: fixed shuffle tree, then wave order).
-template <int NT, int NV>
-__device__ inline void wg_sum_n(double (&v)[NV], double* red) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    double x = v[k];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
-    if (lane == 0) red[wave * NV + k] = x;
-  }
-  __syncthreads();
-  if (threadIdx.x < NV) {
-    double s = 0.0;
-    for (int w = 0; w < Wg<NT>::NW; ++w) s += red[w * NV + threadIdx.x];
-    red[Wg<NT>::NW * NV + threadIdx.x] = s;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < NV; ++k) v[k] = red[Wg<NT>::NW * NV + k];
-  __syncthreads();
-}
-
-template <int NT>
-__device__ inline double wg_sum(double x, double* red) {
-  double v[1] = {x};
-  wg_sum_n<NT, 1>(v, red);
-  return v[0];
-}
 
 __device__ inline void unpack_e(int e, int& mu, int& nu) {
   mu = (e >= 6) ? 3 : (e >= 3) ? 2 : (e >= 1) ? 1 : 0;
@@ -312,237 +280,6 @@ __device__ double density(int n, int ld, int nocc, double wgt, const double* __r
   return wg_sum<NT>(dsq, red);
 }
 
-__device__ inline int rr_player(int pos, int r, int n2) { return pos == 0 ? 0 : 1 + (pos - 1 + r) % (n2 - 1); }
-
-// Packed upper-triangular index of the symmetric element (i, j).
-__device__ inline int pidx(int i, int j) {
-  const int a = i < j ? i : j, b = i < j ? j : i;
-  return b * (b + 1) / 2 + a;
-}
-
-// Row of the packed upper-triangular position t (t = j(j+1)/2 + i, i <= j).
-__device__ inline int tri_row(int t) {
-  int j = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
-  while (j * (j + 1) / 2 > t) --j;
-  while ((j + 1) * (j + 2) / 2 <= t) ++j;
-  return j;
-}
-
-// Parallel cyclic Jacobi on the packed symmetric n2 x n2 matrix A, accumulating the rotations
-// into V (row-major, stride n2). Round-robin pairing gives n2/2 disjoint rotations per round;
-// the 2x2 block (pair k rows, pair l columns) takes both its row and column rotation in one
-// update, so a round is one pass over the packed matrix. Returns the number of sweeps.
-template <int NT>
-__device__ int jacobi_core(int n2, double* A, double* V, double* s_c, double* s_s, double* s_t, int* s_p, int* s_q,
-                           double* red) {
-  constexpr int NX = 32, NY = NT / NX;  // 2-D thread grid: compile-time divisors only
-  const int tx = threadIdx.x % NX, ty = threadIdx.x / NX;
-  const int m = n2 / 2;
-  int sweep = 0;
-  for (; sweep < 40; ++sweep) {
-    double sums[2] = {0.0, 0.0};
-    for (int j = ty; j < n2; j += NY) {
-      const double* rowj = A + j * (j + 1) / 2;
-      for (int i = tx; i <= j; i += NX) {
-        const double v = rowj[i];
-        if (i == j) sums[1] += v * v;
-        else sums[0] += 2.0 * v * v;
-      }
-    }
-    wg_sum_n<NT, 2>(sums, red);
-    if (sums[0] <= 1.0e-26 * (sums[0] + sums[1])) break;
-    for (int r = 0; r < n2 - 1; ++r) {
-      if (threadIdx.x < m) {
-        const int k = threadIdx.x;
-        const int p = rr_player(k, r, n2), q = rr_player(n2 - 1 - k, r, n2);
-        const double apq = A[pidx(p, q)];
-        double cc = 1.0, ss = 0.0, tt = 0.0;
-        if (apq != 0.0) {
-          const double tau = (A[pidx(q, q)] - A[pidx(p, p)]) / (2.0 * apq);
-          if (fabs(tau) > 1.0e150) tt = 0.5 / tau;
-          else tt = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-          cc = 1.0 / sqrt(1.0 + tt * tt);
-          ss = tt * cc;
-        }
-        s_p[k] = p;
-        s_q[k] = q;
-        s_c[k] = cc;
-        s_s[k] = ss;
-        s_t[k] = tt;
-      }
-      __syncthreads();
-      // 2x2 blocks (pair k rows, pair l columns), k <= l, each unordered block once.
-      for (int k = ty; k < m; k += NY) {
-        const int pk = s_p[k], qk = s_q[k];
-        const double ck = s_c[k], sk = s_s[k];
-        for (int l = k + tx; l < m; l += NX) {
-          if (l == k) {
-            const double tk = s_t[k];
-            if (tk == 0.0) continue;
-            const int ipq = pidx(pk, qk);
-            const double apq = A[ipq];
-            A[pidx(pk, pk)] -= tk * apq;
-            A[pidx(qk, qk)] += tk * apq;
-            A[ipq] = 0.0;
-            continue;
-          }
-          const double cl = s_c[l], sl = s_s[l];
-          if (sk == 0.0 && sl == 0.0) continue;
-          const int pl = s_p[l], ql = s_q[l];
-          const int i00 = pidx(pk, pl), i01 = pidx(pk, ql), i10 = pidx(qk, pl), i11 = pidx(qk, ql);
-          const double x00 = A[i00], x01 = A[i01], x10 = A[i10], x11 = A[i11];
-          const double y00 = ck * x00 - sk * x10, y01 = ck * x01 - sk * x11;
-          const double y10 = sk * x00 + ck * x10, y11 = sk * x01 + ck * x11;
-          A[i00] = cl * y00 - sl * y01;
-          A[i01] = sl * y00 + cl * y01;
-          A[i10] = cl * y10 - sl * y11;
-          A[i11] = sl * y10 + cl * y11;
-        }
-      }
-      // Rotation accumulator: rows over ty, column pairs over tx. Each thread issues the loads of
-      // four rows before any store so the memory requests overlap instead of chaining.
-      for (int k = tx; k < m; k += NX) {
-        const double ss = s_s[k];
-        if (ss == 0.0) continue;
-        const int pk = s_p[k], qk = s_q[k];
-        const double cc = s_c[k];
-        for (int i0 = ty; i0 < n2; i0 += 4 * NY) {
-          double vp[4], vq[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const int i = i0 + u * NY;
-            vp[u] = i < n2 ? V[i * n2 + pk] : 0.0;
-            vq[u] = i < n2 ? V[i * n2 + qk] : 0.0;
-          }
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const int i = i0 + u * NY;
-            if (i < n2) {
-              V[i * n2 + pk] = cc * vp[u] - ss * vq[u];
-              V[i * n2 + qk] = ss * vp[u] + cc * vq[u];
-            }
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
-  return sweep;
-}
-
-// Eigenpairs of the symmetric ld x ld matrix W (pad rows/columns zero): ascending eigenvalues
-// eps[0..n) and eigenvectors as the columns of Cout. With Cprev (the previous SCF iteration's
-// eigenvectors) the solve is warm-started: Jacobi diagonalises Cprev^T W Cprev, which is nearly
-// diagonal once the SCF settles, and Cout = Cprev V. A and V sit in LDS when the variant allows.
-template <int NT, bool A_LDS, bool V_LDS>
-__device__ int eigensolve(int n, int ld, const double* __restrict__ W, const double* __restrict__ Cprev,
-                          double* __restrict__ Cout, double* __restrict__ eps, double* Ag, double* Vg,
-                          double* __restrict__ T, double* s_a, double* s_v, double* s_c, double* s_s, double* s_t,
-                          int* s_perm, int* s_p, int* s_q, double* red, unsigned long long* tsub) {
-  const int n2 = ld, nt = ld / 2;
-  unsigned long long t0 = wall_clock64();
-  double* A = A_LDS ? s_a : Ag;
-  double* V = V_LDS ? s_v : Vg;
-  if (Cprev) {
-    // T = W Cprev
-    for (int t = threadIdx.x; t < nt * nt; t += NT) {
-      const int I = t / nt, J = t - I * nt, i0 = 2 * I, j0 = 2 * J;
-      const double* w0 = W + (int64_t)i0 * ld;
-      const double* w1 = w0 + ld;
-      double s00 = 0.0, s01 = 0.0, s10 = 0.0, s11 = 0.0;
-      for (int k = 0; k < n; ++k) {
-        const double a0 = w0[k], a1 = w1[k];
-        const double b0 = Cprev[(int64_t)k * ld + j0], b1 = Cprev[(int64_t)k * ld + j0 + 1];
-        s00 += a0 * b0;
-        s01 += a0 * b1;
-        s10 += a1 * b0;
-        s11 += a1 * b1;
-      }
-      T[(int64_t)i0 * ld + j0] = s00;
-      T[(int64_t)i0 * ld + j0 + 1] = s01;
-      T[(int64_t)(i0 + 1) * ld + j0] = s10;
-      T[(int64_t)(i0 + 1) * ld + j0 + 1] = s11;
-    }
-    __syncthreads();
-    // A = Cprev^T T, upper triangle only.
-    for (int t = threadIdx.x; t < nt * nt; t += NT) {
-      const int I = t / nt, J = t - I * nt;
-      if (I > J) continue;
-      const int i0 = 2 * I, j0 = 2 * J;
-      double s00 = 0.0, s01 = 0.0, s10 = 0.0, s11 = 0.0;
-      for (int k = 0; k < n; ++k) {
-        const double* c = Cprev + (int64_t)k * ld;
-        const double* tr = T + (int64_t)k * ld;
-        const double a0 = c[i0], a1 = c[i0 + 1], b0 = tr[j0], b1 = tr[j0 + 1];
-        s00 += a0 * b0;
-        s01 += a0 * b1;
-        s10 += a1 * b0;
-        s11 += a1 * b1;
-      }
-      A[pidx(i0, j0)] = s00;
-      A[pidx(i0, j0 + 1)] = s01;
-      if (I != J) A[pidx(i0 + 1, j0)] = s10;
-      A[pidx(i0 + 1, j0 + 1)] = s11;
-    }
-  } else {
-    constexpr int NX = 32, NY = NT / NX;
-    for (int j = threadIdx.x / NX; j < n2; j += NY)
-      for (int i = threadIdx.x % NX; i <= j; i += NX) A[j * (j + 1) / 2 + i] = W[(int64_t)i * ld + j];
-  }
-  for (int idx = threadIdx.x; idx < n2 * n2; idx += NT) {
-    const int i = idx / n2, j = idx - i * n2;
-    V[idx] = (i == j) ? 1.0 : 0.0;
-  }
-  __syncthreads();
-  unsigned long long t1 = wall_clock64();
-  const int sweeps = jacobi_core<NT>(n2, A, V, s_c, s_s, s_t, s_p, s_q, red);
-  unsigned long long t2 = wall_clock64();
-  for (int i = threadIdx.x; i < n; i += NT) {
-    const double di = A[pidx(i, i)];
-    int rank = 0;
-    for (int j = 0; j < n; ++j) {
-      const double dj = A[pidx(j, j)];
-      rank += (dj < di || (dj == di && j < i)) ? 1 : 0;
-    }
-    s_perm[rank] = i;
-  }
-  __syncthreads();
-  for (int k = threadIdx.x; k < n; k += NT) eps[k] = A[pidx(s_perm[k], s_perm[k])];
-  if (Cprev) {
-    // Cout = Cprev V(:, perm)
-    for (int t = threadIdx.x; t < nt * nt; t += NT) {
-      const int I = t / nt, J = t - I * nt, i0 = 2 * I, k0 = 2 * J;
-      const int c0 = k0 < n ? s_perm[k0] : -1, c1 = k0 + 1 < n ? s_perm[k0 + 1] : -1;
-      const double* a0p = Cprev + (int64_t)i0 * ld;
-      const double* a1p = a0p + ld;
-      double s00 = 0.0, s01 = 0.0, s10 = 0.0, s11 = 0.0;
-      for (int j = 0; j < n; ++j) {
-        const double a0 = a0p[j], a1 = a1p[j];
-        const double b0 = c0 >= 0 ? V[j * n2 + c0] : 0.0, b1 = c1 >= 0 ? V[j * n2 + c1] : 0.0;
-        s00 += a0 * b0;
-        s01 += a0 * b1;
-        s10 += a1 * b0;
-        s11 += a1 * b1;
-      }
-      Cout[(int64_t)i0 * ld + k0] = s00;
-      Cout[(int64_t)i0 * ld + k0 + 1] = s01;
-      Cout[(int64_t)(i0 + 1) * ld + k0] = s10;
-      Cout[(int64_t)(i0 + 1) * ld + k0 + 1] = s11;
-    }
-  } else {
-    for (int idx = threadIdx.x; idx < n2 * n2; idx += NT) {
-      const int i = idx / n2, k = idx - i * n2;
-      Cout[idx] = (i < n && k < n) ? V[i * n2 + s_perm[k]] : 0.0;
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    tsub[0] += t1 - t0;
-    tsub[1] += t2 - t1;
-    tsub[2] += wall_clock64() - t2;
-  }
-  return sweeps;
-}
 
 // ---- small serial solvers (thread 0) -------------------------------------------------------
 
@@ -924,7 +661,7 @@ __global__ void __launch_bounds__(NT)
       level_shift<NT>(n, ld, W, P, 2.0, s_guard.shift);
       lap(PH_DIIS);
       double* Cnew = c.slot(cbuf ? S_C1 : S_C0);
-      sweeps += eigensolve<NT, A_LDS, V_LDS>(n, ld, W, iter > 0 ? c.slot(cbuf ? S_C0 : S_C1) : nullptr, Cnew, eps_a,
+      sweeps += eigensolve<NT, MAXN, A_LDS, V_LDS>(n, ld, W, iter > 0 ? c.slot(cbuf ? S_C0 : S_C1) : nullptr, Cnew, eps_a,
                                              c.slot(S_AG), c.slot(S_VG), c.slot(S_T), s_a, s_v, s_c, s_s, s_t, s_perm,
                                              s_pq, s_pq + MAXN / 2, s_red, tk + PH_EIG_IN);
       lap(PH_EIG);
@@ -969,7 +706,7 @@ __global__ void __launch_bounds__(NT)
     wg_sum_n<NT, 2>(eh, s_red);
     res.e_el = 0.5 * (eh[0] + eh[1]);
     if (s_guard.shift > 0.0) {
-      sweeps += eigensolve<NT, A_LDS, V_LDS>(n, ld, F, c.slot(cbuf ? S_C0 : S_C1), c.slot(cbuf ? S_C1 : S_C0), eps_a,
+      sweeps += eigensolve<NT, MAXN, A_LDS, V_LDS>(n, ld, F, c.slot(cbuf ? S_C0 : S_C1), c.slot(cbuf ? S_C1 : S_C0), eps_a,
                                              c.slot(S_AG), c.slot(S_VG), c.slot(S_T), s_a, s_v, s_c, s_s, s_t, s_perm,
                                              s_pq, s_pq + MAXN / 2, s_red, tk + PH_EIG_IN);
       cbuf ^= 1;
@@ -1017,10 +754,10 @@ __global__ void __launch_bounds__(NT)
       level_shift<NT>(n, ld, Fb, Pb, 1.0, shift);
       double* Ca = c.slot(cbuf ? U_CA1 : U_CA0);
       double* Cb = c.slot(cbuf ? U_CB1 : U_CB0);
-      sweeps += eigensolve<NT, A_LDS, V_LDS>(n, ld, Fa, iter > 0 ? c.slot(cbuf ? U_CA0 : U_CA1) : nullptr, Ca, eps_a,
+      sweeps += eigensolve<NT, MAXN, A_LDS, V_LDS>(n, ld, Fa, iter > 0 ? c.slot(cbuf ? U_CA0 : U_CA1) : nullptr, Ca, eps_a,
                                              c.slot(U_AG), c.slot(U_VG), c.slot(U_T), s_a, s_v, s_c, s_s, s_t, s_perm,
                                              s_pq, s_pq + MAXN / 2, s_red, tk + PH_EIG_IN);
-      sweeps += eigensolve<NT, A_LDS, V_LDS>(n, ld, Fb, iter > 0 ? c.slot(cbuf ? U_CB0 : U_CB1) : nullptr, Cb, eps_b,
+      sweeps += eigensolve<NT, MAXN, A_LDS, V_LDS>(n, ld, Fb, iter > 0 ? c.slot(cbuf ? U_CB0 : U_CB1) : nullptr, Cb, eps_b,
                                              c.slot(U_AG), c.slot(U_VG), c.slot(U_T), s_a, s_v, s_c, s_s, s_t, s_perm,
                                              s_pq, s_pq + MAXN / 2, s_red, tk + PH_EIG_IN);
       lap(PH_EIG);
@@ -1072,14 +809,14 @@ __global__ void __launch_bounds__(NT)
     int ca_last = cbuf ? U_CA0 : U_CA1, cb_last = cbuf ? U_CB0 : U_CB1;
     if (s_guard.shift > 0.0) {
       const int dst = (ca_last == U_CA0) ? U_CA1 : U_CA0;
-      sweeps += eigensolve<NT, A_LDS, V_LDS>(n, ld, Fa, c.slot(ca_last), c.slot(dst), eps_a, c.slot(U_AG), c.slot(U_VG),
+      sweeps += eigensolve<NT, MAXN, A_LDS, V_LDS>(n, ld, Fa, c.slot(ca_last), c.slot(dst), eps_a, c.slot(U_AG), c.slot(U_VG),
                                              c.slot(U_T), s_a, s_v, s_c, s_s, s_t, s_perm, s_pq,
                                              s_pq + MAXN / 2, s_red, tk + PH_EIG_IN);
       ca_last = dst;
     }
     {
       const int dst = (cb_last == U_CB0) ? U_CB1 : U_CB0;
-      sweeps += eigensolve<NT, A_LDS, V_LDS>(n, ld, Fb, c.slot(cb_last), c.slot(dst), eps_b, c.slot(U_AG), c.slot(U_VG),
+      sweeps += eigensolve<NT, MAXN, A_LDS, V_LDS>(n, ld, Fb, c.slot(cb_last), c.slot(dst), eps_b, c.slot(U_AG), c.slot(U_VG),
                                              c.slot(U_T), s_a, s_v, s_c, s_s, s_t, s_perm, s_pq,
                                              s_pq + MAXN / 2, s_red, tk + PH_EIG_IN);
       cb_last = dst;

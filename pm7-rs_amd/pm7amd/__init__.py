@@ -82,6 +82,8 @@ class _Result(C.Structure):
                 ("charges", C.POINTER(C.c_double)), ("message", C.c_char * 256)]
 
 
+TIMING_SLOTS = 12
+
 ABI_FUNCTIONS = [
     "pm7_ctx_create", "pm7_ctx_destroy", "pm7_last_error", "pm7_abi_version", "pm7_params_load",
     "pm7_default_options", "pm7_run", "pm7_run_batch", "pm7_result_free", "pm7_core_hamiltonian",
@@ -127,26 +129,44 @@ class Atom:
     position: tuple  # Bohr
 
 
-@dataclass
 class Molecule:
-    """pm7-rs ``Molecule`` (src/system.rs:20-39): atoms with Bohr positions."""
+    """pm7-rs ``Molecule`` (src/system.rs:20-39): atoms with Bohr positions.
 
-    atoms: list
-    charge: float = 0.0
-    multiplicity: int = 1
+    Held as the two arrays the C-ABI takes (atomic numbers int32[n], positions float64[n, 3] in
+    Bohr), so handing a molecule to the engine copies nothing; ``atoms`` is a derived view."""
+
+    __slots__ = ("z", "xyz", "charge", "multiplicity")
+
+    def __init__(self, atoms=(), charge: float = 0.0, multiplicity: int = 1, *, z=None, xyz=None):
+        if z is None:
+            atoms = list(atoms)
+            z = [a.z for a in atoms]
+            xyz = [a.position for a in atoms]
+        self.z = np.ascontiguousarray(z, dtype=np.int32).reshape(-1)
+        self.xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        if self.xyz.shape[0] != self.z.shape[0]:
+            raise Pm7Error(1, "atomic numbers and positions differ in length")
+        self.charge = charge
+        self.multiplicity = multiplicity
+
+    @classmethod
+    def from_arrays(cls, z, xyz_bohr, charge: float = 0.0, multiplicity: int = 1) -> "Molecule":
+        return cls(charge=charge, multiplicity=multiplicity, z=z, xyz=xyz_bohr)
+
+    @property
+    def atoms(self):
+        return [Atom(int(zz), tuple(float(v) for v in p)) for zz, p in zip(self.z, self.xyz)]
 
     @classmethod
     def from_symbols_angstrom(cls, atoms: Sequence, charge: float = 0.0, multiplicity: int = 1) -> "Molecule":
-        out = []
-        for sym, x, y, z in atoms:
-            zz = Z_OF[sym.lower()] if isinstance(sym, str) else int(sym)
-            out.append(Atom(zz, (x * ANGSTROM_TO_BOHR, y * ANGSTROM_TO_BOHR, z * ANGSTROM_TO_BOHR)))
-        return cls(out, charge, multiplicity)
+        z = [Z_OF[sym.lower()] if isinstance(sym, str) else int(sym) for sym, _, _, _ in atoms]
+        xyz = np.array([(x, y, zz) for _, x, y, zz in atoms], dtype=np.float64).reshape(-1, 3) * ANGSTROM_TO_BOHR
+        return cls(charge=charge, multiplicity=multiplicity, z=z, xyz=xyz)
 
     @classmethod
     def from_xyz_str(cls, text: str, charge: float = 0.0) -> "Molecule":
         """Plain XYZ (Angstrom) as pm7-rs ``Molecule::from_xyz_str`` reads it (src/system.rs:149-196)."""
-        text = text.lstrip("﻿")
+        text = text.lstrip("\ufeff")
         lines = text.splitlines()
         if not lines:
             raise Pm7Error(1, "empty XYZ")
@@ -168,9 +188,7 @@ class Molecule:
         return cls.from_symbols_angstrom(atoms, charge)
 
     def arrays(self):
-        z = np.ascontiguousarray([a.z for a in self.atoms], dtype=np.int32)
-        xyz = np.ascontiguousarray([a.position for a in self.atoms], dtype=np.float64).reshape(-1, 3)
-        return z, xyz
+        return self.z, self.xyz
 
 
 @dataclass
@@ -272,19 +290,24 @@ def _opt(x: float) -> Optional[float]:
     return None if math.isnan(x) else float(x)
 
 
+def _copy_out(ptr, k):
+    """Copy k doubles that the library allocated into a new numpy array (None for a null pointer)."""
+    addr = C.cast(ptr, C.c_void_p).value
+    if not addr:
+        return None
+    return np.frombuffer((C.c_double * k).from_address(addr), dtype=np.float64).copy()
+
+
 def _convert(r: _Result) -> Pm7Result:
     n = r.nao
     na = r.n_atoms
 
     def mat(ptr, rows, cols):
-        if not ptr:
-            return None
-        return np.ctypeslib.as_array(ptr, shape=(rows * cols,)).reshape(rows, cols).copy()
+        a = _copy_out(ptr, rows * cols)
+        return None if a is None else a.reshape(rows, cols)
 
     def vec(ptr, k):
-        if not ptr:
-            return None
-        return np.ctypeslib.as_array(ptr, shape=(k,)).copy()
+        return _copy_out(ptr, k)
 
     if not r.density:
         empty = np.zeros((0, 0))
@@ -423,9 +446,10 @@ class Engine:
         return e.value
 
     def last_timings_ms(self):
-        """[integrals, scf (all classes), gather, wall, scf class 0, 1, 2, 3] in ms."""
-        buf = (C.c_double * 8)()
-        self._lib.pm7_last_timings(self._h, buf, 8)
+        """[integrals, scf (all classes), gather, wall, scf class 0, 1, 2, 3, host prepare,
+        host launch, device wait, host result assembly] in ms."""
+        buf = (C.c_double * TIMING_SLOTS)()
+        self._lib.pm7_last_timings(self._h, buf, TIMING_SLOTS)
         return list(buf)
 
 

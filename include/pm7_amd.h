@@ -147,7 +147,8 @@ pm7_status pm7_core_core_energy(pm7_ctx* ctx, const pm7_molecule_v1* mol, double
  * the stream each kernel runs on: [0] integrals + H_core, [1] all SCF megakernels, [2] output
  * gather, [3] whole call (host wall clock), [4..7] SCF megakernel of size class 0/1/2/3 (<= 64,
  * <= 112, <= 196, <= 1024 basis functions; 0 when the class was empty), then host wall clock:
- * [8] validation, layout and upload, [9] kernel launches, [10] wait for the device,
+ * [8] validation, layout and upload, [9] kernel launches, host-side corrections and the copy of
+ * the packed results to the host (which waits for the kernels), [10] remaining device wait,
  * [11] result assembly. Writes min(n, PM7_TIMING_SLOTS) values. */
 #define PM7_TIMING_SLOTS 12
 void pm7_last_timings(const pm7_ctx* ctx, double* ms, int32_t n);

@@ -68,8 +68,8 @@ typedef struct {
   int64_t max_memory_mb; /* device-memory budget for this call; <= 0 means 80% of free HBM */
 } pm7_options_v1;
 
-/* pm7-rs Pm7Result (src/scf.rs:262-342), molecular fields. Arrays are allocated by the library
- * and released by pm7_result_free. */
+/* pm7-rs Pm7Result (src/scf.rs:262-342), molecular fields. The arrays of one result are allocated
+ * by the library (one allocation) and released together by pm7_result_free. */
 typedef struct {
   int32_t status;
   int32_t iterations;

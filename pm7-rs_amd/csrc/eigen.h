@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "mfma.h"
 #include "wg.h"
 
 namespace pm7 {
@@ -22,16 +23,21 @@ __device__ inline int pidx(int i, int j) {
 // update, so a round is one pass over the packed matrix. Returns the number of sweeps.
 //
 // The round is bound by LDS traffic and latency, so:
-//  * the m(m+1)/2 blocks are dealt out once per call as a folded triangle (row k beside row
-//    m-1-k, m+1 blocks per folded row), giving every thread the same count; a thread issues the
-//    loads of up to four of its blocks before any store (the blocks of a round are disjoint);
+//  * the m(m+1)/2 blocks are dealt out as a folded triangle (row k beside row m-1-k, m+1 blocks
+//    per folded row), giving every thread the same count; a thread issues the loads of CH of its
+//    blocks before any store (the blocks of a round are disjoint);
 //  * the rotation accumulator loads several rows before storing them (more so when V is in L2);
 //  * the rotation angle needs two square roots and two divisions:
 //      d = a_qq - a_pp, x = 2 a_pq, r = sqrt(d^2 + x^2), t = sgn(d) x / (|d| + r),
 //      c = 1 / sqrt(1 + t^2) = sqrt((|d| + r) / (2 r)), s = t c.
-template <int NT, int MAXN, bool V_LDS>
-__device__ int jacobi_core(int n2, double* A, double* V, double* s_c, double* s_s, double* s_t, int* s_p, int* s_q,
-                           double* red) {
+//
+// TA is the storage type of A: double, or float for the reduced-precision phase of a mixed run.
+// The angles are always formed in FP64 (from the stored values), and V always accumulates in
+// FP64, so the rotations stay orthogonal to FP64 rounding whatever TA is. The sweeps stop when
+// off-diagonal^2 <= tol2 * total^2, or after max_sweeps.
+template <int NT, int MAXN, bool V_LDS, typename TA>
+__device__ int jacobi_core(int n2, TA* A, double* V, double* s_c, double* s_s, double* s_t, int* s_p, int* s_q,
+                           double* red, double tol2, int max_sweeps) {
   constexpr int NX = 32, NY = NT / NX;  // 2-D thread grid: compile-time divisors only
   constexpr int CH = 2;                 // blocks in flight per thread
   constexpr int VR = V_LDS ? 4 : 8;     // accumulator rows in flight per thread
@@ -42,18 +48,18 @@ __device__ int jacobi_core(int n2, double* A, double* V, double* s_c, double* s_
   const int rstep = NT / w1, cstep = NT - rstep * w1;
   const int R0 = threadIdx.x / w1, C0 = threadIdx.x - R0 * w1;
   int sweep = 0;
-  for (; sweep < 40; ++sweep) {
+  for (; sweep < max_sweeps; ++sweep) {
     double sums[2] = {0.0, 0.0};
     for (int j = ty; j < n2; j += NY) {
-      const double* rowj = A + j * (j + 1) / 2;
+      const TA* rowj = A + j * (j + 1) / 2;
       for (int i = tx; i <= j; i += NX) {
-        const double v = rowj[i];
+        const double v = (double)rowj[i];
         if (i == j) sums[1] += v * v;
         else sums[0] += 2.0 * v * v;
       }
     }
     wg_sum_n<NT, 2>(sums, red);
-    if (sums[0] <= 1.0e-26 * (sums[0] + sums[1])) break;
+    if (sums[0] <= tol2 * (sums[0] + sums[1])) break;
     for (int r = 0; r < n2 - 1; ++r) {
       if (threadIdx.x < m) {
         // Round-robin pairing: position 0 stays, positions 1..n2-1 turn by one each round.
@@ -64,10 +70,10 @@ __device__ int jacobi_core(int n2, double* A, double* V, double* s_c, double* s_
         int b = n2 - 2 - k + r;
         b -= (b >= n2 - 1) ? n2 - 1 : 0;
         const int q = 1 + b;
-        const double apq = A[pidx(p, q)];
+        const double apq = (double)A[pidx(p, q)];
         double cc = 1.0, ss = 0.0, tt = 0.0;
         if (apq != 0.0) {
-          const double d = A[pidx(q, q)] - A[pidx(p, p)];
+          const double d = (double)A[pidx(q, q)] - (double)A[pidx(p, p)];
           const double x = 2.0 * apq;
           const double rr = sqrt(d * d + x * x);
           const double den = fabs(d) + rr;
@@ -86,7 +92,7 @@ __device__ int jacobi_core(int n2, double* A, double* V, double* s_c, double* s_
         int R = R0, col = C0;
         for (int lin0 = threadIdx.x; lin0 < total; lin0 += CH * NT) {
           int i0[CH], i1[CH], i2[CH], i3[CH];
-          double x0[CH], x1[CH], x2[CH], x3[CH], ck[CH], sk[CH], cl[CH], sl[CH];
+          TA x0[CH], x1[CH], x2[CH], x3[CH], ck[CH], sk[CH], cl[CH], sl[CH];
           bool act[CH], dg[CH];
 #pragma unroll
           for (int u = 0; u < CH; ++u) {
@@ -112,23 +118,23 @@ __device__ int jacobi_core(int n2, double* A, double* V, double* s_c, double* s_
             dg[u] = k == l;
             if (act[u]) {
               const int pk = s_p[k], qk = s_q[k], pl = s_p[l], ql = s_q[l];
-              ck[u] = s_c[k];
-              sk[u] = s_s[k];
-              cl[u] = s_c[l];
-              sl[u] = s_s[l];
+              ck[u] = (TA)s_c[k];
+              sk[u] = (TA)s_s[k];
+              cl[u] = (TA)s_c[l];
+              sl[u] = (TA)s_s[l];
               if (dg[u]) {
-                sl[u] = s_t[k];  // diagonal block: a_pp -= t a_pq, a_qq += t a_pq, a_pq = 0
+                sl[u] = (TA)s_t[k];  // diagonal block: a_pp -= t a_pq, a_qq += t a_pq, a_pq = 0
                 i0[u] = pidx(pk, qk);
                 i1[u] = pidx(pk, pk);
                 i2[u] = pidx(qk, qk);
                 i3[u] = i0[u];
-                act[u] = sl[u] != 0.0;
+                act[u] = sl[u] != (TA)0;
               } else {
                 i0[u] = pidx(pk, pl);
                 i1[u] = pidx(pk, ql);
                 i2[u] = pidx(qk, pl);
                 i3[u] = pidx(qk, ql);
-                act[u] = !(sk[u] == 0.0 && sl[u] == 0.0);
+                act[u] = !(sk[u] == (TA)0 && sl[u] == (TA)0);
               }
               if (act[u]) {
                 x0[u] = A[i0[u]];
@@ -142,13 +148,13 @@ __device__ int jacobi_core(int n2, double* A, double* V, double* s_c, double* s_
           for (int u = 0; u < CH; ++u) {
             if (!act[u]) continue;
             if (dg[u]) {
-              const double tk = sl[u];
+              const TA tk = sl[u];
               A[i1[u]] = x1[u] - tk * x0[u];
               A[i2[u]] = x2[u] + tk * x0[u];
-              A[i0[u]] = 0.0;
+              A[i0[u]] = (TA)0;
             } else {
-              const double y00 = ck[u] * x0[u] - sk[u] * x2[u], y01 = ck[u] * x1[u] - sk[u] * x3[u];
-              const double y10 = sk[u] * x0[u] + ck[u] * x2[u], y11 = sk[u] * x1[u] + ck[u] * x3[u];
+              const TA y00 = ck[u] * x0[u] - sk[u] * x2[u], y01 = ck[u] * x1[u] - sk[u] * x3[u];
+              const TA y10 = sk[u] * x0[u] + ck[u] * x2[u], y11 = sk[u] * x1[u] + ck[u] * x3[u];
               A[i0[u]] = cl[u] * y00 - sl[u] * y01;
               A[i1[u]] = sl[u] * y00 + cl[u] * y01;
               A[i2[u]] = cl[u] * y10 - sl[u] * y11;
@@ -191,60 +197,52 @@ __device__ int jacobi_core(int n2, double* A, double* V, double* s_c, double* s_
 // eps[0..n) and eigenvectors as the columns of Cout. With Cprev (the previous SCF iteration's
 // eigenvectors) the solve is warm-started: Jacobi diagonalises Cprev^T W Cprev, which is nearly
 // diagonal once the SCF settles, and Cout = Cprev V. A and V sit in LDS when the variant allows.
-template <int NT, int MAXN, bool A_LDS, bool V_LDS>
+// TA, tol2 and max_sweeps are passed to jacobi_core (the products around it are FP64 either way).
+template <int NT, int MAXN, bool A_LDS, bool V_LDS, typename TA>
 __device__ int eigensolve(int n, int ld, const double* __restrict__ W, const double* __restrict__ Cprev,
                           double* __restrict__ Cout, double* __restrict__ eps, double* Ag, double* Vg,
                           double* __restrict__ T, double* s_a, double* s_v, double* s_c, double* s_s, double* s_t,
-                          int* s_perm, int* s_p, int* s_q, double* red, unsigned long long* tsub) {
-  const int n2 = ld, nt = ld / 2;
+                          int* s_perm, int* s_p, int* s_q, double* red, unsigned long long* tsub, double tol2,
+                          int max_sweeps) {
+  const int n2 = ld;
   unsigned long long t0 = wall_clock64();
-  double* A = A_LDS ? s_a : Ag;
+  TA* A = reinterpret_cast<TA*>(A_LDS ? s_a : Ag);
   double* V = V_LDS ? s_v : Vg;
+  constexpr int NW = NT / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, q = lane >> 4;
+  const int nb = (n + 15) >> 4;
   if (Cprev) {
-    // T = W Cprev
-    for (int t = threadIdx.x; t < nt * nt; t += NT) {
-      const int I = t / nt, J = t - I * nt, i0 = 2 * I, j0 = 2 * J;
-      const double* w0 = W + (int64_t)i0 * ld;
-      const double* w1 = w0 + ld;
-      double s00 = 0.0, s01 = 0.0, s10 = 0.0, s11 = 0.0;
-      for (int k = 0; k < n; ++k) {
-        const double a0 = w0[k], a1 = w1[k];
-        const double b0 = Cprev[(int64_t)k * ld + j0], b1 = Cprev[(int64_t)k * ld + j0 + 1];
-        s00 += a0 * b0;
-        s01 += a0 * b1;
-        s10 += a1 * b0;
-        s11 += a1 * b1;
+    // T = W Cprev (W symmetric: W(i, k) read as W(k, i), so both operands stream along rows).
+    for (int t = wave; t < nb * nb; t += NW) {
+      const int I = t / nb, J = t - I * nb, i0 = 16 * I, j0 = 16 * J;
+      auto w_ik = [&](int i, int k) { return (i < n && k < n) ? W[(int64_t)k * ld + i] : 0.0; };
+      auto c_kj = [&](int k, int j) { return (k < n && j < n) ? Cprev[(int64_t)k * ld + j] : 0.0; };
+      const dbl4 x = tile16_f64(n, i0, j0, w_ik, c_kj);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = i0 + tile_row_f64(q, u), j = j0 + r;
+        if (i < n && j < n) T[(int64_t)i * ld + j] = x[u];
       }
-      T[(int64_t)i0 * ld + j0] = s00;
-      T[(int64_t)i0 * ld + j0 + 1] = s01;
-      T[(int64_t)(i0 + 1) * ld + j0] = s10;
-      T[(int64_t)(i0 + 1) * ld + j0 + 1] = s11;
     }
     __syncthreads();
-    // A = Cprev^T T, upper triangle only.
-    for (int t = threadIdx.x; t < nt * nt; t += NT) {
-      const int I = t / nt, J = t - I * nt;
+    // A = Cprev^T T, upper triangle (pad rows and columns up to n2 are zero).
+    for (int t = wave; t < nb * nb; t += NW) {
+      const int I = t / nb, J = t - I * nb;
       if (I > J) continue;
-      const int i0 = 2 * I, j0 = 2 * J;
-      double s00 = 0.0, s01 = 0.0, s10 = 0.0, s11 = 0.0;
-      for (int k = 0; k < n; ++k) {
-        const double* c = Cprev + (int64_t)k * ld;
-        const double* tr = T + (int64_t)k * ld;
-        const double a0 = c[i0], a1 = c[i0 + 1], b0 = tr[j0], b1 = tr[j0 + 1];
-        s00 += a0 * b0;
-        s01 += a0 * b1;
-        s10 += a1 * b0;
-        s11 += a1 * b1;
+      const int i0 = 16 * I, j0 = 16 * J;
+      auto c_ik = [&](int i, int k) { return (i < n && k < n) ? Cprev[(int64_t)k * ld + i] : 0.0; };
+      auto t_kj = [&](int k, int j) { return (k < n && j < n) ? T[(int64_t)k * ld + j] : 0.0; };
+      const dbl4 x = tile16_f64(n, i0, j0, c_ik, t_kj);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = i0 + tile_row_f64(q, u), j = j0 + r;
+        if (i <= j && j < n2) A[pidx(i, j)] = (i < n && j < n) ? (TA)x[u] : (TA)0;
       }
-      A[pidx(i0, j0)] = s00;
-      A[pidx(i0, j0 + 1)] = s01;
-      if (I != J) A[pidx(i0 + 1, j0)] = s10;
-      A[pidx(i0 + 1, j0 + 1)] = s11;
     }
   } else {
     constexpr int NX = 32, NY = NT / NX;
     for (int j = threadIdx.x / NX; j < n2; j += NY)
-      for (int i = threadIdx.x % NX; i <= j; i += NX) A[j * (j + 1) / 2 + i] = W[(int64_t)i * ld + j];
+      for (int i = threadIdx.x % NX; i <= j; i += NX) A[j * (j + 1) / 2 + i] = (TA)W[(int64_t)i * ld + j];
   }
   for (int idx = threadIdx.x; idx < n2 * n2; idx += NT) {
     const int i = idx / n2, j = idx - i * n2;
@@ -252,39 +250,31 @@ __device__ int eigensolve(int n, int ld, const double* __restrict__ W, const dou
   }
   __syncthreads();
   unsigned long long t1 = wall_clock64();
-  const int sweeps = jacobi_core<NT, MAXN, V_LDS>(n2, A, V, s_c, s_s, s_t, s_p, s_q, red);
+  const int sweeps = jacobi_core<NT, MAXN, V_LDS, TA>(n2, A, V, s_c, s_s, s_t, s_p, s_q, red, tol2, max_sweeps);
   unsigned long long t2 = wall_clock64();
   for (int i = threadIdx.x; i < n; i += NT) {
-    const double di = A[pidx(i, i)];
+    const double di = (double)A[pidx(i, i)];
     int rank = 0;
     for (int j = 0; j < n; ++j) {
-      const double dj = A[pidx(j, j)];
+      const double dj = (double)A[pidx(j, j)];
       rank += (dj < di || (dj == di && j < i)) ? 1 : 0;
     }
     s_perm[rank] = i;
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < n; k += NT) eps[k] = A[pidx(s_perm[k], s_perm[k])];
+  for (int k = threadIdx.x; k < n; k += NT) eps[k] = (double)A[pidx(s_perm[k], s_perm[k])];
   if (Cprev) {
-    // Cout = Cprev V(:, perm)
-    for (int t = threadIdx.x; t < nt * nt; t += NT) {
-      const int I = t / nt, J = t - I * nt, i0 = 2 * I, k0 = 2 * J;
-      const int c0 = k0 < n ? s_perm[k0] : -1, c1 = k0 + 1 < n ? s_perm[k0 + 1] : -1;
-      const double* a0p = Cprev + (int64_t)i0 * ld;
-      const double* a1p = a0p + ld;
-      double s00 = 0.0, s01 = 0.0, s10 = 0.0, s11 = 0.0;
-      for (int j = 0; j < n; ++j) {
-        const double a0 = a0p[j], a1 = a1p[j];
-        const double b0 = c0 >= 0 ? V[j * n2 + c0] : 0.0, b1 = c1 >= 0 ? V[j * n2 + c1] : 0.0;
-        s00 += a0 * b0;
-        s01 += a0 * b1;
-        s10 += a1 * b0;
-        s11 += a1 * b1;
+    // Cout = Cprev V(:, perm); pad rows and columns up to ld are written as zero.
+    for (int t = wave; t < nb * nb; t += NW) {
+      const int I = t / nb, J = t - I * nb, i0 = 16 * I, j0 = 16 * J;
+      auto c_ik = [&](int i, int k) { return (i < n && k < n) ? Cprev[(int64_t)i * ld + k] : 0.0; };
+      auto v_kj = [&](int k, int j) { return (k < n && j < n) ? V[k * n2 + s_perm[j]] : 0.0; };
+      const dbl4 x = tile16_f64(n, i0, j0, c_ik, v_kj);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = i0 + tile_row_f64(q, u), j = j0 + r;
+        if (i < ld && j < ld) Cout[(int64_t)i * ld + j] = (i < n && j < n) ? x[u] : 0.0;
       }
-      Cout[(int64_t)i0 * ld + k0] = s00;
-      Cout[(int64_t)i0 * ld + k0 + 1] = s01;
-      Cout[(int64_t)(i0 + 1) * ld + k0] = s10;
-      Cout[(int64_t)(i0 + 1) * ld + k0 + 1] = s11;
     }
   } else {
     for (int idx = threadIdx.x; idx < n2 * n2; idx += NT) {

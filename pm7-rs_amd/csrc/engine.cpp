@@ -102,6 +102,9 @@ struct pm7_ctx {
   DevBuf d_elem, d_pairpar, d_mols, d_list, d_atom_z, d_atom_ao, d_atom_pp, d_xyz, d_pairs, d_items, d_gath_off;
   DevBuf d_w, d_slab, d_vecs, d_out, d_ecore, d_ecore_mol, d_gath;
   double timings[PM7_TIMING_SLOTS] = {};
+  // Jacobi sweep cap per eigensolve in the reduced-precision phase of PRECISION_MIXED runs
+  // (environment PM7_AMD_MIXED_SWEEPS, 1..40).
+  int mixed_sweeps = 2;
   std::mutex mu;
 };
 
@@ -315,6 +318,7 @@ pm7_status prepare(pm7_ctx* ctx, Prepared& P, int64_t max_memory_mb) {
     M.shift_floor = j.o.level_shift;
     M.sad_yy = j.sad_yy;
     M.mixed_switch = j.o.mixed_switch;
+    M.mixed_sweeps = ctx->mixed_sweeps;
     lists[j.variant].push_back(mi);
     atom0 += n;
     pair0 += (int64_t)n * (n - 1) / 2;
@@ -421,10 +425,18 @@ void finish_result(pm7_ctx* ctx, const Job& j, const MolOut& R, double e_core, c
   for (int k = 0; k < 6; ++k) out->device_phase_ms[k] = (double)R.ticks[k] * 1.0e-5;  // 100 MHz clock
   for (int k = 0; k < 3; ++k) out->eig_phase_ms[k] = (double)R.ticks[PH_EIG_IN + k] * 1.0e-5;
   const size_t nn = (size_t)n * n;
-  out->density = (double*)std::malloc(nn * sizeof(double));
-  out->mo_coeff = (double*)std::malloc(nn * sizeof(double));
-  out->mo_energies = (double*)std::malloc(n * sizeof(double));
-  out->charges = (double*)std::malloc(nat * sizeof(double));
+  // One allocation per result, based at `density` (pm7_result_free releases it).
+  const size_t words = 2 * nn + n + nat + (j.uhf ? 2 * nn + n : 0);
+  double* block = (double*)std::malloc(words * sizeof(double));
+  if (!block) {
+    out->status = PM7_ERR_INSUFFICIENT_MEMORY;
+    set_msg(out, "host allocation of the result arrays failed");
+    return;
+  }
+  out->density = block;
+  out->mo_coeff = block + nn;
+  out->mo_energies = block + 2 * nn;
+  out->charges = block + 2 * nn + n;
   std::memcpy(out->mo_energies, eps_ab, n * sizeof(double));
   std::memcpy(out->mo_coeff, gath + nn, nn * sizeof(double));
   if (!j.uhf) {
@@ -432,13 +444,13 @@ void finish_result(pm7_ctx* ctx, const Job& j, const MolOut& R, double e_core, c
   } else {
     const double* pa = gath;
     const double* pb = gath + 2 * nn;
-    out->spin_density = (double*)std::malloc(nn * sizeof(double));
+    out->spin_density = out->charges + nat;
     for (size_t k = 0; k < nn; ++k) {
       out->density[k] = pa[k] + pb[k];
       out->spin_density[k] = pa[k] - pb[k];
     }
-    out->mo_energies_beta = (double*)std::malloc(n * sizeof(double));
-    out->mo_coeff_beta = (double*)std::malloc(nn * sizeof(double));
+    out->mo_energies_beta = out->spin_density + nn;
+    out->mo_coeff_beta = out->mo_energies_beta + n;
     std::memcpy(out->mo_energies_beta, eps_ab + j.ld, n * sizeof(double));
     std::memcpy(out->mo_coeff_beta, gath + 3 * nn, nn * sizeof(double));
   }
@@ -620,6 +632,10 @@ pm7_status pm7_ctx_create(int32_t device, const char* data_dir, pm7_ctx** out) {
   pm7_ctx* ctx = new pm7_ctx();
   ctx->device = device;
   ctx->data_dir = data_dir ? data_dir : default_data_dir();
+  if (const char* ms = std::getenv("PM7_AMD_MIXED_SWEEPS")) {
+    const int v = std::atoi(ms);
+    if (v >= 1 && v <= 40) ctx->mixed_sweeps = v;
+  }
   for (int v = 0; v < SCF_VARIANTS; ++v)
     if (hipStreamCreateWithFlags(&ctx->st[v], hipStreamNonBlocking) != hipSuccess) {
       delete ctx;
@@ -710,12 +726,10 @@ pm7_status pm7_run(pm7_ctx* ctx, const pm7_molecule_v1* mol, const pm7_options_v
 
 void pm7_result_free(pm7_result_v1* r) {
   if (!r) return;
+  std::free(r->density);  // the base of the result's single allocation
   double** arrays[] = {&r->density, &r->spin_density, &r->mo_energies, &r->mo_coeff,
                        &r->mo_energies_beta, &r->mo_coeff_beta, &r->charges};
-  for (double** a : arrays) {
-    std::free(*a);
-    *a = nullptr;
-  }
+  for (double** a : arrays) *a = nullptr;
 }
 
 pm7_status pm7_core_hamiltonian(pm7_ctx* ctx, const pm7_molecule_v1* mol, double* h_out, int32_t* nao_out) {

@@ -1,0 +1,63 @@
+// SPDX-License-Identifier: GPL-3.0-or-later
+//
+// 16x16 matrix-core tiles for the SCF megakernel's dense products (commutator, density, the
+// eigensolver's warm-start transform and back-transform).
+//
+// One wave computes one 16x16 output tile D = sum_k A(i, k) B(k, j) with
+// v_mfma_f64_16x16x4f64 (or v_mfma_f32_16x16x4f32 in the reduced-precision phase). Per
+// instruction, lane l (r = l % 16, q = l / 16) supplies A(i0 + r, k0 + q) and B(k0 + q, j0 + r);
+// the accumulator element t of lane l then holds
+//   FP64: D(q + 4t, r)        FP32: D(4q + t, r)
+// (checked on gfx950 with integer-valued inputs). Operands are fetched through caller lambdas
+// that return 0 outside the matrix, so ragged edges need no padding.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace pm7 {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+// Row of accumulator element t of this lane within the tile.
+__device__ inline int tile_row_f64(int q, int t) { return q + 4 * t; }
+__device__ inline int tile_row_f32(int q, int t) { return 4 * q + t; }
+
+template <class FA, class FB>
+__device__ inline dbl4 tile16_f64(int K, int i0, int j0, FA fa, FB fb) {
+  const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+  dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+  int k0 = 0;
+  // Two steps per trip: both pairs of loads are in flight before the first product.
+  for (; k0 + 4 < K; k0 += 8) {
+    const double a0 = fa(i0 + r, k0 + q), b0 = fb(k0 + q, j0 + r);
+    const double a1 = fa(i0 + r, k0 + 4 + q), b1 = fb(k0 + 4 + q, j0 + r);
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc, 0, 0, 0);
+  }
+  for (; k0 < K; k0 += 4) {
+    const double a0 = fa(i0 + r, k0 + q), b0 = fb(k0 + q, j0 + r);
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+template <class FA, class FB>
+__device__ inline flt4 tile16_f32(int K, int i0, int j0, FA fa, FB fb) {
+  const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+  flt4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  int k0 = 0;
+  for (; k0 + 4 < K; k0 += 8) {
+    const float a0 = fa(i0 + r, k0 + q), b0 = fb(k0 + q, j0 + r);
+    const float a1 = fa(i0 + r, k0 + 4 + q), b1 = fb(k0 + 4 + q, j0 + r);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc, 0, 0, 0);
+  }
+  for (; k0 < K; k0 += 4) {
+    const float a0 = fa(i0 + r, k0 + q), b0 = fb(k0 + q, j0 + r);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+}  // namespace pm7

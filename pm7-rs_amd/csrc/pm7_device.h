@@ -23,6 +23,7 @@ constexpr double KCAL_TO_EV = 1.0 / EV_TO_KCAL;
 constexpr int MAX_Z = 87;
 constexpr int NSLOT = 32;       // square matrices per molecule in the slab
 constexpr int HIST = 8;         // DIIS history depth
+constexpr int MIXED_MAX_ITERS = 30;  // longest reduced-precision phase of a mixed run
 
 // Slab slots, RHF.
 // C0/C1 double-buffer the sorted eigenvectors so each eigensolve can start from the last one;
@@ -53,6 +54,7 @@ struct MolDev {
   int n_atoms, nao, ld, n_occ, n_occ_b, uhf;
   int atom0;            // batch-global index of the first atom
   int max_scf, accel, use_diis, precision, n_pp;
+  int mixed_sweeps;     // Jacobi sweep cap per eigensolve in the reduced-precision phase
   int64_t pair0;        // first pair of the molecule
   int64_t mat0;         // slab offset (doubles)
   int64_t pp0;          // slab offset of the packed densities

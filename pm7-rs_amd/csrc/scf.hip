@@ -21,6 +21,7 @@
 
 #include "eigen.h"
 #include "kernels.h"
+#include "mfma.h"
 #include "pm7_device.h"
 #include "wg.h"
 
@@ -69,8 +70,10 @@ __device__ inline int find_atom(const int* atom_pp, int a0, int na, int t) {
   return lo;
 }
 
-// F^a = H + J(Pt) - xa K(Psa) (and F^b with Psb when given).
-template <int NT>
+// F^a = H + J(Pt) - xa K(Psa) (and F^b with Psb when given). T is the type the two-electron part
+// G(P) is formed in: FP64, or FP32 (integrals and densities rounded to FP32, FP32 accumulation)
+// in the reduced-precision phase of a mixed run; H is added in FP64 either way.
+template <int NT, typename T>
 __device__ void fock_build(const MolCtx& c, const double* __restrict__ Pt, const double* __restrict__ Psa, double xa,
                            const double* __restrict__ Psb, double xb, double* __restrict__ Fa, double* __restrict__ Fb) {
   const MolDev& M = c.M;
@@ -84,7 +87,7 @@ __device__ void fock_build(const MolCtx& c, const double* __restrict__ Pt, const
     int mu, nu;
     unpack_e(e, mu, nu);
     int o = c.atom_ao[a];
-    c.pp[t] = Pt[(int64_t)(o + mu) * ld + o + nu] * (mu == nu ? 1.0 : 2.0);
+    c.pp[t] = (double)(T)(Pt[(int64_t)(o + mu) * ld + o + nu] * (mu == nu ? 1.0 : 2.0));
   }
   __syncthreads();
   // Diagonal blocks: two-centre Coulomb gather + one-centre J - K.
@@ -96,7 +99,7 @@ __device__ void fock_build(const MolCtx& c, const double* __restrict__ Pt, const
     unpack_e(e, mu, nu);
     const ElemDev ea = c.elem[c.atom_z[a]];
     const int npa = npair_of(ea.n_orb);
-    double acc = 0.0;
+    T acc = (T)0;
     for (int lb = 0; lb < M.n_atoms; ++lb) {
       if (lb == la) continue;
       int u = la < lb ? la : lb, v = la < lb ? lb : la;
@@ -108,28 +111,28 @@ __device__ void fock_build(const MolCtx& c, const double* __restrict__ Pt, const
       const double* pb = c.pp + ppb;
       if (P.f == a) {
         const double* row = wp + (int64_t)e * npb;
-        for (int kl = 0; kl < npb; ++kl) acc += row[kl] * pb[kl];
+        for (int kl = 0; kl < npb; ++kl) acc += (T)row[kl] * (T)pb[kl];
       } else {
-        for (int kl = 0; kl < npb; ++kl) acc += wp[(int64_t)kl * npa + e] * pb[kl];
+        for (int kl = 0; kl < npb; ++kl) acc += (T)wp[(int64_t)kl * npa + e] * (T)pb[kl];
       }
     }
     const int o = c.atom_ao[a];
     const int n = ea.n_orb;
-    double ja = 0.0, ka = 0.0, kb = 0.0;
+    T ja = (T)0, ka = (T)0, kb = (T)0;
     for (int l = 0; l < n; ++l)
       for (int s = 0; s < n; ++s) {
         int64_t ix = (int64_t)(o + l) * ld + o + s;
-        ja += Pt[ix] * oc_int(mu, nu, l, s, ea);
-        double gx = oc_int(mu, l, nu, s, ea);
-        ka += Psa[ix] * gx;
-        if (Fb) kb += Psb[ix] * gx;
+        ja += (T)Pt[ix] * (T)oc_int(mu, nu, l, s, ea);
+        const T gx = (T)oc_int(mu, l, nu, s, ea);
+        ka += (T)Psa[ix] * gx;
+        if (Fb) kb += (T)Psb[ix] * gx;
       }
     const int64_t i1 = (int64_t)(o + mu) * ld + o + nu, i2 = (int64_t)(o + nu) * ld + o + mu;
-    double fa = H[i1] + acc + ja - xa * ka;
+    double fa = H[i1] + (double)(acc + ja - (T)xa * ka);
     Fa[i1] = fa;
     Fa[i2] = fa;
     if (Fb) {
-      double fb = H[i1] + acc + ja - xb * kb;
+      double fb = H[i1] + (double)(acc + ja - (T)xb * kb);
       Fb[i1] = fb;
       Fb[i2] = fb;
     }
@@ -144,21 +147,21 @@ __device__ void fock_build(const MolCtx& c, const double* __restrict__ Pt, const
     const int of = c.atom_ao[P.f], os = c.atom_ao[P.s];
     const double* wp = c.w + P.w_off;
     // Same-spin density block held in registers: every index below is a compile-time constant.
-    double Xa[4][4], Xb[4][4];
+    T Xa[4][4], Xb[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const bool in = i < nf && j < ns;
-        Xa[i][j] = in ? Psa[(int64_t)(of + i) * ld + os + j] : 0.0;
-        Xb[i][j] = (in && Fb) ? Psb[(int64_t)(of + i) * ld + os + j] : 0.0;
+        Xa[i][j] = in ? (T)Psa[(int64_t)(of + i) * ld + os + j] : (T)0;
+        Xb[i][j] = (in && Fb) ? (T)Psb[(int64_t)(of + i) * ld + os + j] : (T)0;
       }
 #pragma unroll
     for (int mu = 0; mu < 4; ++mu)
 #pragma unroll
       for (int lam = 0; lam < 4; ++lam) {
         if (mu >= nf || lam >= ns) continue;
-        double acca = 0.0, accb = 0.0;
+        T acca = (T)0, accb = (T)0;
 #pragma unroll
         for (int nu = 0; nu < 4; ++nu) {
           if (nu >= nf) continue;
@@ -166,17 +169,17 @@ __device__ void fock_build(const MolCtx& c, const double* __restrict__ Pt, const
 #pragma unroll
           for (int sg = 0; sg < 4; ++sg) {
             if (sg >= ns) continue;
-            const double wv = row[pack_idx(lam, sg)];
+            const T wv = (T)row[pack_idx(lam, sg)];
             acca += Xa[nu][sg] * wv;
             accb += Xb[nu][sg] * wv;
           }
         }
         const int64_t i1 = (int64_t)(of + mu) * ld + os + lam, i2 = (int64_t)(os + lam) * ld + of + mu;
-        const double fa = H[i1] - xa * acca;
+        const double fa = H[i1] - (double)((T)xa * acca);
         Fa[i1] = fa;
         Fa[i2] = fa;
         if (Fb) {
-          const double fb = H[i1] - xb * accb;
+          const double fb = H[i1] - (double)((T)xb * accb);
           Fb[i1] = fb;
           Fb[i2] = fb;
         }
@@ -196,86 +199,91 @@ __device__ double frob(int n, int ld, const double* X, const double* Y, double* 
   return wg_sum<NT>(acc, red);
 }
 
-// ||FP - (FP)^T||_F^2; the commutator itself is stored into E when E != nullptr.
+// ||FP - (FP)^T||_F^2; the commutator itself is stored into E when E != nullptr. F and P are
+// symmetric, so (FP)^T = PF and E = FP - PF tile by tile on the matrix cores; both products read
+// their operands along rows (X(i, k) = X(k, i)). Each wave owns the upper-triangle tiles
+// wave, wave + NW, ... and writes E(i, j) and E(j, i) = -E(i, j).
 template <int NT>
 __device__ double commutator(int n, int ld, const double* __restrict__ F, const double* __restrict__ P,
                              double* __restrict__ E, double* red) {
-  const int nt = ld / 2;
+  constexpr int NW = NT / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, q = lane >> 4;
+  const int nb = (n + 15) >> 4;
   double nrm = 0.0;
-  for (int t = threadIdx.x; t < nt * nt; t += NT) {
-    int I = t / nt, J = t - I * nt;
+  for (int t = wave; t < nb * nb; t += NW) {
+    const int I = t / nb, J = t - I * nb;
     if (I > J) continue;
-    const int i0 = 2 * I, j0 = 2 * J;
-    double tij[2][2] = {{0, 0}, {0, 0}}, tji[2][2] = {{0, 0}, {0, 0}};
-    const double* F0 = F + (int64_t)i0 * ld;
-    const double* F1 = F0 + ld;
-    const double* G0 = F + (int64_t)j0 * ld;
-    const double* G1 = G0 + ld;
-    const double* P0 = P + (int64_t)i0 * ld;
-    const double* P1 = P0 + ld;
-    const double* Q0 = P + (int64_t)j0 * ld;
-    const double* Q1 = Q0 + ld;
-    for (int k = 0; k < n; ++k) {
-      double f0 = F0[k], f1 = F1[k], g0 = G0[k], g1 = G1[k];
-      double p0 = P0[k], p1 = P1[k], q0 = Q0[k], q1 = Q1[k];
-      tij[0][0] += f0 * q0;
-      tij[0][1] += f0 * q1;
-      tij[1][0] += f1 * q0;
-      tij[1][1] += f1 * q1;
-      tji[0][0] += g0 * p0;
-      tji[0][1] += g0 * p1;
-      tji[1][0] += g1 * p0;
-      tji[1][1] += g1 * p1;
-    }
-    double part = 0.0;
-    for (int a = 0; a < 2; ++a)
-      for (int b = 0; b < 2; ++b) {
-        double e = tij[a][b] - tji[b][a];
-        part += e * e;
+    const int i0 = 16 * I, j0 = 16 * J;
+    auto f_ik = [&](int i, int k) { return (i < n && k < n) ? F[(int64_t)k * ld + i] : 0.0; };
+    auto p_kj = [&](int k, int j) { return (k < n && j < n) ? P[(int64_t)k * ld + j] : 0.0; };
+    auto p_ik = [&](int i, int k) { return (i < n && k < n) ? P[(int64_t)k * ld + i] : 0.0; };
+    auto f_kj = [&](int k, int j) { return (k < n && j < n) ? F[(int64_t)k * ld + j] : 0.0; };
+    const dbl4 fp = tile16_f64(n, i0, j0, f_ik, p_kj);
+    const dbl4 pf = tile16_f64(n, i0, j0, p_ik, f_kj);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + tile_row_f64(q, u), j = j0 + r;
+      if (i >= n || j >= n) continue;
+      const double e = fp[u] - pf[u];
+      if (I == J) {
+        nrm += e * e;
+        if (E) E[(int64_t)i * ld + j] = e;
+      } else {
+        nrm += 2.0 * e * e;
         if (E) {
-          E[(int64_t)(i0 + a) * ld + j0 + b] = e;
-          E[(int64_t)(j0 + b) * ld + i0 + a] = -e;
+          E[(int64_t)i * ld + j] = e;
+          E[(int64_t)j * ld + i] = -e;
         }
       }
-    nrm += (I == J) ? part : 2.0 * part;
+    }
   }
   return wg_sum<NT>(nrm, red);
 }
 
-// P' = w C_occ C_occ^T (C columns sorted ascending); returns sum (P' - P)^2 over n x n.
-template <int NT>
+// P' = w C_occ C_occ^T (C columns sorted ascending) on the matrix cores, upper-triangle tiles
+// mirrored; returns sum (P' - P)^2 over n x n. T is the precision of the product: FP64, or FP32
+// inputs with FP32 accumulation in the reduced-precision phase.
+template <int NT, typename T>
 __device__ double density(int n, int ld, int nocc, double wgt, const double* __restrict__ C,
                           const double* __restrict__ Pold, double* __restrict__ Pn, double* red) {
-  const int nt = ld / 2;
+  constexpr int NW = NT / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, q = lane >> 4;
+  const int nb = (n + 15) >> 4;
   double dsq = 0.0;
-  for (int t = threadIdx.x; t < nt * nt; t += NT) {
-    int I = t / nt, J = t - I * nt;
+  for (int t = wave; t < nb * nb; t += NW) {
+    const int I = t / nb, J = t - I * nb;
     if (I > J) continue;
-    const int i0 = 2 * I, j0 = 2 * J;
-    const double* A0 = C + (int64_t)i0 * ld;
-    const double* A1 = A0 + ld;
-    const double* B0 = C + (int64_t)j0 * ld;
-    const double* B1 = B0 + ld;
-    double s[2][2] = {{0, 0}, {0, 0}};
-    for (int k = 0; k < nocc; ++k) {
-      double a0 = A0[k], a1 = A1[k], b0 = B0[k], b1 = B1[k];
-      s[0][0] += a0 * b0;
-      s[0][1] += a0 * b1;
-      s[1][0] += a1 * b0;
-      s[1][1] += a1 * b1;
-    }
-    double part = 0.0;
-    for (int a = 0; a < 2; ++a)
-      for (int b = 0; b < 2; ++b) {
-        const int i = i0 + a, j = j0 + b;
-        double v = (i < n && j < n) ? wgt * s[a][b] : 0.0;
-        Pn[(int64_t)i * ld + j] = v;
-        Pn[(int64_t)j * ld + i] = v;
-        double d = v - Pold[(int64_t)i * ld + j];
-        part += d * d;
+    const int i0 = 16 * I, j0 = 16 * J;
+    double v[4];
+    int rows[4];
+    if constexpr (sizeof(T) == 8) {
+      auto c_ik = [&](int i, int k) { return (i < n && k < nocc) ? C[(int64_t)i * ld + k] : 0.0; };
+      auto c_kj = [&](int k, int j) { return (k < nocc && j < n) ? C[(int64_t)j * ld + k] : 0.0; };
+      const dbl4 s = tile16_f64(nocc, i0, j0, c_ik, c_kj);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        v[u] = wgt * s[u];
+        rows[u] = tile_row_f64(q, u);
       }
-
-    dsq += (I == J) ? part : 2.0 * part;
+    } else {
+      auto c_ik = [&](int i, int k) { return (i < n && k < nocc) ? (float)C[(int64_t)i * ld + k] : 0.0f; };
+      auto c_kj = [&](int k, int j) { return (k < nocc && j < n) ? (float)C[(int64_t)j * ld + k] : 0.0f; };
+      const flt4 s = tile16_f32(nocc, i0, j0, c_ik, c_kj);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        v[u] = wgt * (double)s[u];
+        rows[u] = tile_row_f32(q, u);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + rows[u], j = j0 + r;
+      if (i >= n || j >= n) continue;
+      Pn[(int64_t)i * ld + j] = v[u];
+      if (I != J) Pn[(int64_t)j * ld + i] = v[u];
+      const double d = v[u] - Pold[(int64_t)i * ld + j];
+      dsq += (I == J) ? d * d : 2.0 * d * d;
+    }
   }
   return wg_sum<NT>(dsq, red);
 }
@@ -519,7 +527,7 @@ __global__ void __launch_bounds__(NT)
   __shared__ double s_G[HIST * HIST], s_M[HIST * HIST], s_coef[HIST + 1];
   __shared__ int s_slot[HIST], s_cslot[HIST];
   __shared__ double s_wk[(HIST + 1) * (HIST + 3)];  // bordered DIIS system
-  __shared__ int s_int[8];     // 0 head, 1 cnt, 2 ncoef, 3 done
+  __shared__ int s_int[8];     // 0 head, 1 cnt, 2 ncoef, 3 done, 4 reduced-precision phase
   __shared__ double s_dbl[8];  // 0 e_old, 1 err, 2 dp
   __shared__ Guard s_guard;
 
@@ -566,6 +574,17 @@ __global__ void __launch_bounds__(NT)
   res.err_norm = 0.0;
   res.fp32_iters = 0;
   res.e_fp32_last = 0.0;
+  if (threadIdx.x == 0) s_int[4] = c.M.precision == 1 ? 1 : 0;
+  __syncthreads();
+  // One FP64 eigensolve: to full accuracy, or, in the reduced-precision phase of a mixed run, to
+  // the accuracy the FP32 density supports (off-diagonal norm 1e-7 of the total) in at most
+  // mixed_sweeps sweeps.
+  auto eig = [&](bool low, const double* Wm, const double* Cp, double* Co, double* ev, int ag, int vg, int tg) -> int {
+    return eigensolve<NT, MAXN, A_LDS, V_LDS, double>(n, ld, Wm, Cp, Co, ev, c.slot(ag), c.slot(vg), c.slot(tg), s_a,
+                                                      s_v, s_c, s_s, s_t, s_perm, s_pq, s_pq + MAXN / 2, s_red,
+                                                      tk + PH_EIG_IN, low ? 1.0e-14 : 1.0e-26,
+                                                      low ? c.M.mixed_sweeps : 40);
+  };
 
   if (!c.M.uhf) {
     // ------------------------------------------------------------------ RHF
@@ -578,7 +597,8 @@ __global__ void __launch_bounds__(NT)
       double* P = c.slot(S_HD + h);
       double* F = c.slot(S_HF + h);
       double* E = c.slot(S_HE + h);
-      fock_build<NT>(c, P, P, 0.5, nullptr, 0.0, F, nullptr);
+      if (s_int[4]) fock_build<NT, float>(c, P, P, 0.5, nullptr, 0.0, F, nullptr);
+      else fock_build<NT, double>(c, P, P, 0.5, nullptr, 0.0, F, nullptr);
       lap(PH_FOCK);
       double eh[2] = {0.0, 0.0};
       for (int idx = threadIdx.x; idx < n * n; idx += NT) {
@@ -661,12 +681,12 @@ __global__ void __launch_bounds__(NT)
       level_shift<NT>(n, ld, W, P, 2.0, s_guard.shift);
       lap(PH_DIIS);
       double* Cnew = c.slot(cbuf ? S_C1 : S_C0);
-      sweeps += eigensolve<NT, MAXN, A_LDS, V_LDS>(n, ld, W, iter > 0 ? c.slot(cbuf ? S_C0 : S_C1) : nullptr, Cnew, eps_a,
-                                             c.slot(S_AG), c.slot(S_VG), c.slot(S_T), s_a, s_v, s_c, s_s, s_t, s_perm,
-                                             s_pq, s_pq + MAXN / 2, s_red, tk + PH_EIG_IN);
+      const bool low = s_int[4] != 0;
+      sweeps += eig(low, W, iter > 0 ? c.slot(cbuf ? S_C0 : S_C1) : nullptr, Cnew, eps_a, S_AG, S_VG, S_T);
       lap(PH_EIG);
       double* Pn = c.slot(S_HD + s_int[0]);
-      const double dsq = density<NT>(n, ld, nocc, 2.0, Cnew, P, Pn, s_red);
+      const double dsq = low ? density<NT, float>(n, ld, nocc, 2.0, Cnew, P, Pn, s_red)
+                             : density<NT, double>(n, ld, nocc, 2.0, Cnew, P, Pn, s_red);
       lap(PH_DENS);
       cbuf ^= 1;
       if (threadIdx.x == 0) {
@@ -675,7 +695,14 @@ __global__ void __launch_bounds__(NT)
         s_dbl[0] = e;
         s_dbl[2] = dp;
         const bool resid_ok = s_guard.shift == 0.0 && err < 1.0e-7;
-        if (iter > 0 && de < c.M.e_tol && (dp < c.M.p_tol || resid_ok)) {
+        const bool conv = iter > 0 && de < c.M.e_tol && (dp < c.M.p_tol || resid_ok);
+        if (low) {
+          // Reduced-precision phase: never converges here; hands over to FP64 below the switch.
+          ++res.fp32_iters;
+          res.e_fp32_last = e;
+          if (conv || dp < c.M.mixed_switch || res.fp32_iters >= MIXED_MAX_ITERS) s_int[4] = 0;
+          else guard_update(s_guard, dp, e);
+        } else if (conv) {
           s_int[3] = 1;
         } else {
           guard_update(s_guard, dp, e);
@@ -695,7 +722,7 @@ __global__ void __launch_bounds__(NT)
     const int pslot = S_HD + s_int[0];
     double* P = c.slot(pslot);
     double* F = c.slot(S_F);
-    fock_build<NT>(c, P, P, 0.5, nullptr, 0.0, F, nullptr);
+    fock_build<NT, double>(c, P, P, 0.5, nullptr, 0.0, F, nullptr);
     double eh[2] = {0.0, 0.0};
     for (int idx = threadIdx.x; idx < n * n; idx += NT) {
       int i = idx / n, j = idx - i * n;
@@ -706,9 +733,7 @@ __global__ void __launch_bounds__(NT)
     wg_sum_n<NT, 2>(eh, s_red);
     res.e_el = 0.5 * (eh[0] + eh[1]);
     if (s_guard.shift > 0.0) {
-      sweeps += eigensolve<NT, MAXN, A_LDS, V_LDS>(n, ld, F, c.slot(cbuf ? S_C0 : S_C1), c.slot(cbuf ? S_C1 : S_C0), eps_a,
-                                             c.slot(S_AG), c.slot(S_VG), c.slot(S_T), s_a, s_v, s_c, s_s, s_t, s_perm,
-                                             s_pq, s_pq + MAXN / 2, s_red, tk + PH_EIG_IN);
+      sweeps += eig(false, F, c.slot(cbuf ? S_C0 : S_C1), c.slot(cbuf ? S_C1 : S_C0), eps_a, S_AG, S_VG, S_T);
       cbuf ^= 1;
     }
     res.p_slot = pslot;
@@ -733,7 +758,8 @@ __global__ void __launch_bounds__(NT)
     for (; iter < c.M.max_scf; ++iter) {
       for (int idx = threadIdx.x; idx < ld * ld; idx += NT) Pt[idx] = Pa[idx] + Pb[idx];
       __syncthreads();
-      fock_build<NT>(c, Pt, Pa, 1.0, Pb, 1.0, Fa, Fb);
+      if (s_int[4]) fock_build<NT, float>(c, Pt, Pa, 1.0, Pb, 1.0, Fa, Fb);
+      else fock_build<NT, double>(c, Pt, Pa, 1.0, Pb, 1.0, Fa, Fb);
       lap(PH_FOCK);
       double eh[3] = {0.0, 0.0, 0.0};
       for (int idx = threadIdx.x; idx < n * n; idx += NT) {
@@ -754,15 +780,14 @@ __global__ void __launch_bounds__(NT)
       level_shift<NT>(n, ld, Fb, Pb, 1.0, shift);
       double* Ca = c.slot(cbuf ? U_CA1 : U_CA0);
       double* Cb = c.slot(cbuf ? U_CB1 : U_CB0);
-      sweeps += eigensolve<NT, MAXN, A_LDS, V_LDS>(n, ld, Fa, iter > 0 ? c.slot(cbuf ? U_CA0 : U_CA1) : nullptr, Ca, eps_a,
-                                             c.slot(U_AG), c.slot(U_VG), c.slot(U_T), s_a, s_v, s_c, s_s, s_t, s_perm,
-                                             s_pq, s_pq + MAXN / 2, s_red, tk + PH_EIG_IN);
-      sweeps += eigensolve<NT, MAXN, A_LDS, V_LDS>(n, ld, Fb, iter > 0 ? c.slot(cbuf ? U_CB0 : U_CB1) : nullptr, Cb, eps_b,
-                                             c.slot(U_AG), c.slot(U_VG), c.slot(U_T), s_a, s_v, s_c, s_s, s_t, s_perm,
-                                             s_pq, s_pq + MAXN / 2, s_red, tk + PH_EIG_IN);
+      const bool low = s_int[4] != 0;
+      sweeps += eig(low, Fa, iter > 0 ? c.slot(cbuf ? U_CA0 : U_CA1) : nullptr, Ca, eps_a, U_AG, U_VG, U_T);
+      sweeps += eig(low, Fb, iter > 0 ? c.slot(cbuf ? U_CB0 : U_CB1) : nullptr, Cb, eps_b, U_AG, U_VG, U_T);
       lap(PH_EIG);
-      const double da = density<NT>(n, ld, na, 1.0, Ca, Pa, Pan, s_red);
-      const double db = density<NT>(n, ld, nb, 1.0, Cb, Pb, Pbn, s_red);
+      const double da = low ? density<NT, float>(n, ld, na, 1.0, Ca, Pa, Pan, s_red)
+                            : density<NT, double>(n, ld, na, 1.0, Ca, Pa, Pan, s_red);
+      const double db = low ? density<NT, float>(n, ld, nb, 1.0, Cb, Pb, Pbn, s_red)
+                            : density<NT, double>(n, ld, nb, 1.0, Cb, Pb, Pbn, s_red);
       lap(PH_DENS);
       cbuf ^= 1;
       if (threadIdx.x == 0) {
@@ -773,8 +798,17 @@ __global__ void __launch_bounds__(NT)
         s_dbl[2] = dp;
         s_dbl[1] = err;
         const bool resid_ok = s_guard.shift == 0.0 && err < 1.0e-7;
-        if (iter > 0 && de < c.M.e_tol && (dp < c.M.p_tol || resid_ok)) s_int[3] = 1;
-        else guard_update(s_guard, dp, e);
+        const bool conv = iter > 0 && de < c.M.e_tol && (dp < c.M.p_tol || resid_ok);
+        if (low) {
+          ++res.fp32_iters;
+          res.e_fp32_last = e;
+          if (conv || dp < c.M.mixed_switch || res.fp32_iters >= MIXED_MAX_ITERS) s_int[4] = 0;
+          else guard_update(s_guard, dp, e);
+        } else if (conv) {
+          s_int[3] = 1;
+        } else {
+          guard_update(s_guard, dp, e);
+        }
       }
       __syncthreads();
       double* tmp = Pa;
@@ -794,7 +828,7 @@ __global__ void __launch_bounds__(NT)
     res.err_norm = s_dbl[1];
     for (int idx = threadIdx.x; idx < ld * ld; idx += NT) Pt[idx] = Pa[idx] + Pb[idx];
     __syncthreads();
-    fock_build<NT>(c, Pt, Pa, 1.0, Pb, 1.0, Fa, Fb);
+    fock_build<NT, double>(c, Pt, Pa, 1.0, Pb, 1.0, Fa, Fb);
     double eh[3] = {0.0, 0.0, 0.0};
     for (int idx = threadIdx.x; idx < n * n; idx += NT) {
       int i = idx / n, j = idx - i * n;
@@ -809,16 +843,12 @@ __global__ void __launch_bounds__(NT)
     int ca_last = cbuf ? U_CA0 : U_CA1, cb_last = cbuf ? U_CB0 : U_CB1;
     if (s_guard.shift > 0.0) {
       const int dst = (ca_last == U_CA0) ? U_CA1 : U_CA0;
-      sweeps += eigensolve<NT, MAXN, A_LDS, V_LDS>(n, ld, Fa, c.slot(ca_last), c.slot(dst), eps_a, c.slot(U_AG), c.slot(U_VG),
-                                             c.slot(U_T), s_a, s_v, s_c, s_s, s_t, s_perm, s_pq,
-                                             s_pq + MAXN / 2, s_red, tk + PH_EIG_IN);
+      sweeps += eig(false, Fa, c.slot(ca_last), c.slot(dst), eps_a, U_AG, U_VG, U_T);
       ca_last = dst;
     }
     {
       const int dst = (cb_last == U_CB0) ? U_CB1 : U_CB0;
-      sweeps += eigensolve<NT, MAXN, A_LDS, V_LDS>(n, ld, Fb, c.slot(cb_last), c.slot(dst), eps_b, c.slot(U_AG), c.slot(U_VG),
-                                             c.slot(U_T), s_a, s_v, s_c, s_s, s_t, s_perm, s_pq,
-                                             s_pq + MAXN / 2, s_red, tk + PH_EIG_IN);
+      sweeps += eig(false, Fb, c.slot(cb_last), c.slot(dst), eps_b, U_AG, U_VG, U_T);
       cb_last = dst;
     }
     res.p_slot = (Pa == c.slot(U_PA)) ? U_PA : U_PAN;
@@ -863,7 +893,7 @@ __global__ void __launch_bounds__(256) k_fock_once(int mi, const MolDev* __restr
   c.ld = c.M.ld;
   c.base = slab + c.M.mat0;
   c.pp = slab + c.M.pp0;
-  fock_build<256>(c, c.slot(pt_slot), c.slot(ps_slot), 1.0, nullptr, 0.0, c.slot(f_slot), nullptr);
+  fock_build<256, double>(c, c.slot(pt_slot), c.slot(ps_slot), 1.0, nullptr, 0.0, c.slot(f_slot), nullptr);
 }
 
 hipError_t launch_scf(int variant, const BatchDev& b, int list_off, int count, hipStream_t s) {

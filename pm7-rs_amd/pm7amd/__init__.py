@@ -290,50 +290,95 @@ def _opt(x: float) -> Optional[float]:
     return None if math.isnan(x) else float(x)
 
 
-def _copy_out(ptr, k):
-    """Copy k doubles that the library allocated into a new numpy array (None for a null pointer)."""
-    addr = C.cast(ptr, C.c_void_p).value
+class _ResultArrays:
+    """Owns one result's library-allocated arrays; frees them when the last numpy view goes."""
+
+    __slots__ = ("_free", "_res")
+
+    def __init__(self, free, res):
+        self._free = free
+        self._res = res
+
+    def __del__(self):
+        self._free(C.byref(self._res))
+
+
+def _view(owner, addr, k):
+    """Zero-copy numpy view of k doubles that the library allocated at `addr` (None for 0)."""
     if not addr:
         return None
-    return np.frombuffer((C.c_double * k).from_address(addr), dtype=np.float64).copy()
+    buf = (C.c_double * k).from_address(addr)
+    buf._owner = owner
+    return np.frombuffer(buf, dtype=np.float64)
 
 
-def _convert(r: _Result) -> Pm7Result:
-    n = r.nao
-    na = r.n_atoms
+def _struct_dtype(cls):
+    """numpy dtype with the exact field offsets of a ctypes Structure (pointers as addresses)."""
+    names, formats, offsets = [], [], []
+    for name, tp in cls._fields_:
+        if issubclass(tp, C._Pointer):
+            fmt = np.uintp
+        elif issubclass(tp, C.Array):
+            fmt = ("S%d" % tp._length_) if tp._type_ is C.c_char else (np.dtype(tp._type_), (tp._length_,))
+        else:
+            fmt = np.dtype(tp)
+        names.append(name)
+        formats.append(fmt)
+        offsets.append(getattr(cls, name).offset)
+    return np.dtype({"names": names, "formats": formats, "offsets": offsets, "itemsize": C.sizeof(cls)})
 
-    def mat(ptr, rows, cols):
-        a = _copy_out(ptr, rows * cols)
-        return None if a is None else a.reshape(rows, cols)
 
-    def vec(ptr, k):
-        return _copy_out(ptr, k)
+_RESULT_DTYPE = _struct_dtype(_Result)
 
-    if not r.density:
-        empty = np.zeros((0, 0))
-        dens = empty
-    else:
-        dens = mat(r.density, n, n)
-    return Pm7Result(
-        status=r.status, density=dens, spin_density=mat(r.spin_density, n, n),
-        mo_energies=vec(r.mo_energies, n) if r.mo_energies else np.zeros(0),
-        mo_coeff=mat(r.mo_coeff, n, n) if r.mo_coeff else np.zeros((0, 0)),
-        n_occ=r.n_occ, mo_energies_beta=vec(r.mo_energies_beta, n), mo_coeff_beta=mat(r.mo_coeff_beta, n, n),
-        n_occ_beta=r.n_occ_beta if r.n_occ_beta >= 0 else None,
-        electronic_ev=r.electronic_ev, core_ev=r.core_ev, total_ev=r.total_ev,
-        heat_of_formation_kcal=r.heat_of_formation_kcal, correction_kcal=r.correction_kcal,
-        charges=vec(r.charges, na) if r.charges else np.zeros(0),
-        dipole_debye=np.array(r.dipole_debye[:]), dipole_magnitude=r.dipole_magnitude,
-        dipole_point_charge_debye=np.array(r.dipole_point_charge_debye[:]),
-        dipole_sp_hybrid_debye=np.array(r.dipole_sp_hybrid_debye[:]),
-        dipole_pd_hybrid_debye=np.array(r.dipole_pd_hybrid_debye[:]),
-        homo_ev=_opt(r.homo_ev), lumo_ev=_opt(r.lumo_ev), homo_ev_beta=_opt(r.homo_ev_beta),
-        lumo_ev_beta=_opt(r.lumo_ev_beta), iterations=r.iterations, converged=bool(r.converged),
-        unrestricted=bool(r.unrestricted), density_error=r.density_error, fp32_iterations=r.fp32_iterations,
-        jacobi_sweeps=r.jacobi_sweeps, device_phase_ms=tuple(r.device_phase_ms[:]),
-        eig_phase_ms=tuple(r.eig_phase_ms[:]),
-        message=r.message.decode(errors="replace"),
-    )
+
+def _convert_all(c_res, count: int, free) -> list:
+    """Pm7Results over a filled `_Result` array without copying the matrices: every scalar field of
+    the batch is read in one pass; each result's arrays are views kept alive by an owner that
+    calls `free` (pm7_result_free) once no view of that result is referenced any more."""
+    cols = np.frombuffer(c_res, dtype=_RESULT_DTYPE, count=count)
+    col = {name: cols[name].tolist() for name in _RESULT_DTYPE.names}
+    out = []
+    for i in range(count):
+        n = col["nao"][i]
+        na = col["n_atoms"][i]
+        owner = _ResultArrays(free, c_res[i])
+        dens = col["density"][i]
+        nob = col["n_occ_beta"][i]
+
+        def mat(addr, rows=n, cols_=n):
+            a = _view(owner, addr, rows * cols_)
+            return None if a is None else a.reshape(rows, cols_)
+
+        out.append(Pm7Result(
+            status=col["status"][i], density=mat(dens) if dens else np.zeros((0, 0)),
+            spin_density=mat(col["spin_density"][i]),
+            mo_energies=_view(owner, col["mo_energies"][i], n) if col["mo_energies"][i] else np.zeros(0),
+            mo_coeff=mat(col["mo_coeff"][i]) if col["mo_coeff"][i] else np.zeros((0, 0)),
+            n_occ=col["n_occ"][i], mo_energies_beta=_view(owner, col["mo_energies_beta"][i], n),
+            mo_coeff_beta=mat(col["mo_coeff_beta"][i]), n_occ_beta=nob if nob >= 0 else None,
+            electronic_ev=col["electronic_ev"][i], core_ev=col["core_ev"][i], total_ev=col["total_ev"][i],
+            heat_of_formation_kcal=col["heat_of_formation_kcal"][i], correction_kcal=col["correction_kcal"][i],
+            charges=_view(owner, col["charges"][i], na) if col["charges"][i] else np.zeros(0),
+            dipole_debye=np.array(col["dipole_debye"][i]), dipole_magnitude=col["dipole_magnitude"][i],
+            dipole_point_charge_debye=np.array(col["dipole_point_charge_debye"][i]),
+            dipole_sp_hybrid_debye=np.array(col["dipole_sp_hybrid_debye"][i]),
+            dipole_pd_hybrid_debye=np.array(col["dipole_pd_hybrid_debye"][i]),
+            homo_ev=_opt(col["homo_ev"][i]), lumo_ev=_opt(col["lumo_ev"][i]),
+            homo_ev_beta=_opt(col["homo_ev_beta"][i]), lumo_ev_beta=_opt(col["lumo_ev_beta"][i]),
+            iterations=col["iterations"][i], converged=bool(col["converged"][i]),
+            unrestricted=bool(col["unrestricted"][i]), density_error=col["density_error"][i],
+            fp32_iterations=col["fp32_iterations"][i], jacobi_sweeps=col["jacobi_sweeps"][i],
+            device_phase_ms=tuple(col["device_phase_ms"][i]), eig_phase_ms=tuple(col["eig_phase_ms"][i]),
+            message=col["message"][i].decode(errors="replace"),
+        ))
+    return out
+
+
+def _convert(r: _Result, free) -> Pm7Result:
+    """One result (see _convert_all)."""
+    arr = (_Result * 1)()
+    C.memmove(arr, C.byref(r), C.sizeof(_Result))
+    return _convert_all(arr, 1, free)[0]
 
 
 class Engine:
@@ -403,13 +448,11 @@ class Engine:
         st = self._lib.pm7_run_batch(self._h, n, c_mols, c_opts, len(opts_list), c_res)
         if st != 0:
             raise Pm7Error(st, self.last_error())
-        out = []
-        for i in range(n):
-            r = _convert(c_res[i])
-            self._lib.pm7_result_free(C.byref(c_res[i]))
-            if raise_on_error and r.status != 0:
-                raise Pm7Error(r.status, r.message, r)
-            out.append(r)
+        out = _convert_all(c_res, n, self._lib.pm7_result_free)
+        if raise_on_error:
+            for r in out:
+                if r.status != 0:
+                    raise Pm7Error(r.status, r.message, r)
         return out
 
     def core_hamiltonian(self, molecule: Molecule) -> np.ndarray:

@@ -222,6 +222,9 @@ def main() -> int:
                   "max_abs_dHf_vs_fp64_kcal": dhf_mx, "max_abs_dE_vs_fp64_ev": de_mx, "max_abs_dP_vs_fp64": dp_mx,
                   "within_1e-4_kcal": mixed_ok,
                   "fp32_iterations_mean": round(float(np.mean([r.fp32_iterations for r in resmx])), 2),
+                  "slowest_molecule": slowest_info(resmx),
+                  "phase_ms_slowest": dict(zip(["fock", "energy_commutator", "diis", "eigensolve", "density", "total"],
+                                               [round(x, 3) for x in max(resmx, key=lambda r: r.device_phase_ms[5]).device_phase_ms])),
                   "iterations_mean": round(float(np.mean([r.iterations for r in resmx])), 2)},
         "single_molecule_fp64": {"solves_per_s": round(single_rate, 3), "molecules": len(single),
                                  "batched_speedup": round((solvesmx if head_mixed else solves64) / single_rate, 3)},

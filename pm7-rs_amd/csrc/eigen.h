@@ -291,4 +291,83 @@ __device__ int eigensolve(int n, int ld, const double* __restrict__ W, const dou
   return sweeps;
 }
 
+// Occupied-virtual rotation of the orbitals without a full eigensolve (the reduced-precision
+// phase of a mixed run): with F' = Cp^T W Cp, each occupied i / virtual a pair gets the 2x2
+// Jacobi angle that annihilates F'_ia (the formula of jacobi_core), all at once as
+// U = exp(K), K_ia = theta, K_ai = -theta (third-order Taylor), and Cn = Cp U is brought back to
+// orthonormal columns by two Newton-Schulz steps C <- C (3 I - C^T C) / 2. Occupied orbitals stay
+// the first nocc columns. Every product is an MFMA pass; scratch: Ts, Ag, Vg (ld x ld each), W
+// (overwritten) and Cp (overwritten). Stationary points of the SCF are the same as with the
+// full eigensolve: at self-consistency F'_ia = 0 and the step is the identity.
+template <int NT>
+__device__ void occ_virt_rotate(int n, int ld, int nocc, double* __restrict__ W, double* __restrict__ Cp,
+                                double* __restrict__ Cn, double* __restrict__ Ts, double* __restrict__ Ag,
+                                double* __restrict__ Vg) {
+  auto rowk = [&](const double* X) {
+    return [=](int i, int k) { return (i < n && k < n) ? X[(int64_t)k * ld + i] : 0.0; };  // X(i,k), X symmetric
+  };
+  auto colk = [&](const double* X) {
+    return [=](int k, int j) { return (k < n && j < n) ? X[(int64_t)k * ld + j] : 0.0; };  // X(k,j)
+  };
+  auto trans = [&](const double* X) {
+    return [=](int i, int k) { return (i < n && k < n) ? X[(int64_t)k * ld + i] : 0.0; };  // X^T(i,k)
+  };
+  auto rowm = [&](const double* X) {
+    return [=](int i, int k) { return (i < n && k < n) ? X[(int64_t)i * ld + k] : 0.0; };  // X(i,k)
+  };
+  auto put = [&](double* X) { return [=](int i, int j, double v) { X[(int64_t)i * ld + j] = v; }; };
+  // T = W Cp; F' = Cp^T T.
+  gemm_f64<NT>(n, n, rowk(W), colk(Cp), put(Ts));
+  __syncthreads();
+  gemm_f64<NT>(n, n, trans(Cp), colk(Ts), put(Ag));
+  __syncthreads();
+  // K into W.
+  for (int idx = threadIdx.x; idx < n * n; idx += NT) {
+    const int i = idx / n, j = idx - i * n;
+    double k = 0.0;
+    const bool iv = i < nocc, jv = j < nocc;
+    if (iv != jv) {
+      const int o = iv ? i : j, v = iv ? j : i;
+      const double apq = Ag[(int64_t)o * ld + v];
+      if (apq != 0.0) {
+        const double d = Ag[(int64_t)v * ld + v] - Ag[(int64_t)o * ld + o];
+        const double x = 2.0 * apq;
+        const double rr = sqrt(d * d + x * x);
+        const double th = atan((d >= 0.0 ? x : -x) / (fabs(d) + rr));
+        k = iv ? th : -th;
+      }
+    }
+    W[(int64_t)i * ld + j] = k;
+  }
+  __syncthreads();
+  // K^2 into Vg; U = I + K + K^2/2 + K K^2/6 into Ts.
+  gemm_f64<NT>(n, n, rowm(W), colk(W), put(Vg));
+  __syncthreads();
+  gemm_f64<NT>(n, n, rowm(W), colk(Vg), [&](int i, int j, double v) {
+    const int64_t x = (int64_t)i * ld + j;
+    Ts[x] = (i == j ? 1.0 : 0.0) + W[x] + 0.5 * Vg[x] + v / 6.0;
+  });
+  __syncthreads();
+  // C1 = Cp U into Ag.
+  gemm_f64<NT>(n, n, rowm(Cp), colk(Ts), put(Ag));
+  __syncthreads();
+  // Two Newton-Schulz steps: Ag -> Cp -> Cn.
+  double* src = Ag;
+  double* dst = Cp;
+  for (int step = 0; step < 2; ++step) {
+    gemm_f64<NT>(n, n, trans(src), colk(src), put(Vg));  // S = C^T C
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < n * n; idx += NT) {
+      const int i = idx / n, j = idx - i * n;
+      const int64_t x = (int64_t)i * ld + j;
+      Vg[x] = (i == j ? 1.5 : 0.0) - 0.5 * Vg[x];
+    }
+    __syncthreads();
+    gemm_f64<NT>(n, n, rowm(src), colk(Vg), put(dst));
+    __syncthreads();
+    src = dst;
+    dst = Cn;
+  }
+}
+
 }  // namespace pm7

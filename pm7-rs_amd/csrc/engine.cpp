@@ -102,9 +102,9 @@ struct pm7_ctx {
   DevBuf d_elem, d_pairpar, d_mols, d_list, d_atom_z, d_atom_ao, d_atom_pp, d_xyz, d_pairs, d_items, d_gath_off;
   DevBuf d_w, d_slab, d_vecs, d_out, d_ecore, d_ecore_mol, d_gath;
   double timings[PM7_TIMING_SLOTS] = {};
-  // Jacobi sweep cap per eigensolve in the reduced-precision phase of PRECISION_MIXED runs
-  // (environment PM7_AMD_MIXED_SWEEPS, 1..40).
-  int mixed_sweeps = 2;
+  // Jacobi sweep cap of the first (cold) eigensolve in the reduced-precision phase of
+  // PRECISION_MIXED runs (environment PM7_AMD_MIXED_SWEEPS, 1..40).
+  int mixed_sweeps = 40;
   std::mutex mu;
 };
 

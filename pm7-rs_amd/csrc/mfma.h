@@ -60,4 +60,23 @@ __device__ inline flt4 tile16_f32(int K, int i0, int j0, FA fa, FB fb) {
   return acc;
 }
 
+// D = A B over an m x m output (k-extent K), every 16x16 tile on the matrix cores, tiles dealt
+// to the NT/64 waves in turn; store(i, j, value) is called for i, j < m. The caller
+// synchronises before D is read.
+template <int NT, class FA, class FB, class ST>
+__device__ inline void gemm_f64(int m, int K, FA fa, FB fb, ST store) {
+  constexpr int NW = NT / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, q = lane >> 4;
+  const int nb = (m + 15) >> 4;
+  for (int t = wave; t < nb * nb; t += NW) {
+    const int I = t / nb, J = t - I * nb, i0 = 16 * I, j0 = 16 * J;
+    const dbl4 x = tile16_f64(K, i0, j0, fa, fb);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + tile_row_f64(q, u), j = j0 + r;
+      if (i < m && j < m) store(i, j, x[u]);
+    }
+  }
+}
+
 }  // namespace pm7

@@ -576,9 +576,10 @@ __global__ void __launch_bounds__(NT)
   res.e_fp32_last = 0.0;
   if (threadIdx.x == 0) s_int[4] = c.M.precision == 1 ? 1 : 0;
   __syncthreads();
-  // One FP64 eigensolve: to full accuracy, or, in the reduced-precision phase of a mixed run, to
-  // the accuracy the FP32 density supports (off-diagonal norm 1e-7 of the total) in at most
-  // mixed_sweeps sweeps.
+  // One FP64 eigensolve: to full accuracy, or, in the reduced-precision phase of a mixed run (its
+  // first iteration; later ones rotate occupied against virtual orbitals only), to the accuracy
+  // the FP32 density supports (off-diagonal norm 1e-7 of the total) in at most mixed_sweeps
+  // sweeps.
   auto eig = [&](bool low, const double* Wm, const double* Cp, double* Co, double* ev, int ag, int vg, int tg) -> int {
     return eigensolve<NT, MAXN, A_LDS, V_LDS, double>(n, ld, Wm, Cp, Co, ev, c.slot(ag), c.slot(vg), c.slot(tg), s_a,
                                                       s_v, s_c, s_s, s_t, s_perm, s_pq, s_pq + MAXN / 2, s_red,
@@ -682,7 +683,10 @@ __global__ void __launch_bounds__(NT)
       lap(PH_DIIS);
       double* Cnew = c.slot(cbuf ? S_C1 : S_C0);
       const bool low = s_int[4] != 0;
-      sweeps += eig(low, W, iter > 0 ? c.slot(cbuf ? S_C0 : S_C1) : nullptr, Cnew, eps_a, S_AG, S_VG, S_T);
+      if (low && iter > 0)
+        occ_virt_rotate<NT>(n, ld, nocc, W, c.slot(cbuf ? S_C0 : S_C1), Cnew, c.slot(S_T), c.slot(S_AG), c.slot(S_VG));
+      else
+        sweeps += eig(low, W, iter > 0 ? c.slot(cbuf ? S_C0 : S_C1) : nullptr, Cnew, eps_a, S_AG, S_VG, S_T);
       lap(PH_EIG);
       double* Pn = c.slot(S_HD + s_int[0]);
       const double dsq = low ? density<NT, float>(n, ld, nocc, 2.0, Cnew, P, Pn, s_red)
@@ -781,8 +785,13 @@ __global__ void __launch_bounds__(NT)
       double* Ca = c.slot(cbuf ? U_CA1 : U_CA0);
       double* Cb = c.slot(cbuf ? U_CB1 : U_CB0);
       const bool low = s_int[4] != 0;
-      sweeps += eig(low, Fa, iter > 0 ? c.slot(cbuf ? U_CA0 : U_CA1) : nullptr, Ca, eps_a, U_AG, U_VG, U_T);
-      sweeps += eig(low, Fb, iter > 0 ? c.slot(cbuf ? U_CB0 : U_CB1) : nullptr, Cb, eps_b, U_AG, U_VG, U_T);
+      if (low && iter > 0) {
+        occ_virt_rotate<NT>(n, ld, na, Fa, c.slot(cbuf ? U_CA0 : U_CA1), Ca, c.slot(U_T), c.slot(U_AG), c.slot(U_VG));
+        occ_virt_rotate<NT>(n, ld, nb, Fb, c.slot(cbuf ? U_CB0 : U_CB1), Cb, c.slot(U_T), c.slot(U_AG), c.slot(U_VG));
+      } else {
+        sweeps += eig(low, Fa, iter > 0 ? c.slot(cbuf ? U_CA0 : U_CA1) : nullptr, Ca, eps_a, U_AG, U_VG, U_T);
+        sweeps += eig(low, Fb, iter > 0 ? c.slot(cbuf ? U_CB0 : U_CB1) : nullptr, Cb, eps_b, U_AG, U_VG, U_T);
+      }
       lap(PH_EIG);
       const double da = low ? density<NT, float>(n, ld, na, 1.0, Ca, Pa, Pan, s_red)
                             : density<NT, double>(n, ld, na, 1.0, Ca, Pa, Pan, s_red);

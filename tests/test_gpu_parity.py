@@ -129,6 +129,42 @@ def test_engine_matches_oracle_density(eng, name):
         assert np.max(np.abs(r.spin_density - spin)) <= 1e-7
 
 
+def test_mixed_precision_golden_batch_within_bars(eng):
+    """PRECISION_MIXED (FP32 G(P) and P with occupied-virtual rotations until the density change
+    is below 1e-5, then FP64 with full eigensolves) reaches the same stationary point: the same
+    energy and heat-of-formation bars as FP64. Its density follows another path to the fixed
+    point, so it is held to the tolerance of that convergence (1e-5), not to the 1e-7 of a path
+    that matches the oracle's step for step."""
+    names = sorted(k for k in CASES if k not in DEGENERATE_OPEN_SHELL)
+    say(f"mixed golden batch of {len(names)}")
+    mols = [mol_of(CASES[k]["atoms"], CASES[k]["charge"], CASES[k]["multiplicity"]) for k in names]
+    mx = eng.run_batch(mols, [pm7amd.Pm7Options(charge=CASES[k]["charge"], multiplicity=CASES[k]["multiplicity"],
+                                                 precision="mixed") for k in names])
+    bad = []
+    for k, r in zip(names, mx):
+        c = CASES[k]
+        de = r.total_ev - c["energy_ev"]
+        dh = r.heat_of_formation_kcal - c["heat_of_formation_kcal"]
+        if r.status != 0 or not r.converged or abs(de) > 1e-6 or abs(dh) > 1e-4:
+            bad.append((k, r.status, de, dh, r.fp32_iterations, r.message))
+    assert not bad, bad
+    assert sum(r.fp32_iterations for r in mx) > 0
+
+
+def test_mixed_precision_organic_batch_vs_fp64(eng):
+    batch = W.organic_batch(48, 3)
+    say("mixed organic batch of 48")
+    mols = [mol_of(a) for a in batch]
+    r64 = eng.run_batch(mols, pm7amd.Pm7Options())
+    rmx = eng.run_batch(mols, pm7amd.Pm7Options(precision="mixed"))
+    for a, b in zip(r64, rmx):
+        assert a.status == 0 and b.status == 0 and b.converged
+        assert abs(a.total_ev - b.total_ev) <= 1e-6
+        assert abs(a.heat_of_formation_kcal - b.heat_of_formation_kcal) <= 1e-4
+        assert np.max(np.abs(a.density - b.density)) <= 1e-5
+        assert b.fp32_iterations > 0
+
+
 def test_config1_water(eng):
     say("config 1 water")
     atoms = W.water()

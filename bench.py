@@ -89,6 +89,9 @@ def main() -> int:
     ap.add_argument("--single", type=int, default=32, help="molecules timed one at a time")
     ap.add_argument("--cpu-sample", type=int, default=24, help="molecules timed on the CPU oracle")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serving", type=int, default=2000, help="requests in the serving stream (0: skip)")
+    ap.add_argument("--serve-lanes", type=int, default=4)
+    ap.add_argument("--serve-batch", type=int, default=64)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -149,6 +152,50 @@ def main() -> int:
     de_mx = max(abs(a.total_ev - b.total_ev) for a, b in zip(resmx, res64))
     dp_mx = max(float(np.max(np.abs(a.density - b.density))) for a, b in zip(resmx, res64))
     mixed_ok = dhf_mx <= 1e-4 and not badmx
+
+    # Serving: Poisson arrivals at 75 % of the measured FP64 batch capacity into the
+    # continuous-batching server; latency is submit -> completion per request.
+    serving = None
+    if args.serving > 0:
+        import threading
+        rate = 0.75 * solves64
+        stream = W.serving_stream(args.serving, seed=5, rate_per_s=rate, frac_sp=0.8, frac_d=0.0)
+        reqs = [(t, pm7amd.Molecule.from_symbols_angstrom(a, c, m), c, m) for t, a, c, m, _ in stream]
+
+        def serve(precision):
+            lat, bad = [], 0
+            with pm7amd.Server(eng, lanes=args.serve_lanes, max_batch=args.serve_batch) as srv:
+                t0 = time.perf_counter()
+
+                def feed():
+                    for t, mol, c, m in reqs:
+                        dt = t - (time.perf_counter() - t0)
+                        if dt > 0:
+                            time.sleep(dt)
+                        srv.submit(mol, pm7amd.Pm7Options(charge=c, multiplicity=m, precision=precision))
+
+                th = threading.Thread(target=feed)
+                th.start()
+                done = 0
+                while done < len(reqs):
+                    out = srv.poll(-1, timeout_ms=60000)
+                    if out is None:
+                        raise SystemExit("serving stalled")
+                    done += 1
+                    lat.append(out[2])
+                    bad += out[1].status != 0
+                elapsed = time.perf_counter() - t0
+                th.join()
+            return {"solves_per_s": round(len(reqs) / elapsed, 3), "failed": bad,
+                    "latency_ms": {"p50": round(pct(lat, 0.5), 3), "p99": round(pct(lat, 0.99), 3)}}
+
+        serve("fp64")  # warm the server lanes' buffers
+        serving = {"requests": len(reqs), "rate_per_s": round(rate, 1), "lanes": args.serve_lanes,
+                   "max_batch": args.serve_batch,
+                   "mix": "80% closed-shell H/C/N/O (3-60 atoms), 20% UHF doublets (one H removed); "
+                          "S/P/Cl requests need the MNDO/d kernel (not built) and are left out",
+                   "fp64": serve("fp64"), "mixed": serve("mixed")}
+        print(f"[bench] serving: {serving['fp64']} / {serving['mixed']}", file=sys.stderr, flush=True)
 
     # One molecule at a time (the plain FP64 GPU path the batched path is compared with).
     single = mols[: args.single]
@@ -226,6 +273,7 @@ def main() -> int:
                   "phase_ms_slowest": dict(zip(["fock", "energy_commutator", "diis", "eigensolve", "density", "total"],
                                                [round(x, 3) for x in max(resmx, key=lambda r: r.device_phase_ms[5]).device_phase_ms])),
                   "iterations_mean": round(float(np.mean([r.iterations for r in resmx])), 2)},
+        "serving": serving,
         "single_molecule_fp64": {"solves_per_s": round(single_rate, 3), "molecules": len(single),
                                  "batched_speedup": round((solvesmx if head_mixed else solves64) / single_rate, 3)},
         "kernel_ms": {"integrals": round(float(np.mean([k[0] for k in kern64])), 4),

@@ -33,7 +33,8 @@ typedef enum {
   PM7_ERR_LINEAR_ALGEBRA = 5,     /* Pm7Error::LinearAlgebra */
   PM7_ERR_SCF_NOT_CONVERGED = 6,  /* Pm7Error::ScfNotConverged (energies are still filled) */
   PM7_ERR_INSUFFICIENT_MEMORY = 7,/* Pm7Error::InsufficientMemory */
-  PM7_ERR_DEVICE = 8              /* HIP runtime failure */
+  PM7_ERR_DEVICE = 8,             /* HIP runtime failure */
+  PM7_NOT_READY = 9               /* request server: the request has not finished yet */
 } pm7_status;
 
 typedef enum { PM7_METHOD_PM7 = 0, PM7_METHOD_PM7_TS = 1, PM7_METHOD_PM7_MINUS = 3 } pm7_method;
@@ -142,6 +143,35 @@ pm7_status pm7_fock_spin(pm7_ctx* ctx, const pm7_molecule_v1* mol, const double*
 
 /* Replaces core_core_energy (src/repulsion.rs:16-36), eV. */
 pm7_status pm7_core_core_energy(pm7_ctx* ctx, const pm7_molecule_v1* mol, double* e_out);
+
+/* Request server: continuous batching of SCF requests (engine extension; no pm7-rs counterpart,
+ * it serves the callers that would otherwise call run_pm7 once per request). A scheduler thread
+ * admits queued requests, in arrival order, into up to `lanes` batches in flight at once (each on
+ * its own streams), as long as the device memory of the in-flight batches stays within the
+ * budget; a request that arrives while others run starts as soon as a lane is free. Each result
+ * equals what pm7_run returns for that request alone (no atomics, fixed reduction orders).
+ * pm7_params_load must not be called while a server runs on the context. */
+typedef struct pm7_server pm7_server;
+typedef struct {
+  int32_t lanes;         /* batches in flight at once, 1..16 (default 4) */
+  int32_t max_batch;     /* molecules per batch (default 256) */
+  int64_t max_memory_mb; /* device memory for the in-flight batches; <= 0: 80% of free HBM */
+} pm7_server_config_v1;
+
+void pm7_server_default_config(pm7_server_config_v1* cfg);
+pm7_status pm7_server_start(pm7_ctx* ctx, const pm7_server_config_v1* cfg, pm7_server** out);
+/* Queue one request (opts may be NULL for the defaults). Invalid input, unsupported elements and
+ * requests larger than the whole budget finish at once with that status. */
+pm7_status pm7_server_submit(pm7_server* srv, const pm7_molecule_v1* mol, const pm7_options_v1* opts,
+                             int64_t* request_id);
+/* Claim a finished request: request_id >= 0 waits for that request, < 0 for any unclaimed one.
+ * timeout_ms: 0 = do not wait, < 0 = wait without limit. Fills *out (release with
+ * pm7_result_free), the id and the latency from submit to completion in ms; PM7_NOT_READY when
+ * nothing finished in time. */
+pm7_status pm7_server_poll(pm7_server* srv, int64_t request_id, int32_t timeout_ms, int64_t* done_id,
+                           pm7_result_v1* out, double* latency_ms);
+/* Finish the queued and in-flight requests, release unclaimed results, stop the thread. */
+void pm7_server_stop(pm7_server* srv);
 
 /* Timing of the last pm7_run / pm7_run_batch call in milliseconds, measured with HIP events on
  * the stream each kernel runs on: [0] integrals + H_core, [1] all SCF megakernels, [2] output

@@ -12,6 +12,10 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <thread>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -89,6 +93,53 @@ struct Job {
 
 }  // namespace
 
+namespace {
+
+// Device resources of one batch in flight: streams (one per SCF size class, stream 0 also
+// carries uploads, integrals and the output gather), events, and grow-only device buffers.
+// The synchronous batch path uses the context's lane; the request server owns several.
+struct Lane {
+  // One stream for everything (request-server lanes: hardware queues are few, and a stream that
+  // shares a queue with another lane's long kernel would wait behind it) or one per size class.
+  bool single_stream = false;
+  hipStream_t st[SCF_VARIANTS] = {};
+  hipEvent_t ev[12] = {};
+  hipEvent_t ev_var[2 * SCF_VARIANTS] = {};
+  DevBuf d_mols, d_list, d_atom_z, d_atom_ao, d_atom_pp, d_xyz, d_pairs, d_items, d_gath_off;
+  DevBuf d_w, d_slab, d_vecs, d_out, d_ecore, d_ecore_mol, d_gath;
+  std::string err;
+  bool create(bool one_stream = false) {
+    single_stream = one_stream;
+    for (int v = 0; v < (single_stream ? 1 : SCF_VARIANTS); ++v)
+      if (hipStreamCreateWithFlags(&st[v], hipStreamNonBlocking) != hipSuccess) return false;
+    for (int v = 1; single_stream && v < SCF_VARIANTS; ++v) st[v] = st[0];
+    for (auto& e : ev)
+      if (hipEventCreate(&e) != hipSuccess) return false;
+    for (auto& e : ev_var)
+      if (hipEventCreate(&e) != hipSuccess) return false;
+    return true;
+  }
+  void destroy() {
+    for (auto& s : st)
+      if (s) (void)hipStreamSynchronize(s);
+    DevBuf* bufs[] = {&d_mols, &d_list, &d_atom_z, &d_atom_ao, &d_atom_pp, &d_xyz, &d_pairs, &d_items, &d_gath_off,
+                      &d_w, &d_slab, &d_vecs, &d_out, &d_ecore, &d_ecore_mol, &d_gath};
+    for (DevBuf* b : bufs) b->release();
+    for (int v = 0; v < (single_stream ? 1 : SCF_VARIANTS); ++v)
+      if (st[v]) (void)hipStreamDestroy(st[v]);
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& e : ev_var)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& s : st) s = nullptr;
+    for (auto& e : ev) e = nullptr;
+    for (auto& e : ev_var) e = nullptr;
+  }
+  size_t held() const { return d_w.cap + d_slab.cap + d_gath.cap + d_pairs.cap + d_ecore.cap; }
+};
+
+}  // namespace
+
 struct pm7_ctx {
   int device = 0;
   std::string data_dir;
@@ -96,11 +147,8 @@ struct pm7_ctx {
   ParamSet params;
   int method = -1;
   std::vector<ElemDev> elem_host;
-  hipStream_t st[SCF_VARIANTS] = {};
-  hipEvent_t ev[12] = {};
-  hipEvent_t ev_var[2 * SCF_VARIANTS] = {};
-  DevBuf d_elem, d_pairpar, d_mols, d_list, d_atom_z, d_atom_ao, d_atom_pp, d_xyz, d_pairs, d_items, d_gath_off;
-  DevBuf d_w, d_slab, d_vecs, d_out, d_ecore, d_ecore_mol, d_gath;
+  DevBuf d_elem, d_pairpar;
+  Lane lane;
   double timings[PM7_TIMING_SLOTS] = {};
   // Jacobi sweep cap of the first (cold) eigensolve in the reduced-precision phase of
   // PRECISION_MIXED runs (environment PM7_AMD_MIXED_SWEEPS, 1..40).
@@ -121,6 +169,16 @@ void set_msg(pm7_result_v1* r, const std::string& m) { std::snprintf(r->message,
   do {                                                                                       \
     hipError_t e_ = (expr);                                                                  \
     if (e_ != hipSuccess) return fail(ctx, PM7_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// Same, recording the message on a lane (lane work may run on the server's thread).
+#define LANECHK(expr)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) {                                                                  \
+      L.err = std::string(#expr ": ") + hipGetErrorString(e_);                               \
+      return PM7_ERR_DEVICE;                                                                 \
+    }                                                                                        \
   } while (0)
 
 // Input validation (pm7-rs validate_input src/scf.rs:937-992, run_pm7_once 530-627).
@@ -241,8 +299,20 @@ size_t bytes_needed(const Prepared& P) {
          P.jobs.size() * (sizeof(MolDev) + sizeof(MolOut) + 64);
 }
 
+// Device bytes one validated job adds to a batch (the terms of bytes_needed).
+size_t job_bytes(const pm7_ctx* ctx, const Job& j) {
+  const int n = (int)j.z.size();
+  int64_t w = 0;
+  for (int u = 0; u < n; ++u)
+    for (int v = u + 1; v < n; ++v)
+      w += (int64_t)npair_of(ctx->params.el[j.z[u]].n_orb) * npair_of(ctx->params.el[j.z[v]].n_orb);
+  const int64_t slab = (int64_t)NSLOT * j.ld * j.ld + ((j.n_pp + 7) / 8) * 8;
+  const int64_t words = w + slab + 2 * j.ld + (int64_t)(j.uhf ? 4 : 2) * j.nao * j.nao;
+  return (size_t)words * 8 + (size_t)n * (n - 1) / 2 * (sizeof(PairDev) + 8) + sizeof(MolDev) + sizeof(MolOut) + 64;
+}
+
 // Lay out, upload and run the once-per-geometry kernels.
-pm7_status prepare(pm7_ctx* ctx, Prepared& P, int64_t max_memory_mb) {
+pm7_status prepare(pm7_ctx* ctx, Lane& L, Prepared& P, int64_t max_memory_mb) {
   const int nm = (int)P.jobs.size();
   std::vector<int> atom_z, atom_ao, atom_pp;
   std::vector<double> xyz;
@@ -341,60 +411,60 @@ pm7_status prepare(pm7_ctx* ctx, Prepared& P, int64_t max_memory_mb) {
   // HBM budget (pm7-rs memory::guard, src/memory.rs:95-118, but for device memory).
   size_t need = bytes_needed(P);
   size_t free_b = 0, total_b = 0;
-  HIPCHK(hipMemGetInfo(&free_b, &total_b));
-  size_t held = ctx->d_w.cap + ctx->d_slab.cap + ctx->d_gath.cap + ctx->d_pairs.cap + ctx->d_ecore.cap;
-  size_t budget = max_memory_mb > 0 ? (size_t)max_memory_mb * 1024 * 1024 : (size_t)(0.8 * (double)(free_b + held));
-  if (need > budget)
-    return fail(ctx, PM7_ERR_INSUFFICIENT_MEMORY,
-                "needs " + std::to_string(need >> 20) + " MiB of device memory, budget " + std::to_string(budget >> 20) +
-                    " MiB (integrals + SCF working set)");
+  LANECHK(hipMemGetInfo(&free_b, &total_b));
+  size_t budget = max_memory_mb > 0 ? (size_t)max_memory_mb * 1024 * 1024 : (size_t)(0.8 * (double)(free_b + L.held()));
+  if (need > budget) {
+    L.err = "needs " + std::to_string(need >> 20) + " MiB of device memory, budget " + std::to_string(budget >> 20) +
+            " MiB (integrals + SCF working set)";
+    return PM7_ERR_INSUFFICIENT_MEMORY;
+  }
   auto up = [&](DevBuf& d, const void* src, size_t bytes) -> hipError_t {
     hipError_t e = d.ensure(bytes);
     if (e != hipSuccess) return e;
-    if (bytes) return hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, ctx->st[0]);
+    if (bytes) return hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, L.st[0]);
     return hipSuccess;
   };
-  HIPCHK(up(ctx->d_mols, P.mols.data(), P.mols.size() * sizeof(MolDev)));
-  HIPCHK(up(ctx->d_list, list.data(), list.size() * sizeof(int)));
-  HIPCHK(up(ctx->d_atom_z, atom_z.data(), atom_z.size() * sizeof(int)));
-  HIPCHK(up(ctx->d_atom_ao, atom_ao.data(), atom_ao.size() * sizeof(int)));
-  HIPCHK(up(ctx->d_atom_pp, atom_pp.data(), atom_pp.size() * sizeof(int)));
-  HIPCHK(up(ctx->d_xyz, xyz.data(), xyz.size() * sizeof(double)));
-  HIPCHK(up(ctx->d_pairs, pairs.data(), pairs.size() * sizeof(PairDev)));
-  HIPCHK(up(ctx->d_items, items.data(), items.size() * sizeof(int2)));
-  HIPCHK(up(ctx->d_gath_off, gath_off.data(), gath_off.size() * sizeof(int64_t)));
-  HIPCHK(ctx->d_w.ensure((size_t)P.w_words * 8));
-  HIPCHK(ctx->d_slab.ensure((size_t)P.slab_words * 8));
-  HIPCHK(ctx->d_vecs.ensure((size_t)P.vec_words * 8));
-  HIPCHK(ctx->d_out.ensure((size_t)nm * sizeof(MolOut)));
-  HIPCHK(ctx->d_ecore.ensure((size_t)P.n_pairs * 8));
-  HIPCHK(ctx->d_ecore_mol.ensure((size_t)nm * 8));
-  HIPCHK(ctx->d_gath.ensure((size_t)P.gath_words * 8));
+  LANECHK(up(L.d_mols, P.mols.data(), P.mols.size() * sizeof(MolDev)));
+  LANECHK(up(L.d_list, list.data(), list.size() * sizeof(int)));
+  LANECHK(up(L.d_atom_z, atom_z.data(), atom_z.size() * sizeof(int)));
+  LANECHK(up(L.d_atom_ao, atom_ao.data(), atom_ao.size() * sizeof(int)));
+  LANECHK(up(L.d_atom_pp, atom_pp.data(), atom_pp.size() * sizeof(int)));
+  LANECHK(up(L.d_xyz, xyz.data(), xyz.size() * sizeof(double)));
+  LANECHK(up(L.d_pairs, pairs.data(), pairs.size() * sizeof(PairDev)));
+  LANECHK(up(L.d_items, items.data(), items.size() * sizeof(int2)));
+  LANECHK(up(L.d_gath_off, gath_off.data(), gath_off.size() * sizeof(int64_t)));
+  LANECHK(L.d_w.ensure((size_t)P.w_words * 8));
+  LANECHK(L.d_slab.ensure((size_t)P.slab_words * 8));
+  LANECHK(L.d_vecs.ensure((size_t)P.vec_words * 8));
+  LANECHK(L.d_out.ensure((size_t)nm * sizeof(MolOut)));
+  LANECHK(L.d_ecore.ensure((size_t)P.n_pairs * 8));
+  LANECHK(L.d_ecore_mol.ensure((size_t)nm * 8));
+  LANECHK(L.d_gath.ensure((size_t)P.gath_words * 8));
   // Zero the slab: pad rows/columns of every matrix must be exact zeros.
-  HIPCHK(hipMemsetAsync(ctx->d_slab.p, 0, (size_t)P.slab_words * 8, ctx->st[0]));
+  LANECHK(hipMemsetAsync(L.d_slab.p, 0, (size_t)P.slab_words * 8, L.st[0]));
   BatchDev& b = P.b;
-  b.mols = ctx->d_mols.as<MolDev>();
-  b.mol_list = ctx->d_list.as<int>();
-  b.atom_z = ctx->d_atom_z.as<int>();
-  b.atom_ao = ctx->d_atom_ao.as<int>();
-  b.atom_pp = ctx->d_atom_pp.as<int>();
-  b.xyz = ctx->d_xyz.as<double>();
-  b.pairs = ctx->d_pairs.as<PairDev>();
+  b.mols = L.d_mols.as<MolDev>();
+  b.mol_list = L.d_list.as<int>();
+  b.atom_z = L.d_atom_z.as<int>();
+  b.atom_ao = L.d_atom_ao.as<int>();
+  b.atom_pp = L.d_atom_pp.as<int>();
+  b.xyz = L.d_xyz.as<double>();
+  b.pairs = L.d_pairs.as<PairDev>();
   b.elem = ctx->d_elem.as<ElemDev>();
   b.pairpar = ctx->d_pairpar.as<double2>();
-  b.hc_items = ctx->d_items.as<int2>();
-  b.gath_off = ctx->d_gath_off.as<int64_t>();
-  b.w = ctx->d_w.as<double>();
-  b.slab = ctx->d_slab.as<double>();
-  b.vecs = ctx->d_vecs.as<double>();
-  b.out = ctx->d_out.as<MolOut>();
-  b.e_core = ctx->d_ecore.as<double>();
-  b.e_core_mol = ctx->d_ecore_mol.as<double>();
-  b.gath = ctx->d_gath.as<double>();
+  b.hc_items = L.d_items.as<int2>();
+  b.gath_off = L.d_gath_off.as<int64_t>();
+  b.w = L.d_w.as<double>();
+  b.slab = L.d_slab.as<double>();
+  b.vecs = L.d_vecs.as<double>();
+  b.out = L.d_out.as<MolOut>();
+  b.e_core = L.d_ecore.as<double>();
+  b.e_core_mol = L.d_ecore_mol.as<double>();
+  b.gath = L.d_gath.as<double>();
   b.vpar = make_double4(ctx->params.vpar[1], ctx->params.vpar[2], ctx->params.vpar[3], ctx->params.vpar[4]);
-  HIPCHK(hipEventRecord(ctx->ev[0], ctx->st[0]));
-  HIPCHK(launch_integrals(b, P.n_pairs, P.n_items, nm, ctx->st[0]));
-  HIPCHK(hipEventRecord(ctx->ev[1], ctx->st[0]));
+  LANECHK(hipEventRecord(L.ev[0], L.st[0]));
+  LANECHK(launch_integrals(b, P.n_pairs, P.n_items, nm, L.st[0]));
+  LANECHK(hipEventRecord(L.ev[1], L.st[0]));
   return PM7_OK;
 }
 
@@ -526,38 +596,9 @@ pm7_status ensure_params(pm7_ctx* ctx, int method) {
   return PM7_OK;
 }
 
-pm7_status run_jobs(pm7_ctx* ctx, std::vector<Job>& jobs, pm7_result_v1* results, int64_t max_memory_mb) {
-  auto t0 = std::chrono::steady_clock::now();
-  Prepared P;
-  P.jobs = std::move(jobs);
-  pm7_status st = prepare(ctx, P, max_memory_mb);
-  const auto t_prep = std::chrono::steady_clock::now();
-  if (st != PM7_OK) {
-    for (auto& j : P.jobs) {
-      results[j.idx].status = st;
-      set_msg(&results[j.idx], ctx->err);
-    }
-    return st;
-  }
-  const int nm = (int)P.jobs.size();
-  // SCF megakernels: one stream per size class, all after the integrals.
-  for (int v = 1; v < SCF_VARIANTS; ++v) HIPCHK(hipStreamWaitEvent(ctx->st[v], ctx->ev[1], 0));
-  int count[SCF_VARIANTS];
-  for (int v = 0; v < SCF_VARIANTS; ++v) {
-    count[v] = P.list_off[v + 1] - P.list_off[v];
-    HIPCHK(hipEventRecord(ctx->ev_var[2 * v], ctx->st[v]));
-    HIPCHK(launch_scf(v, P.b, P.list_off[v], count[v], ctx->st[v]));
-    HIPCHK(hipEventRecord(ctx->ev_var[2 * v + 1], ctx->st[v]));
-  }
-  for (int v = 1; v < SCF_VARIANTS; ++v) {
-    HIPCHK(hipEventRecord(ctx->ev[6 + v], ctx->st[v]));
-    HIPCHK(hipStreamWaitEvent(ctx->st[0], ctx->ev[6 + v], 0));
-  }
-  HIPCHK(hipEventRecord(ctx->ev[2], ctx->st[0]));
-  HIPCHK(launch_gather(P.b, nm, ctx->st[0]));
-  HIPCHK(hipEventRecord(ctx->ev[5], ctx->st[0]));
-  // Host-side geometric corrections overlap the device work.
-  for (auto& j : P.jobs) {
+// Geometric corrections of every job (host; overlaps the device work).
+void host_corrections(std::vector<Job>& jobs) {
+  for (auto& j : jobs) {
     j.corr_kcal = 0.0;
     if (j.o.method != PM7_METHOD_PM7_MINUS) {
       j.corr_kcal += dispersion_kcal(j.z, j.xyz);
@@ -566,27 +607,78 @@ pm7_status run_jobs(pm7_ctx* ctx, std::vector<Job>& jobs, pm7_result_v1* results
       j.corr_kcal += si_o_h_kcal(j.z, j.xyz);
     }
   }
+}
+
+// SCF megakernels (one stream per size class, all after the integrals), then the output gather
+// on stream 0; L.ev[5] marks the end of the batch.
+pm7_status launch_batch(Lane& L, const Prepared& P, int* count) {
+  const int nm = (int)P.jobs.size();
+  for (int v = 1; v < SCF_VARIANTS && !L.single_stream; ++v) LANECHK(hipStreamWaitEvent(L.st[v], L.ev[1], 0));
+  for (int v = 0; v < SCF_VARIANTS; ++v) {
+    count[v] = P.list_off[v + 1] - P.list_off[v];
+    LANECHK(hipEventRecord(L.ev_var[2 * v], L.st[v]));
+    LANECHK(launch_scf(v, P.b, P.list_off[v], count[v], L.st[v]));
+    LANECHK(hipEventRecord(L.ev_var[2 * v + 1], L.st[v]));
+  }
+  for (int v = 1; v < SCF_VARIANTS && !L.single_stream; ++v) {
+    LANECHK(hipEventRecord(L.ev[6 + v], L.st[v]));
+    LANECHK(hipStreamWaitEvent(L.st[0], L.ev[6 + v], 0));
+  }
+  LANECHK(hipEventRecord(L.ev[2], L.st[0]));
+  LANECHK(launch_gather(P.b, nm, L.st[0]));
+  LANECHK(hipEventRecord(L.ev[5], L.st[0]));
+  return PM7_OK;
+}
+
+// Copy a finished batch's outputs to the host and fill results[j.idx] for every job.
+pm7_status collect_batch(pm7_ctx* ctx, Lane& L, const Prepared& P, pm7_result_v1* results) {
+  const int nm = (int)P.jobs.size();
   std::vector<MolOut> outs(nm);
   std::vector<double> ecore(nm), vecs(P.vec_words), gath(P.gath_words);
-  HIPCHK(hipMemcpyAsync(outs.data(), P.b.out, nm * sizeof(MolOut), hipMemcpyDeviceToHost, ctx->st[0]));
-  HIPCHK(hipMemcpyAsync(ecore.data(), P.b.e_core_mol, nm * sizeof(double), hipMemcpyDeviceToHost, ctx->st[0]));
-  HIPCHK(hipMemcpyAsync(vecs.data(), P.b.vecs, P.vec_words * sizeof(double), hipMemcpyDeviceToHost, ctx->st[0]));
-  HIPCHK(hipMemcpyAsync(gath.data(), P.b.gath, P.gath_words * sizeof(double), hipMemcpyDeviceToHost, ctx->st[0]));
-  const auto t_launched = std::chrono::steady_clock::now();
-  HIPCHK(hipStreamSynchronize(ctx->st[0]));
-  const auto t_synced = std::chrono::steady_clock::now();
-  float ms = 0.0f;
-  if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) == hipSuccess) ctx->timings[0] = ms;
-  if (hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]) == hipSuccess) ctx->timings[1] = ms;
-  if (hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[5]) == hipSuccess) ctx->timings[2] = ms;
-  for (int v = 0; v < SCF_VARIANTS; ++v) {
-    ctx->timings[4 + v] = 0.0;
-    if (count[v] > 0 && hipEventElapsedTime(&ms, ctx->ev_var[2 * v], ctx->ev_var[2 * v + 1]) == hipSuccess)
-      ctx->timings[4 + v] = ms;
-  }
+  LANECHK(hipMemcpyAsync(outs.data(), P.b.out, nm * sizeof(MolOut), hipMemcpyDeviceToHost, L.st[0]));
+  LANECHK(hipMemcpyAsync(ecore.data(), P.b.e_core_mol, nm * sizeof(double), hipMemcpyDeviceToHost, L.st[0]));
+  LANECHK(hipMemcpyAsync(vecs.data(), P.b.vecs, P.vec_words * sizeof(double), hipMemcpyDeviceToHost, L.st[0]));
+  LANECHK(hipMemcpyAsync(gath.data(), P.b.gath, P.gath_words * sizeof(double), hipMemcpyDeviceToHost, L.st[0]));
+  LANECHK(hipStreamSynchronize(L.st[0]));
   for (int mi = 0; mi < nm; ++mi) {
     const Job& j = P.jobs[mi];
     finish_result(ctx, j, outs[mi], ecore[mi], vecs.data() + j.vec0, gath.data() + j.gath0, &results[j.idx]);
+  }
+  return PM7_OK;
+}
+
+pm7_status run_jobs(pm7_ctx* ctx, std::vector<Job>& jobs, pm7_result_v1* results, int64_t max_memory_mb) {
+  Lane& L = ctx->lane;
+  auto t0 = std::chrono::steady_clock::now();
+  Prepared P;
+  P.jobs = std::move(jobs);
+  pm7_status st = prepare(ctx, L, P, max_memory_mb);
+  const auto t_prep = std::chrono::steady_clock::now();
+  if (st != PM7_OK) {
+    ctx->err = L.err;
+    for (auto& j : P.jobs) {
+      results[j.idx].status = st;
+      set_msg(&results[j.idx], L.err);
+    }
+    return st;
+  }
+  int count[SCF_VARIANTS];
+  st = launch_batch(L, P, count);
+  if (st != PM7_OK) return fail(ctx, st, L.err);
+  host_corrections(P.jobs);
+  const auto t_launched = std::chrono::steady_clock::now();
+  HIPCHK(hipEventSynchronize(L.ev[5]));
+  const auto t_synced = std::chrono::steady_clock::now();
+  st = collect_batch(ctx, L, P, results);
+  if (st != PM7_OK) return fail(ctx, st, L.err);
+  float ms = 0.0f;
+  if (hipEventElapsedTime(&ms, L.ev[0], L.ev[1]) == hipSuccess) ctx->timings[0] = ms;
+  if (hipEventElapsedTime(&ms, L.ev[1], L.ev[2]) == hipSuccess) ctx->timings[1] = ms;
+  if (hipEventElapsedTime(&ms, L.ev[2], L.ev[5]) == hipSuccess) ctx->timings[2] = ms;
+  for (int v = 0; v < SCF_VARIANTS; ++v) {
+    ctx->timings[4 + v] = 0.0;
+    if (count[v] > 0 && hipEventElapsedTime(&ms, L.ev_var[2 * v], L.ev_var[2 * v + 1]) == hipSuccess)
+      ctx->timings[4 + v] = ms;
   }
   const auto t_end = std::chrono::steady_clock::now();
   using msd = std::chrono::duration<double, std::milli>;
@@ -598,9 +690,272 @@ pm7_status run_jobs(pm7_ctx* ctx, std::vector<Job>& jobs, pm7_result_v1* results
   return PM7_OK;
 }
 
+// ---- request server (continuous batching) ------------------------------------------------
+
+struct SrvReq {
+  Job job;
+  bool valid = false;
+  int state = 0;  // 0 queued, 1 on the device, 2 finished, 3 claimed
+  pm7_result_v1 res{};
+  std::chrono::steady_clock::time_point t_submit, t_done;
+};
+
+// One batch in flight on its own lane.
+struct Flight {
+  Lane lane;
+  Prepared P;
+  std::vector<int64_t> ids;
+  size_t bytes = 0;
+  bool busy = false;
+};
+
+}  // namespace
+
+struct pm7_server {
+  pm7_ctx* ctx = nullptr;
+  pm7_server_config_v1 cfg{};
+  std::vector<std::unique_ptr<Flight>> flights;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<int64_t> queue;     // admitted in arrival order
+  std::deque<int64_t> finished;  // finished and not yet claimed, in completion order
+  std::unordered_map<int64_t, SrvReq> reqs;
+  int64_t next_id = 0;
+  size_t budget = 0, inflight = 0;
+  bool stopping = false;
+  std::thread worker;
+};
+
+namespace {
+
+void srv_finish(pm7_server* s, int64_t id, std::chrono::steady_clock::time_point now) {
+  SrvReq& r = s->reqs[id];
+  r.state = 2;
+  r.t_done = now;
+  s->finished.push_back(id);
+}
+
+// The scheduler: retire finished batches, admit queued requests into free lanes while the
+// in-flight device memory stays within the budget, and repeat. Batches on different lanes run
+// concurrently, so requests that arrive while others are on the device start without waiting
+// for them to finish.
+void srv_loop(pm7_server* s) {
+  pm7_ctx* ctx = s->ctx;
+  (void)hipSetDevice(ctx->device);
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    bool busy = false;
+    for (auto& fp : s->flights) {
+      Flight& f = *fp;
+      if (!f.busy) continue;
+      const hipError_t q = hipEventQuery(f.lane.ev[5]);
+      if (q == hipErrorNotReady) {
+        busy = true;
+        continue;
+      }
+      std::vector<pm7_result_v1> res(f.P.jobs.size());
+      for (auto& r : res) std::memset(&r, 0, sizeof r);
+      lk.unlock();
+      pm7_status st = q == hipSuccess ? collect_batch(ctx, f.lane, f.P, res.data()) : PM7_ERR_DEVICE;
+      lk.lock();
+      const auto now = std::chrono::steady_clock::now();
+      for (size_t k = 0; k < f.ids.size(); ++k) {
+        SrvReq& r = s->reqs[f.ids[k]];
+        r.res = res[k];
+        if (st != PM7_OK) {
+          r.res.status = st;
+          set_msg(&r.res, f.lane.err.empty() ? std::string("device error") : f.lane.err);
+        }
+        srv_finish(s, f.ids[k], now);
+      }
+      f.busy = false;
+      f.ids.clear();
+      s->inflight -= f.bytes;
+      s->cv_done.notify_all();
+    }
+    for (auto& fp : s->flights) {
+      Flight& f = *fp;
+      if (f.busy || s->queue.empty()) continue;
+      // The oldest request and the queued requests of its size class, in arrival order: one
+      // class per batch keeps a batch to a single kernel, whose time is that class's.
+      std::vector<int64_t> ids;
+      size_t bytes = 0;
+      const int cls = s->reqs[s->queue.front()].job.variant;
+      for (auto it = s->queue.begin(); it != s->queue.end() && (int)ids.size() < s->cfg.max_batch;) {
+        SrvReq& r = s->reqs[*it];
+        if (r.job.variant != cls) {
+          ++it;
+          continue;
+        }
+        const size_t b = job_bytes(ctx, r.job);
+        if (s->inflight + bytes + b > s->budget && (s->inflight + bytes) > 0) break;  // wait for memory
+        ids.push_back(*it);
+        it = s->queue.erase(it);
+        bytes += b;
+      }
+      if (ids.empty()) break;
+      f.P = Prepared();
+      for (size_t k = 0; k < ids.size(); ++k) {
+        SrvReq& r = s->reqs[ids[k]];
+        r.state = 1;
+        f.P.jobs.push_back(r.job);
+        f.P.jobs.back().idx = (int)k;
+      }
+      f.ids = ids;
+      f.bytes = bytes;
+      s->inflight += bytes;
+      f.busy = true;
+      lk.unlock();
+      int count[SCF_VARIANTS];
+      pm7_status st = prepare(ctx, f.lane, f.P, 0);
+      if (st == PM7_OK) st = launch_batch(f.lane, f.P, count);
+      if (st == PM7_OK) host_corrections(f.P.jobs);
+      lk.lock();
+      if (st != PM7_OK) {
+        const auto now = std::chrono::steady_clock::now();
+        for (int64_t id : f.ids) {
+          SrvReq& r = s->reqs[id];
+          std::memset(&r.res, 0, sizeof r.res);
+          r.res.status = st;
+          r.res.n_atoms = (int32_t)r.job.z.size();
+          set_msg(&r.res, f.lane.err);
+          srv_finish(s, id, now);
+        }
+        f.busy = false;
+        f.ids.clear();
+        s->inflight -= f.bytes;
+        s->cv_done.notify_all();
+      } else {
+        busy = true;
+      }
+    }
+    if (s->stopping && s->queue.empty() && !busy) break;
+    if (busy) s->cv_work.wait_for(lk, std::chrono::microseconds(50));
+    else s->cv_work.wait(lk, [&] { return s->stopping || !s->queue.empty(); });
+  }
+}
+
 }  // namespace
 
 extern "C" {
+
+void pm7_server_default_config(pm7_server_config_v1* cfg) {
+  if (!cfg) return;
+  cfg->lanes = 4;
+  cfg->max_batch = 256;
+  cfg->max_memory_mb = 0;
+}
+
+pm7_status pm7_server_start(pm7_ctx* ctx, const pm7_server_config_v1* cfg, pm7_server** out) {
+  if (!ctx || !out) return PM7_ERR_INVALID_INPUT;
+  *out = nullptr;
+  pm7_server_config_v1 c;
+  pm7_server_default_config(&c);
+  if (cfg) c = *cfg;
+  if (c.lanes < 1 || c.lanes > 16 || c.max_batch < 1) return fail(ctx, PM7_ERR_INVALID_INPUT, "server: lanes 1..16, max_batch >= 1");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  auto s = std::make_unique<pm7_server>();
+  s->ctx = ctx;
+  s->cfg = c;
+  size_t free_b = 0, total_b = 0;
+  HIPCHK(hipMemGetInfo(&free_b, &total_b));
+  s->budget = c.max_memory_mb > 0 ? (size_t)c.max_memory_mb << 20 : (size_t)(0.8 * (double)free_b);
+  for (int k = 0; k < c.lanes; ++k) {
+    auto f = std::make_unique<Flight>();
+    if (!f->lane.create(true)) {
+      f->lane.destroy();
+      for (auto& g : s->flights) g->lane.destroy();
+      return fail(ctx, PM7_ERR_DEVICE, "server: stream or event creation failed");
+    }
+    s->flights.push_back(std::move(f));
+  }
+  pm7_server* raw = s.release();
+  raw->worker = std::thread(srv_loop, raw);
+  *out = raw;
+  return PM7_OK;
+}
+
+pm7_status pm7_server_submit(pm7_server* s, const pm7_molecule_v1* mol, const pm7_options_v1* opts, int64_t* request_id) {
+  if (!s || !mol || !request_id) return PM7_ERR_INVALID_INPUT;
+  pm7_options_v1 def;
+  pm7_default_options(&def);
+  SrvReq r;
+  r.t_submit = std::chrono::steady_clock::now();
+  std::string msg;
+  pm7_status st = validate(s->ctx, *mol, opts ? *opts : def, r.job, msg);
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->stopping) return PM7_ERR_INVALID_INPUT;
+  const int64_t id = s->next_id++;
+  *request_id = id;
+  if (st == PM7_OK && job_bytes(s->ctx, r.job) > s->budget) {
+    st = PM7_ERR_INSUFFICIENT_MEMORY;
+    msg = "request needs " + std::to_string(job_bytes(s->ctx, r.job) >> 20) + " MiB, server budget " +
+          std::to_string(s->budget >> 20) + " MiB";
+  }
+  if (st != PM7_OK) {
+    r.res.status = st;
+    r.res.n_atoms = mol->n_atoms;
+    set_msg(&r.res, msg);
+    s->reqs.emplace(id, std::move(r));
+    srv_finish(s, id, std::chrono::steady_clock::now());
+    s->cv_done.notify_all();
+    return PM7_OK;
+  }
+  r.valid = true;
+  s->reqs.emplace(id, std::move(r));
+  s->queue.push_back(id);
+  s->cv_work.notify_one();
+  return PM7_OK;
+}
+
+pm7_status pm7_server_poll(pm7_server* s, int64_t request_id, int32_t timeout_ms, int64_t* done_id,
+                           pm7_result_v1* out, double* latency_ms) {
+  if (!s || !out) return PM7_ERR_INVALID_INPUT;
+  std::unique_lock<std::mutex> lk(s->mu);
+  if (request_id >= 0) {
+    auto it = s->reqs.find(request_id);
+    if (it == s->reqs.end() || it->second.state == 3) return PM7_ERR_INVALID_INPUT;
+  }
+  auto ready = [&]() -> int64_t {
+    if (request_id >= 0) return s->reqs[request_id].state == 2 ? request_id : -1;
+    while (!s->finished.empty()) {
+      const int64_t id = s->finished.front();
+      auto it = s->reqs.find(id);
+      if (it != s->reqs.end() && it->second.state == 2) return id;
+      s->finished.pop_front();  // claimed by id already
+    }
+    return -1;
+  };
+  int64_t id = ready();
+  if (id < 0 && timeout_ms != 0) {
+    if (timeout_ms < 0) s->cv_done.wait(lk, [&] { return (id = ready()) >= 0; });
+    else s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), [&] { return (id = ready()) >= 0; });
+  }
+  if (id < 0) return PM7_NOT_READY;
+  auto it = s->reqs.find(id);
+  *out = it->second.res;
+  if (done_id) *done_id = id;
+  if (latency_ms)
+    *latency_ms = std::chrono::duration<double, std::milli>(it->second.t_done - it->second.t_submit).count();
+  s->reqs.erase(it);
+  return PM7_OK;
+}
+
+void pm7_server_stop(pm7_server* s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stopping = true;
+  }
+  s->cv_work.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  (void)hipSetDevice(s->ctx->device);
+  for (auto& kv : s->reqs) pm7_result_free(&kv.second.res);
+  for (auto& f : s->flights) f->lane.destroy();
+  delete s;
+}
+
 
 int32_t pm7_abi_version(void) { return PM7_ABI_VERSION; }
 
@@ -636,21 +991,11 @@ pm7_status pm7_ctx_create(int32_t device, const char* data_dir, pm7_ctx** out) {
     const int v = std::atoi(ms);
     if (v >= 1 && v <= 40) ctx->mixed_sweeps = v;
   }
-  for (int v = 0; v < SCF_VARIANTS; ++v)
-    if (hipStreamCreateWithFlags(&ctx->st[v], hipStreamNonBlocking) != hipSuccess) {
-      delete ctx;
-      return PM7_ERR_DEVICE;
-    }
-  for (auto& e : ctx->ev)
-    if (hipEventCreate(&e) != hipSuccess) {
-      delete ctx;
-      return PM7_ERR_DEVICE;
-    }
-  for (auto& e : ctx->ev_var)
-    if (hipEventCreate(&e) != hipSuccess) {
-      delete ctx;
-      return PM7_ERR_DEVICE;
-    }
+  if (!ctx->lane.create()) {
+    ctx->lane.destroy();
+    delete ctx;
+    return PM7_ERR_DEVICE;
+  }
   pm7_status st = ensure_params(ctx, PM7_METHOD_PM7);
   if (st != PM7_OK) {
     pm7_ctx_destroy(ctx);
@@ -663,18 +1008,9 @@ pm7_status pm7_ctx_create(int32_t device, const char* data_dir, pm7_ctx** out) {
 void pm7_ctx_destroy(pm7_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
-  for (auto& s : ctx->st)
-    if (s) (void)hipStreamSynchronize(s);
-  DevBuf* bufs[] = {&ctx->d_elem, &ctx->d_pairpar, &ctx->d_mols, &ctx->d_list,  &ctx->d_atom_z, &ctx->d_atom_ao,
-                    &ctx->d_atom_pp, &ctx->d_xyz,   &ctx->d_pairs, &ctx->d_items, &ctx->d_gath_off, &ctx->d_w,
-                    &ctx->d_slab, &ctx->d_vecs,    &ctx->d_out,   &ctx->d_ecore, &ctx->d_ecore_mol, &ctx->d_gath};
-  for (DevBuf* b : bufs) b->release();
-  for (auto& s : ctx->st)
-    if (s) (void)hipStreamDestroy(s);
-  for (auto& e : ctx->ev)
-    if (e) (void)hipEventDestroy(e);
-  for (auto& e : ctx->ev_var)
-    if (e) (void)hipEventDestroy(e);
+  ctx->lane.destroy();
+  ctx->d_elem.release();
+  ctx->d_pairpar.release();
   delete ctx;
 }
 
@@ -754,12 +1090,12 @@ pm7_status pm7_core_hamiltonian(pm7_ctx* ctx, const pm7_molecule_v1* mol, double
   if (!h_out) return PM7_OK;
   Prepared P;
   P.jobs.push_back(j);
-  st = prepare(ctx, P, 0);
-  if (st != PM7_OK) return st;
+  st = prepare(ctx, ctx->lane, P, 0);
+  if (st != PM7_OK) return fail(ctx, st, ctx->lane.err);
   const Job& J = P.jobs[0];
   HIPCHK(hipMemcpy2DAsync(h_out, J.nao * sizeof(double), P.b.slab + J.mat0, J.ld * sizeof(double),
-                          J.nao * sizeof(double), J.nao, hipMemcpyDeviceToHost, ctx->st[0]));
-  HIPCHK(hipStreamSynchronize(ctx->st[0]));
+                          J.nao * sizeof(double), J.nao, hipMemcpyDeviceToHost, ctx->lane.st[0]));
+  HIPCHK(hipStreamSynchronize(ctx->lane.st[0]));
   return PM7_OK;
 }
 
@@ -784,19 +1120,19 @@ pm7_status pm7_fock_spin(pm7_ctx* ctx, const pm7_molecule_v1* mol, const double*
   }
   Prepared P;
   P.jobs.push_back(j);
-  st = prepare(ctx, P, 0);
-  if (st != PM7_OK) return st;
+  st = prepare(ctx, ctx->lane, P, 0);
+  if (st != PM7_OK) return fail(ctx, st, ctx->lane.err);
   const Job& J = P.jobs[0];
   const int64_t sq = (int64_t)J.ld * J.ld;
   double* base = P.b.slab + J.mat0;
   HIPCHK(hipMemcpy2DAsync(base + 3 * sq, J.ld * sizeof(double), p_tot, J.nao * sizeof(double), J.nao * sizeof(double),
-                          J.nao, hipMemcpyHostToDevice, ctx->st[0]));
+                          J.nao, hipMemcpyHostToDevice, ctx->lane.st[0]));
   HIPCHK(hipMemcpy2DAsync(base + 4 * sq, J.ld * sizeof(double), p_spin, J.nao * sizeof(double), J.nao * sizeof(double),
-                          J.nao, hipMemcpyHostToDevice, ctx->st[0]));
-  HIPCHK(launch_fock_once(P.b, 0, 3, 4, 1, ctx->st[0]));
+                          J.nao, hipMemcpyHostToDevice, ctx->lane.st[0]));
+  HIPCHK(launch_fock_once(P.b, 0, 3, 4, 1, ctx->lane.st[0]));
   HIPCHK(hipMemcpy2DAsync(f_out, J.nao * sizeof(double), base + 1 * sq, J.ld * sizeof(double), J.nao * sizeof(double),
-                          J.nao, hipMemcpyDeviceToHost, ctx->st[0]));
-  HIPCHK(hipStreamSynchronize(ctx->st[0]));
+                          J.nao, hipMemcpyDeviceToHost, ctx->lane.st[0]));
+  HIPCHK(hipStreamSynchronize(ctx->lane.st[0]));
   return PM7_OK;
 }
 
@@ -818,10 +1154,10 @@ pm7_status pm7_core_core_energy(pm7_ctx* ctx, const pm7_molecule_v1* mol, double
   }
   Prepared P;
   P.jobs.push_back(j);
-  st = prepare(ctx, P, 0);
-  if (st != PM7_OK) return st;
-  HIPCHK(hipMemcpyAsync(e_out, P.b.e_core_mol, sizeof(double), hipMemcpyDeviceToHost, ctx->st[0]));
-  HIPCHK(hipStreamSynchronize(ctx->st[0]));
+  st = prepare(ctx, ctx->lane, P, 0);
+  if (st != PM7_OK) return fail(ctx, st, ctx->lane.err);
+  HIPCHK(hipMemcpyAsync(e_out, P.b.e_core_mol, sizeof(double), hipMemcpyDeviceToHost, ctx->lane.st[0]));
+  HIPCHK(hipStreamSynchronize(ctx->lane.st[0]));
   return PM7_OK;
 }
 

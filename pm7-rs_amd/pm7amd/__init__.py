@@ -26,7 +26,7 @@ EV_TO_KCAL = 23.06054783061903
 
 STATUS_NAMES = {
     0: "Ok", 1: "InvalidInput", 2: "MissingElement", 3: "MissingParameter", 4: "UnsupportedFeature",
-    5: "LinearAlgebra", 6: "ScfNotConverged", 7: "InsufficientMemory", 8: "Device",
+    5: "LinearAlgebra", 6: "ScfNotConverged", 7: "InsufficientMemory", 8: "Device", 9: "NotReady",
 }
 
 METHODS = {"pm7": 0, "pm7-ts": 1, "pm7ts": 1, "pm7-minus": 3, "pm7-": 3, "pm7minus": 3}
@@ -54,6 +54,10 @@ class Pm7Error(RuntimeError):
 
 class _Molecule(C.Structure):
     _fields_ = [("n_atoms", C.c_int32), ("z", C.POINTER(C.c_int32)), ("xyz_bohr", C.POINTER(C.c_double))]
+
+
+class _ServerConfig(C.Structure):
+    _fields_ = [("lanes", C.c_int32), ("max_batch", C.c_int32), ("max_memory_mb", C.c_int64)]
 
 
 class _Options(C.Structure):
@@ -87,7 +91,8 @@ TIMING_SLOTS = 12
 ABI_FUNCTIONS = [
     "pm7_ctx_create", "pm7_ctx_destroy", "pm7_last_error", "pm7_abi_version", "pm7_params_load",
     "pm7_default_options", "pm7_run", "pm7_run_batch", "pm7_result_free", "pm7_core_hamiltonian",
-    "pm7_fock_spin", "pm7_core_core_energy", "pm7_last_timings",
+    "pm7_fock_spin", "pm7_core_core_energy", "pm7_last_timings", "pm7_server_default_config",
+    "pm7_server_start", "pm7_server_submit", "pm7_server_poll", "pm7_server_stop",
 ]
 
 _lib = None
@@ -117,6 +122,13 @@ def library() -> C.CDLL:
                                   C.POINTER(C.c_double)]
     lib.pm7_core_core_energy.argtypes = [vp, C.POINTER(_Molecule), C.POINTER(C.c_double)]
     lib.pm7_last_timings.argtypes = [vp, C.POINTER(C.c_double), C.c_int32]
+    lib.pm7_server_default_config.argtypes = [C.POINTER(_ServerConfig)]
+    lib.pm7_server_start.argtypes = [vp, C.POINTER(_ServerConfig), C.POINTER(vp)]
+    lib.pm7_server_submit.argtypes = [vp, C.POINTER(_Molecule), C.POINTER(_Options), C.POINTER(C.c_int64)]
+    lib.pm7_server_poll.argtypes = [vp, C.c_int64, C.c_int32, C.POINTER(C.c_int64), C.POINTER(_Result),
+                                    C.POINTER(C.c_double)]
+    lib.pm7_server_stop.argtypes = [vp]
+    lib.pm7_server_stop.restype = None
     _lib = lib
     return lib
 
@@ -499,6 +511,63 @@ class Engine:
 _default_engine: Optional[Engine] = None
 
 
+class Server:
+    """Continuous-batching request server on an Engine (pm7_server_*): submit() queues a
+    molecule and returns its id; poll() claims a finished result with its latency (submit to
+    completion, ms). Each result equals Engine.run of that molecule alone."""
+
+    NOT_READY = 9
+
+    def __init__(self, engine: Engine, lanes: int = 4, max_batch: int = 256, max_memory_mb: int = 0):
+        self._engine = engine
+        self._lib = engine._lib
+        cfg = _ServerConfig(int(lanes), int(max_batch), int(max_memory_mb))
+        h = C.c_void_p()
+        st = self._lib.pm7_server_start(engine._h, C.byref(cfg), C.byref(h))
+        if st != 0:
+            raise Pm7Error(st, engine.last_error())
+        self._h = h
+        self._keep = {}
+
+    def submit(self, molecule: Molecule, options: Optional["Pm7Options"] = None) -> int:
+        cm, keep = Engine._mol(molecule)
+        opts = (options or Pm7Options(charge=molecule.charge, multiplicity=molecule.multiplicity)).to_c()
+        rid = C.c_int64(-1)
+        st = self._lib.pm7_server_submit(self._h, C.byref(cm), C.byref(opts), C.byref(rid))
+        if st != 0:
+            raise Pm7Error(st, "submit refused")
+        return rid.value
+
+    def poll(self, request_id: int = -1, timeout_ms: int = -1):
+        """(id, Pm7Result, latency_ms) of a finished request, or None when none finished in time."""
+        res = (_Result * 1)()
+        rid = C.c_int64(-1)
+        lat = C.c_double(0.0)
+        st = self._lib.pm7_server_poll(self._h, int(request_id), int(timeout_ms), C.byref(rid), res, C.byref(lat))
+        if st == self.NOT_READY:
+            return None
+        if st != 0:
+            raise Pm7Error(st, f"poll of request {request_id} failed")
+        return rid.value, _convert_all(res, 1, self._lib.pm7_result_free)[0], lat.value
+
+    def stop(self):
+        if getattr(self, "_h", None):
+            self._lib.pm7_server_stop(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.stop()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.stop()
+        except Exception:
+            pass
+
+
 def default_engine() -> Engine:
     global _default_engine
     if _default_engine is None:
@@ -522,6 +591,6 @@ def run_pm7_batch(molecules: Sequence[Molecule], options=None) -> list:
 
 
 __all__ = [
-    "Atom", "Molecule", "Pm7Parameters", "Pm7Options", "Pm7Result", "Pm7Error", "Engine", "run_pm7",
+    "Atom", "Molecule", "Pm7Parameters", "Pm7Options", "Pm7Result", "Pm7Error", "Engine", "Server", "run_pm7",
     "run_pm7_batch", "default_engine", "library", "ABI_FUNCTIONS", "LIB_PATH", "DATA_DIR",
 ]

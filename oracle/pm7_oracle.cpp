@@ -45,12 +45,17 @@ constexpr double KCAL_TO_EV = 1.0 / EV_TO_KCAL;
 constexpr double AU_DIPOLE_TO_DEBYE = 2.541746473;
 constexpr double PI = 3.14159265358979323846;
 
-// Atomic masses (u) for the centre-of-mass dipole origin (src/data_tables.rs MASS, first 36).
-const double MASS[37] = {0.0,     1.0079,  4.0026,  6.94,    9.01218, 10.81,   12.011,  14.0067,
-                         15.9994, 18.9984, 20.179,  22.98977, 24.305, 26.98154, 28.0855, 30.97376,
-                         32.06,   35.453,  39.948,  39.098,  40.078,  44.956,  47.867,  50.942,
-                         51.996,  54.938,  55.845,  58.933,  58.693,  63.546,  65.38,   69.723,
-                         72.63,   74.922,  78.971,  79.904,  83.798};
+// Atomic masses (u) for the centre-of-mass dipole origin (src/data_tables.rs MASS: standard
+// atomic weights through Rn; heavier elements have none there and count as 0 here).
+const double MASS[87] = {
+    0.0,     1.0079,  4.0026,  6.94,    9.01218, 10.81,   12.011,  14.0067, 15.9994, 18.9984, 20.179,  22.98977,
+    24.305,  26.98154, 28.0855, 30.97376, 32.06,  35.453,  39.948,  39.098,  40.078,  44.956,  47.867,  50.942,
+    51.996,  54.938,  55.845,  58.933,  58.693,  63.546,  65.38,   69.723,  72.63,   74.922,  78.971,  79.904,
+    83.798,  85.468,  87.62,   88.906,  91.224,  92.906,  95.95,   97.0,    101.07,  102.91,  106.42,  107.87,
+    112.41,  114.82,  118.71,  121.76,  127.6,   126.9,   131.29,  132.91,  137.33,  138.91,  140.12,  140.91,
+    144.24,  145.0,   150.36,  151.96,  157.25,  158.93,  162.5,   164.93,  167.26,  168.93,  173.05,  174.97,
+    178.49,  180.95,  183.84,  186.21,  190.23,  192.22,  195.08,  196.97,  200.59,  204.38,  207.2,   208.98,
+    209.0,   210.0,   222.0};
 
 // ---------------------------------------------------------------------------------------------
 // Tiny deterministic parallel-for: static contiguous chunks, each output written by one thread.
@@ -573,7 +578,97 @@ void bintgs(double x, double b[13]) {
   }
 }
 
-// Local-frame sigma/pi overlaps s111 s211 s121 s221 s222 for ea.n >= eb.n (n <= 3).
+// Gauss-Legendre nodes and weights on [-1, 1] by Newton iteration on P_n (src/overlap_numeric.rs:137-175).
+void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.assign(n, 0.0);
+  w.assign(n, 0.0);
+  for (int i = 0; i < (n + 1) / 2; ++i) {
+    double t = std::cos(PI * (i + 0.75) / (n + 0.5));
+    double dp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = t;
+      for (int k = 2; k <= n; ++k) {
+        double p2 = ((2.0 * k - 1.0) * t * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = n * (t * p1 - p0) / (t * t - 1.0);
+      double dx = -p1 / dp;
+      t += dx;
+      if (std::fabs(dx) < 1.0e-15) break;
+    }
+    {
+      double p0 = 1.0, p1 = t;
+      for (int k = 2; k <= n; ++k) {
+        double p2 = ((2.0 * k - 1.0) * t * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = n * (t * p1 - p0) / (t * t - 1.0);
+    }
+    const double wt = 2.0 / ((1.0 - t * t) * dp * dp);
+    x[i] = -t;
+    x[n - 1 - i] = t;
+    w[i] = wt;
+    w[n - 1 - i] = wt;
+  }
+}
+
+// Overlap of Slater orbitals (na, la, m; za) on a and (nb, lb, m; zb) on b (b on +z at distance r)
+// by quadrature in prolate spheroidal coordinates xi = (ra + rb)/r, eta = (ra - rb)/r:
+// dV = (r/2)^3 (xi^2 - eta^2) dxi deta dphi, 48 Gauss-Legendre nodes on xi in [1, 1 + 60/p],
+// p = (za + zb) r / 2, and 40 on eta (src/overlap_numeric.rs:45-115).
+double sto_overlap(int na, double za, int la, int m, int nb, double zb, int lb, double r) {
+  double c;
+  if (la == 0 && lb == 0) c = 0.5;
+  else if ((la == 1) != (lb == 1)) c = (m == 0) ? std::sqrt(3.0) / 2.0 : 0.0;
+  else c = (m == 0) ? 1.5 : 0.75;
+  if (c == 0.0) return 0.0;
+  auto fact = [](int k) {
+    double f = 1.0;
+    for (int i = 2; i <= k; ++i) f *= i;
+    return f;
+  };
+  const double norm_a = std::pow(2.0 * za, na + 0.5) / std::sqrt(fact(2 * na));
+  const double norm_b = std::pow(2.0 * zb, nb + 0.5) / std::sqrt(fact(2 * nb));
+  const double hr = 0.5 * r;
+  const double p = (za + zb) * hr;
+  const double xi_max = 1.0 + 60.0 / std::max(p, 0.05);
+  static std::vector<double> gx48, gw48, gx40, gw40;
+  if (gx48.empty()) {
+    gauss_legendre(48, gx48, gw48);
+    gauss_legendre(40, gx40, gw40);
+  }
+  const double xm = 0.5 * (1.0 + xi_max), xh = 0.5 * (xi_max - 1.0);
+  double sum = 0.0;
+  for (int i = 0; i < 48; ++i) {
+    const double xi = xm + xh * gx48[i], wx = xh * gw48[i];
+    for (int j = 0; j < 40; ++j) {
+      const double eta = gx40[j], we = gw40[j];
+      const double ra = hr * (xi + eta), rb = hr * (xi - eta);
+      if (ra <= 0.0 || rb <= 0.0) continue;
+      const double ca = (1.0 + xi * eta) / (xi + eta), cb = (xi * eta - 1.0) / (xi - eta);
+      const double ang_a = la == 0 ? 1.0 : (m == 0 ? ca : std::sqrt(std::max(1.0 - ca * ca, 0.0)));
+      const double ang_b = lb == 0 ? 1.0 : (m == 0 ? cb : std::sqrt(std::max(1.0 - cb * cb, 0.0)));
+      const double radial = std::pow(ra, na - 1) * std::pow(rb, nb - 1) * std::exp(-za * ra - zb * rb);
+      sum += radial * (wx * we * ang_a * ang_b * (xi * xi - eta * eta));
+    }
+  }
+  return hr * hr * hr * sum * (norm_a * norm_b * c);
+}
+
+// Local-frame overlaps of the numerical path, in the analytic kernel's sign convention
+// (src/overlap_numeric.rs:24-43).
+void overlap_locals_numeric(const Element& ea, const Element& eb, double r, double s[5]) {
+  s[0] = sto_overlap(ea.n, ea.zeta_s, 0, 0, eb.n, eb.zeta_s, 0, r);
+  s[1] = sto_overlap(ea.n, ea.zeta_p, 1, 0, eb.n, eb.zeta_s, 0, r);
+  s[2] = -sto_overlap(ea.n, ea.zeta_s, 0, 0, eb.n, eb.zeta_p, 1, r);
+  s[3] = -sto_overlap(ea.n, ea.zeta_p, 1, 0, eb.n, eb.zeta_p, 1, r);
+  s[4] = sto_overlap(ea.n, ea.zeta_p, 1, 1, eb.n, eb.zeta_p, 1, r);
+}
+
+// Local-frame sigma/pi overlaps s111 s211 s121 s221 s222 for ea.n >= eb.n: closed forms for
+// n <= 3 (src/overlap.rs:120-298), the quadrature above for n >= 4.
 bool overlap_locals(const Element& ea, const Element& eb, double r, double s[5]) {
   int jcall;
   if (ea.n == 1 && eb.n == 1) jcall = 2;
@@ -582,7 +677,10 @@ bool overlap_locals(const Element& ea, const Element& eb, double r, double s[5])
   else if (ea.n == 3 && eb.n == 1) jcall = 431;
   else if (ea.n == 3 && eb.n == 2) jcall = 5;
   else if (ea.n == 3 && eb.n == 3) jcall = 6;
-  else return false;  // n >= 4 needs the numerical quadrature (not restated here)
+  else {
+    overlap_locals_numeric(ea, eb, r, s);
+    return true;
+  }
   double za[2] = {ea.zeta_s, ea.zeta_p}, zb[2] = {eb.zeta_s, eb.zeta_p};
   double a111[13], b111[13], a211[13], b211[13], a121[13], b121[13], a22[13], b22[13];
   aintgs(r * (0.5 * (za[0] + zb[0])), a111);
@@ -749,8 +847,8 @@ int build_mol(const Params& P, int n, const int* z, const double* xyz, Mol& m, s
       err = "missing element " + std::to_string(z[i]);
       return 2;
     }
-    if (e->n_orb == 9 || e->n > 3) {
-      err = "element " + std::to_string(z[i]) + " needs the MNDO/d or numeric-overlap kernels";
+    if (e->n_orb == 9) {
+      err = "element " + std::to_string(z[i]) + " needs the MNDO/d kernels";
       return 3;
     }
     m.el[i] = e;
@@ -2139,6 +2237,24 @@ int oracle_element(const char* data_dir, int method, int z, double* out9) {
   return 0;
 }
 
+// Local-frame overlaps (s111, s211, s121, s221, s222) of two Slater shells sets at distance r:
+// numeric != 0 forces the quadrature (also for n <= 3, to check it against the closed forms).
+int oracle_overlap_locals(int na, double zsa, double zpa, int nb, double zsb, double zpb, double r, int numeric,
+                          double* out5) {
+  Element ea{}, eb{};
+  ea.n = na;
+  ea.zeta_s = zsa;
+  ea.zeta_p = zpa;
+  eb.n = nb;
+  eb.zeta_s = zsb;
+  eb.zeta_p = zpb;
+  if (numeric) {
+    overlap_locals_numeric(ea, eb, r, out5);
+    return 0;
+  }
+  return overlap_locals(ea, eb, r, out5) ? 0 : 1;
+}
+
 int oracle_pair_block(const char* data_dir, int za, int zb, const double* d, double* w, double* e1b,
                       double* e2a, double* s) {
   const Params* P = load_params(data_dir, 0);
@@ -2146,7 +2262,7 @@ int oracle_pair_block(const char* data_dir, int za, int zb, const double* d, dou
   const Element* ea = P->get(za);
   const Element* eb = P->get(zb);
   if (!ea || !eb) return 2;
-  if (ea->n_orb == 9 || eb->n_orb == 9 || ea->n > 3 || eb->n > 3) return 3;
+  if (ea->n_orb == 9 || eb->n_orb == 9) return 3;
   PairBlock pb;
   V3 dv = {d[0], d[1], d[2]};
   pair_two_electron(*ea, *eb, dv, pb);
@@ -2324,7 +2440,7 @@ int oracle_run(const char* data_dir, int n_atoms, const int* z, const double* xy
   if (std::fabs(net) > 0.5) {
     double tot = 0.0;
     for (int i = 0; i < n_atoms; ++i) {
-      double mass = (z[i] >= 0 && z[i] <= 36) ? MASS[z[i]] : 0.0;
+      double mass = (z[i] >= 0 && z[i] <= 86) ? MASS[z[i]] : 0.0;
       origin.x += m.pos[i].x * mass;
       origin.y += m.pos[i].y * mass;
       origin.z += m.pos[i].z * mass;

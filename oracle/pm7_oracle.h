@@ -77,6 +77,10 @@ double oracle_core_core(const char* data_dir, int n_atoms, const int* z, const d
 void oracle_corrections(int n_atoms, const int* z, const double* xyz_bohr, double* out4);
 /* Symmetric eigensolve (Householder + implicit QL), ascending, vectors as columns. */
 int oracle_eigh(int n, const double* a, double* w, double* v);
+/* Local-frame Slater overlaps [s111, s211, s121, s221, s222] (atom a has the higher n);
+ * numeric != 0 forces the prolate-spheroidal quadrature used for n >= 4. */
+int oracle_overlap_locals(int na, double zsa, double zpa, int nb, double zsb, double zpb, double r, int numeric,
+                          double* out5);
 /* Parameter derived quantities for tests: [n_orb, dd, qq, rho0, rho1, rho2, e_isol, eheat_ev, po9]. */
 int oracle_element(const char* data_dir, int method, int z, double* out9);
 

@@ -62,10 +62,17 @@ struct DevBuf {
 
 double nan_() { return std::nan(""); }
 
-const double MASS[37] = {0.0,     1.0079,  4.0026,  6.94,     9.01218, 10.81,    12.011,  14.0067, 15.9994, 18.9984,
-                         20.179,  22.98977, 24.305, 26.98154, 28.0855, 30.97376, 32.06,   35.453,  39.948,  39.098,
-                         40.078,  44.956,  47.867,  50.942,   51.996,  54.938,   55.845,  58.933,  58.693,  63.546,
-                         65.38,   69.723,  72.63,   74.922,   78.971,  79.904,   83.798};
+// Atomic masses (u) for the centre-of-mass dipole origin (pm7-rs src/data_tables.rs MASS: standard
+// atomic weights through Rn; heavier elements have none there and count as 0 here).
+const double MASS[87] = {
+    0.0,     1.0079,  4.0026,  6.94,    9.01218, 10.81,   12.011,  14.0067, 15.9994, 18.9984, 20.179,  22.98977,
+    24.305,  26.98154, 28.0855, 30.97376, 32.06,  35.453,  39.948,  39.098,  40.078,  44.956,  47.867,  50.942,
+    51.996,  54.938,  55.845,  58.933,  58.693,  63.546,  65.38,   69.723,  72.63,   74.922,  78.971,  79.904,
+    83.798,  85.468,  87.62,   88.906,  91.224,  92.906,  95.95,   97.0,    101.07,  102.91,  106.42,  107.87,
+    112.41,  114.82,  118.71,  121.76,  127.6,   126.9,   131.29,  132.91,  137.33,  138.91,  140.12,  140.91,
+    144.24,  145.0,   150.36,  151.96,  157.25,  158.93,  162.5,   164.93,  167.26,  168.93,  173.05,  174.97,
+    178.49,  180.95,  183.84,  186.21,  190.23,  192.22,  195.08,  196.97,  200.59,  204.38,  207.2,   208.98,
+    209.0,   210.0,   222.0};
 
 std::string default_data_dir() {
   Dl_info info;
@@ -106,7 +113,7 @@ struct Lane {
   hipEvent_t ev[12] = {};
   hipEvent_t ev_var[2 * SCF_VARIANTS] = {};
   DevBuf d_mols, d_list, d_atom_z, d_atom_ao, d_atom_pp, d_xyz, d_pairs, d_items, d_gath_off;
-  DevBuf d_w, d_slab, d_vecs, d_out, d_ecore, d_ecore_mol, d_gath;
+  DevBuf d_w, d_slab, d_vecs, d_out, d_ecore, d_ecore_mol, d_gath, d_num, d_snum;
   std::string err;
   bool create(bool one_stream = false) {
     single_stream = one_stream;
@@ -123,7 +130,7 @@ struct Lane {
     for (auto& s : st)
       if (s) (void)hipStreamSynchronize(s);
     DevBuf* bufs[] = {&d_mols, &d_list, &d_atom_z, &d_atom_ao, &d_atom_pp, &d_xyz, &d_pairs, &d_items, &d_gath_off,
-                      &d_w, &d_slab, &d_vecs, &d_out, &d_ecore, &d_ecore_mol, &d_gath};
+                      &d_w, &d_slab, &d_vecs, &d_out, &d_ecore, &d_ecore_mol, &d_gath, &d_num, &d_snum};
     for (DevBuf* b : bufs) b->release();
     for (int v = 0; v < (single_stream ? 1 : SCF_VARIANTS); ++v)
       if (st[v]) (void)hipStreamDestroy(st[v]);
@@ -147,7 +154,7 @@ struct pm7_ctx {
   ParamSet params;
   int method = -1;
   std::vector<ElemDev> elem_host;
-  DevBuf d_elem, d_pairpar;
+  DevBuf d_elem, d_pairpar, d_gl;
   Lane lane;
   double timings[PM7_TIMING_SLOTS] = {};
   // Jacobi sweep cap of the first (cold) eigensolve in the reduced-precision phase of
@@ -241,7 +248,7 @@ pm7_status validate(pm7_ctx* ctx, const pm7_molecule_v1& m, const pm7_options_v1
       return PM7_ERR_MISSING_ELEMENT;
     }
     const HostElement& e = ctx->params.el[z];
-    if (e.n_orb == 9 || e.n > 3) {
+    if (e.n_orb == 9) {
       msg = "element " + std::to_string(z) + " needs the MNDO/d two-centre kernel, not built yet";
       return PM7_ERR_UNSUPPORTED;
     }
@@ -319,6 +326,7 @@ pm7_status prepare(pm7_ctx* ctx, Lane& L, Prepared& P, int64_t max_memory_mb) {
   std::vector<PairDev> pairs;
   std::vector<int2> items;
   std::vector<int64_t> gath_off(nm);
+  std::vector<int64_t> num_pairs;  // pairs whose overlaps need the quadrature (a shell with n >= 4)
   int64_t w_off = 0, mat = 0, vec = 0, gath = 0, pair0 = 0;
   int atom0 = 0;
   P.mols.resize(nm);
@@ -354,6 +362,11 @@ pm7_status prepare(pm7_ctx* ctx, Lane& L, Prepared& P, int64_t max_memory_mb) {
         }
         pd.w_off = w_off;
         w_off += (int64_t)npair_of(std::max(nu, nv)) * npair_of(std::min(nu, nv));
+        if (std::max(ctx->params.el[j.z[u]].n, ctx->params.el[j.z[v]].n) >= 4 ||
+            std::min(ctx->params.el[j.z[u]].n, ctx->params.el[j.z[v]].n) < 1) {
+          num_pairs.push_back((int64_t)pairs.size());
+          pd.pad = (int)num_pairs.size();  // 1-based slot of its numerical overlaps
+        }
         pairs.push_back(pd);
       }
     j.mat0 = mat;
@@ -440,6 +453,8 @@ pm7_status prepare(pm7_ctx* ctx, Lane& L, Prepared& P, int64_t max_memory_mb) {
   LANECHK(L.d_ecore.ensure((size_t)P.n_pairs * 8));
   LANECHK(L.d_ecore_mol.ensure((size_t)nm * 8));
   LANECHK(L.d_gath.ensure((size_t)P.gath_words * 8));
+  LANECHK(up(L.d_num, num_pairs.data(), num_pairs.size() * sizeof(int64_t)));
+  LANECHK(L.d_snum.ensure(num_pairs.size() * 5 * sizeof(double)));
   // Zero the slab: pad rows/columns of every matrix must be exact zeros.
   LANECHK(hipMemsetAsync(L.d_slab.p, 0, (size_t)P.slab_words * 8, L.st[0]));
   BatchDev& b = P.b;
@@ -462,6 +477,10 @@ pm7_status prepare(pm7_ctx* ctx, Lane& L, Prepared& P, int64_t max_memory_mb) {
   b.e_core_mol = L.d_ecore_mol.as<double>();
   b.gath = L.d_gath.as<double>();
   b.vpar = make_double4(ctx->params.vpar[1], ctx->params.vpar[2], ctx->params.vpar[3], ctx->params.vpar[4]);
+  b.num_pairs = L.d_num.as<int64_t>();
+  b.n_num = (int)num_pairs.size();
+  b.gl = ctx->d_gl.as<double>();
+  b.s_num = L.d_snum.as<double>();
   LANECHK(hipEventRecord(L.ev[0], L.st[0]));
   LANECHK(launch_integrals(b, P.n_pairs, P.n_items, nm, L.st[0]));
   LANECHK(hipEventRecord(L.ev[1], L.st[0]));
@@ -541,7 +560,7 @@ void finish_result(pm7_ctx* ctx, const Job& j, const MolOut& R, double e_core, c
   if (std::fabs(net) > 0.5) {
     double tot = 0.0;
     for (int i = 0; i < nat; ++i) {
-      double m = j.z[i] <= 36 ? MASS[j.z[i]] : 0.0;
+      double m = j.z[i] <= 86 ? MASS[j.z[i]] : 0.0;
       for (int k = 0; k < 3; ++k) org[k] += j.xyz[3 * i + k] * m;
       tot += m;
     }
@@ -581,6 +600,43 @@ void finish_result(pm7_ctx* ctx, const Job& j, const MolOut& R, double e_core, c
   }
 }
 
+// Gauss-Legendre nodes and weights on [-1, 1] for 48 and 40 points (x48, w48, x40, w40): roots of
+// P_n by Newton's method from the Chebyshev guess cos(pi (i + 3/4) / (n + 1/2)), weights
+// 2 / ((1 - x^2) P_n'(x)^2) (the rule of pm7-rs src/overlap_numeric.rs:137-175).
+void gauss_legendre_table(std::vector<double>& out) {
+  out.assign(GL_WORDS, 0.0);
+  auto legendre = [](int n, double x, double& dp) {
+    double p0 = 1.0, p1 = x;
+    for (int k = 2; k <= n; ++k) {
+      const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+      p0 = p1;
+      p1 = p2;
+    }
+    dp = n * (x * p1 - p0) / (x * x - 1.0);
+    return p1;
+  };
+  int base = 0;
+  for (int n : {48, 40}) {
+    double* x = out.data() + base;
+    double* w = x + n;
+    for (int i = 0; i < (n + 1) / 2; ++i) {
+      double t = std::cos(M_PI * (i + 0.75) / (n + 0.5)), dp = 1.0;
+      for (int it = 0; it < 100; ++it) {
+        const double dx = -legendre(n, t, dp) / dp;
+        t += dx;
+        if (std::fabs(dx) < 1.0e-15) break;
+      }
+      (void)legendre(n, t, dp);
+      const double wt = 2.0 / ((1.0 - t * t) * dp * dp);
+      x[i] = -t;
+      x[n - 1 - i] = t;
+      w[i] = wt;
+      w[n - 1 - i] = wt;
+    }
+    base += 2 * n;
+  }
+}
+
 pm7_status ensure_params(pm7_ctx* ctx, int method) {
   if (ctx->method == method) return PM7_OK;
   std::string err;
@@ -592,6 +648,12 @@ pm7_status ensure_params(pm7_ctx* ctx, int method) {
   for (int i = 0; i < MAX_Z * MAX_Z; ++i) pp[i] = make_double2(ctx->params.pair[i].first, ctx->params.pair[i].second);
   HIPCHK(ctx->d_pairpar.ensure(pp.size() * sizeof(double2)));
   HIPCHK(hipMemcpy(ctx->d_pairpar.p, pp.data(), pp.size() * sizeof(double2), hipMemcpyHostToDevice));
+  if (!ctx->d_gl.p) {
+    std::vector<double> gl;
+    gauss_legendre_table(gl);
+    HIPCHK(ctx->d_gl.ensure(gl.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(ctx->d_gl.p, gl.data(), gl.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   ctx->method = method;
   return PM7_OK;
 }
@@ -1011,6 +1073,7 @@ void pm7_ctx_destroy(pm7_ctx* ctx) {
   ctx->lane.destroy();
   ctx->d_elem.release();
   ctx->d_pairpar.release();
+  ctx->d_gl.release();
   delete ctx;
 }
 

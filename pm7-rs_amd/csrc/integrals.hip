@@ -310,18 +310,84 @@ __device__ void overlap_locals(const ElemDev& ea, const ElemDev& eb, double r, d
   }
 }
 
-// Molecular-frame 4x4 overlap block S[i][j] (atom f rows, atom s cols); d = R_s - R_f.
-__device__ void overlap_block(const ElemDev& ef, const ElemDev& es, const double d[3], double r, double S[4][4]) {
+// Numerical local overlaps for pairs with a valence shell n >= 4 (pm7-rs src/overlap_numeric.rs:24-115):
+// the five sigma/pi overlaps of Slater shells by quadrature in prolate spheroidal coordinates,
+// xi in [1, 1 + 60/p] (48 Gauss-Legendre nodes, p = (za + zb) r / 2), eta in [-1, 1] (40 nodes),
+// dV = (r/2)^3 (xi^2 - eta^2). One wave per pair: lanes split the 1920 nodes of each integral,
+// then a fixed shuffle tree sums them. Atom a is the one with the higher n.
+__global__ void __launch_bounds__(64) k_overlap_numeric(int n_num, const int64_t* __restrict__ num_pairs,
+                                                         const PairDev* __restrict__ pairs,
+                                                         const int* __restrict__ atom_z, const double* __restrict__ xyz,
+                                                         const ElemDev* __restrict__ elem,
+                                                         const double* __restrict__ gl, double* __restrict__ s_num) {
+  const int item = blockIdx.x;
+  if (item >= n_num) return;
+  const PairDev P = pairs[num_pairs[item]];
+  const ElemDev ef = elem[atom_z[P.f]], es = elem[atom_z[P.s]];
+  const bool swap = ef.n < es.n;
+  const ElemDev& ea = swap ? es : ef;
+  const ElemDev& eb = swap ? ef : es;
+  double d[3] = {xyz[3 * P.s] - xyz[3 * P.f], xyz[3 * P.s + 1] - xyz[3 * P.f + 1], xyz[3 * P.s + 2] - xyz[3 * P.f + 2]};
+  const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  const double hr = 0.5 * r;
+  const int lane = threadIdx.x;
+  // (zeta_a, zeta_b, l_a, l_b, m, sign): ss, p_sigma s, s p_sigma, p_sigma p_sigma, p_pi p_pi.
+  const double zas[5] = {ea.zeta_s, ea.zeta_p, ea.zeta_s, ea.zeta_p, ea.zeta_p};
+  const double zbs[5] = {eb.zeta_s, eb.zeta_s, eb.zeta_p, eb.zeta_p, eb.zeta_p};
+  const int las[5] = {0, 1, 0, 1, 1}, lbs[5] = {0, 0, 1, 1, 1}, ms[5] = {0, 0, 0, 0, 1};
+  const double sgn[5] = {1.0, 1.0, -1.0, -1.0, 1.0};
+  const double cst[5] = {0.5, 0.86602540378443864676, 0.86602540378443864676, 1.5, 0.75};
+  for (int q = 0; q < 5; ++q) {
+    const double za = zas[q], zb = zbs[q];
+    const int la = las[q], lb = lbs[q], m = ms[q];
+    const double p = (za + zb) * hr;
+    const double xi_max = 1.0 + 60.0 / fmax(p, 0.05);
+    const double xm = 0.5 * (1.0 + xi_max), xh = 0.5 * (xi_max - 1.0);
+    double acc = 0.0;
+    for (int t = lane; t < 48 * 40; t += 64) {
+      const int i = t / 40, j = t - 40 * i;
+      const double xi = xm + xh * gl[i], wx = xh * gl[48 + i];
+      const double eta = gl[96 + j], we = gl[136 + j];
+      const double ra = hr * (xi + eta), rb = hr * (xi - eta);
+      if (ra <= 0.0 || rb <= 0.0) continue;
+      const double ca = (1.0 + xi * eta) / (xi + eta), cb = (xi * eta - 1.0) / (xi - eta);
+      const double ang_a = la == 0 ? 1.0 : (m == 0 ? ca : sqrt(fmax(1.0 - ca * ca, 0.0)));
+      const double ang_b = lb == 0 ? 1.0 : (m == 0 ? cb : sqrt(fmax(1.0 - cb * cb, 0.0)));
+      double radial = exp(-za * ra - zb * rb);
+      for (int k = 1; k < ea.n; ++k) radial *= ra;
+      for (int k = 1; k < eb.n; ++k) radial *= rb;
+      if (ea.n < 1) radial /= ra;  // n = 0 shells (a few sparkle-like entries): r^(n-1) = 1/r
+      if (eb.n < 1) radial /= rb;
+      acc += radial * (wx * we * ang_a * ang_b * (xi * xi - eta * eta));
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (lane == 0) {
+      double fa = 1.0, fb = 1.0;
+      for (int k = 2; k <= 2 * ea.n; ++k) fa *= k;
+      for (int k = 2; k <= 2 * eb.n; ++k) fb *= k;
+      const double norm = pow(2.0 * za, ea.n + 0.5) / sqrt(fa) * pow(2.0 * zb, eb.n + 0.5) / sqrt(fb);
+      s_num[5 * (int64_t)item + q] = sgn[q] * hr * hr * hr * acc * norm * cst[q];
+    }
+  }
+}
+
+// Molecular-frame 4x4 overlap block S[i][j] (atom f rows, atom s cols); d = R_s - R_f. snum: the
+// pair's numerical local overlaps (valence n >= 4), else null.
+__device__ void overlap_block(const ElemDev& ef, const ElemDev& es, const double d[3], double r, double S[4][4],
+                              const double* snum) {
   double sl[5];
   double dir[3];
   bool swap = ef.n < es.n;
+  if (snum) {
+    for (int k = 0; k < 5; ++k) sl[k] = snum[k];
+  }
   if (!swap) {
-    overlap_locals(ef, es, r, sl);
+    if (!snum) overlap_locals(ef, es, r, sl);
     dir[0] = d[0] / r;
     dir[1] = d[1] / r;
     dir[2] = d[2] / r;
   } else {
-    overlap_locals(es, ef, r, sl);
+    if (!snum) overlap_locals(es, ef, r, sl);
     dir[0] = -d[0] / r;
     dir[1] = -d[1] / r;
     dir[2] = -d[2] / r;
@@ -398,7 +464,7 @@ __global__ void __launch_bounds__(128) k_pairs(int64_t n_pairs, const PairDev* _
                                                 const int* __restrict__ atom_ao, const double* __restrict__ xyz,
                                                 const ElemDev* __restrict__ elem, const double2* __restrict__ pairpar,
                                                 double4 vpar, double* __restrict__ w, double* __restrict__ slab,
-                                                double* __restrict__ e_core) {
+                                                double* __restrict__ e_core, const double* __restrict__ s_num) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n_pairs) return;
   const PairDev P = pairs[g];
@@ -433,7 +499,7 @@ __global__ void __launch_bounds__(128) k_pairs(int64_t n_pairs, const PairDev* _
   }
   // Resonance beta*S into the two off-diagonal blocks of H.
   double S[4][4];
-  overlap_block(ef, es, d, r, S);
+  overlap_block(ef, es, d, r, S, P.pad > 0 ? s_num + 5 * (int64_t)(P.pad - 1) : nullptr);
   const int ld = M.ld;
   double* H = slab + M.mat0 + (int64_t)S_H * ld * ld;
   int of = atom_ao[P.f], os = atom_ao[P.s];
@@ -532,10 +598,13 @@ __global__ void __launch_bounds__(256) k_gather(const MolDev* __restrict__ mols,
 }
 
 hipError_t launch_integrals(const BatchDev& b, int64_t n_pairs, int n_items, int n_mols, hipStream_t s) {
+  if (b.n_num > 0)
+    hipLaunchKernelGGL(k_overlap_numeric, dim3(b.n_num), dim3(64), 0, s, b.n_num, b.num_pairs, b.pairs, b.atom_z, b.xyz,
+                       b.elem, b.gl, b.s_num);
   if (n_pairs > 0) {
     int64_t blocks = (n_pairs + 127) / 128;
     hipLaunchKernelGGL(k_pairs, dim3((unsigned)blocks), dim3(128), 0, s, n_pairs, b.pairs, b.mols, b.atom_z, b.atom_ao,
-                       b.xyz, b.elem, b.pairpar, b.vpar, b.w, b.slab, b.e_core);
+                       b.xyz, b.elem, b.pairpar, b.vpar, b.w, b.slab, b.e_core, b.s_num);
   }
   if (n_items > 0) {
     hipLaunchKernelGGL(k_hcore_diag, dim3((n_items + 127) / 128), dim3(128), 0, s, n_items, b.hc_items, b.pairs, b.mols,

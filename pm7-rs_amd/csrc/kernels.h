@@ -30,7 +30,15 @@ struct BatchDev {
   double* e_core_mol = nullptr;    // per molecule
   double* gath = nullptr;          // packed outputs
   double4 vpar;
+  // Pairs whose overlap needs the numerical quadrature (a valence shell with n >= 4).
+  const int64_t* num_pairs = nullptr;
+  int n_num = 0;
+  const double* gl = nullptr;      // Gauss-Legendre nodes/weights: x48, w48, x40, w40 on [-1, 1]
+  double* s_num = nullptr;         // 5 local overlaps per numerical pair
 };
+
+// Gauss-Legendre nodes and weights on [-1, 1] for the overlap quadrature (48 and 40 points).
+constexpr int GL_WORDS = 2 * 48 + 2 * 40;
 
 // SCF kernel size classes (padded basis size): 0 <= 64 (256 threads, eigensolver fully in LDS),
 // 1 <= 112 (512 threads, fully in LDS), 2 <= 196 (packed matrix in LDS, rotations in L2),

@@ -20,10 +20,11 @@ constexpr double EV = 27.211386245988;        // eV per Hartree (CODATA 2018, MO
 constexpr double A0 = 0.529177210903;         // Bohr radius in Angstrom
 constexpr double EV_TO_KCAL = 23.06054783061903;
 constexpr double KCAL_TO_EV = 1.0 / EV_TO_KCAL;
-constexpr int MAX_Z = 87;
+constexpr int MAX_Z = 103;     // atomic numbers 1..102 (the parameter tables stop at nobelium)
 constexpr int NSLOT = 32;       // square matrices per molecule in the slab
 constexpr int HIST = 8;         // DIIS history depth
 constexpr int MIXED_MAX_ITERS = 30;  // longest reduced-precision phase of a mixed run
+constexpr double ROTATE_BELOW = 1.0e-3;  // density change below which that phase rotates orbitals
 
 // Slab slots, RHF.
 // C0/C1 double-buffer the sorted eigenvectors so each eigensolve can start from the last one;

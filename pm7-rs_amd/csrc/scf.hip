@@ -555,6 +555,7 @@ __global__ void __launch_bounds__(NT)
     s_int[1] = 0;
     s_int[3] = 0;
     s_dbl[0] = 0.0;
+    s_dbl[2] = 1.0e30;
   }
   __syncthreads();
   double* H = c.slot(S_H);
@@ -574,8 +575,27 @@ __global__ void __launch_bounds__(NT)
   res.err_norm = 0.0;
   res.fp32_iters = 0;
   res.e_fp32_last = 0.0;
-  if (threadIdx.x == 0) s_int[4] = c.M.precision == 1 ? 1 : 0;
-  __syncthreads();
+  // A mixed run starts in reduced precision only when the core Hamiltonian stays within the range
+  // where FP32 rounding (6e-8 relative) is far below the 1e-5 hand-over: max |H_ij| <= 1e3 eV.
+  // (Pseudo-atoms such as MOPAC's capped-bond element carry |beta| up to 1e7 eV.)
+  {
+    double hmax = 0.0;
+    const double* H0 = c.slot(S_H);
+    for (int idx = threadIdx.x; idx < c.M.nao * c.M.nao; idx += NT) {
+      const int i = idx / c.M.nao, j = idx - i * c.M.nao;
+      hmax = fmax(hmax, fabs(H0[(int64_t)i * c.ld + j]));
+    }
+    for (int off = 32; off > 0; off >>= 1) hmax = fmax(hmax, __shfl_down(hmax, off, 64));
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = hmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < Wg<NT>::NW; ++w) hmax = fmax(hmax, s_red[w]);
+      s_int[4] = (c.M.precision == 1 && hmax <= 1.0e3) ? 1 : 0;
+      s_int[5] = 0;         // reduced-precision iterations without progress
+      s_dbl[3] = 1.0e30;    // best density change of that phase
+    }
+    __syncthreads();
+  }
   // One FP64 eigensolve: to full accuracy, or, in the reduced-precision phase of a mixed run (its
   // first iteration; later ones rotate occupied against virtual orbitals only), to the accuracy
   // the FP32 density supports (off-diagonal norm 1e-7 of the total) in at most mixed_sweeps
@@ -683,7 +703,10 @@ __global__ void __launch_bounds__(NT)
       lap(PH_DIIS);
       double* Cnew = c.slot(cbuf ? S_C1 : S_C0);
       const bool low = s_int[4] != 0;
-      if (low && iter > 0)
+      // Occupied-virtual rotations only once the SCF is near its fixed point (last density change
+      // below ROTATE_BELOW); before that the reduced-precision phase diagonalises in full, so
+      // large early steps cannot carry it to another stationary state.
+      if (low && iter > 0 && s_dbl[2] < ROTATE_BELOW)
         occ_virt_rotate<NT>(n, ld, nocc, W, c.slot(cbuf ? S_C0 : S_C1), Cnew, c.slot(S_T), c.slot(S_AG), c.slot(S_VG));
       else
         sweeps += eig(low, W, iter > 0 ? c.slot(cbuf ? S_C0 : S_C1) : nullptr, Cnew, eps_a, S_AG, S_VG, S_T);
@@ -704,7 +727,15 @@ __global__ void __launch_bounds__(NT)
           // Reduced-precision phase: never converges here; hands over to FP64 below the switch.
           ++res.fp32_iters;
           res.e_fp32_last = e;
-          if (conv || dp < c.M.mixed_switch || res.fp32_iters >= MIXED_MAX_ITERS) s_int[4] = 0;
+          // Progress: the density change beats the phase's best by 20 %; three iterations
+          // without it mean FP32 noise has the upper hand, and FP64 takes over.
+          if (dp < 0.8 * s_dbl[3]) {
+            s_dbl[3] = dp;
+            s_int[5] = 0;
+          } else {
+            ++s_int[5];
+          }
+          if (conv || dp < c.M.mixed_switch || res.fp32_iters >= MIXED_MAX_ITERS || s_int[5] >= 3) s_int[4] = 0;
           else guard_update(s_guard, dp, e);
         } else if (conv) {
           s_int[3] = 1;
@@ -785,7 +816,7 @@ __global__ void __launch_bounds__(NT)
       double* Ca = c.slot(cbuf ? U_CA1 : U_CA0);
       double* Cb = c.slot(cbuf ? U_CB1 : U_CB0);
       const bool low = s_int[4] != 0;
-      if (low && iter > 0) {
+      if (low && iter > 0 && s_dbl[2] < ROTATE_BELOW) {
         occ_virt_rotate<NT>(n, ld, na, Fa, c.slot(cbuf ? U_CA0 : U_CA1), Ca, c.slot(U_T), c.slot(U_AG), c.slot(U_VG));
         occ_virt_rotate<NT>(n, ld, nb, Fb, c.slot(cbuf ? U_CB0 : U_CB1), Cb, c.slot(U_T), c.slot(U_AG), c.slot(U_VG));
       } else {
@@ -811,7 +842,15 @@ __global__ void __launch_bounds__(NT)
         if (low) {
           ++res.fp32_iters;
           res.e_fp32_last = e;
-          if (conv || dp < c.M.mixed_switch || res.fp32_iters >= MIXED_MAX_ITERS) s_int[4] = 0;
+          // Progress: the density change beats the phase's best by 20 %; three iterations
+          // without it mean FP32 noise has the upper hand, and FP64 takes over.
+          if (dp < 0.8 * s_dbl[3]) {
+            s_dbl[3] = dp;
+            s_int[5] = 0;
+          } else {
+            ++s_int[5];
+          }
+          if (conv || dp < c.M.mixed_switch || res.fp32_iters >= MIXED_MAX_ITERS || s_int[5] >= 3) s_int[4] = 0;
           else guard_update(s_guard, dp, e);
         } else if (conv) {
           s_int[3] = 1;

@@ -19,9 +19,13 @@ DATA_DIR = str(ROOT / "pm7-rs_amd" / "data")
 
 ANGSTROM_TO_BOHR = 1.0 / 0.529177210903
 
-SYMBOLS = ["X", "H", "He", "Li", "Be", "B", "C", "N", "O", "F", "Ne", "Na", "Mg", "Al", "Si", "P", "S",
-           "Cl", "Ar", "K", "Ca", "Sc", "Ti", "V", "Cr", "Mn", "Fe", "Co", "Ni", "Cu", "Zn", "Ga", "Ge",
-           "As", "Se", "Br", "Kr"]
+SYMBOLS = ["X", "H", "He", "Li", "Be", "B", "C", "N", "O", "F", "Ne", "Na", "Mg", "Al", "Si", "P", "S", "Cl",
+           "Ar", "K", "Ca", "Sc", "Ti", "V", "Cr", "Mn", "Fe", "Co", "Ni", "Cu", "Zn", "Ga", "Ge", "As",
+           "Se", "Br", "Kr", "Rb", "Sr", "Y", "Zr", "Nb", "Mo", "Tc", "Ru", "Rh", "Pd", "Ag", "Cd", "In",
+           "Sn", "Sb", "Te", "I", "Xe", "Cs", "Ba", "La", "Ce", "Pr", "Nd", "Pm", "Sm", "Eu", "Gd", "Tb",
+           "Dy", "Ho", "Er", "Tm", "Yb", "Lu", "Hf", "Ta", "W", "Re", "Os", "Ir", "Pt", "Au", "Hg", "Tl",
+           "Pb", "Bi", "Po", "At", "Rn", "Fr", "Ra", "Ac", "Th", "Pa", "U", "Np", "Pu", "Am", "Cm", "Bk",
+           "Cf", "Es", "Fm", "Md", "No"]
 Z_OF = {s: i for i, s in enumerate(SYMBOLS)}
 
 
@@ -67,6 +71,8 @@ def lib():
         _lib.oracle_eigh.argtypes = [C.c_int, dp, dp, dp]
         _lib.oracle_element.argtypes = [C.c_char_p, C.c_int, C.c_int, dp]
         _lib.oracle_default_options.argtypes = [C.POINTER(OracleOptions)]
+        _lib.oracle_overlap_locals.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
+                                               C.c_double, C.c_int, dp]
     return _lib
 
 
@@ -178,4 +184,13 @@ def element(z, method=0):
     st = lib().oracle_element(DATA_DIR.encode(), method, z, _d(out))
     if st:
         raise RuntimeError(f"oracle_element status {st}")
+    return out
+
+
+def overlap_locals(na, zsa, zpa, nb, zsb, zpb, r, numeric=False):
+    out = np.zeros(5)
+    st = lib().oracle_overlap_locals(int(na), float(zsa), float(zpa), int(nb), float(zsb), float(zpb), float(r),
+                                     1 if numeric else 0, _d(out))
+    if st:
+        raise RuntimeError("no closed form for these principal quantum numbers")
     return out

@@ -35,9 +35,10 @@ __device__ inline int pidx(int i, int j) {
 // The angles are always formed in FP64 (from the stored values), and V always accumulates in
 // FP64, so the rotations stay orthogonal to FP64 rounding whatever TA is. The sweeps stop when
 // off-diagonal^2 <= tol2 * total^2, or after max_sweeps.
+// V is split by rows: rows [0, vrows) at Vl (LDS), the rest at Vg (L2/HBM), stride n2 in both.
 template <int NT, int MAXN, bool V_LDS, typename TA>
-__device__ int jacobi_core(int n2, TA* A, double* V, double* s_c, double* s_s, double* s_t, int* s_p, int* s_q,
-                           double* red, double tol2, int max_sweeps) {
+__device__ int jacobi_core(int n2, TA* A, double* Vl, double* Vg, int vrows, double* s_c, double* s_s, double* s_t,
+                           int* s_p, int* s_q, double* red, double tol2, int max_sweeps) {
   constexpr int NX = 32, NY = NT / NX;  // 2-D thread grid: compile-time divisors only
   constexpr int CH = 2;                 // blocks in flight per thread
   constexpr int VR = V_LDS ? 4 : 8;     // accumulator rows in flight per thread
@@ -163,29 +164,76 @@ __device__ int jacobi_core(int n2, TA* A, double* V, double* s_c, double* s_s, d
           }
         }
       }
-      // Rotation accumulator: column pairs over tx, rows over ty, VR rows loaded before any store.
-      for (int k = tx; k < m; k += NX) {
-        const double ss = s_s[k];
-        if (ss == 0.0) continue;
-        const int pk = s_p[k], qk = s_q[k];
-        const double cc = s_c[k];
-        for (int i0 = ty; i0 < n2; i0 += VR * NY) {
-          double vp[VR], vq[VR];
+      // Rotation accumulator: column pairs over tx, rows over ty, VR rows loaded before any store;
+      // the rows held in LDS and the rest are separate passes, so each pass addresses one
+      // memory space directly.
+      auto rotate_rows = [&](double* __restrict__ V, int r0, int r1) {
+        for (int k = tx; k < m; k += NX) {
+          const double ss = s_s[k];
+          if (ss == 0.0) continue;
+          const int pk = s_p[k], qk = s_q[k];
+          const double cc = s_c[k];
+          for (int i0 = r0 + ty; i0 < r1; i0 += VR * NY) {
+            double vp[VR], vq[VR];
 #pragma unroll
-          for (int u = 0; u < VR; ++u) {
-            const int i = i0 + u * NY;
-            vp[u] = i < n2 ? V[i * n2 + pk] : 0.0;
-            vq[u] = i < n2 ? V[i * n2 + qk] : 0.0;
-          }
+            for (int u = 0; u < VR; ++u) {
+              const int i = i0 + u * NY;
+              vp[u] = i < r1 ? V[i * n2 + pk] : 0.0;
+              vq[u] = i < r1 ? V[i * n2 + qk] : 0.0;
+            }
 #pragma unroll
-          for (int u = 0; u < VR; ++u) {
-            const int i = i0 + u * NY;
-            if (i < n2) {
-              V[i * n2 + pk] = cc * vp[u] - ss * vq[u];
-              V[i * n2 + qk] = ss * vp[u] + cc * vq[u];
+            for (int u = 0; u < VR; ++u) {
+              const int i = i0 + u * NY;
+              if (i < r1) {
+                V[i * n2 + pk] = cc * vp[u] - ss * vq[u];
+                V[i * n2 + qk] = ss * vp[u] + cc * vq[u];
+              }
             }
           }
         }
+      };
+      if (vrows == n2) {
+        rotate_rows(Vl, 0, n2);
+      } else if (vrows > 0 && n2 - vrows <= NY) {
+        // At most one row per thread lives in L2: its loads are issued first and complete
+        // while the thread works through its LDS rows.
+        const int ig = vrows + ty;
+        const bool hg = ig < n2;
+        for (int k = tx; k < m; k += NX) {
+          const double ss = s_s[k];
+          if (ss == 0.0) continue;
+          const int pk = s_p[k], qk = s_q[k];
+          const double cc = s_c[k];
+          double gp = 0.0, gq = 0.0;
+          if (hg) {
+            gp = Vg[ig * n2 + pk];
+            gq = Vg[ig * n2 + qk];
+          }
+          for (int i0 = ty; i0 < vrows; i0 += VR * NY) {
+            double vp[VR], vq[VR];
+#pragma unroll
+            for (int u = 0; u < VR; ++u) {
+              const int i = i0 + u * NY;
+              vp[u] = i < vrows ? Vl[i * n2 + pk] : 0.0;
+              vq[u] = i < vrows ? Vl[i * n2 + qk] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < VR; ++u) {
+              const int i = i0 + u * NY;
+              if (i < vrows) {
+                Vl[i * n2 + pk] = cc * vp[u] - ss * vq[u];
+                Vl[i * n2 + qk] = ss * vp[u] + cc * vq[u];
+              }
+            }
+          }
+          if (hg) {
+            Vg[ig * n2 + pk] = cc * gp - ss * gq;
+            Vg[ig * n2 + qk] = ss * gp + cc * gq;
+          }
+        }
+      } else {
+        if (vrows > 0) rotate_rows(Vl, 0, vrows);
+        rotate_rows(Vg, vrows, n2);
       }
       __syncthreads();
     }
@@ -199,7 +247,7 @@ __device__ int jacobi_core(int n2, TA* A, double* V, double* s_c, double* s_s, d
 // diagonal once the SCF settles, and Cout = Cprev V. A and V sit in LDS when the variant allows.
 // TA, tol2 and max_sweeps are passed to jacobi_core (the products around it are FP64 either way).
 template <int NT, int MAXN, bool A_LDS, bool V_LDS, typename TA>
-__device__ int eigensolve(int n, int ld, const double* __restrict__ W, const double* __restrict__ Cprev,
+__device__ __attribute__((noinline)) int eigensolve(int n, int ld, const double* __restrict__ W, const double* __restrict__ Cprev,
                           double* __restrict__ Cout, double* __restrict__ eps, double* Ag, double* Vg,
                           double* __restrict__ T, double* s_a, double* s_v, double* s_c, double* s_s, double* s_t,
                           int* s_perm, int* s_p, int* s_q, double* red, unsigned long long* tsub, double tol2,
@@ -207,7 +255,15 @@ __device__ int eigensolve(int n, int ld, const double* __restrict__ W, const dou
   const int n2 = ld;
   unsigned long long t0 = wall_clock64();
   TA* A = reinterpret_cast<TA*>(A_LDS ? s_a : Ag);
-  double* V = V_LDS ? s_v : Vg;
+  // Rotation accumulator: all in LDS (V_LDS), or as many leading rows as fit in the LDS the packed
+  // matrix leaves free (A_LDS: the buffer holds MAXN(MAXN+1)/2 doubles), the rest in Vg.
+  double* Vl = V_LDS ? s_v : (A_LDS ? s_a + (n2 * (n2 + 1) / 2 + 1) / 2 * 2 : nullptr);
+  int vrows = n2;
+  if (!V_LDS) {
+    const int free_words = A_LDS ? MAXN * (MAXN + 1) / 2 - (n2 * (n2 + 1) / 2 + 1) / 2 * 2 : 0;
+    vrows = free_words > 0 ? min(n2, free_words / n2) : 0;
+  }
+  auto vat = [&](int i, int j) -> double& { return i < vrows ? Vl[i * n2 + j] : Vg[i * n2 + j]; };
   constexpr int NW = NT / 64;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, q = lane >> 4;
   const int nb = (n + 15) >> 4;
@@ -246,11 +302,12 @@ __device__ int eigensolve(int n, int ld, const double* __restrict__ W, const dou
   }
   for (int idx = threadIdx.x; idx < n2 * n2; idx += NT) {
     const int i = idx / n2, j = idx - i * n2;
-    V[idx] = (i == j) ? 1.0 : 0.0;
+    vat(i, j) = (i == j) ? 1.0 : 0.0;
   }
   __syncthreads();
   unsigned long long t1 = wall_clock64();
-  const int sweeps = jacobi_core<NT, MAXN, V_LDS, TA>(n2, A, V, s_c, s_s, s_t, s_p, s_q, red, tol2, max_sweeps);
+  const int sweeps =
+      jacobi_core<NT, MAXN, V_LDS, TA>(n2, A, Vl, Vg, vrows, s_c, s_s, s_t, s_p, s_q, red, tol2, max_sweeps);
   unsigned long long t2 = wall_clock64();
   for (int i = threadIdx.x; i < n; i += NT) {
     const double di = (double)A[pidx(i, i)];
@@ -268,7 +325,7 @@ __device__ int eigensolve(int n, int ld, const double* __restrict__ W, const dou
     for (int t = wave; t < nb * nb; t += NW) {
       const int I = t / nb, J = t - I * nb, i0 = 16 * I, j0 = 16 * J;
       auto c_ik = [&](int i, int k) { return (i < n && k < n) ? Cprev[(int64_t)i * ld + k] : 0.0; };
-      auto v_kj = [&](int k, int j) { return (k < n && j < n) ? V[k * n2 + s_perm[j]] : 0.0; };
+      auto v_kj = [&](int k, int j) { return (k < n && j < n) ? vat(k, s_perm[j]) : 0.0; };
       const dbl4 x = tile16_f64(n, i0, j0, c_ik, v_kj);
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
@@ -279,7 +336,7 @@ __device__ int eigensolve(int n, int ld, const double* __restrict__ W, const dou
   } else {
     for (int idx = threadIdx.x; idx < n2 * n2; idx += NT) {
       const int i = idx / n2, k = idx - i * n2;
-      Cout[idx] = (i < n && k < n) ? V[i * n2 + s_perm[k]] : 0.0;
+      Cout[idx] = (i < n && k < n) ? vat(i, s_perm[k]) : 0.0;
     }
   }
   __syncthreads();

@@ -153,14 +153,16 @@ def main() -> int:
     dp_mx = max(float(np.max(np.abs(a.density - b.density))) for a, b in zip(resmx, res64))
     mixed_ok = dhf_mx <= 1e-4 and not badmx
 
-    # Serving: Poisson arrivals at 75 % of the measured FP64 batch capacity into the
-    # continuous-batching server; latency is submit -> completion per request.
+    # Serving: the continuous-batching server's FP64 capacity is measured first (every request
+    # submitted at once), then Poisson arrivals at 75 % of it for FP64 and for mixed precision;
+    # latency is submit -> completion per request.
     serving = None
     if args.serving > 0:
         import threading
-        rate = 0.75 * solves64
-        stream = W.serving_stream(args.serving, seed=5, rate_per_s=rate, frac_sp=0.8, frac_d=0.0)
-        reqs = [(t, pm7amd.Molecule.from_symbols_angstrom(a, c, m), c, m) for t, a, c, m, _ in stream]
+        stream = W.serving_stream(args.serving, seed=5, rate_per_s=1.0, frac_sp=0.8, frac_d=0.0)
+        mols_s = [(pm7amd.Molecule.from_symbols_angstrom(a, c, m), c, m) for _, a, c, m, _ in stream]
+        gaps = [t for t, *_ in stream]  # arrival times at rate 1/s; scaled below
+        reqs = [(0.0, mol, c, m) for mol, c, m in mols_s]
 
         def serve(precision):
             lat, bad = [], 0
@@ -190,7 +192,11 @@ def main() -> int:
                     "latency_ms": {"p50": round(pct(lat, 0.5), 3), "p99": round(pct(lat, 0.99), 3)}}
 
         serve("fp64")  # warm the server lanes' buffers
-        serving = {"requests": len(reqs), "rate_per_s": round(rate, 1), "lanes": args.serve_lanes,
+        capacity = serve("fp64")  # all requests at t = 0
+        rate = 0.75 * capacity["solves_per_s"]
+        reqs = [(t / rate, mol, c, m) for t, (mol, c, m) in zip(gaps, mols_s)]
+        serving = {"requests": len(reqs), "capacity_fp64_solves_per_s": capacity["solves_per_s"],
+                   "rate_per_s": round(rate, 1), "lanes": args.serve_lanes,
                    "max_batch": args.serve_batch,
                    "mix": "80% closed-shell H/C/N/O (3-60 atoms), 20% UHF doublets (one H removed); "
                           "S/P/Cl requests need the MNDO/d kernel (not built) and are left out",

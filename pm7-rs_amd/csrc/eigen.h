@@ -37,8 +37,8 @@ __device__ inline int pidx(int i, int j) {
 // off-diagonal^2 <= tol2 * total^2, or after max_sweeps.
 // V is split by rows: rows [0, vrows) at Vl (LDS), the rest at Vg (L2/HBM), stride n2 in both.
 template <int NT, int MAXN, bool V_LDS, typename TA>
-__device__ int jacobi_core(int n2, TA* A, double* Vl, double* Vg, int vrows, double* s_c, double* s_s, double* s_t,
-                           int* s_p, int* s_q, double* red, double tol2, int max_sweeps) {
+__device__ int jacobi_core(int n2, TA* A, double* Vl, double* Vg, int vrows, double2* s_cs, double* s_t, int* s_pq,
+                           double* red, double tol2, int max_sweeps) {
   constexpr int NX = 32, NY = NT / NX;  // 2-D thread grid: compile-time divisors only
   constexpr int CH = 2;                 // blocks in flight per thread
   constexpr int VR = V_LDS ? 4 : 8;     // accumulator rows in flight per thread
@@ -82,10 +82,8 @@ __device__ int jacobi_core(int n2, TA* A, double* Vl, double* Vg, int vrows, dou
           cc = sqrt(den / (2.0 * rr));
           ss = tt * cc;
         }
-        s_p[k] = p;
-        s_q[k] = q;
-        s_c[k] = cc;
-        s_s[k] = ss;
+        s_pq[k] = p | (q << 16);  // n2 <= 1024
+        s_cs[k] = make_double2(cc, ss);
         s_t[k] = tt;
       }
       __syncthreads();
@@ -118,11 +116,13 @@ __device__ int jacobi_core(int n2, TA* A, double* Vl, double* Vg, int vrows, dou
             act[u] = k >= 0;
             dg[u] = k == l;
             if (act[u]) {
-              const int pk = s_p[k], qk = s_q[k], pl = s_p[l], ql = s_q[l];
-              ck[u] = (TA)s_c[k];
-              sk[u] = (TA)s_s[k];
-              cl[u] = (TA)s_c[l];
-              sl[u] = (TA)s_s[l];
+              const int pqk = s_pq[k], pql = s_pq[l];
+              const int pk = pqk & 0xffff, qk = pqk >> 16, pl = pql & 0xffff, ql = pql >> 16;
+              const double2 csk = s_cs[k], csl = s_cs[l];
+              ck[u] = (TA)csk.x;
+              sk[u] = (TA)csk.y;
+              cl[u] = (TA)csl.x;
+              sl[u] = (TA)csl.y;
               if (dg[u]) {
                 sl[u] = (TA)s_t[k];  // diagonal block: a_pp -= t a_pq, a_qq += t a_pq, a_pq = 0
                 i0[u] = pidx(pk, qk);
@@ -169,10 +169,11 @@ __device__ int jacobi_core(int n2, TA* A, double* Vl, double* Vg, int vrows, dou
       // memory space directly.
       auto rotate_rows = [&](double* __restrict__ V, int r0, int r1) {
         for (int k = tx; k < m; k += NX) {
-          const double ss = s_s[k];
+          const double2 cs = s_cs[k];
+          const double cc = cs.x, ss = cs.y;
           if (ss == 0.0) continue;
-          const int pk = s_p[k], qk = s_q[k];
-          const double cc = s_c[k];
+          const int pq = s_pq[k];
+          const int pk = pq & 0xffff, qk = pq >> 16;
           for (int i0 = r0 + ty; i0 < r1; i0 += VR * NY) {
             double vp[VR], vq[VR];
 #pragma unroll
@@ -200,10 +201,11 @@ __device__ int jacobi_core(int n2, TA* A, double* Vl, double* Vg, int vrows, dou
         const int ig = vrows + ty;
         const bool hg = ig < n2;
         for (int k = tx; k < m; k += NX) {
-          const double ss = s_s[k];
+          const double2 cs = s_cs[k];
+          const double cc = cs.x, ss = cs.y;
           if (ss == 0.0) continue;
-          const int pk = s_p[k], qk = s_q[k];
-          const double cc = s_c[k];
+          const int pq = s_pq[k];
+          const int pk = pq & 0xffff, qk = pq >> 16;
           double gp = 0.0, gq = 0.0;
           if (hg) {
             gp = Vg[ig * n2 + pk];
@@ -249,8 +251,8 @@ __device__ int jacobi_core(int n2, TA* A, double* Vl, double* Vg, int vrows, dou
 template <int NT, int MAXN, bool A_LDS, bool V_LDS, typename TA>
 __device__ __attribute__((noinline)) int eigensolve(int n, int ld, const double* __restrict__ W, const double* __restrict__ Cprev,
                           double* __restrict__ Cout, double* __restrict__ eps, double* Ag, double* Vg,
-                          double* __restrict__ T, double* s_a, double* s_v, double* s_c, double* s_s, double* s_t,
-                          int* s_perm, int* s_p, int* s_q, double* red, unsigned long long* tsub, double tol2,
+                          double* __restrict__ T, double* s_a, double* s_v, double2* s_cs, double* s_t,
+                          int* s_perm, int* s_pq, double* red, unsigned long long* tsub, double tol2,
                           int max_sweeps) {
   const int n2 = ld;
   unsigned long long t0 = wall_clock64();
@@ -307,7 +309,7 @@ __device__ __attribute__((noinline)) int eigensolve(int n, int ld, const double*
   __syncthreads();
   unsigned long long t1 = wall_clock64();
   const int sweeps =
-      jacobi_core<NT, MAXN, V_LDS, TA>(n2, A, Vl, Vg, vrows, s_c, s_s, s_t, s_p, s_q, red, tol2, max_sweeps);
+      jacobi_core<NT, MAXN, V_LDS, TA>(n2, A, Vl, Vg, vrows, s_cs, s_t, s_pq, red, tol2, max_sweeps);
   unsigned long long t2 = wall_clock64();
   for (int i = threadIdx.x; i < n; i += NT) {
     const double di = (double)A[pidx(i, i)];

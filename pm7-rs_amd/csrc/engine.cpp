@@ -33,6 +33,29 @@ using namespace pm7;
 
 namespace {
 
+// Grow-only pinned host buffer (device-to-host copies at full link speed).
+struct PinnedBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap && p) return hipSuccess;
+    if (p) {
+      (void)hipHostFree(p);
+      p = nullptr;
+      cap = 0;
+    }
+    size_t want = std::max<size_t>(bytes + bytes / 4, 4096);
+    hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
@@ -114,6 +137,7 @@ struct Lane {
   hipEvent_t ev_var[2 * SCF_VARIANTS] = {};
   DevBuf d_mols, d_list, d_atom_z, d_atom_ao, d_atom_pp, d_xyz, d_pairs, d_items, d_gath_off;
   DevBuf d_w, d_slab, d_vecs, d_out, d_ecore, d_ecore_mol, d_gath, d_num, d_snum;
+  PinnedBuf h_out;  // staging for the packed results
   std::string err;
   bool create(bool one_stream = false) {
     single_stream = one_stream;
@@ -132,6 +156,7 @@ struct Lane {
     DevBuf* bufs[] = {&d_mols, &d_list, &d_atom_z, &d_atom_ao, &d_atom_pp, &d_xyz, &d_pairs, &d_items, &d_gath_off,
                       &d_w, &d_slab, &d_vecs, &d_out, &d_ecore, &d_ecore_mol, &d_gath, &d_num, &d_snum};
     for (DevBuf* b : bufs) b->release();
+    h_out.release();
     for (int v = 0; v < (single_stream ? 1 : SCF_VARIANTS); ++v)
       if (st[v]) (void)hipStreamDestroy(st[v]);
     for (auto& e : ev)
@@ -660,15 +685,23 @@ pm7_status ensure_params(pm7_ctx* ctx, int method) {
 
 // Geometric corrections of every job (host; overlaps the device work).
 void host_corrections(std::vector<Job>& jobs) {
-  for (auto& j : jobs) {
-    j.corr_kcal = 0.0;
-    if (j.o.method != PM7_METHOD_PM7_MINUS) {
-      j.corr_kcal += dispersion_kcal(j.z, j.xyz);
-      j.corr_kcal += hbond_kcal(j.z, j.xyz);
-      j.corr_kcal += c_triple_kcal(j.z, j.xyz);
-      j.corr_kcal += si_o_h_kcal(j.z, j.xyz);
+  auto work = [&](int t, int nt) {
+    for (size_t k = t; k < jobs.size(); k += nt) {
+      Job& j = jobs[k];
+      j.corr_kcal = 0.0;
+      if (j.o.method != PM7_METHOD_PM7_MINUS) {
+        j.corr_kcal += dispersion_kcal(j.z, j.xyz);
+        j.corr_kcal += hbond_kcal(j.z, j.xyz);
+        j.corr_kcal += c_triple_kcal(j.z, j.xyz);
+        j.corr_kcal += si_o_h_kcal(j.z, j.xyz);
+      }
     }
-  }
+  };
+  const int nt = jobs.size() >= 32 ? std::min(8, std::max(1, (int)std::thread::hardware_concurrency() / 2)) : 1;
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(work, t, nt);
+  work(0, nt);
+  for (auto& th : pool) th.join();
 }
 
 // SCF megakernels (one stream per size class, all after the integrals), then the output gather
@@ -695,17 +728,35 @@ pm7_status launch_batch(Lane& L, const Prepared& P, int* count) {
 // Copy a finished batch's outputs to the host and fill results[j.idx] for every job.
 pm7_status collect_batch(pm7_ctx* ctx, Lane& L, const Prepared& P, pm7_result_v1* results) {
   const int nm = (int)P.jobs.size();
-  std::vector<MolOut> outs(nm);
-  std::vector<double> ecore(nm), vecs(P.vec_words), gath(P.gath_words);
-  LANECHK(hipMemcpyAsync(outs.data(), P.b.out, nm * sizeof(MolOut), hipMemcpyDeviceToHost, L.st[0]));
-  LANECHK(hipMemcpyAsync(ecore.data(), P.b.e_core_mol, nm * sizeof(double), hipMemcpyDeviceToHost, L.st[0]));
-  LANECHK(hipMemcpyAsync(vecs.data(), P.b.vecs, P.vec_words * sizeof(double), hipMemcpyDeviceToHost, L.st[0]));
-  LANECHK(hipMemcpyAsync(gath.data(), P.b.gath, P.gath_words * sizeof(double), hipMemcpyDeviceToHost, L.st[0]));
+  // One pinned staging area: MolOut records, core energies, eigenvalues, packed matrices.
+  const size_t o_out = 0;
+  const size_t o_ecore = o_out + ((size_t)nm * sizeof(MolOut) + 255) / 256 * 256;
+  const size_t o_vecs = o_ecore + ((size_t)nm * 8 + 255) / 256 * 256;
+  const size_t o_gath = o_vecs + ((size_t)P.vec_words * 8 + 255) / 256 * 256;
+  const size_t total = o_gath + (size_t)P.gath_words * 8;
+  LANECHK(L.h_out.ensure(total));
+  char* h = static_cast<char*>(L.h_out.p);
+  const MolOut* outs = reinterpret_cast<const MolOut*>(h + o_out);
+  const double* ecore = reinterpret_cast<const double*>(h + o_ecore);
+  const double* vecs = reinterpret_cast<const double*>(h + o_vecs);
+  const double* gath = reinterpret_cast<const double*>(h + o_gath);
+  LANECHK(hipMemcpyAsync(h + o_out, P.b.out, nm * sizeof(MolOut), hipMemcpyDeviceToHost, L.st[0]));
+  LANECHK(hipMemcpyAsync(h + o_ecore, P.b.e_core_mol, nm * sizeof(double), hipMemcpyDeviceToHost, L.st[0]));
+  LANECHK(hipMemcpyAsync(h + o_vecs, P.b.vecs, P.vec_words * sizeof(double), hipMemcpyDeviceToHost, L.st[0]));
+  LANECHK(hipMemcpyAsync(h + o_gath, P.b.gath, P.gath_words * sizeof(double), hipMemcpyDeviceToHost, L.st[0]));
   LANECHK(hipStreamSynchronize(L.st[0]));
-  for (int mi = 0; mi < nm; ++mi) {
-    const Job& j = P.jobs[mi];
-    finish_result(ctx, j, outs[mi], ecore[mi], vecs.data() + j.vec0, gath.data() + j.gath0, &results[j.idx]);
-  }
+  // Results are independent: a few host threads assemble them (each writes only its own).
+  auto work = [&](int t, int nt) {
+    for (int mi = t; mi < nm; mi += nt) {
+      const Job& j = P.jobs[mi];
+      finish_result(ctx, j, outs[mi], ecore[mi], vecs + j.vec0, gath + j.gath0, &results[j.idx]);
+    }
+  };
+  const int nt = nm >= 32 ? std::min(8, std::max(1, (int)std::thread::hardware_concurrency() / 2)) : 1;
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(work, t, nt);
+  work(0, nt);
+  for (auto& th : pool) th.join();
   return PM7_OK;
 }
 

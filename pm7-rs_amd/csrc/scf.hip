@@ -521,9 +521,10 @@ __global__ void __launch_bounds__(NT)
   __shared__ double s_a[A_LDS ? MAXN * (MAXN + 1) / 2 : 1];
   __shared__ double s_v[V_LDS ? MAXN * MAXN : 1];
   __shared__ double s_red[(Wg<NT>::NW + 1) * 3 * HIST];
-  __shared__ double s_c[MAXN / 2], s_s[MAXN / 2], s_t[MAXN / 2];
+  __shared__ double2 s_cs[MAXN / 2];  // this round's (cos, sin) per rotation pair
+  __shared__ double s_t[MAXN / 2];     // and tan
   __shared__ int s_perm[MAXN];
-  __shared__ int s_pq[MAXN];  // this round's rotation pairs (p then q)
+  __shared__ int s_pq[MAXN / 2];  // this round's rotation pairs, p | q << 16
   __shared__ double s_G[HIST * HIST], s_M[HIST * HIST], s_coef[HIST + 1];
   __shared__ int s_slot[HIST], s_cslot[HIST];
   __shared__ double s_wk[(HIST + 1) * (HIST + 3)];  // bordered DIIS system
@@ -602,7 +603,7 @@ __global__ void __launch_bounds__(NT)
   // sweeps.
   auto eig = [&](bool low, const double* Wm, const double* Cp, double* Co, double* ev, int ag, int vg, int tg) -> int {
     return eigensolve<NT, MAXN, A_LDS, V_LDS, double>(n, ld, Wm, Cp, Co, ev, c.slot(ag), c.slot(vg), c.slot(tg), s_a,
-                                                      s_v, s_c, s_s, s_t, s_perm, s_pq, s_pq + MAXN / 2, s_red,
+                                                      s_v, s_cs, s_t, s_perm, s_pq, s_red,
                                                       tk + PH_EIG_IN, low ? 1.0e-14 : 1.0e-26,
                                                       low ? c.M.mixed_sweeps : 40);
   };

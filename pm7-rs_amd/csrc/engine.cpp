@@ -138,6 +138,7 @@ struct Lane {
   DevBuf d_mols, d_list, d_atom_z, d_atom_ao, d_atom_pp, d_xyz, d_pairs, d_items, d_gath_off;
   DevBuf d_w, d_slab, d_vecs, d_out, d_ecore, d_ecore_mol, d_gath, d_num, d_snum;
   PinnedBuf h_out;  // staging for the packed results
+  LargeWork* large = nullptr;  // rocBLAS handle and buffers of the large-system path (lazily made)
   std::string err;
   bool create(bool one_stream = false) {
     single_stream = one_stream;
@@ -157,6 +158,8 @@ struct Lane {
                       &d_w, &d_slab, &d_vecs, &d_out, &d_ecore, &d_ecore_mol, &d_gath, &d_num, &d_snum};
     for (DevBuf* b : bufs) b->release();
     h_out.release();
+    if (large) large_work_destroy(large);
+    large = nullptr;
     for (int v = 0; v < (single_stream ? 1 : SCF_VARIANTS); ++v)
       if (st[v]) (void)hipStreamDestroy(st[v]);
     for (auto& e : ev)
@@ -310,7 +313,7 @@ pm7_status validate(pm7_ctx* ctx, const pm7_molecule_v1& m, const pm7_options_v1
       break;
     }
   if (j.variant < 0) {
-    msg = "molecules above 1024 basis functions need the multi-workgroup path, not built yet";
+    msg = "no SCF path for " + std::to_string(nao) + " basis functions";
     return PM7_ERR_UNSUPPORTED;
   }
   j.o = o;
@@ -707,14 +710,50 @@ void host_corrections(std::vector<Job>& jobs) {
 // SCF megakernels (one stream per size class, all after the integrals), then the output gather
 // on stream 0; L.ev[5] marks the end of the batch.
 pm7_status launch_batch(Lane& L, const Prepared& P, int* count) {
-  const int nm = (int)P.jobs.size();
   for (int v = 1; v < SCF_VARIANTS && !L.single_stream; ++v) LANECHK(hipStreamWaitEvent(L.st[v], L.ev[1], 0));
   for (int v = 0; v < SCF_VARIANTS; ++v) {
     count[v] = P.list_off[v + 1] - P.list_off[v];
+    if (v == LARGE_VARIANT) continue;  // run_large
     LANECHK(hipEventRecord(L.ev_var[2 * v], L.st[v]));
     LANECHK(launch_scf(v, P.b, P.list_off[v], count[v], L.st[v]));
     LANECHK(hipEventRecord(L.ev_var[2 * v + 1], L.st[v]));
   }
+  return PM7_OK;
+}
+
+// The large-class molecules, one after another on the large class's stream (each fills the whole
+// device while the megakernels of the other classes run beside it); their MolOut records are
+// written where the megakernels would have written them.
+pm7_status run_large(Lane& L, const Prepared& P) {
+  const int off = P.list_off[LARGE_VARIANT], cnt = P.list_off[LARGE_VARIANT + 1] - off;
+  if (cnt == 0) return PM7_OK;
+  if (!L.large) {
+    L.large = large_work_create();
+    if (!L.large) {
+      L.err = "rocBLAS handle creation failed";
+      return PM7_ERR_DEVICE;
+    }
+  }
+  hipStream_t s = L.st[LARGE_VARIANT];
+  LANECHK(hipEventRecord(L.ev_var[2 * LARGE_VARIANT], s));
+  std::vector<std::pair<int, int>> order;  // largest first
+  for (int mi = 0; mi < (int)P.jobs.size(); ++mi)
+    if (P.jobs[mi].variant == LARGE_VARIANT) order.push_back({-P.jobs[mi].nao, mi});
+  std::stable_sort(order.begin(), order.end());
+  for (auto& o : order) {
+    const int mi = o.second;
+    MolOut out{};
+    if (large_scf(L.large, P.b, P.mols[mi], mi, s, out, L.err) != hipSuccess) return PM7_ERR_DEVICE;
+    LANECHK(hipMemcpyAsync(P.b.out + mi, &out, sizeof out, hipMemcpyHostToDevice, s));
+    LANECHK(hipStreamSynchronize(s));
+  }
+  LANECHK(hipEventRecord(L.ev_var[2 * LARGE_VARIANT + 1], s));
+  return PM7_OK;
+}
+
+// Join the class streams and gather the outputs on stream 0; L.ev[5] marks the end of the batch.
+pm7_status finish_batch(Lane& L, const Prepared& P) {
+  const int nm = (int)P.jobs.size();
   for (int v = 1; v < SCF_VARIANTS && !L.single_stream; ++v) {
     LANECHK(hipEventRecord(L.ev[6 + v], L.st[v]));
     LANECHK(hipStreamWaitEvent(L.st[0], L.ev[6 + v], 0));
@@ -777,6 +816,8 @@ pm7_status run_jobs(pm7_ctx* ctx, std::vector<Job>& jobs, pm7_result_v1* results
   }
   int count[SCF_VARIANTS];
   st = launch_batch(L, P, count);
+  if (st == PM7_OK) st = run_large(L, P);
+  if (st == PM7_OK) st = finish_batch(L, P);
   if (st != PM7_OK) return fail(ctx, st, L.err);
   host_corrections(P.jobs);
   const auto t_launched = std::chrono::steady_clock::now();
@@ -922,6 +963,8 @@ void srv_loop(pm7_server* s) {
       int count[SCF_VARIANTS];
       pm7_status st = prepare(ctx, f.lane, f.P, 0);
       if (st == PM7_OK) st = launch_batch(f.lane, f.P, count);
+      if (st == PM7_OK) st = run_large(f.lane, f.P);
+      if (st == PM7_OK) st = finish_batch(f.lane, f.P);
       if (st == PM7_OK) host_corrections(f.P.jobs);
       lk.lock();
       if (st != PM7_OK) {

@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 #include "pm7_device.h"
 
 namespace pm7 {
@@ -44,11 +46,20 @@ constexpr int GL_WORDS = 2 * 48 + 2 * 40;
 // 1 <= 112 (512 threads, fully in LDS), 2 <= 196 (packed matrix in LDS, rotations in L2),
 // 3 <= 1024 (eigensolver in L2/HBM).
 constexpr int SCF_VARIANTS = 4;
-constexpr int VARIANT_MAX_NAO[SCF_VARIANTS] = {64, 112, 196, 1024};
+constexpr int VARIANT_MAX_NAO[SCF_VARIANTS] = {64, 112, 196, 1 << 30};
+constexpr int LARGE_VARIANT = 3;  // above 196 basis functions: the grid-wide path (large.hip)
 
 hipError_t launch_integrals(const BatchDev& b, int64_t n_pairs, int n_items, int n_mols, hipStream_t s);
 hipError_t launch_scf(int variant, const BatchDev& b, int list_off, int count, hipStream_t s);
 hipError_t launch_gather(const BatchDev& b, int n_mols, hipStream_t s);
 hipError_t launch_fock_once(const BatchDev& b, int mol, int pt_slot, int ps_slot, int f_slot, hipStream_t s);
+
+// Large-system SCF of molecule `mi` (host copy M) of a prepared batch: grid-wide kernels and
+// rocBLAS/rocSOLVER on stream s; fills `out` as the megakernel would. Blocks the calling thread.
+struct LargeWork;
+LargeWork* large_work_create();
+void large_work_destroy(LargeWork* w);
+hipError_t large_scf(LargeWork* w, const BatchDev& b, const MolDev& M, int mi, hipStream_t s, MolOut& out,
+                     std::string& err);
 
 }  // namespace pm7

@@ -1,0 +1,752 @@
+// SPDX-License-Identifier: GPL-3.0-or-later
+//
+// The large-system SCF path (basis sizes above the one-workgroup classes, up to thousands of AOs):
+// one molecule owns the whole device and every step of an iteration is a grid-wide kernel or a
+// library call, in the order of pm7-rs rhf_loop / uhf_loop (src/scf.rs:1231-1295, 1380-1470):
+//
+//   F = H + G(P)          grid kernels over (atom, orbital pair) items and atom pairs (fock.h)
+//   E = 1/2 <P, H + F>    two-stage fixed-order reduction
+//   e = FP - PF           rocBLAS dgemm + antisymmetrise, |e|^2 reduced in fixed order
+//   DIIS                  history dots (one pass), CDIIS / A-DIIS on one wavefront (diis.h)
+//   (eps, C) = eig(F')    rocSOLVER dsyevd (FP64); in the reduced-precision phase of a mixed run
+//                         near convergence, occupied-virtual rotations on rocBLAS GEMMs instead
+//   P = 2 C_occ C_occ^T   rocBLAS dgemm, or sgemm on FP32 copies in the reduced-precision phase
+//
+// Scalars (ring of DIIS slots, coefficients, stall guard, convergence) live in a control block in
+// device memory, updated by one-wavefront control kernels; the host reads only the stop flag.
+// Orientation: rocBLAS/rocSOLVER are column-major. Our square matrices are row-major with leading
+// dimension ld; symmetric ones are the same either way. After dsyevd the eigenvector array read
+// row-major is C^T (row k = orbital k), so the orbitals are kept as Ct = C^T during the SCF and
+// transposed into the C slot at the end.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <cstddef>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "diis.h"
+#include "fock.h"
+#include "kernels.h"
+#include "pm7_device.h"
+
+namespace pm7 {
+
+namespace {
+
+constexpr int RED_BLOCKS = 512;  // partial sums per reduction (fixed, so the order is fixed)
+constexpr int NDOT = 3 * HIST + 8;
+
+// Device control block of one large SCF.
+struct LargeCtl {
+  int head, cnt, ncoef, done, low, fp32_iters, stall, iter;
+  int slot[HIST], cslot[HIST];
+  double coef[HIST + 1];
+  double G[HIST * HIST], M[HIST * HIST];
+  double e_old, e_cur, err, dp, best_dp, e_low;
+  double dots[NDOT];  // reductions land here
+  Guard guard;
+  double wk[(HIST + 1) * (HIST + 3)];
+};
+
+struct Buf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap && p) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 256));
+    if (e == hipSuccess) cap = std::max<size_t>(bytes, 256);
+    return e;
+  }
+  ~Buf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+__device__ MolCtx make_ctx(const MolDev& M, const BatchDev& b) {
+  MolCtx c;
+  c.M = M;
+  c.atom_z = b.atom_z;
+  c.atom_ao = b.atom_ao;
+  c.atom_pp = b.atom_pp;
+  c.pairs = b.pairs;
+  c.elem = b.elem;
+  c.w = b.w;
+  c.ld = M.ld;
+  c.base = b.slab + M.mat0;
+  c.pp = b.slab + M.pp0;
+  return c;
+}
+
+// ---- kernels --------------------------------------------------------------------------------
+
+template <typename T>
+__global__ void k_lg_pack(int mi, BatchDev b, const double* __restrict__ Pt) {
+  const MolCtx c = make_ctx(b.mols[mi], b);
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < c.M.n_pp; t += gridDim.x * blockDim.x)
+    fock_pack_item<T>(c, t, Pt);
+}
+
+template <typename T>
+__global__ void k_lg_fock(int mi, BatchDev b, const double* __restrict__ Pt, const double* __restrict__ Psa,
+                          double xa, const double* __restrict__ Psb, double xb, double* __restrict__ Fa,
+                          double* __restrict__ Fb) {
+  const MolCtx c = make_ctx(b.mols[mi], b);
+  const int64_t npairs = (int64_t)c.M.n_atoms * (c.M.n_atoms - 1) / 2;
+  const int64_t total = c.M.n_pp + npairs;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    if (t < c.M.n_pp) fock_diag_item<T>(c, (int)t, Pt, Psa, xa, Psb, xb, Fa, Fb);
+    else fock_pair_item<T>(c, (int)(t - c.M.n_pp), Psa, xa, Psb, xb, Fa, Fb);
+  }
+}
+
+// Block-partial sums of up to 3 dot products <X_k, Y_k> over the n x n block, into
+// part[block * 3 + k]; every block visits a fixed strided set of elements.
+__global__ void __launch_bounds__(256) k_lg_dots(int n, int ld, int k, const double* __restrict__ X0,
+                                                 const double* __restrict__ Y0, const double* __restrict__ X1,
+                                                 const double* __restrict__ Y1, const double* __restrict__ X2,
+                                                 const double* __restrict__ Y2, double* __restrict__ part) {
+  double acc[3] = {0.0, 0.0, 0.0};
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < nn; idx += (int64_t)gridDim.x * 256) {
+    const int64_t i = idx / n, j = idx - i * n, x = i * ld + j;
+    acc[0] += X0[x] * Y0[x];
+    if (k > 1) acc[1] += X1[x] * Y1[x];
+    if (k > 2) acc[2] += X2[x] * Y2[x];
+  }
+  __shared__ double red[3][4];
+  for (int q = 0; q < 3; ++q) {
+    double v = acc[q];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) part[blockIdx.x * 3 + threadIdx.x] = red[threadIdx.x][0] + red[threadIdx.x][1] +
+                                                             red[threadIdx.x][2] + red[threadIdx.x][3];
+}
+
+// Sum the block partials of k_lg_dots in block order into dst[0..k).
+__global__ void k_lg_finish(int nblocks, int k, const double* __restrict__ part, double* __restrict__ dst) {
+  if (threadIdx.x < k) {
+    double s = 0.0;
+    for (int bI = 0; bI < nblocks; ++bI) s += part[bI * 3 + threadIdx.x];
+    dst[threadIdx.x] = s;
+  }
+}
+
+// E = X - X^T (X = FP) into E (when given) and block partials of sum E^2.
+__global__ void __launch_bounds__(256) k_lg_comm(int n, int ld, const double* __restrict__ X, double* __restrict__ E,
+                                                 double* __restrict__ part) {
+  double acc = 0.0;
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < nn; idx += (int64_t)gridDim.x * 256) {
+    const int64_t i = idx / n, j = idx - i * n;
+    const double e = X[i * ld + j] - X[j * ld + i];
+    if (E) E[i * ld + j] = e;
+    acc += e * e;
+  }
+  __shared__ double red[4];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[blockIdx.x * 3] = red[0] + red[1] + red[2] + red[3];
+    part[blockIdx.x * 3 + 1] = 0.0;
+    part[blockIdx.x * 3 + 2] = 0.0;
+  }
+}
+
+// W = sum_i coef_i F_{cslot_i} (or F when no coefficients) + shift (1 - P/occ) on the virtual
+// projector (the megakernel's level_shift: W += shift (I - P/occ) over the n x n block).
+// F and W may be the same matrix (the in-place shift of the UHF path).
+__global__ void __launch_bounds__(256) k_lg_combine(int n, int ld, const LargeCtl* __restrict__ ctl,
+                                                    const double* __restrict__ hist_f, const double* F,
+                                                    const double* __restrict__ P, double occ, double* W,
+                                                    int use_diis) {
+  const int ncoef = use_diis ? ctl->ncoef : 0;
+  const double shift = ctl->guard.shift;
+  const int64_t sq = (int64_t)ld * ld;
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < nn; idx += (int64_t)gridDim.x * 256) {
+    const int64_t i = idx / n, j = idx - i * n, x = i * ld + j;
+    double v;
+    if (ncoef == 0) {
+      v = F[x];
+    } else {
+      v = 0.0;
+      for (int k = 0; k < ncoef; ++k) v += ctl->coef[k] * hist_f[(int64_t)ctl->cslot[k] * sq + x];
+    }
+    if (shift > 0.0) v += shift * ((i == j ? 1.0 : 0.0) - P[x] / occ);
+    W[x] = v;
+  }
+}
+
+// Pn = w Ct_occ^T Ct_occ was formed by a library product into Pn; here the change against P,
+// block partials of sum (Pn - P)^2.
+__global__ void __launch_bounds__(256) k_lg_diff(int n, int ld, const double* __restrict__ Pn,
+                                                 const double* __restrict__ P, double* __restrict__ part) {
+  double acc = 0.0;
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < nn; idx += (int64_t)gridDim.x * 256) {
+    const int64_t i = idx / n, j = idx - i * n, x = i * ld + j;
+    const double d = Pn[x] - P[x];
+    acc += d * d;
+  }
+  __shared__ double red[4];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[blockIdx.x * 3] = red[0] + red[1] + red[2] + red[3];
+    part[blockIdx.x * 3 + 1] = 0.0;
+    part[blockIdx.x * 3 + 2] = 0.0;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_lg_to_f32(int64_t count, const double* __restrict__ src, float* __restrict__ dst) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256) dst[i] = (float)src[i];
+}
+__global__ void __launch_bounds__(256) k_lg_to_f64(int64_t count, const float* __restrict__ src, double* __restrict__ dst) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256) dst[i] = (double)src[i];
+}
+__global__ void __launch_bounds__(256) k_lg_copy(int64_t count, const double* __restrict__ src, double* __restrict__ dst) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256) dst[i] = src[i];
+}
+// dst = src^T over the ld x ld square (pad rows/columns of src are zero).
+__global__ void __launch_bounds__(256) k_lg_transpose(int ld, const double* __restrict__ src, double* __restrict__ dst) {
+  const int64_t nn = (int64_t)ld * ld;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < nn; idx += (int64_t)gridDim.x * 256) {
+    const int64_t i = idx / ld, j = idx - i * ld;
+    dst[j * ld + i] = src[idx];
+  }
+}
+
+// Occupied-virtual generator K from F' (symmetric, column-major = row-major): K_ia = theta,
+// K_ai = -theta with the 2x2 Jacobi angle of pair (i, a) (eigen.h occ_virt_rotate), column-major.
+__global__ void __launch_bounds__(256) k_lg_kgen(int n, int ld, int nocc, const double* __restrict__ Fp,
+                                                 double* __restrict__ K) {
+  const int64_t nn = (int64_t)ld * ld;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < nn; idx += (int64_t)gridDim.x * 256) {
+    const int j = (int)(idx / ld), i = (int)(idx - (int64_t)j * ld);  // column-major (i, j)
+    double k = 0.0;
+    if (i < n && j < n) {
+      const bool io = i < nocc, jo = j < nocc;
+      if (io != jo) {
+        const int o = io ? i : j, v = io ? j : i;
+        const double apq = Fp[(int64_t)v * ld + o];
+        if (apq != 0.0) {
+          const double d = Fp[(int64_t)v * ld + v] - Fp[(int64_t)o * ld + o];
+          const double x = 2.0 * apq;
+          const double rr = sqrt(d * d + x * x);
+          const double th = atan((d >= 0.0 ? x : -x) / (fabs(d) + rr));
+          k = io ? th : -th;
+        }
+      }
+    }
+    K[idx] = k;
+  }
+}
+
+// U = I + K + K2/2 + K3/6 (all column-major, ld x ld; pad diagonal entries 1 keep U orthogonal).
+__global__ void __launch_bounds__(256) k_lg_taylor(int ld, const double* __restrict__ K, const double* __restrict__ K2,
+                                                   const double* __restrict__ K3, double* __restrict__ U) {
+  const int64_t nn = (int64_t)ld * ld;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < nn; idx += (int64_t)gridDim.x * 256) {
+    const int64_t j = idx / ld, i = idx - j * ld;
+    U[idx] = (i == j ? 1.0 : 0.0) + K[idx] + 0.5 * K2[idx] + K3[idx] / 6.0;
+  }
+}
+
+// S <- 1.5 I - 0.5 S (Newton-Schulz factor), ld x ld.
+__global__ void __launch_bounds__(256) k_lg_ns(int ld, double* __restrict__ S) {
+  const int64_t nn = (int64_t)ld * ld;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < nn; idx += (int64_t)gridDim.x * 256) {
+    const int64_t j = idx / ld, i = idx - j * ld;
+    S[idx] = (i == j ? 1.5 : 0.0) - 0.5 * S[idx];
+  }
+}
+
+// Control A (one wavefront): advance the DIIS ring, fill the Gram and mixing matrices from the
+// history dots, pick CDIIS or A-DIIS coefficients (src/scf.rs:1240-1266).
+__global__ void __launch_bounds__(64) k_lg_ctrl_a(LargeCtl* __restrict__ ctl, int use_diis, int accel,
+                                                  double adiis_switch) {
+  __shared__ int s_slot[HIST];
+  __shared__ double s_M[HIST * HIST];
+  __shared__ double s_coef[HIST + 1];
+  __shared__ int s_cnt, s_h, s_use_adiis;
+  if (threadIdx.x == 0) {
+    const int h = ctl->head;
+    if (ctl->cnt < HIST) ctl->cnt++;
+    const int cnt = ctl->cnt;
+    ctl->head = (h + 1) % HIST;
+    for (int i = 0; i < cnt; ++i) ctl->slot[i] = (ctl->head - cnt + i + 2 * HIST) % HIST;
+    for (int q = 0; q < cnt; ++q) {
+      const int t = ctl->slot[q];
+      ctl->G[h * HIST + t] = ctl->dots[3 * q];
+      ctl->G[t * HIST + h] = ctl->dots[3 * q];
+      ctl->M[h * HIST + t] = ctl->dots[3 * q + 1];
+      ctl->M[t * HIST + h] = ctl->dots[3 * q + 2];
+    }
+    const double err = ctl->err;
+    int ncoef = 0, skip = 0;
+    if (use_diis && accel != 0 && (accel == 1 || err <= adiis_switch)) {
+      ncoef = cdiis(cnt, ctl->slot, ctl->G, ctl->coef, &skip, ctl->wk);
+      for (int i = 0; i < ncoef; ++i) ctl->cslot[i] = ctl->slot[skip + i];
+    }
+    ctl->ncoef = ncoef;
+    s_use_adiis = (use_diis && accel == 2 && err > adiis_switch && cnt >= 2) ? 1 : 0;
+    s_cnt = cnt;
+    s_h = h;
+    for (int i = 0; i < cnt; ++i) s_slot[i] = ctl->slot[i];
+    for (int i = 0; i < HIST * HIST; ++i) s_M[i] = ctl->M[i];
+  }
+  __syncthreads();
+  if (s_use_adiis) {
+    adiis_wave(s_cnt, s_slot, s_M, s_coef);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int i = 0; i < s_cnt; ++i) {
+        ctl->coef[i] = s_coef[i];
+        ctl->cslot[i] = s_slot[i];
+      }
+      ctl->ncoef = s_cnt;
+    }
+  }
+  (void)s_h;
+}
+
+// Control B (one thread): convergence, stall guard and the precision hand-over
+// (src/scf.rs:1274-1287; the megakernel's tail of an iteration).
+__global__ void k_lg_ctrl_b(LargeCtl* __restrict__ ctl, int iter, int64_t nn, double e_tol, double p_tol,
+                            double mixed_switch, int uhf) {
+  if (threadIdx.x != 0) return;
+  const double e = ctl->e_cur;
+  const double dp = uhf ? sqrt(ctl->dots[0] / (double)nn) + sqrt(ctl->dots[1] / (double)nn)
+                        : sqrt(ctl->dots[0] / (double)nn);
+  const double de = fabs(e - ctl->e_old);
+  ctl->e_old = e;
+  ctl->dp = dp;
+  const bool resid_ok = ctl->guard.shift == 0.0 && ctl->err < 1.0e-7;
+  const bool conv = iter > 0 && de < e_tol && (dp < p_tol || resid_ok);
+  if (ctl->low) {
+    ctl->fp32_iters++;
+    ctl->e_low = e;
+    if (dp < 0.8 * ctl->best_dp) {
+      ctl->best_dp = dp;
+      ctl->stall = 0;
+    } else {
+      ctl->stall++;
+    }
+    if (conv || dp < mixed_switch || ctl->fp32_iters >= MIXED_MAX_ITERS || ctl->stall >= 3) ctl->low = 0;
+    else guard_update(ctl->guard, dp, e);
+  } else if (conv) {
+    ctl->done = 1;
+  } else {
+    guard_update(ctl->guard, dp, e);
+  }
+  ctl->iter = iter + 1;
+}
+
+// Energy from the dots: e = 1/2 (<P, H> + <P, F>) (RHF) or 1/2 (<Pt, H> + <Pa, Fa> + <Pb, Fb>).
+__global__ void k_lg_energy(LargeCtl* __restrict__ ctl, const double* __restrict__ d3, int uhf) {
+  if (threadIdx.x == 0) ctl->e_cur = 0.5 * (d3[0] + d3[1] + (uhf ? d3[2] : 0.0));
+}
+__global__ void k_lg_err(LargeCtl* __restrict__ ctl, const double* __restrict__ d3, int terms) {
+  if (threadIdx.x == 0) ctl->err = sqrt(d3[0] + (terms > 1 ? d3[1] : 0.0));
+}
+
+__global__ void k_lg_init(LargeCtl* __restrict__ ctl, double shift_floor, int low) {
+  if (threadIdx.x != 0) return;
+  ctl->head = 0;
+  ctl->cnt = 0;
+  ctl->ncoef = 0;
+  ctl->done = 0;
+  ctl->low = low;
+  ctl->fp32_iters = 0;
+  ctl->stall = 0;
+  ctl->iter = 0;
+  ctl->e_old = 0.0;
+  ctl->e_cur = 0.0;
+  ctl->err = 0.0;
+  ctl->dp = 1.0e30;
+  ctl->best_dp = 1.0e30;
+  ctl->e_low = 0.0;
+  ctl->guard.floor_ = shift_floor;
+  ctl->guard.shift = shift_floor;
+  ctl->guard.n = 0;
+}
+
+// SAD guess of the large path (pm7-rs src/scf.rs:1050-1114): diagonal tore/4 per sp AO, minus the
+// charge spread (sad_yy), scaled; pads and off-diagonal stay zero (the slab is zeroed).
+__global__ void k_lg_sad(int mi, BatchDev b, double* __restrict__ P, double scale) {
+  const MolDev M = b.mols[mi];
+  for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < M.n_atoms; a += gridDim.x * blockDim.x) {
+    const ElemDev e = b.elem[b.atom_z[M.atom0 + a]];
+    const int o = b.atom_ao[M.atom0 + a];
+    const double occ = e.n_orb == 1 ? e.core_charge : e.core_charge / 4.0;
+    for (int k = 0; k < e.n_orb; ++k) P[(int64_t)(o + k) * M.ld + o + k] = (occ - M.sad_yy) * scale;
+  }
+}
+
+}  // namespace
+
+// ---- host driver --------------------------------------------------------------------------------
+
+struct LargeWork {
+  rocblas_handle blas = nullptr;
+  Buf ctl, part, dots, info, f32a, f32b;
+  std::vector<hipEvent_t> ev;
+};
+
+LargeWork* large_work_create() {
+  auto* w = new LargeWork();
+  if (rocblas_create_handle(&w->blas) != rocblas_status_success) {
+    delete w;
+    return nullptr;
+  }
+  w->ev.resize(8);
+  for (auto& e : w->ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      large_work_destroy(w);
+      return nullptr;
+    }
+  return w;
+}
+
+void large_work_destroy(LargeWork* w) {
+  if (!w) return;
+  if (w->blas) rocblas_destroy_handle(w->blas);
+  for (auto& e : w->ev)
+    if (e) (void)hipEventDestroy(e);
+  delete w;
+}
+
+namespace {
+
+#define LG(expr)                                                           \
+  do {                                                                     \
+    hipError_t e_ = (expr);                                                \
+    if (e_ != hipSuccess) {                                                \
+      err = std::string(#expr ": ") + hipGetErrorString(e_);              \
+      return e_;                                                           \
+    }                                                                      \
+  } while (0)
+#define LGB(expr)                                                          \
+  do {                                                                     \
+    rocblas_status s_ = (expr);                                            \
+    if (s_ != rocblas_status_success) {                                    \
+      err = std::string(#expr ": ") + rocblas_status_to_string(s_);       \
+      return hipErrorUnknown;                                              \
+    }                                                                      \
+  } while (0)
+
+int grid_for(int64_t items) { return (int)std::min<int64_t>(std::max<int64_t>((items + 255) / 256, 1), 4096); }
+
+}  // namespace
+
+hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, hipStream_t s, MolOut& out,
+                     std::string& err) {
+  const int n = M.nao, ld = M.ld;
+  const int64_t sq = (int64_t)ld * ld, nn = (int64_t)n * n;
+  double* base = b.slab + M.mat0;
+  auto slot = [&](int k) { return base + (int64_t)k * sq; };
+  LG(lw->ctl.ensure(sizeof(LargeCtl)));
+  LG(lw->part.ensure((size_t)RED_BLOCKS * 3 * sizeof(double)));
+  LG(lw->dots.ensure((size_t)(NDOT + 8) * sizeof(double)));
+  LG(lw->info.ensure(sizeof(rocblas_int)));
+  LGB(rocblas_set_stream(lw->blas, s));
+  LargeCtl* ctl = static_cast<LargeCtl*>(lw->ctl.p);
+  double* ctl_dots = reinterpret_cast<double*>(static_cast<char*>(lw->ctl.p) + offsetof(LargeCtl, dots));
+  double* part = static_cast<double*>(lw->part.p);
+  double* d3 = static_cast<double*>(lw->dots.p);
+  rocblas_int* info = static_cast<rocblas_int*>(lw->info.p);
+  const bool mixed = M.precision == 1;
+  if (mixed) {
+    LG(lw->f32a.ensure((size_t)sq * sizeof(float)));
+    LG(lw->f32b.ensure((size_t)sq * sizeof(float)));
+  }
+  float* fa = static_cast<float*>(lw->f32a.p);
+  float* fb = static_cast<float*>(lw->f32b.p);
+  const int RB = std::min<int>(RED_BLOCKS, grid_for(nn));
+  // Reduction helpers: k dots over n x n, finished into dst (device).
+  auto dots = [&](int k, const double* x0, const double* y0, const double* x1, const double* y1, const double* x2,
+                  const double* y2, double* dst) -> hipError_t {
+    hipLaunchKernelGGL(k_lg_dots, dim3(RB), dim3(256), 0, s, n, ld, k, x0, y0, x1, y1, x2, y2, part);
+    hipLaunchKernelGGL(k_lg_finish, dim3(1), dim3(64), 0, s, RB, k, part, dst);
+    return hipGetLastError();
+  };
+  const double one = 1.0, zero = 0.0;
+  const float onef = 1.0f, zerof = 0.0f;
+  // Column-major product D = op(A) op(B) on ld x ld operands.
+  auto gemm = [&](rocblas_operation ta, rocblas_operation tb, int m_, int n_, int k_, double alpha, const double* A,
+                  const double* B, double beta, double* D) {
+    return rocblas_dgemm(lw->blas, ta, tb, m_, n_, k_, &alpha, A, ld, B, ld, &beta, D, ld);
+  };
+  // Orbitals in Ct (row-major, row k = orbital k) == C column-major: Ccm = Ct.
+  // Density Pn = w C_occ C_occ^T (symmetric): column-major Ccm[:, :nocc] Ccm[:, :nocc]^T.
+  auto density = [&](const double* Ct, int nocc, double wgt, double* Pn, bool low) -> hipError_t {
+    if (nocc <= 0) return hipMemsetAsync(Pn, 0, (size_t)sq * sizeof(double), s);
+    if (!low) {
+      if (rocblas_dgemm(lw->blas, rocblas_operation_none, rocblas_operation_transpose, n, n, nocc, &wgt, Ct, ld, Ct,
+                        ld, &zero, Pn, ld) != rocblas_status_success)
+        return hipErrorUnknown;
+    } else {
+      // FP32 inputs, FP32 accumulation (matrix cores), result widened to FP64.
+      hipLaunchKernelGGL(k_lg_to_f32, dim3(grid_for(sq)), dim3(256), 0, s, sq, Ct, fa);
+      const float w32 = (float)wgt;
+      if (rocblas_sgemm(lw->blas, rocblas_operation_none, rocblas_operation_transpose, n, n, nocc, &w32, fa, ld, fa,
+                        ld, &zerof, fb, ld) != rocblas_status_success)
+        return hipErrorUnknown;
+      hipLaunchKernelGGL(k_lg_to_f64, dim3(grid_for(sq)), dim3(256), 0, s, sq, fb, Pn);
+    }
+    (void)onef;
+    return hipGetLastError();
+  };
+  // Full FP64 eigensolve: W (destroyed) -> eps, orbitals into Ct.
+  auto eig_full = [&](double* W, double* eps, double* Ct) -> hipError_t {
+    // Pad rows/columns of W are zero: restrict the solve to the n x n block.
+    if (rocsolver_dsyevd(lw->blas, rocblas_evect_original, rocblas_fill_upper, n, W, ld, eps, slot(S_AG), info) !=
+        rocblas_status_success)
+      return hipErrorUnknown;
+    hipLaunchKernelGGL(k_lg_copy, dim3(grid_for(sq)), dim3(256), 0, s, sq, W, Ct);
+    return hipGetLastError();
+  };
+  // Occupied-virtual rotation of Ct (the reduced-precision phase near convergence).
+  auto eig_rotate = [&](const double* W, int nocc, double* Ct) -> hipError_t {
+    double* T1 = slot(S_T);
+    double* Fp = slot(S_AG);
+    double* K = slot(S_VG);
+    double* K2 = slot(S_C1);
+    // F' = C^T W C (column-major, C = Ccm = Ct)
+    if (gemm(rocblas_operation_transpose, rocblas_operation_none, n, n, n, 1.0, Ct, W, 0.0, T1) != rocblas_status_success)
+      return hipErrorUnknown;
+    if (gemm(rocblas_operation_none, rocblas_operation_none, n, n, n, 1.0, T1, Ct, 0.0, Fp) != rocblas_status_success)
+      return hipErrorUnknown;
+    hipLaunchKernelGGL(k_lg_kgen, dim3(grid_for(sq)), dim3(256), 0, s, n, ld, nocc, Fp, K);
+    if (gemm(rocblas_operation_none, rocblas_operation_none, n, n, n, 1.0, K, K, 0.0, K2) != rocblas_status_success)
+      return hipErrorUnknown;
+    if (gemm(rocblas_operation_none, rocblas_operation_none, n, n, n, 1.0, K, K2, 0.0, T1) != rocblas_status_success)
+      return hipErrorUnknown;  // K^3
+    hipLaunchKernelGGL(k_lg_taylor, dim3(grid_for(sq)), dim3(256), 0, s, ld, K, K2, T1, Fp);  // U into Fp
+    if (gemm(rocblas_operation_none, rocblas_operation_none, n, n, n, 1.0, Ct, Fp, 0.0, K) != rocblas_status_success)
+      return hipErrorUnknown;  // C1 = C U into K
+    double* src = K;
+    double* dst = K2;
+    for (int step = 0; step < 2; ++step) {
+      if (gemm(rocblas_operation_transpose, rocblas_operation_none, n, n, n, 1.0, src, src, 0.0, T1) !=
+          rocblas_status_success)
+        return hipErrorUnknown;  // S = C^T C
+      hipLaunchKernelGGL(k_lg_ns, dim3(grid_for(sq)), dim3(256), 0, s, ld, T1);
+      double* to = step == 1 ? Ct : dst;
+      if (gemm(rocblas_operation_none, rocblas_operation_none, n, n, n, 1.0, src, T1, 0.0, to) != rocblas_status_success)
+        return hipErrorUnknown;
+      src = to;
+    }
+    return hipGetLastError();
+  };
+
+  const bool mixed_ok = mixed;  // the FP32 range guard of the megakernel is applied below
+  hipLaunchKernelGGL(k_lg_init, dim3(1), dim3(64), 0, s, ctl, M.shift_floor, mixed_ok ? 1 : 0);
+  std::vector<float> phase(5, 0.0f);
+  LargeCtl host{};
+  int iter = 0;
+  int low = mixed_ok ? 1 : 0;
+  int cslot_c = S_C0;  // Ct lives here during the SCF
+  double dp_prev = 1.0e30;
+  if (!M.uhf) {
+    const int nocc = M.n_occ;
+    hipLaunchKernelGGL(k_lg_sad, dim3(grid_for(M.n_atoms)), dim3(256), 0, s, mi, b, slot(S_HD), 1.0);
+    for (; iter < M.max_scf; ++iter) {
+      const int h = iter % HIST;
+      const int cnt = std::min(iter + 1, HIST);
+      double* P = slot(S_HD + h);
+      double* F = slot(S_HF + h);
+      double* E = slot(S_HE + h);
+      LG(hipEventRecord(lw->ev[0], s));
+      if (low) {
+        hipLaunchKernelGGL(k_lg_pack<float>, dim3(grid_for(M.n_pp)), dim3(256), 0, s, mi, b, P);
+        hipLaunchKernelGGL(k_lg_fock<float>, dim3(grid_for(M.n_pp + (int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256),
+                           0, s, mi, b, P, P, 0.5, nullptr, 0.0, F, nullptr);
+      } else {
+        hipLaunchKernelGGL(k_lg_pack<double>, dim3(grid_for(M.n_pp)), dim3(256), 0, s, mi, b, P);
+        hipLaunchKernelGGL(k_lg_fock<double>, dim3(grid_for(M.n_pp + (int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256),
+                           0, s, mi, b, P, P, 0.5, nullptr, 0.0, F, nullptr);
+      }
+      LG(hipGetLastError());
+      LG(hipEventRecord(lw->ev[1], s));
+      // energy and commutator
+      LG(dots(2, P, slot(S_H), P, F, nullptr, nullptr, d3));
+      hipLaunchKernelGGL(k_lg_energy, dim3(1), dim3(64), 0, s, ctl, d3, 0);
+      LGB(gemm(rocblas_operation_none, rocblas_operation_none, n, n, n, 1.0, P, F, 0.0, slot(S_T)));  // row-major FP
+      hipLaunchKernelGGL(k_lg_comm, dim3(RB), dim3(256), 0, s, n, ld, slot(S_T), E, part);
+      hipLaunchKernelGGL(k_lg_finish, dim3(1), dim3(64), 0, s, RB, 1, part, d3 + 4);
+      hipLaunchKernelGGL(k_lg_err, dim3(1), dim3(64), 0, s, ctl, d3 + 4, 1);
+      LG(hipEventRecord(lw->ev[2], s));
+      // history dots of the new entry against every live one (ring positions as in k_lg_ctrl_a)
+      const int head_next = (h + 1) % HIST;
+      for (int q = 0; q < cnt; ++q) {
+        const int t = (head_next - cnt + q + 2 * HIST) % HIST;
+        LG(dots(3, E, slot(S_HE + t), P, slot(S_HF + t), slot(S_HD + t), F, ctl_dots + 3 * q));
+      }
+      hipLaunchKernelGGL(k_lg_ctrl_a, dim3(1), dim3(64), 0, s, ctl, M.use_diis, M.accel, M.adiis_switch);
+      hipLaunchKernelGGL(k_lg_combine, dim3(grid_for(nn)), dim3(256), 0, s, n, ld, ctl, slot(S_HF), F, P, 2.0,
+                         slot(S_W), M.use_diis);
+      LG(hipGetLastError());
+      LG(hipEventRecord(lw->ev[3], s));
+      double* eps = b.vecs + M.vec0;
+      if (low && iter > 0 && dp_prev < ROTATE_BELOW) LG(eig_rotate(slot(S_W), nocc, slot(cslot_c)));
+      else LG(eig_full(slot(S_W), eps, slot(cslot_c)));
+      LG(hipEventRecord(lw->ev[4], s));
+      double* Pn = slot(S_HD + head_next);
+      LG(density(slot(cslot_c), nocc, 2.0, Pn, low != 0));
+      hipLaunchKernelGGL(k_lg_diff, dim3(RB), dim3(256), 0, s, n, ld, Pn, P, part);
+      hipLaunchKernelGGL(k_lg_finish, dim3(1), dim3(64), 0, s, RB, 1, part, ctl_dots);
+      hipLaunchKernelGGL(k_lg_ctrl_b, dim3(1), dim3(64), 0, s, ctl, iter, nn, M.e_tol, M.p_tol, M.mixed_switch, 0);
+      LG(hipEventRecord(lw->ev[5], s));
+      LG(hipMemcpyAsync(&host, ctl, sizeof host, hipMemcpyDeviceToHost, s));
+      LG(hipStreamSynchronize(s));
+      float ms;
+      for (int k = 0; k < 5; ++k)
+        if (hipEventElapsedTime(&ms, lw->ev[k], lw->ev[k + 1]) == hipSuccess) phase[k] += ms;
+      low = host.low;
+      dp_prev = host.dp;
+      if (host.done) {
+        ++iter;
+        break;
+      }
+    }
+    // Final energy and (with a level shift) the unshifted orbitals from the converged density.
+    const int hp = iter % HIST;
+    double* P = slot(S_HD + hp);
+    double* F = slot(S_F);
+    hipLaunchKernelGGL(k_lg_pack<double>, dim3(grid_for(M.n_pp)), dim3(256), 0, s, mi, b, P);
+    hipLaunchKernelGGL(k_lg_fock<double>, dim3(grid_for(M.n_pp + (int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256), 0, s,
+                       mi, b, P, P, 0.5, nullptr, 0.0, F, nullptr);
+    LG(dots(2, P, slot(S_H), P, F, nullptr, nullptr, d3));
+    hipLaunchKernelGGL(k_lg_energy, dim3(1), dim3(64), 0, s, ctl, d3, 0);
+    if (host.guard.shift > 0.0) {
+      hipLaunchKernelGGL(k_lg_copy, dim3(grid_for(sq)), dim3(256), 0, s, sq, F, slot(S_W));
+      LG(eig_full(slot(S_W), b.vecs + M.vec0, slot(cslot_c)));
+    }
+    // C = Ct^T into the C1 slot for the output gather.
+    hipLaunchKernelGGL(k_lg_transpose, dim3(grid_for(sq)), dim3(256), 0, s, ld, slot(cslot_c), slot(S_C1));
+    LG(hipMemcpyAsync(&host, ctl, sizeof host, hipMemcpyDeviceToHost, s));
+    LG(hipStreamSynchronize(s));
+    out.p_slot = S_HD + hp;
+    out.c_slot = S_C1;
+    out.pb_slot = -1;
+    out.cb_slot = -1;
+  } else {
+    const int na = M.n_occ, nb = M.n_occ_b;
+    const double ntot = (double)std::max(na + nb, 1);
+    double *Pa = slot(U_PA), *Pb = slot(U_PB), *Pan = slot(U_PAN), *Pbn = slot(U_PBN), *Pt = slot(U_PT);
+    double *Fa = slot(U_FA), *Fb = slot(U_FB);
+    hipLaunchKernelGGL(k_lg_sad, dim3(grid_for(M.n_atoms)), dim3(256), 0, s, mi, b, Pa, na / ntot);
+    hipLaunchKernelGGL(k_lg_sad, dim3(grid_for(M.n_atoms)), dim3(256), 0, s, mi, b, Pb, nb / ntot);
+    double* eps_a = b.vecs + M.vec0;
+    double* eps_b = eps_a + ld;
+    for (; iter < M.max_scf; ++iter) {
+      LG(hipEventRecord(lw->ev[0], s));
+      // Pt = Pa + Pb via two library updates: Pt = Pa; Pt += Pb
+      hipLaunchKernelGGL(k_lg_copy, dim3(grid_for(sq)), dim3(256), 0, s, sq, Pa, Pt);
+      LGB(rocblas_daxpy(lw->blas, (rocblas_int)sq, &one, Pb, 1, Pt, 1));
+      if (low) {
+        hipLaunchKernelGGL(k_lg_pack<float>, dim3(grid_for(M.n_pp)), dim3(256), 0, s, mi, b, Pt);
+        hipLaunchKernelGGL(k_lg_fock<float>, dim3(grid_for(M.n_pp + (int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256),
+                           0, s, mi, b, Pt, Pa, 1.0, Pb, 1.0, Fa, Fb);
+      } else {
+        hipLaunchKernelGGL(k_lg_pack<double>, dim3(grid_for(M.n_pp)), dim3(256), 0, s, mi, b, Pt);
+        hipLaunchKernelGGL(k_lg_fock<double>, dim3(grid_for(M.n_pp + (int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256),
+                           0, s, mi, b, Pt, Pa, 1.0, Pb, 1.0, Fa, Fb);
+      }
+      LG(hipGetLastError());
+      LG(hipEventRecord(lw->ev[1], s));
+      LG(dots(3, Pt, slot(U_H), Pa, Fa, Pb, Fb, d3));
+      hipLaunchKernelGGL(k_lg_energy, dim3(1), dim3(64), 0, s, ctl, d3, 1);
+      LGB(gemm(rocblas_operation_none, rocblas_operation_none, n, n, n, 1.0, Pa, Fa, 0.0, slot(U_T)));
+      hipLaunchKernelGGL(k_lg_comm, dim3(RB), dim3(256), 0, s, n, ld, slot(U_T), nullptr, part);
+      hipLaunchKernelGGL(k_lg_finish, dim3(1), dim3(64), 0, s, RB, 1, part, d3 + 4);
+      LGB(gemm(rocblas_operation_none, rocblas_operation_none, n, n, n, 1.0, Pb, Fb, 0.0, slot(U_T)));
+      hipLaunchKernelGGL(k_lg_comm, dim3(RB), dim3(256), 0, s, n, ld, slot(U_T), nullptr, part);
+      hipLaunchKernelGGL(k_lg_finish, dim3(1), dim3(64), 0, s, RB, 1, part, d3 + 5);
+      hipLaunchKernelGGL(k_lg_err, dim3(1), dim3(64), 0, s, ctl, d3 + 4, 2);
+      LG(hipEventRecord(lw->ev[2], s));
+      // level shift in place (no DIIS for UHF): F += shift (I - P)
+      hipLaunchKernelGGL(k_lg_combine, dim3(grid_for(nn)), dim3(256), 0, s, n, ld, ctl, slot(S_HF), Fa, Pa, 1.0, Fa, 0);
+      hipLaunchKernelGGL(k_lg_combine, dim3(grid_for(nn)), dim3(256), 0, s, n, ld, ctl, slot(S_HF), Fb, Pb, 1.0, Fb, 0);
+      LG(hipEventRecord(lw->ev[3], s));
+      if (low && iter > 0 && dp_prev < ROTATE_BELOW) {
+        LG(eig_rotate(Fa, na, slot(U_CA0)));
+        LG(eig_rotate(Fb, nb, slot(U_CB0)));
+      } else {
+        LG(eig_full(Fa, eps_a, slot(U_CA0)));
+        LG(eig_full(Fb, eps_b, slot(U_CB0)));
+      }
+      LG(hipEventRecord(lw->ev[4], s));
+      LG(density(slot(U_CA0), na, 1.0, Pan, low != 0));
+      LG(density(slot(U_CB0), nb, 1.0, Pbn, low != 0));
+      hipLaunchKernelGGL(k_lg_diff, dim3(RB), dim3(256), 0, s, n, ld, Pan, Pa, part);
+      hipLaunchKernelGGL(k_lg_finish, dim3(1), dim3(64), 0, s, RB, 1, part, ctl_dots);
+      hipLaunchKernelGGL(k_lg_diff, dim3(RB), dim3(256), 0, s, n, ld, Pbn, Pb, part);
+      hipLaunchKernelGGL(k_lg_finish, dim3(1), dim3(64), 0, s, RB, 1, part, ctl_dots + 1);
+      hipLaunchKernelGGL(k_lg_ctrl_b, dim3(1), dim3(64), 0, s, ctl, iter, nn, M.e_tol, M.p_tol, M.mixed_switch, 1);
+      LG(hipEventRecord(lw->ev[5], s));
+      LG(hipMemcpyAsync(&host, ctl, sizeof host, hipMemcpyDeviceToHost, s));
+      LG(hipStreamSynchronize(s));
+      float ms;
+      for (int k = 0; k < 5; ++k)
+        if (hipEventElapsedTime(&ms, lw->ev[k], lw->ev[k + 1]) == hipSuccess) phase[k] += ms;
+      std::swap(Pa, Pan);
+      std::swap(Pb, Pbn);
+      low = host.low;
+      dp_prev = host.dp;
+      if (host.done) {
+        ++iter;
+        break;
+      }
+    }
+    hipLaunchKernelGGL(k_lg_copy, dim3(grid_for(sq)), dim3(256), 0, s, sq, Pa, Pt);
+    LGB(rocblas_daxpy(lw->blas, (rocblas_int)sq, &one, Pb, 1, Pt, 1));
+    hipLaunchKernelGGL(k_lg_pack<double>, dim3(grid_for(M.n_pp)), dim3(256), 0, s, mi, b, Pt);
+    hipLaunchKernelGGL(k_lg_fock<double>, dim3(grid_for(M.n_pp + (int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256), 0, s,
+                       mi, b, Pt, Pa, 1.0, Pb, 1.0, Fa, Fb);
+    LG(dots(3, Pt, slot(U_H), Pa, Fa, Pb, Fb, d3));
+    hipLaunchKernelGGL(k_lg_energy, dim3(1), dim3(64), 0, s, ctl, d3, 1);
+    if (host.guard.shift > 0.0) LG(eig_full(Fa, eps_a, slot(U_CA0)));
+    hipLaunchKernelGGL(k_lg_copy, dim3(grid_for(sq)), dim3(256), 0, s, sq, Fb, slot(U_T));
+    LG(eig_full(slot(U_T), eps_b, slot(U_CB0)));
+    hipLaunchKernelGGL(k_lg_transpose, dim3(grid_for(sq)), dim3(256), 0, s, ld, slot(U_CA0), slot(U_CA1));
+    hipLaunchKernelGGL(k_lg_transpose, dim3(grid_for(sq)), dim3(256), 0, s, ld, slot(U_CB0), slot(U_CB1));
+    LG(hipMemcpyAsync(&host, ctl, sizeof host, hipMemcpyDeviceToHost, s));
+    LG(hipStreamSynchronize(s));
+    out.p_slot = (Pa == slot(U_PA)) ? U_PA : U_PAN;
+    out.pb_slot = (Pb == slot(U_PB)) ? U_PB : U_PBN;
+    out.c_slot = U_CA1;
+    out.cb_slot = U_CB1;
+  }
+  out.e_el = host.e_cur;
+  out.iterations = iter > M.max_scf ? M.max_scf : iter;
+  out.converged = host.done;
+  out.dp_err = host.dp;
+  out.err_norm = host.err;
+  out.shift = host.guard.shift;
+  out.e_fp32_last = host.e_low;
+  out.fp32_iters = host.fp32_iters;
+  out.sweeps = 0;
+  for (int k = 0; k < NPHASE; ++k) out.ticks[k] = 0;
+  // phases: fock, energy+commutator, diis, eigensolve, density (ms -> 100 MHz ticks)
+  for (int k = 0; k < 5; ++k) out.ticks[k] = (unsigned long long)(phase[k] * 1.0e5);
+  double tot = 0.0;
+  for (int k = 0; k < 5; ++k) tot += phase[k];
+  out.ticks[PH_TOTAL] = (unsigned long long)(tot * 1.0e5);
+  out.ticks[PH_EIG_SWEEP] = out.ticks[PH_EIG];
+  (void)cslot_c;
+  return hipSuccess;
+}
+
+}  // namespace pm7

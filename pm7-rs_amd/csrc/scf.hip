@@ -19,7 +19,9 @@
 // order, so a molecule's result does not depend on which other molecules share the launch.
 #include <hip/hip_runtime.h>
 
+#include "diis.h"
 #include "eigen.h"
+#include "fock.h"
 #include "kernels.h"
 #include "mfma.h"
 #include "pm7_device.h"
@@ -27,177 +29,6 @@
 
 namespace pm7 {
 
-
-__device__ inline void unpack_e(int e, int& mu, int& nu) {
-  mu = (e >= 6) ? 3 : (e >= 3) ? 2 : (e >= 1) ? 1 : 0;
-  nu = e - mu * (mu + 1) / 2;
-}
-
-// One-centre two-electron integral (mu nu | la si) of an sp atom (pm7-rs src/fock.rs:69-107).
-__device__ inline double oc_int(int a, int b, int c, int d, const ElemDev& e) {
-  if (a == b && c == d) {
-    if (a == 0 && c == 0) return e.g_ss;
-    if (a == 0 || c == 0) return e.g_sp;
-    return a == c ? e.g_pp : e.g_p2;
-  }
-  int ba = min(a, b), bb = max(a, b), kc = min(c, d), kd = max(c, d);
-  if (ba == 0 && bb != 0 && kc == 0 && kd != 0 && bb == kd) return e.h_sp;
-  if (ba != 0 && bb != 0 && ba != bb && ba == kc && bb == kd) return 0.5 * (e.g_pp - e.g_p2);
-  return 0.0;
-}
-
-struct MolCtx {
-  MolDev M;
-  const int* atom_z;
-  const int* atom_ao;
-  const int* atom_pp;
-  const PairDev* pairs;
-  const ElemDev* elem;
-  const double* w;
-  double* base;  // molecule slab
-  double* pp;    // packed densities
-  int ld;
-  __device__ double* slot(int s) const { return base + (int64_t)s * ld * ld; }
-};
-
-__device__ inline int find_atom(const int* atom_pp, int a0, int na, int t) {
-  int lo = 0, hi = na - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (atom_pp[a0 + mid] <= t) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
-// F^a = H + J(Pt) - xa K(Psa) (and F^b with Psb when given). T is the type the two-electron part
-// G(P) is formed in: FP64, or FP32 (integrals and densities rounded to FP32, FP32 accumulation)
-// in the reduced-precision phase of a mixed run; H is added in FP64 either way.
-template <int NT, typename T>
-__device__ void fock_build(const MolCtx& c, const double* __restrict__ Pt, const double* __restrict__ Psa, double xa,
-                           const double* __restrict__ Psb, double xb, double* __restrict__ Fa, double* __restrict__ Fb) {
-  const MolDev& M = c.M;
-  const int ld = c.ld;
-  const double* H = c.slot(S_H);
-  // Packed per-atom densities: pp[kl] = P[la][si] * (la == si ? 1 : 2).
-  for (int t = threadIdx.x; t < M.n_pp; t += NT) {
-    int la = find_atom(c.atom_pp, M.atom0, M.n_atoms, t);
-    int a = M.atom0 + la;
-    int e = t - c.atom_pp[a];
-    int mu, nu;
-    unpack_e(e, mu, nu);
-    int o = c.atom_ao[a];
-    c.pp[t] = (double)(T)(Pt[(int64_t)(o + mu) * ld + o + nu] * (mu == nu ? 1.0 : 2.0));
-  }
-  __syncthreads();
-  // Diagonal blocks: two-centre Coulomb gather + one-centre J - K.
-  for (int t = threadIdx.x; t < M.n_pp; t += NT) {
-    int la = find_atom(c.atom_pp, M.atom0, M.n_atoms, t);
-    int a = M.atom0 + la;
-    int e = t - c.atom_pp[a];
-    int mu, nu;
-    unpack_e(e, mu, nu);
-    const ElemDev ea = c.elem[c.atom_z[a]];
-    const int npa = npair_of(ea.n_orb);
-    T acc = (T)0;
-    for (int lb = 0; lb < M.n_atoms; ++lb) {
-      if (lb == la) continue;
-      int u = la < lb ? la : lb, v = la < lb ? lb : la;
-      const PairDev P = c.pairs[M.pair0 + tri_index(u, v, M.n_atoms)];
-      int b = M.atom0 + lb;
-      int ppb = c.atom_pp[b];
-      int npb = (lb + 1 < M.n_atoms ? c.atom_pp[b + 1] : M.n_pp) - ppb;
-      const double* wp = c.w + P.w_off;
-      const double* pb = c.pp + ppb;
-      if (P.f == a) {
-        const double* row = wp + (int64_t)e * npb;
-        for (int kl = 0; kl < npb; ++kl) acc += (T)row[kl] * (T)pb[kl];
-      } else {
-        for (int kl = 0; kl < npb; ++kl) acc += (T)wp[(int64_t)kl * npa + e] * (T)pb[kl];
-      }
-    }
-    const int o = c.atom_ao[a];
-    const int n = ea.n_orb;
-    T ja = (T)0, ka = (T)0, kb = (T)0;
-    for (int l = 0; l < n; ++l)
-      for (int s = 0; s < n; ++s) {
-        int64_t ix = (int64_t)(o + l) * ld + o + s;
-        ja += (T)Pt[ix] * (T)oc_int(mu, nu, l, s, ea);
-        const T gx = (T)oc_int(mu, l, nu, s, ea);
-        ka += (T)Psa[ix] * gx;
-        if (Fb) kb += (T)Psb[ix] * gx;
-      }
-    const int64_t i1 = (int64_t)(o + mu) * ld + o + nu, i2 = (int64_t)(o + nu) * ld + o + mu;
-    double fa = H[i1] + (double)(acc + ja - (T)xa * ka);
-    Fa[i1] = fa;
-    Fa[i2] = fa;
-    if (Fb) {
-      double fb = H[i1] + (double)(acc + ja - (T)xb * kb);
-      Fb[i1] = fb;
-      Fb[i2] = fb;
-    }
-  }
-  // Off-diagonal blocks: exchange, one owner per atom pair.
-  const int npairs = M.n_atoms * (M.n_atoms - 1) / 2;
-  for (int p = threadIdx.x; p < npairs; p += NT) {
-    const PairDev P = c.pairs[M.pair0 + p];
-    const ElemDev ef = c.elem[c.atom_z[P.f]];
-    const ElemDev es = c.elem[c.atom_z[P.s]];
-    const int nf = ef.n_orb, ns = es.n_orb, nps = npair_of(ns);
-    const int of = c.atom_ao[P.f], os = c.atom_ao[P.s];
-    const double* wp = c.w + P.w_off;
-    // Same-spin density block held in registers: every index below is a compile-time constant.
-    T Xa[4][4], Xb[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const bool in = i < nf && j < ns;
-        Xa[i][j] = in ? (T)Psa[(int64_t)(of + i) * ld + os + j] : (T)0;
-        Xb[i][j] = (in && Fb) ? (T)Psb[(int64_t)(of + i) * ld + os + j] : (T)0;
-      }
-#pragma unroll
-    for (int mu = 0; mu < 4; ++mu)
-#pragma unroll
-      for (int lam = 0; lam < 4; ++lam) {
-        if (mu >= nf || lam >= ns) continue;
-        T acca = (T)0, accb = (T)0;
-#pragma unroll
-        for (int nu = 0; nu < 4; ++nu) {
-          if (nu >= nf) continue;
-          const double* row = wp + (int64_t)pack_idx(mu, nu) * nps;
-#pragma unroll
-          for (int sg = 0; sg < 4; ++sg) {
-            if (sg >= ns) continue;
-            const T wv = (T)row[pack_idx(lam, sg)];
-            acca += Xa[nu][sg] * wv;
-            accb += Xb[nu][sg] * wv;
-          }
-        }
-        const int64_t i1 = (int64_t)(of + mu) * ld + os + lam, i2 = (int64_t)(os + lam) * ld + of + mu;
-        const double fa = H[i1] - (double)((T)xa * acca);
-        Fa[i1] = fa;
-        Fa[i2] = fa;
-        if (Fb) {
-          const double fb = H[i1] - (double)((T)xb * accb);
-          Fb[i1] = fb;
-          Fb[i2] = fb;
-        }
-      }
-  }
-  __syncthreads();
-}
-
-// sum_ij X_ij Y_ij over the n x n block (pad entries are zero).
-template <int NT>
-__device__ double frob(int n, int ld, const double* X, const double* Y, double* red) {
-  double acc = 0.0;
-  for (int idx = threadIdx.x; idx < n * n; idx += NT) {
-    int i = idx / n, j = idx - i * n;
-    acc += X[(int64_t)i * ld + j] * Y[(int64_t)i * ld + j];
-  }
-  return wg_sum<NT>(acc, red);
-}
 
 // ||FP - (FP)^T||_F^2; the commutator itself is stored into E when E != nullptr. F and P are
 // symmetric, so (FP)^T = PF and E = FP - PF tile by tile on the matrix cores; both products read
@@ -288,193 +119,6 @@ __device__ double density(int n, int ld, int nocc, double wgt, const double* __r
   return wg_sum<NT>(dsq, red);
 }
 
-
-// ---- small serial solvers (thread 0) -------------------------------------------------------
-
-__device__ bool solve_small(double* m, double* rhs, int n, double* x) {
-  for (int col = 0; col < n; ++col) {
-    int piv = col;
-    double best = fabs(m[col * n + col]);
-    for (int row = col + 1; row < n; ++row) {
-      double v = fabs(m[row * n + col]);
-      if (v > best) {
-        best = v;
-        piv = row;
-      }
-    }
-    if (best < 1.0e-12) return false;
-    if (piv != col) {
-      for (int j = 0; j < n; ++j) {
-        double t = m[col * n + j];
-        m[col * n + j] = m[piv * n + j];
-        m[piv * n + j] = t;
-      }
-      double t = rhs[col];
-      rhs[col] = rhs[piv];
-      rhs[piv] = t;
-    }
-    for (int row = col + 1; row < n; ++row) {
-      double factor = m[row * n + col] / m[col * n + col];
-      if (factor == 0.0) continue;
-      for (int j = col; j < n; ++j) m[row * n + j] -= factor * m[col * n + j];
-      rhs[row] -= factor * rhs[col];
-    }
-  }
-  for (int col = n - 1; col >= 0; --col) {
-    double sum = rhs[col];
-    for (int j = col + 1; j < n; ++j) sum -= m[col * n + j] * x[j];
-    x[col] = sum / m[col * n + col];
-  }
-  return true;
-}
-
-// CDIIS over the chronological window (pm7-rs src/scf.rs:1684-1771), run by one thread with its
-// <= 9x9 bordered system in the LDS scratch `wk` (b: 81, rhs: 9, x: 9). Returns the number of
-// coefficients (0 on failure); coefficient i belongs to chronological entry skip + i.
-__device__ int cdiis(int cnt, const int* slot, const double* G, double* coef, int* skip_out, double* wk) {
-  if (cnt < 2) return 0;
-  double* b = wk;
-  double* rhs = wk + (HIST + 1) * (HIST + 1);
-  double* x = rhs + HIST + 1;
-  for (int skip = 0; skip < cnt - 1; ++skip) {
-    const int m = cnt - skip, dim = m + 1;
-    double scale = 0.0;
-    for (int i = 0; i < m; ++i) scale = fmax(scale, fabs(G[slot[skip + i] * HIST + slot[skip + i]]));
-    if (!isfinite(scale) || scale <= 0.0) continue;
-    for (int i = 0; i < dim * dim; ++i) b[i] = 0.0;
-    for (int i = 0; i < m; ++i) {
-      for (int j = 0; j < m; ++j) b[i * dim + j] = G[slot[skip + i] * HIST + slot[skip + j]] / scale;
-      b[i * dim + m] = -1.0;
-      b[m * dim + i] = -1.0;
-      rhs[i] = 0.0;
-    }
-    rhs[m] = -1.0;
-    if (!solve_small(b, rhs, dim, x)) continue;
-    bool finite = true;
-    for (int i = 0; i < dim; ++i) finite = finite && isfinite(x[i]);
-    if (!finite) continue;
-    double sum = 0.0;
-    for (int i = 0; i < m; ++i) sum += fabs(x[i]);
-    if (sum <= 25.0) {
-      for (int i = 0; i < m; ++i) coef[i] = x[i];
-      *skip_out = skip;
-      return m;
-    }
-  }
-  return 0;
-}
-
-__device__ inline double wave_max(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-  return v;
-}
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-// A-DIIS on the simplex (pm7-rs src/scf.rs:1812-1916) from M[s][t] = <D_s, F_t>, run by one full
-// wavefront: lane i owns coefficient i, its gradient row and its rank in the simplex projection,
-// so the 400-step projected-gradient loop needs no memory traffic. Writes `cnt` coefficients.
-__device__ void adiis_wave(int cnt, const int* slot, const double* Mm, double* coef) {
-  const int lane = threadIdx.x & 63;
-  const int n = cnt;
-  const bool act = lane < n;
-  const int sn = slot[n - 1];
-  const double mnn = Mm[sn * HIST + sn];
-  double d = 0.0;
-  double srow[HIST];  // s[lane][j] + s[j][lane]
-#pragma unroll
-  for (int j = 0; j < HIST; ++j) srow[j] = 0.0;
-  if (act) {
-    const int si = slot[lane];
-    d = Mm[si * HIST + sn] - mnn;
-#pragma unroll
-    for (int j = 0; j < HIST; ++j) {
-      if (j < n) {
-        const int sj = slot[j];
-        const double sij = Mm[si * HIST + sj] - Mm[si * HIST + sn] - Mm[sn * HIST + sj] + mnn;
-        const double sji = Mm[sj * HIST + si] - Mm[sj * HIST + sn] - Mm[sn * HIST + si] + mnn;
-        srow[j] = sij + sji;
-      }
-    }
-  }
-  double rowabs = 0.0;
-#pragma unroll
-  for (int j = 0; j < HIST; ++j) rowabs += fabs(srow[j]);
-  const double lr = 1.0 / fmax(1.0e-12, wave_max(act ? rowabs : 0.0));
-  double c = (act && lane == n - 1) ? 1.0 : 0.0;
-  for (int it = 0; it < 400; ++it) {
-    double g = 2.0 * d;
-#pragma unroll
-    for (int j = 0; j < HIST; ++j) g += srow[j] * __shfl(c, j, 64);
-    const double trial = act ? c - lr * g : -INFINITY;
-    // Descending rank among the active lanes (ties by lane) and the inclusive prefix sum.
-    int rank = 0;
-    double css = trial;
-#pragma unroll
-    for (int j = 0; j < HIST; ++j) {
-      const double tj = __shfl(trial, j, 64);
-      const bool before = (j < n) && (tj > trial || (tj == trial && j < lane));
-      rank += before ? 1 : 0;
-      css += before ? tj : 0.0;
-    }
-    const double t = (css - 1.0) / (double)(rank + 1);
-    const bool cond = act && (trial - t > 0.0);
-    const double best = wave_max(cond ? (double)rank : -1.0);
-    const unsigned long long win = __ballot(cond && (double)rank == best);
-    const int wl = win ? (__ffsll((long long)win) - 1) : 0;
-    const double theta = win ? __shfl(t, wl, 64) : 0.0;
-    const double proj = act ? fmax(trial - theta, 0.0) : 0.0;
-    const double delta = wave_sum(act ? fabs(proj - c) : 0.0);
-    c = proj;
-    if (delta < 1.0e-12) break;
-  }
-  if (act) coef[lane] = c;
-}
-
-// Level-shift stall guard (pm7-rs src/scf.rs:1581-1656); state lives in LDS.
-struct Guard {
-  double floor_, shift;
-  double steps[10], energies[10];
-  int n;
-};
-
-__device__ void guard_update(Guard& g, double step, double energy) {
-  g.steps[g.n] = step;
-  g.energies[g.n] = energy;
-  g.n++;
-  if (g.n < 10) return;
-  double sn = INFINITY, sw = INFINITY, en = INFINITY, ew = INFINITY;
-  for (int i = 0; i < 5; ++i) {
-    sw = fmin(sw, g.steps[i]);
-    ew = fmin(ew, g.energies[i]);
-    sn = fmin(sn, g.steps[5 + i]);
-    en = fmin(en, g.energies[5 + i]);
-  }
-  bool moved = false;
-  if (sn <= sw * 0.1) {
-    if (g.shift > g.floor_) {
-      double relaxed = fmax(g.shift * 0.5, g.floor_);
-      g.shift = relaxed < 0.25 ? g.floor_ : relaxed;
-      moved = true;
-    }
-  } else if (sn > sw * 0.9 && en > ew - 1.0e-9 && g.shift < 64.0) {
-    g.shift = g.shift < 1.0 ? 1.0 : fmin(g.shift * 2.0, 64.0);
-    moved = true;
-  }
-  if (moved) {
-    g.n = 0;
-  } else {
-    for (int i = 0; i < 9; ++i) {
-      g.steps[i] = g.steps[i + 1];
-      g.energies[i] = g.energies[i + 1];
-    }
-    g.n = 9;
-  }
-}
 
 template <int NT>
 __device__ void level_shift(int n, int ld, double* W, const double* P, double occ, double shift) {
@@ -923,7 +567,6 @@ __global__ void __launch_bounds__(NT)
 PM7_SCF_INSTANCE(256, 64, true, true)
 PM7_SCF_INSTANCE(512, 112, true, true)
 PM7_SCF_INSTANCE(512, 196, true, false)
-PM7_SCF_INSTANCE(512, 1024, false, false)
 
 // One Fock build on a molecule's slab (the build_fock_spin entry point): F = H + J(Pt) - K(Ps).
 __global__ void __launch_bounds__(256) k_fock_once(int mi, const MolDev* __restrict__ mols, const int* __restrict__ atom_z,
@@ -962,9 +605,7 @@ hipError_t launch_scf(int variant, const BatchDev& b, int list_off, int count, h
                          b.atom_ao, b.atom_pp, b.pairs, b.elem, b.w, b.slab, b.vecs, b.out);
       break;
     default:
-      hipLaunchKernelGGL((k_scf<512, 1024, false, false>), dim3(count), dim3(512), 0, s, list, b.mols, b.atom_z,
-                         b.atom_ao, b.atom_pp, b.pairs, b.elem, b.w, b.slab, b.vecs, b.out);
-      break;
+      return hipErrorInvalidValue;  // the large class runs large_scf, not a megakernel
   }
   return hipGetLastError();
 }

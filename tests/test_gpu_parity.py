@@ -169,6 +169,27 @@ def test_mixed_precision_organic_batch_vs_fp64(eng):
         assert b.fp32_iterations > 0
 
 
+@pytest.mark.parametrize("n_water,precision", [(40, "fp64"), (40, "mixed"), (70, "fp64")])
+def test_large_path_water_cluster_matches_oracle(eng, n_water, precision):
+    """Basis sizes above 196 run the grid-wide path (rocSOLVER dsyevd, rocBLAS products, grid Fock
+    kernels): same bars against the oracle as the megakernel (FP64; mixed to its own tolerance)."""
+    atoms = W.water_cluster(n_water, 4)
+    say(f"large path: {n_water} waters ({6 * n_water} AOs), {precision}")
+    r = eng.run(mol_of(atoms), pm7amd.Pm7Options(precision=precision))
+    res, dens, *_ = oracle_of(atoms)
+    assert r.converged and r.density.shape == (6 * n_water, 6 * n_water)
+    assert abs(r.total_ev - res.total_ev) <= 1e-6
+    assert abs(r.heat_of_formation_kcal - res.hof_kcal) <= 1e-4
+    assert np.max(np.abs(r.density - dens)) <= (1e-7 if precision == "fp64" else 1e-5)
+    if precision == "mixed":
+        assert r.fp32_iterations > 0
+    # orbitals: orthonormal columns that reproduce the density
+    c = r.mo_coeff
+    assert np.max(np.abs(c.T @ c - np.eye(c.shape[0]))) <= 1e-9
+    occ = c[:, : r.n_occ]
+    assert np.max(np.abs(2.0 * occ @ occ.T - r.density)) <= 1e-9
+
+
 def test_config1_water(eng):
     say("config 1 water")
     atoms = W.water()

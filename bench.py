@@ -90,6 +90,9 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=24, help="molecules timed on the CPU oracle")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serving", type=int, default=2000, help="requests in the serving stream (0: skip)")
+    ap.add_argument("--large-waters", type=int, default=500,
+                    help="water cluster of the large-system run (0: skip); 500 waters = 3000 AOs")
+    ap.add_argument("--large-only", action="store_true", help="only the large-system run")
     ap.add_argument("--serve-lanes", type=int, default=4)
     ap.add_argument("--serve-batch", type=int, default=64)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -116,6 +119,45 @@ def main() -> int:
 
     import pm7amd
     from pm7amd import workloads as W
+
+    def large_system(eng):
+        """One large molecule (water cluster, seed 4) in FP64 and mixed precision on the grid-wide
+        path; the FP64 dense eigensolve is reported on its own."""
+        atoms = W.water_cluster(args.large_waters, 4)
+        mol = pm7amd.Molecule.from_symbols_angstrom(atoms)
+        out = {"workload": f"({args.large_waters} H2O) cluster, 1.0 g/cm3, O-O >= 2.6 A, seed 4",
+               "atoms": len(atoms), "nao": 6 * args.large_waters}
+        ref = None
+        for precision in ("fp64", "mixed"):
+            s0 = time.perf_counter()
+            r = eng.run(mol, pm7amd.Pm7Options(precision=precision), raise_on_error=False)
+            wall = time.perf_counter() - s0
+            ph = dict(zip(["fock", "energy_commutator", "diis", "eigensolve", "density", "total"],
+                          [round(x, 2) for x in r.device_phase_ms]))
+            occ = r.mo_coeff[:, : r.n_occ]
+            rec = {"status": r.status, "converged": r.converged, "iterations": r.iterations,
+                   "fp32_iterations": r.fp32_iterations, "wall_s": round(wall, 3),
+                   "solves_per_s": round(1.0 / wall, 4), "phase_ms": ph,
+                   "eigensolve_ms_per_iteration": round(r.device_phase_ms[3] / max(1, r.iterations), 2),
+                   "heat_of_formation_kcal": r.heat_of_formation_kcal, "total_ev": r.total_ev,
+                   "trace_P_minus_electrons": float(np.trace(r.density) - 2 * r.n_occ),
+                   "max_abs_2CCt_minus_P": float(np.max(np.abs(2.0 * occ @ occ.T - r.density)))}
+            if precision == "fp64":
+                ref = r
+            else:
+                rec["abs_dHf_vs_fp64_kcal"] = abs(r.heat_of_formation_kcal - ref.heat_of_formation_kcal)
+                rec["abs_dE_vs_fp64_ev"] = abs(r.total_ev - ref.total_ev)
+                rec["max_abs_dP_vs_fp64"] = float(np.max(np.abs(r.density - ref.density)))
+            out[precision] = rec
+            print(f"[bench] large {precision}: {rec['wall_s']} s, {rec['iterations']} iterations",
+                  file=sys.stderr, flush=True)
+        return out
+
+    if args.large_only:
+        eng = pm7amd.Engine(0)
+        print(json.dumps({"large_system": large_system(eng)}), flush=True)
+        eng.close()
+        return 0
 
     t0 = time.time()
     batch = W.organic_batch(args.molecules, 3)
@@ -203,6 +245,8 @@ def main() -> int:
                    "fp64": serve("fp64"), "mixed": serve("mixed")}
         print(f"[bench] serving: {serving['fp64']} / {serving['mixed']}", file=sys.stderr, flush=True)
 
+    large = large_system(eng) if args.large_waters > 0 else None
+
     # One molecule at a time (the plain FP64 GPU path the batched path is compared with).
     single = mols[: args.single]
     for m in single[:2]:
@@ -280,6 +324,7 @@ def main() -> int:
                                                [round(x, 3) for x in max(resmx, key=lambda r: r.device_phase_ms[5]).device_phase_ms])),
                   "iterations_mean": round(float(np.mean([r.iterations for r in resmx])), 2)},
         "serving": serving,
+        "large_system": large,
         "single_molecule_fp64": {"solves_per_s": round(single_rate, 3), "molecules": len(single),
                                  "batched_speedup": round((solvesmx if head_mixed else solves64) / single_rate, 3)},
         "kernel_ms": {"integrals": round(float(np.mean([k[0] for k in kern64])), 4),

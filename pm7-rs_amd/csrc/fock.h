@@ -123,6 +123,67 @@ __device__ inline void fock_diag_item(const MolCtx& c, int t, const double* __re
   }
 }
 
+// The diagonal-block element t with the partner sum split over the 64 lanes of one wavefront
+// (partner lb handled by lane lb % 64, then a fixed shuffle tree): for molecules with many atoms,
+// where one thread per element would walk thousands of partners in sequence. All lanes of the
+// wave call it with the same t; lane 0 writes.
+template <typename T>
+__device__ inline void fock_diag_item_wave(const MolCtx& c, int t, const double* __restrict__ Pt,
+                                           const double* __restrict__ Psa, double xa,
+                                           const double* __restrict__ Psb, double xb, double* __restrict__ Fa,
+                                           double* __restrict__ Fb) {
+  const MolDev& M = c.M;
+  const int ld = c.ld;
+  const int lane = threadIdx.x & 63;
+  int la = find_atom(c.atom_pp, M.atom0, M.n_atoms, t);
+  int a = M.atom0 + la;
+  int e = t - c.atom_pp[a];
+  int mu, nu;
+  unpack_e(e, mu, nu);
+  const ElemDev ea = c.elem[c.atom_z[a]];
+  const int npa = npair_of(ea.n_orb);
+  T acc = (T)0;
+  for (int lb = lane; lb < M.n_atoms; lb += 64) {
+    if (lb == la) continue;
+    int u = la < lb ? la : lb, v = la < lb ? lb : la;
+    const PairDev P = c.pairs[M.pair0 + tri_index(u, v, M.n_atoms)];
+    int b = M.atom0 + lb;
+    int ppb = c.atom_pp[b];
+    int npb = (lb + 1 < M.n_atoms ? c.atom_pp[b + 1] : M.n_pp) - ppb;
+    const double* wp = c.w + P.w_off;
+    const double* pb = c.pp + ppb;
+    if (P.f == a) {
+      const double* row = wp + (int64_t)e * npb;
+      for (int kl = 0; kl < npb; ++kl) acc += (T)row[kl] * (T)pb[kl];
+    } else {
+      for (int kl = 0; kl < npb; ++kl) acc += (T)wp[(int64_t)kl * npa + e] * (T)pb[kl];
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if (lane != 0) return;
+  const double* H = c.slot(S_H);
+  const int o = c.atom_ao[a];
+  const int n = ea.n_orb;
+  T ja = (T)0, ka = (T)0, kb = (T)0;
+  for (int l = 0; l < n; ++l)
+    for (int s = 0; s < n; ++s) {
+      int64_t ix = (int64_t)(o + l) * ld + o + s;
+      ja += (T)Pt[ix] * (T)oc_int(mu, nu, l, s, ea);
+      const T gx = (T)oc_int(mu, l, nu, s, ea);
+      ka += (T)Psa[ix] * gx;
+      if (Fb) kb += (T)Psb[ix] * gx;
+    }
+  const int64_t i1 = (int64_t)(o + mu) * ld + o + nu, i2 = (int64_t)(o + nu) * ld + o + mu;
+  double fa = H[i1] + (double)(acc + ja - (T)xa * ka);
+  Fa[i1] = fa;
+  Fa[i2] = fa;
+  if (Fb) {
+    double fb = H[i1] + (double)(acc + ja - (T)xb * kb);
+    Fb[i1] = fb;
+    Fb[i2] = fb;
+  }
+}
+
 template <typename T>
 __device__ inline void fock_pair_item(const MolCtx& c, int p, const double* __restrict__ Psa, double xa,
                                       const double* __restrict__ Psb, double xb, double* __restrict__ Fa,

@@ -92,17 +92,27 @@ __global__ void k_lg_pack(int mi, BatchDev b, const double* __restrict__ Pt) {
     fock_pack_item<T>(c, t, Pt);
 }
 
+// Diagonal blocks: one wavefront per (atom, orbital pair) element, lanes over the partner atoms.
 template <typename T>
-__global__ void k_lg_fock(int mi, BatchDev b, const double* __restrict__ Pt, const double* __restrict__ Psa,
-                          double xa, const double* __restrict__ Psb, double xb, double* __restrict__ Fa,
-                          double* __restrict__ Fb) {
+__global__ void __launch_bounds__(256) k_lg_fock_diag(int mi, BatchDev b, const double* __restrict__ Pt,
+                                                      const double* __restrict__ Psa, double xa,
+                                                      const double* __restrict__ Psb, double xb,
+                                                      double* __restrict__ Fa, double* __restrict__ Fb) {
+  const MolCtx c = make_ctx(b.mols[mi], b);
+  const int waves = gridDim.x * (blockDim.x >> 6);
+  for (int t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); t < c.M.n_pp; t += waves)
+    fock_diag_item_wave<T>(c, t, Pt, Psa, xa, Psb, xb, Fa, Fb);
+}
+
+// Off-diagonal blocks: one thread per atom pair.
+template <typename T>
+__global__ void __launch_bounds__(256) k_lg_fock_pairs(int mi, BatchDev b, const double* __restrict__ Psa, double xa,
+                                                       const double* __restrict__ Psb, double xb,
+                                                       double* __restrict__ Fa, double* __restrict__ Fb) {
   const MolCtx c = make_ctx(b.mols[mi], b);
   const int64_t npairs = (int64_t)c.M.n_atoms * (c.M.n_atoms - 1) / 2;
-  const int64_t total = c.M.n_pp + npairs;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-    if (t < c.M.n_pp) fock_diag_item<T>(c, (int)t, Pt, Psa, xa, Psb, xb, Fa, Fb);
-    else fock_pair_item<T>(c, (int)(t - c.M.n_pp), Psa, xa, Psb, xb, Fa, Fb);
-  }
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += (int64_t)gridDim.x * blockDim.x)
+    fock_pair_item<T>(c, (int)p, Psa, xa, Psb, xb, Fa, Fb);
 }
 
 // Block-partial sums of up to 3 dot products <X_k, Y_k> over the n x n block, into
@@ -570,12 +580,12 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
       LG(hipEventRecord(lw->ev[0], s));
       if (low) {
         hipLaunchKernelGGL(k_lg_pack<float>, dim3(grid_for(M.n_pp)), dim3(256), 0, s, mi, b, P);
-        hipLaunchKernelGGL(k_lg_fock<float>, dim3(grid_for(M.n_pp + (int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256),
-                           0, s, mi, b, P, P, 0.5, nullptr, 0.0, F, nullptr);
+        hipLaunchKernelGGL(k_lg_fock_diag<float>, dim3(grid_for((int64_t)M.n_pp * 64)), dim3(256), 0, s, mi, b, P, P, 0.5, nullptr, 0.0, F, nullptr);
+        hipLaunchKernelGGL(k_lg_fock_pairs<float>, dim3(grid_for((int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256), 0, s, mi, b, P, 0.5, nullptr, 0.0, F, nullptr);
       } else {
         hipLaunchKernelGGL(k_lg_pack<double>, dim3(grid_for(M.n_pp)), dim3(256), 0, s, mi, b, P);
-        hipLaunchKernelGGL(k_lg_fock<double>, dim3(grid_for(M.n_pp + (int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256),
-                           0, s, mi, b, P, P, 0.5, nullptr, 0.0, F, nullptr);
+        hipLaunchKernelGGL(k_lg_fock_diag<double>, dim3(grid_for((int64_t)M.n_pp * 64)), dim3(256), 0, s, mi, b, P, P, 0.5, nullptr, 0.0, F, nullptr);
+        hipLaunchKernelGGL(k_lg_fock_pairs<double>, dim3(grid_for((int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256), 0, s, mi, b, P, 0.5, nullptr, 0.0, F, nullptr);
       }
       LG(hipGetLastError());
       LG(hipEventRecord(lw->ev[1], s));
@@ -625,8 +635,8 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
     double* P = slot(S_HD + hp);
     double* F = slot(S_F);
     hipLaunchKernelGGL(k_lg_pack<double>, dim3(grid_for(M.n_pp)), dim3(256), 0, s, mi, b, P);
-    hipLaunchKernelGGL(k_lg_fock<double>, dim3(grid_for(M.n_pp + (int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256), 0, s,
-                       mi, b, P, P, 0.5, nullptr, 0.0, F, nullptr);
+    hipLaunchKernelGGL(k_lg_fock_diag<double>, dim3(grid_for((int64_t)M.n_pp * 64)), dim3(256), 0, s, mi, b, P, P, 0.5, nullptr, 0.0, F, nullptr);
+        hipLaunchKernelGGL(k_lg_fock_pairs<double>, dim3(grid_for((int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256), 0, s, mi, b, P, 0.5, nullptr, 0.0, F, nullptr);
     LG(dots(2, P, slot(S_H), P, F, nullptr, nullptr, d3));
     hipLaunchKernelGGL(k_lg_energy, dim3(1), dim3(64), 0, s, ctl, d3, 0);
     if (host.guard.shift > 0.0) {
@@ -657,12 +667,12 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
       LGB(rocblas_daxpy(lw->blas, (rocblas_int)sq, &one, Pb, 1, Pt, 1));
       if (low) {
         hipLaunchKernelGGL(k_lg_pack<float>, dim3(grid_for(M.n_pp)), dim3(256), 0, s, mi, b, Pt);
-        hipLaunchKernelGGL(k_lg_fock<float>, dim3(grid_for(M.n_pp + (int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256),
-                           0, s, mi, b, Pt, Pa, 1.0, Pb, 1.0, Fa, Fb);
+        hipLaunchKernelGGL(k_lg_fock_diag<float>, dim3(grid_for((int64_t)M.n_pp * 64)), dim3(256), 0, s, mi, b, Pt, Pa, 1.0, Pb, 1.0, Fa, Fb);
+        hipLaunchKernelGGL(k_lg_fock_pairs<float>, dim3(grid_for((int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256), 0, s, mi, b, Pa, 1.0, Pb, 1.0, Fa, Fb);
       } else {
         hipLaunchKernelGGL(k_lg_pack<double>, dim3(grid_for(M.n_pp)), dim3(256), 0, s, mi, b, Pt);
-        hipLaunchKernelGGL(k_lg_fock<double>, dim3(grid_for(M.n_pp + (int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256),
-                           0, s, mi, b, Pt, Pa, 1.0, Pb, 1.0, Fa, Fb);
+        hipLaunchKernelGGL(k_lg_fock_diag<double>, dim3(grid_for((int64_t)M.n_pp * 64)), dim3(256), 0, s, mi, b, Pt, Pa, 1.0, Pb, 1.0, Fa, Fb);
+        hipLaunchKernelGGL(k_lg_fock_pairs<double>, dim3(grid_for((int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256), 0, s, mi, b, Pa, 1.0, Pb, 1.0, Fa, Fb);
       }
       LG(hipGetLastError());
       LG(hipEventRecord(lw->ev[1], s));
@@ -713,8 +723,8 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
     hipLaunchKernelGGL(k_lg_copy, dim3(grid_for(sq)), dim3(256), 0, s, sq, Pa, Pt);
     LGB(rocblas_daxpy(lw->blas, (rocblas_int)sq, &one, Pb, 1, Pt, 1));
     hipLaunchKernelGGL(k_lg_pack<double>, dim3(grid_for(M.n_pp)), dim3(256), 0, s, mi, b, Pt);
-    hipLaunchKernelGGL(k_lg_fock<double>, dim3(grid_for(M.n_pp + (int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256), 0, s,
-                       mi, b, Pt, Pa, 1.0, Pb, 1.0, Fa, Fb);
+    hipLaunchKernelGGL(k_lg_fock_diag<double>, dim3(grid_for((int64_t)M.n_pp * 64)), dim3(256), 0, s, mi, b, Pt, Pa, 1.0, Pb, 1.0, Fa, Fb);
+        hipLaunchKernelGGL(k_lg_fock_pairs<double>, dim3(grid_for((int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256), 0, s, mi, b, Pa, 1.0, Pb, 1.0, Fa, Fb);
     LG(dots(3, Pt, slot(U_H), Pa, Fa, Pb, Fb, d3));
     hipLaunchKernelGGL(k_lg_energy, dim3(1), dim3(64), 0, s, ctl, d3, 1);
     if (host.guard.shift > 0.0) LG(eig_full(Fa, eps_a, slot(U_CA0)));

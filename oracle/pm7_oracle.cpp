@@ -10,12 +10,15 @@
 // (tools/oracle/baselines/reference.json), by the reference's unit-test constants and by closed
 // forms; see tests/test_oracle_*.py.
 //
-// Scope of this restatement: the sp Dewar-Sabelli-Klopman path (every element whose valence
-// shell is s or sp with principal quantum number <= 3: H He Li Be B C N O F Ne Na Mg), RHF and
-// UHF, A-DIIS -> CDIIS, the level-shift stall guard, core-core repulsion, D2 dispersion, the EH+
-// hydrogen-bond term, the MM corrections, Mulliken charges, the dipole breakdown and the heat of
-// formation. Molecules holding a d-bearing element (Al..Cl and heavier) are reported as
-// unsupported (status 3): the MNDO/d two-centre kernel is not restated here.
+// Scope of this restatement: the sp Dewar-Sabelli-Klopman path (closed-form Slater overlaps for
+// n <= 3, the prolate-spheroidal quadrature for heavier sp shells), the MNDO/d path that every
+// molecule holding a d-bearing element takes (one-centre spd integrals, multipole two-centre
+// integrals, the spd rotation, the PM7 d-balance, spd overlaps), RHF and UHF, A-DIIS -> CDIIS,
+// the level-shift stall guard, core-core repulsion, D2 dispersion, the EH+ hydrogen-bond term, the
+// MM corrections, Mulliken charges, the dipole breakdown (point charge, sp and pd hybrid) and the
+// heat of formation. Two tables are data that the formulas need as given: the multipole content
+// of each local charge distribution (MULTIPOLES) and the one-centre spd integral map
+// (ONE_CENTRE_D).
 
 #include "pm7_oracle.h"
 
@@ -84,6 +87,17 @@ void parallel_for(long n, F&& body) {
 // ---------------------------------------------------------------------------------------------
 // Parameters (src/params.rs:205-432, 604-652)
 // ---------------------------------------------------------------------------------------------
+// MNDO/d one-centre data of an element (src/mndod.rs:188-204), 1-based slots.
+struct DShell {
+  double repd[53] = {0};  // one-centre d integrals (eV)
+  double po[10] = {0};    // additive terms (Bohr): 1 ss, 2 sp, 3 pp, 4 sd, 5 pd, 6 dd, 7 pp mono, 8 dd mono, 9 core
+  double ddp[7] = {0};    // charge separations (Bohr): 2 sp, 3 pp, 4 sd, 5 pd, 6 dd
+  double aij[7] = {0};    // multipole coefficients
+  double eiscor = 0.0;    // isolated-atom d-shell correction (eV)
+  bool has_d = false;
+  std::vector<double> oc;  // 45 x 45 one-centre matrix of a d-bearing element
+};
+
 struct Element {
   int z = 0, n = 0, ios = 0, iop = 0, iod = 0, n_orb = 0;
   double core_charge = 0, eheat_ev = 0, u_ss = 0, u_pp = 0, u_dd = 0;
@@ -92,6 +106,7 @@ struct Element {
   double zsn = 0, zpn = 0, zdn = 0, f0sd = 0, g2sd = 0, poc = 0;
   std::vector<std::array<double, 3>> gauss;  // (K, L, M)
   double e_isol = 0, dd = 0, qq = 0, rho0 = 0, rho1 = 0, rho2 = 0;
+  DShell ds;
   bool has_p() const { return n_orb >= 4; }
   double po9() const { return poc > 1.0e-5 ? poc : rho0; }  // src/params.rs:80-87
 };
@@ -177,6 +192,8 @@ std::vector<std::vector<std::string>> read_tsv(const std::string& path) {
 
 std::map<std::string, Params> g_params_cache;
 
+void derive_dshell(Element& e);
+
 const Params* load_params(const char* data_dir, int method) {
   std::string key = std::string(data_dir ? data_dir : ".") + "#" + std::to_string(method);
   auto it = g_params_cache.find(key);
@@ -237,8 +254,8 @@ const Params* load_params(const char* data_dir, int method) {
       double k = v(a), l = v(b), m = v(c);
       if (k != 0.0 || l != 0.0 || m != 0.0) e.gauss.push_back({k, l, m});
     }
-    // Isolated-atom energy (src/params.rs:268-286). The d-shell `eiscor` term is zero for every
-    // element this restatement accepts.
+    // Isolated-atom energy (src/params.rs:268-286); the d-shell eiscor term is added below
+    // (src/params.rs:316-340).
     double gssc = std::max(e.ios - 1, 0);
     double p = e.iop;
     double l = std::min(p, 6.0 - p);
@@ -249,6 +266,8 @@ const Params* load_params(const char* data_dir, int method) {
     e.e_isol = e.u_ss * e.ios + e.u_pp * p + e.u_dd * e.iod + e.g_ss * gssc + e.g_pp * gppc +
                e.g_sp * gspc + e.g_p2 * gp2c + e.h_sp * hspc;
     e.rho0 = e.g_ss != 0.0 ? EV / (2.0 * e.g_ss) : 0.0;
+    derive_dshell(e);  // every element: the MNDO/d kernel needs po/ddp of both atoms
+    e.e_isol += e.ds.eiscor;
     P.el[e.z] = e;
   }
   for (auto& row : read_tsv(prefix + "_pairs.tsv")) {
@@ -486,9 +505,9 @@ void rotate_xx(const double ri[22], const double* r0, const double* r1, const do
 
 struct PairBlock {
   int a = 0, b = 0;      // ordered: a has at least as many AOs as b
-  int npa = 1, npb = 1;  // packed pair counts (1 or 10)
+  int npa = 1, npb = 1;  // packed pair counts (1, 10 or 45)
   std::vector<double> w;  // npa x npb
-  double e1b[4][4] = {{0}}, e2a[4][4] = {{0}}, s[4][4] = {{0}};
+  double e1b[9][9] = {{0}}, e2a[9][9] = {{0}}, s[9][9] = {{0}};
 };
 
 // src/integrals.rs:186-271 for the sp path.
@@ -824,6 +843,907 @@ bool diatom_overlap(const Element& ei, const Element& ej, V3 d, double out[4][4]
 }
 
 // ---------------------------------------------------------------------------------------------
+// MNDO/d one-centre machinery (src/mndod.rs:188-735): per element, the 52 one-centre d integrals
+// repd, the Klopman-Ohno additive terms po(1..9), the multipole charge separations ddp(2..6), the
+// isolated-atom d-shell correction eiscor and, for a d-bearing element, the 45 x 45 one-centre
+// matrix. Arrays keep the 1-based slots of the formulas (slot 0 unused).
+// ---------------------------------------------------------------------------------------------
+int shell_n_sp(int z) {  // principal quantum number of the s/p shell (src/mndod.rs:206-216)
+  if (z <= 2) return 1;
+  if (z <= 10) return 2;
+  if (z <= 18) return 3;
+  if (z <= 36) return 4;
+  if (z <= 54) return 5;
+  if (z <= 86) return 6;
+  return 0;
+}
+int shell_n_d(int z) {  // principal quantum number of the d shell (src/mndod.rs:219-227)
+  if (z <= 30) return 3;
+  if (z <= 48) return 4;
+  if (z <= 80) return 5;
+  if (z <= 86) return 6;
+  return 0;
+}
+bool is_main_group(int z) {  // false only for the transition metals (src/mndod.rs:230-232)
+  return !((z >= 21 && z <= 29) || (z >= 39 && z <= 47) || (z >= 57 && z <= 79));
+}
+
+// fx[i] = (i-1)!, bin[i][j] = C(i-1, j-1) (src/mndod.rs:235-258).
+struct FactBinom {
+  double fx[31] = {0};
+  double bin[31][31] = {{0}};
+  FactBinom() {
+    fx[1] = 1.0;
+    for (int i = 2; i <= 30; ++i) fx[i] = fx[i - 1] * (i - 1.0);
+    for (int i = 0; i <= 30; ++i) bin[i][1] = 1.0;
+    for (int i = 2; i <= 30; ++i)
+      for (int j = 2; j <= i; ++j) bin[i][j] = bin[i - 1][j - 1] + bin[i - 1][j];
+  }
+};
+const FactBinom& fbx() {
+  static const FactBinom t;
+  return t;
+}
+
+// Radial Slater-Condon integral R^k (na ea nb eb | nc ec nd ed) in eV (src/mndod.rs:262-331).
+double slater_condon(int k, int na, double ea, int nb, double eb, int nc, double ec, int nd, double ed) {
+  const FactBinom& T = fbx();
+  const double la = std::log(ea), lb = std::log(eb), lc = std::log(ec), ld = std::log(ed);
+  const int nab = na + nb, ncd = nc + nd, n = nab + ncd;
+  const double ecd = ec + ed, eab = ea + eb, e = ecd + eab;
+  const double le = std::log(e), l2 = std::log(2.0), lcd = std::log(ecd), lab = std::log(eab);
+  const double ff = T.fx[n] / std::sqrt(T.fx[2 * na + 1] * T.fx[2 * nb + 1] * T.fx[2 * nc + 1] * T.fx[2 * nd + 1]);
+  const double c = EV * ff *
+                   std::exp(na * la + nb * lb + nc * lc + nd * ld + 0.5 * (la + lb + lc + ld) + l2 * (n + 2.0) -
+                            le * n);
+  double s0 = 1.0 / e, s1 = 0.0, s2 = 0.0;
+  const int m = ncd - k;
+  for (int i = 1; i <= m; ++i) {
+    s0 *= e / ecd;
+    s1 += s0 * (T.bin[ncd - k][i] - T.bin[ncd + k + 1][i]) / T.bin[n][i];
+  }
+  const int m2 = ncd + k + 1;
+  for (int i = m + 1; i <= m2; ++i) {
+    s0 *= e / ecd;
+    s2 += s0 * T.bin[m2][i] / T.bin[n][i];
+  }
+  const double s3 = std::exp(le * n - lcd * m2 - lab * (nab - k)) / T.bin[n][m2];
+  return c * (s1 - s2 + s3);
+}
+
+// Isolated-atom d-shell correction: reference-configuration weights of the radial integrals
+// F0(dd) G2(sd) ... per transition metal (src/mndod.rs:480-520), then the weighted sum.
+double d_shell_correction(int z, double r016, double r066, double r244, double r266, double r466) {
+  struct W { int z, a016, a066, a244, a266, a466; };
+  static const W tab[] = {
+      {21, 2, 0, 1, 0, 0},    {22, 4, 1, 2, 8, 1},    {23, 6, 3, 3, 15, 8},   {24, 5, 10, 5, 35, 35},
+      {25, 10, 10, 5, 35, 35}, {26, 12, 15, 6, 35, 35}, {27, 14, 21, 7, 43, 36}, {28, 16, 28, 8, 50, 43},
+      {29, 10, 45, 5, 70, 70}, {39, 2, 0, 1, 0, 0},    {40, 4, 1, 2, 8, 1},    {41, 4, 6, 4, 21, 21},
+      {42, 5, 10, 5, 35, 35},  {43, 10, 10, 5, 35, 35}, {44, 7, 21, 5, 43, 36}, {45, 8, 28, 5, 50, 43},
+      {46, 0, 45, 0, 70, 70},  {47, 10, 45, 5, 70, 70}, {57, 2, 0, 1, 0, 0},    {71, 2, 0, 1, 0, 0},
+      {72, 4, 1, 2, 8, 1},     {73, 6, 3, 3, 15, 8},   {74, 5, 10, 5, 35, 35}, {75, 10, 10, 5, 35, 35},
+      {76, 12, 15, 6, 35, 35}, {77, 14, 21, 7, 43, 36}, {78, 9, 36, 5, 56, 56}, {79, 10, 45, 5, 70, 70}};
+  for (const W& w : tab)
+    if (w.z == z)
+      return w.a016 * r016 + w.a066 * r066 - w.a244 * r244 / 5.0 - w.a266 * r266 / 49.0 - w.a466 * r466 / 49.0;
+  return 0.0;
+}
+
+// The 52 one-centre d integrals from the twelve radial parameters (src/mndod.rs:334-471). The
+// square roots are MOPAC's own truncated constants.
+void one_centre_d(int z, double zsn, double zpn, double zdn, double f0sd, double g2sd, double repd[53],
+                  double& eiscor) {
+  const int ns = shell_n_sp(z), nd = shell_n_d(z);
+  const double es = zsn, ep = zpn, ed = zdn;
+  double r016 = slater_condon(0, ns, es, ns, es, nd, ed, nd, ed);
+  const double r036 = slater_condon(0, ns, ep, ns, ep, nd, ed, nd, ed);
+  const double r066 = slater_condon(0, nd, ed, nd, ed, nd, ed, nd, ed);
+  const double r155 = slater_condon(1, ns, ep, nd, ed, ns, ep, nd, ed);
+  const double r125 = slater_condon(1, ns, es, ns, ep, ns, ep, nd, ed);
+  double r244 = slater_condon(2, ns, es, nd, ed, ns, es, nd, ed);
+  const double r236 = slater_condon(2, ns, ep, ns, ep, nd, ed, nd, ed);
+  const double r266 = slater_condon(2, nd, ed, nd, ed, nd, ed, nd, ed);
+  const double r234 = slater_condon(2, ns, ep, ns, ep, ns, es, nd, ed);
+  const double r246 = slater_condon(2, ns, es, nd, ed, nd, ed, nd, ed);
+  const double r355 = slater_condon(3, ns, ep, nd, ed, ns, ep, nd, ed);
+  const double r466 = slater_condon(4, nd, ed, nd, ed, nd, ed, nd, ed);
+  if (f0sd > 0.001) r016 = f0sd;
+  if (g2sd > 0.001) r244 = g2sd;
+  eiscor = d_shell_correction(z, r016, r066, r244, r266, r466);
+  const double s3 = 1.7320508, s5 = 2.23606797, s15 = 3.87298334;
+  double* d = repd;
+  for (int i = 0; i < 53; ++i) d[i] = 0.0;
+  d[1] = r016;
+  d[2] = 2.0 / (3.0 * s5) * r125;
+  d[3] = 1.0 / s15 * r125;
+  d[4] = 2.0 / (5.0 * s5) * r234;
+  d[5] = r036 + 4.0 / 35.0 * r236;
+  d[6] = r036 + 2.0 / 35.0 * r236;
+  d[7] = r036 - 4.0 / 35.0 * r236;
+  d[8] = -1.0 / (3.0 * s5) * r125;
+  d[9] = std::sqrt(3.0 / 125.0) * r234;
+  d[10] = s3 / 35.0 * r236;
+  d[11] = 3.0 / 35.0 * r236;
+  d[12] = -1.0 / (5.0 * s5) * r234;
+  d[13] = r036 - 2.0 / 35.0 * r236;
+  d[14] = -2.0 * s3 / 35.0 * r236;
+  d[15] = -d[3];
+  d[16] = -d[11];
+  d[17] = -d[9];
+  d[18] = -d[14];
+  d[19] = 1.0 / 5.0 * r244;
+  d[20] = 2.0 / (7.0 * s5) * r246;
+  d[21] = d[20] / 2.0;
+  d[22] = -d[20];
+  d[23] = 4.0 / 15.0 * r155 + 27.0 / 245.0 * r355;
+  d[24] = 2.0 * s3 / 15.0 * r155 - 9.0 * s3 / 245.0 * r355;
+  d[25] = 1.0 / 15.0 * r155 + 18.0 / 245.0 * r355;
+  d[26] = -s3 / 15.0 * r155 + 12.0 * s3 / 245.0 * r355;
+  d[27] = -s3 / 15.0 * r155 - 3.0 * s3 / 245.0 * r355;
+  d[28] = -d[27];
+  d[29] = r066 + 4.0 / 49.0 * r266 + 4.0 / 49.0 * r466;
+  d[30] = r066 + 2.0 / 49.0 * r266 - 24.0 / 441.0 * r466;
+  d[31] = r066 - 4.0 / 49.0 * r266 + 6.0 / 441.0 * r466;
+  d[32] = std::sqrt(3.0 / 245.0) * r246;
+  d[33] = 1.0 / 5.0 * r155 + 24.0 / 245.0 * r355;
+  d[34] = 1.0 / 5.0 * r155 - 6.0 / 245.0 * r355;
+  d[35] = 3.0 / 49.0 * r355;
+  d[36] = 1.0 / 49.0 * r266 + 30.0 / 441.0 * r466;
+  d[37] = s3 / 49.0 * r266 - 5.0 * s3 / 441.0 * r466;
+  d[38] = r066 - 2.0 / 49.0 * r266 - 4.0 / 441.0 * r466;
+  d[39] = -2.0 * s3 / 49.0 * r266 + 10.0 * s3 / 441.0 * r466;
+  d[40] = -d[32];
+  d[41] = -d[34];
+  d[42] = -d[35];
+  d[43] = -d[37];
+  d[44] = 3.0 / 49.0 * r266 + 20.0 / 441.0 * r466;
+  d[45] = -d[39];
+  d[46] = 1.0 / 5.0 * r155 - 3.0 / 35.0 * r355;
+  d[47] = -d[46];
+  d[48] = 4.0 / 49.0 * r266 + 15.0 / 441.0 * r466;
+  d[49] = 3.0 / 49.0 * r266 - 5.0 / 147.0 * r466;
+  d[50] = -d[49];
+  d[51] = r066 + 4.0 / 49.0 * r266 - 34.0 / 441.0 * r466;
+  d[52] = 35.0 / 441.0 * r466;
+}
+
+// Multipole coefficient a_l of the charge distribution (n1 z1)(n2 z2) (Thiel-Voityuk Eq. 7;
+// src/mndod.rs:523-535).
+double multipole_coef(double z1, double z2, int n1, int n2, int l) {
+  const FactBinom& T = fbx();
+  const double zz = z1 + z2 + 1.0e-20;
+  const double a = 2.0 * z1 / zz, b = 2.0 * z2 / zz;
+  return T.fx[n1 + n2 + l + 1] / std::sqrt(T.fx[2 * n1 + 1] * T.fx[2 * n2 + 1]) * std::pow(a, n1) * std::sqrt(a) *
+         std::pow(b, n2) * std::sqrt(b) * std::pow(2.0, l) / std::pow(zz, l);
+}
+
+// Additive term rho of multipole order l whose self-interaction at zero separation equals fg
+// (src/mndod.rs:557-609): closed form for l = 0, else a 100-step golden-section search on
+// [0.1, 5] Bohr with the squared residual.
+double additive_term(int l, double d, double fg) {
+  if (l == 0) return 0.5 * EV / fg;
+  const double dsq = d * d, ev4 = EV * 0.25, ev8 = EV / 8.0;
+  auto resid = [&](double y) {
+    double v = l == 1 ? ev4 * (1.0 / y - 1.0 / std::sqrt(y * y + dsq))
+                      : ev8 * (1.0 / y - 2.0 / std::sqrt(y * y + dsq * 0.5) + 1.0 / std::sqrt(y * y + dsq));
+    return (v - fg) * (v - fg);
+  };
+  double a1 = 0.1, a2 = 5.0, f1 = 0.0, f2 = 0.0;
+  for (int it = 0; it < 100; ++it) {
+    const double delta = a2 - a1;
+    if (delta < 1.0e-8) break;
+    const double y1 = a1 + delta * 0.382, y2 = a1 + delta * 0.618;
+    f1 = resid(y1);
+    f2 = resid(y2);
+    if (f1 < f2) a2 = y2;
+    else a1 = y1;
+  }
+  return f1 >= f2 ? a2 : a1;
+}
+
+// The one-centre spd matrix of a d-bearing element (src/mndod.rs:693-735): sp block from
+// gss gsp gpp gp2 hsp, d block from repd. Each entry (ij, kl, n) below sets the packed pairs
+// (ij, kl) and (kl, ij) to repd[n]; pairs are packed h(h+1)/2 + l over the orbital order
+// s px py pz dx2-y2 dxz dz2 dyz dxy.
+const short ONE_CENTRE_D[] = {
+    0,14,1, 0,20,1, 0,27,1, 0,35,1, 0,44,1, 6,24,2, 1,11,3, 1,18,3,
+    1,38,3, 3,31,3, 3,37,3, 6,16,3, 6,30,3, 9,21,4, 9,27,5, 2,14,6,
+    2,20,6, 2,44,6, 5,14,6, 5,35,6, 5,44,6, 9,20,6, 9,35,6, 2,35,7,
+    5,20,7, 9,14,7, 9,44,7, 1,22,8, 3,23,8, 2,10,9, 4,36,9, 7,15,9,
+    8,28,9, 7,26,10, 8,34,10, 4,33,11, 7,19,11, 7,43,11, 8,41,11, 2,21,12,
+    5,21,12, 2,27,13, 5,27,13, 2,25,14, 4,42,14, 3,12,15, 8,32,16, 5,10,17,
+    5,25,18, 10,10,19, 15,15,19, 21,21,19, 28,28,19, 36,36,19, 21,27,20, 15,26,21,
+    20,21,21, 21,35,21, 28,34,21, 10,25,22, 14,21,22, 21,44,22, 36,42,22, 24,24,23,
+    16,24,24, 24,30,24, 22,22,25, 23,23,25, 18,22,26, 23,31,26, 11,22,27, 22,38,27,
+    23,37,27, 12,23,28, 14,14,29, 20,20,29, 27,27,29, 35,35,29, 44,44,29, 20,27,30,
+    27,35,30, 14,27,31, 27,44,31, 10,20,32, 15,19,32, 15,43,32, 28,41,32, 33,36,32,
+    11,11,33, 12,12,33, 16,16,33, 18,18,33, 30,30,33, 31,31,33, 37,37,33, 38,38,33,
+    11,18,34, 16,30,34, 18,38,34, 31,37,34, 13,13,35, 13,16,35, 17,17,35, 17,29,35,
+    17,39,35, 29,29,35, 29,39,35, 39,39,35, 26,26,36, 34,34,36, 19,26,37, 26,43,37,
+    34,41,37, 14,20,38, 14,35,38, 20,35,38, 20,44,38, 35,44,38, 20,25,39, 33,42,39,
+    10,35,40, 28,32,40, 12,31,41, 13,30,42, 32,34,43, 19,19,44, 32,32,44, 33,33,44,
+    41,41,44, 43,43,44, 25,35,45, 11,38,46, 12,37,47, 25,25,48, 42,42,48, 19,43,49,
+    32,41,50, 14,44,51, 40,40,52,
+};
+
+void derive_dshell(Element& e) {  // src/mndod.rs:612-690 (MOPAC inid for one element)
+  DShell& D = e.ds;
+  D = DShell();
+  D.has_d = e.zeta_d > 1.0e-20;
+  const int z = e.z;
+  if (z >= 3 && !(e.zeta_s * e.zeta_p < 0.01)) {  // src/mndod.rs:538-555
+    const int nsp = shell_n_sp(z);
+    D.aij[2] = multipole_coef(e.zeta_s, e.zeta_p, nsp, nsp, 1);
+    D.aij[3] = multipole_coef(e.zeta_p, e.zeta_p, nsp, nsp, 2);
+    if (D.has_d) {
+      const int nd = shell_n_d(z);
+      D.aij[4] = multipole_coef(e.zeta_s, e.zeta_d, nsp, nd, 2);
+      D.aij[5] = multipole_coef(e.zeta_p, e.zeta_d, nsp, nd, 1);
+      D.aij[6] = multipole_coef(e.zeta_d, e.zeta_d, nd, nd, 2);
+    }
+  }
+  if (D.has_d && e.zdn > 1.0e-4) one_centre_d(z, e.zsn, e.zpn, e.zdn, e.f0sd, e.g2sd, D.repd, D.eiscor);
+  const double* rp = D.repd;
+  if (e.g_ss > 0.1) D.po[1] = additive_term(0, 1.0, e.g_ss);
+  if (z >= 3) {
+    D.ddp[2] = D.aij[2] / std::sqrt(12.0);
+    D.po[2] = additive_term(1, D.ddp[2], e.h_sp);
+    D.po[7] = D.po[1];
+    D.ddp[3] = std::sqrt(D.aij[3] * 0.1);
+    D.po[3] = additive_term(2, D.ddp[3], 0.5 * (e.g_pp - e.g_p2));
+    if (D.has_d) {
+      D.ddp[4] = std::sqrt(D.aij[4] * std::sqrt(1.0 / 60.0));
+      D.po[4] = additive_term(2, D.ddp[4], rp[19]);
+      D.ddp[5] = D.aij[5] / std::sqrt(20.0);
+      D.po[5] = additive_term(1, D.ddp[5], rp[23] - 1.8 * rp[35]);
+      const double fg = 0.2 * (rp[29] + 2.0 * rp[30] + 2.0 * rp[31]);
+      D.po[8] = fg > 1.0e-5 ? additive_term(0, 1.0, fg) : 1.0e5;
+      D.ddp[6] = std::sqrt(D.aij[6] / 14.0);
+      D.po[6] = additive_term(2, D.ddp[6], rp[44] - (20.0 / 35.0) * rp[52]);
+    }
+  }
+  if (is_main_group(z) || !D.has_d) {  // the validated sp terms win for main-group elements
+    D.po[1] = e.rho0;
+    if (e.rho1 > 1.0e-5) D.po[2] = e.rho1;
+    if (e.rho2 > 1.0e-5) D.po[3] = e.rho2;
+    D.po[7] = D.po[1];
+    D.ddp[2] = e.dd;
+    D.ddp[3] = e.qq * std::sqrt(2.0);
+  }
+  D.po[9] = e.poc > 1.0e-5 ? e.poc : D.po[1];
+  if (!D.has_d) return;
+  D.oc.assign(45 * 45, 0.0);
+  auto set = [&](int a, int b, double v) { D.oc[a * 45 + b] = v; D.oc[b * 45 + a] = v; };
+  set(0, 0, e.g_ss);
+  const int pd[3] = {2, 5, 9};  // packed (px px), (py py), (pz pz)
+  for (int a = 0; a < 3; ++a) {
+    set(pd[a], 0, e.g_sp);
+    set(pd[a], pd[a], e.g_pp);
+    for (int b = 0; b < a; ++b) set(pd[a], pd[b], e.g_p2);
+  }
+  for (int p : {1, 3, 6}) set(p, p, e.h_sp);                   // (s p | s p)
+  for (int p : {4, 7, 8}) set(p, p, 0.5 * (e.g_pp - e.g_p2));  // (p p' | p p')
+  for (size_t i = 0; i < sizeof(ONE_CENTRE_D) / sizeof(short); i += 3)
+    set(ONE_CENTRE_D[i], ONE_CENTRE_D[i + 1], rp[ONE_CENTRE_D[i + 2]]);
+}
+// ---------------------------------------------------------------------------------------------
+// MNDO/d two-centre integrals (src/mndod_twocenter.rs). Every pair of a molecule holding a
+// d-bearing atom goes through here (src/hamiltonian.rs:347-354, 424-428). Local frame: first
+// p orbital along the bond (sigma), then the two pi; d orbitals d1 = d-sigma, d2 d3 = d-pi (with
+// the first and second pi), d4 d5 = d-delta. Local and molecular charge distributions are both
+// packed h(h+1)/2 + l over (s, p1, p2, p3, d1..d5).
+// ---------------------------------------------------------------------------------------------
+// Interaction of two point-charge multipole configurations in Hartree (src/mndod.rs:30-186):
+// order l1 on the first atom, l2 on the second, azimuthal |m|, separations da db, additive add.
+double multipole_pair(double r, int l1, int l2, int m, double da, double db, double add) {
+  // q(x, s2) = 1 / sqrt(x^2 + (s2 + add))
+  auto q = [add](double x, double s2) { return 1.0 / std::sqrt(x * x + (s2 + add)); };
+  const double dm2 = (da - db) * (da - db), dp2 = (da + db) * (da + db);
+  if (l1 == 0 && l2 == 0) return q(r, 0.0);
+  if (l1 == 1 && l2 == 0) return (q(r - da, 0.0) - q(r + da, 0.0)) * 0.5;
+  if (l1 == 0 && l2 == 1) return (q(r + db, 0.0) - q(r - db, 0.0)) * 0.5;
+  if (l1 == 1 && l2 == 1) {
+    if (m == 0) return (q(r + da - db, 0.0) + q(r - da + db, 0.0) - q(r - da - db, 0.0) - q(r + da + db, 0.0)) * 0.25;
+    return (q(r, dm2) - q(r, dp2)) * 0.5;
+  }
+  if (l1 == 0 && l2 == 2) return (q(r - db, 0.0) - q(r, db * db) * 2.0 + q(r + db, 0.0)) * 0.25;
+  if (l1 == 2 && l2 == 0) return (q(r - da, 0.0) - q(r, da * da) * 2.0 + q(r + da, 0.0)) * 0.25;
+  if (l1 == 1 && l2 == 2) {
+    if (m == 0)
+      return (q(r - da - db, 0.0) - q(r - da, db * db) * 2.0 + q(r + db - da, 0.0) - q(r - db + da, 0.0) +
+              q(r + da, db * db) * 2.0 - q(r + da + db, 0.0)) * 0.125;
+    const double ab = db / std::sqrt(2.0), u = (da - ab) * (da - ab), v = (da + ab) * (da + ab);
+    return (q(r - ab, u) * -2.0 + q(r + ab, u) * 2.0 + q(r - ab, v) * 2.0 - q(r + ab, v) * 2.0) * 0.125;
+  }
+  if (l1 == 2 && l2 == 1) {
+    if (m == 0)
+      return (-q(r - da - db, 0.0) + q(r - db, da * da) * 2.0 - q(r + da - db, 0.0) + q(r - da + db, 0.0) -
+              q(r + db, da * da) * 2.0 + q(r + da + db, 0.0)) * 0.125;
+    const double aa = da / std::sqrt(2.0), u = (aa - db) * (aa - db), v = (aa + db) * (aa + db);
+    return (q(r + aa, u) * -2.0 + q(r - aa, u) * 2.0 + q(r + aa, v) * 2.0 - q(r - aa, v) * 2.0) * 0.125;
+  }
+  // l1 == l2 == 2
+  const double xyxy = q(r, dm2) * 4.0 + q(r, dp2) * 4.0 - q(r, da * da + db * db) * 8.0;
+  if (m == 0) {
+    const double zzzz = q(r - da - db, 0.0) + q(r + da + db, 0.0) + q(r - da + db, 0.0) + q(r + da - db, 0.0) -
+                        q(r - da, db * db) * 2.0 - q(r - db, da * da) * 2.0 - q(r + da, db * db) * 2.0 -
+                        q(r + db, da * da) * 2.0 + q(r, dm2) * 2.0 + q(r, dp2) * 2.0;
+    return zzzz * (1.0 / 16.0) - xyxy * (1.0 / 64.0);
+  }
+  if (m == 1) {
+    const double aa = da / std::sqrt(2.0), ab = db / std::sqrt(2.0);
+    const double u = (aa - ab) * (aa - ab), v = (aa + ab) * (aa + ab);
+    return (q(r + aa - ab, u) * 2.0 - q(r + aa + ab, u) * 2.0 - q(r - aa - ab, u) * 2.0 + q(r - aa + ab, u) * 2.0 -
+            q(r + aa - ab, v) * 2.0 + q(r + aa + ab, v) * 2.0 + q(r - aa - ab, v) * 2.0 - q(r - aa + ab, v) * 2.0) *
+           (1.0 / 16.0);
+  }
+  return xyxy * (1.0 / 16.0);
+}
+
+// Multipole content of each local charge distribution (a >= b): coefficient c of the (l, m)
+// multipole, truncated at l = 2 as in MNDO/d (src/mndod_tables.rs CH; the decimals are MOPAC's).
+// Local orbitals 0 s, 1 p-sigma, 2 p-pi, 3 p-pi', 4 d-sigma, 5 d-pi, 6 d-pi', 7 d-delta, 8 d-delta'.
+struct MultipoleTerm {
+  signed char a, b, l, m;
+  double c;
+};
+const MultipoleTerm MULTIPOLES[] = {
+    {0, 0, 0, 0, 1.0},         {1, 0, 1, 0, 1.0},        {2, 0, 1, 1, 1.0},         {3, 0, 1, -1, 1.0},
+    {4, 0, 2, 0, 1.1547005},   {5, 0, 2, 1, 1.0},        {6, 0, 2, -1, 1.0},        {7, 0, 2, 2, 1.0},
+    {8, 0, 2, -2, 1.0},        {1, 1, 0, 0, 1.0},        {1, 1, 2, 0, 1.3333333},   {2, 1, 2, 1, 1.0},
+    {3, 1, 2, -1, 1.0},        {4, 1, 1, 0, 1.1547005},  {5, 1, 1, 1, 1.0},         {6, 1, 1, -1, 1.0},
+    {2, 2, 0, 0, 1.0},         {2, 2, 2, 0, -0.66666667}, {2, 2, 2, 2, 1.0},        {3, 2, 2, -2, 1.0},
+    {4, 2, 1, 1, -0.57735027}, {5, 2, 1, 0, 1.0},        {7, 2, 1, 1, 1.0},         {8, 2, 1, -1, 1.0},
+    {3, 3, 0, 0, 1.0},         {3, 3, 2, 0, -0.66666667}, {3, 3, 2, 2, -1.0},       {4, 3, 1, -1, -0.57735027},
+    {6, 3, 1, 0, 1.0},         {7, 3, 1, -1, -1.0},      {8, 3, 1, 1, 1.0},         {4, 4, 0, 0, 1.0},
+    {4, 4, 2, 0, 1.3333333},   {5, 4, 2, 1, 0.57735027}, {6, 4, 2, -1, 0.57735027}, {7, 4, 2, 2, -1.1547005},
+    {8, 4, 2, -2, -1.1547005}, {5, 5, 0, 0, 1.0},        {5, 5, 2, 0, 0.66666667},  {5, 5, 2, 2, 1.0},
+    {6, 5, 2, -2, 1.0},        {7, 5, 2, 1, 1.0},        {8, 5, 2, -1, 1.0},        {6, 6, 0, 0, 1.0},
+    {6, 6, 2, 0, 0.66666667},  {6, 6, 2, 2, -1.0},       {7, 6, 2, -1, -1.0},       {8, 6, 2, 1, 1.0},
+    {7, 7, 0, 0, 1.0},         {7, 7, 2, 0, -1.3333333}, {8, 8, 0, 0, 1.0},         {8, 8, 2, 0, -1.3333333}};
+
+struct MultipoleTable {
+  double c[45][3][5] = {{{0}}};  // [packed pair][l][m + 2]
+  MultipoleTable() {
+    for (const MultipoleTerm& t : MULTIPOLES) c[pack(t.a, t.b)][t.l][t.m + 2] = t.c;
+  }
+};
+const MultipoleTable& multipoles() {
+  static const MultipoleTable t;
+  return t;
+}
+
+inline int ang_l(int orb) { return orb == 0 ? 0 : (orb <= 3 ? 1 : 2); }
+inline int pair_kind(int la, int lb) {  // 1 ss, 2 sp, 3 pp, 4 sd, 5 pd, 6 dd
+  int h = std::max(la, lb) + 1, l = std::min(la, lb) + 1;
+  return h * (h - 1) / 2 + l;
+}
+
+// One local two-centre integral (ab | cd) in Hartree by the multipole expansion
+// (src/mndod_twocenter.rs:582-654). core_side 1 (2): the monopole of an (ss) distribution on the
+// first (second) atom takes the core additive term po(9).
+double local_multipole(const DShell& A, const DShell& B, int a, int b, int c, int d, int core_side, double r) {
+  const MultipoleTable& T = multipoles();
+  const int la = ang_l(a), lb = ang_l(b), lc = ang_l(c), ld = ang_l(d);
+  const int kij = pair_kind(la, lb), kkl = pair_kind(lc, ld);
+  const int pij = pack(a, b), pkl = pack(c, d);
+  auto mono = [](const DShell& D, int kind, bool core) {
+    if (kind == 1) return core ? D.po[9] : D.po[1];
+    if (kind == 3) return D.po[7];
+    if (kind == 6) return D.po[8];
+    return D.po[1];
+  };
+  double sum = 0.0;
+  for (int l1 = std::min(std::abs(la - lb), 2); l1 <= std::min(la + lb, 2); ++l1) {
+    const double dij = l1 == 0 ? 0.0 : A.ddp[kij];
+    const double aij = l1 == 0 ? mono(A, kij, core_side == 1) : A.po[kij];
+    for (int l2 = std::min(std::abs(lc - ld), 2); l2 <= std::min(lc + ld, 2); ++l2) {
+      const double dkl = l2 == 0 ? 0.0 : B.ddp[kkl];
+      const double akl = l2 == 0 ? mono(B, kkl, core_side == 2) : B.po[kkl];
+      const double add = (aij + akl) * (aij + akl);
+      const int mm = std::min(l1, l2);
+      for (int m = -mm; m <= mm; ++m) {
+        const double ccc = T.c[pij][l1][m + 2] * T.c[pkl][l2][m + 2];
+        if (ccc == 0.0) continue;
+        sum += multipole_pair(r, l1, l2, std::abs(m), dij, dkl, add) * ccc;
+      }
+    }
+  }
+  return sum;
+}
+
+// Local (ab | cd) over the s/p distributions (packed 0..9) as an index into the 22 sp integrals
+// ri[1..22] (0: zero by symmetry); the standard MNDO layout (src/mndod_tables.rs IND2 with
+// src/mndod_twocenter.rs:31-37 IPOS).
+const signed char SP_LOCAL[10][10] = {
+    {1, 5, 11, 0, 0, 12, 0, 0, 0, 12},   {2, 6, 13, 0, 0, 14, 0, 0, 0, 14},
+    {3, 8, 16, 0, 0, 18, 0, 0, 0, 18},   {0, 0, 0, 7, 15, 0, 0, 0, 0, 0},
+    {0, 0, 0, 10, 20, 0, 0, 0, 0, 0},    {4, 9, 17, 0, 0, 19, 0, 0, 0, 21},
+    {0, 0, 0, 0, 0, 0, 7, 15, 0, 0},     {0, 0, 0, 0, 0, 0, 10, 20, 0, 0},
+    {0, 0, 0, 0, 0, 0, 0, 0, 22, 0},     {4, 9, 17, 0, 0, 21, 0, 0, 0, 19}};
+
+// Sign of each sp integral in the MNDO/d local frame relative to the sp kernel's ri
+// (src/mndod_twocenter.rs:39-43 NRI).
+const double SP_SIGN[23] = {0, 1, -1, 1, 1, -1, 1, 1, -1, -1, -1, 1, 1, -1, -1, -1, 1, 1, 1, 1, 1, 1, 1};
+
+// The 22 sp local integrals of the MNDO/d path (src/mndod_twocenter.rs:244-470): the sp kernel's
+// formulas with the po(1..3) additive terms, feathered, in the MNDO/d sign convention (1-based).
+void local_sp_ri(const Element& ei, const Element& ej, double r, double ri[23]) {
+  Element a = ei, b = ej;
+  a.rho0 = ei.ds.po[1], a.rho1 = ei.ds.po[2], a.rho2 = ei.ds.po[3];
+  b.rho0 = ej.ds.po[1], b.rho1 = ej.ds.po[2], b.rho2 = ej.ds.po[3];
+  for (int k = 0; k < 23; ++k) ri[k] = 0.0;
+  const bool si = ei.has_p(), sj = ej.has_p();
+  if (si && sj) {
+    double t[22];
+    local_xx(a, b, r, t);
+    for (int k = 0; k < 22; ++k) ri[k + 1] = t[k];
+  } else if (si) {
+    double t[4];
+    local_xh(a, b, r, t);
+    for (int k = 0; k < 4; ++k) ri[k + 1] = t[k];
+  } else if (sj) {
+    double t[4];
+    local_xh(b, a, r, t);  // the mirrored pair: (ss|ss) (ss|ps) (ss|pp) (ss|pi pi)
+    ri[1] = t[0];
+    ri[5] = -t[1];
+    ri[11] = t[2];
+    ri[12] = t[3];
+  } else {
+    double aee = (a.rho0 + b.rho0) * (a.rho0 + b.rho0);
+    double cfrac, point;
+    feather_to_point(r, cfrac, point);
+    ri[1] = EV / std::sqrt(r * r + aee) * cfrac + point * (1.0 - cfrac);
+  }
+  for (int k = 1; k <= 22; ++k) ri[k] *= SP_SIGN[k];
+}
+
+// Orbital rotation R[local][molecular] for the pair direction v = R_j - R_i
+// (src/mndod_twocenter.rs:73-147): the p block takes the local (sigma, pi, pi') onto x y z, the d
+// block the local (d-sigma, d-pi, d-pi', d-delta, d-delta') onto dx2-y2 dxz dz2 dyz dxy.
+double orbital_rotation(V3 v, bool with_d, double R[9][9]) {
+  const double pt5sq3 = 0.8660254037841;
+  const double b = v.x * v.x + v.y * v.y;
+  const double r = std::sqrt(b + v.z * v.z);
+  const double sqb = std::sqrt(b);
+  double sb = sqb / r, ca, sa, cb;
+  if (sb > 1.0e-7) {
+    ca = v.x / sqb;
+    sa = v.y / sqb;
+    cb = v.z / r;
+  } else {
+    sa = 0.0;
+    sb = 0.0;
+    ca = cb = v.z < 0.0 ? -1.0 : (v.z > 0.0 ? 1.0 : 0.0);
+  }
+  for (int i = 0; i < 9; ++i)
+    for (int j = 0; j < 9; ++j) R[i][j] = 0.0;
+  R[0][0] = 1.0;
+  double p[3][3] = {{ca * sb, sa * sb, cb}, {ca * cb, sa * cb, -sb}, {-sa, ca, 0.0}};
+  for (int k = 0; k < 3; ++k)
+    for (int K = 0; K < 3; ++K) R[1 + k][1 + K] = p[k][K];
+  if (with_d) {
+    const double c2a = ca * ca * 2.0 - 1.0, c2b = cb * cb * 2.0 - 1.0, s2a = sa * ca * 2.0, s2b = sb * cb * 2.0;
+    const double d[5][5] = {
+        {c2a * sb * sb * pt5sq3, ca * s2b * pt5sq3, cb * cb - sb * sb * 0.5, sa * s2b * pt5sq3, s2a * sb * sb * pt5sq3},
+        {c2a * s2b * 0.5, ca * c2b, -(s2b * pt5sq3), sa * c2b, s2a * s2b * 0.5},
+        {-(s2a * sb), -(sa * cb), 0.0, ca * cb, c2a * sb},
+        {c2a * (cb * cb + sb * sb * 0.5), ca * s2b * -0.5, sb * sb * pt5sq3, sa * s2b * -0.5, s2a * (cb * cb + sb * sb * 0.5)},
+        {-(s2a * cb), sa * sb, 0.0, -(ca * sb), c2a * cb}};
+    for (int k = 0; k < 5; ++k)
+      for (int K = 0; K < 5; ++K) R[4 + k][4 + K] = d[k][K];
+  }
+  return r;
+}
+
+// Charge-distribution transform: T[IJ][ab] takes local distribution ab (a >= b) into molecular
+// IJ (I >= J) for an atom with n orbitals (src/mndod_twocenter.rs:742-915, tx + rotate_left).
+void distribution_transform(int n, const double R[9][9], std::vector<double>& T) {
+  const int np = n * (n + 1) / 2;
+  T.assign((size_t)np * np, 0.0);
+  for (int I = 0; I < n; ++I)
+    for (int J = 0; J <= I; ++J)
+      for (int a = 0; a < n; ++a)
+        for (int b = 0; b <= a; ++b)
+          T[(size_t)pack(I, J) * np + pack(a, b)] =
+              a == b ? R[a][I] * R[a][J] : R[a][I] * R[b][J] + R[b][I] * R[a][J];
+}
+
+// PM7 d-imbalance correction (src/mndod_twocenter.rs:918-1000): the mean Coulomb integral
+// between a transition metal's d shell and the partner's shells is set to (ss|ss), and the
+// matching electron-core terms likewise. W is npa x npb, core[k][side] with k 1..10.
+void d_balance(const Element& ei, const Element& ej, std::vector<double>& W, int npb, double core[11][3]) {
+  auto w = [&](int ij, int kl) -> double& { return W[(size_t)ij * npb + kl]; };
+  const double w00 = W[0];
+  const int ni = ei.n_orb, nj = ej.n_orb;
+  const int P[3] = {pack(1, 1), pack(2, 2), pack(3, 3)};
+  int Dd[5];
+  for (int k = 0; k < 5; ++k) Dd[k] = pack(4 + k, 4 + k);
+  if (ei.iod > 0) {
+    if (nj > 1) {
+      double sum = 0.0;
+      for (int a : Dd) sum += w(a, P[0]) + w(a, P[1]) + w(a, P[2]);
+      sum = w00 - sum * (1.0 / 15.0);
+      for (int a : Dd)
+        for (int b : P) w(a, b) += sum;
+      const double s = core[1][1] - (core[3][1] + core[4][1] * 2.0) * (1.0 / 3.0);
+      core[3][1] += s;
+      core[4][1] += s;
+    }
+    double sum = 0.0;
+    for (int a : Dd) sum += w(a, 0);
+    sum = w00 - sum * (1.0 / 5.0);
+    for (int a : Dd) w(a, 0) += sum;
+    const double s = core[1][1] - (core[7][1] + core[9][1] * 2.0 + core[10][1] * 2.0) * (1.0 / 5.0);
+    core[7][1] += s;
+    core[9][1] += s;
+    core[10][1] += s;
+  }
+  if (ej.iod > 0) {
+    if (ei.iod > 0) {
+      double sum = 0.0;
+      for (int a : Dd)
+        for (int b : Dd) sum += w(a, b);
+      sum = w00 - sum * (1.0 / 25.0);
+      for (int a : Dd)
+        for (int b : Dd) w(a, b) += sum;
+    }
+    if (ni > 1) {
+      double sum = 0.0;
+      for (int a : P)
+        for (int b : Dd) sum += w(a, b);
+      sum = w00 - sum * (1.0 / 15.0);
+      for (int a : P)
+        for (int b : Dd) w(a, b) += sum;
+      const double s = core[1][2] - (core[3][2] + core[4][2] * 2.0) * (1.0 / 3.0);
+      core[3][2] += s;
+      core[4][2] += s;
+    }
+    double sum = 0.0;
+    for (int b : Dd) sum += w(0, b);
+    sum = w00 - sum * (1.0 / 5.0);
+    for (int b : Dd) w(0, b) += sum;
+    const double s = core[1][2] - (core[7][2] + core[9][2] * 2.0 + core[10][2] * 2.0) * (1.0 / 5.0);
+    core[7][2] += s;
+    core[9][2] += s;
+    core[10][2] += s;
+  }
+}
+
+// Electron-core attraction block of one atom in the molecular frame from the ten local terms
+// core[k][side] (src/mndod_twocenter.rs:1003-1045): E = R^T E_local R.
+void electron_core_block(const double core[11][3], int side, int n, const double R[9][9], double E[9][9]) {
+  double L[9][9] = {{0}};
+  auto c = [&](int k) { return core[k][side]; };
+  L[0][0] = c(1);
+  L[0][1] = L[1][0] = c(2);
+  L[1][1] = c(3);
+  L[2][2] = L[3][3] = c(4);
+  L[0][4] = L[4][0] = c(5);
+  L[1][4] = L[4][1] = c(6);
+  L[4][4] = c(7);
+  L[2][5] = L[5][2] = c(8);
+  L[3][6] = L[6][3] = c(8);
+  L[5][5] = L[6][6] = c(9);
+  L[7][7] = L[8][8] = c(10);
+  for (int K = 0; K < 9; ++K)
+    for (int M = 0; M < 9; ++M) E[K][M] = 0.0;
+  for (int K = 0; K < n; ++K)
+    for (int M = 0; M < n; ++M) {
+      double acc = 0.0;
+      for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b) acc += R[a][K] * L[a][b] * R[b][M];
+      E[K][M] = acc;
+    }
+}
+
+// Local electron-core terms core[1..4][side] of the s/p shells (src/mndod_twocenter.rs:482-572).
+void local_core_sp(const Element& ei, const Element& ej, double r, double core[11][3]) {
+  const DShell &A = ei.ds, &B = ej.ds;
+  const double r2 = r * r, aci = A.po[9], acj = B.po[9];
+  for (int k = 0; k < 11; ++k) core[k][0] = core[k][1] = core[k][2] = 0.0;
+  core[1][1] = -(1.0 / std::sqrt(r2 + (acj + A.po[1]) * (acj + A.po[1]))) * (ej.core_charge * EV);
+  core[1][2] = -(1.0 / std::sqrt(r2 + (aci + B.po[1]) * (aci + B.po[1]))) * (ei.core_charge * EV);
+  // terms of a p shell on atom X in the field of the other core (additive ac)
+  auto shell = [&](const DShell& X, double ac, double& t2, double& t3, double& t4) {
+    const double pp = (ac + X.po[7]) * (ac + X.po[7]);
+    const double da = X.ddp[2], qa = X.ddp[3] / std::sqrt(2.0), q2 = qa + qa;
+    const double ad = (X.po[2] + ac) * (X.po[2] + ac), aq = (X.po[3] + ac) * (X.po[3] + ac);
+    const double x1 = 1.0 / std::sqrt(r2 + pp);
+    const double x2 = 1.0 / std::sqrt(r2 + aq) * -0.5;
+    const double x3 = 1.0 / std::sqrt((r + da) * (r + da) + ad) * -0.5;
+    const double x4 = 1.0 / std::sqrt((r - da) * (r - da) + ad) * 0.5;
+    const double x5 = 1.0 / std::sqrt((r - q2) * (r - q2) + aq) * 0.25;
+    const double x6 = 1.0 / std::sqrt((r + q2) * (r + q2) + aq) * 0.25;
+    const double x7 = 1.0 / std::sqrt(r2 + (q2 * q2 + aq)) * 0.5;
+    t2 = (x3 + x4) * EV;
+    t3 = (x1 + x2 + x5 + x6) * EV;
+    t4 = (x1 + x2 + x7) * EV;
+  };
+  if (ei.has_p()) {
+    double t2, t3, t4;
+    shell(A, acj, t2, t3, t4);
+    core[2][1] = -t2 * ej.core_charge;
+    core[3][1] = -t3 * ej.core_charge;
+    core[4][1] = -t4 * ej.core_charge;
+  }
+  if (ej.has_p()) {
+    double t2, t3, t4;
+    shell(B, aci, t2, t3, t4);
+    core[2][2] = t2 * ei.core_charge;  // the dipole term changes sign seen from the second atom
+    core[3][2] = -t3 * ei.core_charge;
+    core[4][2] = -t4 * ei.core_charge;
+  }
+}
+
+// Two-centre two-electron and electron-core integrals of the ordered pair (ei first), d = R_j -
+// R_i (src/mndod_twocenter.rs:1050-1112). Fills out.w (npa x npb), e1b and e2a.
+void pair_two_electron_d(const Element& ei, const Element& ej, V3 d, PairBlock& out) {
+  const DShell &A = ei.ds, &B = ej.ds;
+  const int ni = ei.n_orb, nj = ej.n_orb;
+  const bool with_d = ni == 9 || nj == 9;
+  double R[9][9];
+  const double r = orbital_rotation(d, with_d, R);
+  double ri[23];
+  local_sp_ri(ei, ej, r, ri);
+  double core[11][3];
+  local_core_sp(ei, ej, r, core);
+  const int npa = ni * (ni + 1) / 2, npb = nj * (nj + 1) / 2;
+  // Local integrals: s/p distributions from ri, the rest by the multipole expansion, feathered
+  // (src/mndod_twocenter.rs:657-708).
+  double cfrac, point;
+  feather_to_point(r, cfrac, point);
+  const double point_add = point * (1.0 - cfrac);
+  std::vector<double> L((size_t)npa * npb, 0.0);
+  for (int a = 0; a < ni; ++a)
+    for (int b = 0; b <= a; ++b)
+      for (int c = 0; c < nj; ++c)
+        for (int e = 0; e <= c; ++e) {
+          const int ij = pack(a, b), kl = pack(c, e);
+          double v;
+          if (a < 4 && c < 4) {
+            const int k = SP_LOCAL[ij][kl];
+            v = k ? ri[k] : 0.0;
+          } else {
+            v = local_multipole(A, B, a, b, c, e, 0, r) * EV * cfrac;
+            if (a == b && c == e) v += point_add;
+          }
+          L[(size_t)ij * npb + kl] = v;
+        }
+  if (with_d) {  // d electron-core terms (src/mndod_twocenter.rs:709-734)
+    // distributions (d-sigma s) (d-sigma p-sigma) (d-sigma d-sigma) (d-pi p-pi) (d-pi d-pi) (d-delta d-delta)
+    const int da[6] = {4, 4, 4, 5, 5, 7}, db[6] = {0, 1, 4, 2, 5, 7};
+    for (int k = 0; k < 6; ++k) {
+      if (nj == 9) core[5 + k][2] = -local_multipole(A, B, 0, 0, da[k], db[k], 1, r) * (EV * ei.core_charge);
+      if (ni == 9) core[5 + k][1] = -local_multipole(A, B, da[k], db[k], 0, 0, 2, r) * (EV * ej.core_charge);
+    }
+  }
+  // Feathering of the electron-core terms toward the point charge (src/mndod_twocenter.rs:1064-1081).
+  {
+    const double p1 = (1.0 / r) * (-EV * ej.core_charge), p2 = (1.0 / r) * (-EV * ei.core_charge);
+    for (int k = 1; k <= 10; ++k) {
+      core[k][1] *= cfrac;
+      core[k][2] *= cfrac;
+    }
+    for (int k : {1, 3, 4, 7, 9, 10}) {
+      core[k][1] += p1 * (1.0 - cfrac);
+      core[k][2] += p2 * (1.0 - cfrac);
+    }
+  }
+  // Rotate both distribution indices into the molecular frame: W = Ti L Tj^T.
+  std::vector<double> Ti, Tj;
+  distribution_transform(ni, R, Ti);
+  distribution_transform(nj, R, Tj);
+  std::vector<double> LT((size_t)npa * npb, 0.0);
+  for (int ij = 0; ij < npa; ++ij)
+    for (int KL = 0; KL < npb; ++KL) {
+      double acc = 0.0;
+      for (int kl = 0; kl < npb; ++kl) acc += L[(size_t)ij * npb + kl] * Tj[(size_t)KL * npb + kl];
+      LT[(size_t)ij * npb + KL] = acc;
+    }
+  out.npa = npa;
+  out.npb = npb;
+  out.w.assign((size_t)npa * npb, 0.0);
+  for (int IJ = 0; IJ < npa; ++IJ)
+    for (int KL = 0; KL < npb; ++KL) {
+      double acc = 0.0;
+      for (int ij = 0; ij < npa; ++ij) acc += Ti[(size_t)IJ * npa + ij] * LT[(size_t)ij * npb + KL];
+      out.w[(size_t)IJ * npb + KL] = acc;
+    }
+  d_balance(ei, ej, out.w, npb, core);
+  electron_core_block(core, 1, ni, R, out.e1b);
+  electron_core_block(core, 2, nj, R, out.e2a);
+}
+
+// ---- Overlaps of the spd basis (src/overlap_d.rs) ----
+double fact_d(int n) {
+  static double f[25];
+  static bool init = false;
+  if (!init) {
+    f[0] = 1.0;
+    for (int i = 1; i < 25; ++i) f[i] = f[i - 1] * i;
+    init = true;
+  }
+  return f[n];
+}
+
+// B integrals bf[0..12] of argument x (src/overlap_d.rs:39-82).
+void b_integrals(double x, double bf[13]) {
+  const double ax = std::fabs(x);
+  if (ax > 3.0) {
+    const double ex = std::exp(x), emx = 1.0 / ex, inv = 1.0 / x;
+    bf[0] = (ex - emx) * inv;
+    for (int i = 1; i <= 12; ++i) bf[i] = (bf[i - 1] * i + ex * (i % 2 == 0 ? 1.0 : -1.0) - emx) * inv;
+  } else if (ax <= 1.0e-6) {
+    for (int i = 0; i <= 12; ++i) bf[i] = (2 * ((i + 1) % 2)) / (i + 1.0);
+  } else {
+    const int last = ax > 2.0 ? 15 : (ax > 1.0 ? 12 : (ax > 0.5 ? 7 : 6));
+    for (int i = 0; i <= 12; ++i) {
+      double y = 0.0;
+      for (int m = 0; m <= last; ++m) {
+        const double xf = m != 0 ? fact_d(m) : 1.0;
+        const double coeff = (2 * ((m + i + 1) % 2)) / (xf * (m + i + 1));
+        double xm = 1.0;
+        for (int t = 0; t < m; ++t) xm *= -x;
+        y += xm * coeff;
+      }
+      bf[i] = y;
+    }
+  }
+}
+
+// Overlap of Slater shells (na, la) on a and (nb, lb) on b with azimuthal component m, exponents
+// ua ub, distance r, in the local frame (src/overlap_d.rs:84-176); l and m are 0-based here.
+double shell_overlap(int na, int nb, int la, int lb, int m, double ua, double ub, double r) {
+  auto aff = [](int l, int mm, int i) {
+    if (l == 0 && mm == 0 && i == 0) return 1.0;
+    if (l == 1 && mm == 0 && i == 0) return 1.0;
+    if (l == 1 && mm == 1 && i == 0) return std::sqrt(0.5);
+    if (l == 2 && mm == 0 && i == 0) return 1.5;
+    if (l == 2 && mm == 1 && i == 0) return std::sqrt(1.5);
+    if (l == 2 && mm == 2 && i == 0) return std::sqrt(0.375);
+    if (l == 2 && mm == 0 && i == 2) return -0.5;
+    return 0.0;
+  };
+  auto bi = [](int i, int j) { return (j < 0 || j > i) ? 0.0 : fact_d(i) / (fact_d(j) * fact_d(i - j)); };
+  const double p = r * ((ua + ub) * 0.5), bb = r * ((ua - ub) * 0.5);
+  double af[20];
+  const double quo = 1.0 / p;
+  af[0] = quo * std::exp(-p);
+  for (int n = 1; n < 20; ++n) af[n] = af[n - 1] * n * quo + af[0];
+  double bf[13];
+  b_integrals(bb, bf);
+  double sum = 0.0;
+  for (int i = 0; i <= la - m; i += 2) {
+    const int ia = na + i - la, ic = la - i - m;
+    for (int j = 0; j <= lb - m; j += 2) {
+      const int ib = nb + j - lb, id = lb - j - m;
+      double sum1 = 0.0;
+      const int iab = ia + ib;
+      for (int k1 = 0; k1 <= ia; ++k1)
+        for (int k2 = 0; k2 <= ib; ++k2)
+          for (int k3 = 0; k3 <= ic; ++k3)
+            for (int k4 = 0; k4 <= id; ++k4)
+              for (int k5 = 0; k5 <= m; ++k5) {
+                const int iaf = iab - k1 - k2 + k3 + k4 + 2 * k5;
+                for (int k6 = 0; k6 <= m; ++k6) {
+                  const int ibf = k1 + k2 + k3 + k4 + 2 * k6;
+                  const double sign = 1 - 2 * ((m + k2 + k4 + k5 + k6) % 2);
+                  const double coeff = bi(id, k4) * bi(ic, k3) * bi(ib, k2) * bi(ia, k1) * bi(m, k5) * bi(m, k6) * sign;
+                  sum1 += af[iaf] * bf[ibf] * coeff;
+                }
+              }
+      sum += sum1 * (aff(la, m, i) * aff(lb, m, j));
+    }
+  }
+  double rpow = 1.0;
+  for (int t = 0; t < na + nb + 1; ++t) rpow *= r;
+  const double norm = std::sqrt(ua * ub / (fact_d(na + na) * fact_d(nb + nb)) * ((la + la + 1) * (lb + lb + 1)));
+  return sum * rpow * (std::pow(ua, na) * std::pow(ub, nb) / 2.0 * norm);
+}
+
+// Molecular-frame overlap block S (9 x 9) of the ordered pair, d = R_j - R_i
+// (src/overlap_d.rs:178-330): local sigma/pi/delta shell overlaps combined with the
+// real-harmonic direction coefficients c(shell, component, local axis).
+void diat_overlap_d(const Element& ei, const Element& ej, V3 d, double S[9][9]) {
+  for (int i = 0; i < 9; ++i)
+    for (int j = 0; j < 9; ++j) S[i][j] = 0.0;
+  const int ni = ei.n_orb, nj = ej.n_orb;
+  if (ni == 0 || nj == 0) return;
+  const int nij = std::max(ni, nj);
+  const double xy2 = d.x * d.x + d.y * d.y;
+  const double r = std::sqrt(xy2 + d.z * d.z), xy = std::sqrt(xy2);
+  double ca, cb, sa, sb;
+  if (xy >= 1.0e-10) {
+    ca = d.x / xy;
+    cb = d.z / r;
+    sa = d.y / xy;
+    sb = xy / r;
+  } else if (d.z < 0.0) {
+    ca = cb = -1.0;
+    sa = sb = 0.0;
+  } else if (d.z == 0.0) {
+    ca = cb = sa = sb = 0.0;
+  } else {
+    ca = cb = 1.0;
+    sa = sb = 0.0;
+  }
+  // c[shell 1..3][component 1..5][local axis 1..5]; axis 3 sigma, 2 and 4 pi, 1 and 5 delta.
+  double c[4][6][6] = {{{0}}};
+  c[1][3][3] = 1.0;
+  if (nij >= 2) {
+    c[2][4][4] = ca * cb;
+    c[2][4][3] = ca * sb;
+    c[2][4][2] = -sa;
+    c[2][2][4] = sa * cb;
+    c[2][2][3] = sa * sb;
+    c[2][2][2] = ca;
+    c[2][3][4] = -sb;
+    c[2][3][3] = cb;
+    c[2][3][2] = 0.0;
+    if (nij >= 5) {
+      const double rt34 = 0.86602540378444, rt13 = 0.57735026918963;
+      const double c2a = ca * ca * 2.0 - 1.0, c2b = cb * cb * 2.0 - 1.0, s2a = sa * ca * 2.0, s2b = sb * cb * 2.0;
+      c[3][5][5] = c2a * cb * cb + c2a * sb * sb * 0.5;
+      c[3][5][4] = c2a * s2b * 0.5;
+      c[3][5][3] = c2a * sb * sb * rt34;
+      c[3][5][2] = -(s2a * sb);
+      c[3][5][1] = -(s2a * cb);
+      c[3][4][5] = ca * s2b * -0.5;
+      c[3][4][4] = ca * c2b;
+      c[3][4][3] = ca * s2b * rt34;
+      c[3][4][2] = -(sa * cb);
+      c[3][4][1] = sa * sb;
+      c[3][3][5] = sb * sb * (rt13 * 1.5);
+      c[3][3][4] = -(s2b * rt34);
+      c[3][3][3] = cb * cb - sb * sb * 0.5;
+      c[3][2][5] = sa * s2b * -0.5;
+      c[3][2][4] = sa * c2b;
+      c[3][2][3] = sa * s2b * rt34;
+      c[3][2][2] = ca * cb;
+      c[3][2][1] = -(ca * sb);
+      c[3][1][5] = s2a * cb * cb + s2a * sb * sb * 0.5;
+      c[3][1][4] = s2a * s2b * 0.5;
+      c[3][1][3] = s2a * sb * sb * rt34;
+      c[3][1][2] = c2a * sb;
+      c[3][1][1] = c2a * cb;
+    }
+  }
+  // orbital of (shell, component): p components 2 3 4 are py pz px, d components 1..5 are
+  // dxy dyz dz2 dxz dx2-y2 (0-based orbital indices).
+  const int orbital[4][6] = {{-1, -1, -1, -1, -1, -1}, {-1, 0, 0, 0, 0, -1}, {-1, -1, 2, 3, 1, -1}, {-1, 8, 7, 6, 5, 4}};
+  auto shells = [](int n) { return n == 1 ? 1 : (n == 4 ? 2 : 3); };
+  const int sa_n = shells(ni), sb_n = shells(nj);
+  const double ei_exp[3] = {ei.zeta_s, ei.zeta_p, std::max(ei.zeta_d, 0.3)};
+  const double ej_exp[3] = {ej.zeta_s, ej.zeta_p, std::max(ej.zeta_d, 0.3)};
+  double s[4][4][4] = {{{0}}};
+  const int newk = std::min(sa_n - 1, sb_n - 1);
+  for (int i = 1; i <= sa_n; ++i) {
+    const int pq1 = i == 3 ? shell_n_d(ei.z) : shell_n_sp(ei.z);
+    for (int j = 1; j <= sb_n; ++j) {
+      const int pq2 = j == 3 ? shell_n_d(ej.z) : shell_n_sp(ej.z);
+      for (int k = 1; k <= newk + 1; ++k) {
+        if (k > i || k > j) continue;
+        s[i][j][k] = shell_overlap(std::max(pq1, i), std::max(pq2, j), i - 1, j - 1, k - 1, ei_exp[i - 1],
+                                   ej_exp[j - 1], r);
+      }
+    }
+  }
+  for (int i = 1; i <= sa_n; ++i)
+    for (int j = 1; j <= sb_n; ++j) {
+      const double aa = j == 2 ? -1.0 : 1.0, bb = j == 3 ? -1.0 : 1.0;
+      for (int k = 4 - i; k <= 2 + i; ++k)
+        for (int l = 4 - j; l <= 2 + j; ++l) {
+          const int oi = orbital[i][k], oj = orbital[j][l];
+          if (oi < 0 || oj < 0) continue;
+          S[oi][oj] = s[i][j][1] * (c[i][k][3] * c[j][l][3]) * aa +
+                      s[i][j][2] * (c[i][k][4] * c[j][l][4] + c[i][k][2] * c[j][l][2]) * bb +
+                      s[i][j][3] * (c[i][k][5] * c[j][l][5] + c[i][k][1] * c[j][l][1]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Core Hamiltonian (src/hamiltonian.rs:310-638, molecular branch)
 // ---------------------------------------------------------------------------------------------
 double beta_of(const Element& e, int orb) { return orb == 0 ? e.beta_s : (orb <= 3 ? e.beta_p : e.beta_d); }
@@ -831,6 +1751,7 @@ double beta_of(const Element& e, int orb) { return orb == 0 ? e.beta_s : (orb <=
 struct Core {
   Mat h;
   std::vector<PairBlock> pairs;
+  bool d_path = false;  // every pair through the MNDO/d kernels
 };
 
 int build_mol(const Params& P, int n, const int* z, const double* xyz, Mol& m, std::string& err) {
@@ -846,10 +1767,6 @@ int build_mol(const Params& P, int n, const int* z, const double* xyz, Mol& m, s
     if (!e) {
       err = "missing element " + std::to_string(z[i]);
       return 2;
-    }
-    if (e->n_orb == 9) {
-      err = "element " + std::to_string(z[i]) + " needs the MNDO/d kernels";
-      return 3;
     }
     m.el[i] = e;
     m.off[i] = nao;
@@ -874,6 +1791,7 @@ Core build_core(const Mol& m) {
   for (int u = 0; u < n; ++u)
     for (int v = u + 1; v < n; ++v) list.push_back({u, v});
   c.pairs.resize(list.size());
+  for (int i = 0; i < n; ++i) c.d_path = c.d_path || m.el[i]->n_orb == 9;  // src/hamiltonian.rs:347-354
   parallel_for((long)list.size(), [&](long p) {
     int u = list[p].first, v = list[p].second;
     V3 d = sub(m.pos[v], m.pos[u]);
@@ -886,10 +1804,18 @@ Core build_core(const Mol& m) {
     PairBlock& pb = c.pairs[p];
     pb.a = a;
     pb.b = b;
+    if (c.d_path) {
+      pair_two_electron_d(*m.el[a], *m.el[b], d, pb);
+      diat_overlap_d(*m.el[a], *m.el[b], d, pb.s);
+      return;
+    }
     double r = norm(d);
     V3 dn = {d.x / r * r, d.y / r * r, d.z / r * r};  // pair_two_electron(ea, eb, d/r, r)
     pair_two_electron(*m.el[a], *m.el[b], dn, pb);
-    diatom_overlap(*m.el[a], *m.el[b], d, pb.s);
+    double s4[4][4];
+    diatom_overlap(*m.el[a], *m.el[b], d, s4);
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) pb.s[i][j] = s4[i][j];
   });
   for (const auto& pb : c.pairs) {
     const Element& ea = *m.el[pb.a];
@@ -937,13 +1863,17 @@ Mat build_fock_spin(const Mol& m, const Core& c, const Mat& pt, const Mat& ps) {
     int off = m.off[ia], nn = m.norb[ia];
     const Element& e = *m.el[ia];
     oc[ia].assign(nn * nn, 0.0);
+    // spd atoms read the 45 x 45 one-centre matrix (src/fock.rs:171-189)
+    auto g = [&](int a, int b, int cc, int d) {
+      return nn == 9 ? e.ds.oc[pack(a, b) * 45 + pack(cc, d)] : oc_two_electron(a, b, cc, d, e);
+    };
     for (int mu = 0; mu < nn; ++mu)
       for (int nu = 0; nu < nn; ++nu) {
         double acc = 0.0;
         for (int la = 0; la < nn; ++la)
           for (int si = 0; si < nn; ++si) {
-            acc += P(pt, off + la, off + si) * oc_two_electron(mu, nu, la, si, e);
-            acc -= P(ps, off + la, off + si) * oc_two_electron(mu, la, nu, si, e);
+            acc += P(pt, off + la, off + si) * g(mu, nu, la, si);
+            acc -= P(ps, off + la, off + si) * g(mu, la, nu, si);
           }
         oc[ia][mu * nn + nu] = acc;
       }
@@ -955,7 +1885,7 @@ Mat build_fock_spin(const Mol& m, const Core& c, const Mat& pt, const Mat& ps) {
   }
   // Two-centre blocks.
   struct TC {
-    double jaa[16], jbb[16], kab[16];
+    double jaa[81], jbb[81], kab[81];
   };
   std::vector<TC> tc(c.pairs.size());
   parallel_for((long)c.pairs.size(), [&](long p) {
@@ -1503,7 +2433,7 @@ void push_hist(std::vector<Mat>& h, const Mat& m, size_t max) {
   h.push_back(m);
 }
 
-// MOPAC diagonal guess (src/scf.rs:1050-1114), sp atoms only.
+// MOPAC diagonal guess (src/scf.rs:1050-1114).
 Mat sad_density(const Mol& m, double n_electrons) {
   int nao = m.nao;
   Mat p((size_t)nao * nao, 0.0);
@@ -1518,6 +2448,24 @@ Mat sad_density(const Mol& m, double n_electrons) {
     } else if (m.norb[i] == 4) {
       double w = tore * 0.25 - yy;
       for (int k = 0; k < 4; ++k) p[(size_t)(off + k) * nao + off + k] = w;
+    } else if (m.norb[i] == 9) {
+      const int zi = m.z[i];
+      auto at = [&](int k) -> double& { return p[(size_t)(off + k) * nao + off + k]; };
+      if (zi < 21 || (zi > 30 && zi < 39) || (zi > 48 && zi < 57)) {
+        double w = tore * 0.25 - yy;
+        for (int k = 0; k < 4; ++k) at(k) = w;
+        for (int k = 4; k < 9; ++k) at(k) = -yy;
+      } else {  // transition metal: s (<= 2), then d, then p
+        double sum = tore - 9.0 * yy;
+        at(0) = std::min(std::max(sum, 0.0), 2.0);
+        sum -= 2.0;
+        if (sum > 0.0) {
+          for (int k = 4; k < 9; ++k) at(k) = std::min(std::max(sum * 0.2, 0.0), 2.0);
+          sum -= 10.0;
+          if (sum > 0.0)
+            for (int k = 1; k < 4; ++k) at(k) = sum / 3.0;
+        }
+      }
     }
   }
   return p;
@@ -2448,7 +3396,7 @@ int oracle_run(const char* data_dir, int n_atoms, const int* z, const double* xy
     }
     if (tot > 0) origin = {origin.x / tot, origin.y / tot, origin.z / tot};
   }
-  double pc[3] = {0, 0, 0}, sp[3] = {0, 0, 0};
+  double pc[3] = {0, 0, 0}, sp[3] = {0, 0, 0}, pd[3] = {0, 0, 0};
   for (int i = 0; i < n_atoms; ++i) {
     pc[0] += (m.pos[i].x - origin.x) * q[i];
     pc[1] += (m.pos[i].y - origin.y) * q[i];
@@ -2458,12 +3406,21 @@ int oracle_run(const char* data_dir, int n_atoms, const int* z, const double* xy
       double hyb = -2.0 * m.el[i]->dd;
       for (int k = 0; k < 3; ++k) sp[k] += hyb * s.density[(size_t)off * nao + off + 1 + k];
     }
+    if (m.norb[i] == 9) {  // p-d hybrid term (src/dipole.rs:332-340)
+      int off = m.off[i];
+      auto P = [&](int a, int b) { return s.density[(size_t)(off + a) * nao + off + b]; };
+      const double inv3 = 1.0 / std::sqrt(3.0);
+      double hyb = -2.0 * m.el[i]->ds.ddp[5];
+      pd[0] += hyb * (P(5, 3) + P(4, 1) + P(8, 2) - inv3 * P(6, 1));
+      pd[1] += hyb * (P(7, 3) - P(4, 2) + P(8, 1) - inv3 * P(6, 2));
+      pd[2] += hyb * (P(5, 1) + P(7, 2) + 2.0 * inv3 * P(6, 3));
+    }
   }
   for (int k = 0; k < 3; ++k) {
     out->dipole_point[k] = pc[k] * AU_DIPOLE_TO_DEBYE;
     out->dipole_sp[k] = sp[k] * AU_DIPOLE_TO_DEBYE;
-    out->dipole_pd[k] = 0.0;
-    out->dipole[k] = out->dipole_point[k] + out->dipole_sp[k];
+    out->dipole_pd[k] = pd[k] * AU_DIPOLE_TO_DEBYE;
+    out->dipole[k] = out->dipole_point[k] + out->dipole_sp[k] + out->dipole_pd[k];
   }
   out->homo_ev = s.n_occ >= 1 ? s.eps[s.n_occ - 1] : NAN;
   out->lumo_ev = s.n_occ < nao ? s.eps[s.n_occ] : NAN;

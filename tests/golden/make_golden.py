@@ -139,6 +139,41 @@ def dumbbell(a, b, bond, cap_a, cap_b, twist=60.0):
     return out
 
 
+def square_planar(c, lig, r):
+    return [(c, 0.0, 0.0, 0.0)] + [(lig, r * x, r * y, 0.0) for x, y in ((1, 0), (-1, 0), (0, 1), (0, -1))]
+
+
+def carbonyl(metal, m_c, c_o, shape):
+    """Homoleptic M(CO)n with every CO radial: metal first, then C, O per direction."""
+    dirs = {
+        "tetrahedral": [tuple(c / math.sqrt(3.0) for c in v) for v in TET_DIRS],
+        "octahedral": [tuple(float(c) for c in v) for v in OCT_DIRS],
+        "bipyramidal": [(1.0, 0.0, 0.0), (-0.5, 0.866025403784, 0.0), (-0.5, -0.866025403784, 0.0),
+                        (0.0, 0.0, 1.0), (0.0, 0.0, -1.0)],
+    }[shape]
+    out = [(metal, 0.0, 0.0, 0.0)]
+    for u in dirs:
+        out.append(("C", m_c * u[0], m_c * u[1], m_c * u[2]))
+        reach = m_c + c_o
+        out.append(("O", reach * u[0], reach * u[1], reach * u[2]))
+    return out
+
+
+def metallocene(metal, height, ring_radius, c_h, tilt=0.0):
+    """Two C5H5 rings on a metal: eclipsed-staggered sandwich, or bent by moving the ring
+    centroids to y = +-tilt (both at z = height)."""
+    out = [(metal, 0.0, 0.0, 0.0)]
+    for sign in (1.0, -1.0):
+        cz = sign * height if tilt == 0.0 else height
+        cy = 0.0 if tilt == 0.0 else sign * tilt
+        for k in range(5):
+            t = 2.0 * math.pi * k / 5.0 + (0.0 if sign > 0 else math.pi / 5.0)
+            out.append(("C", ring_radius * math.cos(t), ring_radius * math.sin(t) + cy, cz))
+            reach = ring_radius + c_h
+            out.append(("H", reach * math.cos(t), reach * math.sin(t) + cy, cz))
+    return out
+
+
 def ring_atoms(els, radius, hs, h_radius, height=0.0):
     n = len(els)
     out = [(el, radius * math.cos(2 * math.pi * k / n), radius * math.sin(2 * math.pi * k / n), height)
@@ -349,6 +384,110 @@ def cases() -> dict[str, tuple[list[Atom], int, int]]:
         "dimethylzinc": ([("Zn", 0.0, 0.0, 0.0), ("C", 1.93, 0.0, 0.0), ("C", -1.93, 0.0, 0.0)]
                          + cap((1.93, 0.0, 0.0), (0.0, 0.0, 0.0), "H", 1.09, 3)
                          + cap((-1.93, 0.0, 0.0), (0.0, 0.0, 0.0), "H", 1.09, 3, twist=60.0), 0, 1),
+        # MNDO/d: every molecule below holds an element with a d shell (Al..Cl, heavier p block,
+        # transition metals), so all of its pairs take the MNDO/d kernels.
+        "hydrogen_sulfide": (xyz(["S 0.0 0.0 0.0", "H 1.34 0.0 0.0", "H -0.35 1.29 0.0"]), 0, 1),
+        "sulfur_dioxide": (xyz(["S 0.0 0.0 0.0", "O 1.24 0.0 0.72", "O -1.24 0.0 0.72"]), 0, 1),
+        "phosphorus_trifluoride": (xyz(["P 0.0 0.0 0.0", "F 1.24 0.0 0.51", "F -0.62 1.07 0.51",
+                                        "F -0.62 -1.07 0.51"]), 0, 1),
+        "silane": (tetrahedral("Si", "H", 1.48), 0, 1),
+        "phosphine": (pyramidal("P", "H", 1.42, 93.5), 0, 1),
+        "aluminium_trichloride": (trigonal_planar("Al", "Cl", 2.06), 0, 1),
+        "silicon_tetrafluoride": (tetrahedral("Si", "F", 1.55), 0, 1),
+        "carbon_tetrachloride": (tetrahedral("C", "Cl", 1.77), 0, 1),
+        "sulfur_hexafluoride": (octahedral("S", "F", 1.56), 0, 1),
+        "carbon_disulfide": (linear(["S", "C", "S"], [1.55, 1.55]), 0, 1),
+        "carbonyl_sulfide": (linear(["O", "C", "S"], [1.16, 1.56]), 0, 1),
+        "thioformaldehyde": (xyz(["C 0.0 0.0 0.0", "S 0.0 0.0 1.61", "H 0.93 0.0 -0.56", "H -0.93 0.0 -0.56"]), 0, 1),
+        "methanethiol": (dumbbell("C", "S", 1.82, ("H", 1.09, 3), ("H", 1.34, 1)), 0, 1),
+        "silanol": (dumbbell("Si", "O", 1.63, ("H", 1.48, 3), ("H", 0.96, 1)), 0, 1),
+        "hydrogen_chloride": (diatomic("H", "Cl", 1.27), 0, 1),
+        "hydrogen_bromide": (diatomic("H", "Br", 1.41), 0, 1),
+        "hydrogen_iodide": (diatomic("H", "I", 1.61), 0, 1),
+        "chlorine": (diatomic("Cl", "Cl", 1.99), 0, 1),
+        "bromine": (diatomic("Br", "Br", 2.28), 0, 1),
+        "iodine": (diatomic("I", "I", 2.67), 0, 1),
+        "chloromethane": (substituted_methane([("H", 1.09), ("H", 1.09), ("H", 1.09), ("Cl", 1.78)]), 0, 1),
+        "bromomethane": (substituted_methane([("H", 1.09), ("H", 1.09), ("H", 1.09), ("Br", 1.93)]), 0, 1),
+        "iodomethane": (substituted_methane([("H", 1.09), ("H", 1.09), ("H", 1.09), ("I", 2.14)]), 0, 1),
+        "chloroform": (substituted_methane([("H", 1.09), ("Cl", 1.77), ("Cl", 1.77), ("Cl", 1.77)]), 0, 1),
+        "chlorobenzene": (substituted_benzene([("Cl", 3.17, 0.0, 0.0)]), 0, 1),
+        "sodium_chloride": (diatomic("Na", "Cl", 2.36), 0, 1),
+        "potassium_bromide": (diatomic("K", "Br", 2.82), 0, 1),
+        "magnesium_dichloride": (linear(["Cl", "Mg", "Cl"], [2.18, 2.18]), 0, 1),
+        "thiophene": (ring_atoms(["S", "C", "C", "C", "C"], 1.25, [False] + [True] * 4, 2.34), 0, 1),
+        "chloride": (xyz(["Cl 0.0 0.0 0.0"]), -1, 1),
+        "sulfate": (tetrahedral("S", "O", 1.49), -2, 1),
+        "chlorine_dioxide": (bent("Cl", "O", 1.47, 117.0), 0, 2),
+        "rubidium_chloride": (diatomic("Rb", "Cl", 2.79), 0, 1),
+        "caesium_chloride": (diatomic("Cs", "Cl", 2.91), 0, 1),
+        "strontium_dichloride": (linear(["Cl", "Sr", "Cl"], [2.63, 2.63]), 0, 1),
+        "barium_dichloride": (linear(["Cl", "Ba", "Cl"], [2.78, 2.78]), 0, 1),
+        "gallium_trichloride": (trigonal_planar("Ga", "Cl", 2.09), 0, 1),
+        "arsine": (pyramidal("As", "H", 1.51, 92.1), 0, 1),
+        "indium_trichloride": (trigonal_planar("In", "Cl", 2.29), 0, 1),
+        "stibine": (pyramidal("Sb", "H", 1.70, 91.7), 0, 1),
+        "thallium_chloride": (diatomic("Tl", "Cl", 2.48), 0, 1),
+        "lead_dichloride": (bent("Pb", "Cl", 2.44, 96.0), 0, 1),
+        "bismuth_trichloride": (pyramidal("Bi", "Cl", 2.42, 100.0), 0, 1),
+        "scandium_trifluoride": (trigonal_planar("Sc", "F", 1.85), 0, 1),
+        "titanium_tetrachloride": (tetrahedral("Ti", "Cl", 2.17), 0, 1),
+        "vanadium_oxytrichloride": ([("V", 0.0, 0.0, 0.0), ("O", 0.0, 0.0, 1.57)]
+                                    + [("Cl", 2.14 * x / math.sqrt(3.0), 2.14 * y / math.sqrt(3.0),
+                                        -2.14 * abs(z) / math.sqrt(3.0)) for x, y, z in TET_DIRS[:3]], 0, 1),
+        "chromyl_chloride": ([("Cr", 0.0, 0.0, 0.0), ("O", 1.58, 0.0, 0.79), ("O", -1.58, 0.0, 0.79),
+                              ("Cl", 0.0, 1.76, -0.88), ("Cl", 0.0, -1.76, -0.88)], 0, 1),
+        "permanganate": (tetrahedral("Mn", "O", 1.63), -1, 1),
+        "yttrium_trichloride": (trigonal_planar("Y", "Cl", 2.47), 0, 1),
+        "zirconium_tetrachloride": (tetrahedral("Zr", "Cl", 2.32), 0, 1),
+        "niobium_pentachloride": ([("Nb", 0.0, 0.0, 0.0)]
+                                  + [("Cl", 2.30 * x, 2.30 * y, 0.0) for x, y in
+                                     ((1.0, 0.0), (-0.5, 0.866025), (-0.5, -0.866025))]
+                                  + [("Cl", 0.0, 0.0, 2.34), ("Cl", 0.0, 0.0, -2.34)], 0, 1),
+        "molybdenum_hexafluoride": (octahedral("Mo", "F", 1.82), 0, 1),
+        "pertechnetate": (tetrahedral("Tc", "O", 1.71), -1, 1),
+        "lanthanum_trichloride": (trigonal_planar("La", "Cl", 2.59), 0, 1),
+        "lutetium_trifluoride": (trigonal_planar("Lu", "F", 2.02), 0, 1),
+        "hafnium_tetrachloride": (tetrahedral("Hf", "Cl", 2.32), 0, 1),
+        "tantalum_pentachloride": ([("Ta", 0.0, 0.0, 0.0)]
+                                   + [("Cl", 2.28 * x, 2.28 * y, 0.0) for x, y in
+                                      ((1.0, 0.0), (-0.5, 0.866025), (-0.5, -0.866025))]
+                                   + [("Cl", 0.0, 0.0, 2.32), ("Cl", 0.0, 0.0, -2.32)], 0, 1),
+        "tungsten_hexafluoride": (octahedral("W", "F", 1.83), 0, 1),
+        "perrhenate": (tetrahedral("Re", "O", 1.72), -1, 1),
+        "osmium_tetroxide": (tetrahedral("Os", "O", 1.71), 0, 1),
+        "copper_chloride": (diatomic("Cu", "Cl", 2.05), 0, 1),
+        "zinc_dichloride": (linear(["Cl", "Zn", "Cl"], [2.07, 2.07]), 0, 1),
+        "silver_chloride": (diatomic("Ag", "Cl", 2.28), 0, 1),
+        "cadmium_dichloride": (linear(["Cl", "Cd", "Cl"], [2.28, 2.28]), 0, 1),
+        "mercury_dichloride": (linear(["Cl", "Hg", "Cl"], [2.25, 2.25]), 0, 1),
+        "tetrachloropalladate": (square_planar("Pd", "Cl", 2.31), -2, 1),
+        "tetrachloroplatinate": (square_planar("Pt", "Cl", 2.32), -2, 1),
+        "gold_chloride": (diatomic("Au", "Cl", 2.20), 0, 1),
+        "nickel_tetracarbonyl": (carbonyl("Ni", 1.84, 1.13, "tetrahedral"), 0, 1),
+        "iron_pentacarbonyl": (carbonyl("Fe", 1.81, 1.15, "bipyramidal"), 0, 1),
+        "chromium_hexacarbonyl": (carbonyl("Cr", 1.92, 1.14, "octahedral"), 0, 1),
+        "molybdenum_hexacarbonyl": (carbonyl("Mo", 2.06, 1.15, "octahedral"), 0, 1),
+        "tungsten_hexacarbonyl": (carbonyl("W", 2.06, 1.15, "octahedral"), 0, 1),
+        "ruthenium_pentacarbonyl": (carbonyl("Ru", 1.94, 1.14, "bipyramidal"), 0, 1),
+        "osmium_pentacarbonyl": (carbonyl("Os", 1.95, 1.14, "bipyramidal"), 0, 1),
+        "manganese_carbonyl_hydride": (carbonyl("Mn", 1.85, 1.14, "octahedral")[: 1 + 2 * 5]
+                                       + [("H", 0.0, 0.0, -1.58)], 0, 1),
+        "cobalt_tetracarbonyl_anion": (carbonyl("Co", 1.80, 1.15, "tetrahedral"), -1, 1),
+        "rhodium_dicarbonyl_chloride": ([("Rh", 0.0, 0.0, 0.0), ("C", 1.85, 0.0, 0.0), ("O", 2.99, 0.0, 0.0),
+                                         ("C", 0.0, 1.85, 0.0), ("O", 0.0, 2.99, 0.0), ("Cl", -2.32, 0.0, 0.0),
+                                         ("Cl", 0.0, -2.32, 0.0)], -1, 1),
+        "iridium_dicarbonyl_dichloride": ([("Ir", 0.0, 0.0, 0.0), ("C", 1.87, 0.0, 0.0), ("O", 3.01, 0.0, 0.0),
+                                           ("C", 0.0, 1.87, 0.0), ("O", 0.0, 3.01, 0.0), ("Cl", -2.35, 0.0, 0.0),
+                                           ("Cl", 0.0, -2.35, 0.0)], -1, 1),
+        "ferrocene": (metallocene("Fe", 1.66, 1.43, 1.08), 0, 1),
+        "methylmagnesium_chloride": ([("Mg", 0.0, 0.0, 0.0), ("C", 2.10, 0.0, 0.0), ("Cl", -2.20, 0.0, 0.0)]
+                                     + cap((2.10, 0.0, 0.0), (0.0, 0.0, 0.0), "H", 1.09, 3), 0, 1),
+        "titanocene_dichloride": (metallocene("Ti", 2.06, 1.42, 1.08, tilt=1.30)
+                                  + [("Cl", 0.0, 2.10, -1.55), ("Cl", 0.0, -2.10, -1.55)], 0, 1),
+        "phosphorus_atom_quartet": (xyz(["P 0.0 0.0 0.0"]), 0, 4),
+        "sulfur_atom_triplet": (xyz(["S 0.0 0.0 0.0"]), 0, 3),
+        "silicon_atom_triplet": (xyz(["Si 0.0 0.0 0.0"]), 0, 3),
         "tetramethyltin": ([("Sn", 0.0, 0.0, 0.0)]
                            + [("C", 2.14 * x / math.sqrt(3.0), 2.14 * y / math.sqrt(3.0), 2.14 * z / math.sqrt(3.0))
                               for x, y, z in TET_DIRS]

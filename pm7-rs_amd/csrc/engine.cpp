@@ -115,6 +115,7 @@ struct Job {
   std::vector<int> z;
   std::vector<double> xyz;
   int nao = 0, ld = 0, n_pp = 0, na = 0, nb = 0, uhf = 0, variant = 0;
+  int dpath = 0;  // holds a d-bearing atom: every pair through the MNDO/d kernels
   double sad_yy = 0.0;
   int64_t pair0 = 0, mat0 = 0, pp0 = 0, vec0 = 0, gath0 = 0, w0 = 0;
   int atom0 = 0;
@@ -136,7 +137,7 @@ struct Lane {
   hipEvent_t ev[12] = {};
   hipEvent_t ev_var[2 * SCF_VARIANTS] = {};
   DevBuf d_mols, d_list, d_atom_z, d_atom_ao, d_atom_pp, d_xyz, d_pairs, d_items, d_gath_off;
-  DevBuf d_w, d_slab, d_vecs, d_out, d_ecore, d_ecore_mol, d_gath, d_num, d_snum;
+  DevBuf d_w, d_slab, d_vecs, d_out, d_ecore, d_ecore_mol, d_gath, d_num, d_snum, d_dpairs;
   PinnedBuf h_out;  // staging for the packed results
   LargeWork* large = nullptr;  // rocBLAS handle and buffers of the large-system path (lazily made)
   std::string err;
@@ -155,7 +156,8 @@ struct Lane {
     for (auto& s : st)
       if (s) (void)hipStreamSynchronize(s);
     DevBuf* bufs[] = {&d_mols, &d_list, &d_atom_z, &d_atom_ao, &d_atom_pp, &d_xyz, &d_pairs, &d_items, &d_gath_off,
-                      &d_w, &d_slab, &d_vecs, &d_out, &d_ecore, &d_ecore_mol, &d_gath, &d_num, &d_snum};
+                      &d_w, &d_slab, &d_vecs, &d_out, &d_ecore, &d_ecore_mol, &d_gath, &d_num, &d_snum,
+                      &d_dpairs};
     for (DevBuf* b : bufs) b->release();
     h_out.release();
     if (large) large_work_destroy(large);
@@ -182,7 +184,7 @@ struct pm7_ctx {
   ParamSet params;
   int method = -1;
   std::vector<ElemDev> elem_host;
-  DevBuf d_elem, d_pairpar, d_gl;
+  DevBuf d_elem, d_pairpar, d_gl, d_ocd;
   Lane lane;
   double timings[PM7_TIMING_SLOTS] = {};
   // Jacobi sweep cap of the first (cold) eigensolve in the reduced-precision phase of
@@ -276,11 +278,8 @@ pm7_status validate(pm7_ctx* ctx, const pm7_molecule_v1& m, const pm7_options_v1
       return PM7_ERR_MISSING_ELEMENT;
     }
     const HostElement& e = ctx->params.el[z];
-    if (e.n_orb == 9) {
-      msg = "element " + std::to_string(z) + " needs the MNDO/d two-centre kernel, not built yet";
-      return PM7_ERR_UNSUPPORTED;
-    }
     tore += e.core_charge;
+    if (e.n_orb == 9) j.dpath = 1;
     nao += e.n_orb;
     npp += npair_of(e.n_orb);
   }
@@ -323,15 +322,22 @@ pm7_status validate(pm7_ctx* ctx, const pm7_molecule_v1& m, const pm7_options_v1
 struct Prepared {
   std::vector<Job> jobs;
   std::vector<MolDev> mols;
-  int64_t n_pairs = 0, w_words = 0, slab_words = 0, vec_words = 0, gath_words = 0;
+  int64_t n_pairs = 0, w_words = 0, slab_words = 0, vec_words = 0, gath_words = 0, n_dpairs = 0;
   int n_items = 0;
   int list_off[SCF_VARIANTS + 1] = {};
   BatchDev b;
 };
 
 size_t bytes_needed(const Prepared& P) {
-  return (size_t)(P.w_words + P.slab_words + P.vec_words + P.gath_words) * 8 + (size_t)P.n_pairs * (sizeof(PairDev) + 8) +
+  return (size_t)(P.w_words + P.slab_words + P.vec_words + P.gath_words + P.n_dpairs) * 8 +
+         (size_t)P.n_pairs * (sizeof(PairDev) + 8) +
          P.jobs.size() * (sizeof(MolDev) + sizeof(MolOut) + 64);
+}
+
+// Words a pair of a d-path molecule takes in the integral array: the npair(f) x npair(s) block and
+// the packed electron-core blocks of both atoms.
+int64_t pair_words_d(int nf, int ns) {
+  return (int64_t)npair_of(nf) * npair_of(ns) + npair_of(nf) + npair_of(ns);
 }
 
 // Device bytes one validated job adds to a batch (the terms of bytes_needed).
@@ -339,8 +345,10 @@ size_t job_bytes(const pm7_ctx* ctx, const Job& j) {
   const int n = (int)j.z.size();
   int64_t w = 0;
   for (int u = 0; u < n; ++u)
-    for (int v = u + 1; v < n; ++v)
-      w += (int64_t)npair_of(ctx->params.el[j.z[u]].n_orb) * npair_of(ctx->params.el[j.z[v]].n_orb);
+    for (int v = u + 1; v < n; ++v) {
+      const int nu = ctx->params.el[j.z[u]].n_orb, nv = ctx->params.el[j.z[v]].n_orb;
+      w += j.dpath ? pair_words_d(nu, nv) + 1 : (int64_t)npair_of(nu) * npair_of(nv);
+    }
   const int64_t slab = (int64_t)NSLOT * j.ld * j.ld + ((j.n_pp + 7) / 8) * 8;
   const int64_t words = w + slab + 2 * j.ld + (int64_t)(j.uhf ? 4 : 2) * j.nao * j.nao;
   return (size_t)words * 8 + (size_t)n * (n - 1) / 2 * (sizeof(PairDev) + 8) + sizeof(MolDev) + sizeof(MolOut) + 64;
@@ -355,6 +363,7 @@ pm7_status prepare(pm7_ctx* ctx, Lane& L, Prepared& P, int64_t max_memory_mb) {
   std::vector<int2> items;
   std::vector<int64_t> gath_off(nm);
   std::vector<int64_t> num_pairs;  // pairs whose overlaps need the quadrature (a shell with n >= 4)
+  std::vector<int64_t> d_pairs;    // pairs of molecules with a d-bearing atom
   int64_t w_off = 0, mat = 0, vec = 0, gath = 0, pair0 = 0;
   int atom0 = 0;
   P.mols.resize(nm);
@@ -389,9 +398,14 @@ pm7_status prepare(pm7_ctx* ctx, Lane& L, Prepared& P, int64_t max_memory_mb) {
           pd.s = atom0 + u;
         }
         pd.w_off = w_off;
-        w_off += (int64_t)npair_of(std::max(nu, nv)) * npair_of(std::min(nu, nv));
-        if (std::max(ctx->params.el[j.z[u]].n, ctx->params.el[j.z[v]].n) >= 4 ||
-            std::min(ctx->params.el[j.z[u]].n, ctx->params.el[j.z[v]].n) < 1) {
+        if (j.dpath) {  // w block, then both electron-core blocks; spd overlaps in closed form
+          w_off += pair_words_d(std::max(nu, nv), std::min(nu, nv));
+          d_pairs.push_back((int64_t)pairs.size());
+        } else {
+          w_off += (int64_t)npair_of(std::max(nu, nv)) * npair_of(std::min(nu, nv));
+        }
+        if (!j.dpath && (std::max(ctx->params.el[j.z[u]].n, ctx->params.el[j.z[v]].n) >= 4 ||
+                         std::min(ctx->params.el[j.z[u]].n, ctx->params.el[j.z[v]].n) < 1)) {
           num_pairs.push_back((int64_t)pairs.size());
           pd.pad = (int)num_pairs.size();  // 1-based slot of its numerical overlaps
         }
@@ -430,6 +444,7 @@ pm7_status prepare(pm7_ctx* ctx, Lane& L, Prepared& P, int64_t max_memory_mb) {
     M.sad_yy = j.sad_yy;
     M.mixed_switch = j.o.mixed_switch;
     M.mixed_sweeps = ctx->mixed_sweeps;
+    M.dpath = j.dpath;
     lists[j.variant].push_back(mi);
     atom0 += n;
     pair0 += (int64_t)n * (n - 1) / 2;
@@ -449,6 +464,7 @@ pm7_status prepare(pm7_ctx* ctx, Lane& L, Prepared& P, int64_t max_memory_mb) {
   P.vec_words = vec;
   P.gath_words = gath;
   P.n_items = (int)items.size();
+  P.n_dpairs = (int64_t)d_pairs.size();
   // HBM budget (pm7-rs memory::guard, src/memory.rs:95-118, but for device memory).
   size_t need = bytes_needed(P);
   size_t free_b = 0, total_b = 0;
@@ -483,6 +499,7 @@ pm7_status prepare(pm7_ctx* ctx, Lane& L, Prepared& P, int64_t max_memory_mb) {
   LANECHK(L.d_gath.ensure((size_t)P.gath_words * 8));
   LANECHK(up(L.d_num, num_pairs.data(), num_pairs.size() * sizeof(int64_t)));
   LANECHK(L.d_snum.ensure(num_pairs.size() * 5 * sizeof(double)));
+  LANECHK(up(L.d_dpairs, d_pairs.data(), d_pairs.size() * sizeof(int64_t)));
   // Zero the slab: pad rows/columns of every matrix must be exact zeros.
   LANECHK(hipMemsetAsync(L.d_slab.p, 0, (size_t)P.slab_words * 8, L.st[0]));
   BatchDev& b = P.b;
@@ -509,6 +526,9 @@ pm7_status prepare(pm7_ctx* ctx, Lane& L, Prepared& P, int64_t max_memory_mb) {
   b.n_num = (int)num_pairs.size();
   b.gl = ctx->d_gl.as<double>();
   b.s_num = L.d_snum.as<double>();
+  b.d_pairs = L.d_dpairs.as<int64_t>();
+  b.n_dp = (int64_t)d_pairs.size();
+  b.ocd = ctx->d_ocd.as<double>();
   LANECHK(hipEventRecord(L.ev[0], L.st[0]));
   LANECHK(launch_integrals(b, P.n_pairs, P.n_items, nm, L.st[0]));
   LANECHK(hipEventRecord(L.ev[1], L.st[0]));
@@ -596,6 +616,18 @@ void finish_result(pm7_ctx* ctx, const Job& j, const MolOut& R, double e_core, c
       for (int k = 0; k < 3; ++k) org[k] /= tot;
   }
   const double D = 2.541746473;
+  // p-d hybrid term of a d-bearing atom (pm7-rs src/dipole.rs:332-340); orbitals
+  // s px py pz dx2-y2 dxz dz2 dyz dxy.
+  double pd[3] = {0.0, 0.0, 0.0};
+  for (int i = 0; i < nat; ++i) {
+    const HostElement& e = ctx->params.el[j.z[i]];
+    if (e.n_orb != 9) continue;
+    auto P = [&](int a, int b) { return out->density[(size_t)(off[i] + a) * n + off[i] + b]; };
+    const double inv3 = 1.0 / std::sqrt(3.0), hyb = -2.0 * e.ddp[5];
+    pd[0] += hyb * (P(5, 3) + P(4, 1) + P(8, 2) - inv3 * P(6, 1));
+    pd[1] += hyb * (P(7, 3) - P(4, 2) + P(8, 1) - inv3 * P(6, 2));
+    pd[2] += hyb * (P(5, 1) + P(7, 2) + 2.0 * inv3 * P(6, 3));
+  }
   for (int k = 0; k < 3; ++k) {
     double pc = 0.0, sp = 0.0;
     for (int i = 0; i < nat; ++i) {
@@ -605,8 +637,8 @@ void finish_result(pm7_ctx* ctx, const Job& j, const MolOut& R, double e_core, c
     }
     out->dipole_point_charge_debye[k] = pc * D;
     out->dipole_sp_hybrid_debye[k] = sp * D;
-    out->dipole_pd_hybrid_debye[k] = 0.0;
-    out->dipole_debye[k] = pc * D + sp * D;
+    out->dipole_pd_hybrid_debye[k] = pd[k] * D;
+    out->dipole_debye[k] = pc * D + sp * D + pd[k] * D;
   }
   out->dipole_magnitude = std::sqrt(out->dipole_debye[0] * out->dipole_debye[0] +
                                     out->dipole_debye[1] * out->dipole_debye[1] +
@@ -676,6 +708,14 @@ pm7_status ensure_params(pm7_ctx* ctx, int method) {
   for (int i = 0; i < MAX_Z * MAX_Z; ++i) pp[i] = make_double2(ctx->params.pair[i].first, ctx->params.pair[i].second);
   HIPCHK(ctx->d_pairpar.ensure(pp.size() * sizeof(double2)));
   HIPCHK(hipMemcpy(ctx->d_pairpar.p, pp.data(), pp.size() * sizeof(double2), hipMemcpyHostToDevice));
+  {
+    std::vector<double> ocd((size_t)MAX_Z * OC_D_WORDS, 0.0);
+    for (int z = 0; z < MAX_Z; ++z)
+      if (ctx->params.el[z].oc_d.size() == (size_t)OC_D_WORDS)
+        std::copy(ctx->params.el[z].oc_d.begin(), ctx->params.el[z].oc_d.end(), ocd.begin() + (size_t)z * OC_D_WORDS);
+    HIPCHK(ctx->d_ocd.ensure(ocd.size() * sizeof(double)));
+    HIPCHK(hipMemcpy(ctx->d_ocd.p, ocd.data(), ocd.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   if (!ctx->d_gl.p) {
     std::vector<double> gl;
     gauss_legendre_table(gl);
@@ -1168,6 +1208,7 @@ void pm7_ctx_destroy(pm7_ctx* ctx) {
   ctx->d_elem.release();
   ctx->d_pairpar.release();
   ctx->d_gl.release();
+  ctx->d_ocd.release();
   delete ctx;
 }
 

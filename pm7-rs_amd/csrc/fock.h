@@ -16,8 +16,10 @@
 
 namespace pm7 {
 
+// Packed pair e = mu(mu+1)/2 + nu (mu >= nu) back to its orbitals, e < 45.
 __device__ inline void unpack_e(int e, int& mu, int& nu) {
-  mu = (e >= 6) ? 3 : (e >= 3) ? 2 : (e >= 1) ? 1 : 0;
+  mu = (e >= 36) ? 8 : (e >= 28) ? 7 : (e >= 21) ? 6 : (e >= 15) ? 5 : (e >= 10) ? 4 : (e >= 6) ? 3 : (e >= 3) ? 2
+                                                                                        : (e >= 1) ? 1 : 0;
   nu = e - mu * (mu + 1) / 2;
 }
 
@@ -45,8 +47,47 @@ struct MolCtx {
   double* base;  // molecule slab
   double* pp;    // packed densities
   int ld;
+  const double* ocd;  // one-centre spd matrices by atomic number (pm7-rs src/fock.rs:171-189)
   __device__ double* slot(int s) const { return base + (int64_t)s * ld * ld; }
 };
+
+// One-centre (mu nu | la si) of an atom: the sp rules, or the element's 45 x 45 spd matrix.
+__device__ inline double oc_val(const MolCtx& c, const ElemDev& e, int mu, int nu, int la, int si) {
+  if (e.n_orb == 9) return c.ocd[(int64_t)e.z * OC_D_WORDS + pack_idx(mu, nu) * 45 + pack_idx(la, si)];
+  return oc_int(mu, nu, la, si, e);
+}
+
+// MOPAC diagonal guess for one atom (pm7-rs src/scf.rs:1050-1114): the core charge spread over
+// the s/p orbitals (a main-group d shell gets only the charge correction), a transition metal's
+// filled s, then d, then p; yy spreads the molecular charge. Writes the n_orb diagonal entries.
+__device__ inline void sad_atom(const ElemDev& e, double yy, double scale, double* P, int o, int ld) {
+  double occ[9];
+  const int n = e.n_orb;
+  const double tore = e.core_charge;
+  for (int k = 0; k < 9; ++k) occ[k] = 0.0;
+  if (n == 1) {
+    occ[0] = tore - yy;
+  } else if (n == 4) {
+    for (int k = 0; k < 4; ++k) occ[k] = tore * 0.25 - yy;
+  } else if (n == 9) {
+    const int z = e.z;
+    if (z < 21 || (z > 30 && z < 39) || (z > 48 && z < 57)) {
+      for (int k = 0; k < 4; ++k) occ[k] = tore * 0.25 - yy;
+      for (int k = 4; k < 9; ++k) occ[k] = -yy;
+    } else {
+      double sum = tore - 9.0 * yy;
+      occ[0] = fmin(fmax(sum, 0.0), 2.0);
+      sum -= 2.0;
+      if (sum > 0.0) {
+        for (int k = 4; k < 9; ++k) occ[k] = fmin(fmax(sum * 0.2, 0.0), 2.0);
+        sum -= 10.0;
+        if (sum > 0.0)
+          for (int k = 1; k < 4; ++k) occ[k] = sum / 3.0;
+      }
+    }
+  }
+  for (int k = 0; k < n; ++k) P[(int64_t)(o + k) * ld + o + k] = occ[k] * scale;
+}
 
 __device__ inline int find_atom(const int* atom_pp, int a0, int na, int t) {
   int lo = 0, hi = na - 1;
@@ -107,8 +148,8 @@ __device__ inline void fock_diag_item(const MolCtx& c, int t, const double* __re
   for (int l = 0; l < n; ++l)
     for (int s = 0; s < n; ++s) {
       int64_t ix = (int64_t)(o + l) * ld + o + s;
-      ja += (T)Pt[ix] * (T)oc_int(mu, nu, l, s, ea);
-      const T gx = (T)oc_int(mu, l, nu, s, ea);
+      ja += (T)Pt[ix] * (T)oc_val(c, ea, mu, nu, l, s);
+      const T gx = (T)oc_val(c, ea, mu, l, nu, s);
       ka += (T)Psa[ix] * gx;
       if (Fb) kb += (T)Psb[ix] * gx;
     }
@@ -168,8 +209,8 @@ __device__ inline void fock_diag_item_wave(const MolCtx& c, int t, const double*
   for (int l = 0; l < n; ++l)
     for (int s = 0; s < n; ++s) {
       int64_t ix = (int64_t)(o + l) * ld + o + s;
-      ja += (T)Pt[ix] * (T)oc_int(mu, nu, l, s, ea);
-      const T gx = (T)oc_int(mu, l, nu, s, ea);
+      ja += (T)Pt[ix] * (T)oc_val(c, ea, mu, nu, l, s);
+      const T gx = (T)oc_val(c, ea, mu, l, nu, s);
       ka += (T)Psa[ix] * gx;
       if (Fb) kb += (T)Psb[ix] * gx;
     }
@@ -184,6 +225,40 @@ __device__ inline void fock_diag_item_wave(const MolCtx& c, int t, const double*
   }
 }
 
+// Exchange block of a pair with a 9-orbital atom: runtime loops, densities read from the slab.
+template <typename T>
+__device__ void fock_pair_item_spd(const MolCtx& c, const PairDev& P, int nf, int ns, const double* __restrict__ Psa,
+                                   double xa, const double* __restrict__ Psb, double xb, double* __restrict__ Fa,
+                                   double* __restrict__ Fb) {
+  const int ld = c.ld, nps = npair_of(ns);
+  const double* H = c.slot(S_H);
+  const int of = c.atom_ao[P.f], os = c.atom_ao[P.s];
+  const double* wp = c.w + P.w_off;
+  for (int mu = 0; mu < nf; ++mu)
+    for (int lam = 0; lam < ns; ++lam) {
+      T acca = (T)0, accb = (T)0;
+      for (int nu = 0; nu < nf; ++nu) {
+        const double* row = wp + (int64_t)pack_idx(mu, nu) * nps;
+        const double* pa = Psa + (int64_t)(of + nu) * ld + os;
+        const double* pb = Fb ? Psb + (int64_t)(of + nu) * ld + os : nullptr;
+        for (int sg = 0; sg < ns; ++sg) {
+          const T wv = (T)row[pack_idx(lam, sg)];
+          acca += (T)pa[sg] * wv;
+          if (Fb) accb += (T)pb[sg] * wv;
+        }
+      }
+      const int64_t i1 = (int64_t)(of + mu) * ld + os + lam, i2 = (int64_t)(os + lam) * ld + of + mu;
+      const double fa = H[i1] - (double)((T)xa * acca);
+      Fa[i1] = fa;
+      Fa[i2] = fa;
+      if (Fb) {
+        const double fb = H[i1] - (double)((T)xb * accb);
+        Fb[i1] = fb;
+        Fb[i2] = fb;
+      }
+    }
+}
+
 template <typename T>
 __device__ inline void fock_pair_item(const MolCtx& c, int p, const double* __restrict__ Psa, double xa,
                                       const double* __restrict__ Psb, double xb, double* __restrict__ Fa,
@@ -195,6 +270,10 @@ __device__ inline void fock_pair_item(const MolCtx& c, int p, const double* __re
   const ElemDev ef = c.elem[c.atom_z[P.f]];
   const ElemDev es = c.elem[c.atom_z[P.s]];
   const int nf = ef.n_orb, ns = es.n_orb, nps = npair_of(ns);
+  if (nf == 9) {  // the first atom carries at least as many orbitals as the second
+    fock_pair_item_spd<T>(c, P, nf, ns, Psa, xa, Psb, xb, Fa, Fb);
+    return;
+  }
   const int of = c.atom_ao[P.f], os = c.atom_ao[P.s];
   const double* wp = c.w + P.w_off;
   // Same-spin density block held in registers: every index below is a compile-time constant.

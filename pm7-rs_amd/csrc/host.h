@@ -20,7 +20,16 @@ struct HostElement {
   double beta_s = 0, beta_p = 0, beta_d = 0;
   double g_ss = 0, g_sp = 0, g_pp = 0, g_p2 = 0, h_sp = 0, poc = 0;
   double dd = 0, qq = 0, rho0 = 0, rho1 = 0, rho2 = 0;
+  double zsn = 0, zpn = 0, zdn = 0, f0sd = 0, g2sd = 0;
   std::vector<std::array<double, 3>> gauss;
+  // MNDO/d data (1-based slots): additive terms po(1..9), charge separations ddp(2..6), the
+  // one-centre d integrals repd(1..52), the isolated-atom d-shell correction, and for a d-bearing
+  // element the 45 x 45 one-centre matrix over packed orbital pairs.
+  std::array<double, 10> po{};
+  std::array<double, 7> ddp{};
+  std::array<double, 53> repd{};
+  double eiscor = 0;
+  std::vector<double> oc_d;
 };
 
 struct ParamSet {
@@ -31,6 +40,8 @@ struct ParamSet {
 };
 
 bool load_params(const std::string& dir, int method, ParamSet& P, std::string& err);
+void derive_mndod(HostElement& e);
+int shell_principal(int z, bool d_shell);
 void to_device_table(const ParamSet& P, std::vector<ElemDev>& out);
 
 // Geometric post-SCF corrections in kcal/mol (coordinates in Bohr).

@@ -469,6 +469,7 @@ __global__ void __launch_bounds__(128) k_pairs(int64_t n_pairs, const PairDev* _
   if (g >= n_pairs) return;
   const PairDev P = pairs[g];
   const MolDev& M = mols[P.mol];
+  if (M.dpath) return;  // k_pairs_d owns the pairs of molecules with a d shell
   int zf = atom_z[P.f], zs = atom_z[P.s];
   const ElemDev ef = elem[zf];
   const ElemDev es = elem[zs];
@@ -529,13 +530,13 @@ __global__ void __launch_bounds__(128) k_hcore_diag(int n_items, const int2* __r
   const MolDev& M = mols[mol];
   const ElemDev ea = elem[atom_z[a]];
   int mu = 0, nu = 0;
-  for (int h = 0, c = 0; h < 4; ++h)
+  for (int h = 0, c = 0; h < 9; ++h)
     for (int l = 0; l <= h; ++l, ++c)
       if (c == e) {
         mu = h;
         nu = l;
       }
-  double acc = (mu == nu) ? (mu == 0 ? ea.u_ss : ea.u_pp) : 0.0;
+  double acc = (mu == nu) ? (mu == 0 ? ea.u_ss : (mu <= 3 ? ea.u_pp : ea.u_dd)) : 0.0;
   int la = a - M.atom0;
   int npa = npair_of(ea.n_orb);
   for (int lb = 0; lb < M.n_atoms; ++lb) {
@@ -545,15 +546,21 @@ __global__ void __launch_bounds__(128) k_hcore_diag(int n_items, const int2* __r
     int b = M.atom0 + lb;
     const ElemDev eb = elem[atom_z[b]];
     const double* wp = w + P.w_off;
+    const int npb = npair_of(eb.n_orb);
     double val;
+    if (M.dpath) {
+      // electron-core blocks stored after w: first atom's, then the second's
+      val = P.f == a ? wp[(int64_t)npa * npb + e] : wp[(int64_t)npa * npb + npb + e];
+      acc += val;
+      continue;
+    }
     if (P.f == a) {
-      val = wp[(int64_t)e * npair_of(eb.n_orb)];      // (mu nu | s s) with a first
+      val = wp[(int64_t)e * npb];      // (mu nu | s s) with a first
     } else {
-      val = wp[e];                                      // row 0 of the (b, a) block
+      val = wp[e];                      // row 0 of the (b, a) block
     }
     acc += -eb.core_charge * val;
   }
-  (void)npa;
   const int ld = M.ld;
   double* H = slab + M.mat0;
   int o = atom_ao[a];
@@ -597,7 +604,16 @@ __global__ void __launch_bounds__(256) k_gather(const MolDev* __restrict__ mols,
   }
 }
 
+}  // namespace pm7
+
+#include "integrals_d.h"
+
+namespace pm7 {
+
 hipError_t launch_integrals(const BatchDev& b, int64_t n_pairs, int n_items, int n_mols, hipStream_t s) {
+  if (b.n_dp > 0)
+    hipLaunchKernelGGL(k_pairs_d, dim3((unsigned)b.n_dp), dim3(64), 0, s, b.d_pairs, b.pairs, b.mols, b.atom_z,
+                       b.atom_ao, b.xyz, b.elem, b.pairpar, b.vpar, b.w, b.slab, b.e_core);
   if (b.n_num > 0)
     hipLaunchKernelGGL(k_overlap_numeric, dim3(b.n_num), dim3(64), 0, s, b.n_num, b.num_pairs, b.pairs, b.atom_z, b.xyz,
                        b.elem, b.gl, b.s_num);

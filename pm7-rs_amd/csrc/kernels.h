@@ -37,6 +37,11 @@ struct BatchDev {
   int n_num = 0;
   const double* gl = nullptr;      // Gauss-Legendre nodes/weights: x48, w48, x40, w40 on [-1, 1]
   double* s_num = nullptr;         // 5 local overlaps per numerical pair
+  // Pairs of molecules holding a d-bearing atom (the MNDO/d kernels), and the one-centre spd
+  // matrices of the d-bearing elements (OC_D_WORDS per atomic number).
+  const int64_t* d_pairs = nullptr;
+  int64_t n_dp = 0;
+  const double* ocd = nullptr;
 };
 
 // Gauss-Legendre nodes and weights on [-1, 1] for the overlap quadrature (48 and 40 points).

@@ -76,6 +76,7 @@ __device__ MolCtx make_ctx(const MolDev& M, const BatchDev& b) {
   c.atom_pp = b.atom_pp;
   c.pairs = b.pairs;
   c.elem = b.elem;
+  c.ocd = b.ocd;
   c.w = b.w;
   c.ld = M.ld;
   c.base = b.slab + M.mat0;
@@ -396,10 +397,7 @@ __global__ void k_lg_init(LargeCtl* __restrict__ ctl, double shift_floor, int lo
 __global__ void k_lg_sad(int mi, BatchDev b, double* __restrict__ P, double scale) {
   const MolDev M = b.mols[mi];
   for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < M.n_atoms; a += gridDim.x * blockDim.x) {
-    const ElemDev e = b.elem[b.atom_z[M.atom0 + a]];
-    const int o = b.atom_ao[M.atom0 + a];
-    const double occ = e.n_orb == 1 ? e.core_charge : e.core_charge / 4.0;
-    for (int k = 0; k < e.n_orb; ++k) P[(int64_t)(o + k) * M.ld + o + k] = (occ - M.sad_yy) * scale;
+    sad_atom(b.elem[b.atom_z[M.atom0 + a]], M.sad_yy, scale, P, b.atom_ao[M.atom0 + a], M.ld);
   }
 }
 

@@ -113,6 +113,11 @@ bool load_params(const std::string& dir, int method, ParamSet& P, std::string& e
     e.g_p2 = v("g_p2");
     e.h_sp = v("h_sp");
     e.poc = v("poc");
+    e.zsn = v("zsn");
+    e.zpn = v("zpn");
+    e.zdn = v("zdn");
+    e.f0sd = v("f0sd");
+    e.g2sd = v("g2sd");
     e.n_orb = e.zeta_d > 1.0e-20 ? 9 : (e.zeta_p > 1.0e-20 ? 4 : 1);
     if (e.n_orb >= 4) {
       double zp = std::max(e.zeta_p, 0.3);
@@ -135,6 +140,10 @@ bool load_params(const std::string& dir, int method, ParamSet& P, std::string& e
     e.e_isol = e.u_ss * e.ios + e.u_pp * p + e.u_dd * e.iod + e.g_ss * std::max(e.ios - 1, 0) +
                e.g_pp * (-0.5 * l * (l - 1.0) / 2.0) + e.g_sp * (e.ios * p) +
                e.g_p2 * (p * (p - 1.0) / 2.0 + 0.5 * l * (l - 1.0) / 2.0) + e.h_sp * (-p * e.ios * 0.5);
+    // MNDO/d data for every element (a pair in a molecule with a d shell needs both atoms'
+    // po/ddp); the partly-filled d-shell term joins the isolated-atom energy.
+    derive_mndod(e);
+    e.e_isol += e.eiscor;
   }
   std::vector<double> alpb(MAX_Z * MAX_Z, 0.0), xfac(MAX_Z * MAX_Z, 0.0);
   std::vector<char> has(MAX_Z * MAX_Z, 0);
@@ -198,6 +207,15 @@ void to_device_table(const ParamSet& P, std::vector<ElemDev>& out) {
     d.po9 = e.poc > 1.0e-5 ? e.poc : e.rho0;
     d.n = e.n;
     d.n_orb = e.n_orb;
+    d.z = z;
+    d.iod = e.iod;
+    d.u_dd = e.u_dd;
+    d.beta_d = e.beta_d;
+    d.zeta_d = e.zeta_d;
+    d.n_sp = shell_principal(z, false);
+    d.n_d = shell_principal(z, true);
+    for (int k = 0; k < 10; ++k) d.po[k] = e.po[k];
+    for (int k = 0; k < 7; ++k) d.ddp[k] = e.ddp[k];
     d.n_gauss = (int)std::min<size_t>(4, e.gauss.size());
     for (int k = 0; k < d.n_gauss; ++k) {
       d.gk[k] = e.gauss[k][0];

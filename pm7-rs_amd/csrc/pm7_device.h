@@ -41,8 +41,16 @@ struct ElemDev {
   double g_ss, g_sp, g_pp, g_p2, h_sp;
   double zeta_s, zeta_p, dd, qq, rho0, rho1, rho2, po9;
   double gk[4], gl[4], gm[4];
+  // spd basis (MNDO/d): d-shell one-electron terms, additive terms po(1..9) and charge
+  // separations ddp(2..6) of the multipole model (1-based slots)
+  double u_dd, beta_d, zeta_d;
+  double po[10], ddp[7];
   int n, n_orb, n_gauss, present;
+  int z, iod, n_sp, n_d;  // atomic number, d occupation of the reference atom, shell quantum numbers
 };
+
+// Words of the one-centre spd matrix of a d-bearing element (45 x 45 packed orbital pairs).
+constexpr int OC_D_WORDS = 45 * 45;
 
 struct PairDev {
   int mol;       // batch-local molecule index
@@ -56,6 +64,8 @@ struct MolDev {
   int atom0;            // batch-global index of the first atom
   int max_scf, accel, use_diis, precision, n_pp;
   int mixed_sweeps;     // Jacobi sweep cap per eigensolve in the reduced-precision phase
+  int dpath;            // 1: the molecule holds a d-bearing atom, every pair takes the MNDO/d kernels
+  int pad_;
   int64_t pair0;        // first pair of the molecule
   int64_t mat0;         // slab offset (doubles)
   int64_t pp0;          // slab offset of the packed densities

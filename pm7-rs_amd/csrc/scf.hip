@@ -140,10 +140,7 @@ __device__ void sad_guess(const MolCtx& c, double* P, double scale) {
   __syncthreads();
   for (int la = threadIdx.x; la < c.M.n_atoms; la += NT) {
     int a = c.M.atom0 + la;
-    const ElemDev e = c.elem[c.atom_z[a]];
-    int o = c.atom_ao[a];
-    double v = e.n_orb == 1 ? e.core_charge - c.M.sad_yy : e.core_charge * 0.25 - c.M.sad_yy;
-    for (int k = 0; k < e.n_orb; ++k) P[(int64_t)(o + k) * ld + o + k] = v * scale;
+    sad_atom(c.elem[c.atom_z[a]], c.M.sad_yy, scale, P, c.atom_ao[a], ld);
   }
   __syncthreads();
 }
@@ -160,7 +157,8 @@ template <int NT, int MAXN, bool A_LDS, bool V_LDS>
 __global__ void __launch_bounds__(NT)
     k_scf(const int* __restrict__ mol_list, const MolDev* __restrict__ mols, const int* __restrict__ atom_z,
           const int* __restrict__ atom_ao, const int* __restrict__ atom_pp, const PairDev* __restrict__ pairs,
-          const ElemDev* __restrict__ elem, const double* __restrict__ w, double* __restrict__ slab,
+          const ElemDev* __restrict__ elem, const double* __restrict__ ocd, const double* __restrict__ w,
+          double* __restrict__ slab,
           double* __restrict__ vecs, MolOut* __restrict__ out) {
   __shared__ double s_a[A_LDS ? MAXN * (MAXN + 1) / 2 : 1];
   __shared__ double s_v[V_LDS ? MAXN * MAXN : 1];
@@ -184,6 +182,7 @@ __global__ void __launch_bounds__(NT)
   c.atom_pp = atom_pp;
   c.pairs = pairs;
   c.elem = elem;
+  c.ocd = ocd;
   c.w = w;
   c.ld = c.M.ld;
   c.base = slab + c.M.mat0;
@@ -562,7 +561,8 @@ __global__ void __launch_bounds__(NT)
 // accumulator in LDS; <= 196 keeps the packed matrix in LDS; larger molecules run from L2/HBM.
 #define PM7_SCF_INSTANCE(NT, MAXN, AL, VL)                                                                        \
   template __global__ void k_scf<NT, MAXN, AL, VL>(const int*, const MolDev*, const int*, const int*, const int*, \
-                                                   const PairDev*, const ElemDev*, const double*, double*, double*, \
+                                                   const PairDev*, const ElemDev*, const double*, const double*, \
+                                                   double*, double*, \
                                                    MolOut*);
 PM7_SCF_INSTANCE(256, 64, true, true)
 PM7_SCF_INSTANCE(512, 112, true, true)
@@ -572,8 +572,8 @@ PM7_SCF_INSTANCE(512, 196, true, false)
 __global__ void __launch_bounds__(256) k_fock_once(int mi, const MolDev* __restrict__ mols, const int* __restrict__ atom_z,
                                                     const int* __restrict__ atom_ao, const int* __restrict__ atom_pp,
                                                     const PairDev* __restrict__ pairs, const ElemDev* __restrict__ elem,
-                                                    const double* __restrict__ w, double* __restrict__ slab, int pt_slot,
-                                                    int ps_slot, int f_slot) {
+                                                    const double* __restrict__ ocd, const double* __restrict__ w,
+                                                    double* __restrict__ slab, int pt_slot, int ps_slot, int f_slot) {
   MolCtx c;
   c.M = mols[mi];
   c.atom_z = atom_z;
@@ -581,6 +581,7 @@ __global__ void __launch_bounds__(256) k_fock_once(int mi, const MolDev* __restr
   c.atom_pp = atom_pp;
   c.pairs = pairs;
   c.elem = elem;
+  c.ocd = ocd;
   c.w = w;
   c.ld = c.M.ld;
   c.base = slab + c.M.mat0;
@@ -594,15 +595,15 @@ hipError_t launch_scf(int variant, const BatchDev& b, int list_off, int count, h
   switch (variant) {
     case 0:
       hipLaunchKernelGGL((k_scf<256, 64, true, true>), dim3(count), dim3(256), 0, s, list, b.mols, b.atom_z,
-                         b.atom_ao, b.atom_pp, b.pairs, b.elem, b.w, b.slab, b.vecs, b.out);
+                         b.atom_ao, b.atom_pp, b.pairs, b.elem, b.ocd, b.w, b.slab, b.vecs, b.out);
       break;
     case 1:
       hipLaunchKernelGGL((k_scf<512, 112, true, true>), dim3(count), dim3(512), 0, s, list, b.mols, b.atom_z,
-                         b.atom_ao, b.atom_pp, b.pairs, b.elem, b.w, b.slab, b.vecs, b.out);
+                         b.atom_ao, b.atom_pp, b.pairs, b.elem, b.ocd, b.w, b.slab, b.vecs, b.out);
       break;
     case 2:
       hipLaunchKernelGGL((k_scf<512, 196, true, false>), dim3(count), dim3(512), 0, s, list, b.mols, b.atom_z,
-                         b.atom_ao, b.atom_pp, b.pairs, b.elem, b.w, b.slab, b.vecs, b.out);
+                         b.atom_ao, b.atom_pp, b.pairs, b.elem, b.ocd, b.w, b.slab, b.vecs, b.out);
       break;
     default:
       return hipErrorInvalidValue;  // the large class runs large_scf, not a megakernel
@@ -612,7 +613,7 @@ hipError_t launch_scf(int variant, const BatchDev& b, int list_off, int count, h
 
 hipError_t launch_fock_once(const BatchDev& b, int mol, int pt_slot, int ps_slot, int f_slot, hipStream_t s) {
   hipLaunchKernelGGL(k_fock_once, dim3(1), dim3(256), 0, s, mol, b.mols, b.atom_z, b.atom_ao, b.atom_pp, b.pairs,
-                     b.elem, b.w, b.slab, pt_slot, ps_slot, f_slot);
+                     b.elem, b.ocd, b.w, b.slab, pt_slot, ps_slot, f_slot);
   return hipGetLastError();
 }
 

@@ -54,6 +54,55 @@ def test_core_hamiltonian_matches_oracle(eng, name):
     assert np.max(np.abs(h - ho)) <= 1e-9
 
 
+# Molecules with a d-bearing atom: every pair through the MNDO/d kernels (spd rotation, multipole
+# integrals, d-balance for transition metals, spd overlaps).
+D_PATH = ["hydrogen_sulfide", "sulfur_hexafluoride", "thiophene", "titanium_tetrachloride", "permanganate",
+          "chromium_hexacarbonyl", "ferrocene", "tetrachloroplatinate", "titanocene_dichloride", "iodomethane"]
+
+
+@pytest.mark.parametrize("name", D_PATH)
+def test_core_hamiltonian_d_path_matches_oracle(eng, name):
+    say(f"hcore d {name}")
+    c = CASES[name]
+    h = eng.core_hamiltonian(mol_of(c["atoms"], c["charge"], c["multiplicity"]))
+    z, xyz = ob.atoms_to_arrays(c["atoms"])
+    ho = ob.hcore(z, xyz)
+    assert h.shape == ho.shape
+    assert np.max(np.abs(h - ho)) <= 1e-9
+
+
+@pytest.mark.parametrize("name", ["hydrogen_sulfide", "sulfur_hexafluoride", "ferrocene", "permanganate"])
+def test_fock_build_d_path_matches_oracle(eng, name):
+    say(f"fock d {name}")
+    c = CASES[name]
+    z, xyz = ob.atoms_to_arrays(c["atoms"])
+    n = ob.nao(z)
+    rng = np.random.default_rng(11)
+    a = rng.normal(size=(n, n)) * 0.3
+    pt = a + a.T
+    b = rng.normal(size=(n, n)) * 0.2
+    ps = b + b.T
+    f = eng.fock_spin(mol_of(c["atoms"], c["charge"], c["multiplicity"]), pt, ps)
+    fo = ob.fock(z, xyz, pt, ps)
+    assert np.max(np.abs(f - fo)) <= 1e-9
+
+
+@pytest.mark.parametrize("name", ["thiophene", "ferrocene", "iron_pentacarbonyl", "chlorine_dioxide",
+                                  "tetrachloroplatinate", "perrhenate"])
+def test_engine_d_path_matches_oracle_density(eng, name):
+    say(f"scf d {name}")
+    c = CASES[name]
+    r = eng.run(mol_of(c["atoms"], c["charge"], c["multiplicity"]),
+                pm7amd.Pm7Options(charge=c["charge"], multiplicity=c["multiplicity"]))
+    res, dens, spin, eps, q = oracle_of(c["atoms"], c["charge"], c["multiplicity"])
+    assert abs(r.total_ev - res.total_ev) <= 1e-6
+    assert abs(r.heat_of_formation_kcal - res.hof_kcal) <= 1e-4
+    assert np.max(np.abs(r.density - dens)) <= 1e-7
+    assert np.max(np.abs(np.array(r.dipole_pd_hybrid_debye) - np.array(res.dipole_pd[:]))) <= 1e-5
+    if r.unrestricted:
+        assert np.max(np.abs(r.spin_density - spin)) <= 1e-7
+
+
 @pytest.mark.parametrize("name", ["water", "formic_acid_dimer", "alkane_c20", "water_wire_8"])
 def test_fock_build_matches_oracle(eng, name):
     say(f"fock {name}")
@@ -231,12 +280,14 @@ def test_error_paths_are_reported_per_molecule(eng):
     say("error paths")
     water = mol_of(W.water())
     coincident = mol_of([["O", 0, 0, 0], ["H", 0, 0, 0], ["H", 0.96, 0, 0]])
-    sulfur = mol_of([["S", 0, 0, 0], ["H", 1.34, 0, 0], ["H", -0.35, 1.29, 0]])
+    sulfur = mol_of(CASES["hydrogen_sulfide"]["atoms"])
     odd = mol_of([["O", 0, 0, 0], ["H", 0.97, 0, 0]])
-    res = eng.run_batch([water, coincident, sulfur, odd, water],
+    europium = mol_of([["Eu", 0, 0, 0], ["F", 2.0, 0, 0]])  # a sparkle: no orbitals, no SCF here
+    res = eng.run_batch([water, coincident, sulfur, odd, water, europium],
                         [pm7amd.Pm7Options(), pm7amd.Pm7Options(), pm7amd.Pm7Options(), pm7amd.Pm7Options(),
-                         pm7amd.Pm7Options(max_scf=1)])
-    assert [r.status for r in res] == [0, 1, 4, 1, 6]
+                         pm7amd.Pm7Options(max_scf=1), pm7amd.Pm7Options()])
+    assert [r.status for r in res] == [0, 1, 0, 1, 6, 2]
+    assert res[2].total_ev == pytest.approx(CASES["hydrogen_sulfide"]["energy_ev"], abs=1e-6)
     assert np.isfinite(res[4].total_ev) and res[4].iterations == 1
     assert res[0].total_ev == pytest.approx(oracle_of(W.water())[0].total_ev, abs=1e-6)
 

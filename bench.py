@@ -201,7 +201,7 @@ def main() -> int:
     serving = None
     if args.serving > 0:
         import threading
-        stream = W.serving_stream(args.serving, seed=5, rate_per_s=1.0, frac_sp=0.8, frac_d=0.0)
+        stream = W.serving_stream(args.serving, seed=5, rate_per_s=1.0, frac_sp=0.7, frac_d=0.2)
         mols_s = [(pm7amd.Molecule.from_symbols_angstrom(a, c, m), c, m) for _, a, c, m, _ in stream]
         gaps = [t for t, *_ in stream]  # arrival times at rate 1/s; scaled below
         reqs = [(0.0, mol, c, m) for mol, c, m in mols_s]
@@ -240,8 +240,8 @@ def main() -> int:
         serving = {"requests": len(reqs), "capacity_fp64_solves_per_s": capacity["solves_per_s"],
                    "rate_per_s": round(rate, 1), "lanes": args.serve_lanes,
                    "max_batch": args.serve_batch,
-                   "mix": "80% closed-shell H/C/N/O (3-60 atoms), 20% UHF doublets (one H removed); "
-                          "S/P/Cl requests need the MNDO/d kernel (not built) and are left out",
+                   "mix": "70% closed-shell H/C/N/O (3-60 atoms), 20% S/P/Cl molecules of 10-200 atoms "
+                          "(MNDO/d pairs), 10% UHF doublets (one H removed)",
                    "fp64": serve("fp64"), "mixed": serve("mixed")}
         print(f"[bench] serving: {serving['fp64']} / {serving['mixed']}", file=sys.stderr, flush=True)
 

@@ -242,6 +242,31 @@ def water_cluster(n_water: int = 500, seed: int = 4, density_g_cm3: float = 1.0,
     return atoms
 
 
+def hetero_chain(rng: Rng, n_carbon: int, hetero: str):
+    """A jittered all-trans alkane with one heteroatom: Cl in place of a terminal H (C-Cl 1.78 A),
+    or an interior CH2 turned into S (thioether, both H removed) or into PH (one H removed).
+    3 n_carbon + 2 atoms for Cl, one or two fewer for P and S."""
+    atoms = alkane_chain(n_carbon, seed=rng.randint(1, 1 << 30), jitter=0.02)
+    n = n_carbon
+
+    def hs(i):  # hydrogens of carbon i (appended after the carbons, in carbon order)
+        if i == 0:
+            return [n, n + 1, n + 2]
+        s0 = n + 3 + 2 * (i - 1)
+        return [s0, s0 + 1, s0 + 2] if i == n - 1 else [s0, s0 + 1]
+
+    if hetero == "Cl":
+        h = hs(0)[0]
+        c = atoms[0][1:]
+        d = _unit([atoms[h][1 + k] - c[k] for k in range(3)])
+        atoms[h] = ["Cl"] + [c[k] + 1.78 * d[k] for k in range(3)]
+        return atoms
+    k = rng.randint(1, n - 2)
+    drop = set(hs(k) if hetero == "S" else hs(k)[:1])
+    atoms[k][0] = hetero
+    return [a for j, a in enumerate(atoms) if j not in drop]
+
+
 def remove_one_h(atoms):
     """A doublet radical: the last hydrogen removed."""
     idx = max(i for i, a in enumerate(atoms) if a[0] == "H")
@@ -252,8 +277,9 @@ def serving_stream(n_requests: int = 2000, seed: int = 5, rate_per_s: float = 10
                    frac_sp: float = 0.7, frac_d: float = 0.2):
     """Poisson arrivals of (arrival_s, atoms, charge, multiplicity, kind) requests.
 
-    kind "sp": closed-shell H/C/N/O as in organic_batch; "d": a molecule of 10-200 atoms holding
-    S, P or Cl (needs the MNDO/d kernel); "uhf": an sp molecule with one H removed (doublet).
+    kind "sp": closed-shell H/C/N/O as in organic_batch; "d": a molecule of 12-200 atoms holding
+    S, P or Cl (every pair through the MNDO/d kernels; `hetero_chain` with 4-66 carbons); "uhf":
+    an sp molecule with one H removed (doublet).
     """
     rng = Rng(seed)
     t = 0.0
@@ -270,13 +296,8 @@ def serving_stream(n_requests: int = 2000, seed: int = 5, rate_per_s: float = 10
             else:
                 out.append((t, remove_one_h(mol), 0.0, 2, "uhf"))
         else:
-            mol = None
-            while mol is None:
-                mol = random_organic(rng, rng.randint(3, 50), 200)
-            heavy = [i for i, a in enumerate(mol) if a[0] == "C"]
-            k = heavy[rng.randint(0, len(heavy) - 1)]
-            mol = [list(a) for a in mol]
-            mol[k][0] = ("S", "P", "Cl")[rng.randint(0, 2)]
+            n_carbon = rng.randint(4, 66)
+            mol = hetero_chain(rng, n_carbon, ("S", "P", "Cl")[rng.randint(0, 2)])
             out.append((t, mol, 0.0, 1, "d"))
     return out
 

@@ -29,7 +29,8 @@ def eng():
 
 
 def _stream(n, seed):
-    reqs = W.serving_stream(n, seed=seed, rate_per_s=2000.0, frac_sp=0.8, frac_d=0.0)
+    # the full request mix: closed-shell H/C/N/O, S/P/Cl molecules (MNDO/d pairs), UHF doublets
+    reqs = W.serving_stream(n, seed=seed, rate_per_s=2000.0, frac_sp=0.7, frac_d=0.2)
     return [(t, pm7amd.Molecule.from_symbols_angstrom(a, c, m), pm7amd.Pm7Options(charge=c, multiplicity=m))
             for t, a, c, m, _ in reqs]
 

@@ -15,6 +15,7 @@
 #include <condition_variable>
 #include <deque>
 #include <memory>
+#include <atomic>
 #include <thread>
 #include <cmath>
 #include <cstdio>
@@ -895,12 +896,19 @@ struct SrvReq {
 };
 
 // One batch in flight on its own lane.
+// One batch in flight on its own lane. The batch runs in its own thread (upload, integrals, the
+// megakernels, the grid-wide large-system SCFs, which block on their host control loop, and the
+// result assembly), so the scheduler keeps admitting work to the other lanes meanwhile.
 struct Flight {
   Lane lane;
   Prepared P;
   std::vector<int64_t> ids;
   size_t bytes = 0;
   bool busy = false;
+  std::thread thr;
+  std::atomic<int> done{0};  // 1: results (or the error) are ready for the scheduler
+  pm7_status st = PM7_OK;
+  std::vector<pm7_result_v1> res;
 };
 
 }  // namespace
@@ -942,27 +950,26 @@ void srv_loop(pm7_server* s) {
     for (auto& fp : s->flights) {
       Flight& f = *fp;
       if (!f.busy) continue;
-      const hipError_t q = hipEventQuery(f.lane.ev[5]);
-      if (q == hipErrorNotReady) {
+      if (!f.done.load(std::memory_order_acquire)) {
         busy = true;
         continue;
       }
-      std::vector<pm7_result_v1> res(f.P.jobs.size());
-      for (auto& r : res) std::memset(&r, 0, sizeof r);
-      lk.unlock();
-      pm7_status st = q == hipSuccess ? collect_batch(ctx, f.lane, f.P, res.data()) : PM7_ERR_DEVICE;
-      lk.lock();
+      f.thr.join();
       const auto now = std::chrono::steady_clock::now();
       for (size_t k = 0; k < f.ids.size(); ++k) {
         SrvReq& r = s->reqs[f.ids[k]];
-        r.res = res[k];
-        if (st != PM7_OK) {
-          r.res.status = st;
+        if (f.st == PM7_OK) {
+          r.res = f.res[k];
+        } else {
+          std::memset(&r.res, 0, sizeof r.res);
+          r.res.status = f.st;
+          r.res.n_atoms = (int32_t)r.job.z.size();
           set_msg(&r.res, f.lane.err.empty() ? std::string("device error") : f.lane.err);
         }
         srv_finish(s, f.ids[k], now);
       }
       f.busy = false;
+      f.done.store(0);
       f.ids.clear();
       s->inflight -= f.bytes;
       s->cv_done.notify_all();
@@ -975,7 +982,10 @@ void srv_loop(pm7_server* s) {
       std::vector<int64_t> ids;
       size_t bytes = 0;
       const int cls = s->reqs[s->queue.front()].job.variant;
-      for (auto it = s->queue.begin(); it != s->queue.end() && (int)ids.size() < s->cfg.max_batch;) {
+      // A large-system SCF owns its lane's stream for its whole run: one per batch, so several
+      // run side by side on different lanes.
+      const int cap = cls == LARGE_VARIANT ? 1 : s->cfg.max_batch;
+      for (auto it = s->queue.begin(); it != s->queue.end() && (int)ids.size() < cap;) {
         SrvReq& r = s->reqs[*it];
         if (r.job.variant != cls) {
           ++it;
@@ -999,31 +1009,30 @@ void srv_loop(pm7_server* s) {
       f.bytes = bytes;
       s->inflight += bytes;
       f.busy = true;
-      lk.unlock();
-      int count[SCF_VARIANTS];
-      pm7_status st = prepare(ctx, f.lane, f.P, 0);
-      if (st == PM7_OK) st = launch_batch(f.lane, f.P, count);
-      if (st == PM7_OK) st = run_large(f.lane, f.P);
-      if (st == PM7_OK) st = finish_batch(f.lane, f.P);
-      if (st == PM7_OK) host_corrections(f.P.jobs);
-      lk.lock();
-      if (st != PM7_OK) {
-        const auto now = std::chrono::steady_clock::now();
-        for (int64_t id : f.ids) {
-          SrvReq& r = s->reqs[id];
-          std::memset(&r.res, 0, sizeof r.res);
-          r.res.status = st;
-          r.res.n_atoms = (int32_t)r.job.z.size();
-          set_msg(&r.res, f.lane.err);
-          srv_finish(s, id, now);
+      f.done.store(0);
+      Flight* fl = &f;
+      f.thr = std::thread([ctx, s, fl] {
+        (void)hipSetDevice(ctx->device);
+        Flight& g = *fl;
+        int count[SCF_VARIANTS];
+        pm7_status st = prepare(ctx, g.lane, g.P, 0);
+        if (st == PM7_OK) st = launch_batch(g.lane, g.P, count);
+        if (st == PM7_OK) st = run_large(g.lane, g.P);
+        if (st == PM7_OK) st = finish_batch(g.lane, g.P);
+        if (st == PM7_OK) host_corrections(g.P.jobs);
+        g.res.assign(g.P.jobs.size(), pm7_result_v1{});
+        for (auto& r : g.res) std::memset(&r, 0, sizeof r);
+        if (st == PM7_OK)
+          st = hipEventSynchronize(g.lane.ev[5]) == hipSuccess ? collect_batch(ctx, g.lane, g.P, g.res.data())
+                                                                : PM7_ERR_DEVICE;
+        g.st = st;
+        {
+          std::lock_guard<std::mutex> wl(s->mu);
+          g.done.store(1, std::memory_order_release);
         }
-        f.busy = false;
-        f.ids.clear();
-        s->inflight -= f.bytes;
-        s->cv_done.notify_all();
-      } else {
-        busy = true;
-      }
+        s->cv_work.notify_all();
+      });
+      busy = true;
     }
     if (s->stopping && s->queue.empty() && !busy) break;
     if (busy) s->cv_work.wait_for(lk, std::chrono::microseconds(50));
@@ -1148,7 +1157,10 @@ void pm7_server_stop(pm7_server* s) {
   if (s->worker.joinable()) s->worker.join();
   (void)hipSetDevice(s->ctx->device);
   for (auto& kv : s->reqs) pm7_result_free(&kv.second.res);
-  for (auto& f : s->flights) f->lane.destroy();
+  for (auto& f : s->flights) {
+    if (f->thr.joinable()) f->thr.join();
+    f->lane.destroy();
+  }
   delete s;
 }
 

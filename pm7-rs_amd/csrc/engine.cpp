@@ -141,9 +141,14 @@ struct Lane {
   DevBuf d_w, d_slab, d_vecs, d_out, d_ecore, d_ecore_mol, d_gath, d_num, d_snum, d_dpairs;
   PinnedBuf h_out;  // staging for the packed results
   LargeWork* large = nullptr;  // rocBLAS handle and buffers of the large-system path (lazily made)
+  // Several large-system SCFs in lockstep: one work set and one stream each (lazily made).
+  std::vector<LargeWork*> group_work;
+  std::vector<hipStream_t> group_st;
+  int device = 0;
   std::string err;
   bool create(bool one_stream = false) {
     single_stream = one_stream;
+    if (hipGetDevice(&device) != hipSuccess) return false;
     for (int v = 0; v < (single_stream ? 1 : SCF_VARIANTS); ++v)
       if (hipStreamCreateWithFlags(&st[v], hipStreamNonBlocking) != hipSuccess) return false;
     for (int v = 1; single_stream && v < SCF_VARIANTS; ++v) st[v] = st[0];
@@ -163,6 +168,13 @@ struct Lane {
     h_out.release();
     if (large) large_work_destroy(large);
     large = nullptr;
+    for (size_t k = 0; k < group_work.size(); ++k) {
+      if (group_st[k]) (void)hipStreamSynchronize(group_st[k]);
+      large_work_destroy(group_work[k]);
+      if (group_st[k]) (void)hipStreamDestroy(group_st[k]);
+    }
+    group_work.clear();
+    group_st.clear();
     for (int v = 0; v < (single_stream ? 1 : SCF_VARIANTS); ++v)
       if (st[v]) (void)hipStreamDestroy(st[v]);
     for (auto& e : ev)
@@ -768,27 +780,75 @@ pm7_status launch_batch(Lane& L, const Prepared& P, int* count) {
 pm7_status run_large(Lane& L, const Prepared& P) {
   const int off = P.list_off[LARGE_VARIANT], cnt = P.list_off[LARGE_VARIANT + 1] - off;
   if (cnt == 0) return PM7_OK;
-  if (!L.large) {
-    L.large = large_work_create();
-    if (!L.large) {
-      L.err = "rocBLAS handle creation failed";
-      return PM7_ERR_DEVICE;
-    }
-  }
-  hipStream_t s = L.st[LARGE_VARIANT];
-  LANECHK(hipEventRecord(L.ev_var[2 * LARGE_VARIANT], s));
   std::vector<std::pair<int, int>> order;  // largest first
   for (int mi = 0; mi < (int)P.jobs.size(); ++mi)
     if (P.jobs[mi].variant == LARGE_VARIANT) order.push_back({-P.jobs[mi].nao, mi});
   std::stable_sort(order.begin(), order.end());
-  for (auto& o : order) {
-    const int mi = o.second;
-    MolOut out{};
-    if (large_scf(L.large, P.b, P.mols[mi], mi, s, out, L.err) != hipSuccess) return PM7_ERR_DEVICE;
-    LANECHK(hipMemcpyAsync(P.b.out + mi, &out, sizeof out, hipMemcpyHostToDevice, s));
-    LANECHK(hipStreamSynchronize(s));
+  // Molecules up to LARGE_GROUP_MAX_NAO run in groups, each SCF in its own thread and stream, their
+  // full eigensolves batched in lockstep; larger ones one after another.
+  size_t single = 0;
+  while (single < order.size() && -order[single].first > LARGE_GROUP_MAX_NAO) ++single;
+  if (order.size() - single < 2) single = order.size();
+  if (single > 0) {
+    if (!L.large) {
+      L.large = large_work_create();
+      if (!L.large) {
+        L.err = "rocBLAS handle creation failed";
+        return PM7_ERR_DEVICE;
+      }
+    }
+    hipStream_t s = L.st[LARGE_VARIANT];
+    LANECHK(hipEventRecord(L.ev_var[2 * LARGE_VARIANT], s));
+    for (size_t k = 0; k < single; ++k) {
+      const int mi = order[k].second;
+      MolOut out{};
+      if (large_scf(L.large, P.b, P.mols[mi], mi, s, out, L.err) != hipSuccess) return PM7_ERR_DEVICE;
+      LANECHK(hipMemcpyAsync(P.b.out + mi, &out, sizeof out, hipMemcpyHostToDevice, s));
+      LANECHK(hipStreamSynchronize(s));
+    }
+    LANECHK(hipEventRecord(L.ev_var[2 * LARGE_VARIANT + 1], s));
   }
-  LANECHK(hipEventRecord(L.ev_var[2 * LARGE_VARIANT + 1], s));
+  for (size_t g0 = single; g0 < order.size(); g0 += LARGE_GROUP) {
+    const int gn = (int)std::min<size_t>(LARGE_GROUP, order.size() - g0);
+    while ((int)L.group_work.size() < gn) {
+      LargeWork* w = large_work_create();
+      hipStream_t st = nullptr;
+      if (!w || hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+        large_work_destroy(w);
+        L.err = "large-system work set creation failed";
+        return PM7_ERR_DEVICE;
+      }
+      L.group_work.push_back(w);
+      L.group_st.push_back(st);
+    }
+    int nmax = 0;
+    for (int k = 0; k < gn; ++k) nmax = std::max(nmax, P.mols[order[g0 + k].second].ld);
+    EigBatcher* eb = eig_batcher_create(gn, nmax, L.err);
+    if (!eb) return PM7_ERR_DEVICE;
+    std::vector<std::string> errs(gn);
+    std::vector<int> bad(gn, 0);
+    std::vector<std::thread> th;
+    for (int k = 0; k < gn; ++k) {
+      if (hipStreamWaitEvent(L.group_st[k], L.ev[1], 0) != hipSuccess) bad[k] = 1;  // integrals done
+      th.emplace_back([&, k] {
+        (void)hipSetDevice(L.device);
+        const int mi = order[g0 + k].second;
+        MolOut out{};
+        hipStream_t st = L.group_st[k];
+        hipError_t r = large_scf(L.group_work[k], P.b, P.mols[mi], mi, st, out, errs[k], eb);
+        if (r == hipSuccess) r = hipMemcpyAsync(P.b.out + mi, &out, sizeof out, hipMemcpyHostToDevice, st);
+        if (r == hipSuccess) r = hipStreamSynchronize(st);
+        if (r != hipSuccess) bad[k] = 1;
+      });
+    }
+    for (auto& t : th) t.join();
+    eig_batcher_destroy(eb);
+    for (int k = 0; k < gn; ++k)
+      if (bad[k]) {
+        L.err = errs[k].empty() ? std::string("large-system SCF failed") : errs[k];
+        return PM7_ERR_DEVICE;
+      }
+  }
   return PM7_OK;
 }
 
@@ -982,9 +1042,9 @@ void srv_loop(pm7_server* s) {
       std::vector<int64_t> ids;
       size_t bytes = 0;
       const int cls = s->reqs[s->queue.front()].job.variant;
-      // A large-system SCF owns its lane's stream for its whole run: one per batch, so several
-      // run side by side on different lanes.
-      const int cap = cls == LARGE_VARIANT ? 1 : s->cfg.max_batch;
+      // Large-system SCFs of one batch run as a lockstep group (batched eigensolves), at most
+      // LARGE_GROUP of them, so further ones start on the other lanes.
+      const int cap = cls == LARGE_VARIANT ? std::min(LARGE_GROUP, s->cfg.max_batch) : s->cfg.max_batch;
       for (auto it = s->queue.begin(); it != s->queue.end() && (int)ids.size() < cap;) {
         SrvReq& r = s->reqs[*it];
         if (r.job.variant != cls) {

@@ -53,6 +53,10 @@ constexpr int GL_WORDS = 2 * 48 + 2 * 40;
 constexpr int SCF_VARIANTS = 4;
 constexpr int VARIANT_MAX_NAO[SCF_VARIANTS] = {64, 112, 196, 1 << 30};
 constexpr int LARGE_VARIANT = 3;  // above 196 basis functions: the grid-wide path (large.hip)
+// Large-class molecules up to this basis size run in groups of up to LARGE_GROUP SCFs whose full
+// eigensolves are batched (one strided rocSOLVER call per iteration for the whole group).
+constexpr int LARGE_GROUP_MAX_NAO = 512;
+constexpr int LARGE_GROUP = 16;
 
 hipError_t launch_integrals(const BatchDev& b, int64_t n_pairs, int n_items, int n_mols, hipStream_t s);
 hipError_t launch_scf(int variant, const BatchDev& b, int list_off, int count, hipStream_t s);
@@ -64,7 +68,12 @@ hipError_t launch_fock_once(const BatchDev& b, int mol, int pt_slot, int ps_slot
 struct LargeWork;
 LargeWork* large_work_create();
 void large_work_destroy(LargeWork* w);
+// Several large-system SCFs in their own threads can share a batched eigensolve: created for
+// `participants` SCFs of padded size up to nmax; each passes it to large_scf.
+struct EigBatcher;
+EigBatcher* eig_batcher_create(int participants, int nmax, std::string& err);
+void eig_batcher_destroy(EigBatcher* b);
 hipError_t large_scf(LargeWork* w, const BatchDev& b, const MolDev& M, int mi, hipStream_t s, MolOut& out,
-                     std::string& err);
+                     std::string& err, EigBatcher* eb = nullptr);
 
 }  // namespace pm7

@@ -22,8 +22,11 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <condition_variable>
 #include <cstddef>
+#include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -227,6 +230,29 @@ __global__ void __launch_bounds__(256) k_lg_to_f64(int64_t count, const float* _
 }
 __global__ void __launch_bounds__(256) k_lg_copy(int64_t count, const double* __restrict__ src, double* __restrict__ dst) {
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256) dst[i] = src[i];
+}
+// W (n x n of leading dimension ld) into an nmax x nmax slot of the batched eigensolve; the pad
+// diagonal gets 1e5 + i, far above any Fock eigenvalue, so the pad eigenpairs sort last and stay
+// decoupled from the molecule's own (the pad block has no coupling to it).
+__global__ void __launch_bounds__(256) k_lg_pad(int n, int ld, int nmax, const double* __restrict__ W,
+                                                double* __restrict__ dst) {
+  const int64_t nn = (int64_t)nmax * nmax;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < nn; idx += (int64_t)gridDim.x * 256) {
+    const int j = (int)(idx / nmax), i = (int)(idx - (int64_t)j * nmax);
+    dst[idx] = (i < n && j < n) ? W[(int64_t)j * ld + i] : (i == j ? 1.0e5 + i : 0.0);
+  }
+}
+// The first n eigenpairs of a slot back into the molecule's orbitals (Ct, column j = orbital j)
+// and eigenvalues; pad rows/columns of Ct are zero.
+__global__ void __launch_bounds__(256) k_lg_unpad(int n, int ld, int nmax, const double* __restrict__ V,
+                                                  const double* __restrict__ D, double* __restrict__ Ct,
+                                                  double* __restrict__ eps) {
+  const int64_t nn = (int64_t)ld * ld;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < nn; idx += (int64_t)gridDim.x * 256) {
+    const int j = (int)(idx / ld), i = (int)(idx - (int64_t)j * ld);
+    Ct[idx] = (i < n && j < n) ? V[(int64_t)j * nmax + i] : 0.0;
+    if (i == 0 && j < n) eps[j] = D[j];
+  }
 }
 // dst = src^T over the ld x ld square (pad rows/columns of src are zero).
 __global__ void __launch_bounds__(256) k_lg_transpose(int ld, const double* __restrict__ src, double* __restrict__ dst) {
@@ -434,6 +460,7 @@ void large_work_destroy(LargeWork* w) {
   delete w;
 }
 
+
 namespace {
 
 #define LG(expr)                                                           \
@@ -457,8 +484,132 @@ int grid_for(int64_t items) { return (int)std::min<int64_t>(std::max<int64_t>((i
 
 }  // namespace
 
+// Lockstep batching of the full eigensolves of several large-system SCFs running in their own
+// threads: every participant hands in its matrices for the round (or none), the last one to
+// arrive pads them into one strided array and runs one rocSOLVER dsyevd_strided_batched, whose
+// time for 16 matrices of a few hundred rows is close to that of one.
+struct EigBatcher {
+  struct Item {
+    const double* W;
+    int n, ld;
+    double* eps;
+    double* Ct;
+  };
+  std::mutex mu;
+  std::condition_variable cv;
+  int active = 0, arrived = 0;
+  uint64_t round = 0;
+  hipError_t status = hipSuccess;
+  std::vector<Item> items;
+  std::vector<hipEvent_t> ready;
+  rocblas_handle blas = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t done = nullptr;
+  Buf work, D, E, info;
+  int nmax = 0, cap = 0;
+
+  // Called with the lock held by the last participant of a round.
+  void run_round() {
+    if (!items.empty() && status == hipSuccess) {
+      const int cnt = (int)items.size();
+      const int64_t nn = (int64_t)nmax * nmax;
+      double* wk = static_cast<double*>(work.p);
+      for (int k = 0; k < cnt && status == hipSuccess; ++k) {
+        status = hipStreamWaitEvent(st, ready[k], 0);
+        hipLaunchKernelGGL(k_lg_pad, dim3(grid_for(nn)), dim3(256), 0, st, items[k].n, items[k].ld, nmax, items[k].W,
+                           wk + k * nn);
+      }
+      if (status == hipSuccess &&
+          rocsolver_dsyevd_strided_batched(blas, rocblas_evect_original, rocblas_fill_upper, nmax, wk, nmax, nn,
+                                           static_cast<double*>(D.p), nmax, static_cast<double*>(E.p), nmax,
+                                           static_cast<rocblas_int*>(info.p), cnt) != rocblas_status_success)
+        status = hipErrorUnknown;
+      for (int k = 0; k < cnt && status == hipSuccess; ++k) {
+        const int64_t sq = (int64_t)items[k].ld * items[k].ld;
+        hipLaunchKernelGGL(k_lg_unpad, dim3(grid_for(sq)), dim3(256), 0, st, items[k].n, items[k].ld, nmax,
+                           wk + k * nn, static_cast<double*>(D.p) + (int64_t)k * nmax, items[k].Ct, items[k].eps);
+      }
+      if (status == hipSuccess) status = hipGetLastError();
+      if (status == hipSuccess) status = hipEventRecord(done, st);
+    }
+    items.clear();
+    arrived = 0;
+    ++round;
+    cv.notify_all();
+  }
+
+  // This participant's matrices for the round (k = 0: none), then wait for the round; the
+  // caller's stream is ordered after the batched solve.
+  hipError_t submit(const Item* its, int k, hipStream_t s) {
+    std::unique_lock<std::mutex> lk(mu);
+    for (int q = 0; q < k; ++q) {
+      if ((int)items.size() >= cap) {
+        status = hipErrorInvalidValue;
+        break;
+      }
+      const int slot = (int)items.size();
+      items.push_back(its[q]);
+      if (hipEventRecord(ready[slot], s) != hipSuccess) status = hipErrorUnknown;
+    }
+    const uint64_t mine = round;
+    if (++arrived == active) run_round();
+    else cv.wait(lk, [&] { return round != mine; });
+    const hipError_t st_ = status;
+    lk.unlock();
+    if (k > 0 && st_ == hipSuccess) return hipStreamWaitEvent(s, done, 0);
+    return st_;
+  }
+
+  void leave() {
+    std::lock_guard<std::mutex> lk(mu);
+    --active;
+    if (active > 0 && arrived == active) run_round();
+  }
+};
+
+EigBatcher* eig_batcher_create(int participants, int nmax, std::string& err) {
+  auto* b = new EigBatcher();
+  b->active = participants;
+  b->nmax = nmax;
+  b->cap = 2 * participants;  // UHF hands in alpha and beta
+  bool ok = rocblas_create_handle(&b->blas) == rocblas_status_success && hipStreamCreate(&b->st) == hipSuccess &&
+            rocblas_set_stream(b->blas, b->st) == rocblas_status_success && hipEventCreate(&b->done) == hipSuccess;
+  b->ready.assign(b->cap, nullptr);
+  for (auto& e : b->ready) ok = ok && hipEventCreate(&e) == hipSuccess;
+  ok = ok && b->work.ensure((size_t)b->cap * nmax * nmax * sizeof(double)) == hipSuccess &&
+       b->D.ensure((size_t)b->cap * nmax * sizeof(double)) == hipSuccess &&
+       b->E.ensure((size_t)b->cap * nmax * sizeof(double)) == hipSuccess &&
+       b->info.ensure((size_t)b->cap * sizeof(rocblas_int)) == hipSuccess;
+  if (!ok) {
+    err = "batched eigensolver set-up failed";
+    eig_batcher_destroy(b);
+    return nullptr;
+  }
+  return b;
+}
+
+void eig_batcher_destroy(EigBatcher* b) {
+  if (!b) return;
+  if (b->st) (void)hipStreamSynchronize(b->st);
+  if (b->blas) rocblas_destroy_handle(b->blas);
+  for (auto& e : b->ready)
+    if (e) (void)hipEventDestroy(e);
+  if (b->done) (void)hipEventDestroy(b->done);
+  if (b->st) (void)hipStreamDestroy(b->st);
+  delete b;
+}
+
 hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, hipStream_t s, MolOut& out,
-                     std::string& err) {
+                     std::string& err, EigBatcher* eb) {
+  // A participant of a batched eigensolve leaves the group when its SCF loop ends (or fails).
+  struct Leave {
+    EigBatcher* e;
+    void now() {
+      if (e) e->leave();
+      e = nullptr;
+    }
+    ~Leave() { now(); }
+  } leave{eb};
   const int n = M.nao, ld = M.ld;
   const int64_t sq = (int64_t)ld * ld, nn = (int64_t)n * n;
   double* base = b.slab + M.mat0;
@@ -524,6 +675,19 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
     hipLaunchKernelGGL(k_lg_copy, dim3(grid_for(sq)), dim3(256), 0, s, sq, W, Ct);
     return hipGetLastError();
   };
+  // One SCF iteration's full eigensolves (one RHF matrix, or alpha and beta): through the group's
+  // batched solve when this SCF runs in one, else directly.
+  auto eig_round = [&](int k, double* W0, double* e0, double* C0, double* W1, double* e1, double* C1) -> hipError_t {
+    if (!leave.e) {
+      hipError_t e_ = eig_full(W0, e0, C0);
+      if (e_ == hipSuccess && k > 1) e_ = eig_full(W1, e1, C1);
+      return e_;
+    }
+    const EigBatcher::Item its[2] = {{W0, n, ld, e0, C0}, {W1, n, ld, e1, C1}};
+    return leave.e->submit(its, k, s);
+  };
+  // A round in which this SCF needs no full eigensolve (orbital rotations instead).
+  auto eig_skip = [&]() -> hipError_t { return leave.e ? leave.e->submit(nullptr, 0, s) : hipSuccess; };
   // Occupied-virtual rotation of Ct (the reduced-precision phase near convergence).
   auto eig_rotate = [&](const double* W, int nocc, double* Ct) -> hipError_t {
     double* T1 = slot(S_T);
@@ -607,8 +771,12 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
       LG(hipGetLastError());
       LG(hipEventRecord(lw->ev[3], s));
       double* eps = b.vecs + M.vec0;
-      if (low && iter > 0 && dp_prev < ROTATE_BELOW) LG(eig_rotate(slot(S_W), nocc, slot(cslot_c)));
-      else LG(eig_full(slot(S_W), eps, slot(cslot_c)));
+      if (low && iter > 0 && dp_prev < ROTATE_BELOW) {
+        LG(eig_skip());
+        LG(eig_rotate(slot(S_W), nocc, slot(cslot_c)));
+      } else {
+        LG(eig_round(1, slot(S_W), eps, slot(cslot_c), nullptr, nullptr, nullptr));
+      }
       LG(hipEventRecord(lw->ev[4], s));
       double* Pn = slot(S_HD + head_next);
       LG(density(slot(cslot_c), nocc, 2.0, Pn, low != 0));
@@ -628,6 +796,7 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
         break;
       }
     }
+    leave.now();
     // Final energy and (with a level shift) the unshifted orbitals from the converged density.
     const int hp = iter % HIST;
     double* P = slot(S_HD + hp);
@@ -689,11 +858,11 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
       hipLaunchKernelGGL(k_lg_combine, dim3(grid_for(nn)), dim3(256), 0, s, n, ld, ctl, slot(S_HF), Fb, Pb, 1.0, Fb, 0);
       LG(hipEventRecord(lw->ev[3], s));
       if (low && iter > 0 && dp_prev < ROTATE_BELOW) {
+        LG(eig_skip());
         LG(eig_rotate(Fa, na, slot(U_CA0)));
         LG(eig_rotate(Fb, nb, slot(U_CB0)));
       } else {
-        LG(eig_full(Fa, eps_a, slot(U_CA0)));
-        LG(eig_full(Fb, eps_b, slot(U_CB0)));
+        LG(eig_round(2, Fa, eps_a, slot(U_CA0), Fb, eps_b, slot(U_CB0)));
       }
       LG(hipEventRecord(lw->ev[4], s));
       LG(density(slot(U_CA0), na, 1.0, Pan, low != 0));
@@ -718,6 +887,7 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
         break;
       }
     }
+    leave.now();
     hipLaunchKernelGGL(k_lg_copy, dim3(grid_for(sq)), dim3(256), 0, s, sq, Pa, Pt);
     LGB(rocblas_daxpy(lw->blas, (rocblas_int)sq, &one, Pb, 1, Pt, 1));
     hipLaunchKernelGGL(k_lg_pack<double>, dim3(grid_for(M.n_pp)), dim3(256), 0, s, mi, b, Pt);

@@ -239,6 +239,27 @@ def test_large_path_water_cluster_matches_oracle(eng, n_water, precision):
     assert np.max(np.abs(2.0 * occ @ occ.T - r.density)) <= 1e-9
 
 
+def test_large_group_matches_single_runs(eng):
+    """Large-class molecules (197-512 basis functions) of one batch run as a lockstep group with
+    batched eigensolves; each result equals the molecule's own run to the parity bars, and one of
+    them matches the oracle."""
+    rng = W.Rng(21)
+    atoms = [W.hetero_chain(rng, nc, het) for nc, het in ((34, "Cl"), (40, "S"), (45, "P"), (52, "Cl"))]
+    say("large group of %d" % len(atoms))
+    mols = [mol_of(a) for a in atoms]
+    group = eng.run_batch(mols, [pm7amd.Pm7Options()] * len(mols))
+    for m, g in zip(mols, group):
+        alone = eng.run(m, pm7amd.Pm7Options())
+        assert g.status == 0 and g.converged
+        assert abs(g.total_ev - alone.total_ev) <= 1e-6
+        assert abs(g.heat_of_formation_kcal - alone.heat_of_formation_kcal) <= 1e-4
+        assert np.max(np.abs(g.density - alone.density)) <= 1e-7
+    res, dens, spin, eps, q = oracle_of(atoms[0])
+    assert abs(group[0].total_ev - res.total_ev) <= 1e-6
+    assert abs(group[0].heat_of_formation_kcal - res.hof_kcal) <= 1e-4
+    assert np.max(np.abs(group[0].density - dens)) <= 1e-7
+
+
 def test_config1_water(eng):
     say("config 1 water")
     atoms = W.water()

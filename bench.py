@@ -203,20 +203,22 @@ def main() -> int:
         import threading
         stream = W.serving_stream(args.serving, seed=5, rate_per_s=1.0, frac_sp=0.7, frac_d=0.2)
         mols_s = [(pm7amd.Molecule.from_symbols_angstrom(a, c, m), c, m) for _, a, c, m, _ in stream]
+        kinds = [k for *_, k in stream]
         gaps = [t for t, *_ in stream]  # arrival times at rate 1/s; scaled below
         reqs = [(0.0, mol, c, m) for mol, c, m in mols_s]
 
         def serve(precision):
             lat, bad = [], 0
+            by_kind, kind_of, got = {}, {}, []
             with pm7amd.Server(eng, lanes=args.serve_lanes, max_batch=args.serve_batch) as srv:
                 t0 = time.perf_counter()
 
                 def feed():
-                    for t, mol, c, m in reqs:
+                    for k, (t, mol, c, m) in enumerate(reqs):
                         dt = t - (time.perf_counter() - t0)
                         if dt > 0:
                             time.sleep(dt)
-                        srv.submit(mol, pm7amd.Pm7Options(charge=c, multiplicity=m, precision=precision))
+                        kind_of[srv.submit(mol, pm7amd.Pm7Options(charge=c, multiplicity=m, precision=precision))] = kinds[k]
 
                 th = threading.Thread(target=feed)
                 th.start()
@@ -227,11 +229,16 @@ def main() -> int:
                         raise SystemExit("serving stalled")
                     done += 1
                     lat.append(out[2])
+                    got.append((out[0], out[2]))
                     bad += out[1].status != 0
                 elapsed = time.perf_counter() - t0
                 th.join()
+            for rid, ms in got:
+                by_kind.setdefault(kind_of.get(rid, "?"), []).append(ms)
             return {"solves_per_s": round(len(reqs) / elapsed, 3), "failed": bad,
-                    "latency_ms": {"p50": round(pct(lat, 0.5), 3), "p99": round(pct(lat, 0.99), 3)}}
+                    "latency_ms": {"p50": round(pct(lat, 0.5), 3), "p99": round(pct(lat, 0.99), 3)},
+                    "latency_ms_by_kind": {k: {"n": len(v), "p50": round(pct(v, 0.5), 3), "p99": round(pct(v, 0.99), 3)}
+                                           for k, v in sorted(by_kind.items())}}
 
         serve("fp64")  # warm the server lanes' buffers
         capacity = serve("fp64")  # all requests at t = 0
@@ -240,7 +247,7 @@ def main() -> int:
         serving = {"requests": len(reqs), "capacity_fp64_solves_per_s": capacity["solves_per_s"],
                    "rate_per_s": round(rate, 1), "lanes": args.serve_lanes,
                    "max_batch": args.serve_batch,
-                   "mix": "70% closed-shell H/C/N/O (3-60 atoms), 20% S/P/Cl molecules of 10-200 atoms "
+                   "mix": "70% closed-shell H/C/N/O (3-60 atoms), 20% S/P/Cl molecules of 12-200 atoms "
                           "(MNDO/d pairs), 10% UHF doublets (one H removed)",
                    "fp64": serve("fp64"), "mixed": serve("mixed")}
         print(f"[bench] serving: {serving['fp64']} / {serving['mixed']}", file=sys.stderr, flush=True)

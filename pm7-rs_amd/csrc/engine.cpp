@@ -1041,7 +1041,21 @@ void srv_loop(pm7_server* s) {
       // class per batch keeps a batch to a single kernel, whose time is that class's.
       std::vector<int64_t> ids;
       size_t bytes = 0;
-      const int cls = s->reqs[s->queue.front()].job.variant;
+      // Large-system groups hold a lane for many small-molecule batch times: one lane is kept
+      // for the other classes, so small requests behind a run of large ones still find a lane.
+      int large_flights = 0;
+      for (auto& gp : s->flights)
+        if (gp->busy && !gp->P.jobs.empty() && gp->P.jobs.front().variant == LARGE_VARIANT) ++large_flights;
+      const bool large_ok = large_flights < std::max(1, (int)s->flights.size() - 1);
+      int cls = -1;
+      for (int64_t id : s->queue) {
+        const int v = s->reqs[id].job.variant;
+        if (v != LARGE_VARIANT || large_ok) {
+          cls = v;
+          break;
+        }
+      }
+      if (cls < 0) break;  // only large requests queued and their lanes are full
       // Large-system SCFs of one batch run as a lockstep group (batched eigensolves), at most
       // LARGE_GROUP of them, so further ones start on the other lanes.
       const int cap = cls == LARGE_VARIANT ? std::min(LARGE_GROUP, s->cfg.max_batch) : s->cfg.max_batch;

@@ -93,6 +93,8 @@ def main() -> int:
     ap.add_argument("--large-waters", type=int, default=500,
                     help="water cluster of the large-system run (0: skip); 500 waters = 3000 AOs")
     ap.add_argument("--large-only", action="store_true", help="only the large-system run")
+    ap.add_argument("--d-batch", type=int, default=64,
+                    help="S/P/Cl molecules (MNDO/d pairs, <= 196 AOs) batched in FP64 and mixed (0: skip)")
     ap.add_argument("--serve-lanes", type=int, default=4)
     ap.add_argument("--serve-batch", type=int, default=64)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -254,6 +256,32 @@ def main() -> int:
 
     large = large_system(eng) if args.large_waters > 0 else None
 
+    # A batch of molecules holding S, P or Cl: every pair through the MNDO/d pair kernel, the
+    # 45-word spd one-centre blocks in the Fock build; the megakernel classes only (<= 196 AOs).
+    d_batch = None
+    if args.d_batch > 0:
+        rng_d = W.Rng(11)
+        d_atoms = [W.hetero_chain(rng_d, rng_d.randint(4, 30), ("S", "P", "Cl")[rng_d.randint(0, 2)])
+                   for _ in range(args.d_batch)]
+        d_mols = [pm7amd.Molecule.from_symbols_angstrom(a) for a in d_atoms]
+        d_out = {"molecules": len(d_mols), "atoms": [min(map(len, d_atoms)), max(map(len, d_atoms))],
+                 "workload": "hetero_chain seed 11: alkanes of 4-30 C with one S (thioether), PH or Cl"}
+        d_res = {}
+        for precision in ("fp64", "mixed"):
+            o = pm7amd.Pm7Options(precision=precision)
+            eng.run_batch(d_mols, o)
+            t = time.perf_counter()
+            reps = max(1, args.steps)
+            for _ in range(reps):
+                d_res[precision] = eng.run_batch(d_mols, o)
+            dt = (time.perf_counter() - t) / reps
+            d_out[precision] = {"solves_per_s": round(len(d_mols) / dt, 3), "ms_per_batch": round(dt * 1e3, 3),
+                                "failed": sum(r.status != 0 for r in d_res[precision])}
+        d_out["mixed"]["max_abs_dHf_vs_fp64_kcal"] = max(
+            abs(a.heat_of_formation_kcal - b.heat_of_formation_kcal) for a, b in zip(d_res["mixed"], d_res["fp64"]))
+        d_batch = d_out
+        print(f"[bench] d batch: {d_out['fp64']} / {d_out['mixed']}", file=sys.stderr, flush=True)
+
     # One molecule at a time (the plain FP64 GPU path the batched path is compared with).
     single = mols[: args.single]
     for m in single[:2]:
@@ -332,6 +360,7 @@ def main() -> int:
                   "iterations_mean": round(float(np.mean([r.iterations for r in resmx])), 2)},
         "serving": serving,
         "large_system": large,
+        "d_batch": d_batch,
         "single_molecule_fp64": {"solves_per_s": round(single_rate, 3), "molecules": len(single),
                                  "batched_speedup": round((solvesmx if head_mixed else solves64) / single_rate, 3)},
         "kernel_ms": {"integrals": round(float(np.mean([k[0] for k in kern64])), 4),

@@ -729,6 +729,7 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
   int iter = 0;
   int low = mixed_ok ? 1 : 0;
   int cslot_c = S_C0;  // Ct lives here during the SCF
+  bool last_rot = false;  // the last SCF step rotated the orbitals instead of diagonalising
   double dp_prev = 1.0e30;
   if (!M.uhf) {
     const int nocc = M.n_occ;
@@ -771,7 +772,10 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
       LG(hipGetLastError());
       LG(hipEventRecord(lw->ev[3], s));
       double* eps = b.vecs + M.vec0;
-      if (low && iter > 0 && dp_prev < ROTATE_BELOW) {
+      // PRECISION_MIXED keeps rotating after the switch to FP64 (rocBLAS dgemm); the reported
+      // orbitals come from one full eigensolve of the converged Fock after the loop.
+      last_rot = (low || mixed) && iter > 0 && dp_prev < ROTATE_BELOW;
+      if (last_rot) {
         LG(eig_skip());
         LG(eig_rotate(slot(S_W), nocc, slot(cslot_c)));
       } else {
@@ -806,15 +810,28 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
         hipLaunchKernelGGL(k_lg_fock_pairs<double>, dim3(grid_for((int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256), 0, s, mi, b, P, 0.5, nullptr, 0.0, F, nullptr);
     LG(dots(2, P, slot(S_H), P, F, nullptr, nullptr, d3));
     hipLaunchKernelGGL(k_lg_energy, dim3(1), dim3(64), 0, s, ctl, d3, 0);
-    if (host.guard.shift > 0.0) {
+    int pslot = S_HD + hp;
+    if (host.guard.shift > 0.0 || last_rot) {
       hipLaunchKernelGGL(k_lg_copy, dim3(grid_for(sq)), dim3(256), 0, s, sq, F, slot(S_W));
       LG(eig_full(slot(S_W), b.vecs + M.vec0, slot(cslot_c)));
+      if (last_rot && host.guard.shift == 0.0) {
+        // The reported orbitals diagonalise the converged Fock; the reported density and energy
+        // are theirs (one more Roothaan step from the converged density).
+        pslot = S_HD + (hp + 1) % HIST;
+        double* P2 = slot(pslot);
+        LG(density(slot(cslot_c), M.n_occ, 2.0, P2, false));
+        hipLaunchKernelGGL(k_lg_pack<double>, dim3(grid_for(M.n_pp)), dim3(256), 0, s, mi, b, P2);
+        hipLaunchKernelGGL(k_lg_fock_diag<double>, dim3(grid_for((int64_t)M.n_pp * 64)), dim3(256), 0, s, mi, b, P2, P2, 0.5, nullptr, 0.0, F, nullptr);
+        hipLaunchKernelGGL(k_lg_fock_pairs<double>, dim3(grid_for((int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256), 0, s, mi, b, P2, 0.5, nullptr, 0.0, F, nullptr);
+        LG(dots(2, P2, slot(S_H), P2, F, nullptr, nullptr, d3));
+        hipLaunchKernelGGL(k_lg_energy, dim3(1), dim3(64), 0, s, ctl, d3, 0);
+      }
     }
     // C = Ct^T into the C1 slot for the output gather.
     hipLaunchKernelGGL(k_lg_transpose, dim3(grid_for(sq)), dim3(256), 0, s, ld, slot(cslot_c), slot(S_C1));
     LG(hipMemcpyAsync(&host, ctl, sizeof host, hipMemcpyDeviceToHost, s));
     LG(hipStreamSynchronize(s));
-    out.p_slot = S_HD + hp;
+    out.p_slot = pslot;
     out.c_slot = S_C1;
     out.pb_slot = -1;
     out.cb_slot = -1;
@@ -857,7 +874,8 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
       hipLaunchKernelGGL(k_lg_combine, dim3(grid_for(nn)), dim3(256), 0, s, n, ld, ctl, slot(S_HF), Fa, Pa, 1.0, Fa, 0);
       hipLaunchKernelGGL(k_lg_combine, dim3(grid_for(nn)), dim3(256), 0, s, n, ld, ctl, slot(S_HF), Fb, Pb, 1.0, Fb, 0);
       LG(hipEventRecord(lw->ev[3], s));
-      if (low && iter > 0 && dp_prev < ROTATE_BELOW) {
+      last_rot = (low || mixed) && iter > 0 && dp_prev < ROTATE_BELOW;
+      if (last_rot) {
         LG(eig_skip());
         LG(eig_rotate(Fa, na, slot(U_CA0)));
         LG(eig_rotate(Fb, nb, slot(U_CB0)));
@@ -895,9 +913,23 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
         hipLaunchKernelGGL(k_lg_fock_pairs<double>, dim3(grid_for((int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256), 0, s, mi, b, Pa, 1.0, Pb, 1.0, Fa, Fb);
     LG(dots(3, Pt, slot(U_H), Pa, Fa, Pb, Fb, d3));
     hipLaunchKernelGGL(k_lg_energy, dim3(1), dim3(64), 0, s, ctl, d3, 1);
-    if (host.guard.shift > 0.0) LG(eig_full(Fa, eps_a, slot(U_CA0)));
+    if (host.guard.shift > 0.0 || last_rot) LG(eig_full(Fa, eps_a, slot(U_CA0)));
     hipLaunchKernelGGL(k_lg_copy, dim3(grid_for(sq)), dim3(256), 0, s, sq, Fb, slot(U_T));
     LG(eig_full(slot(U_T), eps_b, slot(U_CB0)));
+    if (last_rot && host.guard.shift == 0.0) {
+      // Densities and energy of the reported orbitals (see the RHF branch).
+      LG(density(slot(U_CA0), na, 1.0, Pan, false));
+      LG(density(slot(U_CB0), nb, 1.0, Pbn, false));
+      std::swap(Pa, Pan);
+      std::swap(Pb, Pbn);
+      hipLaunchKernelGGL(k_lg_copy, dim3(grid_for(sq)), dim3(256), 0, s, sq, Pa, Pt);
+      LGB(rocblas_daxpy(lw->blas, (rocblas_int)sq, &one, Pb, 1, Pt, 1));
+      hipLaunchKernelGGL(k_lg_pack<double>, dim3(grid_for(M.n_pp)), dim3(256), 0, s, mi, b, Pt);
+      hipLaunchKernelGGL(k_lg_fock_diag<double>, dim3(grid_for((int64_t)M.n_pp * 64)), dim3(256), 0, s, mi, b, Pt, Pa, 1.0, Pb, 1.0, Fa, Fb);
+      hipLaunchKernelGGL(k_lg_fock_pairs<double>, dim3(grid_for((int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256), 0, s, mi, b, Pa, 1.0, Pb, 1.0, Fa, Fb);
+      LG(dots(3, Pt, slot(U_H), Pa, Fa, Pb, Fb, d3));
+      hipLaunchKernelGGL(k_lg_energy, dim3(1), dim3(64), 0, s, ctl, d3, 1);
+    }
     hipLaunchKernelGGL(k_lg_transpose, dim3(grid_for(sq)), dim3(256), 0, s, ld, slot(U_CA0), slot(U_CA1));
     hipLaunchKernelGGL(k_lg_transpose, dim3(grid_for(sq)), dim3(256), 0, s, ld, slot(U_CB0), slot(U_CB1));
     LG(hipMemcpyAsync(&host, ctl, sizeof host, hipMemcpyDeviceToHost, s));

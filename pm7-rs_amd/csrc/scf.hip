@@ -191,6 +191,7 @@ __global__ void __launch_bounds__(NT)
   double* eps_a = vecs + c.M.vec0;
   double* eps_b = eps_a + ld;
   int sweeps = 0;
+  bool last_rot = false;  // the last SCF step rotated the orbitals instead of diagonalising
   if (threadIdx.x == 0) {
     s_guard.floor_ = c.M.shift_floor;
     s_guard.shift = c.M.shift_floor;
@@ -348,12 +349,16 @@ __global__ void __launch_bounds__(NT)
       double* Cnew = c.slot(cbuf ? S_C1 : S_C0);
       const bool low = s_int[4] != 0;
       // Occupied-virtual rotations only once the SCF is near its fixed point (last density change
-      // below ROTATE_BELOW); before that the reduced-precision phase diagonalises in full, so
-      // large early steps cannot carry it to another stationary state.
-      if (low && iter > 0 && s_dbl[2] < ROTATE_BELOW)
+      // below ROTATE_BELOW); before that every iteration diagonalises in full, so large early steps
+      // cannot carry it to another stationary state. PRECISION_MIXED keeps rotating after the
+      // switch to FP64 (the rotation is FP64 GEMMs on the matrix cores); the orbitals and
+      // eigenvalues it reports come from one full eigensolve of the converged Fock after the loop.
+      const bool rot = iter > 0 && s_dbl[2] < ROTATE_BELOW && (low || c.M.precision == 1);
+      if (rot)
         occ_virt_rotate<NT>(n, ld, nocc, W, c.slot(cbuf ? S_C0 : S_C1), Cnew, c.slot(S_T), c.slot(S_AG), c.slot(S_VG));
       else
         sweeps += eig(low, W, iter > 0 ? c.slot(cbuf ? S_C0 : S_C1) : nullptr, Cnew, eps_a, S_AG, S_VG, S_T);
+      last_rot = rot;
       lap(PH_EIG);
       double* Pn = c.slot(S_HD + s_int[0]);
       const double dsq = low ? density<NT, float>(n, ld, nocc, 2.0, Cnew, P, Pn, s_red)
@@ -398,22 +403,33 @@ __global__ void __launch_bounds__(NT)
     res.dp_err = s_dbl[2];
     res.err_norm = s_dbl[1];
     // Final energy from the converged density.
-    const int pslot = S_HD + s_int[0];
+    int pslot = S_HD + s_int[0];
     double* P = c.slot(pslot);
     double* F = c.slot(S_F);
-    fock_build<NT, double>(c, P, P, 0.5, nullptr, 0.0, F, nullptr);
-    double eh[2] = {0.0, 0.0};
-    for (int idx = threadIdx.x; idx < n * n; idx += NT) {
-      int i = idx / n, j = idx - i * n;
-      int64_t x = (int64_t)i * ld + j;
-      eh[0] += P[x] * H[x];
-      eh[1] += P[x] * F[x];
-    }
-    wg_sum_n<NT, 2>(eh, s_red);
-    res.e_el = 0.5 * (eh[0] + eh[1]);
-    if (s_guard.shift > 0.0) {
+    auto final_energy = [&](const double* Pf) {
+      fock_build<NT, double>(c, Pf, Pf, 0.5, nullptr, 0.0, F, nullptr);
+      double eh[2] = {0.0, 0.0};
+      for (int idx = threadIdx.x; idx < n * n; idx += NT) {
+        int i = idx / n, j = idx - i * n;
+        int64_t x = (int64_t)i * ld + j;
+        eh[0] += Pf[x] * H[x];
+        eh[1] += Pf[x] * F[x];
+      }
+      wg_sum_n<NT, 2>(eh, s_red);
+      res.e_el = 0.5 * (eh[0] + eh[1]);
+    };
+    final_energy(P);
+    if (s_guard.shift > 0.0 || last_rot) {
       sweeps += eig(false, F, c.slot(cbuf ? S_C0 : S_C1), c.slot(cbuf ? S_C1 : S_C0), eps_a, S_AG, S_VG, S_T);
       cbuf ^= 1;
+      if (last_rot && s_guard.shift == 0.0) {
+        // The reported orbitals diagonalise the converged Fock; the reported density and energy
+        // are theirs (one more Roothaan step from the converged density).
+        pslot = S_HD + (s_int[0] + 1) % HIST;
+        double* P2 = c.slot(pslot);
+        (void)density<NT, double>(n, ld, nocc, 2.0, c.slot(cbuf ? S_C0 : S_C1), P, P2, s_red);
+        final_energy(P2);
+      }
     }
     res.p_slot = pslot;
     res.c_slot = cbuf ? S_C0 : S_C1;
@@ -460,7 +476,9 @@ __global__ void __launch_bounds__(NT)
       double* Ca = c.slot(cbuf ? U_CA1 : U_CA0);
       double* Cb = c.slot(cbuf ? U_CB1 : U_CB0);
       const bool low = s_int[4] != 0;
-      if (low && iter > 0 && s_dbl[2] < ROTATE_BELOW) {
+      const bool rot = iter > 0 && s_dbl[2] < ROTATE_BELOW && (low || c.M.precision == 1);
+      last_rot = rot;
+      if (rot) {
         occ_virt_rotate<NT>(n, ld, na, Fa, c.slot(cbuf ? U_CA0 : U_CA1), Ca, c.slot(U_T), c.slot(U_AG), c.slot(U_VG));
         occ_virt_rotate<NT>(n, ld, nb, Fb, c.slot(cbuf ? U_CB0 : U_CB1), Cb, c.slot(U_T), c.slot(U_AG), c.slot(U_VG));
       } else {
@@ -533,7 +551,7 @@ __global__ void __launch_bounds__(NT)
     res.e_el = 0.5 * (eh[0] + eh[1] + eh[2]);
     // The last loop solve left alpha/beta vectors in buffer (cbuf ^ 1).
     int ca_last = cbuf ? U_CA0 : U_CA1, cb_last = cbuf ? U_CB0 : U_CB1;
-    if (s_guard.shift > 0.0) {
+    if (s_guard.shift > 0.0 || last_rot) {
       const int dst = (ca_last == U_CA0) ? U_CA1 : U_CA0;
       sweeps += eig(false, Fa, c.slot(ca_last), c.slot(dst), eps_a, U_AG, U_VG, U_T);
       ca_last = dst;
@@ -542,6 +560,30 @@ __global__ void __launch_bounds__(NT)
       const int dst = (cb_last == U_CB0) ? U_CB1 : U_CB0;
       sweeps += eig(false, Fb, c.slot(cb_last), c.slot(dst), eps_b, U_AG, U_VG, U_T);
       cb_last = dst;
+    }
+    if (last_rot && s_guard.shift == 0.0) {
+      // Densities and energy of the reported orbitals (see the RHF branch).
+      (void)density<NT, double>(n, ld, na, 1.0, c.slot(ca_last), Pa, Pan, s_red);
+      (void)density<NT, double>(n, ld, nb, 1.0, c.slot(cb_last), Pb, Pbn, s_red);
+      double* tmp = Pa;
+      Pa = Pan;
+      Pan = tmp;
+      tmp = Pb;
+      Pb = Pbn;
+      Pbn = tmp;
+      for (int idx = threadIdx.x; idx < ld * ld; idx += NT) Pt[idx] = Pa[idx] + Pb[idx];
+      __syncthreads();
+      fock_build<NT, double>(c, Pt, Pa, 1.0, Pb, 1.0, Fa, Fb);
+      double e3[3] = {0.0, 0.0, 0.0};
+      for (int idx = threadIdx.x; idx < n * n; idx += NT) {
+        int i = idx / n, j = idx - i * n;
+        int64_t x = (int64_t)i * ld + j;
+        e3[0] += Pt[x] * H[x];
+        e3[1] += Pa[x] * Fa[x];
+        e3[2] += Pb[x] * Fb[x];
+      }
+      wg_sum_n<NT, 3>(e3, s_red);
+      res.e_el = 0.5 * (e3[0] + e3[1] + e3[2]);
     }
     res.p_slot = (Pa == c.slot(U_PA)) ? U_PA : U_PAN;
     res.pb_slot = (Pb == c.slot(U_PB)) ? U_PB : U_PBN;

@@ -203,6 +203,9 @@ struct pm7_ctx {
   // Jacobi sweep cap of the first (cold) eigensolve in the reduced-precision phase of
   // PRECISION_MIXED runs (environment PM7_AMD_MIXED_SWEEPS, 1..40).
   int mixed_sweeps = 40;
+  // PRECISION_MIXED: density change below which the SCF rotates orbitals instead of
+  // diagonalising (environment PM7_AMD_ROTATE_BELOW, 1e-6..1e-1).
+  double rotate_below = ROTATE_BELOW;
   std::mutex mu;
 };
 
@@ -458,6 +461,7 @@ pm7_status prepare(pm7_ctx* ctx, Lane& L, Prepared& P, int64_t max_memory_mb) {
     M.mixed_switch = j.o.mixed_switch;
     M.mixed_sweeps = ctx->mixed_sweeps;
     M.dpath = j.dpath;
+    M.rotate_below = ctx->rotate_below;
     lists[j.variant].push_back(mi);
     atom0 += n;
     pair0 += (int64_t)n * (n - 1) / 2;
@@ -1272,6 +1276,10 @@ pm7_status pm7_ctx_create(int32_t device, const char* data_dir, pm7_ctx** out) {
   if (const char* ms = std::getenv("PM7_AMD_MIXED_SWEEPS")) {
     const int v = std::atoi(ms);
     if (v >= 1 && v <= 40) ctx->mixed_sweeps = v;
+  }
+  if (const char* rb = std::getenv("PM7_AMD_ROTATE_BELOW")) {
+    const double v = std::strtod(rb, nullptr);
+    if (v >= 1.0e-6 && v <= 1.0e-1) ctx->rotate_below = v;
   }
   if (!ctx->lane.create()) {
     ctx->lane.destroy();

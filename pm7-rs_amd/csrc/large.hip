@@ -774,7 +774,7 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
       double* eps = b.vecs + M.vec0;
       // PRECISION_MIXED keeps rotating after the switch to FP64 (rocBLAS dgemm); the reported
       // orbitals come from one full eigensolve of the converged Fock after the loop.
-      last_rot = (low || mixed) && iter > 0 && dp_prev < ROTATE_BELOW;
+      last_rot = (low || mixed) && iter > 0 && dp_prev < M.rotate_below;
       if (last_rot) {
         LG(eig_skip());
         LG(eig_rotate(slot(S_W), nocc, slot(cslot_c)));
@@ -874,7 +874,7 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
       hipLaunchKernelGGL(k_lg_combine, dim3(grid_for(nn)), dim3(256), 0, s, n, ld, ctl, slot(S_HF), Fa, Pa, 1.0, Fa, 0);
       hipLaunchKernelGGL(k_lg_combine, dim3(grid_for(nn)), dim3(256), 0, s, n, ld, ctl, slot(S_HF), Fb, Pb, 1.0, Fb, 0);
       LG(hipEventRecord(lw->ev[3], s));
-      last_rot = (low || mixed) && iter > 0 && dp_prev < ROTATE_BELOW;
+      last_rot = (low || mixed) && iter > 0 && dp_prev < M.rotate_below;
       if (last_rot) {
         LG(eig_skip());
         LG(eig_rotate(Fa, na, slot(U_CA0)));

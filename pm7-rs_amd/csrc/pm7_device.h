@@ -24,7 +24,7 @@ constexpr int MAX_Z = 103;     // atomic numbers 1..102 (the parameter tables st
 constexpr int NSLOT = 32;       // square matrices per molecule in the slab
 constexpr int HIST = 8;         // DIIS history depth
 constexpr int MIXED_MAX_ITERS = 30;  // longest reduced-precision phase of a mixed run
-constexpr double ROTATE_BELOW = 1.0e-3;  // density change below which that phase rotates orbitals
+constexpr double ROTATE_BELOW = 1.0e-2;  // default density change below which mixed runs rotate orbitals
 
 // Slab slots, RHF.
 // C0/C1 double-buffer the sorted eigenvectors so each eigensolve can start from the last one;
@@ -71,6 +71,7 @@ struct MolDev {
   int64_t pp0;          // slab offset of the packed densities
   int64_t vec0;         // offset into the eigenvalue array
   double e_tol, p_tol, adiis_switch, shift_floor, sad_yy, mixed_switch;
+  double rotate_below;  // mixed precision: density change below which orbitals are rotated
 };
 
 // Phases of an SCF iteration timed inside the megakernel (device wall clock, 100 MHz ticks).

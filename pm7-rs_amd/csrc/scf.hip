@@ -353,7 +353,7 @@ __global__ void __launch_bounds__(NT)
       // cannot carry it to another stationary state. PRECISION_MIXED keeps rotating after the
       // switch to FP64 (the rotation is FP64 GEMMs on the matrix cores); the orbitals and
       // eigenvalues it reports come from one full eigensolve of the converged Fock after the loop.
-      const bool rot = iter > 0 && s_dbl[2] < ROTATE_BELOW && (low || c.M.precision == 1);
+      const bool rot = iter > 0 && s_dbl[2] < c.M.rotate_below && (low || c.M.precision == 1);
       if (rot)
         occ_virt_rotate<NT>(n, ld, nocc, W, c.slot(cbuf ? S_C0 : S_C1), Cnew, c.slot(S_T), c.slot(S_AG), c.slot(S_VG));
       else
@@ -476,7 +476,7 @@ __global__ void __launch_bounds__(NT)
       double* Ca = c.slot(cbuf ? U_CA1 : U_CA0);
       double* Cb = c.slot(cbuf ? U_CB1 : U_CB0);
       const bool low = s_int[4] != 0;
-      const bool rot = iter > 0 && s_dbl[2] < ROTATE_BELOW && (low || c.M.precision == 1);
+      const bool rot = iter > 0 && s_dbl[2] < c.M.rotate_below && (low || c.M.precision == 1);
       last_rot = rot;
       if (rot) {
         occ_virt_rotate<NT>(n, ld, na, Fa, c.slot(cbuf ? U_CA0 : U_CA1), Ca, c.slot(U_T), c.slot(U_AG), c.slot(U_VG));

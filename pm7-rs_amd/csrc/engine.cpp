@@ -200,9 +200,10 @@ struct pm7_ctx {
   DevBuf d_elem, d_pairpar, d_gl, d_ocd;
   Lane lane;
   double timings[PM7_TIMING_SLOTS] = {};
-  // Jacobi sweep cap of the first (cold) eigensolve in the reduced-precision phase of
-  // PRECISION_MIXED runs (environment PM7_AMD_MIXED_SWEEPS, 1..40).
-  int mixed_sweeps = 40;
+  // Jacobi sweep cap of each full eigensolve in the reduced-precision phase of PRECISION_MIXED
+  // runs (environment PM7_AMD_MIXED_SWEEPS, 1..40): three sweeps of the warm-started solve are
+  // enough for the FP32 density; the DIIS-driven iterations absorb the rest.
+  int mixed_sweeps = 3;
   // PRECISION_MIXED: density change below which the SCF rotates orbitals instead of
   // diagonalising (environment PM7_AMD_ROTATE_BELOW, 1e-6..1e-1).
   double rotate_below = ROTATE_BELOW;

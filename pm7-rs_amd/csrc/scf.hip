@@ -241,10 +241,10 @@ __global__ void __launch_bounds__(NT)
     }
     __syncthreads();
   }
-  // One FP64 eigensolve: to full accuracy, or, in the reduced-precision phase of a mixed run (its
-  // first iteration; later ones rotate occupied against virtual orbitals only), to the accuracy
-  // the FP32 density supports (off-diagonal norm 1e-7 of the total) in at most mixed_sweeps
-  // sweeps.
+  // One FP64 eigensolve: to full accuracy, or, in the reduced-precision phase of a mixed run (the
+  // iterations before the density change drops below rotate_below; later ones rotate occupied
+  // against virtual orbitals only), to the accuracy the FP32 density supports (off-diagonal norm
+  // 1e-7 of the total) in at most mixed_sweeps sweeps.
   auto eig = [&](bool low, const double* Wm, const double* Cp, double* Co, double* ev, int ag, int vg, int tg) -> int {
     return eigensolve<NT, MAXN, A_LDS, V_LDS, double>(n, ld, Wm, Cp, Co, ev, c.slot(ag), c.slot(vg), c.slot(tg), s_a,
                                                       s_v, s_cs, s_t, s_perm, s_pq, s_red,

@@ -351,6 +351,19 @@ size_t bytes_needed(const Prepared& P) {
          P.jobs.size() * (sizeof(MolDev) + sizeof(MolOut) + 64);
 }
 
+}  // namespace
+
+int pm7::large_group_size() {
+  static const int g = [] {
+    const char* v = std::getenv("PM7_AMD_LARGE_GROUP");
+    const int x = v ? std::atoi(v) : 0;
+    return (x >= 2 && x <= 64) ? x : LARGE_GROUP_DEFAULT;
+  }();
+  return g;
+}
+
+namespace {
+
 // Words a pair of a d-path molecule takes in the integral array: the npair(f) x npair(s) block and
 // the packed electron-core blocks of both atoms.
 int64_t pair_words_d(int nf, int ns) {
@@ -813,8 +826,9 @@ pm7_status run_large(Lane& L, const Prepared& P) {
     }
     LANECHK(hipEventRecord(L.ev_var[2 * LARGE_VARIANT + 1], s));
   }
-  for (size_t g0 = single; g0 < order.size(); g0 += LARGE_GROUP) {
-    const int gn = (int)std::min<size_t>(LARGE_GROUP, order.size() - g0);
+  const int group = large_group_size();
+  for (size_t g0 = single; g0 < order.size(); g0 += group) {
+    const int gn = (int)std::min<size_t>(group, order.size() - g0);
     while ((int)L.group_work.size() < gn) {
       LargeWork* w = large_work_create();
       hipStream_t st = nullptr;
@@ -1062,8 +1076,8 @@ void srv_loop(pm7_server* s) {
       }
       if (cls < 0) break;  // only large requests queued and their lanes are full
       // Large-system SCFs of one batch run as a lockstep group (batched eigensolves), at most
-      // LARGE_GROUP of them, so further ones start on the other lanes.
-      const int cap = cls == LARGE_VARIANT ? std::min(LARGE_GROUP, s->cfg.max_batch) : s->cfg.max_batch;
+      // large_group_size() of them, so further ones start on the other lanes.
+      const int cap = cls == LARGE_VARIANT ? std::min(large_group_size(), s->cfg.max_batch) : s->cfg.max_batch;
       for (auto it = s->queue.begin(); it != s->queue.end() && (int)ids.size() < cap;) {
         SrvReq& r = s->reqs[*it];
         if (r.job.variant != cls) {

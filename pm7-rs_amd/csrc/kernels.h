@@ -53,10 +53,13 @@ constexpr int GL_WORDS = 2 * 48 + 2 * 40;
 constexpr int SCF_VARIANTS = 4;
 constexpr int VARIANT_MAX_NAO[SCF_VARIANTS] = {64, 112, 196, 1 << 30};
 constexpr int LARGE_VARIANT = 3;  // above 196 basis functions: the grid-wide path (large.hip)
-// Large-class molecules up to this basis size run in groups of up to LARGE_GROUP SCFs whose full
+// Large-class molecules up to this basis size run in groups of up to large_group_size() SCFs whose full
 // eigensolves are batched (one strided rocSOLVER call per iteration for the whole group).
 constexpr int LARGE_GROUP_MAX_NAO = 512;
-constexpr int LARGE_GROUP = 16;
+// 64: on a serving mix of 12-200 atom S/P/Cl chains, groups of 16/32/64 gave 134/140/146 large
+// solves/s; the batcher's matrices take 2 x 64 x 512^2 doubles (268 MB) at most.
+constexpr int LARGE_GROUP_DEFAULT = 64;
+int large_group_size();  // LARGE_GROUP_DEFAULT, or PM7_AMD_LARGE_GROUP (2..64)
 
 hipError_t launch_integrals(const BatchDev& b, int64_t n_pairs, int n_items, int n_mols, hipStream_t s);
 hipError_t launch_scf(int variant, const BatchDev& b, int list_off, int count, hipStream_t s);

@@ -207,6 +207,11 @@ struct pm7_ctx {
   // PRECISION_MIXED: density change below which the SCF rotates orbitals instead of
   // diagonalising (environment PM7_AMD_ROTATE_BELOW, 1e-6..1e-1).
   double rotate_below = ROTATE_BELOW;
+  // Stopping test of the full-accuracy Jacobi eigensolves: off-diagonal^2 <= 10^-eig_tol_exp x
+  // total^2 (environment PM7_AMD_EIG_TOL_EXP, 16..30). 1e-20 leaves the orbitals accurate to
+  // ~1e-10 of the matrix norm, far inside the 1e-7 density bar; against 1e-26 it saves 0.45
+  // sweeps per solve (FP64 organic batch 7.40k -> 7.80k solves/s, S/P/Cl batch 415 -> 460/s).
+  int eig_tol_exp = 20;
   std::mutex mu;
 };
 
@@ -476,6 +481,7 @@ pm7_status prepare(pm7_ctx* ctx, Lane& L, Prepared& P, int64_t max_memory_mb) {
     M.mixed_sweeps = ctx->mixed_sweeps;
     M.dpath = j.dpath;
     M.rotate_below = ctx->rotate_below;
+    M.eig_tol_exp = ctx->eig_tol_exp;
     lists[j.variant].push_back(mi);
     atom0 += n;
     pair0 += (int64_t)n * (n - 1) / 2;
@@ -1295,6 +1301,10 @@ pm7_status pm7_ctx_create(int32_t device, const char* data_dir, pm7_ctx** out) {
   if (const char* rb = std::getenv("PM7_AMD_ROTATE_BELOW")) {
     const double v = std::strtod(rb, nullptr);
     if (v >= 1.0e-6 && v <= 1.0e-1) ctx->rotate_below = v;
+  }
+  if (const char* te = std::getenv("PM7_AMD_EIG_TOL_EXP")) {
+    const int v = std::atoi(te);
+    if (v >= 16 && v <= 30) ctx->eig_tol_exp = v;
   }
   if (!ctx->lane.create()) {
     ctx->lane.destroy();

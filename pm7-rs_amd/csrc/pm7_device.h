@@ -65,7 +65,7 @@ struct MolDev {
   int max_scf, accel, use_diis, precision, n_pp;
   int mixed_sweeps;     // Jacobi sweep cap per eigensolve in the reduced-precision phase
   int dpath;            // 1: the molecule holds a d-bearing atom, every pair takes the MNDO/d kernels
-  int pad_;
+  int eig_tol_exp;      // full-accuracy eigensolves stop at off-diagonal^2 <= 10^-eig_tol_exp x total^2
   int64_t pair0;        // first pair of the molecule
   int64_t mat0;         // slab offset (doubles)
   int64_t pp0;          // slab offset of the packed densities

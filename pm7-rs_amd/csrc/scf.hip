@@ -245,10 +245,11 @@ __global__ void __launch_bounds__(NT)
   // iterations before the density change drops below rotate_below; later ones rotate occupied
   // against virtual orbitals only), to the accuracy the FP32 density supports (off-diagonal norm
   // 1e-7 of the total) in at most mixed_sweeps sweeps.
+  const double tol_full = exp10((double)-c.M.eig_tol_exp);
   auto eig = [&](bool low, const double* Wm, const double* Cp, double* Co, double* ev, int ag, int vg, int tg) -> int {
     return eigensolve<NT, MAXN, A_LDS, V_LDS, double>(n, ld, Wm, Cp, Co, ev, c.slot(ag), c.slot(vg), c.slot(tg), s_a,
                                                       s_v, s_cs, s_t, s_perm, s_pq, s_red,
-                                                      tk + PH_EIG_IN, low ? 1.0e-14 : 1.0e-26,
+                                                      tk + PH_EIG_IN, low ? 1.0e-14 : tol_full,
                                                       low ? c.M.mixed_sweeps : 40);
   };
 

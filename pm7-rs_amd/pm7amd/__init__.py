@@ -181,27 +181,13 @@ class Molecule:
 
     @classmethod
     def from_xyz_str(cls, text: str, charge: float = 0.0) -> "Molecule":
-        """Plain XYZ (Angstrom) as pm7-rs ``Molecule::from_xyz_str`` reads it (src/system.rs:149-196)."""
-        text = text.lstrip("\ufeff")
-        lines = text.splitlines()
-        if not lines:
-            raise Pm7Error(1, "empty XYZ")
-        try:
-            n = int(lines[0].strip())
-        except ValueError:
-            raise Pm7Error(1, f"invalid XYZ atom count: {lines[0]}") from None
-        atoms = []
-        for k in range(n):
-            if 2 + k >= len(lines):
-                raise Pm7Error(1, f"XYZ ended before atom {k}")
-            parts = lines[2 + k].split()
-            if len(parts) < 4:
-                raise Pm7Error(1, f"XYZ line {k + 3} has fewer than 4 fields")
-            sym = parts[0].lower()
-            if sym not in Z_OF:
-                raise Pm7Error(1, f"unknown element on line {k + 3}: {parts[0]}")
-            atoms.append((sym, float(parts[1]), float(parts[2]), float(parts[3])))
-        return cls.from_symbols_angstrom(atoms, charge)
+        """pm7-rs ``Molecule::from_xyz_str`` (src/system.rs:149-196); see ``formats.read_xyz``."""
+        from .formats import read_xyz
+        return read_xyz(text, charge)
+
+    def to_xyz_string(self, title: str = "") -> str:
+        from .formats import to_xyz_string
+        return to_xyz_string(self, title)
 
     def arrays(self):
         return self.z, self.xyz
@@ -295,11 +281,18 @@ class Pm7Result:
     eig_phase_ms: tuple = ()
     message: str = ""
 
+    orbital_source: str = "molecular"
+
     @property
     def gap_ev(self) -> Optional[float]:
-        if self.homo_ev is None or self.lumo_ev is None:
-            return None
-        return self.lumo_ev - self.homo_ev
+        """pm7-rs ``Pm7Result::gap_ev`` (src/scf.rs:462-472), resolved over both spin channels."""
+        from .formats import gap_ev
+        return gap_ev(self)
+
+    def spin_squared(self) -> Optional[float]:
+        """pm7-rs ``Pm7Result::spin_squared`` (src/scf.rs:423-437); None for restricted runs."""
+        from .formats import spin_squared
+        return spin_squared(self)
 
 
 def _opt(x: float) -> Optional[float]:

@@ -133,6 +133,20 @@ pm7_status pm7_run_batch(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols,
 
 void pm7_result_free(pm7_result_v1* res);
 
+/* Replaces closed_form_gradient (src/gradient.rs:500-555) for many molecules: the SCF of each
+ * (results[i] exactly as pm7_run_batch fills it), then dE_total/dR at the converged density on the
+ * device, in eV/Bohr, plus the gradient of the geometric corrections (host). Molecule i's atom a,
+ * axis k lands at gradient[3 * (n_atoms of molecules 0..i-1) + 3 * a + k]; the caller sizes the
+ * array to 3 x the total atom count. Molecules with MNDO/d pairs or valence shells n >= 4 get NaN
+ * and results[i].status = PM7_ERR_UNSUPPORTED (their energies are still filled). */
+pm7_status pm7_gradient_batch(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols,
+                              const pm7_options_v1* opts, int32_t n_opts, pm7_result_v1* results,
+                              double* gradient);
+
+/* One molecule (gradient: 3 * n_atoms doubles, eV/Bohr); returns the result's status. */
+pm7_status pm7_gradient(pm7_ctx* ctx, const pm7_molecule_v1* mol, const pm7_options_v1* opts,
+                        pm7_result_v1* out, double* gradient);
+
 /* Replaces build_core (src/hamiltonian.rs:224-231): the core Hamiltonian H (nao*nao). */
 pm7_status pm7_core_hamiltonian(pm7_ctx* ctx, const pm7_molecule_v1* mol, double* h_out,
                                 int32_t* nao_out);

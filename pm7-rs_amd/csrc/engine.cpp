@@ -139,6 +139,7 @@ struct Lane {
   hipEvent_t ev_var[2 * SCF_VARIANTS] = {};
   DevBuf d_mols, d_list, d_atom_z, d_atom_ao, d_atom_pp, d_xyz, d_pairs, d_items, d_gath_off;
   DevBuf d_w, d_slab, d_vecs, d_out, d_ecore, d_ecore_mol, d_gath, d_num, d_snum, d_dpairs;
+  DevBuf d_gpair, d_gatom, d_gitems;  // nuclear gradient (pm7_gradient_batch only)
   PinnedBuf h_out;  // staging for the packed results
   LargeWork* large = nullptr;  // rocBLAS handle and buffers of the large-system path (lazily made)
   // Several large-system SCFs in lockstep: one work set and one stream each (lazily made).
@@ -163,7 +164,7 @@ struct Lane {
       if (s) (void)hipStreamSynchronize(s);
     DevBuf* bufs[] = {&d_mols, &d_list, &d_atom_z, &d_atom_ao, &d_atom_pp, &d_xyz, &d_pairs, &d_items, &d_gath_off,
                       &d_w, &d_slab, &d_vecs, &d_out, &d_ecore, &d_ecore_mol, &d_gath, &d_num, &d_snum,
-                      &d_dpairs};
+                      &d_dpairs, &d_gpair, &d_gatom, &d_gitems};
     for (DevBuf* b : bufs) b->release();
     h_out.release();
     if (large) large_work_destroy(large);
@@ -925,7 +926,93 @@ pm7_status collect_batch(pm7_ctx* ctx, Lane& L, const Prepared& P, pm7_result_v1
   return PM7_OK;
 }
 
-pm7_status run_jobs(pm7_ctx* ctx, std::vector<Job>& jobs, pm7_result_v1* results, int64_t max_memory_mb) {
+// Where pm7_gradient_batch wants each molecule's gradient: flat output, offset per input index.
+struct GradReq {
+  double* out = nullptr;
+  std::vector<int64_t> off;  // by Job::idx, in doubles
+};
+
+// Geometric corrections of one molecule in kcal/mol (the terms host_corrections adds).
+double corrections_kcal(const Job& j, const std::vector<double>& xyz) {
+  if (j.o.method == PM7_METHOD_PM7_MINUS) return 0.0;
+  return dispersion_kcal(j.z, xyz) + hbond_kcal(j.z, xyz) + c_triple_kcal(j.z, xyz) + si_o_h_kcal(j.z, xyz);
+}
+
+// Device gradient of every job of a finished batch (enqueued on stream 0 after the SCF and the
+// gather), then, on the host, the corrections' gradient by the same fourth-order central difference
+// (h = 1e-4 Bohr) and the assembly into the caller's array. A molecule whose pairs have no device
+// gradient (MNDO/d pairs, numerical overlaps) gets NaN and PM7_ERR_UNSUPPORTED.
+pm7_status launch_gradient_batch(Lane& L, const Prepared& P, int* n_items_out) {
+  std::vector<int2> items;
+  int n_atoms = 0;
+  for (size_t mi = 0; mi < P.jobs.size(); ++mi) {
+    const Job& j = P.jobs[mi];
+    for (int k = 0; k < (int)j.z.size(); ++k) items.push_back(make_int2((int)mi, j.atom0 + k));
+    n_atoms = std::max(n_atoms, j.atom0 + (int)j.z.size());
+  }
+  *n_items_out = (int)items.size();
+  LANECHK(L.d_gitems.ensure(std::max<size_t>(1, items.size()) * sizeof(int2)));
+  LANECHK(L.d_gpair.ensure(std::max<size_t>(1, (size_t)P.n_pairs * 3) * sizeof(double)));
+  LANECHK(L.d_gatom.ensure(std::max<size_t>(1, (size_t)n_atoms * 3) * sizeof(double)));
+  if (!items.empty())
+    LANECHK(hipMemcpyAsync(L.d_gitems.p, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice, L.st[0]));
+  LANECHK(launch_gradient(P.b, P.n_pairs, static_cast<const int2*>(L.d_gitems.p), (int)items.size(),
+                          static_cast<double*>(L.d_gpair.p), static_cast<double*>(L.d_gatom.p), L.st[0]));
+  return PM7_OK;
+}
+
+pm7_status collect_gradient(Lane& L, const Prepared& P, const GradReq& g, pm7_result_v1* results) {
+  int n_atoms = 0;
+  for (const Job& j : P.jobs) n_atoms = std::max(n_atoms, j.atom0 + (int)j.z.size());
+  std::vector<double> ga((size_t)n_atoms * 3);
+  if (n_atoms > 0) {
+    LANECHK(hipMemcpyAsync(ga.data(), L.d_gatom.p, ga.size() * sizeof(double), hipMemcpyDeviceToHost, L.st[0]));
+    LANECHK(hipStreamSynchronize(L.st[0]));
+  }
+  auto work = [&](int t, int nt) {
+    for (size_t mi = t; mi < P.jobs.size(); mi += nt) {
+      const Job& j = P.jobs[mi];
+      const int na = (int)j.z.size();
+      double* out = g.out + g.off[j.idx];
+      bool ok = true;
+      for (int k = 0; k < 3 * na; ++k) {
+        out[k] = ga[(size_t)3 * j.atom0 + k];
+        ok = ok && std::isfinite(out[k]);
+      }
+      pm7_result_v1& r = results[j.idx];
+      if (!ok) {
+        for (int k = 0; k < 3 * na; ++k) out[k] = std::nan("");
+        if (r.status == PM7_OK || r.status == PM7_ERR_SCF_NOT_CONVERGED) {
+          r.status = PM7_ERR_UNSUPPORTED;
+          set_msg(&r, "gradient: molecules with MNDO/d pairs or valence shells n >= 4 have no gradient here");
+        }
+        continue;
+      }
+      if (j.o.method == PM7_METHOD_PM7_MINUS) continue;
+      std::vector<double> x = j.xyz;
+      constexpr double h = 1.0e-4;
+      for (int c = 0; c < 3 * na; ++c) {
+        double e[4];
+        const double steps[4] = {h, -h, 2.0 * h, -2.0 * h};
+        for (int t2 = 0; t2 < 4; ++t2) {
+          x[c] = j.xyz[c] + steps[t2];
+          e[t2] = corrections_kcal(j, x);
+        }
+        x[c] = j.xyz[c];
+        out[c] += (8.0 * (e[0] - e[1]) - (e[2] - e[3])) / (12.0 * h) / EV_TO_KCAL;
+      }
+    }
+  };
+  const int nt = P.jobs.size() >= 8 ? std::min(16, std::max(1, (int)std::thread::hardware_concurrency())) : 1;
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(work, t, nt);
+  work(0, nt);
+  for (auto& th : pool) th.join();
+  return PM7_OK;
+}
+
+pm7_status run_jobs(pm7_ctx* ctx, std::vector<Job>& jobs, pm7_result_v1* results, int64_t max_memory_mb,
+                    const GradReq* grad = nullptr) {
   Lane& L = ctx->lane;
   auto t0 = std::chrono::steady_clock::now();
   Prepared P;
@@ -944,12 +1031,15 @@ pm7_status run_jobs(pm7_ctx* ctx, std::vector<Job>& jobs, pm7_result_v1* results
   st = launch_batch(L, P, count);
   if (st == PM7_OK) st = run_large(L, P);
   if (st == PM7_OK) st = finish_batch(L, P);
+  int n_gitems = 0;
+  if (st == PM7_OK && grad) st = launch_gradient_batch(L, P, &n_gitems);
   if (st != PM7_OK) return fail(ctx, st, L.err);
   host_corrections(P.jobs);
   const auto t_launched = std::chrono::steady_clock::now();
   HIPCHK(hipEventSynchronize(L.ev[5]));
   const auto t_synced = std::chrono::steady_clock::now();
   st = collect_batch(ctx, L, P, results);
+  if (st == PM7_OK && grad) st = collect_gradient(L, P, *grad, results);
   if (st != PM7_OK) return fail(ctx, st, L.err);
   float ms = 0.0f;
   if (hipEventElapsedTime(&ms, L.ev[0], L.ev[1]) == hipSuccess) ctx->timings[0] = ms;
@@ -1340,8 +1430,33 @@ pm7_status pm7_params_load(pm7_ctx* ctx, int32_t method) {
   return ensure_params(ctx, method == PM7_METHOD_PM7_TS ? PM7_METHOD_PM7_TS : PM7_METHOD_PM7);
 }
 
+namespace {
+pm7_status run_batch_impl(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols, const pm7_options_v1* opts,
+                          int32_t n_opts, pm7_result_v1* results, double* gradient);
+}  // namespace
+
 pm7_status pm7_run_batch(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols, const pm7_options_v1* opts,
                          int32_t n_opts, pm7_result_v1* results) {
+  return run_batch_impl(ctx, n, mols, opts, n_opts, results, nullptr);
+}
+
+pm7_status pm7_gradient_batch(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols, const pm7_options_v1* opts,
+                              int32_t n_opts, pm7_result_v1* results, double* gradient) {
+  if (n > 0 && !gradient) return PM7_ERR_INVALID_INPUT;
+  return run_batch_impl(ctx, n, mols, opts, n_opts, results, gradient);
+}
+
+pm7_status pm7_gradient(pm7_ctx* ctx, const pm7_molecule_v1* mol, const pm7_options_v1* opts, pm7_result_v1* out,
+                        double* gradient) {
+  if (!ctx || !mol || !out || !gradient) return PM7_ERR_INVALID_INPUT;
+  pm7_status st = pm7_gradient_batch(ctx, 1, mol, opts, opts ? 1 : 0, out, gradient);
+  if (st != PM7_OK) return st;
+  return (pm7_status)out->status;
+}
+
+namespace {
+pm7_status run_batch_impl(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols, const pm7_options_v1* opts,
+                          int32_t n_opts, pm7_result_v1* results, double* gradient) {
   if (!ctx || n < 0 || (n > 0 && (!mols || !results))) return PM7_ERR_INVALID_INPUT;
   std::lock_guard<std::mutex> lk(ctx->mu);
   (void)hipSetDevice(ctx->device);
@@ -1365,10 +1480,23 @@ pm7_status pm7_run_batch(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols, c
     if (o.max_memory_mb > 0) budget = budget > 0 ? std::min(budget, o.max_memory_mb) : o.max_memory_mb;
     jobs.push_back(std::move(j));
   }
+  GradReq greq;
+  if (gradient) {
+    greq.out = gradient;
+    greq.off.resize(n);
+    int64_t off = 0;
+    for (int i = 0; i < n; ++i) {
+      greq.off[i] = off;
+      const int na = std::max(0, mols[i].n_atoms);
+      for (int k = 0; k < 3 * na; ++k) gradient[off + k] = std::nan("");
+      off += 3 * (int64_t)na;
+    }
+  }
   if (jobs.empty()) return PM7_OK;
-  pm7_status st = run_jobs(ctx, jobs, results, budget);
+  pm7_status st = run_jobs(ctx, jobs, results, budget, gradient ? &greq : nullptr);
   return st == PM7_ERR_INSUFFICIENT_MEMORY ? PM7_OK : st;
 }
+}  // namespace
 
 pm7_status pm7_run(pm7_ctx* ctx, const pm7_molecule_v1* mol, const pm7_options_v1* opts, pm7_result_v1* out) {
   if (!ctx || !mol || !out) return PM7_ERR_INVALID_INPUT;

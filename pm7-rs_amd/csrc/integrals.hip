@@ -610,6 +610,168 @@ __global__ void __launch_bounds__(256) k_gather(const MolDev* __restrict__ mols,
 
 namespace pm7 {
 
+// ---- nuclear gradient at the converged density (sp pairs) ----------------------------------
+//
+// At fixed density the NDDO energy is a sum of one-centre terms, which do not move, and one term
+// per atom pair that depends only on the pair's separation d = R_s - R_f (pm7-rs src/gradient.rs:
+// fixed-density Hellmann-Feynman gradient, 300-420; no Pulay term: the basis is orthonormal by
+// construction). pair_energy is that term:
+//   resonance       2 sum_{i in f, j in s} P_ij (beta_i + beta_j) S_ij / 2
+//   electron-core   sum_{mu nu in f} P_mu nu (-Z_s)(mu nu|s s) + sum_{la si in s} P_la si (-Z_f)(s s|la si)
+//   Coulomb         sum_{mu nu in f} sum_{la si in s} P_mu nu P_la si (mu nu|la si)
+//   exchange        -sum_spin sum P^spin_mu la P^spin_nu si (mu nu|la si)    (P^a = P^b = P/2 for RHF)
+//   core-core       ccrep(r)
+// with the integrals built exactly as k_pairs builds them. Its derivative along d comes from the
+// fourth-order central difference (8 [E(h) - E(-h)] - [E(2h) - E(-2h)]) / 12h, h = 1e-3 Bohr
+// (truncation ~h^4, rounding ~1e-11 eV/Bohr).
+
+// Densities of one pair, gathered once: packed one-centre blocks (off-diagonal doubled) and the
+// f x s blocks of the total density and of each spin (nspin = 1 for RHF: P/2 stands for both).
+struct PairDens {
+  double pf[10], ps[10];
+  double pfs[4][4];
+  double xs[2][4][4];
+  int nspin;
+};
+
+__device__ double pair_energy(const ElemDev& ef, const ElemDev& es, int zf, int zs, const double d[3],
+                              const PairDens& D, const double2* __restrict__ pairpar, double4 vpar) {
+  const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  const int nf = ef.n_orb, ns = es.n_orb;
+  const int npf = npair_of(nf), nps = npair_of(ns);
+  double w[100];
+  double R[3][3];
+  rot_to_x(-d[0] / r, -d[1] / r, -d[2] / r, R);
+  if (nf < 4 && ns < 4) {
+    double cf, pt;
+    feather(r, cf, pt);
+    w[0] = EV / sqrt(r * r + sq(ef.rho0 + es.rho0)) * cf + pt * (1.0 - cf);
+  } else if (ns < 4) {
+    double ri[4];
+    local_xh(ef, es, r, ri);
+    w[0] = ri[0];
+    for (int k = 0; k < 3; ++k) w[pack_idx(k + 1, 0)] = ri[1] * R[0][k];
+    for (int k = 0; k < 3; ++k)
+      for (int l = 0; l <= k; ++l)
+        w[pack_idx(k + 1, l + 1)] = ri[2] * (R[0][k] * R[0][l]) + ri[3] * (R[1][k] * R[1][l] + R[2][k] * R[2][l]);
+  } else {
+    double ri[22];
+    local_xx(ef, es, r, ri);
+    rotate_xx_store(ri, R, w);
+  }
+  double S[4][4];
+  overlap_block(ef, es, d, r, S, nullptr);
+  double e = 0.0;
+  for (int i = 0; i < nf; ++i) {
+    const double bi = i == 0 ? ef.beta_s : ef.beta_p;
+    for (int j = 0; j < ns; ++j) e += D.pfs[i][j] * (bi + (j == 0 ? es.beta_s : es.beta_p)) * S[i][j];
+  }
+  for (int a = 0; a < npf; ++a) {
+    double row = 0.0;
+    for (int b = 0; b < nps; ++b) row += w[a * nps + b] * D.ps[b];
+    e += D.pf[a] * (row - es.core_charge * w[a * nps]);
+  }
+  for (int b = 0; b < nps; ++b) e -= ef.core_charge * D.ps[b] * w[b];
+  const double kx = D.nspin == 1 ? -2.0 : -1.0;
+  for (int sp = 0; sp < D.nspin; ++sp) {
+    double ex = 0.0;
+    for (int mu = 0; mu < nf; ++mu)
+      for (int nu = 0; nu < nf; ++nu)
+        for (int la = 0; la < ns; ++la)
+          for (int si = 0; si < ns; ++si)
+            ex += D.xs[sp][mu][la] * D.xs[sp][nu][si] * w[pack_idx(mu, nu) * nps + pack_idx(la, si)];
+    e += kx * ex;
+  }
+  return e + pair_core(ef, es, zf, zs, r, pairpar, vpar);
+}
+
+// dE/dd of every pair of the molecules whose result holds a converged density (out[mol].p_slot);
+// pairs of MNDO/d molecules and pairs whose overlap needs the numerical quadrature get NaN, which
+// the host reports as an unsupported gradient.
+__global__ void __launch_bounds__(128) k_pair_grad(int64_t n_pairs, const PairDev* __restrict__ pairs,
+                                                    const MolDev* __restrict__ mols, const MolOut* __restrict__ out,
+                                                    const int* __restrict__ atom_z, const int* __restrict__ atom_ao,
+                                                    const double* __restrict__ xyz, const ElemDev* __restrict__ elem,
+                                                    const double2* __restrict__ pairpar, double4 vpar,
+                                                    const double* __restrict__ slab, double* __restrict__ gpair) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_pairs) return;
+  const PairDev P = pairs[g];
+  const MolDev& M = mols[P.mol];
+  const MolOut& O = out[P.mol];
+  double* go = gpair + 3 * g;
+  if (M.dpath || P.pad > 0 || O.p_slot < 0) {
+    go[0] = go[1] = go[2] = NAN;
+    return;
+  }
+  const int zf = atom_z[P.f], zs = atom_z[P.s];
+  const ElemDev ef = elem[zf];
+  const ElemDev es = elem[zs];
+  const int ld = M.ld, of = atom_ao[P.f], os = atom_ao[P.s];
+  const int nf = ef.n_orb, ns = es.n_orb;
+  const double* Pa = slab + M.mat0 + (int64_t)O.p_slot * ld * ld;
+  const double* Pb = M.uhf ? slab + M.mat0 + (int64_t)O.pb_slot * ld * ld : nullptr;
+  auto tot = [&](int i, int j) { return Pb ? Pa[(int64_t)i * ld + j] + Pb[(int64_t)i * ld + j] : Pa[(int64_t)i * ld + j]; };
+  PairDens D;
+  D.nspin = Pb ? 2 : 1;
+  for (int mu = 0; mu < nf; ++mu)
+    for (int nu = 0; nu <= mu; ++nu) D.pf[pack_idx(mu, nu)] = (mu == nu ? 1.0 : 2.0) * tot(of + mu, of + nu);
+  for (int la = 0; la < ns; ++la)
+    for (int si = 0; si <= la; ++si) D.ps[pack_idx(la, si)] = (la == si ? 1.0 : 2.0) * tot(os + la, os + si);
+  for (int i = 0; i < nf; ++i)
+    for (int j = 0; j < ns; ++j) {
+      D.pfs[i][j] = tot(of + i, os + j);
+      // RHF: P/2 for both spins (the exchange weight carries the factor 2); UHF: P^a and P^b.
+      D.xs[0][i][j] = Pb ? Pa[(int64_t)(of + i) * ld + os + j] : 0.5 * Pa[(int64_t)(of + i) * ld + os + j];
+      D.xs[1][i][j] = Pb ? Pb[(int64_t)(of + i) * ld + os + j] : 0.0;
+    }
+  const double d0[3] = {xyz[3 * P.s] - xyz[3 * P.f], xyz[3 * P.s + 1] - xyz[3 * P.f + 1],
+                        xyz[3 * P.s + 2] - xyz[3 * P.f + 2]};
+  constexpr double h = 1.0e-3;
+  for (int k = 0; k < 3; ++k) {
+    double ev[4];
+    const double steps[4] = {h, -h, 2.0 * h, -2.0 * h};
+    for (int t = 0; t < 4; ++t) {
+      double d[3] = {d0[0], d0[1], d0[2]};
+      d[k] += steps[t];
+      ev[t] = pair_energy(ef, es, zf, zs, d, D, pairpar, vpar);
+    }
+    go[k] = (8.0 * (ev[0] - ev[1]) - (ev[2] - ev[3])) / (12.0 * h);
+  }
+}
+
+// Gradient of each listed atom (items: molecule, batch-global atom): its pairs' dE/dd in partner
+// order, + for the pair's second atom, - for its first.
+__global__ void __launch_bounds__(128) k_atom_grad(int n_items, const int2* __restrict__ items,
+                                                    const PairDev* __restrict__ pairs, const MolDev* __restrict__ mols,
+                                                    const double* __restrict__ gpair, double* __restrict__ gatom) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_items) return;
+  const int2 it = items[t];
+  const MolDev& M = mols[it.x];
+  const int a = it.y, la = a - M.atom0;
+  double g[3] = {0.0, 0.0, 0.0};
+  for (int lb = 0; lb < M.n_atoms; ++lb) {
+    if (lb == la) continue;
+    const int u = la < lb ? la : lb, v = la < lb ? lb : la;
+    const int64_t pi = M.pair0 + tri_index(u, v, M.n_atoms);
+    const double sgn = pairs[pi].s == a ? 1.0 : -1.0;
+    for (int k = 0; k < 3; ++k) g[k] += sgn * gpair[3 * pi + k];
+  }
+  for (int k = 0; k < 3; ++k) gatom[3 * a + k] = g[k];
+}
+
+hipError_t launch_gradient(const BatchDev& b, int64_t n_pairs, const int2* items, int n_items, double* gpair,
+                           double* gatom, hipStream_t s) {
+  if (n_pairs > 0)
+    hipLaunchKernelGGL(k_pair_grad, dim3((unsigned)((n_pairs + 127) / 128)), dim3(128), 0, s, n_pairs, b.pairs, b.mols,
+                       b.out, b.atom_z, b.atom_ao, b.xyz, b.elem, b.pairpar, b.vpar, b.slab, gpair);
+  if (n_items > 0)
+    hipLaunchKernelGGL(k_atom_grad, dim3((n_items + 127) / 128), dim3(128), 0, s, n_items, items, b.pairs, b.mols, gpair,
+                       gatom);
+  return hipGetLastError();
+}
+
 hipError_t launch_integrals(const BatchDev& b, int64_t n_pairs, int n_items, int n_mols, hipStream_t s) {
   if (b.n_dp > 0)
     hipLaunchKernelGGL(k_pairs_d, dim3((unsigned)b.n_dp), dim3(64), 0, s, b.d_pairs, b.pairs, b.mols, b.atom_z,

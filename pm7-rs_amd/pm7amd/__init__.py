@@ -96,7 +96,8 @@ ABI_FUNCTIONS = [
     "pm7_ctx_create", "pm7_ctx_destroy", "pm7_last_error", "pm7_abi_version", "pm7_params_load",
     "pm7_default_options", "pm7_run", "pm7_run_batch", "pm7_result_free", "pm7_core_hamiltonian",
     "pm7_fock_spin", "pm7_core_core_energy", "pm7_last_timings", "pm7_server_default_config",
-    "pm7_server_start", "pm7_server_submit", "pm7_server_poll", "pm7_server_stop",
+    "pm7_server_start", "pm7_server_submit", "pm7_server_poll", "pm7_server_stop", "pm7_gradient_batch",
+    "pm7_gradient",
 ]
 
 _lib = None
@@ -121,6 +122,10 @@ def library() -> C.CDLL:
     lib.pm7_run_batch.argtypes = [vp, C.c_int32, C.POINTER(_Molecule), C.POINTER(_Options), C.c_int32,
                                   C.POINTER(_Result)]
     lib.pm7_result_free.argtypes = [C.POINTER(_Result)]
+    lib.pm7_gradient_batch.argtypes = [vp, C.c_int32, C.POINTER(_Molecule), C.POINTER(_Options), C.c_int32,
+                                       C.POINTER(_Result), C.POINTER(C.c_double)]
+    lib.pm7_gradient.argtypes = [vp, C.POINTER(_Molecule), C.POINTER(_Options), C.POINTER(_Result),
+                                 C.POINTER(C.c_double)]
     lib.pm7_core_hamiltonian.argtypes = [vp, C.POINTER(_Molecule), C.POINTER(C.c_double), C.POINTER(C.c_int32)]
     lib.pm7_fock_spin.argtypes = [vp, C.POINTER(_Molecule), C.POINTER(C.c_double), C.POINTER(C.c_double),
                                   C.POINTER(C.c_double)]
@@ -440,6 +445,26 @@ class Engine:
 
     def run_batch(self, molecules: Sequence[Molecule], options=None, raise_on_error: bool = False):
         """Many molecules in one device pass; ``options`` is one Pm7Options or one per molecule."""
+        return self._batch(molecules, options, raise_on_error, None)
+
+    def gradient(self, molecule: Molecule, options: Optional[Pm7Options] = None, raise_on_error: bool = True):
+        """pm7-rs ``closed_form_gradient`` (src/gradient.rs:500-555): (result, gradient[n_atoms, 3]
+        in eV/Bohr at the converged density, geometric corrections included)."""
+        res, grads = self.gradient_batch([molecule], options, raise_on_error)
+        return res[0], grads[0]
+
+    def gradient_batch(self, molecules: Sequence[Molecule], options=None, raise_on_error: bool = False):
+        """Batched SCF + gradients: (results, [gradient[n_atoms_i, 3]]); NaN rows and status
+        UnsupportedFeature for molecules with MNDO/d pairs or valence shells n >= 4."""
+        flat = np.full(3 * sum(len(m.z) for m in molecules), np.nan)
+        res = self._batch(molecules, options, raise_on_error, flat)
+        grads, off = [], 0
+        for m in molecules:
+            grads.append(flat[off:off + 3 * len(m.z)].reshape(-1, 3))
+            off += 3 * len(m.z)
+        return res, grads
+
+    def _batch(self, molecules, options, raise_on_error, grad_flat):
         n = len(molecules)
         if n == 0:
             return []
@@ -454,7 +479,11 @@ class Engine:
             keep.append(k)
         c_opts = (_Options * len(opts_list))(*[o.to_c() for o in opts_list])
         c_res = (_Result * n)()
-        st = self._lib.pm7_run_batch(self._h, n, c_mols, c_opts, len(opts_list), c_res)
+        if grad_flat is None:
+            st = self._lib.pm7_run_batch(self._h, n, c_mols, c_opts, len(opts_list), c_res)
+        else:
+            st = self._lib.pm7_gradient_batch(self._h, n, c_mols, c_opts, len(opts_list), c_res,
+                                              grad_flat.ctypes.data_as(C.POINTER(C.c_double)))
         if st != 0:
             raise Pm7Error(st, self.last_error())
         out = _convert_all(c_res, n, self._lib.pm7_result_free)

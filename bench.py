@@ -95,6 +95,7 @@ def main() -> int:
     ap.add_argument("--large-only", action="store_true", help="only the large-system run")
     ap.add_argument("--d-batch", type=int, default=64,
                     help="S/P/Cl molecules (MNDO/d pairs, <= 196 AOs) batched in FP64 and mixed (0: skip)")
+    ap.add_argument("--gradient", type=int, default=1, help="time batched nuclear gradients (0: skip)")
     ap.add_argument("--serve-lanes", type=int, default=4)
     ap.add_argument("--serve-batch", type=int, default=64)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -329,6 +330,27 @@ def main() -> int:
         d_batch = d_out
         print(f"[bench] d batch: {d_out['fp64']} / {d_out['mixed']}", file=sys.stderr, flush=True)
 
+    # Nuclear gradients of the same 256-molecule batch: one SCF per molecule, then the pair and atom
+    # gradient kernels on the density left in HBM and the corrections' gradient on the host. Timed
+    # as whole calls through the Python mirror, beside the plain batched SCF call.
+    gradient = None
+    if args.gradient:
+        gradient = {"workload": "the 256-molecule organic batch", "timing": "whole call through the Python mirror"}
+        for precision, api_scf in (("fp64", api64), ("mixed", apimx)):
+            o = pm7amd.Pm7Options(precision=precision)
+            eng.gradient_batch(mols, o)
+            reps = max(1, min(args.steps, 3))
+            t = time.perf_counter()
+            for _ in range(reps):
+                g_res, g_vals = eng.gradient_batch(mols, o)
+            dt = (time.perf_counter() - t) / reps
+            gradient[precision] = {
+                "gradients_per_s": round(len(mols) / dt, 3), "ms_per_call": round(dt * 1e3, 3),
+                "scf_only_ms_per_call": round(float(np.mean(api_scf)), 3),
+                "failed": sum(r.status != 0 for r in g_res),
+                "max_abs_gradient_ev_bohr": round(max(float(np.max(np.abs(g))) for g in g_vals), 6)}
+        print(f"[bench] gradient: {gradient}", file=sys.stderr, flush=True)
+
     # One molecule at a time (the plain FP64 GPU path the batched path is compared with).
     single = mols[: args.single]
     for m in single[:2]:
@@ -437,6 +459,7 @@ def main() -> int:
         "large_system": large,
         "hbm_copy_probe_gbs": copy_gbs,
         "d_batch": d_batch,
+        "gradient": gradient,
         "single_molecule_fp64": {"solves_per_s": round(single_rate, 3), "molecules": len(single),
                                  "batched_speedup": round((solvesmx if head_mixed else solves64) / single_rate, 3)},
         "kernel_ms": {"integrals": round(float(np.mean([k[0] for k in kern64])), 4),

@@ -286,7 +286,11 @@ double eh_plus(const std::vector<P3>& c, const std::vector<int>& z, const HB& b)
 
 }  // namespace
 
-double dispersion_kcal(const std::vector<int>& z, const std::vector<double>& xyz) {
+namespace {
+// D2 dispersion energy; with g, also its gradient (kcal/mol/Bohr, accumulated into g[3n]) at the
+// same bond topology: the coordination counts that pick the carbon C6 are step functions of the
+// distances and contribute nothing, so each pair's term differentiates through its distance only.
+double dispersion_impl(const std::vector<int>& z, const std::vector<double>& xyz, double* g) {
   auto c6t = [](int q) {
     switch (q) {
       case 1: return 0.16; case 2: return 0.084; case 5: return 5.79; case 6: return 1.65; case 7: return 1.11;
@@ -336,9 +340,32 @@ double dispersion_kcal(const std::vector<int>& z, const std::vector<double>& xyz
       double rnm = std::sqrt(d2(at(xyz, i), at(xyz, j))) * (A0 * 0.1);
       double denom = std::exp((rnm / (S * r0) - 1.0) * -ALPHA) + 1.0;
       sum -= CSCALE * (c6 / (1000.0 * 4.184)) / (std::pow(rnm, 6) * denom);
+      if (g) {
+        // E = -K / (rnm^6 D), D = 1 + exp(-ALPHA (rnm / (S r0) - 1)):
+        // dE/drnm = K (6 / rnm + D' / D) / (rnm^6 D), D' = -(ALPHA / (S r0)) (D - 1).
+        const double k = CSCALE * (c6 / (1000.0 * 4.184));
+        const double dd = -(ALPHA / (S * r0)) * (denom - 1.0);
+        const double de_drnm = k * (6.0 / rnm + dd / denom) / (std::pow(rnm, 6) * denom);
+        const double r_bohr = rnm / (A0 * 0.1);
+        const double f = de_drnm * (A0 * 0.1) / r_bohr;
+        for (int c = 0; c < 3; ++c) {
+          const double dx = xyz[3 * i + c] - xyz[3 * j + c];
+          g[3 * i + c] += f * dx;
+          g[3 * j + c] -= f * dx;
+        }
+      }
     }
   }
   return sum;
+}
+}  // namespace
+
+double dispersion_kcal(const std::vector<int>& z, const std::vector<double>& xyz) {
+  return dispersion_impl(z, xyz, nullptr);
+}
+
+void dispersion_grad_kcal(const std::vector<int>& z, const std::vector<double>& xyz, double* g) {
+  (void)dispersion_impl(z, xyz, g);
 }
 
 double hbond_kcal(const std::vector<int>& z, const std::vector<double>& xyz) {

@@ -932,15 +932,40 @@ struct GradReq {
   std::vector<int64_t> off;  // by Job::idx, in doubles
 };
 
-// Geometric corrections of one molecule in kcal/mol (the terms host_corrections adds).
-double corrections_kcal(const Job& j, const std::vector<double>& xyz) {
-  if (j.o.method == PM7_METHOD_PM7_MINUS) return 0.0;
-  return dispersion_kcal(j.z, xyz) + hbond_kcal(j.z, xyz) + c_triple_kcal(j.z, xyz) + si_o_h_kcal(j.z, xyz);
+// Gradient of the geometric corrections host_corrections adds, kcal/mol/Bohr into g[3 n]:
+// dispersion analytically; the EH+, C#C and Si-O-H terms, each only when it is non-zero at this
+// geometry (zero means the molecule has no such contact or motif), by differences with
+// h = 1e-5 Bohr. These terms select contacts with hard tests (distance and angle cutoffs, first
+// match), so a geometry can sit exactly where a test flips and the energy jumps. A jump J shows
+// as one one-sided quotient ~J/h (above 10 kcal/mol/Bohr and ten times the other): the other side
+// keeps the selection made at this geometry, and its derivative is the one reported (as an
+// analytic derivative of the selected terms would be); otherwise the central difference (at a
+// kink, the mean of the two one-sided slopes).
+void corrections_grad_kcal(const Job& j, double* g) {
+  dispersion_grad_kcal(j.z, j.xyz, g);
+  using Term = double (*)(const std::vector<int>&, const std::vector<double>&);
+  const Term terms[3] = {hbond_kcal, c_triple_kcal, si_o_h_kcal};
+  std::vector<double> x = j.xyz;
+  constexpr double h = 1.0e-5;
+  for (Term f : terms) {
+    const double e0 = f(j.z, j.xyz);
+    if (e0 == 0.0) continue;
+    for (size_t c = 0; c < x.size(); ++c) {
+      x[c] = j.xyz[c] + h;
+      const double ep = f(j.z, x);
+      x[c] = j.xyz[c] - h;
+      const double em = f(j.z, x);
+      x[c] = j.xyz[c];
+      const double df = (ep - e0) / h, db = (e0 - em) / h;
+      const double big = std::max(std::fabs(df), std::fabs(db)), small = std::min(std::fabs(df), std::fabs(db));
+      const bool jump = big > 10.0 && small < 0.1 * big;
+      g[c] += jump ? (std::fabs(df) < std::fabs(db) ? df : db) : 0.5 * (df + db);
+    }
+  }
 }
 
 // Device gradient of every job of a finished batch (enqueued on stream 0 after the SCF and the
-// gather), then, on the host, the corrections' gradient by the same fourth-order central difference
-// (h = 1e-4 Bohr) and the assembly into the caller's array. A molecule whose pairs have no device
+// gather), then, on the host, the corrections' gradient and the assembly into the caller's array. A molecule whose pairs have no device
 // gradient (MNDO/d pairs, numerical overlaps) gets NaN and PM7_ERR_UNSUPPORTED.
 pm7_status launch_gradient_batch(Lane& L, const Prepared& P, int* n_items_out) {
   std::vector<int2> items;
@@ -989,18 +1014,9 @@ pm7_status collect_gradient(Lane& L, const Prepared& P, const GradReq& g, pm7_re
         continue;
       }
       if (j.o.method == PM7_METHOD_PM7_MINUS) continue;
-      std::vector<double> x = j.xyz;
-      constexpr double h = 1.0e-4;
-      for (int c = 0; c < 3 * na; ++c) {
-        double e[4];
-        const double steps[4] = {h, -h, 2.0 * h, -2.0 * h};
-        for (int t2 = 0; t2 < 4; ++t2) {
-          x[c] = j.xyz[c] + steps[t2];
-          e[t2] = corrections_kcal(j, x);
-        }
-        x[c] = j.xyz[c];
-        out[c] += (8.0 * (e[0] - e[1]) - (e[2] - e[3])) / (12.0 * h) / EV_TO_KCAL;
-      }
+      std::vector<double> gc((size_t)3 * na, 0.0);
+      corrections_grad_kcal(j, gc.data());
+      for (int c = 0; c < 3 * na; ++c) out[c] += gc[c] / EV_TO_KCAL;
     }
   };
   const int nt = P.jobs.size() >= 8 ? std::min(16, std::max(1, (int)std::thread::hardware_concurrency())) : 1;

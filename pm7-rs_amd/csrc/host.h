@@ -46,6 +46,8 @@ void to_device_table(const ParamSet& P, std::vector<ElemDev>& out);
 
 // Geometric post-SCF corrections in kcal/mol (coordinates in Bohr).
 double dispersion_kcal(const std::vector<int>& z, const std::vector<double>& xyz);
+// Analytic gradient of dispersion_kcal, kcal/mol/Bohr, accumulated into g[3 n].
+void dispersion_grad_kcal(const std::vector<int>& z, const std::vector<double>& xyz, double* g);
 double hbond_kcal(const std::vector<int>& z, const std::vector<double>& xyz);
 double c_triple_kcal(const std::vector<int>& z, const std::vector<double>& xyz);
 double si_o_h_kcal(const std::vector<int>& z, const std::vector<double>& xyz);

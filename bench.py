@@ -137,6 +137,24 @@ def main() -> int:
         torch.cuda.empty_cache()
         return round(gbs, 1)
 
+    def dgemm_tflops(n=8192, reps=4):
+        # The matrix-core FP64 rate this box reaches through the vendor BLAS (torch -> rocBLAS/
+        # hipBLASLt dgemm), the measured ceiling beside the published 78.6 TFLOP/s.
+        a = torch.randn(n, n, dtype=torch.float64, device="cuda")
+        b = torch.randn(n, n, dtype=torch.float64, device="cuda")
+        c = a @ b
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            c = a @ b
+        e1.record()
+        torch.cuda.synchronize()
+        tf = 2.0 * n ** 3 * reps / (e0.elapsed_time(e1) * 1e-3) / 1e12
+        del a, b, c
+        torch.cuda.empty_cache()
+        return round(tf, 2)
+
     import pm7amd
     from pm7amd import workloads as W
 
@@ -298,6 +316,7 @@ def main() -> int:
 
     large = large_system(eng) if args.large_waters > 0 else None
     copy_gbs = copy_bandwidth_gbs()
+    dgemm_tf = dgemm_tflops()
 
     # A batch of molecules holding S, P or Cl: every pair through the MNDO/d pair kernel, the
     # 45-word spd one-centre blocks in the Fock build; the megakernel classes only (<= 196 AOs).
@@ -373,7 +392,8 @@ def main() -> int:
     roofline = {"bound": "mfma", "kernel": kname,
                 "achieved": round(achieved_tf, 6), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved_tf / FP64_PEAK_TFLOPS, 8), "traffic": measured_traffic(kname),
-                "avg_launch_ms": round(cls_ms[dom], 4), "class_ms": [round(x, 4) for x in cls_ms]}
+                "avg_launch_ms": round(cls_ms[dom], 4), "class_ms": [round(x, 4) for x in cls_ms],
+                "measured_dgemm_peak": dgemm_tf, "frac_of_measured_dgemm": round(achieved_tf / dgemm_tf, 8)}
 
     cpu = None
     parity = None
@@ -458,6 +478,7 @@ def main() -> int:
         "serving": serving,
         "large_system": large,
         "hbm_copy_probe_gbs": copy_gbs,
+        "dgemm_probe_tflops": dgemm_tf,
         "d_batch": d_batch,
         "gradient": gradient,
         "single_molecule_fp64": {"solves_per_s": round(single_rate, 3), "molecules": len(single),

@@ -965,8 +965,9 @@ void corrections_grad_kcal(const Job& j, double* g) {
 }
 
 // Device gradient of every job of a finished batch (enqueued on stream 0 after the SCF and the
-// gather), then, on the host, the corrections' gradient and the assembly into the caller's array. A molecule whose pairs have no device
-// gradient (MNDO/d pairs, numerical overlaps) gets NaN and PM7_ERR_UNSUPPORTED.
+// gather); collect_gradient adds the corrections' gradient on the host and fills the caller's
+// array. A molecule with a pair that has no device gradient (an sp atom whose valence shell has
+// n >= 4, i.e. a numerical overlap) gets NaN and PM7_ERR_UNSUPPORTED.
 pm7_status launch_gradient_batch(Lane& L, const Prepared& P, int* n_items_out) {
   std::vector<int2> items;
   int n_atoms = 0;
@@ -1009,7 +1010,7 @@ pm7_status collect_gradient(Lane& L, const Prepared& P, const GradReq& g, pm7_re
         for (int k = 0; k < 3 * na; ++k) out[k] = std::nan("");
         if (r.status == PM7_OK || r.status == PM7_ERR_SCF_NOT_CONVERGED) {
           r.status = PM7_ERR_UNSUPPORTED;
-          set_msg(&r, "gradient: molecules with MNDO/d pairs or valence shells n >= 4 have no gradient here");
+          set_msg(&r, "gradient: sp atoms with valence shells n >= 4 (numerical overlaps) have no gradient here");
         }
         continue;
       }

@@ -685,9 +685,9 @@ __device__ double pair_energy(const ElemDev& ef, const ElemDev& es, int zf, int 
   return e + pair_core(ef, es, zf, zs, r, pairpar, vpar);
 }
 
-// dE/dd of every pair of the molecules whose result holds a converged density (out[mol].p_slot);
-// pairs of MNDO/d molecules and pairs whose overlap needs the numerical quadrature get NaN, which
-// the host reports as an unsupported gradient.
+// dE/dd of every sp-path pair of the molecules whose result holds a converged density
+// (out[mol].p_slot); pairs whose overlap needs the numerical quadrature get NaN, which the host
+// reports as an unsupported gradient. Pairs of MNDO/d molecules are k_pair_grad_d's.
 __global__ void __launch_bounds__(128) k_pair_grad(int64_t n_pairs, const PairDev* __restrict__ pairs,
                                                     const MolDev* __restrict__ mols, const MolOut* __restrict__ out,
                                                     const int* __restrict__ atom_z, const int* __restrict__ atom_ao,
@@ -700,7 +700,8 @@ __global__ void __launch_bounds__(128) k_pair_grad(int64_t n_pairs, const PairDe
   const MolDev& M = mols[P.mol];
   const MolOut& O = out[P.mol];
   double* go = gpair + 3 * g;
-  if (M.dpath || P.pad > 0 || O.p_slot < 0) {
+  if (M.dpath) return;  // k_pair_grad_d owns the pairs of molecules with a d shell
+  if (P.pad > 0 || O.p_slot < 0) {
     go[0] = go[1] = go[2] = NAN;
     return;
   }
@@ -740,6 +741,94 @@ __global__ void __launch_bounds__(128) k_pair_grad(int64_t n_pairs, const PairDe
   }
 }
 
+// dE/dd of every pair of an MNDO/d molecule, one 64-lane workgroup per pair: the pair's energy at
+// fixed density (the terms of pair_energy, with the spd blocks of pair_d_eval: w, the electron-core
+// blocks, the resonance block, core-core) at d +- h, d +- 2h along each axis, summed over the lanes
+// in a fixed order, and the same fourth-order difference.
+__global__ void __launch_bounds__(64) k_pair_grad_d(const int64_t* __restrict__ dpairs, const PairDev* __restrict__ pairs,
+                                                     const MolDev* __restrict__ mols, const MolOut* __restrict__ out,
+                                                     const int* __restrict__ atom_z, const int* __restrict__ atom_ao,
+                                                     const double* __restrict__ xyz, const ElemDev* __restrict__ elem,
+                                                     const double2* __restrict__ pairpar, double4 vpar,
+                                                     const double* __restrict__ slab, double* __restrict__ gpair) {
+  __shared__ DPairShared s;
+  __shared__ double pf[45], ps[45], pfs[81], xs[2][81], red[64], ev[4];
+  const int lane = threadIdx.x;
+  const int64_t g = dpairs[blockIdx.x];
+  const PairDev P = pairs[g];
+  const MolDev& M = mols[P.mol];
+  const MolOut& O = out[P.mol];
+  double* go = gpair + 3 * g;
+  if (O.p_slot < 0) {
+    if (lane == 0) go[0] = go[1] = go[2] = NAN;
+    return;
+  }
+  const int zf = atom_z[P.f], zs = atom_z[P.s];
+  const ElemDev ef = elem[zf];
+  const ElemDev es = elem[zs];
+  const int nf = ef.n_orb, ns = es.n_orb, npf = npair_of(nf), nps = npair_of(ns);
+  const int ld = M.ld, of = atom_ao[P.f], os = atom_ao[P.s];
+  const double* Pa = slab + M.mat0 + (int64_t)O.p_slot * ld * ld;
+  const double* Pb = M.uhf ? slab + M.mat0 + (int64_t)O.pb_slot * ld * ld : nullptr;
+  auto tot = [&](int i, int j) { return Pb ? Pa[(int64_t)i * ld + j] + Pb[(int64_t)i * ld + j] : Pa[(int64_t)i * ld + j]; };
+  for (int e = lane; e < npf; e += 64) {
+    const int mu = c_dt.pa[e], nu = c_dt.pb[e];
+    pf[e] = (mu == nu ? 1.0 : 2.0) * tot(of + mu, of + nu);
+  }
+  for (int e = lane; e < nps; e += 64) {
+    const int la = c_dt.pa[e], si = c_dt.pb[e];
+    ps[e] = (la == si ? 1.0 : 2.0) * tot(os + la, os + si);
+  }
+  for (int t = lane; t < nf * ns; t += 64) {
+    const int i = t / ns, j = t - i * ns;
+    pfs[t] = tot(of + i, os + j);
+    const double a = Pa[(int64_t)(of + i) * ld + os + j];
+    xs[0][t] = Pb ? a : 0.5 * a;  // RHF: P/2 stands for both spins
+    xs[1][t] = Pb ? Pb[(int64_t)(of + i) * ld + os + j] : 0.0;
+  }
+  const int nspin = Pb ? 2 : 1;
+  const double kx = Pb ? -1.0 : -2.0;
+  const double d0[3] = {xyz[3 * P.s] - xyz[3 * P.f], xyz[3 * P.s + 1] - xyz[3 * P.f + 1],
+                        xyz[3 * P.s + 2] - xyz[3 * P.f + 2]};
+  __syncthreads();
+  constexpr double h = 1.0e-3;
+  const double steps[4] = {h, -h, 2.0 * h, -2.0 * h};
+  for (int k = 0; k < 3; ++k) {
+    for (int t = 0; t < 4; ++t) {
+      double d[3] = {d0[0], d0[1], d0[2]};
+      d[k] += steps[t];
+      pair_d_eval(ef, es, zf, zs, d, pairpar, vpar, s);
+      double acc = 0.0;
+      for (int idx = lane; idx < npf * nps; idx += 64) {
+        const int ij = idx / nps, kl = idx - ij * nps;
+        acc += pf[ij] * s.L[idx] * ps[kl];
+      }
+      for (int e = lane; e < npf + nps; e += 64) acc += (e < npf ? pf[e] : ps[e - npf]) * s.ecb[e];
+      for (int t2 = lane; t2 < nf * ns; t2 += 64) acc += 2.0 * pfs[t2] * s.hres[t2];
+      for (int q = lane; q < nf * nf; q += 64) {
+        const int mu = q / nf, nu = q - mu * nf;
+        const double* row = s.L + pack_idx(mu, nu) * nps;
+        double ex = 0.0;
+        for (int sp = 0; sp < nspin; ++sp)
+          for (int la = 0; la < ns; ++la)
+            for (int si = 0; si < ns; ++si)
+              ex += xs[sp][mu * ns + la] * xs[sp][nu * ns + si] * row[pack_idx(la, si)];
+        acc += kx * ex;
+      }
+      red[lane] = acc;
+      __syncthreads();
+      if (lane == 0) {
+        double e = s.e_core;
+        for (int i = 0; i < 64; ++i) e += red[i];
+        ev[t] = e;
+      }
+      __syncthreads();
+    }
+    if (lane == 0) go[k] = (8.0 * (ev[0] - ev[1]) - (ev[2] - ev[3])) / (12.0 * h);
+    __syncthreads();
+  }
+}
+
 // Gradient of each listed atom (items: molecule, batch-global atom): its pairs' dE/dd in partner
 // order, + for the pair's second atom, - for its first.
 __global__ void __launch_bounds__(128) k_atom_grad(int n_items, const int2* __restrict__ items,
@@ -766,6 +855,9 @@ hipError_t launch_gradient(const BatchDev& b, int64_t n_pairs, const int2* items
   if (n_pairs > 0)
     hipLaunchKernelGGL(k_pair_grad, dim3((unsigned)((n_pairs + 127) / 128)), dim3(128), 0, s, n_pairs, b.pairs, b.mols,
                        b.out, b.atom_z, b.atom_ao, b.xyz, b.elem, b.pairpar, b.vpar, b.slab, gpair);
+  if (b.n_dp > 0)
+    hipLaunchKernelGGL(k_pair_grad_d, dim3((unsigned)b.n_dp), dim3(64), 0, s, b.d_pairs, b.pairs, b.mols, b.out,
+                       b.atom_z, b.atom_ao, b.xyz, b.elem, b.pairpar, b.vpar, b.slab, gpair);
   if (n_items > 0)
     hipLaunchKernelGGL(k_atom_grad, dim3((n_items + 127) / 128), dim3(128), 0, s, n_items, items, b.pairs, b.mols, gpair,
                        gatom);

@@ -413,43 +413,41 @@ __host__ __device__ inline int64_t dpair_words(int nf, int ns) {
   return (int64_t)npair_of(nf) * npair_of(ns) + npair_of(nf) + npair_of(ns);
 }
 
-__global__ void __launch_bounds__(64) k_pairs_d(const int64_t* __restrict__ dpairs, const PairDev* __restrict__ pairs,
-                                                const MolDev* __restrict__ mols, const int* __restrict__ atom_z,
-                                                const int* __restrict__ atom_ao, const double* __restrict__ xyz,
-                                                const ElemDev* __restrict__ elem, const double2* __restrict__ pairpar,
-                                                double4 vpar, double* __restrict__ w, double* __restrict__ slab,
-                                                double* __restrict__ e_core) {
-  __shared__ double sL[45 * 45];
-  __shared__ double sT[45 * 45];
-  __shared__ double sR[81];
-  __shared__ double sri[23];
-  __shared__ double score[11][3];
-  __shared__ double sS[4][4][4];
-  __shared__ double sC[4][6][6];
+// LDS working set of one MNDO/d pair evaluation by a 64-lane workgroup.
+struct DPairShared {
+  double L[45 * 45];
+  double T[45 * 45];
+  double R[81];
+  double ri[23];
+  double core[11][3];
+  double S[4][4][4];
+  double C[4][6][6];
+  double ecb[90];   // electron-core blocks E = R^T E_local R: first atom's packed pairs, then the second's
+  double hres[81];  // resonance block (beta_i + beta_j) S_ij / 2, nf x ns row-major
+  double e_core;    // core-core energy
+};
+
+// Everything one MNDO/d pair contributes at separation d (second atom minus first, Bohr): the
+// rotated two-electron block (s.L, npf x nps), the electron-core blocks, the resonance block and
+// the core-core energy. All 64 lanes call it; it ends with the results visible to all of them.
+__device__ void pair_d_eval(const ElemDev& ef, const ElemDev& es, int zf, int zs, const double d[3],
+                            const double2* __restrict__ pairpar, double4 vpar, DPairShared& s) {
   const int lane = threadIdx.x;
-  const int64_t g = dpairs[blockIdx.x];
-  const PairDev P = pairs[g];
-  const MolDev& M = mols[P.mol];
-  const int zf = atom_z[P.f], zs = atom_z[P.s];
-  const ElemDev ef = elem[zf];
-  const ElemDev es = elem[zs];
   const int nf = ef.n_orb, ns = es.n_orb, npf = npair_of(nf), nps = npair_of(ns);
-  const double d[3] = {xyz[3 * P.s] - xyz[3 * P.f], xyz[3 * P.s + 1] - xyz[3 * P.f + 1],
-                       xyz[3 * P.s + 2] - xyz[3 * P.f + 2]};
   const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
   const bool with_d = nf == 9 || ns == 9;
   double cf, pt;
   feather(r, cf, pt);
   // Stage 1: rotation, sp local integrals, electron-core terms, shell overlaps (lanes in parallel).
-  if (lane == 0) orbital_rotation(d, r, with_d, sR);
-  if (lane == 1) local_sp_ri(ef, es, r, sri);
+  if (lane == 0) orbital_rotation(d, r, with_d, s.R);
+  if (lane == 1) local_sp_ri(ef, es, r, s.ri);
   if (lane == 2) {
     double core[11][3];
     for (int k = 0; k < 11; ++k) core[k][0] = core[k][1] = core[k][2] = 0.0;
     local_core_sp(ef, es, r, core);
     for (int k = 1; k <= 4; ++k) {
-      score[k][1] = core[k][1];
-      score[k][2] = core[k][2];
+      s.core[k][1] = core[k][1];
+      s.core[k][2] = core[k][2];
     }
   }
   if (lane >= 4 && lane < 16) {  // d electron-core terms: 6 distributions x 2 sides
@@ -459,9 +457,9 @@ __global__ void __launch_bounds__(64) k_pairs_d(const int64_t* __restrict__ dpai
     double v = 0.0;
     if (side == 2 && ns == 9) v = -local_multipole(ef, es, 0, dist[k], 1, r) * (EV * ef.core_charge);
     if (side == 1 && nf == 9) v = -local_multipole(ef, es, dist[k], 0, 2, r) * (EV * es.core_charge);
-    score[5 + k][side] = v;
+    s.core[5 + k][side] = v;
   }
-  if (lane == 20) overlap_directions(d, max(nf, ns), sC);
+  if (lane == 20) overlap_directions(d, max(nf, ns), s.C);
   {
     // shell overlaps s[i][j][k]: lanes 32.. take (i, j, k) in 1..3
     const int t = lane - 32;
@@ -476,7 +474,7 @@ __global__ void __launch_bounds__(64) k_pairs_d(const int64_t* __restrict__ dpai
         const int pq1 = i == 3 ? ef.n_d : ef.n_sp, pq2 = j == 3 ? es.n_d : es.n_sp;
         v = shell_overlap(max(pq1, i), max(pq2, j), i - 1, j - 1, k - 1, ei[i - 1], ej[j - 1], r);
       }
-      sS[i][j][k] = v;
+      s.S[i][j][k] = v;
     }
   }
   __syncthreads();
@@ -484,10 +482,10 @@ __global__ void __launch_bounds__(64) k_pairs_d(const int64_t* __restrict__ dpai
   if (lane < 20) {
     const int k = 1 + lane / 2, side = 1 + (lane & 1);
     if (k <= 10) {
-      double v = score[k][side] * cf;
+      double v = s.core[k][side] * cf;
       if (k == 1 || k == 3 || k == 4 || k == 7 || k == 9 || k == 10)
         v += (1.0 / r) * (-EV * (side == 1 ? es.core_charge : ef.core_charge)) * (1.0 - cf);
-      score[k][side] = v;
+      s.core[k][side] = v;
     }
   }
   const double point_add = pt * (1.0 - cf);
@@ -496,12 +494,12 @@ __global__ void __launch_bounds__(64) k_pairs_d(const int64_t* __restrict__ dpai
     double v;
     if (ij < 10 && kl < 10) {
       const int k = c_dt.sp_local[ij][kl];
-      v = k ? sri[k] : 0.0;
+      v = k ? s.ri[k] : 0.0;
     } else {
       v = local_multipole(ef, es, ij, kl, 0, r) * EV * cf;
       if (c_dt.pa[ij] == c_dt.pb[ij] && c_dt.pa[kl] == c_dt.pb[kl]) v += point_add;
     }
-    sL[idx] = v;
+    s.L[idx] = v;
   }
   __syncthreads();
   // Stage 3: rotate the second index, then the first (only same-class distributions mix).
@@ -510,8 +508,8 @@ __global__ void __launch_bounds__(64) k_pairs_d(const int64_t* __restrict__ dpai
     const int cl = c_dt.cls[KL];
     double acc = 0.0;
     for (int kl = 0; kl < nps; ++kl)
-      if (c_dt.cls[kl] == cl) acc += sL[ij * nps + kl] * dist_coef(sR, KL, kl);
-    sT[idx] = acc;
+      if (c_dt.cls[kl] == cl) acc += s.L[ij * nps + kl] * dist_coef(s.R, KL, kl);
+    s.T[idx] = acc;
   }
   __syncthreads();
   for (int idx = lane; idx < npf * nps; idx += 64) {
@@ -519,14 +517,14 @@ __global__ void __launch_bounds__(64) k_pairs_d(const int64_t* __restrict__ dpai
     const int cl = c_dt.cls[IJ];
     double acc = 0.0;
     for (int ij = 0; ij < npf; ++ij)
-      if (c_dt.cls[ij] == cl) acc += dist_coef(sR, IJ, ij) * sT[ij * nps + KL];
-    sL[idx] = acc;
+      if (c_dt.cls[ij] == cl) acc += dist_coef(s.R, IJ, ij) * s.T[ij * nps + KL];
+    s.L[idx] = acc;
   }
   __syncthreads();
   // Stage 4: PM7 d-imbalance correction (src/mndod_twocenter.rs:918-1000), one lane.
   if (lane == 0 && (ef.iod > 0 || es.iod > 0)) {
-    auto W = [&](int ij, int kl) -> double& { return sL[ij * nps + kl]; };
-    const double w00 = sL[0];
+    auto W = [&](int ij, int kl) -> double& { return s.L[ij * nps + kl]; };
+    const double w00 = s.L[0];
     const int Pp[3] = {2, 5, 9}, Dd[5] = {14, 20, 27, 35, 44};
     if (ef.iod > 0) {
       if (ns > 1) {
@@ -535,18 +533,18 @@ __global__ void __launch_bounds__(64) k_pairs_d(const int64_t* __restrict__ dpai
         sum = w00 - sum * (1.0 / 15.0);
         for (int a : Dd)
           for (int b : Pp) W(a, b) += sum;
-        const double s = score[1][1] - (score[3][1] + score[4][1] * 2.0) * (1.0 / 3.0);
-        score[3][1] += s;
-        score[4][1] += s;
+        const double sh = s.core[1][1] - (s.core[3][1] + s.core[4][1] * 2.0) * (1.0 / 3.0);
+        s.core[3][1] += sh;
+        s.core[4][1] += sh;
       }
       double sum = 0.0;
       for (int a : Dd) sum += W(a, 0);
       sum = w00 - sum * (1.0 / 5.0);
       for (int a : Dd) W(a, 0) += sum;
-      const double s = score[1][1] - (score[7][1] + score[9][1] * 2.0 + score[10][1] * 2.0) * (1.0 / 5.0);
-      score[7][1] += s;
-      score[9][1] += s;
-      score[10][1] += s;
+      const double sh = s.core[1][1] - (s.core[7][1] + s.core[9][1] * 2.0 + s.core[10][1] * 2.0) * (1.0 / 5.0);
+      s.core[7][1] += sh;
+      s.core[9][1] += sh;
+      s.core[10][1] += sh;
     }
     if (es.iod > 0) {
       if (ef.iod > 0) {
@@ -564,25 +562,23 @@ __global__ void __launch_bounds__(64) k_pairs_d(const int64_t* __restrict__ dpai
         sum = w00 - sum * (1.0 / 15.0);
         for (int a : Pp)
           for (int b : Dd) W(a, b) += sum;
-        const double s = score[1][2] - (score[3][2] + score[4][2] * 2.0) * (1.0 / 3.0);
-        score[3][2] += s;
-        score[4][2] += s;
+        const double sh = s.core[1][2] - (s.core[3][2] + s.core[4][2] * 2.0) * (1.0 / 3.0);
+        s.core[3][2] += sh;
+        s.core[4][2] += sh;
       }
       double sum = 0.0;
       for (int b : Dd) sum += W(0, b);
       sum = w00 - sum * (1.0 / 5.0);
       for (int b : Dd) W(0, b) += sum;
-      const double s = score[1][2] - (score[7][2] + score[9][2] * 2.0 + score[10][2] * 2.0) * (1.0 / 5.0);
-      score[7][2] += s;
-      score[9][2] += s;
-      score[10][2] += s;
+      const double sh = s.core[1][2] - (s.core[7][2] + s.core[9][2] * 2.0 + s.core[10][2] * 2.0) * (1.0 / 5.0);
+      s.core[7][2] += sh;
+      s.core[9][2] += sh;
+      s.core[10][2] += sh;
     }
   }
   __syncthreads();
-  // Stage 5: store w, the electron-core blocks E = R^T E_local R (src/mndod_twocenter.rs:1003-1045),
-  // the resonance blocks beta*S and the core-core energy.
-  double* wp = w + P.w_off;
-  for (int idx = lane; idx < npf * nps; idx += 64) wp[idx] = sL[idx];
+  // Stage 5: the electron-core blocks E = R^T E_local R (src/mndod_twocenter.rs:1003-1045), the
+  // resonance blocks beta*S and the core-core energy.
   for (int t = lane; t < npf + nps; t += 64) {
     const int side = t < npf ? 1 : 2;
     const int p = side == 1 ? t : t - npf, n = side == 1 ? nf : ns;
@@ -593,47 +589,73 @@ __global__ void __launch_bounds__(64) k_pairs_d(const int64_t* __restrict__ dpai
         a = b;
         b = x;
       }
-      if (a == 0 && b == 0) return score[1][side];
-      if (a == 0 && b == 1) return score[2][side];
-      if (a == 1 && b == 1) return score[3][side];
-      if ((a == 2 && b == 2) || (a == 3 && b == 3)) return score[4][side];
-      if (a == 0 && b == 4) return score[5][side];
-      if (a == 1 && b == 4) return score[6][side];
-      if (a == 4 && b == 4) return score[7][side];
-      if ((a == 2 && b == 5) || (a == 3 && b == 6)) return score[8][side];
-      if ((a == 5 && b == 5) || (a == 6 && b == 6)) return score[9][side];
-      if ((a == 7 && b == 7) || (a == 8 && b == 8)) return score[10][side];
+      if (a == 0 && b == 0) return s.core[1][side];
+      if (a == 0 && b == 1) return s.core[2][side];
+      if (a == 1 && b == 1) return s.core[3][side];
+      if ((a == 2 && b == 2) || (a == 3 && b == 3)) return s.core[4][side];
+      if (a == 0 && b == 4) return s.core[5][side];
+      if (a == 1 && b == 4) return s.core[6][side];
+      if (a == 4 && b == 4) return s.core[7][side];
+      if ((a == 2 && b == 5) || (a == 3 && b == 6)) return s.core[8][side];
+      if ((a == 5 && b == 5) || (a == 6 && b == 6)) return s.core[9][side];
+      if ((a == 7 && b == 7) || (a == 8 && b == 8)) return s.core[10][side];
       return 0.0;
     };
     double acc = 0.0;
     for (int a = 0; a < n; ++a)
       for (int b = 0; b < n; ++b) {
         const double l = Lc(a, b);
-        if (l != 0.0) acc += sR[a * 9 + K] * l * sR[b * 9 + Mo];
+        if (l != 0.0) acc += s.R[a * 9 + K] * l * s.R[b * 9 + Mo];
       }
-    wp[(int64_t)npf * nps + t] = acc;
+    s.ecb[t] = acc;
   }
-  {
-    const int ld = M.ld;
-    double* H = slab + M.mat0 + (int64_t)S_H * ld * ld;
-    const int of = atom_ao[P.f], os = atom_ao[P.s];
-    for (int t = lane; t < nf * ns; t += 64) {
-      const int oi = t / ns, oj = t - oi * ns;
-      int i, k, j, l;
-      orbital_shell(oi, i, k);
-      orbital_shell(oj, j, l);
-      const double aa = j == 2 ? -1.0 : 1.0, bb = j == 3 ? -1.0 : 1.0;
-      const double s = sS[i][j][1] * (sC[i][k][3] * sC[j][l][3]) * aa +
-                       sS[i][j][2] * (sC[i][k][4] * sC[j][l][4] + sC[i][k][2] * sC[j][l][2]) * bb +
-                       sS[i][j][3] * (sC[i][k][5] * sC[j][l][5] + sC[i][k][1] * sC[j][l][1]);
-      const double bi = oi == 0 ? ef.beta_s : (oi <= 3 ? ef.beta_p : ef.beta_d);
-      const double bj = oj == 0 ? es.beta_s : (oj <= 3 ? es.beta_p : es.beta_d);
-      const double v = 0.5 * (bi + bj) * s;
-      H[(int64_t)(of + oi) * ld + os + oj] = v;
-      H[(int64_t)(os + oj) * ld + of + oi] = v;
-    }
+  for (int t = lane; t < nf * ns; t += 64) {
+    const int oi = t / ns, oj = t - oi * ns;
+    int i, k, j, l;
+    orbital_shell(oi, i, k);
+    orbital_shell(oj, j, l);
+    const double aa = j == 2 ? -1.0 : 1.0, bb = j == 3 ? -1.0 : 1.0;
+    const double sv = s.S[i][j][1] * (s.C[i][k][3] * s.C[j][l][3]) * aa +
+                      s.S[i][j][2] * (s.C[i][k][4] * s.C[j][l][4] + s.C[i][k][2] * s.C[j][l][2]) * bb +
+                      s.S[i][j][3] * (s.C[i][k][5] * s.C[j][l][5] + s.C[i][k][1] * s.C[j][l][1]);
+    const double bi = oi == 0 ? ef.beta_s : (oi <= 3 ? ef.beta_p : ef.beta_d);
+    const double bj = oj == 0 ? es.beta_s : (oj <= 3 ? es.beta_p : es.beta_d);
+    s.hres[t] = 0.5 * (bi + bj) * sv;
   }
-  if (lane == 0) e_core[g] = pair_core(ef, es, zf, zs, r, pairpar, vpar);
+  if (lane == 0) s.e_core = pair_core(ef, es, zf, zs, r, pairpar, vpar);
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(64) k_pairs_d(const int64_t* __restrict__ dpairs, const PairDev* __restrict__ pairs,
+                                                const MolDev* __restrict__ mols, const int* __restrict__ atom_z,
+                                                const int* __restrict__ atom_ao, const double* __restrict__ xyz,
+                                                const ElemDev* __restrict__ elem, const double2* __restrict__ pairpar,
+                                                double4 vpar, double* __restrict__ w, double* __restrict__ slab,
+                                                double* __restrict__ e_core) {
+  __shared__ DPairShared s;
+  const int lane = threadIdx.x;
+  const int64_t g = dpairs[blockIdx.x];
+  const PairDev P = pairs[g];
+  const MolDev& M = mols[P.mol];
+  const int zf = atom_z[P.f], zs = atom_z[P.s];
+  const ElemDev ef = elem[zf];
+  const ElemDev es = elem[zs];
+  const int nf = ef.n_orb, ns = es.n_orb, npf = npair_of(nf), nps = npair_of(ns);
+  const double d[3] = {xyz[3 * P.s] - xyz[3 * P.f], xyz[3 * P.s + 1] - xyz[3 * P.f + 1],
+                       xyz[3 * P.s + 2] - xyz[3 * P.f + 2]};
+  pair_d_eval(ef, es, zf, zs, d, pairpar, vpar, s);
+  double* wp = w + P.w_off;
+  for (int idx = lane; idx < npf * nps; idx += 64) wp[idx] = s.L[idx];
+  for (int t = lane; t < npf + nps; t += 64) wp[(int64_t)npf * nps + t] = s.ecb[t];
+  const int ld = M.ld;
+  double* H = slab + M.mat0 + (int64_t)S_H * ld * ld;
+  const int of = atom_ao[P.f], os = atom_ao[P.s];
+  for (int t = lane; t < nf * ns; t += 64) {
+    const int oi = t / ns, oj = t - oi * ns;
+    H[(int64_t)(of + oi) * ld + os + oj] = s.hres[t];
+    H[(int64_t)(os + oj) * ld + of + oi] = s.hres[t];
+  }
+  if (lane == 0) e_core[g] = s.e_core;
 }
 
 }  // namespace pm7

@@ -52,10 +52,12 @@ def oracle_fd_gradient(z, xyz, charge, mult, h=1.0e-3):
     return g
 
 
-# water: closed shell; methyl radical: UHF; water dimer and formamide: H-bond and dispersion
-# corrections active; methylamine, hydrogen fluoride chain: other sp elements
+# water: closed shell; methyl radical: UHF; water dimer: H-bond and dispersion corrections active;
+# methylamine, hydrogen fluoride chain: other sp elements; thiophene, H2S, ClO2 (UHF), TiCl4,
+# ferrocene: every pair through the MNDO/d kernels (k_pair_grad_d)
 @pytest.mark.parametrize("name", ["water", "methyl_radical", "water_dimer", "formaldehyde", "methylamine",
-                                  "hf_chain_8"])
+                                  "hf_chain_8", "thiophene", "hydrogen_sulfide", "chlorine_dioxide",
+                                  "titanium_tetrachloride", "ferrocene"])
 def test_gradient_matches_oracle_scf_finite_difference(eng, name):
     say(f"gradient {name}")
     c = CASES[name]
@@ -101,8 +103,8 @@ def test_mixed_precision_gradient_agrees_with_fp64(eng):
 
 
 def test_unsupported_molecules_keep_their_energies(eng):
-    say("gradient unsupported d path")
-    c = CASES["thiophene"]
+    say("gradient unsupported n >= 4 shell")
+    c = CASES["germane"]
     mol = pm7amd.Molecule.from_symbols_angstrom(c["atoms"], c["charge"], c["multiplicity"])
     water = pm7amd.Molecule.from_symbols_angstrom(CASES["water"]["atoms"])
     res, grads = eng.gradient_batch([mol, water], pm7amd.Pm7Options())

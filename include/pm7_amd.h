@@ -137,9 +137,8 @@ void pm7_result_free(pm7_result_v1* res);
  * (results[i] exactly as pm7_run_batch fills it), then dE_total/dR at the converged density on the
  * device, in eV/Bohr, plus the gradient of the geometric corrections (host). Molecule i's atom a,
  * axis k lands at gradient[3 * (n_atoms of molecules 0..i-1) + 3 * a + k]; the caller sizes the
- * array to 3 x the total atom count. Molecules holding an sp atom whose valence shell has n >= 4
- * (Ge, Sn, Br, I, ...; their overlaps need the numerical quadrature) get NaN and
- * results[i].status = PM7_ERR_UNSUPPORTED (their energies are still filled). */
+ * array to 3 x the total atom count. A molecule whose SCF left no density gets NaN and
+ * results[i].status = PM7_ERR_UNSUPPORTED (its energies, if any, are still filled). */
 pm7_status pm7_gradient_batch(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols,
                               const pm7_options_v1* opts, int32_t n_opts, pm7_result_v1* results,
                               double* gradient);

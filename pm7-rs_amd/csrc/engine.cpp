@@ -139,7 +139,7 @@ struct Lane {
   hipEvent_t ev_var[2 * SCF_VARIANTS] = {};
   DevBuf d_mols, d_list, d_atom_z, d_atom_ao, d_atom_pp, d_xyz, d_pairs, d_items, d_gath_off;
   DevBuf d_w, d_slab, d_vecs, d_out, d_ecore, d_ecore_mol, d_gath, d_num, d_snum, d_dpairs;
-  DevBuf d_gpair, d_gatom, d_gitems;  // nuclear gradient (pm7_gradient_batch only)
+  DevBuf d_gpair, d_gatom, d_gitems, d_snumg;  // nuclear gradient (pm7_gradient_batch only)
   PinnedBuf h_out;  // staging for the packed results
   LargeWork* large = nullptr;  // rocBLAS handle and buffers of the large-system path (lazily made)
   // Several large-system SCFs in lockstep: one work set and one stream each (lazily made).
@@ -164,7 +164,7 @@ struct Lane {
       if (s) (void)hipStreamSynchronize(s);
     DevBuf* bufs[] = {&d_mols, &d_list, &d_atom_z, &d_atom_ao, &d_atom_pp, &d_xyz, &d_pairs, &d_items, &d_gath_off,
                       &d_w, &d_slab, &d_vecs, &d_out, &d_ecore, &d_ecore_mol, &d_gath, &d_num, &d_snum,
-                      &d_dpairs, &d_gpair, &d_gatom, &d_gitems};
+                      &d_dpairs, &d_gpair, &d_gatom, &d_gitems, &d_snumg};
     for (DevBuf* b : bufs) b->release();
     h_out.release();
     if (large) large_work_destroy(large);
@@ -966,8 +966,8 @@ void corrections_grad_kcal(const Job& j, double* g) {
 
 // Device gradient of every job of a finished batch (enqueued on stream 0 after the SCF and the
 // gather); collect_gradient adds the corrections' gradient on the host and fills the caller's
-// array. A molecule with a pair that has no device gradient (an sp atom whose valence shell has
-// n >= 4, i.e. a numerical overlap) gets NaN and PM7_ERR_UNSUPPORTED.
+// array. A molecule with a pair that has no device gradient (its SCF left no density) gets NaN and
+// PM7_ERR_UNSUPPORTED.
 pm7_status launch_gradient_batch(Lane& L, const Prepared& P, int* n_items_out) {
   std::vector<int2> items;
   int n_atoms = 0;
@@ -982,8 +982,10 @@ pm7_status launch_gradient_batch(Lane& L, const Prepared& P, int* n_items_out) {
   LANECHK(L.d_gatom.ensure(std::max<size_t>(1, (size_t)n_atoms * 3) * sizeof(double)));
   if (!items.empty())
     LANECHK(hipMemcpyAsync(L.d_gitems.p, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice, L.st[0]));
+  LANECHK(L.d_snumg.ensure(std::max<size_t>(1, (size_t)P.b.n_num * 60) * sizeof(double)));
   LANECHK(launch_gradient(P.b, P.n_pairs, static_cast<const int2*>(L.d_gitems.p), (int)items.size(),
-                          static_cast<double*>(L.d_gpair.p), static_cast<double*>(L.d_gatom.p), L.st[0]));
+                          static_cast<double*>(L.d_gpair.p), static_cast<double*>(L.d_gatom.p),
+                          static_cast<double*>(L.d_snumg.p), L.st[0]));
   return PM7_OK;
 }
 
@@ -1010,7 +1012,7 @@ pm7_status collect_gradient(Lane& L, const Prepared& P, const GradReq& g, pm7_re
         for (int k = 0; k < 3 * na; ++k) out[k] = std::nan("");
         if (r.status == PM7_OK || r.status == PM7_ERR_SCF_NOT_CONVERGED) {
           r.status = PM7_ERR_UNSUPPORTED;
-          set_msg(&r, "gradient: sp atoms with valence shells n >= 4 (numerical overlaps) have no gradient here");
+          set_msg(&r, "gradient: no converged density to differentiate at");
         }
         continue;
       }

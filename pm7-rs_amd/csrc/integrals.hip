@@ -315,19 +315,30 @@ __device__ void overlap_locals(const ElemDev& ea, const ElemDev& eb, double r, d
 // xi in [1, 1 + 60/p] (48 Gauss-Legendre nodes, p = (za + zb) r / 2), eta in [-1, 1] (40 nodes),
 // dV = (r/2)^3 (xi^2 - eta^2). One wave per pair: lanes split the 1920 nodes of each integral,
 // then a fixed shuffle tree sums them. Atom a is the one with the higher n.
+// With ndisp = GRAD_DISP (gradient calls) item = pair * GRAD_DISP + axis * 4 + step evaluates the
+// pair at the displaced separation d + GRAD_STEPS[step] along that axis, as k_pair_grad needs.
+constexpr int GRAD_DISP = 12;
+constexpr double GRAD_H = 1.0e-3;
+__device__ inline double grad_step(int t) { return (t & 1 ? -1.0 : 1.0) * (t < 2 ? GRAD_H : 2.0 * GRAD_H); }
+
 __global__ void __launch_bounds__(64) k_overlap_numeric(int n_num, const int64_t* __restrict__ num_pairs,
                                                          const PairDev* __restrict__ pairs,
                                                          const int* __restrict__ atom_z, const double* __restrict__ xyz,
                                                          const ElemDev* __restrict__ elem,
-                                                         const double* __restrict__ gl, double* __restrict__ s_num) {
+                                                         const double* __restrict__ gl, double* __restrict__ s_num,
+                                                         int ndisp) {
   const int item = blockIdx.x;
-  if (item >= n_num) return;
-  const PairDev P = pairs[num_pairs[item]];
+  if (item >= n_num * (ndisp > 0 ? ndisp : 1)) return;
+  const PairDev P = pairs[num_pairs[ndisp > 0 ? item / ndisp : item]];
   const ElemDev ef = elem[atom_z[P.f]], es = elem[atom_z[P.s]];
   const bool swap = ef.n < es.n;
   const ElemDev& ea = swap ? es : ef;
   const ElemDev& eb = swap ? ef : es;
   double d[3] = {xyz[3 * P.s] - xyz[3 * P.f], xyz[3 * P.s + 1] - xyz[3 * P.f + 1], xyz[3 * P.s + 2] - xyz[3 * P.f + 2]};
+  if (ndisp > 0) {
+    const int t = item % ndisp;
+    d[t / 4] += grad_step(t % 4);
+  }
   const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
   const double hr = 0.5 * r;
   const int lane = threadIdx.x;
@@ -635,7 +646,8 @@ struct PairDens {
 };
 
 __device__ double pair_energy(const ElemDev& ef, const ElemDev& es, int zf, int zs, const double d[3],
-                              const PairDens& D, const double2* __restrict__ pairpar, double4 vpar) {
+                              const PairDens& D, const double2* __restrict__ pairpar, double4 vpar,
+                              const double* snum) {
   const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
   const int nf = ef.n_orb, ns = es.n_orb;
   const int npf = npair_of(nf), nps = npair_of(ns);
@@ -660,7 +672,7 @@ __device__ double pair_energy(const ElemDev& ef, const ElemDev& es, int zf, int 
     rotate_xx_store(ri, R, w);
   }
   double S[4][4];
-  overlap_block(ef, es, d, r, S, nullptr);
+  overlap_block(ef, es, d, r, S, snum);
   double e = 0.0;
   for (int i = 0; i < nf; ++i) {
     const double bi = i == 0 ? ef.beta_s : ef.beta_p;
@@ -693,7 +705,8 @@ __global__ void __launch_bounds__(128) k_pair_grad(int64_t n_pairs, const PairDe
                                                     const int* __restrict__ atom_z, const int* __restrict__ atom_ao,
                                                     const double* __restrict__ xyz, const ElemDev* __restrict__ elem,
                                                     const double2* __restrict__ pairpar, double4 vpar,
-                                                    const double* __restrict__ slab, double* __restrict__ gpair) {
+                                                    const double* __restrict__ slab, const double* __restrict__ s_numg,
+                                                    double* __restrict__ gpair) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n_pairs) return;
   const PairDev P = pairs[g];
@@ -701,7 +714,7 @@ __global__ void __launch_bounds__(128) k_pair_grad(int64_t n_pairs, const PairDe
   const MolOut& O = out[P.mol];
   double* go = gpair + 3 * g;
   if (M.dpath) return;  // k_pair_grad_d owns the pairs of molecules with a d shell
-  if (P.pad > 0 || O.p_slot < 0) {
+  if ((P.pad > 0 && !s_numg) || O.p_slot < 0) {
     go[0] = go[1] = go[2] = NAN;
     return;
   }
@@ -728,16 +741,16 @@ __global__ void __launch_bounds__(128) k_pair_grad(int64_t n_pairs, const PairDe
     }
   const double d0[3] = {xyz[3 * P.s] - xyz[3 * P.f], xyz[3 * P.s + 1] - xyz[3 * P.f + 1],
                         xyz[3 * P.s + 2] - xyz[3 * P.f + 2]};
-  constexpr double h = 1.0e-3;
+  // numerical overlaps at the displaced separations, from k_overlap_numeric(ndisp = GRAD_DISP)
+  const double* sn = P.pad > 0 ? s_numg + 5 * (int64_t)GRAD_DISP * (P.pad - 1) : nullptr;
   for (int k = 0; k < 3; ++k) {
     double ev[4];
-    const double steps[4] = {h, -h, 2.0 * h, -2.0 * h};
     for (int t = 0; t < 4; ++t) {
       double d[3] = {d0[0], d0[1], d0[2]};
-      d[k] += steps[t];
-      ev[t] = pair_energy(ef, es, zf, zs, d, D, pairpar, vpar);
+      d[k] += grad_step(t);
+      ev[t] = pair_energy(ef, es, zf, zs, d, D, pairpar, vpar, sn ? sn + 5 * (k * 4 + t) : nullptr);
     }
-    go[k] = (8.0 * (ev[0] - ev[1]) - (ev[2] - ev[3])) / (12.0 * h);
+    go[k] = (8.0 * (ev[0] - ev[1]) - (ev[2] - ev[3])) / (12.0 * GRAD_H);
   }
 }
 
@@ -791,12 +804,10 @@ __global__ void __launch_bounds__(64) k_pair_grad_d(const int64_t* __restrict__ 
   const double d0[3] = {xyz[3 * P.s] - xyz[3 * P.f], xyz[3 * P.s + 1] - xyz[3 * P.f + 1],
                         xyz[3 * P.s + 2] - xyz[3 * P.f + 2]};
   __syncthreads();
-  constexpr double h = 1.0e-3;
-  const double steps[4] = {h, -h, 2.0 * h, -2.0 * h};
   for (int k = 0; k < 3; ++k) {
     for (int t = 0; t < 4; ++t) {
       double d[3] = {d0[0], d0[1], d0[2]};
-      d[k] += steps[t];
+      d[k] += grad_step(t);
       pair_d_eval(ef, es, zf, zs, d, pairpar, vpar, s);
       double acc = 0.0;
       for (int idx = lane; idx < npf * nps; idx += 64) {
@@ -824,7 +835,7 @@ __global__ void __launch_bounds__(64) k_pair_grad_d(const int64_t* __restrict__ 
       }
       __syncthreads();
     }
-    if (lane == 0) go[k] = (8.0 * (ev[0] - ev[1]) - (ev[2] - ev[3])) / (12.0 * h);
+    if (lane == 0) go[k] = (8.0 * (ev[0] - ev[1]) - (ev[2] - ev[3])) / (12.0 * GRAD_H);
     __syncthreads();
   }
 }
@@ -851,10 +862,14 @@ __global__ void __launch_bounds__(128) k_atom_grad(int n_items, const int2* __re
 }
 
 hipError_t launch_gradient(const BatchDev& b, int64_t n_pairs, const int2* items, int n_items, double* gpair,
-                           double* gatom, hipStream_t s) {
+                           double* gatom, double* s_numg, hipStream_t s) {
+  if (b.n_num > 0)
+    hipLaunchKernelGGL(k_overlap_numeric, dim3(b.n_num * GRAD_DISP), dim3(64), 0, s, b.n_num, b.num_pairs, b.pairs,
+                       b.atom_z, b.xyz, b.elem, b.gl, s_numg, GRAD_DISP);
   if (n_pairs > 0)
     hipLaunchKernelGGL(k_pair_grad, dim3((unsigned)((n_pairs + 127) / 128)), dim3(128), 0, s, n_pairs, b.pairs, b.mols,
-                       b.out, b.atom_z, b.atom_ao, b.xyz, b.elem, b.pairpar, b.vpar, b.slab, gpair);
+                       b.out, b.atom_z, b.atom_ao, b.xyz, b.elem, b.pairpar, b.vpar, b.slab,
+                       b.n_num > 0 ? s_numg : nullptr, gpair);
   if (b.n_dp > 0)
     hipLaunchKernelGGL(k_pair_grad_d, dim3((unsigned)b.n_dp), dim3(64), 0, s, b.d_pairs, b.pairs, b.mols, b.out,
                        b.atom_z, b.atom_ao, b.xyz, b.elem, b.pairpar, b.vpar, b.slab, gpair);
@@ -870,7 +885,7 @@ hipError_t launch_integrals(const BatchDev& b, int64_t n_pairs, int n_items, int
                        b.atom_ao, b.xyz, b.elem, b.pairpar, b.vpar, b.w, b.slab, b.e_core);
   if (b.n_num > 0)
     hipLaunchKernelGGL(k_overlap_numeric, dim3(b.n_num), dim3(64), 0, s, b.n_num, b.num_pairs, b.pairs, b.atom_z, b.xyz,
-                       b.elem, b.gl, b.s_num);
+                       b.elem, b.gl, b.s_num, 0);
   if (n_pairs > 0) {
     int64_t blocks = (n_pairs + 127) / 128;
     hipLaunchKernelGGL(k_pairs, dim3((unsigned)blocks), dim3(128), 0, s, n_pairs, b.pairs, b.mols, b.atom_z, b.atom_ao,

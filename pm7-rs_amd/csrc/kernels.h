@@ -66,8 +66,9 @@ hipError_t launch_scf(int variant, const BatchDev& b, int list_off, int count, h
 hipError_t launch_gather(const BatchDev& b, int n_mols, hipStream_t s);
 // Nuclear gradient at the converged densities of a finished batch: dE/dd per pair into gpair
 // (3 per pair), then per listed atom (items: molecule, batch-global atom) into gatom (3 per atom).
+// s_numg: 60 doubles per numerical-overlap pair (its overlaps at the 12 displaced separations).
 hipError_t launch_gradient(const BatchDev& b, int64_t n_pairs, const int2* items, int n_items, double* gpair,
-                           double* gatom, hipStream_t s);
+                           double* gatom, double* s_numg, hipStream_t s);
 hipError_t launch_fock_once(const BatchDev& b, int mol, int pt_slot, int ps_slot, int f_slot, hipStream_t s);
 
 // Large-system SCF of molecule `mi` (host copy M) of a prepared batch: grid-wide kernels and

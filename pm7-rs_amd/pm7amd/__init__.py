@@ -455,7 +455,7 @@ class Engine:
 
     def gradient_batch(self, molecules: Sequence[Molecule], options=None, raise_on_error: bool = False):
         """Batched SCF + gradients: (results, [gradient[n_atoms_i, 3]]); NaN rows and status
-        UnsupportedFeature for molecules holding an sp atom whose valence shell has n >= 4."""
+        UnsupportedFeature for a molecule whose SCF left no density."""
         flat = np.full(3 * sum(len(m.z) for m in molecules), np.nan)
         res = self._batch(molecules, options, raise_on_error, flat)
         grads, off = [], 0

@@ -54,10 +54,11 @@ def oracle_fd_gradient(z, xyz, charge, mult, h=1.0e-3):
 
 # water: closed shell; methyl radical: UHF; water dimer: H-bond and dispersion corrections active;
 # methylamine, hydrogen fluoride chain: other sp elements; thiophene, H2S, ClO2 (UHF), TiCl4,
-# ferrocene: every pair through the MNDO/d kernels (k_pair_grad_d)
+# ferrocene: every pair through the MNDO/d kernels (k_pair_grad_d); germane, tetramethyltin:
+# valence n >= 4, overlaps by the numerical quadrature at the displaced separations
 @pytest.mark.parametrize("name", ["water", "methyl_radical", "water_dimer", "formaldehyde", "methylamine",
                                   "hf_chain_8", "thiophene", "hydrogen_sulfide", "chlorine_dioxide",
-                                  "titanium_tetrachloride", "ferrocene"])
+                                  "titanium_tetrachloride", "ferrocene", "germane", "tetramethyltin"])
 def test_gradient_matches_oracle_scf_finite_difference(eng, name):
     say(f"gradient {name}")
     c = CASES[name]
@@ -102,12 +103,15 @@ def test_mixed_precision_gradient_agrees_with_fp64(eng):
         assert np.max(np.abs(a - b)) <= 1e-4
 
 
-def test_unsupported_molecules_keep_their_energies(eng):
-    say("gradient unsupported n >= 4 shell")
-    c = CASES["germane"]
-    mol = pm7amd.Molecule.from_symbols_angstrom(c["atoms"], c["charge"], c["multiplicity"])
-    water = pm7amd.Molecule.from_symbols_angstrom(CASES["water"]["atoms"])
-    res, grads = eng.gradient_batch([mol, water], pm7amd.Pm7Options())
-    assert res[0].status == 4 and np.all(np.isnan(grads[0]))
-    assert np.isfinite(res[0].total_ev)
-    assert res[1].status == 0 and np.all(np.isfinite(grads[1]))
+def test_mixed_kinds_in_one_batch_match_alone(eng):
+    """sp, MNDO/d and numerical-overlap molecules in one call: each gradient lands at its own offset
+    and equals the molecule's gradient alone, bit for bit."""
+    say("gradient mixed kinds batch")
+    names = ["water", "thiophene", "germane", "methyl_radical", "titanium_tetrachloride"]
+    mols = [pm7amd.Molecule.from_symbols_angstrom(CASES[n]["atoms"], CASES[n]["charge"], CASES[n]["multiplicity"])
+            for n in names]
+    opts = [pm7amd.Pm7Options(charge=CASES[n]["charge"], multiplicity=CASES[n]["multiplicity"]) for n in names]
+    res, grads = eng.gradient_batch(mols, opts, raise_on_error=True)
+    for m, o, r, g in zip(mols, opts, res, grads):
+        r1, g1 = eng.gradient(m, o)
+        assert g.shape == (len(m.z), 3) and np.array_equal(g, g1) and r.total_ev == r1.total_ev

@@ -209,10 +209,11 @@ struct pm7_ctx {
   // diagonalising (environment PM7_AMD_ROTATE_BELOW, 1e-6..1e-1).
   double rotate_below = ROTATE_BELOW;
   // Stopping test of the full-accuracy Jacobi eigensolves: off-diagonal^2 <= 10^-eig_tol_exp x
-  // total^2 (environment PM7_AMD_EIG_TOL_EXP, 16..30). 1e-20 leaves the orbitals accurate to
-  // ~1e-10 of the matrix norm, far inside the 1e-7 density bar; against 1e-26 it saves 0.45
-  // sweeps per solve (FP64 organic batch 7.40k -> 7.80k solves/s, S/P/Cl batch 415 -> 460/s).
-  int eig_tol_exp = 20;
+  // total^2 (environment PM7_AMD_EIG_TOL_EXP, 16..30). 1e-22 keeps every density of the
+  // 256-molecule organic batch within 1e-7 of a tightly converged CPU solve (1e-20 left one at
+  // 1.3e-7); against 1e-26 it saves 0.25 sweeps per solve (FP64 organic batch 7.40k -> 7.57k
+  // solves/s, S/P/Cl batch 415 -> 437/s).
+  int eig_tol_exp = 22;
   std::mutex mu;
 };
 

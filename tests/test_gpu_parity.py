@@ -204,6 +204,24 @@ def test_mixed_precision_golden_batch_within_bars(eng):
     assert sum(r.fp32_iterations for r in mx) > 0
 
 
+def test_benchmark_batch_fp64_matches_oracle_everywhere(eng):
+    """The whole 256-molecule organic benchmark batch (seed 3), FP64, each molecule against the
+    oracle at the full bar: the eigensolver's stopping test must leave every density within 1e-7,
+    not only the golden cases'."""
+    batch = W.organic_batch(256, 3)
+    say("benchmark batch of 256 vs oracle")
+    res = eng.run_batch([mol_of(a) for a in batch], pm7amd.Pm7Options())
+    worst = (0.0, 0.0, 0.0)
+    for a, r in zip(batch, res):
+        ref, dens, *_ = oracle_of(a)
+        assert r.status == 0 and r.converged
+        d = (abs(r.total_ev - ref.total_ev), abs(r.heat_of_formation_kcal - ref.hof_kcal),
+             float(np.max(np.abs(r.density - dens))))
+        worst = tuple(max(x, y) for x, y in zip(worst, d))
+    say(f"worst |dE| {worst[0]:.2e} eV, |dHf| {worst[1]:.2e} kcal/mol, |dP| {worst[2]:.2e}")
+    assert worst[0] <= 1e-6 and worst[1] <= 1e-4 and worst[2] <= 1e-7
+
+
 def test_mixed_precision_organic_batch_vs_fp64(eng):
     batch = W.organic_batch(48, 3)
     say("mixed organic batch of 48")

@@ -155,6 +155,15 @@ pm7_status pm7_core_hamiltonian(pm7_ctx* ctx, const pm7_molecule_v1* mol, double
 pm7_status pm7_fock_spin(pm7_ctx* ctx, const pm7_molecule_v1* mol, const double* p_tot,
                          const double* p_spin, double* f_out);
 
+/* Replaces the CPHF response kernel of cphf_ov (src/hessian.rs:1074-1081), which calls
+ * build_fock_x (src/fock.rs:665-684) once per response density and subtracts H_core:
+ * g_out[k] = build_fock_x(dp[k]) - H = J(dp[k]) - K(dp[k] / 2), for n_dens symmetric densities of one
+ * molecule (each nao*nao, row-major, back to back; g_out the same shape). The pair integrals are
+ * built once and every density is contracted against them on the device (one workgroup each).
+ * n_dens = 0 does nothing; PM7_ERR_INSUFFICIENT_MEMORY when the densities do not fit. */
+pm7_status pm7_response_fock_batch(pm7_ctx* ctx, const pm7_molecule_v1* mol, int32_t n_dens,
+                                   const double* dp, double* g_out);
+
 /* Replaces core_core_energy (src/repulsion.rs:16-36), eV. */
 pm7_status pm7_core_core_energy(pm7_ctx* ctx, const pm7_molecule_v1* mol, double* e_out);
 

@@ -140,6 +140,7 @@ struct Lane {
   DevBuf d_mols, d_list, d_atom_z, d_atom_ao, d_atom_pp, d_xyz, d_pairs, d_items, d_gath_off;
   DevBuf d_w, d_slab, d_vecs, d_out, d_ecore, d_ecore_mol, d_gath, d_num, d_snum, d_dpairs;
   DevBuf d_gpair, d_gatom, d_gitems, d_snumg;  // nuclear gradient (pm7_gradient_batch only)
+  DevBuf d_rx, d_rg, d_rpp;                    // response densities, their G, packed scratch
   PinnedBuf h_out;  // staging for the packed results
   LargeWork* large = nullptr;  // rocBLAS handle and buffers of the large-system path (lazily made)
   // Several large-system SCFs in lockstep: one work set and one stream each (lazily made).
@@ -164,7 +165,7 @@ struct Lane {
       if (s) (void)hipStreamSynchronize(s);
     DevBuf* bufs[] = {&d_mols, &d_list, &d_atom_z, &d_atom_ao, &d_atom_pp, &d_xyz, &d_pairs, &d_items, &d_gath_off,
                       &d_w, &d_slab, &d_vecs, &d_out, &d_ecore, &d_ecore_mol, &d_gath, &d_num, &d_snum,
-                      &d_dpairs, &d_gpair, &d_gatom, &d_gitems, &d_snumg};
+                      &d_dpairs, &d_gpair, &d_gatom, &d_gitems, &d_snumg, &d_rx, &d_rg, &d_rpp};
     for (DevBuf* b : bufs) b->release();
     h_out.release();
     if (large) large_work_destroy(large);
@@ -1598,6 +1599,53 @@ pm7_status pm7_fock_spin(pm7_ctx* ctx, const pm7_molecule_v1* mol, const double*
   HIPCHK(hipMemcpy2DAsync(f_out, J.nao * sizeof(double), base + 1 * sq, J.ld * sizeof(double), J.nao * sizeof(double),
                           J.nao, hipMemcpyDeviceToHost, ctx->lane.st[0]));
   HIPCHK(hipStreamSynchronize(ctx->lane.st[0]));
+  return PM7_OK;
+}
+
+pm7_status pm7_response_fock_batch(pm7_ctx* ctx, const pm7_molecule_v1* mol, int32_t n_dens, const double* dp,
+                                   double* g_out) {
+  if (!ctx || !mol || n_dens < 0 || (n_dens > 0 && (!dp || !g_out))) return PM7_ERR_INVALID_INPUT;
+  int32_t nao = 0;
+  pm7_status st = pm7_core_hamiltonian(ctx, mol, nullptr, &nao);
+  if (st != PM7_OK) return st;
+  if (n_dens == 0) return PM7_OK;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  pm7_options_v1 o;
+  pm7_default_options(&o);
+  o.method = ctx->method;
+  o.reference = PM7_REF_UHF;
+  Job j;
+  std::string msg;
+  st = validate(ctx, *mol, o, j, msg);
+  if (st != PM7_OK) {
+    o.multiplicity = 2;
+    st = validate(ctx, *mol, o, j, msg);
+    if (st != PM7_OK) return fail(ctx, st, msg);
+  }
+  Prepared P;
+  P.jobs.push_back(j);
+  st = prepare(ctx, ctx->lane, P, 0);
+  if (st != PM7_OK) return fail(ctx, st, ctx->lane.err);
+  const Job& J = P.jobs[0];
+  Lane& L = ctx->lane;
+  const int64_t slice = (int64_t)J.nao * J.ld;  // densities are stored with row stride ld, nao rows each
+  const int64_t pp_stride = ((int64_t)J.n_pp + 31) / 32 * 32;
+  const size_t mat_bytes = (size_t)n_dens * (size_t)slice * sizeof(double);
+  if (L.d_rx.ensure(mat_bytes) != hipSuccess || L.d_rg.ensure(mat_bytes) != hipSuccess ||
+      L.d_rpp.ensure((size_t)n_dens * (size_t)pp_stride * sizeof(double)) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(ctx, PM7_ERR_INSUFFICIENT_MEMORY,
+                "response Fock: no device memory for " + std::to_string(n_dens) + " densities of " +
+                    std::to_string(J.nao) + " AOs");
+  }
+  double* xs = static_cast<double*>(L.d_rx.p);
+  double* gs = static_cast<double*>(L.d_rg.p);
+  HIPCHK(hipMemcpy2DAsync(xs, J.ld * sizeof(double), dp, J.nao * sizeof(double), J.nao * sizeof(double),
+                          (size_t)n_dens * J.nao, hipMemcpyHostToDevice, L.st[0]));
+  HIPCHK(launch_fock_response(P.b, 0, xs, gs, static_cast<double*>(L.d_rpp.p), pp_stride, n_dens, L.st[0]));
+  HIPCHK(hipMemcpy2DAsync(g_out, J.nao * sizeof(double), gs, J.ld * sizeof(double), J.nao * sizeof(double),
+                          (size_t)n_dens * J.nao, hipMemcpyDeviceToHost, L.st[0]));
+  HIPCHK(hipStreamSynchronize(L.st[0]));
   return PM7_OK;
 }
 

@@ -70,6 +70,10 @@ hipError_t launch_gather(const BatchDev& b, int n_mols, hipStream_t s);
 hipError_t launch_gradient(const BatchDev& b, int64_t n_pairs, const int2* items, int n_items, double* gpair,
                            double* gatom, double* s_numg, hipStream_t s);
 hipError_t launch_fock_once(const BatchDev& b, int mol, int pt_slot, int ps_slot, int f_slot, hipStream_t s);
+// G_k = J(X_k) - K(X_k / 2) for n_dens densities of molecule `mol` (slices of nao*ld at xs / gs;
+// pp_stride doubles of packed-density scratch per density at pps).
+hipError_t launch_fock_response(const BatchDev& b, int mol, const double* xs, double* gs, double* pps,
+                                int64_t pp_stride, int n_dens, hipStream_t s);
 
 // Large-system SCF of molecule `mi` (host copy M) of a prepared batch: grid-wide kernels and
 // rocBLAS/rocSOLVER on stream s; fills `out` as the megakernel would. Blocks the calling thread.

@@ -632,6 +632,45 @@ __global__ void __launch_bounds__(256) k_fock_once(int mi, const MolDev* __restr
   fock_build<256, double>(c, c.slot(pt_slot), c.slot(ps_slot), 1.0, nullptr, 0.0, c.slot(f_slot), nullptr);
 }
 
+// Two-electron response Fock of many densities of one molecule against its one integral set
+// (the CPHF kernel of pm7-rs cphf_ov, src/hessian.rs:1074-1081: build_fock_x(X) - H_core, with
+// build_fock_x(X) = H + J(X) - K(X/2), src/fock.rs:665-684). Workgroup k takes density k:
+// X_k at xs + k*nao*ld (row stride ld), G_k written at gs + k*nao*ld, packed-density scratch at
+// pps + k*pp_stride. The densities are symmetric, as response densities C_v U C_o^T + transpose are.
+__global__ void __launch_bounds__(256) k_fock_response(int mi, const MolDev* __restrict__ mols,
+                                                        const int* __restrict__ atom_z, const int* __restrict__ atom_ao,
+                                                        const int* __restrict__ atom_pp,
+                                                        const PairDev* __restrict__ pairs,
+                                                        const ElemDev* __restrict__ elem,
+                                                        const double* __restrict__ ocd, const double* __restrict__ w,
+                                                        double* __restrict__ slab, const double* __restrict__ xs,
+                                                        double* __restrict__ gs, double* __restrict__ pps,
+                                                        int64_t pp_stride) {
+  MolCtx c;
+  c.M = mols[mi];
+  c.atom_z = atom_z;
+  c.atom_ao = atom_ao;
+  c.atom_pp = atom_pp;
+  c.pairs = pairs;
+  c.elem = elem;
+  c.ocd = ocd;
+  c.w = w;
+  c.ld = c.M.ld;
+  c.base = slab + c.M.mat0;
+  c.pp = pps + (int64_t)blockIdx.x * pp_stride;
+  const int64_t sq = (int64_t)c.M.nao * c.ld;  // rows below nao only: nothing reads or writes the rest
+  const double* X = xs + blockIdx.x * sq;
+  double* G = gs + blockIdx.x * sq;
+  fock_build<256, double>(c, X, X, 0.5, nullptr, 0.0, G, nullptr);
+  // F(X) - H, element by element, as the reference's kernel subtracts h_core from the Fock matrix
+  const double* H = c.slot(S_H);
+  const int n = c.M.nao;
+  for (int t = threadIdx.x; t < n * n; t += 256) {
+    const int64_t i = (int64_t)(t / n) * c.ld + t % n;
+    G[i] -= H[i];
+  }
+}
+
 hipError_t launch_scf(int variant, const BatchDev& b, int list_off, int count, hipStream_t s) {
   if (count <= 0) return hipSuccess;
   const int* list = b.mol_list + list_off;
@@ -657,6 +696,14 @@ hipError_t launch_scf(int variant, const BatchDev& b, int list_off, int count, h
 hipError_t launch_fock_once(const BatchDev& b, int mol, int pt_slot, int ps_slot, int f_slot, hipStream_t s) {
   hipLaunchKernelGGL(k_fock_once, dim3(1), dim3(256), 0, s, mol, b.mols, b.atom_z, b.atom_ao, b.atom_pp, b.pairs,
                      b.elem, b.ocd, b.w, b.slab, pt_slot, ps_slot, f_slot);
+  return hipGetLastError();
+}
+
+hipError_t launch_fock_response(const BatchDev& b, int mol, const double* xs, double* gs, double* pps,
+                                int64_t pp_stride, int n_dens, hipStream_t s) {
+  if (n_dens <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fock_response, dim3(n_dens), dim3(256), 0, s, mol, b.mols, b.atom_z, b.atom_ao, b.atom_pp,
+                     b.pairs, b.elem, b.ocd, b.w, b.slab, xs, gs, pps, pp_stride);
   return hipGetLastError();
 }
 

@@ -97,7 +97,7 @@ ABI_FUNCTIONS = [
     "pm7_default_options", "pm7_run", "pm7_run_batch", "pm7_result_free", "pm7_core_hamiltonian",
     "pm7_fock_spin", "pm7_core_core_energy", "pm7_last_timings", "pm7_server_default_config",
     "pm7_server_start", "pm7_server_submit", "pm7_server_poll", "pm7_server_stop", "pm7_gradient_batch",
-    "pm7_gradient",
+    "pm7_gradient", "pm7_response_fock_batch",
 ]
 
 _lib = None
@@ -130,6 +130,8 @@ def library() -> C.CDLL:
     lib.pm7_fock_spin.argtypes = [vp, C.POINTER(_Molecule), C.POINTER(C.c_double), C.POINTER(C.c_double),
                                   C.POINTER(C.c_double)]
     lib.pm7_core_core_energy.argtypes = [vp, C.POINTER(_Molecule), C.POINTER(C.c_double)]
+    lib.pm7_response_fock_batch.argtypes = [vp, C.POINTER(_Molecule), C.c_int32, C.POINTER(C.c_double),
+                                            C.POINTER(C.c_double)]
     lib.pm7_last_timings.argtypes = [vp, C.POINTER(C.c_double), C.c_int32]
     lib.pm7_server_default_config.argtypes = [C.POINTER(_ServerConfig)]
     lib.pm7_server_start.argtypes = [vp, C.POINTER(_ServerConfig), C.POINTER(vp)]
@@ -517,6 +519,30 @@ class Engine:
         if st != 0:
             raise Pm7Error(st, self.last_error())
         return f
+
+    def response_fock(self, molecule: Molecule, densities: np.ndarray) -> np.ndarray:
+        """G(X) = build_fock_x(X) - H_core = J(X) - K(X/2) for each symmetric density X of one
+        molecule, the two-electron response kernel of the CPHF solver (pm7-rs cphf_ov,
+        src/hessian.rs:1074-1081). ``densities`` is (k, nao, nao) or (nao, nao); the result has the
+        same shape. The integrals are built once for all k."""
+        cm, keep = self._mol(molecule)
+        x = np.ascontiguousarray(densities, dtype=np.float64)
+        single = x.ndim == 2
+        if single:
+            x = x[None]
+        nao = C.c_int32(0)
+        st = self._lib.pm7_core_hamiltonian(self._h, C.byref(cm), None, C.byref(nao))
+        if st != 0:
+            raise Pm7Error(st, self.last_error())
+        if x.ndim != 3 or x.shape[1:] != (nao.value, nao.value):
+            raise ValueError(f"densities must be (k, {nao.value}, {nao.value}), got {densities.shape}")
+        g = np.zeros_like(x)
+        st = self._lib.pm7_response_fock_batch(self._h, C.byref(cm), x.shape[0],
+                                               x.ctypes.data_as(C.POINTER(C.c_double)),
+                                               g.ctypes.data_as(C.POINTER(C.c_double)))
+        if st != 0:
+            raise Pm7Error(st, self.last_error())
+        return g[0] if single else g
 
     def core_core_energy(self, molecule: Molecule) -> float:
         cm, keep = self._mol(molecule)

@@ -96,6 +96,7 @@ def main() -> int:
     ap.add_argument("--d-batch", type=int, default=64,
                     help="S/P/Cl molecules (MNDO/d pairs, <= 196 AOs) batched in FP64 and mixed (0: skip)")
     ap.add_argument("--gradient", type=int, default=1, help="time batched nuclear gradients (0: skip)")
+    ap.add_argument("--response", type=int, default=1, help="time the CPHF response Fock (0: skip)")
     ap.add_argument("--serve-lanes", type=int, default=4)
     ap.add_argument("--serve-batch", type=int, default=64)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -370,6 +371,59 @@ def main() -> int:
                 "max_abs_gradient_ev_bohr": round(max(float(np.max(np.abs(g))) for g in g_vals), 6)}
         print(f"[bench] gradient: {gradient}", file=sys.stderr, flush=True)
 
+    # Response Fock (the CPHF kernel of the Hessian): 3N symmetric densities of the C32 alkane
+    # (98 atoms, 194 AOs, the size of the 102-atom Hessian molecule pm7-rs profiles) against its one
+    # integral set, one launch. Algorithmic HBM bytes per launch: the pair integrals and H once,
+    # each density read and its G written once.
+    response = None
+    if args.response:
+        ratoms = W.alkane_chain(32)
+        rmol = pm7amd.Molecule.from_symbols_angstrom(ratoms)
+        rn = 4 * 32 + 66
+        rk = 3 * len(ratoms)
+        rng = np.random.default_rng(17)
+        ra = rng.normal(size=(rk, rn, rn)) * 0.05
+        rxs = ra + ra.transpose(0, 2, 1)
+        eng.response_fock(rmol, rxs)
+        reps = max(2, min(args.steps, 5))
+        kms, t = [], time.perf_counter()
+        for _ in range(reps):
+            eng.response_fock(rmol, rxs)
+            kms.append(eng.last_timings_ms()[1])
+        dt = (time.perf_counter() - t) / reps
+        kernel_ms = float(np.mean(kms))
+        n_orb = [4 if a[0] != "H" else 1 for a in ratoms]
+        npair = {1: 1, 4: 10}
+        w_words = sum(npair[n_orb[u]] * npair[n_orb[v]] for u in range(len(n_orb)) for v in range(u + 1, len(n_orb)))
+        alg_bytes = 8.0 * (w_words + rn * rn + 2 * rk * rn * rn)
+        response = {"workload": f"C32H66 alkane (98 atoms, {rn} AOs), {rk} symmetric response densities (3N), one call",
+                    "densities_per_s": round(rk / (kernel_ms * 1e-3), 1), "kernel_ms": round(kernel_ms, 4),
+                    "api_ms_per_call": round(dt * 1e3, 3),
+                    "roofline": {"bound": "hbm", "achieved": round(alg_bytes / (kernel_ms * 1e-3) / 1e9, 1),
+                                 "peak": 8000.0, "unit": "GB/s",
+                                 "frac": round(alg_bytes / (kernel_ms * 1e-3) / 8.0e12, 5),
+                                 "algorithmic_bytes": int(alg_bytes),
+                                 "integral_words": int(w_words)}}
+        if not args.no_cpu:
+            # the oracle's Fock build minus H, one thread, on the first 24 densities
+            sys.path.insert(0, str(ROOT / "tests"))
+            import oracle_binding as ob
+
+            t = time.perf_counter()
+            h = ob.hcore(rmol.z, rmol.xyz)
+            t_int = time.perf_counter() - t
+            t = time.perf_counter()
+            for x in rxs[:24]:
+                ob.fock(rmol.z, rmol.xyz, x, 0.5 * x) - h
+            per = (time.perf_counter() - t) / 24
+            # each oracle call rebuilds the pair integrals (as long as one H_core build); the
+            # reference keeps them, so that time is taken out of the rate
+            response["cpu_baseline"] = {"densities_per_s": round(1.0 / max(per - t_int, 1e-9), 1), "cores": 1,
+                                        "kind": "port",
+                                        "sample": "24 of the densities through the CPU oracle's Fock build minus H, "
+                                                  f"{per * 1e3:.2f} ms each less one integral build ({t_int * 1e3:.2f} ms)"}
+        print(f"[bench] response fock: {response}", file=sys.stderr, flush=True)
+
     # One molecule at a time (the plain FP64 GPU path the batched path is compared with).
     single = mols[: args.single]
     for m in single[:2]:
@@ -481,6 +535,7 @@ def main() -> int:
         "dgemm_probe_tflops": dgemm_tf,
         "d_batch": d_batch,
         "gradient": gradient,
+        "response_fock": response,
         "single_molecule_fp64": {"solves_per_s": round(single_rate, 3), "molecules": len(single),
                                  "batched_speedup": round((solvesmx if head_mixed else solves64) / single_rate, 3)},
         "kernel_ms": {"integrals": round(float(np.mean([k[0] for k in kern64])), 4),

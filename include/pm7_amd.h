@@ -202,7 +202,8 @@ void pm7_server_stop(pm7_server* srv);
  * <= 112, <= 196, <= 1024 basis functions; 0 when the class was empty), then host wall clock:
  * [8] validation, layout and upload, [9] kernel launches, host-side corrections and the copy of
  * the packed results to the host (which waits for the kernels), [10] remaining device wait,
- * [11] result assembly. Writes min(n, PM7_TIMING_SLOTS) values. */
+ * [11] result assembly. After pm7_response_fock_batch: [1] its response kernel, the rest 0.
+ * Writes min(n, PM7_TIMING_SLOTS) values. */
 #define PM7_TIMING_SLOTS 12
 void pm7_last_timings(const pm7_ctx* ctx, double* ms, int32_t n);
 

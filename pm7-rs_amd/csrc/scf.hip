@@ -19,6 +19,8 @@
 // order, so a molecule's result does not depend on which other molecules share the launch.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "diis.h"
 #include "eigen.h"
 #include "fock.h"
@@ -637,7 +639,8 @@ __global__ void __launch_bounds__(256) k_fock_once(int mi, const MolDev* __restr
 // build_fock_x(X) = H + J(X) - K(X/2), src/fock.rs:665-684). Workgroup k takes density k:
 // X_k at xs + k*nao*ld (row stride ld), G_k written at gs + k*nao*ld, packed-density scratch at
 // pps + k*pp_stride. The densities are symmetric, as response densities C_v U C_o^T + transpose are.
-__global__ void __launch_bounds__(256) k_fock_response(int mi, const MolDev* __restrict__ mols,
+template <int NT>
+__global__ void __launch_bounds__(NT) k_fock_response(int mi, const MolDev* __restrict__ mols,
                                                         const int* __restrict__ atom_z, const int* __restrict__ atom_ao,
                                                         const int* __restrict__ atom_pp,
                                                         const PairDev* __restrict__ pairs,
@@ -661,11 +664,11 @@ __global__ void __launch_bounds__(256) k_fock_response(int mi, const MolDev* __r
   const int64_t sq = (int64_t)c.M.nao * c.ld;  // rows below nao only: nothing reads or writes the rest
   const double* X = xs + blockIdx.x * sq;
   double* G = gs + blockIdx.x * sq;
-  fock_build<256, double>(c, X, X, 0.5, nullptr, 0.0, G, nullptr);
+  fock_build<NT, double>(c, X, X, 0.5, nullptr, 0.0, G, nullptr);
   // F(X) - H, element by element, as the reference's kernel subtracts h_core from the Fock matrix
   const double* H = c.slot(S_H);
   const int n = c.M.nao;
-  for (int t = threadIdx.x; t < n * n; t += 256) {
+  for (int t = threadIdx.x; t < n * n; t += NT) {
     const int64_t i = (int64_t)(t / n) * c.ld + t % n;
     G[i] -= H[i];
   }
@@ -702,8 +705,19 @@ hipError_t launch_fock_once(const BatchDev& b, int mol, int pt_slot, int ps_slot
 hipError_t launch_fock_response(const BatchDev& b, int mol, const double* xs, double* gs, double* pps,
                                 int64_t pp_stride, int n_dens, hipStream_t s) {
   if (n_dens <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fock_response, dim3(n_dens), dim3(256), 0, s, mol, b.mols, b.atom_z, b.atom_ao, b.atom_pp,
-                     b.pairs, b.elem, b.ocd, b.w, b.slab, xs, gs, pps, pp_stride);
+  // Threads per density (environment PM7_AMD_RESPONSE_NT: 256, 512 or 1024), read once.
+  static const int nt = [] {
+    const char* e = std::getenv("PM7_AMD_RESPONSE_NT");
+    const int v = e ? std::atoi(e) : RESPONSE_NT_DEFAULT;
+    return (v == 256 || v == 512 || v == 1024) ? v : RESPONSE_NT_DEFAULT;
+  }();
+#define PM7_RESPONSE_LAUNCH(NT)                                                                                    \
+  hipLaunchKernelGGL(k_fock_response<NT>, dim3(n_dens), dim3(NT), 0, s, mol, b.mols, b.atom_z, b.atom_ao,        \
+                     b.atom_pp, b.pairs, b.elem, b.ocd, b.w, b.slab, xs, gs, pps, pp_stride)
+  if (nt == 1024) PM7_RESPONSE_LAUNCH(1024);
+  else if (nt == 512) PM7_RESPONSE_LAUNCH(512);
+  else PM7_RESPONSE_LAUNCH(256);
+#undef PM7_RESPONSE_LAUNCH
   return hipGetLastError();
 }
 

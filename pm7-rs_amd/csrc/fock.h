@@ -48,8 +48,17 @@ struct MolCtx {
   double* pp;    // packed densities
   int ld;
   const double* ocd;  // one-centre spd matrices by atomic number (pm7-rs src/fock.rs:171-189)
+  bool minus_h = false;  // store F - H (the response kernel G(X)) instead of F
   __device__ double* slot(int s) const { return base + (int64_t)s * ld * ld; }
 };
+
+// Stores one Fock element and its mirror; with c.minus_h, F - H, rounded as the two-step
+// (H + g) - H is, so G(X) equals build_fock_x(X) - H formed from the full matrix.
+__device__ inline void put_f(const MolCtx& c, double* __restrict__ F, int64_t i1, int64_t i2, double f, double h) {
+  const double v = c.minus_h ? f - h : f;
+  F[i1] = v;
+  F[i2] = v;
+}
 
 // One-centre (mu nu | la si) of an atom: the sp rules, or the element's 45 x 45 spd matrix.
 __device__ inline double oc_val(const MolCtx& c, const ElemDev& e, int mu, int nu, int la, int si) {
@@ -155,12 +164,10 @@ __device__ inline void fock_diag_item(const MolCtx& c, int t, const double* __re
     }
   const int64_t i1 = (int64_t)(o + mu) * ld + o + nu, i2 = (int64_t)(o + nu) * ld + o + mu;
   double fa = H[i1] + (double)(acc + ja - (T)xa * ka);
-  Fa[i1] = fa;
-  Fa[i2] = fa;
+  put_f(c, Fa, i1, i2, fa, H[i1]);
   if (Fb) {
     double fb = H[i1] + (double)(acc + ja - (T)xb * kb);
-    Fb[i1] = fb;
-    Fb[i2] = fb;
+    put_f(c, Fb, i1, i2, fb, H[i1]);
   }
 }
 
@@ -216,12 +223,10 @@ __device__ inline void fock_diag_item_wave(const MolCtx& c, int t, const double*
     }
   const int64_t i1 = (int64_t)(o + mu) * ld + o + nu, i2 = (int64_t)(o + nu) * ld + o + mu;
   double fa = H[i1] + (double)(acc + ja - (T)xa * ka);
-  Fa[i1] = fa;
-  Fa[i2] = fa;
+  put_f(c, Fa, i1, i2, fa, H[i1]);
   if (Fb) {
     double fb = H[i1] + (double)(acc + ja - (T)xb * kb);
-    Fb[i1] = fb;
-    Fb[i2] = fb;
+    put_f(c, Fb, i1, i2, fb, H[i1]);
   }
 }
 
@@ -249,12 +254,10 @@ __device__ void fock_pair_item_spd(const MolCtx& c, const PairDev& P, int nf, in
       }
       const int64_t i1 = (int64_t)(of + mu) * ld + os + lam, i2 = (int64_t)(os + lam) * ld + of + mu;
       const double fa = H[i1] - (double)((T)xa * acca);
-      Fa[i1] = fa;
-      Fa[i2] = fa;
+      put_f(c, Fa, i1, i2, fa, H[i1]);
       if (Fb) {
         const double fb = H[i1] - (double)((T)xb * accb);
-        Fb[i1] = fb;
-        Fb[i2] = fb;
+        put_f(c, Fb, i1, i2, fb, H[i1]);
       }
     }
 }
@@ -306,12 +309,10 @@ __device__ inline void fock_pair_item(const MolCtx& c, int p, const double* __re
       }
       const int64_t i1 = (int64_t)(of + mu) * ld + os + lam, i2 = (int64_t)(os + lam) * ld + of + mu;
       const double fa = H[i1] - (double)((T)xa * acca);
-      Fa[i1] = fa;
-      Fa[i2] = fa;
+      put_f(c, Fa, i1, i2, fa, H[i1]);
       if (Fb) {
         const double fb = H[i1] - (double)((T)xb * accb);
-        Fb[i1] = fb;
-        Fb[i2] = fb;
+        put_f(c, Fb, i1, i2, fb, H[i1]);
       }
     }
 }

@@ -661,17 +661,11 @@ __global__ void __launch_bounds__(NT) k_fock_response(int mi, const MolDev* __re
   c.ld = c.M.ld;
   c.base = slab + c.M.mat0;
   c.pp = pps + (int64_t)blockIdx.x * pp_stride;
+  c.minus_h = true;  // F(X) - H at each store, as the reference subtracts h_core from the Fock matrix
   const int64_t sq = (int64_t)c.M.nao * c.ld;  // rows below nao only: nothing reads or writes the rest
   const double* X = xs + blockIdx.x * sq;
   double* G = gs + blockIdx.x * sq;
   fock_build<NT, double>(c, X, X, 0.5, nullptr, 0.0, G, nullptr);
-  // F(X) - H, element by element, as the reference's kernel subtracts h_core from the Fock matrix
-  const double* H = c.slot(S_H);
-  const int n = c.M.nao;
-  for (int t = threadIdx.x; t < n * n; t += NT) {
-    const int64_t i = (int64_t)(t / n) * c.ld + t % n;
-    G[i] -= H[i];
-  }
 }
 
 hipError_t launch_scf(int variant, const BatchDev& b, int list_off, int count, hipStream_t s) {

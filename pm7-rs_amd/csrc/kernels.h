@@ -72,7 +72,8 @@ hipError_t launch_gradient(const BatchDev& b, int64_t n_pairs, const int2* items
 hipError_t launch_fock_once(const BatchDev& b, int mol, int pt_slot, int ps_slot, int f_slot, hipStream_t s);
 // G_k = J(X_k) - K(X_k / 2) for n_dens densities of molecule `mol` (slices of nao*ld at xs / gs;
 // pp_stride doubles of packed-density scratch per density at pps).
-// 512 threads per density: 0.41 ms for 294 densities of a 194-AO molecule (256: 0.58, 1024: 0.55).
+// 512 threads per density: 0.41 ms for 294 densities of a 194-AO molecule (256: 0.58, 1024: 0.55),
+// measured with the subtraction of H as a separate pass (0.38 ms with it fused into the stores).
 constexpr int RESPONSE_NT_DEFAULT = 512;
 hipError_t launch_fock_response(const BatchDev& b, int mol, const double* xs, double* gs, double* pps,
                                 int64_t pp_stride, int n_dens, hipStream_t s);

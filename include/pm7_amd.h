@@ -164,6 +164,13 @@ pm7_status pm7_fock_spin(pm7_ctx* ctx, const pm7_molecule_v1* mol, const double*
 pm7_status pm7_response_fock_batch(pm7_ctx* ctx, const pm7_molecule_v1* mol, int32_t n_dens,
                                    const double* dp, double* g_out);
 
+/* The open-shell response kernel of analytic_hessian_uhf (src/hessian.rs:1850-1880):
+ * ga_out[k] = build_fock_spin_x(dpa[k] + dpb[k], dpa[k]) - H and gb_out[k] likewise with dpb[k],
+ * for n_dens pairs of symmetric alpha/beta response densities (each array n_dens nao*nao matrices). */
+pm7_status pm7_response_fock_spin_batch(pm7_ctx* ctx, const pm7_molecule_v1* mol, int32_t n_dens,
+                                        const double* dpa, const double* dpb, double* ga_out,
+                                        double* gb_out);
+
 /* Replaces core_core_energy (src/repulsion.rs:16-36), eV. */
 pm7_status pm7_core_core_energy(pm7_ctx* ctx, const pm7_molecule_v1* mol, double* e_out);
 

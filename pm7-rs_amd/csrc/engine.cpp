@@ -1602,9 +1602,16 @@ pm7_status pm7_fock_spin(pm7_ctx* ctx, const pm7_molecule_v1* mol, const double*
   return PM7_OK;
 }
 
-pm7_status pm7_response_fock_batch(pm7_ctx* ctx, const pm7_molecule_v1* mol, int32_t n_dens, const double* dp,
-                                   double* g_out) {
-  if (!ctx || !mol || n_dens < 0 || (n_dens > 0 && (!dp || !g_out))) return PM7_ERR_INVALID_INPUT;
+namespace {
+
+// n_dens closed-shell densities (spin = false: in[0] -> out[0]) or alpha/beta pairs (spin = true:
+// in[0], in[1] -> out[0], out[1]) of one molecule through the response kernel, one launch.
+pm7_status response_fock_impl(pm7_ctx* ctx, const pm7_molecule_v1* mol, int32_t n_dens, bool spin,
+                              const double* const* in, double* const* out) {
+  const int n_sp = spin ? 2 : 1;
+  if (!ctx || !mol || n_dens < 0) return PM7_ERR_INVALID_INPUT;
+  for (int k = 0; k < n_sp && n_dens > 0; ++k)
+    if (!in[k] || !out[k]) return PM7_ERR_INVALID_INPUT;
   int32_t nao = 0;
   pm7_status st = pm7_core_hamiltonian(ctx, mol, nullptr, &nao);
   if (st != PM7_OK) return st;
@@ -1630,10 +1637,11 @@ pm7_status pm7_response_fock_batch(pm7_ctx* ctx, const pm7_molecule_v1* mol, int
   Lane& L = ctx->lane;
   const int64_t slice = (int64_t)J.nao * J.ld;  // densities are stored with row stride ld, nao rows each
   const int64_t pp_stride = ((int64_t)J.n_pp + 31) / 32 * 32;
-  const size_t mat_bytes = (size_t)n_dens * (size_t)slice * sizeof(double);
+  // alpha and beta slices back to back (plus the alpha + beta sums for the open-shell kernel)
+  const size_t mat_bytes = (size_t)n_sp * n_dens * (size_t)slice * sizeof(double);
   const size_t host_bytes = (size_t)n_dens * (size_t)J.nao * J.nao * sizeof(double);
-  if (L.d_rx.ensure(mat_bytes) != hipSuccess || L.d_rg.ensure(mat_bytes) != hipSuccess ||
-      L.d_rstage.ensure(host_bytes) != hipSuccess ||
+  if (L.d_rx.ensure(mat_bytes + (spin ? mat_bytes / 2 : 0)) != hipSuccess || L.d_rg.ensure(mat_bytes) != hipSuccess ||
+      L.d_rstage.ensure(n_sp * host_bytes) != hipSuccess ||
       L.d_rpp.ensure((size_t)n_dens * (size_t)pp_stride * sizeof(double)) != hipSuccess) {
     (void)hipGetLastError();
     return fail(ctx, PM7_ERR_INSUFFICIENT_MEMORY,
@@ -1645,20 +1653,43 @@ pm7_status pm7_response_fock_batch(pm7_ctx* ctx, const pm7_molecule_v1* mol, int
   // One contiguous copy each way over PCIe (a pitched copy from pageable memory goes row by row);
   // the change of row stride nao <-> ld is a device-to-device copy.
   double* stage = static_cast<double*>(L.d_rstage.p);
-  HIPCHK(hipMemcpyAsync(stage, dp, host_bytes, hipMemcpyHostToDevice, L.st[0]));
+  const int64_t host_words = (int64_t)n_dens * J.nao * J.nao;
+  for (int k = 0; k < n_sp; ++k)
+    HIPCHK(hipMemcpyAsync(stage + k * host_words, in[k], host_bytes, hipMemcpyHostToDevice, L.st[0]));
   HIPCHK(hipMemcpy2DAsync(xs, J.ld * sizeof(double), stage, J.nao * sizeof(double), J.nao * sizeof(double),
-                          (size_t)n_dens * J.nao, hipMemcpyDeviceToDevice, L.st[0]));
+                          (size_t)n_sp * n_dens * J.nao, hipMemcpyDeviceToDevice, L.st[0]));
   HIPCHK(hipEventRecord(L.ev[0], L.st[0]));
-  HIPCHK(launch_fock_response(P.b, 0, xs, gs, static_cast<double*>(L.d_rpp.p), pp_stride, n_dens, L.st[0]));
+  if (spin)
+    HIPCHK(launch_fock_response_spin(P.b, 0, xs, xs + 2 * n_dens * slice, gs, static_cast<double*>(L.d_rpp.p),
+                                     pp_stride, n_dens, L.st[0]));
+  else
+    HIPCHK(launch_fock_response(P.b, 0, xs, gs, static_cast<double*>(L.d_rpp.p), pp_stride, n_dens, L.st[0]));
   HIPCHK(hipEventRecord(L.ev[1], L.st[0]));
   HIPCHK(hipMemcpy2DAsync(stage, J.nao * sizeof(double), gs, J.ld * sizeof(double), J.nao * sizeof(double),
-                          (size_t)n_dens * J.nao, hipMemcpyDeviceToDevice, L.st[0]));
-  HIPCHK(hipMemcpyAsync(g_out, stage, host_bytes, hipMemcpyDeviceToHost, L.st[0]));
+                          (size_t)n_sp * n_dens * J.nao, hipMemcpyDeviceToDevice, L.st[0]));
+  for (int k = 0; k < n_sp; ++k)
+    HIPCHK(hipMemcpyAsync(out[k], stage + k * host_words, host_bytes, hipMemcpyDeviceToHost, L.st[0]));
   HIPCHK(hipStreamSynchronize(L.st[0]));
   float ms = 0.0f;
   for (double& t : ctx->timings) t = 0.0;
   if (hipEventElapsedTime(&ms, L.ev[0], L.ev[1]) == hipSuccess) ctx->timings[1] = ms;
   return PM7_OK;
+}
+
+}  // namespace
+
+pm7_status pm7_response_fock_batch(pm7_ctx* ctx, const pm7_molecule_v1* mol, int32_t n_dens, const double* dp,
+                                   double* g_out) {
+  const double* in[1] = {dp};
+  double* out[1] = {g_out};
+  return response_fock_impl(ctx, mol, n_dens, false, in, out);
+}
+
+pm7_status pm7_response_fock_spin_batch(pm7_ctx* ctx, const pm7_molecule_v1* mol, int32_t n_dens,
+                                        const double* dpa, const double* dpb, double* ga_out, double* gb_out) {
+  const double* in[2] = {dpa, dpb};
+  double* out[2] = {ga_out, gb_out};
+  return response_fock_impl(ctx, mol, n_dens, true, in, out);
 }
 
 pm7_status pm7_core_core_energy(pm7_ctx* ctx, const pm7_molecule_v1* mol, double* e_out) {

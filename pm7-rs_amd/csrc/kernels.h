@@ -77,6 +77,10 @@ hipError_t launch_fock_once(const BatchDev& b, int mol, int pt_slot, int ps_slot
 constexpr int RESPONSE_NT_DEFAULT = 512;
 hipError_t launch_fock_response(const BatchDev& b, int mol, const double* xs, double* gs, double* pps,
                                 int64_t pp_stride, int n_dens, hipStream_t s);
+// Open shell: G^a, G^b = J(Xa + Xb) - K(X^s) for n_dens pairs (alpha slices then beta slices at xs
+// and gs, sums at ts; each slice nao*ld).
+hipError_t launch_fock_response_spin(const BatchDev& b, int mol, const double* xs, double* ts, double* gs,
+                                     double* pps, int64_t pp_stride, int n_dens, hipStream_t s);
 
 // Large-system SCF of molecule `mi` (host copy M) of a prepared batch: grid-wide kernels and
 // rocBLAS/rocSOLVER on stream s; fills `out` as the megakernel would. Blocks the calling thread.

@@ -668,6 +668,48 @@ __global__ void __launch_bounds__(NT) k_fock_response(int mi, const MolDev* __re
   fock_build<NT, double>(c, X, X, 0.5, nullptr, 0.0, G, nullptr);
 }
 
+// The open-shell response kernel (pm7-rs analytic_hessian_uhf's step, src/hessian.rs:1850-1880):
+// G^s = build_fock_spin_x(Xa + Xb, X^s) - H_core for s = alpha, beta. Workgroup k takes the pair
+// (Xa_k, Xb_k) at xs + k*sq and xs + (n + k)*sq (sq = nao*ld), forms Xt = Xa + Xb at
+// ts + k*sq, and writes Ga at gs + k*sq, Gb at gs + (n + k)*sq.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_fock_response_spin(int mi, const MolDev* __restrict__ mols,
+                                                            const int* __restrict__ atom_z,
+                                                            const int* __restrict__ atom_ao,
+                                                            const int* __restrict__ atom_pp,
+                                                            const PairDev* __restrict__ pairs,
+                                                            const ElemDev* __restrict__ elem,
+                                                            const double* __restrict__ ocd,
+                                                            const double* __restrict__ w, double* __restrict__ slab,
+                                                            const double* __restrict__ xs, double* __restrict__ ts,
+                                                            double* __restrict__ gs, double* __restrict__ pps,
+                                                            int64_t pp_stride) {
+  MolCtx c;
+  c.M = mols[mi];
+  c.atom_z = atom_z;
+  c.atom_ao = atom_ao;
+  c.atom_pp = atom_pp;
+  c.pairs = pairs;
+  c.elem = elem;
+  c.ocd = ocd;
+  c.w = w;
+  c.ld = c.M.ld;
+  c.base = slab + c.M.mat0;
+  c.pp = pps + (int64_t)blockIdx.x * pp_stride;
+  c.minus_h = true;
+  const int n = c.M.nao;
+  const int64_t sq = (int64_t)n * c.ld;
+  const double* Xa = xs + blockIdx.x * sq;
+  const double* Xb = xs + (gridDim.x + blockIdx.x) * sq;
+  double* Xt = ts + blockIdx.x * sq;
+  for (int t = threadIdx.x; t < n * n; t += NT) {
+    const int64_t i = (int64_t)(t / n) * c.ld + t % n;
+    Xt[i] = Xa[i] + Xb[i];
+  }
+  __syncthreads();
+  fock_build<NT, double>(c, Xt, Xa, 1.0, Xb, 1.0, gs + blockIdx.x * sq, gs + (gridDim.x + blockIdx.x) * sq);
+}
+
 hipError_t launch_scf(int variant, const BatchDev& b, int list_off, int count, hipStream_t s) {
   if (count <= 0) return hipSuccess;
   const int* list = b.mol_list + list_off;
@@ -712,6 +754,15 @@ hipError_t launch_fock_response(const BatchDev& b, int mol, const double* xs, do
   else if (nt == 512) PM7_RESPONSE_LAUNCH(512);
   else PM7_RESPONSE_LAUNCH(256);
 #undef PM7_RESPONSE_LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t launch_fock_response_spin(const BatchDev& b, int mol, const double* xs, double* ts, double* gs,
+                                     double* pps, int64_t pp_stride, int n_dens, hipStream_t s) {
+  if (n_dens <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fock_response_spin<RESPONSE_NT_DEFAULT>, dim3(n_dens), dim3(RESPONSE_NT_DEFAULT), 0, s, mol,
+                     b.mols, b.atom_z, b.atom_ao, b.atom_pp, b.pairs, b.elem, b.ocd, b.w, b.slab, xs, ts, gs, pps,
+                     pp_stride);
   return hipGetLastError();
 }
 

@@ -97,7 +97,7 @@ ABI_FUNCTIONS = [
     "pm7_default_options", "pm7_run", "pm7_run_batch", "pm7_result_free", "pm7_core_hamiltonian",
     "pm7_fock_spin", "pm7_core_core_energy", "pm7_last_timings", "pm7_server_default_config",
     "pm7_server_start", "pm7_server_submit", "pm7_server_poll", "pm7_server_stop", "pm7_gradient_batch",
-    "pm7_gradient", "pm7_response_fock_batch",
+    "pm7_gradient", "pm7_response_fock_batch", "pm7_response_fock_spin_batch",
 ]
 
 _lib = None
@@ -132,6 +132,9 @@ def library() -> C.CDLL:
     lib.pm7_core_core_energy.argtypes = [vp, C.POINTER(_Molecule), C.POINTER(C.c_double)]
     lib.pm7_response_fock_batch.argtypes = [vp, C.POINTER(_Molecule), C.c_int32, C.POINTER(C.c_double),
                                             C.POINTER(C.c_double)]
+    lib.pm7_response_fock_spin_batch.argtypes = [vp, C.POINTER(_Molecule), C.c_int32, C.POINTER(C.c_double),
+                                                 C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                                 C.POINTER(C.c_double)]
     lib.pm7_last_timings.argtypes = [vp, C.POINTER(C.c_double), C.c_int32]
     lib.pm7_server_default_config.argtypes = [C.POINTER(_ServerConfig)]
     lib.pm7_server_start.argtypes = [vp, C.POINTER(_ServerConfig), C.POINTER(vp)]
@@ -543,6 +546,29 @@ class Engine:
         if st != 0:
             raise Pm7Error(st, self.last_error())
         return g[0] if single else g
+
+    def response_fock_spin(self, molecule: Molecule, dpa: np.ndarray, dpb: np.ndarray):
+        """(G^a, G^b) with G^s = build_fock_spin_x(dPa + dPb, dP^s) - H_core for each pair of symmetric
+        alpha/beta response densities, the open-shell CPHF kernel (pm7-rs analytic_hessian_uhf,
+        src/hessian.rs:1850-1880). Arrays are (k, nao, nao) or (nao, nao)."""
+        cm, keep = self._mol(molecule)
+        a = np.ascontiguousarray(dpa, dtype=np.float64)
+        b = np.ascontiguousarray(dpb, dtype=np.float64)
+        single = a.ndim == 2
+        if single:
+            a, b = a[None], b[None]
+        nao = C.c_int32(0)
+        st = self._lib.pm7_core_hamiltonian(self._h, C.byref(cm), None, C.byref(nao))
+        if st != 0:
+            raise Pm7Error(st, self.last_error())
+        if a.ndim != 3 or a.shape[1:] != (nao.value, nao.value) or b.shape != a.shape:
+            raise ValueError(f"densities must be two (k, {nao.value}, {nao.value}) arrays, got {dpa.shape}, {dpb.shape}")
+        ga, gb = np.zeros_like(a), np.zeros_like(b)
+        ptr = lambda x: x.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+        st = self._lib.pm7_response_fock_spin_batch(self._h, C.byref(cm), a.shape[0], ptr(a), ptr(b), ptr(ga), ptr(gb))
+        if st != 0:
+            raise Pm7Error(st, self.last_error())
+        return (ga[0], gb[0]) if single else (ga, gb)
 
     def core_core_energy(self, molecule: Molecule) -> float:
         cm, keep = self._mol(molecule)

@@ -3,7 +3,9 @@
 (pm7_response_fock_batch; pm7-rs cphf_ov's kernel, src/hessian.rs:1074-1081: build_fock_x(X) - H_core,
 src/fock.rs:665-684). Held to the CPU oracle's Fock build minus its core Hamiltonian, to the
 linearity every response kernel has (G(aX + bY) = aG(X) + bG(Y), at any size), to the converged
-SCF (H + G(P) commutes with P), and to batch/alone identity."""
+SCF (H + G(P) commutes with P), and to batch/alone identity. The open-shell kernel
+(pm7_response_fock_spin_batch; analytic_hessian_uhf, src/hessian.rs:1850-1880) is held to the same
+oracle and reduces to the closed-shell one for equal alpha and beta halves."""
 
 from __future__ import annotations
 
@@ -120,3 +122,27 @@ def test_response_fock_edge_cases(eng):
     assert np.array_equal(eng.response_fock(mol, np.zeros((2, 6, 6))), np.zeros((2, 6, 6)))
     with pytest.raises(ValueError):
         eng.response_fock(mol, np.zeros((1, 5, 5)))
+
+
+@pytest.mark.parametrize("name", ["methyl_radical", "chlorine_dioxide", "water", "thiophene"])
+def test_response_fock_spin_matches_oracle(eng, name):
+    say(f"response fock spin {name}")
+    c = CASES[name]
+    mol = pm7amd.Molecule.from_symbols_angstrom(c["atoms"], c["charge"], c["multiplicity"])
+    n = ob.nao(mol.z)
+    rng = np.random.default_rng(21)
+    a = rng.normal(size=(3, n, n)) * 0.2
+    b = rng.normal(size=(3, n, n)) * 0.2
+    xa, xb = a + a.transpose(0, 2, 1), b + b.transpose(0, 2, 1)
+    ga, gb = eng.response_fock_spin(mol, xa, xb)
+    h = ob.hcore(mol.z, mol.xyz)
+    for k in range(3):
+        assert np.max(np.abs(ga[k] - (ob.fock(mol.z, mol.xyz, xa[k] + xb[k], xa[k]) - h))) <= 1e-9
+        assert np.max(np.abs(gb[k] - (ob.fock(mol.z, mol.xyz, xa[k] + xb[k], xb[k]) - h))) <= 1e-9
+    # equal halves: the open-shell kernel is the closed-shell one
+    g = eng.response_fock(mol, xa)
+    ha, hb = eng.response_fock_spin(mol, 0.5 * xa, 0.5 * xa)
+    assert np.max(np.abs(ha - g)) <= 1e-12 * max(1.0, np.max(np.abs(g))) and np.array_equal(ha, hb)
+    # one pair alone equals the same pair in the batch
+    a1, b1 = eng.response_fock_spin(mol, xa[2], xb[2])
+    assert np.array_equal(a1, ga[2]) and np.array_equal(b1, gb[2])

@@ -31,6 +31,7 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 SCF_KERNELS = ["pm7::k_scf<256, 64, true, true>", "pm7::k_scf<512, 112, true, true>",
                "pm7::k_scf<512, 196, true, false>", "pm7::k_scf<512, 1024, false, false>"]
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_organic256_fp64.json"
+RESPONSE_NT = int(os.environ.get("PM7_AMD_RESPONSE_NT", "512"))  # threads per density of the response kernel
 
 
 def measured_traffic(kernel):
@@ -402,6 +403,7 @@ def main() -> int:
                     "roofline": {"bound": "hbm", "achieved": round(alg_bytes / (kernel_ms * 1e-3) / 1e9, 1),
                                  "peak": 8000.0, "unit": "GB/s",
                                  "frac": round(alg_bytes / (kernel_ms * 1e-3) / 8.0e12, 5),
+                                 "traffic": measured_traffic(f"pm7::k_fock_response<{RESPONSE_NT}>"),
                                  "algorithmic_bytes": int(alg_bytes),
                                  "integral_words": int(w_words)}}
         if not args.no_cpu:

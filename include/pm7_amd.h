@@ -65,7 +65,7 @@ typedef struct {
   double level_shift;    /* static level-shift floor in eV, default 0 */
   int32_t reference;     /* pm7_reference, default AUTO */
   int32_t precision;     /* pm7_precision, default FP64 (engine extension) */
-  double mixed_switch;   /* FP32 -> FP64 hand-over density change for PRECISION_MIXED (1e-5) */
+  double mixed_switch;   /* FP32 -> FP64 hand-over density change for PRECISION_MIXED (1e-3) */
   int64_t max_memory_mb; /* device-memory budget for this call; <= 0 means 80% of free HBM */
 } pm7_options_v1;
 

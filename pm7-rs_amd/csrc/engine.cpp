@@ -1391,7 +1391,7 @@ void pm7_default_options(pm7_options_v1* o) {
   o->level_shift = 0.0;
   o->reference = PM7_REF_AUTO;
   o->precision = PM7_PRECISION_FP64;
-  o->mixed_switch = 1.0e-5;
+  o->mixed_switch = 1.0e-3;
   o->max_memory_mb = 0;
 }
 

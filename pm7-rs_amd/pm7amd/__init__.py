@@ -230,7 +230,7 @@ class Pm7Options:
     level_shift: float = 0.0
     reference: str = "auto"
     precision: str = "fp64"
-    mixed_switch: float = 1.0e-5
+    mixed_switch: float = 1.0e-3
     max_memory_mb: Optional[int] = None
 
     def to_c(self) -> _Options:

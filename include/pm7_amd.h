@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define PM7_ABI_VERSION 1
+#define PM7_ABI_VERSION 2
 
 typedef enum {
   PM7_OK = 0,
@@ -37,10 +37,27 @@ typedef enum {
   PM7_NOT_READY = 9               /* request server: the request has not finished yet */
 } pm7_status;
 
-typedef enum { PM7_METHOD_PM7 = 0, PM7_METHOD_PM7_TS = 1, PM7_METHOD_PM7_MINUS = 3 } pm7_method;
+/* pm7-rs Pm7Method (src/method.rs:9-16) with its parameter-cache slots (src/params.rs:145-150).
+ * PM7-Sparkle (lanthanide +3 point cores, src/params.rs:171-172) is refused with
+ * PM7_ERR_UNSUPPORTED; any other code is PM7_ERR_INVALID_INPUT. PM7-HH is PM7 plus the H-H
+ * repulsion correction (src/hh_rep.rs:39-60, src/scf.rs:922-923). */
+typedef enum {
+  PM7_METHOD_PM7 = 0,
+  PM7_METHOD_PM7_TS = 1,
+  PM7_METHOD_PM7_SPARKLE = 2,
+  PM7_METHOD_PM7_MINUS = 3,
+  PM7_METHOD_PM7_HH = 4
+} pm7_method;
 typedef enum { PM7_ACCEL_NONE = 0, PM7_ACCEL_CDIIS = 1, PM7_ACCEL_ADIIS_CDIIS = 2 } pm7_accelerator;
 typedef enum { PM7_REF_AUTO = 0, PM7_REF_RHF = 1, PM7_REF_UHF = 2 } pm7_reference;
 typedef enum { PM7_PRECISION_FP64 = 0, PM7_PRECISION_MIXED = 1 } pm7_precision;
+/* pm7-rs DipoleOrigin (src/dipole.rs:71-96): origin of the point-charge dipole of an ion
+ * (a molecule with |net charge| <= 0.5 always uses the coordinate origin, src/dipole.rs:143-146). */
+typedef enum {
+  PM7_DIPOLE_CENTRE_OF_MASS = 0,
+  PM7_DIPOLE_COORDINATES = 1,
+  PM7_DIPOLE_CENTRE_OF_CHARGE = 2
+} pm7_dipole_origin;
 
 typedef struct pm7_ctx pm7_ctx;
 
@@ -51,7 +68,9 @@ typedef struct {
   const double* xyz_bohr;
 } pm7_molecule_v1;
 
-/* pm7-rs Pm7Options (src/scf.rs:73-234), defaults from pm7_default_options (src/scf.rs:207-234). */
+/* pm7-rs Pm7Options (src/scf.rs:73-234), defaults from pm7_default_options (src/scf.rs:207-234).
+ * Not offered: periodic cells, external fields, stability analysis and the Hessian's exchange cutoff
+ * and CPHF budget (outside the molecular single point). */
 typedef struct {
   int32_t method;        /* pm7_method; must match the loaded tables (src/scf.rs:530-535) */
   double charge;         /* options.charge is the charge used (src/scf.rs:592) */
@@ -67,7 +86,17 @@ typedef struct {
   int32_t precision;     /* pm7_precision, default FP64 (engine extension) */
   double mixed_switch;   /* FP32 -> FP64 hand-over density change for PRECISION_MIXED (1e-3) */
   int64_t max_memory_mb; /* device-memory budget for this call; <= 0 means 80% of free HBM */
-} pm7_options_v1;
+  int32_t force_dpath;   /* every pair through the MNDO/d kernels, sp molecules too (src/scf.rs:97) */
+  int32_t smooth_dispersion; /* continuous carbon C6 (src/scf.rs:120, src/dispersion.rs:133-175) */
+  int32_t dipole_origin; /* pm7_dipole_origin (src/scf.rs:187) */
+  int32_t initial_density_nao; /* rows = columns of the two arrays below; must equal the basis size */
+  /* Start the SCF from this density instead of the atomic guess (src/scf.rs:161-163, 1201-1210),
+   * nao*nao row-major, or NULL. UHF: P_alpha = n_alpha/n P, P_beta = n_beta/n P (src/scf.rs:1341-1360). */
+  const double* initial_density;
+  /* UHF only: add this spin density P_alpha - P_beta (P_alpha += s/2, P_beta -= s/2;
+   * src/scf.rs:170, 1361-1378), nao*nao row-major, or NULL. */
+  const double* initial_spin_density;
+} pm7_options_v2;
 
 /* pm7-rs Pm7Result (src/scf.rs:262-342), molecular fields. The arrays of one result are allocated
  * by the library (one allocation) and released together by pm7_result_free. */
@@ -98,6 +127,17 @@ typedef struct {
                                 commutator, DIIS extrapolation, eigensolve, density, whole SCF */
   double eig_phase_ms[3];    /* eigensolve split: warm-start transform, Jacobi sweeps, sort and
                                 back-transform */
+  /* PRECISION_MIXED: how far the result can be from the FP64 one. The reported density and energy
+   * come from one full FP64 eigensolve of the Fock matrix of the converged density (a Roothaan
+   * step); err_vs_fp64_ev = |E(step) - E(converged)| and err_vs_fp64_density = max |P(step) -
+   * P(converged)|, both 0 at an exact fixed point. An FP64 run reports 0 for both (it is the
+   * reference). */
+  double err_vs_fp64_ev;
+  double err_vs_fp64_density;
+  int32_t missing_element;   /* PM7_ERR_MISSING_ELEMENT: the atomic number (Pm7Error::MissingElement) */
+  int32_t reserved_;
+  double needed_mb, budget_mb; /* PM7_ERR_INSUFFICIENT_MEMORY (Pm7Error::InsufficientMemory); what
+                                  ran out is named in message */
   double* density;           /* nao*nao, total density */
   double* spin_density;      /* nao*nao, P_alpha - P_beta, UHF only (else NULL) */
   double* mo_energies;       /* nao */
@@ -106,7 +146,7 @@ typedef struct {
   double* mo_coeff_beta;     /* UHF only */
   double* charges;           /* n_atoms Mulliken net charges */
   char message[256];
-} pm7_result_v1;
+} pm7_result_v2;
 
 /* Context: device + streams + parameter tables + pooled device memory.
  * data_dir: directory of the bundled parameter tables (NULL: next to the library). */
@@ -118,20 +158,20 @@ int32_t pm7_abi_version(void);
 /* Replaces Pm7Parameters::method (src/params.rs:130-155): (re)load the bundled tables. */
 pm7_status pm7_params_load(pm7_ctx* ctx, int32_t method);
 
-void pm7_default_options(pm7_options_v1* opts);
+void pm7_default_options(pm7_options_v2* opts);
 
 /* Replaces run_pm7 (src/scf.rs:512-867) for one molecule. */
-pm7_status pm7_run(pm7_ctx* ctx, const pm7_molecule_v1* mol, const pm7_options_v1* opts,
-                   pm7_result_v1* out);
+pm7_status pm7_run(pm7_ctx* ctx, const pm7_molecule_v1* mol, const pm7_options_v2* opts,
+                   pm7_result_v2* out);
 
 /* Many molecules of uneven size in one persistent launch per size class; results[i] is
  * exactly what pm7_run would return for mols[i] (fixed-order reductions, no atomics).
  * opts may hold one entry (shared) or n entries. Returns PM7_OK when every call was dispatched;
  * per-molecule outcomes are in results[i].status. */
 pm7_status pm7_run_batch(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols,
-                         const pm7_options_v1* opts, int32_t n_opts, pm7_result_v1* results);
+                         const pm7_options_v2* opts, int32_t n_opts, pm7_result_v2* results);
 
-void pm7_result_free(pm7_result_v1* res);
+void pm7_result_free(pm7_result_v2* res);
 
 /* Replaces closed_form_gradient (src/gradient.rs:500-555) for many molecules: the SCF of each
  * (results[i] exactly as pm7_run_batch fills it), then dE_total/dR at the converged density on the
@@ -140,12 +180,12 @@ void pm7_result_free(pm7_result_v1* res);
  * array to 3 x the total atom count. A molecule whose SCF left no density gets NaN and
  * results[i].status = PM7_ERR_UNSUPPORTED (its energies, if any, are still filled). */
 pm7_status pm7_gradient_batch(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols,
-                              const pm7_options_v1* opts, int32_t n_opts, pm7_result_v1* results,
+                              const pm7_options_v2* opts, int32_t n_opts, pm7_result_v2* results,
                               double* gradient);
 
 /* One molecule (gradient: 3 * n_atoms doubles, eV/Bohr); returns the result's status. */
-pm7_status pm7_gradient(pm7_ctx* ctx, const pm7_molecule_v1* mol, const pm7_options_v1* opts,
-                        pm7_result_v1* out, double* gradient);
+pm7_status pm7_gradient(pm7_ctx* ctx, const pm7_molecule_v1* mol, const pm7_options_v2* opts,
+                        pm7_result_v2* out, double* gradient);
 
 /* Replaces build_core (src/hamiltonian.rs:224-231): the core Hamiltonian H (nao*nao). */
 pm7_status pm7_core_hamiltonian(pm7_ctx* ctx, const pm7_molecule_v1* mol, double* h_out,
@@ -192,14 +232,14 @@ void pm7_server_default_config(pm7_server_config_v1* cfg);
 pm7_status pm7_server_start(pm7_ctx* ctx, const pm7_server_config_v1* cfg, pm7_server** out);
 /* Queue one request (opts may be NULL for the defaults). Invalid input, unsupported elements and
  * requests larger than the whole budget finish at once with that status. */
-pm7_status pm7_server_submit(pm7_server* srv, const pm7_molecule_v1* mol, const pm7_options_v1* opts,
+pm7_status pm7_server_submit(pm7_server* srv, const pm7_molecule_v1* mol, const pm7_options_v2* opts,
                              int64_t* request_id);
 /* Claim a finished request: request_id >= 0 waits for that request, < 0 for any unclaimed one.
  * timeout_ms: 0 = do not wait, < 0 = wait without limit. Fills *out (release with
  * pm7_result_free), the id and the latency from submit to completion in ms; PM7_NOT_READY when
  * nothing finished in time. */
 pm7_status pm7_server_poll(pm7_server* srv, int64_t request_id, int32_t timeout_ms, int64_t* done_id,
-                           pm7_result_v1* out, double* latency_ms);
+                           pm7_result_v2* out, double* latency_ms);
 /* Finish the queued and in-flight requests, release unclaimed results, stop the thread. */
 void pm7_server_stop(pm7_server* srv);
 

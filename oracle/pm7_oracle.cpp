@@ -1777,7 +1777,7 @@ int build_mol(const Params& P, int n, const int* z, const double* xyz, Mol& m, s
   return 0;
 }
 
-Core build_core(const Mol& m) {
+Core build_core(const Mol& m, bool force_dpath = false) {
   int nao = m.nao, n = m.n();
   Core c;
   c.h.assign((size_t)nao * nao, 0.0);
@@ -1791,6 +1791,7 @@ Core build_core(const Mol& m) {
   for (int u = 0; u < n; ++u)
     for (int v = u + 1; v < n; ++v) list.push_back({u, v});
   c.pairs.resize(list.size());
+  c.d_path = force_dpath;  // force_dpath (src/scf.rs:97, 544-582)
   for (int i = 0; i < n; ++i) c.d_path = c.d_path || m.el[i]->n_orb == 9;  // src/hamiltonian.rs:347-354
   parallel_for((long)list.size(), [&](long p) {
     int u = list[p].first, v = list[p].second;
@@ -2484,7 +2485,9 @@ ScfState rhf_loop(const Mol& m, const Core& core, int n_occ, const oracle_option
   int nao = m.nao;
   ScfState st;
   st.n_occ = n_occ;
-  Mat density = sad_density(m, 2.0 * n_occ);
+  // The caller's density when there is one (src/scf.rs:1201-1210).
+  Mat density = o.initial_density ? Mat(o.initial_density, o.initial_density + (size_t)m.nao * m.nao)
+                                  : sad_density(m, 2.0 * n_occ);
   double e_old = 0.0;
   std::vector<Mat> hf, he, hd;
   int accel = o.use_diis ? o.accelerator : 0;
@@ -2545,12 +2548,18 @@ ScfState uhf_loop(const Mol& m, const Core& core, int na, int nb, const oracle_o
   st.uhf = true;
   st.n_occ = na;
   st.n_occ_b = nb;
-  Mat total = sad_density(m, (double)(na + nb));
+  Mat total = o.initial_density ? Mat(o.initial_density, o.initial_density + (size_t)m.nao * m.nao)
+                                : sad_density(m, (double)(na + nb));  // src/scf.rs:1341-1360
   double ntot = std::max(na + nb, 1);
   double fa = na / ntot, fb = nb / ntot;
   Mat pa = total, pb = total;
   for (auto& v : pa) v *= fa;
   for (auto& v : pb) v *= fb;
+  if (o.initial_spin_density)  // P_alpha += s/2, P_beta -= s/2 (src/scf.rs:1361-1378)
+    for (size_t k = 0; k < pa.size(); ++k) {
+      pa[k] += 0.5 * o.initial_spin_density[k];
+      pb[k] -= 0.5 * o.initial_spin_density[k];
+    }
   double e_old = 0.0;
   StallGuard guard(o.level_shift);
   std::vector<double> eps_a;
@@ -2677,15 +2686,21 @@ double core_core_energy(const Params& P, const Mol& m) {
 // ---------------------------------------------------------------------------------------------
 // Post-SCF geometric corrections (kcal/mol)
 // ---------------------------------------------------------------------------------------------
-// Covalent radii (src/constants.rs covalent_radius_angstrom), Z = 0..20.
+// Covalent radii (src/constants.rs:64-74 covalent_radius_angstrom), Z = 0..86, else 1.5.
 double covalent_radius(int z) {
-  static const double R[21] = {0.0,  0.31, 0.28, 1.28, 0.96, 0.84, 0.76, 0.71, 0.66, 0.57, 0.58,
-                               1.66, 1.41, 1.21, 1.11, 1.07, 1.05, 1.02, 1.06, 2.03, 1.76};
-  return (z >= 0 && z <= 20) ? R[z] : 1.5;
+  static const double R[87] = {
+      0.0,  0.31, 0.28, 1.28, 0.96, 0.84, 0.76, 0.71, 0.66, 0.57, 0.58, 1.66, 1.41, 1.21, 1.11,
+      1.07, 1.05, 1.02, 1.06, 2.03, 1.76, 1.70, 1.60, 1.53, 1.39, 1.39, 1.32, 1.26, 1.24, 1.32,
+      1.22, 1.22, 1.20, 1.19, 1.20, 1.20, 1.16, 2.20, 1.95, 1.90, 1.75, 1.64, 1.54, 1.47, 1.46,
+      1.42, 1.39, 1.45, 1.44, 1.42, 1.39, 1.39, 1.38, 1.39, 1.40, 2.44, 2.15, 2.07, 2.04, 2.03,
+      2.01, 1.99, 1.98, 1.98, 1.96, 1.94, 1.92, 1.92, 1.89, 1.90, 1.87, 1.87, 1.75, 1.70, 1.62,
+      1.51, 1.44, 1.41, 1.36, 1.36, 1.32, 1.45, 1.46, 1.48, 1.40, 1.50, 1.50};
+  return (z >= 0 && z <= 86) ? R[z] : 1.5;
 }
 
-// D2 dispersion (src/dispersion.rs:19-264).
-double dispersion_energy(const std::vector<int>& z, const std::vector<V3>& pos) {
+// D2 dispersion (src/dispersion.rs:19-264); smooth: the continuous carbon C6 of
+// dispersion_energy_smooth_cut (src/dispersion.rs:72-175, 383-412).
+double dispersion_energy(const std::vector<int>& z, const std::vector<V3>& pos, bool smooth = false) {
   static const double ALPHA = 15.450118, S = 1.226593, CSCALE = 2.286419;
   auto c6tab = [](int zz) {
     switch (zz) {
@@ -2716,6 +2731,7 @@ double dispersion_energy(const std::vector<int>& z, const std::vector<V3>& pos) 
   };
   int n = (int)z.size();
   std::vector<int> nb(n, 0);
+  std::vector<double> cn(n, 0.0);  // smooth neighbour counts (count_weight, src/dispersion.rs:106-120)
   for (int i = 0; i < n; ++i)
     for (int j = i + 1; j < n; ++j) {
       double d = norm(sub(pos[j], pos[i]));
@@ -2723,7 +2739,18 @@ double dispersion_energy(const std::vector<int>& z, const std::vector<V3>& pos) 
         nb[i]++;
         nb[j]++;
       }
+      const double rcut = 1.3 * (covalent_radius(z[i]) + covalent_radius(z[j])) / A0;
+      const double ex = std::exp(-26.0 * (rcut / d - 1.0));
+      const double w = std::isfinite(ex) ? 1.0 / (1.0 + ex) : 0.0;
+      cn[i] += w;
+      cn[j] += w;
     }
+  // c6_atom_smooth (src/dispersion.rs:133-140): 1.65 + (0.95 - 1.65) s, s = 1/(1 + e^{-40 (cn - 3.5)}).
+  auto c6c = [&](int i) {
+    if (!smooth) return nb[i] == 4 ? 0.95 : 1.65;
+    const double sw = 1.0 / (1.0 + std::exp(-(cn[i] - 3.5) * 40.0));
+    return 1.65 + (0.95 - 1.65) * sw;
+  };
   double sum = 0.0;
   for (int i = 0; i < n; ++i)
     for (int j = i + 1; j < n; ++j) {
@@ -2732,8 +2759,8 @@ double dispersion_energy(const std::vector<int>& z, const std::vector<V3>& pos) 
                 r0tab(zj) != 0 && nefftab(zj) != 0;
       if (!ok) continue;
       double ni = nefftab(zi), nj = nefftab(zj);
-      double c6i = zi == 6 ? (nb[i] == 4 ? 0.95 : 1.65) : c6tab(zi);
-      double c6j = zj == 6 ? (nb[j] == 4 ? 0.95 : 1.65) : c6tab(zj);
+      double c6i = zi == 6 ? c6c(i) : c6tab(zi);
+      double c6j = zj == 6 ? c6c(j) : c6tab(zj);
       double c6 = 2.0 * std::pow(c6i * c6i * c6j * c6j * ni * nj, 1.0 / 3.0) /
                   (std::pow(c6i * nj * nj, 1.0 / 3.0) + std::pow(c6j * ni * ni, 1.0 / 3.0));
       double ri = r0tab(zi), rj = r0tab(zj);
@@ -2743,6 +2770,29 @@ double dispersion_energy(const std::vector<int>& z, const std::vector<V3>& pos) 
       double damp_denom = std::exp((rij_nm * (1.0 / (S * r0)) - 1.0) * (-ALPHA)) + 1.0;
       double e = (1.0 / std::pow(rij_nm, 6)) * (c6 / (1000.0 * 4.184)) / damp_denom;
       sum += e * (-CSCALE);
+    }
+  return sum;
+}
+
+// PM7-HH H-H repulsion (src/hh_rep.rs:21-60): every H-H pair adds poly(r / Angstrom), kcal/mol.
+double hh_repulsion_energy(const std::vector<int>& z, const std::vector<V3>& pos) {
+  double sum = 0.0;
+  const int n = (int)z.size();
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      if (z[i] != 1 || z[j] != 1) continue;
+      const double r = norm(sub(pos[j], pos[i])) * A0;
+      double e;
+      if (r <= 1.0) {
+        e = 25.46293603147693;
+      } else if (r < 1.5) {
+        e = std::pow(r, 5) * -2714.9523516034697 + std::pow(r, 4) * 17103.650110591705 +
+            std::pow(r, 3) * -42511.85798221796 + r * r * 52063.19679913834 + r * -31430.65833597229 +
+            7516.08469609514;
+      } else {
+        e = std::exp(std::pow(r, 1.72905) * -1.53965) * 118.7326;
+      }
+      sum += e;
     }
   return sum;
 }
@@ -3161,6 +3211,12 @@ void oracle_default_options(oracle_options* o) {
   o->p_tol = 1.0e-7;
   o->adiis_switch = 0.1;
   o->level_shift = 0.0;
+  o->force_dpath = 0;
+  o->smooth_dispersion = 0;
+  o->dipole_origin = 0;
+  o->pad_ = 0;
+  o->initial_density = nullptr;
+  o->initial_spin_density = nullptr;
 }
 
 int oracle_nao(const char* data_dir, int n_atoms, const int* z) {
@@ -3266,6 +3322,14 @@ double oracle_core_core(const char* data_dir, int n_atoms, const int* z, const d
   return core_core_energy(*P, m);
 }
 
+void oracle_corrections2(int n_atoms, const int* z, const double* xyz, double* out2) {
+  std::vector<int> zz(z, z + n_atoms);
+  std::vector<V3> pos(n_atoms);
+  for (int i = 0; i < n_atoms; ++i) pos[i] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+  out2[0] = dispersion_energy(zz, pos, true);
+  out2[1] = hh_repulsion_energy(zz, pos);
+}
+
 void oracle_corrections(int n_atoms, const int* z, const double* xyz, double* out4) {
   std::vector<int> zz(z, z + n_atoms);
   std::vector<V3> pos(n_atoms);
@@ -3348,13 +3412,14 @@ int oracle_run(const char* data_dir, int n_atoms, const int* z, const double* xy
     set_msg(out, "restricted reference needs a closed shell");
     return out->status = 1;
   }
-  Core core = build_core(m);
+  Core core = build_core(m, o.force_dpath != 0);
   ScfState s = uhf ? uhf_loop(m, core, na, nb, o) : rhf_loop(m, core, na, o);
   double core_ev = core_core_energy(*P, m);
   double corr = 0.0;
-  if (o.method != 3) {
-    corr += dispersion_energy(m.z, m.pos);
+  if (o.method != 3) {  // correction_energy (src/scf.rs:907-932)
+    corr += dispersion_energy(m.z, m.pos, o.smooth_dispersion != 0);
     corr += hbond_energy(m.z, m.pos);
+    if (o.method == 4) corr += hh_repulsion_energy(m.z, m.pos);
     corr += c_triple_bond_energy(m.z, m.pos);
     corr += si_o_h_energy(m.z, m.pos);
   }
@@ -3384,11 +3449,11 @@ int oracle_run(const char* data_dir, int n_atoms, const int* z, const double* xy
   }
   double net = 0.0;
   for (double v : q) net += v;
-  V3 origin = {0, 0, 0};
-  if (std::fabs(net) > 0.5) {
+  V3 origin = {0, 0, 0};  // dipole_origin (src/dipole.rs:137-177)
+  if (std::fabs(net) > 0.5 && o.dipole_origin != 1) {
     double tot = 0.0;
     for (int i = 0; i < n_atoms; ++i) {
-      double mass = (z[i] >= 0 && z[i] <= 86) ? MASS[z[i]] : 0.0;
+      double mass = o.dipole_origin == 2 ? m.el[i]->core_charge : ((z[i] >= 0 && z[i] <= 86) ? MASS[z[i]] : 0.0);
       origin.x += m.pos[i].x * mass;
       origin.y += m.pos[i].y * mass;
       origin.z += m.pos[i].z * mass;

@@ -15,7 +15,7 @@ extern "C" {
 #endif
 
 typedef struct {
-  int method;          /* 0 = PM7, 1 = PM7-TS, 3 = PM7-minus (no post-SCF corrections) */
+  int method;          /* 0 = PM7, 1 = PM7-TS, 3 = PM7-minus (no post-SCF corrections), 4 = PM7-HH */
   int multiplicity;    /* >= 1 */
   int max_scf;         /* default 200 */
   int use_diis;        /* default 1 */
@@ -27,6 +27,12 @@ typedef struct {
   double p_tol;        /* default 1e-7 */
   double adiis_switch; /* default 0.1 */
   double level_shift;  /* default 0 eV */
+  int force_dpath;     /* every pair through the MNDO/d integrals (src/scf.rs:97) */
+  int smooth_dispersion; /* continuous carbon C6 (src/dispersion.rs:72-175) */
+  int dipole_origin;   /* 0 centre of mass, 1 coordinates, 2 centre of core charge (src/dipole.rs:71-96) */
+  int pad_;
+  const double* initial_density;      /* nao*nao or NULL (src/scf.rs:1201-1210, 1341-1360) */
+  const double* initial_spin_density; /* nao*nao or NULL, UHF only (src/scf.rs:1361-1378) */
 } oracle_options;
 
 typedef struct {
@@ -75,6 +81,8 @@ int oracle_pair_block(const char* data_dir, int za, int zb, const double* d_bohr
 double oracle_core_core(const char* data_dir, int n_atoms, const int* z, const double* xyz_bohr);
 /* Post-SCF geometric corrections in kcal/mol: [dispersion, hbond, c_triple, si_o_h]. */
 void oracle_corrections(int n_atoms, const int* z, const double* xyz_bohr, double* out4);
+/* [smooth dispersion, PM7-HH H-H repulsion] in kcal/mol. */
+void oracle_corrections2(int n_atoms, const int* z, const double* xyz_bohr, double* out2);
 /* Symmetric eigensolve (Householder + implicit QL), ascending, vectors as columns. */
 int oracle_eigh(int n, const double* a, double* w, double* v);
 /* Local-frame Slater overlaps [s111, s211, s121, s221, s222] (atom a has the higher n);

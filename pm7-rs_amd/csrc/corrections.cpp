@@ -287,10 +287,16 @@ double eh_plus(const std::vector<P3>& c, const std::vector<int>& z, const HB& b)
 }  // namespace
 
 namespace {
-// D2 dispersion energy; with g, also its gradient (kcal/mol/Bohr, accumulated into g[3n]) at the
-// same bond topology: the coordination counts that pick the carbon C6 are step functions of the
-// distances and contribute nothing, so each pair's term differentiates through its distance only.
-double dispersion_impl(const std::vector<int>& z, const std::vector<double>& xyz, double* g) {
+// D2 dispersion energy (src/dispersion.rs:210-283); with g, also its gradient (kcal/mol/Bohr,
+// accumulated into g[3n]).
+//  * Discrete path (MOPAC's): a carbon with four covalent neighbours takes C6 = 0.95, any other
+//    1.65. The counts are step functions of the distances and contribute nothing to the gradient,
+//    so each pair's term differentiates through its distance only (at fixed bond topology).
+//  * Smooth path (smooth_dispersion, src/dispersion.rs:72-175, 416-500): each neighbour counts
+//    w = 1 / (1 + exp(-26 (r_cut / r - 1))), and the carbon C6 is 1.65 - 0.70 s with
+//    s = 1 / (1 + exp(-40 (count - 3.5))); the gradient then has a second part, dE/dcount_i
+//    (from each pair's Slater-Kirkwood combination) times dcount_i/dR over the counting pairs.
+double dispersion_impl(const std::vector<int>& z, const std::vector<double>& xyz, double* g, bool smooth) {
   auto c6t = [](int q) {
     switch (q) {
       case 1: return 0.16; case 2: return 0.084; case 5: return 5.79; case 6: return 1.65; case 7: return 1.11;
@@ -315,15 +321,50 @@ double dispersion_impl(const std::vector<int>& z, const std::vector<double>& xyz
       case 53: return 6.95; case 54: return 7.25; default: return 0.0;
     }
   };
+  constexpr double COUNT_K = 26.0, SWITCH_K = 40.0;
   const int n = (int)z.size();
-  std::vector<int> nb(n, 0);
-  for (int i = 0; i < n; ++i)
-    for (int j = i + 1; j < n; ++j)
-      if (std::sqrt(d2(at(xyz, i), at(xyz, j))) * A0 < 1.3 * (covalent_radius(z[i]) + covalent_radius(z[j]))) {
-        nb[i]++;
-        nb[j]++;
+  auto rcut = [&](int i, int j) { return 1.3 * (covalent_radius(z[i]) + covalent_radius(z[j])) / A0; };  // Bohr
+  // Neighbour weight 1/(1 + exp(-k (r_cut/r - 1))) and its r-derivative (0 where exp overflows).
+  auto weight = [&](double r, double rc, double* dw) {
+    const double e = std::exp(-COUNT_K * (rc / r - 1.0));
+    if (!std::isfinite(e)) {
+      *dw = 0.0;
+      return 0.0;
+    }
+    const double w = 1.0 / (1.0 + e);
+    *dw = -w * (1.0 - w) * COUNT_K * rc / (r * r);
+    return w;
+  };
+  std::vector<double> c6a(n, 0.0), dc6(n, 0.0);
+  {
+    std::vector<double> cnt(n, 0.0);
+    for (int i = 0; i < n; ++i)
+      for (int j = i + 1; j < n; ++j) {
+        const double r = std::sqrt(d2(at(xyz, i), at(xyz, j)));
+        double w;
+        if (smooth) {
+          double dw;
+          w = weight(r, rcut(i, j), &dw);
+        } else {
+          w = (r * A0 < 1.3 * (covalent_radius(z[i]) + covalent_radius(z[j]))) ? 1.0 : 0.0;
+        }
+        cnt[i] += w;
+        cnt[j] += w;
       }
+    for (int i = 0; i < n; ++i) {
+      if (z[i] != 6) {
+        c6a[i] = c6t(z[i]);
+      } else if (!smooth) {
+        c6a[i] = cnt[i] == 4.0 ? 0.95 : 1.65;
+      } else {
+        const double sw = 1.0 / (1.0 + std::exp(-(cnt[i] - 3.5) * SWITCH_K));
+        c6a[i] = 1.65 + (0.95 - 1.65) * sw;
+        dc6[i] = (0.95 - 1.65) * sw * (1.0 - sw) * SWITCH_K;
+      }
+    }
+  }
   const double ALPHA = 15.450118, S = 1.226593, CSCALE = 2.286419;
+  std::vector<double> dedcn(smooth && g ? n : 0, 0.0);
   double sum = 0.0;
   for (int i = 0; i < n; ++i) {
     int zi = z[i];
@@ -332,14 +373,15 @@ double dispersion_impl(const std::vector<int>& z, const std::vector<double>& xyz
       int zj = z[j];
       if (c6t(zj) == 0.0 || r0t(zj) == 0.0 || nft(zj) == 0.0) continue;
       double ni = nft(zi), nj = nft(zj);
-      double ci = zi == 6 ? (nb[i] == 4 ? 0.95 : 1.65) : c6t(zi);
-      double cj = zj == 6 ? (nb[j] == 4 ? 0.95 : 1.65) : c6t(zj);
-      double c6 = 2.0 * std::cbrt(ci * ci * cj * cj * ni * nj) / (std::cbrt(ci * nj * nj) + std::cbrt(cj * ni * ni));
+      double ci = c6a[i], cj = c6a[j];
+      const double bi = std::cbrt(ci * nj * nj), bj = std::cbrt(cj * ni * ni);
+      double c6 = 2.0 * std::cbrt(ci * ci * cj * cj * ni * nj) / (bi + bj);
       double ri = r0t(zi), rj = r0t(zj);
       double r0 = (ri * ri * ri + rj * rj * rj) / (ri * ri + rj * rj) / 1000.0 * 2.0;
       double rnm = std::sqrt(d2(at(xyz, i), at(xyz, j))) * (A0 * 0.1);
       double denom = std::exp((rnm / (S * r0) - 1.0) * -ALPHA) + 1.0;
-      sum -= CSCALE * (c6 / (1000.0 * 4.184)) / (std::pow(rnm, 6) * denom);
+      const double e = -CSCALE * (c6 / (1000.0 * 4.184)) / (std::pow(rnm, 6) * denom);
+      sum += e;
       if (g) {
         // E = -K / (rnm^6 D), D = 1 + exp(-ALPHA (rnm / (S r0) - 1)):
         // dE/drnm = K (6 / rnm + D' / D) / (rnm^6 D), D' = -(ALPHA / (S r0)) (D - 1).
@@ -353,19 +395,88 @@ double dispersion_impl(const std::vector<int>& z, const std::vector<double>& xyz
           g[3 * i + c] += f * dx;
           g[3 * j + c] -= f * dx;
         }
+        if (smooth) {
+          // E is linear in the combined C6, so dE/dc6_i = E d ln C6 / dc6_i, with
+          // d ln C6 / dc6_i = (2/3 - b_i / (3 (b_i + b_j))) / c6_i (Slater-Kirkwood rule).
+          const double sb = bi + bj;
+          if (sb > 0.0 && ci > 0.0 && cj > 0.0) {
+            dedcn[i] += e * (2.0 / 3.0 - bi / (3.0 * sb)) / ci * dc6[i];
+            dedcn[j] += e * (2.0 / 3.0 - bj / (3.0 * sb)) / cj * dc6[j];
+          }
+        }
       }
     }
   }
+  if (g && smooth) {
+    for (int i = 0; i < n; ++i)
+      for (int j = i + 1; j < n; ++j) {
+        if (dedcn[i] == 0.0 && dedcn[j] == 0.0) continue;
+        const double r = std::sqrt(d2(at(xyz, i), at(xyz, j)));
+        double dw;
+        (void)weight(r, rcut(i, j), &dw);
+        if (dw == 0.0) continue;
+        const double f = (dedcn[i] + dedcn[j]) * dw / r;
+        for (int c = 0; c < 3; ++c) {
+          const double dx = xyz[3 * i + c] - xyz[3 * j + c];
+          g[3 * i + c] += f * dx;
+          g[3 * j + c] -= f * dx;
+        }
+      }
+  }
   return sum;
+}
+
+// MOPAC's H-H repulsion (src/hh_rep.rs:21-36) for r in Angstrom, kcal/mol, and dE/dr.
+double hh_poly(double r, double* d) {
+  if (r <= 1.0) {
+    *d = 0.0;
+    return 25.46293603147693;
+  }
+  if (r < 1.5) {
+    const double c5 = -2714.9523516034697, c4 = 17103.650110591705, c3 = -42511.85798221796,
+                 c2 = 52063.19679913834, c1 = -31430.65833597229, c0 = 7516.08469609514;
+    *d = (((5.0 * c5 * r + 4.0 * c4) * r + 3.0 * c3) * r + 2.0 * c2) * r + c1;
+    return ((((c5 * r + c4) * r + c3) * r + c2) * r + c1) * r + c0;
+  }
+  const double p = std::pow(r, 1.72905), e = 118.7326 * std::exp(-1.53965 * p);
+  *d = e * (-1.53965 * 1.72905 * p / r);
+  return e;
 }
 }  // namespace
 
-double dispersion_kcal(const std::vector<int>& z, const std::vector<double>& xyz) {
-  return dispersion_impl(z, xyz, nullptr);
+double dispersion_kcal(const std::vector<int>& z, const std::vector<double>& xyz, bool smooth) {
+  return dispersion_impl(z, xyz, nullptr, smooth);
 }
 
-void dispersion_grad_kcal(const std::vector<int>& z, const std::vector<double>& xyz, double* g) {
-  (void)dispersion_impl(z, xyz, g);
+void dispersion_grad_kcal(const std::vector<int>& z, const std::vector<double>& xyz, double* g, bool smooth) {
+  (void)dispersion_impl(z, xyz, g, smooth);
+}
+
+// PM7-HH: sum over H-H pairs of hh_poly(r) (src/hh_rep.rs:48-60).
+double hh_repulsion_kcal(const std::vector<int>& z, const std::vector<double>& xyz) {
+  double e = 0.0, d;
+  const int n = (int)z.size();
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j)
+      if (z[i] == 1 && z[j] == 1) e += hh_poly(std::sqrt(d2(at(xyz, i), at(xyz, j))) * A0, &d);
+  return e;
+}
+
+void hh_repulsion_grad_kcal(const std::vector<int>& z, const std::vector<double>& xyz, double* g) {
+  const int n = (int)z.size();
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      if (z[i] != 1 || z[j] != 1) continue;
+      const double r = std::sqrt(d2(at(xyz, i), at(xyz, j)));
+      double d;
+      (void)hh_poly(r * A0, &d);
+      const double f = d * A0 / r;  // dE/dr_bohr / r
+      for (int c = 0; c < 3; ++c) {
+        const double dx = xyz[3 * i + c] - xyz[3 * j + c];
+        g[3 * i + c] += f * dx;
+        g[3 * j + c] -= f * dx;
+      }
+    }
 }
 
 double hbond_kcal(const std::vector<int>& z, const std::vector<double>& xyz) {

@@ -243,17 +243,32 @@ __device__ int jacobi_core(int n2, TA* A, double* Vl, double* Vg, int vrows, dou
   return sweep;
 }
 
+// Relative width of an eigenvalue cluster: eigenvalues whose neighbouring gaps are all below
+// TIE_REL * max(1, |eps|) eV count as one degenerate set.
+constexpr double TIE_REL = 1.0e-10;
+
 // Eigenpairs of the symmetric ld x ld matrix W (pad rows/columns zero): ascending eigenvalues
 // eps[0..n) and eigenvectors as the columns of Cout. With Cprev (the previous SCF iteration's
 // eigenvectors) the solve is warm-started: Jacobi diagonalises Cprev^T W Cprev, which is nearly
 // diagonal once the SCF settles, and Cout = Cprev V. A and V sit in LDS when the variant allows.
 // TA, tol2 and max_sweeps are passed to jacobi_core (the products around it are FP64 either way).
+//
+// Degenerate sets. Inside a set of eigenvalues that differ by rounding only (a cluster, TIE_REL),
+// the order of the vectors is decided by Jacobi slot, ascending (tie_desc = 0) or descending
+// (tie_desc = 1), not by the last bits of the eigenvalues. It matters only where the occupied
+// count splits such a set: an open shell whose odd electrons of both spins fall into degenerate
+// orbitals (O2 2+ triplet: one alpha pi* and one beta pi electron). The UHF loop solves alpha with
+// ascending and beta with descending slot order, which puts the two into different members of
+// their sets; with both in the same member the SCF ends 0.99 eV higher on O2 2+. Only a cold
+// solve (no Cprev: the first SCF iteration) uses the descending order; a warm-started solve keeps
+// ascending slot order, i.e. the previous iteration's order, so a set cannot swap back and forth.
+// Eigenvalues inside a cluster may be out of ascending order by less than its width.
 template <int NT, int MAXN, bool A_LDS, bool V_LDS, typename TA>
 __device__ __attribute__((noinline)) int eigensolve(int n, int ld, const double* __restrict__ W, const double* __restrict__ Cprev,
                           double* __restrict__ Cout, double* __restrict__ eps, double* Ag, double* Vg,
                           double* __restrict__ T, double* s_a, double* s_v, double2* s_cs, double* s_t,
                           int* s_perm, int* s_pq, double* red, unsigned long long* tsub, double tol2,
-                          int max_sweeps) {
+                          int max_sweeps, int tie_desc) {
   const int n2 = ld;
   unsigned long long t0 = wall_clock64();
   TA* A = reinterpret_cast<TA*>(A_LDS ? s_a : Ag);
@@ -321,6 +336,42 @@ __device__ __attribute__((noinline)) int eigensolve(int n, int ld, const double*
     s_perm[rank] = i;
   }
   __syncthreads();
+  {
+    // Any neighbours closer than the cluster width? (s_pq is free until the next Jacobi round.)
+    int near = 0;
+    for (int k = 1 + threadIdx.x; k < n; k += NT) {
+      const double lo = (double)A[pidx(s_perm[k - 1], s_perm[k - 1])], hi = (double)A[pidx(s_perm[k], s_perm[k])];
+      near |= (hi - lo <= TIE_REL * fmax(1.0, fabs(hi))) ? 1 : 0;
+    }
+    if (threadIdx.x == 0) s_pq[0] = 0;
+    __syncthreads();
+    if (near) s_pq[0] = 1;  // every writer stores the same value
+    __syncthreads();
+    const bool desc = tie_desc && !Cprev;
+    if (s_pq[0] && threadIdx.x == 0) {
+      // Order each cluster by slot (insertion sort; clusters hold a few vectors).
+      int k = 0;
+      while (k < n) {
+        int e = k + 1;
+        while (e < n) {
+          const double lo = (double)A[pidx(s_perm[e - 1], s_perm[e - 1])], hi = (double)A[pidx(s_perm[e], s_perm[e])];
+          if (hi - lo > TIE_REL * fmax(1.0, fabs(hi))) break;
+          ++e;
+        }
+        for (int a = k + 1; a < e; ++a) {
+          const int v = s_perm[a];
+          int b = a;
+          while (b > k && (desc ? s_perm[b - 1] < v : s_perm[b - 1] > v)) {
+            s_perm[b] = s_perm[b - 1];
+            --b;
+          }
+          s_perm[b] = v;
+        }
+        k = e;
+      }
+    }
+    __syncthreads();
+  }
   for (int k = threadIdx.x; k < n; k += NT) eps[k] = (double)A[pidx(s_perm[k], s_perm[k])];
   if (Cprev) {
     // Cout = Cprev V(:, perm); pad rows and columns up to ld are written as zero.

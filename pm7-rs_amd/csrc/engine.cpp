@@ -14,6 +14,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <list>
 #include <memory>
 #include <atomic>
 #include <thread>
@@ -112,12 +113,16 @@ std::string default_data_dir() {
 // One validated molecule of a call.
 struct Job {
   int idx = 0;
-  pm7_options_v1 o{};
+  pm7_options_v2 o{};
   std::vector<int> z;
   std::vector<double> xyz;
   int nao = 0, ld = 0, n_pp = 0, na = 0, nb = 0, uhf = 0, variant = 0;
-  int dpath = 0;  // holds a d-bearing atom: every pair through the MNDO/d kernels
+  int dpath = 0;  // holds a d-bearing atom (or force_dpath): every pair through the MNDO/d kernels
   double sad_yy = 0.0;
+  int missing_z = 0;  // MISSING_ELEMENT: the atomic number without parameters
+  // Starting densities (initial_density / initial_spin_density), already split into the
+  // matrices the SCF starts from: RHF P (p0), UHF P_alpha (p0) and P_beta (p1); n x n row-major.
+  std::vector<double> p0, p1;
   int64_t pair0 = 0, mat0 = 0, pp0 = 0, vec0 = 0, gath0 = 0, w0 = 0;
   int atom0 = 0;
   double corr_kcal = 0.0;
@@ -197,7 +202,8 @@ struct pm7_ctx {
   std::string data_dir;
   std::string err;
   ParamSet params;
-  int method = -1;
+  int method = -1;  // pm7_method of the loaded parameters (what options.method must equal)
+  int tables = -1;  // which bundled tables are loaded (PM7 or PM7-TS)
   std::vector<ElemDev> elem_host;
   DevBuf d_elem, d_pairpar, d_gl, d_ocd;
   Lane lane;
@@ -225,7 +231,7 @@ pm7_status fail(pm7_ctx* ctx, pm7_status st, const std::string& msg) {
   return st;
 }
 
-void set_msg(pm7_result_v1* r, const std::string& m) { std::snprintf(r->message, sizeof r->message, "%s", m.c_str()); }
+void set_msg(pm7_result_v2* r, const std::string& m) { std::snprintf(r->message, sizeof r->message, "%s", m.c_str()); }
 
 #define HIPCHK(expr)                                                                         \
   do {                                                                                       \
@@ -243,8 +249,31 @@ void set_msg(pm7_result_v1* r, const std::string& m) { std::snprintf(r->message,
     }                                                                                        \
   } while (0)
 
+const char* method_name(int m) {
+  switch (m) {
+    case PM7_METHOD_PM7: return "pm7";
+    case PM7_METHOD_PM7_TS: return "pm7-ts";
+    case PM7_METHOD_PM7_SPARKLE: return "pm7-sparkle";
+    case PM7_METHOD_PM7_MINUS: return "pm7-minus";
+    case PM7_METHOD_PM7_HH: return "pm7-hh";
+    default: return "unknown";
+  }
+}
+
+// Methods this engine runs: the four pm7-rs methods with orbital atoms only (src/method.rs:9-16).
+pm7_status method_ok(int m, std::string& msg) {
+  if (m == PM7_METHOD_PM7 || m == PM7_METHOD_PM7_TS || m == PM7_METHOD_PM7_MINUS || m == PM7_METHOD_PM7_HH)
+    return PM7_OK;
+  if (m == PM7_METHOD_PM7_SPARKLE) {
+    msg = "PM7-Sparkle (lanthanide point-charge cores) is not offered by this engine";
+    return PM7_ERR_UNSUPPORTED;
+  }
+  msg = "unknown PM7 method code " + std::to_string(m);
+  return PM7_ERR_INVALID_INPUT;
+}
+
 // Input validation (pm7-rs validate_input src/scf.rs:937-992, run_pm7_once 530-627).
-pm7_status validate(pm7_ctx* ctx, const pm7_molecule_v1& m, const pm7_options_v1& o, Job& j, std::string& msg) {
+pm7_status validate(pm7_ctx* ctx, const pm7_molecule_v1& m, const pm7_options_v2& o, Job& j, std::string& msg) {
   if (m.n_atoms <= 0 || !m.z || !m.xyz_bohr) {
     msg = "a PM7 calculation needs at least one atom";
     return PM7_ERR_INVALID_INPUT;
@@ -258,8 +287,17 @@ pm7_status validate(pm7_ctx* ctx, const pm7_molecule_v1& m, const pm7_options_v1
   if (!(std::isfinite(o.adiis_switch) && o.adiis_switch >= 0))
     return msg = "adiis_switch must be finite and non-negative", PM7_ERR_INVALID_INPUT;
   if (o.multiplicity < 1) return msg = "multiplicity must be >= 1", PM7_ERR_INVALID_INPUT;
-  if ((o.method == PM7_METHOD_PM7_TS) != (ctx->method == PM7_METHOD_PM7_TS))
-    return msg = "options method does not match the loaded parameter method", PM7_ERR_INVALID_INPUT;
+  {
+    std::string m2;
+    const pm7_status ms = method_ok(o.method, m2);
+    if (ms != PM7_OK) return msg = m2, ms;
+  }
+  if (o.method != ctx->method)
+    return msg = std::string("options method `") + method_name(o.method) + "` does not match parameter method `" +
+                 method_name(ctx->method) + "`",
+           PM7_ERR_INVALID_INPUT;
+  if (o.dipole_origin < PM7_DIPOLE_CENTRE_OF_MASS || o.dipole_origin > PM7_DIPOLE_CENTRE_OF_CHARGE)
+    return msg = "unknown dipole origin " + std::to_string(o.dipole_origin), PM7_ERR_INVALID_INPUT;
   const int n = m.n_atoms;
   j.z.assign(m.z, m.z + n);
   j.xyz.assign(m.xyz_bohr, m.xyz_bohr + 3 * n);
@@ -300,6 +338,7 @@ pm7_status validate(pm7_ctx* ctx, const pm7_molecule_v1& m, const pm7_options_v1
     int z = j.z[i];
     if (z <= 0 || z >= MAX_Z || !ctx->params.el[z].present) {
       msg = "element " + std::to_string(z) + " has no PM7 parameters";
+      j.missing_z = z;
       return PM7_ERR_MISSING_ELEMENT;
     }
     const HostElement& e = ctx->params.el[z];
@@ -321,11 +360,50 @@ pm7_status validate(pm7_ctx* ctx, const pm7_molecule_v1& m, const pm7_options_v1
   }
   j.na = (int)((ne + unp) / 2);
   j.nb = (int)((ne - unp) / 2);
+  if (j.na > nao) {
+    // More electrons of one spin than orbitals (e.g. a large negative charge): the reference's
+    // occupied-column slice would fail; no density can hold them.
+    msg = std::to_string(j.na) + " electrons of one spin do not fit in " + std::to_string(nao) + " orbitals";
+    return PM7_ERR_INVALID_INPUT;
+  }
   if (o.reference == PM7_REF_RHF && j.na != j.nb) {
     msg = "restricted (RHF) needs a closed shell; use UHF";
     return PM7_ERR_INVALID_INPUT;
   }
   j.uhf = o.reference == PM7_REF_AUTO ? (j.na != j.nb) : (o.reference == PM7_REF_UHF);
+  if (o.force_dpath) j.dpath = 1;
+  // Seeded densities (src/scf.rs:1201-1210 RHF, 1341-1378 UHF): a wrong shape is refused.
+  if (o.initial_density || o.initial_spin_density) {
+    if (o.initial_density_nao != nao) {
+      msg = "the supplied initial density is " + std::to_string(o.initial_density_nao) + "x" +
+            std::to_string(o.initial_density_nao) + " and this system's basis is " + std::to_string(nao) + "x" +
+            std::to_string(nao);
+      return PM7_ERR_INVALID_INPUT;
+    }
+    if (o.initial_spin_density && !o.initial_density) {
+      msg = "an initial spin density needs an initial density";
+      return PM7_ERR_INVALID_INPUT;
+    }
+    const size_t nn = (size_t)nao * nao;
+    for (size_t k = 0; k < nn; ++k)
+      if (!std::isfinite(o.initial_density[k]) || (o.initial_spin_density && !std::isfinite(o.initial_spin_density[k]))) {
+        msg = "the supplied initial density has non-finite entries";
+        return PM7_ERR_INVALID_INPUT;
+      }
+    if (!j.uhf) {
+      j.p0.assign(o.initial_density, o.initial_density + nn);
+    } else {
+      const double ntot = (double)std::max(j.na + j.nb, 1);
+      const double fa = j.na / ntot, fb = j.nb / ntot;
+      j.p0.resize(nn);
+      j.p1.resize(nn);
+      for (size_t k = 0; k < nn; ++k) {
+        const double sd = o.initial_spin_density ? 0.5 * o.initial_spin_density[k] : 0.0;
+        j.p0[k] = o.initial_density[k] * fa + sd;
+        j.p1[k] = o.initial_density[k] * fb - sd;
+      }
+    }
+  }
   j.nao = nao;
   j.ld = nao + (nao & 1);
   j.n_pp = npp;
@@ -341,11 +419,16 @@ pm7_status validate(pm7_ctx* ctx, const pm7_molecule_v1& m, const pm7_options_v1
     return PM7_ERR_UNSUPPORTED;
   }
   j.o = o;
+  // The seeded densities were copied into the job (a queued request outlives the caller's arrays).
+  j.o.initial_density = nullptr;
+  j.o.initial_spin_density = nullptr;
   return PM7_OK;
 }
 
 struct Prepared {
   std::vector<Job> jobs;
+  std::list<std::vector<double>> staging;  // host copies of seeded densities (stable addresses)
+  double need_mb = 0.0, budget_mb = 0.0;   // INSUFFICIENT_MEMORY payload
   std::vector<MolDev> mols;
   int64_t n_pairs = 0, w_words = 0, slab_words = 0, vec_words = 0, gath_words = 0, n_dpairs = 0;
   int n_items = 0;
@@ -485,6 +568,7 @@ pm7_status prepare(pm7_ctx* ctx, Lane& L, Prepared& P, int64_t max_memory_mb) {
     M.dpath = j.dpath;
     M.rotate_below = ctx->rotate_below;
     M.eig_tol_exp = ctx->eig_tol_exp;
+    M.init_p = j.p0.empty() ? 0 : 1;
     lists[j.variant].push_back(mi);
     atom0 += n;
     pair0 += (int64_t)n * (n - 1) / 2;
@@ -513,6 +597,8 @@ pm7_status prepare(pm7_ctx* ctx, Lane& L, Prepared& P, int64_t max_memory_mb) {
   if (need > budget) {
     L.err = "needs " + std::to_string(need >> 20) + " MiB of device memory, budget " + std::to_string(budget >> 20) +
             " MiB (integrals + SCF working set)";
+    P.need_mb = (double)need / (1024.0 * 1024.0);
+    P.budget_mb = (double)budget / (1024.0 * 1024.0);
     return PM7_ERR_INSUFFICIENT_MEMORY;
   }
   auto up = [&](DevBuf& d, const void* src, size_t bytes) -> hipError_t {
@@ -542,6 +628,21 @@ pm7_status prepare(pm7_ctx* ctx, Lane& L, Prepared& P, int64_t max_memory_mb) {
   LANECHK(up(L.d_dpairs, d_pairs.data(), d_pairs.size() * sizeof(int64_t)));
   // Zero the slab: pad rows/columns of every matrix must be exact zeros.
   LANECHK(hipMemsetAsync(L.d_slab.p, 0, (size_t)P.slab_words * 8, L.st[0]));
+  // Seeded starting densities into the first density slot(s) (RHF: S_HD; UHF: U_PA, U_PB), rows
+  // padded to the slab's stride. The staging copies live in P until the call ends.
+  for (const Job& j : P.jobs) {
+    if (j.p0.empty()) continue;
+    const int nsrc = j.uhf ? 2 : 1;
+    const int slots[2] = {j.uhf ? (int)U_PA : (int)S_HD, (int)U_PB};
+    for (int k = 0; k < nsrc; ++k) {
+      const std::vector<double>& src = k == 0 ? j.p0 : j.p1;
+      P.staging.emplace_back((size_t)j.ld * j.ld, 0.0);
+      std::vector<double>& pad = P.staging.back();
+      for (int r = 0; r < j.nao; ++r) std::copy(src.begin() + (size_t)r * j.nao, src.begin() + (size_t)(r + 1) * j.nao, pad.begin() + (size_t)r * j.ld);
+      double* dst = L.d_slab.as<double>() + j.mat0 + (int64_t)slots[k] * j.ld * j.ld;
+      LANECHK(hipMemcpyAsync(dst, pad.data(), pad.size() * 8, hipMemcpyHostToDevice, L.st[0]));
+    }
+  }
   BatchDev& b = P.b;
   b.mols = L.d_mols.as<MolDev>();
   b.mol_list = L.d_list.as<int>();
@@ -576,7 +677,7 @@ pm7_status prepare(pm7_ctx* ctx, Lane& L, Prepared& P, int64_t max_memory_mb) {
 }
 
 void finish_result(pm7_ctx* ctx, const Job& j, const MolOut& R, double e_core, const double* eps_ab,
-                   const double* gath, pm7_result_v1* out) {
+                   const double* gath, pm7_result_v2* out) {
   const int n = j.nao, nat = (int)j.z.size();
   std::memset(out, 0, sizeof *out);
   out->nao = n;
@@ -587,6 +688,8 @@ void finish_result(pm7_ctx* ctx, const Job& j, const MolOut& R, double e_core, c
   out->n_occ = j.na;
   out->n_occ_beta = j.uhf ? j.nb : -1;
   out->fp32_iterations = R.fp32_iters;
+  out->err_vs_fp64_ev = R.check_de;
+  out->err_vs_fp64_density = R.check_dp;
   out->jacobi_sweeps = R.sweeps;
   out->electronic_ev = R.e_el;
   out->core_ev = e_core;
@@ -644,11 +747,15 @@ void finish_result(pm7_ctx* ctx, const Job& j, const MolOut& R, double e_core, c
     net += out->charges[i];
     ao += e.n_orb;
   }
+  // Origin of the point-charge part (dipole_origin, src/dipole.rs:137-177): the coordinate origin
+  // for a neutral molecule (|net| <= 0.5) whatever the option, else the centre of mass (default),
+  // the coordinate origin or the centre of core charge.
   double org[3] = {0, 0, 0};
-  if (std::fabs(net) > 0.5) {
+  if (std::fabs(net) > 0.5 && j.o.dipole_origin != PM7_DIPOLE_COORDINATES) {
     double tot = 0.0;
     for (int i = 0; i < nat; ++i) {
-      double m = j.z[i] <= 86 ? MASS[j.z[i]] : 0.0;
+      const double m = j.o.dipole_origin == PM7_DIPOLE_CENTRE_OF_CHARGE ? ctx->params.el[j.z[i]].core_charge
+                                                                         : (j.z[i] <= 86 ? MASS[j.z[i]] : 0.0);
       for (int k = 0; k < 3; ++k) org[k] += j.xyz[3 * i + k] * m;
       tot += m;
     }
@@ -737,10 +844,15 @@ void gauss_legendre_table(std::vector<double>& out) {
   }
 }
 
+// PM7, PM7- and PM7-HH share the PM7 tables; PM7-TS has its own (src/params.rs:158-170).
 pm7_status ensure_params(pm7_ctx* ctx, int method) {
-  if (ctx->method == method) return PM7_OK;
+  const int tables = method == PM7_METHOD_PM7_TS ? PM7_METHOD_PM7_TS : PM7_METHOD_PM7;
+  if (ctx->tables == tables) {
+    ctx->method = method;
+    return PM7_OK;
+  }
   std::string err;
-  if (!load_params(ctx->data_dir, method, ctx->params, err)) return fail(ctx, PM7_ERR_MISSING_PARAMETER, err);
+  if (!load_params(ctx->data_dir, tables, ctx->params, err)) return fail(ctx, PM7_ERR_MISSING_PARAMETER, err);
   to_device_table(ctx->params, ctx->elem_host);
   HIPCHK(ctx->d_elem.ensure(ctx->elem_host.size() * sizeof(ElemDev)));
   HIPCHK(hipMemcpy(ctx->d_elem.p, ctx->elem_host.data(), ctx->elem_host.size() * sizeof(ElemDev), hipMemcpyHostToDevice));
@@ -762,6 +874,7 @@ pm7_status ensure_params(pm7_ctx* ctx, int method) {
     HIPCHK(ctx->d_gl.ensure(gl.size() * sizeof(double)));
     HIPCHK(hipMemcpy(ctx->d_gl.p, gl.data(), gl.size() * sizeof(double), hipMemcpyHostToDevice));
   }
+  ctx->tables = tables;
   ctx->method = method;
   return PM7_OK;
 }
@@ -772,9 +885,10 @@ void host_corrections(std::vector<Job>& jobs) {
     for (size_t k = t; k < jobs.size(); k += nt) {
       Job& j = jobs[k];
       j.corr_kcal = 0.0;
-      if (j.o.method != PM7_METHOD_PM7_MINUS) {
-        j.corr_kcal += dispersion_kcal(j.z, j.xyz);
+      if (j.o.method != PM7_METHOD_PM7_MINUS) {  // correction_energy, src/scf.rs:907-932
+        j.corr_kcal += dispersion_kcal(j.z, j.xyz, j.o.smooth_dispersion != 0);
         j.corr_kcal += hbond_kcal(j.z, j.xyz);
+        if (j.o.method == PM7_METHOD_PM7_HH) j.corr_kcal += hh_repulsion_kcal(j.z, j.xyz);
         j.corr_kcal += c_triple_kcal(j.z, j.xyz);
         j.corr_kcal += si_o_h_kcal(j.z, j.xyz);
       }
@@ -894,7 +1008,7 @@ pm7_status finish_batch(Lane& L, const Prepared& P) {
 }
 
 // Copy a finished batch's outputs to the host and fill results[j.idx] for every job.
-pm7_status collect_batch(pm7_ctx* ctx, Lane& L, const Prepared& P, pm7_result_v1* results) {
+pm7_status collect_batch(pm7_ctx* ctx, Lane& L, const Prepared& P, pm7_result_v2* results) {
   const int nm = (int)P.jobs.size();
   // One pinned staging area: MolOut records, core energies, eigenvalues, packed matrices.
   const size_t o_out = 0;
@@ -944,7 +1058,8 @@ struct GradReq {
 // analytic derivative of the selected terms would be); otherwise the central difference (at a
 // kink, the mean of the two one-sided slopes).
 void corrections_grad_kcal(const Job& j, double* g) {
-  dispersion_grad_kcal(j.z, j.xyz, g);
+  dispersion_grad_kcal(j.z, j.xyz, g, j.o.smooth_dispersion != 0);
+  if (j.o.method == PM7_METHOD_PM7_HH) hh_repulsion_grad_kcal(j.z, j.xyz, g);
   using Term = double (*)(const std::vector<int>&, const std::vector<double>&);
   const Term terms[3] = {hbond_kcal, c_triple_kcal, si_o_h_kcal};
   std::vector<double> x = j.xyz;
@@ -991,7 +1106,7 @@ pm7_status launch_gradient_batch(Lane& L, const Prepared& P, int* n_items_out) {
   return PM7_OK;
 }
 
-pm7_status collect_gradient(Lane& L, const Prepared& P, const GradReq& g, pm7_result_v1* results) {
+pm7_status collect_gradient(Lane& L, const Prepared& P, const GradReq& g, pm7_result_v2* results) {
   int n_atoms = 0;
   for (const Job& j : P.jobs) n_atoms = std::max(n_atoms, j.atom0 + (int)j.z.size());
   std::vector<double> ga((size_t)n_atoms * 3);
@@ -1009,7 +1124,7 @@ pm7_status collect_gradient(Lane& L, const Prepared& P, const GradReq& g, pm7_re
         out[k] = ga[(size_t)3 * j.atom0 + k];
         ok = ok && std::isfinite(out[k]);
       }
-      pm7_result_v1& r = results[j.idx];
+      pm7_result_v2& r = results[j.idx];
       if (!ok) {
         for (int k = 0; k < 3 * na; ++k) out[k] = std::nan("");
         if (r.status == PM7_OK || r.status == PM7_ERR_SCF_NOT_CONVERGED) {
@@ -1032,7 +1147,7 @@ pm7_status collect_gradient(Lane& L, const Prepared& P, const GradReq& g, pm7_re
   return PM7_OK;
 }
 
-pm7_status run_jobs(pm7_ctx* ctx, std::vector<Job>& jobs, pm7_result_v1* results, int64_t max_memory_mb,
+pm7_status run_jobs(pm7_ctx* ctx, std::vector<Job>& jobs, pm7_result_v2* results, int64_t max_memory_mb,
                     const GradReq* grad = nullptr) {
   Lane& L = ctx->lane;
   auto t0 = std::chrono::steady_clock::now();
@@ -1044,6 +1159,8 @@ pm7_status run_jobs(pm7_ctx* ctx, std::vector<Job>& jobs, pm7_result_v1* results
     ctx->err = L.err;
     for (auto& j : P.jobs) {
       results[j.idx].status = st;
+      results[j.idx].needed_mb = P.need_mb;
+      results[j.idx].budget_mb = P.budget_mb;
       set_msg(&results[j.idx], L.err);
     }
     return st;
@@ -1087,7 +1204,7 @@ struct SrvReq {
   Job job;
   bool valid = false;
   int state = 0;  // 0 queued, 1 on the device, 2 finished, 3 claimed
-  pm7_result_v1 res{};
+  pm7_result_v2 res{};
   std::chrono::steady_clock::time_point t_submit, t_done;
 };
 
@@ -1104,7 +1221,7 @@ struct Flight {
   std::thread thr;
   std::atomic<int> done{0};  // 1: results (or the error) are ready for the scheduler
   pm7_status st = PM7_OK;
-  std::vector<pm7_result_v1> res;
+  std::vector<pm7_result_v2> res;
 };
 
 }  // namespace
@@ -1230,7 +1347,7 @@ void srv_loop(pm7_server* s) {
         if (st == PM7_OK) st = run_large(g.lane, g.P);
         if (st == PM7_OK) st = finish_batch(g.lane, g.P);
         if (st == PM7_OK) host_corrections(g.P.jobs);
-        g.res.assign(g.P.jobs.size(), pm7_result_v1{});
+        g.res.assign(g.P.jobs.size(), pm7_result_v2{});
         for (auto& r : g.res) std::memset(&r, 0, sizeof r);
         if (st == PM7_OK)
           st = hipEventSynchronize(g.lane.ev[5]) == hipSuccess ? collect_batch(ctx, g.lane, g.P, g.res.data())
@@ -1291,9 +1408,9 @@ pm7_status pm7_server_start(pm7_ctx* ctx, const pm7_server_config_v1* cfg, pm7_s
   return PM7_OK;
 }
 
-pm7_status pm7_server_submit(pm7_server* s, const pm7_molecule_v1* mol, const pm7_options_v1* opts, int64_t* request_id) {
+pm7_status pm7_server_submit(pm7_server* s, const pm7_molecule_v1* mol, const pm7_options_v2* opts, int64_t* request_id) {
   if (!s || !mol || !request_id) return PM7_ERR_INVALID_INPUT;
-  pm7_options_v1 def;
+  pm7_options_v2 def;
   pm7_default_options(&def);
   SrvReq r;
   r.t_submit = std::chrono::steady_clock::now();
@@ -1307,10 +1424,13 @@ pm7_status pm7_server_submit(pm7_server* s, const pm7_molecule_v1* mol, const pm
     st = PM7_ERR_INSUFFICIENT_MEMORY;
     msg = "request needs " + std::to_string(job_bytes(s->ctx, r.job) >> 20) + " MiB, server budget " +
           std::to_string(s->budget >> 20) + " MiB";
+    r.res.needed_mb = (double)job_bytes(s->ctx, r.job) / (1024.0 * 1024.0);
+    r.res.budget_mb = (double)s->budget / (1024.0 * 1024.0);
   }
   if (st != PM7_OK) {
     r.res.status = st;
     r.res.n_atoms = mol->n_atoms;
+    r.res.missing_element = r.job.missing_z;
     set_msg(&r.res, msg);
     s->reqs.emplace(id, std::move(r));
     srv_finish(s, id, std::chrono::steady_clock::now());
@@ -1325,7 +1445,7 @@ pm7_status pm7_server_submit(pm7_server* s, const pm7_molecule_v1* mol, const pm
 }
 
 pm7_status pm7_server_poll(pm7_server* s, int64_t request_id, int32_t timeout_ms, int64_t* done_id,
-                           pm7_result_v1* out, double* latency_ms) {
+                           pm7_result_v2* out, double* latency_ms) {
   if (!s || !out) return PM7_ERR_INVALID_INPUT;
   std::unique_lock<std::mutex> lk(s->mu);
   if (request_id >= 0) {
@@ -1377,12 +1497,18 @@ void pm7_server_stop(pm7_server* s) {
 
 int32_t pm7_abi_version(void) { return PM7_ABI_VERSION; }
 
-void pm7_default_options(pm7_options_v1* o) {
+void pm7_default_options(pm7_options_v2* o) {
   std::memset(o, 0, sizeof *o);
   o->method = PM7_METHOD_PM7;
   o->charge = 0.0;
   o->multiplicity = 1;
   o->max_scf = 200;
+  o->force_dpath = 0;
+  o->smooth_dispersion = 0;
+  o->dipole_origin = PM7_DIPOLE_CENTRE_OF_MASS;
+  o->initial_density_nao = 0;
+  o->initial_density = nullptr;
+  o->initial_spin_density = nullptr;
   o->e_tol = 1.0e-8;
   o->p_tol = 1.0e-7;
   o->use_diis = 1;
@@ -1448,26 +1574,29 @@ pm7_status pm7_params_load(pm7_ctx* ctx, int32_t method) {
   if (!ctx) return PM7_ERR_INVALID_INPUT;
   std::lock_guard<std::mutex> lk(ctx->mu);
   (void)hipSetDevice(ctx->device);
-  return ensure_params(ctx, method == PM7_METHOD_PM7_TS ? PM7_METHOD_PM7_TS : PM7_METHOD_PM7);
+  std::string msg;
+  const pm7_status st = method_ok(method, msg);
+  if (st != PM7_OK) return fail(ctx, st, msg);
+  return ensure_params(ctx, method);
 }
 
 namespace {
-pm7_status run_batch_impl(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols, const pm7_options_v1* opts,
-                          int32_t n_opts, pm7_result_v1* results, double* gradient);
+pm7_status run_batch_impl(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols, const pm7_options_v2* opts,
+                          int32_t n_opts, pm7_result_v2* results, double* gradient);
 }  // namespace
 
-pm7_status pm7_run_batch(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols, const pm7_options_v1* opts,
-                         int32_t n_opts, pm7_result_v1* results) {
+pm7_status pm7_run_batch(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols, const pm7_options_v2* opts,
+                         int32_t n_opts, pm7_result_v2* results) {
   return run_batch_impl(ctx, n, mols, opts, n_opts, results, nullptr);
 }
 
-pm7_status pm7_gradient_batch(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols, const pm7_options_v1* opts,
-                              int32_t n_opts, pm7_result_v1* results, double* gradient) {
+pm7_status pm7_gradient_batch(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols, const pm7_options_v2* opts,
+                              int32_t n_opts, pm7_result_v2* results, double* gradient) {
   if (n > 0 && !gradient) return PM7_ERR_INVALID_INPUT;
   return run_batch_impl(ctx, n, mols, opts, n_opts, results, gradient);
 }
 
-pm7_status pm7_gradient(pm7_ctx* ctx, const pm7_molecule_v1* mol, const pm7_options_v1* opts, pm7_result_v1* out,
+pm7_status pm7_gradient(pm7_ctx* ctx, const pm7_molecule_v1* mol, const pm7_options_v2* opts, pm7_result_v2* out,
                         double* gradient) {
   if (!ctx || !mol || !out || !gradient) return PM7_ERR_INVALID_INPUT;
   pm7_status st = pm7_gradient_batch(ctx, 1, mol, opts, opts ? 1 : 0, out, gradient);
@@ -1476,18 +1605,18 @@ pm7_status pm7_gradient(pm7_ctx* ctx, const pm7_molecule_v1* mol, const pm7_opti
 }
 
 namespace {
-pm7_status run_batch_impl(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols, const pm7_options_v1* opts,
-                          int32_t n_opts, pm7_result_v1* results, double* gradient) {
+pm7_status run_batch_impl(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols, const pm7_options_v2* opts,
+                          int32_t n_opts, pm7_result_v2* results, double* gradient) {
   if (!ctx || n < 0 || (n > 0 && (!mols || !results))) return PM7_ERR_INVALID_INPUT;
   std::lock_guard<std::mutex> lk(ctx->mu);
   (void)hipSetDevice(ctx->device);
-  pm7_options_v1 def;
+  pm7_options_v2 def;
   pm7_default_options(&def);
   std::vector<Job> jobs;
   int64_t budget = 0;
   for (int i = 0; i < n; ++i) {
     std::memset(&results[i], 0, sizeof results[i]);
-    const pm7_options_v1& o = (opts && n_opts > 0) ? opts[n_opts == 1 ? 0 : i] : def;
+    const pm7_options_v2& o = (opts && n_opts > 0) ? opts[n_opts == 1 ? 0 : i] : def;
     Job j;
     j.idx = i;
     std::string msg;
@@ -1495,6 +1624,7 @@ pm7_status run_batch_impl(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols, 
     if (st != PM7_OK) {
       results[i].status = st;
       results[i].n_atoms = mols[i].n_atoms;
+      results[i].missing_element = j.missing_z;
       set_msg(&results[i], msg);
       continue;
     }
@@ -1519,14 +1649,14 @@ pm7_status run_batch_impl(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols, 
 }
 }  // namespace
 
-pm7_status pm7_run(pm7_ctx* ctx, const pm7_molecule_v1* mol, const pm7_options_v1* opts, pm7_result_v1* out) {
+pm7_status pm7_run(pm7_ctx* ctx, const pm7_molecule_v1* mol, const pm7_options_v2* opts, pm7_result_v2* out) {
   if (!ctx || !mol || !out) return PM7_ERR_INVALID_INPUT;
   pm7_status st = pm7_run_batch(ctx, 1, mol, opts, opts ? 1 : 0, out);
   if (st != PM7_OK) return st;
   return (pm7_status)out->status;
 }
 
-void pm7_result_free(pm7_result_v1* r) {
+void pm7_result_free(pm7_result_v2* r) {
   if (!r) return;
   std::free(r->density);  // the base of the result's single allocation
   double** arrays[] = {&r->density, &r->spin_density, &r->mo_energies, &r->mo_coeff,
@@ -1538,7 +1668,7 @@ pm7_status pm7_core_hamiltonian(pm7_ctx* ctx, const pm7_molecule_v1* mol, double
   if (!ctx || !mol) return PM7_ERR_INVALID_INPUT;
   std::lock_guard<std::mutex> lk(ctx->mu);
   (void)hipSetDevice(ctx->device);
-  pm7_options_v1 o;
+  pm7_options_v2 o;
   pm7_default_options(&o);
   o.method = ctx->method;
   o.reference = PM7_REF_UHF;  // any electron count is fine for the core Hamiltonian
@@ -1572,7 +1702,7 @@ pm7_status pm7_fock_spin(pm7_ctx* ctx, const pm7_molecule_v1* mol, const double*
   pm7_status st = pm7_core_hamiltonian(ctx, mol, nullptr, &nao);
   if (st != PM7_OK) return st;
   std::lock_guard<std::mutex> lk(ctx->mu);
-  pm7_options_v1 o;
+  pm7_options_v2 o;
   pm7_default_options(&o);
   o.method = ctx->method;
   o.reference = PM7_REF_UHF;
@@ -1617,7 +1747,7 @@ pm7_status response_fock_impl(pm7_ctx* ctx, const pm7_molecule_v1* mol, int32_t 
   if (st != PM7_OK) return st;
   if (n_dens == 0) return PM7_OK;
   std::lock_guard<std::mutex> lk(ctx->mu);
-  pm7_options_v1 o;
+  pm7_options_v2 o;
   pm7_default_options(&o);
   o.method = ctx->method;
   o.reference = PM7_REF_UHF;
@@ -1696,7 +1826,7 @@ pm7_status pm7_core_core_energy(pm7_ctx* ctx, const pm7_molecule_v1* mol, double
   if (!ctx || !mol || !e_out) return PM7_ERR_INVALID_INPUT;
   std::lock_guard<std::mutex> lk(ctx->mu);
   (void)hipSetDevice(ctx->device);
-  pm7_options_v1 o;
+  pm7_options_v2 o;
   pm7_default_options(&o);
   o.method = ctx->method;
   o.reference = PM7_REF_UHF;

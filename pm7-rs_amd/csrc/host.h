@@ -45,9 +45,13 @@ int shell_principal(int z, bool d_shell);
 void to_device_table(const ParamSet& P, std::vector<ElemDev>& out);
 
 // Geometric post-SCF corrections in kcal/mol (coordinates in Bohr).
-double dispersion_kcal(const std::vector<int>& z, const std::vector<double>& xyz);
+// D2 dispersion; smooth: the carbon C6 from a continuous neighbour count (smooth_dispersion).
+double dispersion_kcal(const std::vector<int>& z, const std::vector<double>& xyz, bool smooth = false);
 // Analytic gradient of dispersion_kcal, kcal/mol/Bohr, accumulated into g[3 n].
-void dispersion_grad_kcal(const std::vector<int>& z, const std::vector<double>& xyz, double* g);
+void dispersion_grad_kcal(const std::vector<int>& z, const std::vector<double>& xyz, double* g, bool smooth = false);
+// PM7-HH hydrogen-hydrogen repulsion and its analytic gradient (accumulated into g[3 n]).
+double hh_repulsion_kcal(const std::vector<int>& z, const std::vector<double>& xyz);
+void hh_repulsion_grad_kcal(const std::vector<int>& z, const std::vector<double>& xyz, double* g);
 double hbond_kcal(const std::vector<int>& z, const std::vector<double>& xyz);
 double c_triple_kcal(const std::vector<int>& z, const std::vector<double>& xyz);
 double si_o_h_kcal(const std::vector<int>& z, const std::vector<double>& xyz);

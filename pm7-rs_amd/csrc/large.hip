@@ -49,6 +49,7 @@ struct LargeCtl {
   double coef[HIST + 1];
   double G[HIST * HIST], M[HIST * HIST];
   double e_old, e_cur, err, dp, best_dp, e_low;
+  double e_conv, check_de, check_dp;  // mixed precision: the final FP64 Roothaan step's size
   double dots[NDOT];  // reductions land here
   Guard guard;
   double wk[(HIST + 1) * (HIST + 3)];
@@ -222,6 +223,38 @@ __global__ void __launch_bounds__(256) k_lg_diff(int n, int ld, const double* __
   }
 }
 
+// Per-block max |Pn - P| over n x n into part[block] (the check of a mixed run's last step).
+__global__ void __launch_bounds__(256) k_lg_maxdiff(int n, int ld, const double* __restrict__ Pn,
+                                                    const double* __restrict__ P, double* __restrict__ part) {
+  double m = 0.0;
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < nn; idx += (int64_t)gridDim.x * 256) {
+    const int64_t i = idx / n, j = idx - i * n, x = i * ld + j;
+    m = fmax(m, fabs(Pn[x] - P[x]));
+  }
+  __shared__ double red[4];
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+// what 0: remember the converged energy; 1: the step's energy change and (with nb blocks of
+// part) its max density change, folded into the control block.
+__global__ void k_lg_check(LargeCtl* __restrict__ ctl, int what, int nb, const double* __restrict__ part) {
+  if (threadIdx.x != 0) return;
+  if (what == 0) {
+    ctl->e_conv = ctl->e_cur;
+    ctl->check_de = 0.0;
+    ctl->check_dp = 0.0;
+    return;
+  }
+  double m = 0.0;
+  for (int k = 0; k < nb; ++k) m = fmax(m, part[k]);
+  ctl->check_dp = fmax(ctl->check_dp, m);
+  ctl->check_de = fabs(ctl->e_cur - ctl->e_conv);
+}
+
 __global__ void __launch_bounds__(256) k_lg_to_f32(int64_t count, const double* __restrict__ src, float* __restrict__ dst) {
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256) dst[i] = (float)src[i];
 }
@@ -360,7 +393,7 @@ __global__ void __launch_bounds__(64) k_lg_ctrl_a(LargeCtl* __restrict__ ctl, in
 // Control B (one thread): convergence, stall guard and the precision hand-over
 // (src/scf.rs:1274-1287; the megakernel's tail of an iteration).
 __global__ void k_lg_ctrl_b(LargeCtl* __restrict__ ctl, int iter, int64_t nn, double e_tol, double p_tol,
-                            double mixed_switch, int uhf) {
+                            double resid_tol, double mixed_switch, int uhf) {
   if (threadIdx.x != 0) return;
   const double e = ctl->e_cur;
   const double dp = uhf ? sqrt(ctl->dots[0] / (double)nn) + sqrt(ctl->dots[1] / (double)nn)
@@ -368,7 +401,7 @@ __global__ void k_lg_ctrl_b(LargeCtl* __restrict__ ctl, int iter, int64_t nn, do
   const double de = fabs(e - ctl->e_old);
   ctl->e_old = e;
   ctl->dp = dp;
-  const bool resid_ok = ctl->guard.shift == 0.0 && ctl->err < 1.0e-7;
+  const bool resid_ok = ctl->guard.shift == 0.0 && ctl->err < resid_tol;
   const bool conv = iter > 0 && de < e_tol && (dp < p_tol || resid_ok);
   if (ctl->low) {
     ctl->fp32_iters++;
@@ -625,6 +658,7 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
   double* d3 = static_cast<double*>(lw->dots.p);
   rocblas_int* info = static_cast<rocblas_int*>(lw->info.p);
   const bool mixed = M.precision == 1;
+  const double tail = mixed ? MIXED_TAIL : 1.0;  // a mixed run's FP64 tail converges tighter
   if (mixed) {
     LG(lw->f32a.ensure((size_t)sq * sizeof(float)));
     LG(lw->f32b.ensure((size_t)sq * sizeof(float)));
@@ -733,7 +767,7 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
   double dp_prev = 1.0e30;
   if (!M.uhf) {
     const int nocc = M.n_occ;
-    hipLaunchKernelGGL(k_lg_sad, dim3(grid_for(M.n_atoms)), dim3(256), 0, s, mi, b, slot(S_HD), 1.0);
+    if (!M.init_p) hipLaunchKernelGGL(k_lg_sad, dim3(grid_for(M.n_atoms)), dim3(256), 0, s, mi, b, slot(S_HD), 1.0);
     for (; iter < M.max_scf; ++iter) {
       const int h = iter % HIST;
       const int cnt = std::min(iter + 1, HIST);
@@ -786,7 +820,8 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
       LG(density(slot(cslot_c), nocc, 2.0, Pn, low != 0));
       hipLaunchKernelGGL(k_lg_diff, dim3(RB), dim3(256), 0, s, n, ld, Pn, P, part);
       hipLaunchKernelGGL(k_lg_finish, dim3(1), dim3(64), 0, s, RB, 1, part, ctl_dots);
-      hipLaunchKernelGGL(k_lg_ctrl_b, dim3(1), dim3(64), 0, s, ctl, iter, nn, M.e_tol, M.p_tol, M.mixed_switch, 0);
+      hipLaunchKernelGGL(k_lg_ctrl_b, dim3(1), dim3(64), 0, s, ctl, iter, nn, M.e_tol, M.p_tol * tail, 1.0e-7 * tail,
+                         M.mixed_switch, 0);
       LG(hipEventRecord(lw->ev[5], s));
       LG(hipMemcpyAsync(&host, ctl, sizeof host, hipMemcpyDeviceToHost, s));
       LG(hipStreamSynchronize(s));
@@ -811,10 +846,14 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
     LG(dots(2, P, slot(S_H), P, F, nullptr, nullptr, d3));
     hipLaunchKernelGGL(k_lg_energy, dim3(1), dim3(64), 0, s, ctl, d3, 0);
     int pslot = S_HD + hp;
-    if (host.guard.shift > 0.0 || last_rot) {
+    // A mixed run always ends with a full FP64 Roothaan step; its size is the stated error
+    // against FP64 (as in the megakernel).
+    const bool check = mixed && host.guard.shift == 0.0;
+    hipLaunchKernelGGL(k_lg_check, dim3(1), dim3(64), 0, s, ctl, 0, 0, part);
+    if (host.guard.shift > 0.0 || last_rot || check) {
       hipLaunchKernelGGL(k_lg_copy, dim3(grid_for(sq)), dim3(256), 0, s, sq, F, slot(S_W));
       LG(eig_full(slot(S_W), b.vecs + M.vec0, slot(cslot_c)));
-      if (last_rot && host.guard.shift == 0.0) {
+      if ((last_rot || check) && host.guard.shift == 0.0) {
         // The reported orbitals diagonalise the converged Fock; the reported density and energy
         // are theirs (one more Roothaan step from the converged density).
         pslot = S_HD + (hp + 1) % HIST;
@@ -825,6 +864,10 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
         hipLaunchKernelGGL(k_lg_fock_pairs<double>, dim3(grid_for((int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256), 0, s, mi, b, P2, 0.5, nullptr, 0.0, F, nullptr);
         LG(dots(2, P2, slot(S_H), P2, F, nullptr, nullptr, d3));
         hipLaunchKernelGGL(k_lg_energy, dim3(1), dim3(64), 0, s, ctl, d3, 0);
+        if (check) {
+          hipLaunchKernelGGL(k_lg_maxdiff, dim3(RB), dim3(256), 0, s, n, ld, P2, P, part);
+          hipLaunchKernelGGL(k_lg_check, dim3(1), dim3(64), 0, s, ctl, 1, RB, part);
+        }
       }
     }
     // C = Ct^T into the C1 slot for the output gather.
@@ -840,8 +883,10 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
     const double ntot = (double)std::max(na + nb, 1);
     double *Pa = slot(U_PA), *Pb = slot(U_PB), *Pan = slot(U_PAN), *Pbn = slot(U_PBN), *Pt = slot(U_PT);
     double *Fa = slot(U_FA), *Fb = slot(U_FB);
-    hipLaunchKernelGGL(k_lg_sad, dim3(grid_for(M.n_atoms)), dim3(256), 0, s, mi, b, Pa, na / ntot);
-    hipLaunchKernelGGL(k_lg_sad, dim3(grid_for(M.n_atoms)), dim3(256), 0, s, mi, b, Pb, nb / ntot);
+    if (!M.init_p) {  // else the caller's alpha and beta densities are there
+      hipLaunchKernelGGL(k_lg_sad, dim3(grid_for(M.n_atoms)), dim3(256), 0, s, mi, b, Pa, na / ntot);
+      hipLaunchKernelGGL(k_lg_sad, dim3(grid_for(M.n_atoms)), dim3(256), 0, s, mi, b, Pb, nb / ntot);
+    }
     double* eps_a = b.vecs + M.vec0;
     double* eps_b = eps_a + ld;
     for (; iter < M.max_scf; ++iter) {
@@ -889,7 +934,8 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
       hipLaunchKernelGGL(k_lg_finish, dim3(1), dim3(64), 0, s, RB, 1, part, ctl_dots);
       hipLaunchKernelGGL(k_lg_diff, dim3(RB), dim3(256), 0, s, n, ld, Pbn, Pb, part);
       hipLaunchKernelGGL(k_lg_finish, dim3(1), dim3(64), 0, s, RB, 1, part, ctl_dots + 1);
-      hipLaunchKernelGGL(k_lg_ctrl_b, dim3(1), dim3(64), 0, s, ctl, iter, nn, M.e_tol, M.p_tol, M.mixed_switch, 1);
+      hipLaunchKernelGGL(k_lg_ctrl_b, dim3(1), dim3(64), 0, s, ctl, iter, nn, M.e_tol, M.p_tol * tail, 1.0e-7 * tail,
+                         M.mixed_switch, 1);
       LG(hipEventRecord(lw->ev[5], s));
       LG(hipMemcpyAsync(&host, ctl, sizeof host, hipMemcpyDeviceToHost, s));
       LG(hipStreamSynchronize(s));
@@ -913,13 +959,21 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
         hipLaunchKernelGGL(k_lg_fock_pairs<double>, dim3(grid_for((int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256), 0, s, mi, b, Pa, 1.0, Pb, 1.0, Fa, Fb);
     LG(dots(3, Pt, slot(U_H), Pa, Fa, Pb, Fb, d3));
     hipLaunchKernelGGL(k_lg_energy, dim3(1), dim3(64), 0, s, ctl, d3, 1);
-    if (host.guard.shift > 0.0 || last_rot) LG(eig_full(Fa, eps_a, slot(U_CA0)));
+    const bool check = mixed && host.guard.shift == 0.0;
+    hipLaunchKernelGGL(k_lg_check, dim3(1), dim3(64), 0, s, ctl, 0, 0, part);
+    if (host.guard.shift > 0.0 || last_rot || check) LG(eig_full(Fa, eps_a, slot(U_CA0)));
     hipLaunchKernelGGL(k_lg_copy, dim3(grid_for(sq)), dim3(256), 0, s, sq, Fb, slot(U_T));
     LG(eig_full(slot(U_T), eps_b, slot(U_CB0)));
-    if (last_rot && host.guard.shift == 0.0) {
+    if ((last_rot || check) && host.guard.shift == 0.0) {
       // Densities and energy of the reported orbitals (see the RHF branch).
       LG(density(slot(U_CA0), na, 1.0, Pan, false));
       LG(density(slot(U_CB0), nb, 1.0, Pbn, false));
+      if (check) {
+        hipLaunchKernelGGL(k_lg_maxdiff, dim3(RB), dim3(256), 0, s, n, ld, Pan, Pa, part);
+        hipLaunchKernelGGL(k_lg_check, dim3(1), dim3(64), 0, s, ctl, 1, RB, part);
+        hipLaunchKernelGGL(k_lg_maxdiff, dim3(RB), dim3(256), 0, s, n, ld, Pbn, Pb, part);
+        hipLaunchKernelGGL(k_lg_check, dim3(1), dim3(64), 0, s, ctl, 1, RB, part);
+      }
       std::swap(Pa, Pan);
       std::swap(Pb, Pbn);
       hipLaunchKernelGGL(k_lg_copy, dim3(grid_for(sq)), dim3(256), 0, s, sq, Pa, Pt);
@@ -929,6 +983,7 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
       hipLaunchKernelGGL(k_lg_fock_pairs<double>, dim3(grid_for((int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256), 0, s, mi, b, Pa, 1.0, Pb, 1.0, Fa, Fb);
       LG(dots(3, Pt, slot(U_H), Pa, Fa, Pb, Fb, d3));
       hipLaunchKernelGGL(k_lg_energy, dim3(1), dim3(64), 0, s, ctl, d3, 1);
+      if (check) hipLaunchKernelGGL(k_lg_check, dim3(1), dim3(64), 0, s, ctl, 1, 0, part);
     }
     hipLaunchKernelGGL(k_lg_transpose, dim3(grid_for(sq)), dim3(256), 0, s, ld, slot(U_CA0), slot(U_CA1));
     hipLaunchKernelGGL(k_lg_transpose, dim3(grid_for(sq)), dim3(256), 0, s, ld, slot(U_CB0), slot(U_CB1));
@@ -947,6 +1002,8 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
   out.shift = host.guard.shift;
   out.e_fp32_last = host.e_low;
   out.fp32_iters = host.fp32_iters;
+  out.check_de = host.check_de;
+  out.check_dp = host.check_dp;
   out.sweeps = 0;
   for (int k = 0; k < NPHASE; ++k) out.ticks[k] = 0;
   // phases: fock, energy+commutator, diis, eigensolve, density (ms -> 100 MHz ticks)

@@ -153,6 +153,17 @@ __device__ void copy_mat(int ld, const double* src, double* dst) {
   __syncthreads();
 }
 
+// max |A - B| over the n x n block.
+template <int NT>
+__device__ double max_abs_diff(int n, int ld, const double* A, const double* B, double* red) {
+  double m = 0.0;
+  for (int idx = threadIdx.x; idx < n * n; idx += NT) {
+    const int i = idx / n, j = idx - i * n;
+    m = fmax(m, fabs(A[(int64_t)i * ld + j] - B[(int64_t)i * ld + j]));
+  }
+  return wg_max<NT>(m, red);
+}
+
 // MAXN: largest padded basis of the size class; A_LDS / V_LDS: packed Fock matrix / rotation
 // accumulator of the eigensolver resident in LDS (otherwise in the molecule's slab).
 template <int NT, int MAXN, bool A_LDS, bool V_LDS>
@@ -222,6 +233,8 @@ __global__ void __launch_bounds__(NT)
   res.err_norm = 0.0;
   res.fp32_iters = 0;
   res.e_fp32_last = 0.0;
+  res.check_de = 0.0;
+  res.check_dp = 0.0;
   // A mixed run starts in reduced precision only when the core Hamiltonian stays within the range
   // where FP32 rounding (6e-8 relative) is far below the 1e-5 hand-over: max |H_ij| <= 1e3 eV.
   // (Pseudo-atoms such as MOPAC's capped-bond element carry |beta| up to 1e7 eV.)
@@ -248,16 +261,18 @@ __global__ void __launch_bounds__(NT)
   // against virtual orbitals only), to the accuracy the FP32 density supports (off-diagonal norm
   // 1e-7 of the total) in at most mixed_sweeps sweeps.
   const double tol_full = exp10((double)-c.M.eig_tol_exp);
-  auto eig = [&](bool low, const double* Wm, const double* Cp, double* Co, double* ev, int ag, int vg, int tg) -> int {
+  // spin: 0 alpha (and RHF), 1 beta; it sets the slot order inside degenerate clusters.
+  auto eig = [&](bool low, const double* Wm, const double* Cp, double* Co, double* ev, int ag, int vg, int tg,
+                 int spin) -> int {
     return eigensolve<NT, MAXN, A_LDS, V_LDS, double>(n, ld, Wm, Cp, Co, ev, c.slot(ag), c.slot(vg), c.slot(tg), s_a,
                                                       s_v, s_cs, s_t, s_perm, s_pq, s_red,
                                                       tk + PH_EIG_IN, low ? 1.0e-14 : tol_full,
-                                                      low ? c.M.mixed_sweeps : 40);
+                                                      low ? c.M.mixed_sweeps : 40, spin);
   };
 
   if (!c.M.uhf) {
     // ------------------------------------------------------------------ RHF
-    sad_guess<NT>(c, c.slot(S_HD), 1.0);
+    if (!c.M.init_p) sad_guess<NT>(c, c.slot(S_HD), 1.0);  // else the caller's density is there
     const int nocc = c.M.n_occ;
     int cbuf = 0;  // eigenvector buffer the next solve writes
     int iter = 0;
@@ -360,7 +375,7 @@ __global__ void __launch_bounds__(NT)
       if (rot)
         occ_virt_rotate<NT>(n, ld, nocc, W, c.slot(cbuf ? S_C0 : S_C1), Cnew, c.slot(S_T), c.slot(S_AG), c.slot(S_VG));
       else
-        sweeps += eig(low, W, iter > 0 ? c.slot(cbuf ? S_C0 : S_C1) : nullptr, Cnew, eps_a, S_AG, S_VG, S_T);
+        sweeps += eig(low, W, iter > 0 ? c.slot(cbuf ? S_C0 : S_C1) : nullptr, Cnew, eps_a, S_AG, S_VG, S_T, 0);
       last_rot = rot;
       lap(PH_EIG);
       double* Pn = c.slot(S_HD + s_int[0]);
@@ -373,8 +388,9 @@ __global__ void __launch_bounds__(NT)
         const double de = fabs(e - s_dbl[0]);
         s_dbl[0] = e;
         s_dbl[2] = dp;
-        const bool resid_ok = s_guard.shift == 0.0 && err < 1.0e-7;
-        const bool conv = iter > 0 && de < c.M.e_tol && (dp < c.M.p_tol || resid_ok);
+        const double tail = c.M.precision == 1 ? MIXED_TAIL : 1.0;
+        const bool resid_ok = s_guard.shift == 0.0 && err < 1.0e-7 * tail;
+        const bool conv = iter > 0 && de < c.M.e_tol && (dp < c.M.p_tol * tail || resid_ok);
         if (low) {
           // Reduced-precision phase: never converges here; hands over to FP64 below the switch.
           ++res.fp32_iters;
@@ -422,16 +438,24 @@ __global__ void __launch_bounds__(NT)
       res.e_el = 0.5 * (eh[0] + eh[1]);
     };
     final_energy(P);
-    if (s_guard.shift > 0.0 || last_rot) {
-      sweeps += eig(false, F, c.slot(cbuf ? S_C0 : S_C1), c.slot(cbuf ? S_C1 : S_C0), eps_a, S_AG, S_VG, S_T);
+    // A mixed run always ends with a full FP64 Roothaan step; its size is the result's stated
+    // error against FP64 (err_vs_fp64_*).
+    const bool check = c.M.precision == 1 && s_guard.shift == 0.0;
+    if (s_guard.shift > 0.0 || last_rot || check) {
+      sweeps += eig(false, F, c.slot(cbuf ? S_C0 : S_C1), c.slot(cbuf ? S_C1 : S_C0), eps_a, S_AG, S_VG, S_T, 0);
       cbuf ^= 1;
-      if (last_rot && s_guard.shift == 0.0) {
+      if ((last_rot || check) && s_guard.shift == 0.0) {
         // The reported orbitals diagonalise the converged Fock; the reported density and energy
         // are theirs (one more Roothaan step from the converged density).
         pslot = S_HD + (s_int[0] + 1) % HIST;
         double* P2 = c.slot(pslot);
         (void)density<NT, double>(n, ld, nocc, 2.0, c.slot(cbuf ? S_C0 : S_C1), P, P2, s_red);
+        const double e_conv = res.e_el;
         final_energy(P2);
+        if (check) {
+          res.check_de = fabs(res.e_el - e_conv);
+          res.check_dp = max_abs_diff<NT>(n, ld, P2, P, s_red);
+        }
       }
     }
     res.p_slot = pslot;
@@ -449,8 +473,10 @@ __global__ void __launch_bounds__(NT)
     double* Pt = c.slot(U_PT);
     double* Fa = c.slot(U_FA);
     double* Fb = c.slot(U_FB);
-    sad_guess<NT>(c, Pa, na / ntot);
-    sad_guess<NT>(c, Pb, nb / ntot);
+    if (!c.M.init_p) {  // else the caller's alpha and beta densities are there
+      sad_guess<NT>(c, Pa, na / ntot);
+      sad_guess<NT>(c, Pb, nb / ntot);
+    }
     int cbuf = 0;
     int iter = 0;
     for (; iter < c.M.max_scf; ++iter) {
@@ -485,8 +511,8 @@ __global__ void __launch_bounds__(NT)
         occ_virt_rotate<NT>(n, ld, na, Fa, c.slot(cbuf ? U_CA0 : U_CA1), Ca, c.slot(U_T), c.slot(U_AG), c.slot(U_VG));
         occ_virt_rotate<NT>(n, ld, nb, Fb, c.slot(cbuf ? U_CB0 : U_CB1), Cb, c.slot(U_T), c.slot(U_AG), c.slot(U_VG));
       } else {
-        sweeps += eig(low, Fa, iter > 0 ? c.slot(cbuf ? U_CA0 : U_CA1) : nullptr, Ca, eps_a, U_AG, U_VG, U_T);
-        sweeps += eig(low, Fb, iter > 0 ? c.slot(cbuf ? U_CB0 : U_CB1) : nullptr, Cb, eps_b, U_AG, U_VG, U_T);
+        sweeps += eig(low, Fa, iter > 0 ? c.slot(cbuf ? U_CA0 : U_CA1) : nullptr, Ca, eps_a, U_AG, U_VG, U_T, 0);
+        sweeps += eig(low, Fb, iter > 0 ? c.slot(cbuf ? U_CB0 : U_CB1) : nullptr, Cb, eps_b, U_AG, U_VG, U_T, 1);
       }
       lap(PH_EIG);
       const double da = low ? density<NT, float>(n, ld, na, 1.0, Ca, Pa, Pan, s_red)
@@ -502,8 +528,9 @@ __global__ void __launch_bounds__(NT)
         s_dbl[0] = e;
         s_dbl[2] = dp;
         s_dbl[1] = err;
-        const bool resid_ok = s_guard.shift == 0.0 && err < 1.0e-7;
-        const bool conv = iter > 0 && de < c.M.e_tol && (dp < c.M.p_tol || resid_ok);
+        const double tail = c.M.precision == 1 ? MIXED_TAIL : 1.0;
+        const bool resid_ok = s_guard.shift == 0.0 && err < 1.0e-7 * tail;
+        const bool conv = iter > 0 && de < c.M.e_tol && (dp < c.M.p_tol * tail || resid_ok);
         if (low) {
           ++res.fp32_iters;
           res.e_fp32_last = e;
@@ -556,18 +583,27 @@ __global__ void __launch_bounds__(NT)
     int ca_last = cbuf ? U_CA0 : U_CA1, cb_last = cbuf ? U_CB0 : U_CB1;
     if (s_guard.shift > 0.0 || last_rot) {
       const int dst = (ca_last == U_CA0) ? U_CA1 : U_CA0;
-      sweeps += eig(false, Fa, c.slot(ca_last), c.slot(dst), eps_a, U_AG, U_VG, U_T);
+      sweeps += eig(false, Fa, c.slot(ca_last), c.slot(dst), eps_a, U_AG, U_VG, U_T, 0);
       ca_last = dst;
     }
     {
       const int dst = (cb_last == U_CB0) ? U_CB1 : U_CB0;
-      sweeps += eig(false, Fb, c.slot(cb_last), c.slot(dst), eps_b, U_AG, U_VG, U_T);
+      sweeps += eig(false, Fb, c.slot(cb_last), c.slot(dst), eps_b, U_AG, U_VG, U_T, 1);
       cb_last = dst;
     }
-    if (last_rot && s_guard.shift == 0.0) {
+    const bool check = c.M.precision == 1 && s_guard.shift == 0.0;
+    if (check && !last_rot) {  // alpha was not re-solved above
+      const int dst = (ca_last == U_CA0) ? U_CA1 : U_CA0;
+      sweeps += eig(false, Fa, c.slot(ca_last), c.slot(dst), eps_a, U_AG, U_VG, U_T, 0);
+      ca_last = dst;
+    }
+    if ((last_rot || check) && s_guard.shift == 0.0) {
       // Densities and energy of the reported orbitals (see the RHF branch).
       (void)density<NT, double>(n, ld, na, 1.0, c.slot(ca_last), Pa, Pan, s_red);
       (void)density<NT, double>(n, ld, nb, 1.0, c.slot(cb_last), Pb, Pbn, s_red);
+      if (check)
+        res.check_dp = fmax(max_abs_diff<NT>(n, ld, Pan, Pa, s_red), max_abs_diff<NT>(n, ld, Pbn, Pb, s_red));
+      const double e_conv = res.e_el;
       double* tmp = Pa;
       Pa = Pan;
       Pan = tmp;
@@ -587,6 +623,7 @@ __global__ void __launch_bounds__(NT)
       }
       wg_sum_n<NT, 3>(e3, s_red);
       res.e_el = 0.5 * (e3[0] + e3[1] + e3[2]);
+      if (check) res.check_de = fabs(res.e_el - e_conv);
     }
     res.p_slot = (Pa == c.slot(U_PA)) ? U_PA : U_PAN;
     res.pb_slot = (Pb == c.slot(U_PB)) ? U_PB : U_PBN;

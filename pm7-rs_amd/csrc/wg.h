@@ -43,4 +43,23 @@ __device__ inline double wg_sum(double x, double* red) {
   return v[0];
 }
 
+// Maximum over the workgroup (max is order-independent).
+template <int NT>
+__device__ inline double wg_max(double x, double* red) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_down(x, off, 64));
+  if (lane == 0) red[wave] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int w = 1; w < Wg<NT>::NW; ++w) m = fmax(m, red[w]);
+    red[Wg<NT>::NW] = m;
+  }
+  __syncthreads();
+  const double m = red[Wg<NT>::NW];
+  __syncthreads();
+  return m;
+}
+
 }  // namespace pm7

@@ -29,7 +29,10 @@ STATUS_NAMES = {
     5: "LinearAlgebra", 6: "ScfNotConverged", 7: "InsufficientMemory", 8: "Device", 9: "NotReady",
 }
 
-METHODS = {"pm7": 0, "pm7-ts": 1, "pm7ts": 1, "pm7-minus": 3, "pm7-": 3, "pm7minus": 3}
+METHODS = {"pm7": 0, "pm7-ts": 1, "pm7ts": 1, "pm7-sparkle": 2, "pm7+sparkle": 2, "sparkle": 2,
+           "pm7-minus": 3, "pm7-": 3, "pm7minus": 3, "pm7-hh": 4, "pm7hh": 4}
+DIPOLE_ORIGINS = {"centre-of-mass": 0, "center-of-mass": 0, "com": 0, "coordinates": 1, "origin": 1,
+                  "centre-of-charge": 2, "center-of-charge": 2}
 ACCELERATORS = {"none": 0, "cdiis": 1, "adiis-cdiis": 2, "adiis_cdiis": 2, "hybrid": 2}
 REFERENCES = {"auto": 0, "rhf": 1, "restricted": 1, "r": 1, "uhf": 2, "unrestricted": 2, "u": 2}
 PRECISIONS = {"fp64": 0, "mixed": 1}
@@ -69,7 +72,10 @@ class _Options(C.Structure):
                 ("max_scf", C.c_int32), ("e_tol", C.c_double), ("p_tol", C.c_double),
                 ("use_diis", C.c_int32), ("accelerator", C.c_int32), ("adiis_switch", C.c_double),
                 ("level_shift", C.c_double), ("reference", C.c_int32), ("precision", C.c_int32),
-                ("mixed_switch", C.c_double), ("max_memory_mb", C.c_int64)]
+                ("mixed_switch", C.c_double), ("max_memory_mb", C.c_int64), ("force_dpath", C.c_int32),
+                ("smooth_dispersion", C.c_int32), ("dipole_origin", C.c_int32),
+                ("initial_density_nao", C.c_int32), ("initial_density", C.POINTER(C.c_double)),
+                ("initial_spin_density", C.POINTER(C.c_double))]
 
 
 class _Result(C.Structure):
@@ -84,7 +90,9 @@ class _Result(C.Structure):
                 ("dipole_point_charge_debye", C.c_double * 3), ("dipole_sp_hybrid_debye", C.c_double * 3),
                 ("dipole_pd_hybrid_debye", C.c_double * 3), ("dipole_magnitude", C.c_double),
                 ("device_phase_ms", C.c_double * 6), ("eig_phase_ms", C.c_double * 3),
-                ("density", C.POINTER(C.c_double)), ("spin_density", C.POINTER(C.c_double)),
+                ("err_vs_fp64_ev", C.c_double), ("err_vs_fp64_density", C.c_double),
+                ("missing_element", C.c_int32), ("reserved_", C.c_int32), ("needed_mb", C.c_double),
+                ("budget_mb", C.c_double), ("density", C.POINTER(C.c_double)), ("spin_density", C.POINTER(C.c_double)),
                 ("mo_energies", C.POINTER(C.c_double)), ("mo_coeff", C.POINTER(C.c_double)),
                 ("mo_energies_beta", C.POINTER(C.c_double)), ("mo_coeff_beta", C.POINTER(C.c_double)),
                 ("charges", C.POINTER(C.c_double)), ("message", C.c_char * 256)]
@@ -232,6 +240,11 @@ class Pm7Options:
     precision: str = "fp64"
     mixed_switch: float = 1.0e-3
     max_memory_mb: Optional[int] = None
+    force_dpath: bool = False
+    smooth_dispersion: bool = False
+    dipole_origin: str = "centre-of-mass"
+    initial_density: Optional[np.ndarray] = None
+    initial_spin_density: Optional[np.ndarray] = None
 
     def to_c(self) -> _Options:
         o = _Options()
@@ -250,6 +263,27 @@ class Pm7Options:
         o.precision = PRECISIONS[self.precision.lower()]
         o.mixed_switch = float(self.mixed_switch)
         o.max_memory_mb = int(self.max_memory_mb or 0)
+        o.force_dpath = 1 if self.force_dpath else 0
+        o.smooth_dispersion = 1 if self.smooth_dispersion else 0
+        o.dipole_origin = DIPOLE_ORIGINS[self.dipole_origin.lower()]
+        keep = []
+        if self.initial_density is not None:
+            d = np.ascontiguousarray(self.initial_density, dtype=np.float64)
+            if d.ndim != 2 or d.shape[0] != d.shape[1]:
+                raise Pm7Error(1, f"the supplied initial density has shape {d.shape}")
+            o.initial_density_nao = d.shape[0]
+            o.initial_density = d.ctypes.data_as(C.POINTER(C.c_double))
+            keep.append(d)
+        if self.initial_spin_density is not None:
+            sd = np.ascontiguousarray(self.initial_spin_density, dtype=np.float64)
+            if sd.ndim != 2 or sd.shape[0] != sd.shape[1] or (self.initial_density is not None
+                                                                 and sd.shape != keep[0].shape):
+                raise Pm7Error(1, f"the supplied initial spin density has shape {sd.shape}")
+            if self.initial_density is None:
+                o.initial_density_nao = sd.shape[0]
+            o.initial_spin_density = sd.ctypes.data_as(C.POINTER(C.c_double))
+            keep.append(sd)
+        o._keep = keep  # the arrays must outlive the call that reads them
         return o
 
 
@@ -290,6 +324,11 @@ class Pm7Result:
     device_phase_ms: tuple = ()
     eig_phase_ms: tuple = ()
     message: str = ""
+    err_vs_fp64_ev: float = 0.0
+    err_vs_fp64_density: float = 0.0
+    missing_element: int = 0
+    needed_mb: float = 0.0
+    budget_mb: float = 0.0
 
     orbital_source: str = "molecular"
 
@@ -389,6 +428,9 @@ def _convert_all(c_res, count: int, free) -> list:
             fp32_iterations=col["fp32_iterations"][i], jacobi_sweeps=col["jacobi_sweeps"][i],
             device_phase_ms=tuple(col["device_phase_ms"][i]), eig_phase_ms=tuple(col["eig_phase_ms"][i]),
             message=col["message"][i].decode(errors="replace"),
+            err_vs_fp64_ev=col["err_vs_fp64_ev"][i], err_vs_fp64_density=col["err_vs_fp64_density"][i],
+            missing_element=col["missing_element"][i], needed_mb=col["needed_mb"][i],
+            budget_mb=col["budget_mb"][i],
         ))
     return out
 
@@ -411,7 +453,8 @@ class Engine:
             raise Pm7Error(st, f"pm7_ctx_create failed on device {device}")
         self._h = h
         self._lib = lib
-        if METHODS[method.lower()] == 1:
+        self.method = 0  # pm7_ctx_create loads the PM7 tables
+        if METHODS[method.lower()] != 0:
             self.load(method)
 
     def close(self):
@@ -435,9 +478,12 @@ class Engine:
         return self._lib.pm7_last_error(self._h).decode(errors="replace")
 
     def load(self, method: str):
-        st = self._lib.pm7_params_load(self._h, METHODS[method.lower()])
+        """pm7-rs ``Pm7Parameters::method`` (src/params.rs:130-155) on this context."""
+        code = METHODS[method.lower()]
+        st = self._lib.pm7_params_load(self._h, code)
         if st != 0:
             raise Pm7Error(st, self.last_error())
+        self.method = code
 
     @staticmethod
     def _mol(m: Molecule):
@@ -482,7 +528,13 @@ class Engine:
             cm, k = self._mol(m)
             c_mols[i] = cm
             keep.append(k)
-        c_opts = (_Options * len(opts_list))(*[o.to_c() for o in opts_list])
+        # One method for the whole call selects the matching tables, as a caller of run_pm7 passes
+        # Pm7Parameters::method(options.method); mixed methods are checked per molecule.
+        methods = {METHODS[o.method.lower()] for o in opts_list}
+        if len(methods) == 1 and next(iter(methods)) != self.method and next(iter(methods)) in (0, 1, 3, 4):
+            self.load(opts_list[0].method)
+        c_list = [o.to_c() for o in opts_list]  # keeps seeded densities alive for the call
+        c_opts = (_Options * len(opts_list))(*c_list)
         c_res = (_Result * n)()
         if grad_flat is None:
             st = self._lib.pm7_run_batch(self._h, n, c_mols, c_opts, len(opts_list), c_res)
@@ -660,6 +712,8 @@ def run_pm7(molecule: Molecule, params: Optional[Pm7Parameters] = None,
     if params is not None and METHODS[params.method.lower()] != METHODS[options.method.lower()]:
         raise Pm7Error(1, f"options method `{options.method}` does not match parameter method `{params.method}`")
     eng = default_engine()
+    if params is not None and METHODS[params.method.lower()] != eng.method:
+        eng.load(params.method)
     return eng.run(molecule, options, raise_on_error=True)
 
 

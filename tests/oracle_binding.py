@@ -33,7 +33,10 @@ class OracleOptions(C.Structure):
     _fields_ = [("method", C.c_int), ("multiplicity", C.c_int), ("max_scf", C.c_int),
                 ("use_diis", C.c_int), ("accelerator", C.c_int), ("reference", C.c_int),
                 ("threads", C.c_int), ("charge", C.c_double), ("e_tol", C.c_double),
-                ("p_tol", C.c_double), ("adiis_switch", C.c_double), ("level_shift", C.c_double)]
+                ("p_tol", C.c_double), ("adiis_switch", C.c_double), ("level_shift", C.c_double),
+                ("force_dpath", C.c_int), ("smooth_dispersion", C.c_int), ("dipole_origin", C.c_int),
+                ("pad_", C.c_int), ("initial_density", C.POINTER(C.c_double)),
+                ("initial_spin_density", C.POINTER(C.c_double))]
 
 
 class OracleResult(C.Structure):
@@ -68,6 +71,7 @@ def lib():
         _lib.oracle_core_core.argtypes = [C.c_char_p, C.c_int, ip, dp]
         _lib.oracle_core_core.restype = C.c_double
         _lib.oracle_corrections.argtypes = [C.c_int, ip, dp, dp]
+        _lib.oracle_corrections2.argtypes = [C.c_int, ip, dp, dp]
         _lib.oracle_eigh.argtypes = [C.c_int, dp, dp, dp]
         _lib.oracle_element.argtypes = [C.c_char_p, C.c_int, C.c_int, dp]
         _lib.oracle_default_options.argtypes = [C.POINTER(OracleOptions)]
@@ -104,12 +108,19 @@ def nao(z) -> int:
     return lib().oracle_nao(DATA_DIR.encode(), len(z), _i(z))
 
 
-def run(z, xyz_bohr, charge=0.0, multiplicity=1, **kw):
+def run(z, xyz_bohr, charge=0.0, multiplicity=1, initial_density=None, initial_spin_density=None, **kw):
     z = np.ascontiguousarray(z, dtype=np.int32)
     xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
     n = len(z)
     m = max(nao(z), 1)
     o = default_options(charge=float(charge), multiplicity=int(multiplicity), **kw)
+    keep = []
+    for name, arr in (("initial_density", initial_density), ("initial_spin_density", initial_spin_density)):
+        if arr is not None:
+            a = np.ascontiguousarray(arr, dtype=np.float64)
+            assert a.shape == (m, m), (name, a.shape)
+            keep.append(a)
+            setattr(o, name, _d(a))
     res = OracleResult()
     dens = np.zeros((m, m))
     spin = np.zeros((m, m))
@@ -160,6 +171,15 @@ def core_core(z, xyz_bohr):
     z = np.ascontiguousarray(z, dtype=np.int32)
     xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
     return lib().oracle_core_core(DATA_DIR.encode(), len(z), _i(z), _d(xyz))
+
+
+def corrections2(z, xyz_bohr):
+    """[smooth dispersion, PM7-HH H-H repulsion] in kcal/mol."""
+    z = np.ascontiguousarray(z, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
+    out = np.zeros(2)
+    lib().oracle_corrections2(len(z), _i(z), _d(xyz), _d(out))
+    return out
 
 
 def corrections(z, xyz_bohr):

@@ -253,22 +253,24 @@ constexpr double TIE_REL = 1.0e-10;
 // diagonal once the SCF settles, and Cout = Cprev V. A and V sit in LDS when the variant allows.
 // TA, tol2 and max_sweeps are passed to jacobi_core (the products around it are FP64 either way).
 //
-// Degenerate sets. Inside a set of eigenvalues that differ by rounding only (a cluster, TIE_REL),
-// the order of the vectors is decided by Jacobi slot, ascending (tie_desc = 0) or descending
-// (tie_desc = 1), not by the last bits of the eigenvalues. It matters only where the occupied
-// count splits such a set: an open shell whose odd electrons of both spins fall into degenerate
-// orbitals (O2 2+ triplet: one alpha pi* and one beta pi electron). The UHF loop solves alpha with
-// ascending and beta with descending slot order, which puts the two into different members of
-// their sets; with both in the same member the SCF ends 0.99 eV higher on O2 2+. Only a cold
-// solve (no Cprev: the first SCF iteration) uses the descending order; a warm-started solve keeps
-// ascending slot order, i.e. the previous iteration's order, so a set cannot swap back and forth.
-// Eigenvalues inside a cluster may be out of ascending order by less than its width.
+// Degenerate sets (tie_mode). 0 (RHF): eigenvalues ascending, exact ties by Jacobi slot. 1 and 2
+// (UHF alpha and beta): inside a set of eigenvalues that differ by rounding only (a cluster,
+// TIE_REL) the order of the vectors is decided by slot, not by the last bits of the eigenvalues:
+// ascending for alpha, descending for beta. It matters only where the occupied count splits such a
+// set: an open shell whose odd electrons of both spins fall into degenerate orbitals (O2 2+
+// triplet: one alpha pi* and one beta pi electron). Opposite slot orders put the two into different
+// members of their sets; with both in the same member the SCF ends 0.99 eV higher on O2 2+. Only a
+// cold solve (no Cprev: the first SCF iteration) uses the descending order; a warm-started solve
+// keeps ascending slot order, i.e. the previous iteration's order, so a set cannot swap back and
+// forth. RHF keeps the plain order: a closed shell whose frontier splits a degenerate set (the
+// D3h frontier of CfF3) then settles the reference's state. Eigenvalues inside a cluster may be
+// out of ascending order by less than its width.
 template <int NT, int MAXN, bool A_LDS, bool V_LDS, typename TA>
 __device__ __attribute__((noinline)) int eigensolve(int n, int ld, const double* __restrict__ W, const double* __restrict__ Cprev,
                           double* __restrict__ Cout, double* __restrict__ eps, double* Ag, double* Vg,
                           double* __restrict__ T, double* s_a, double* s_v, double2* s_cs, double* s_t,
                           int* s_perm, int* s_pq, double* red, unsigned long long* tsub, double tol2,
-                          int max_sweeps, int tie_desc) {
+                          int max_sweeps, int tie_mode) {
   const int n2 = ld;
   unsigned long long t0 = wall_clock64();
   TA* A = reinterpret_cast<TA*>(A_LDS ? s_a : Ag);
@@ -345,9 +347,9 @@ __device__ __attribute__((noinline)) int eigensolve(int n, int ld, const double*
     }
     if (threadIdx.x == 0) s_pq[0] = 0;
     __syncthreads();
-    if (near) s_pq[0] = 1;  // every writer stores the same value
+    if (near && tie_mode != 0) s_pq[0] = 1;  // every writer stores the same value
     __syncthreads();
-    const bool desc = tie_desc && !Cprev;
+    const bool desc = tie_mode == 2 && !Cprev;
     if (s_pq[0] && threadIdx.x == 0) {
       // Order each cluster by slot (insertion sort; clusters hold a few vectors).
       int k = 0;

@@ -261,13 +261,13 @@ __global__ void __launch_bounds__(NT)
   // against virtual orbitals only), to the accuracy the FP32 density supports (off-diagonal norm
   // 1e-7 of the total) in at most mixed_sweeps sweeps.
   const double tol_full = exp10((double)-c.M.eig_tol_exp);
-  // spin: 0 alpha (and RHF), 1 beta; it sets the slot order inside degenerate clusters.
+  // tie: 0 RHF, 1 UHF alpha, 2 UHF beta; it sets the order inside degenerate clusters.
   auto eig = [&](bool low, const double* Wm, const double* Cp, double* Co, double* ev, int ag, int vg, int tg,
-                 int spin) -> int {
+                 int tie) -> int {
     return eigensolve<NT, MAXN, A_LDS, V_LDS, double>(n, ld, Wm, Cp, Co, ev, c.slot(ag), c.slot(vg), c.slot(tg), s_a,
                                                       s_v, s_cs, s_t, s_perm, s_pq, s_red,
                                                       tk + PH_EIG_IN, low ? 1.0e-14 : tol_full,
-                                                      low ? c.M.mixed_sweeps : 40, spin);
+                                                      low ? c.M.mixed_sweeps : 40, tie);
   };
 
   if (!c.M.uhf) {
@@ -511,8 +511,8 @@ __global__ void __launch_bounds__(NT)
         occ_virt_rotate<NT>(n, ld, na, Fa, c.slot(cbuf ? U_CA0 : U_CA1), Ca, c.slot(U_T), c.slot(U_AG), c.slot(U_VG));
         occ_virt_rotate<NT>(n, ld, nb, Fb, c.slot(cbuf ? U_CB0 : U_CB1), Cb, c.slot(U_T), c.slot(U_AG), c.slot(U_VG));
       } else {
-        sweeps += eig(low, Fa, iter > 0 ? c.slot(cbuf ? U_CA0 : U_CA1) : nullptr, Ca, eps_a, U_AG, U_VG, U_T, 0);
-        sweeps += eig(low, Fb, iter > 0 ? c.slot(cbuf ? U_CB0 : U_CB1) : nullptr, Cb, eps_b, U_AG, U_VG, U_T, 1);
+        sweeps += eig(low, Fa, iter > 0 ? c.slot(cbuf ? U_CA0 : U_CA1) : nullptr, Ca, eps_a, U_AG, U_VG, U_T, 1);
+        sweeps += eig(low, Fb, iter > 0 ? c.slot(cbuf ? U_CB0 : U_CB1) : nullptr, Cb, eps_b, U_AG, U_VG, U_T, 2);
       }
       lap(PH_EIG);
       const double da = low ? density<NT, float>(n, ld, na, 1.0, Ca, Pa, Pan, s_red)
@@ -583,18 +583,18 @@ __global__ void __launch_bounds__(NT)
     int ca_last = cbuf ? U_CA0 : U_CA1, cb_last = cbuf ? U_CB0 : U_CB1;
     if (s_guard.shift > 0.0 || last_rot) {
       const int dst = (ca_last == U_CA0) ? U_CA1 : U_CA0;
-      sweeps += eig(false, Fa, c.slot(ca_last), c.slot(dst), eps_a, U_AG, U_VG, U_T, 0);
+      sweeps += eig(false, Fa, c.slot(ca_last), c.slot(dst), eps_a, U_AG, U_VG, U_T, 1);
       ca_last = dst;
     }
     {
       const int dst = (cb_last == U_CB0) ? U_CB1 : U_CB0;
-      sweeps += eig(false, Fb, c.slot(cb_last), c.slot(dst), eps_b, U_AG, U_VG, U_T, 1);
+      sweeps += eig(false, Fb, c.slot(cb_last), c.slot(dst), eps_b, U_AG, U_VG, U_T, 2);
       cb_last = dst;
     }
     const bool check = c.M.precision == 1 && s_guard.shift == 0.0;
     if (check && !last_rot) {  // alpha was not re-solved above
       const int dst = (ca_last == U_CA0) ? U_CA1 : U_CA0;
-      sweeps += eig(false, Fa, c.slot(ca_last), c.slot(dst), eps_a, U_AG, U_VG, U_T, 0);
+      sweeps += eig(false, Fa, c.slot(ca_last), c.slot(dst), eps_a, U_AG, U_VG, U_T, 1);
       ca_last = dst;
     }
     if ((last_rot || check) && s_guard.shift == 0.0) {

@@ -627,7 +627,7 @@ pm7_status prepare(pm7_ctx* ctx, Lane& L, Prepared& P, int64_t max_memory_mb) {
   LANECHK(L.d_snum.ensure(num_pairs.size() * 5 * sizeof(double)));
   LANECHK(up(L.d_dpairs, d_pairs.data(), d_pairs.size() * sizeof(int64_t)));
   // Zero the slab: pad rows/columns of every matrix must be exact zeros.
-  LANECHK(hipMemsetAsync(L.d_slab.p, 0, (size_t)P.slab_words * 8, L.st[0]));
+  LANECHK(launch_zero(L.d_slab.as<double>(), P.slab_words, L.st[0]));
   // Seeded starting densities into the first density slot(s) (RHF: S_HD; UHF: U_PA, U_PB), rows
   // padded to the slab's stride. The staging copies live in P until the call ends.
   for (const Job& j : P.jobs) {

@@ -14,6 +14,8 @@
 //               electron-core attraction, read back out of the w blocks (e1b = -Z_b (mu nu|s s)).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.h"
 #include "pm7_device.h"
 
@@ -896,6 +898,19 @@ hipError_t launch_integrals(const BatchDev& b, int64_t n_pairs, int n_items, int
                        b.atom_z, b.atom_ao, b.elem, b.w, b.slab);
   }
   if (n_mols > 0) hipLaunchKernelGGL(k_core_reduce, dim3(n_mols), dim3(256), 0, s, b.mols, b.e_core, b.e_core_mol);
+  return hipGetLastError();
+}
+
+// Zero `words` doubles (a fill kernel rather than a memset, so the clearing is an ordinary kernel
+// node wherever the stream is captured).
+__global__ void __launch_bounds__(256) k_zero(double* __restrict__ p, int64_t words) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < words; i += (int64_t)gridDim.x * 256) p[i] = 0.0;
+}
+
+hipError_t launch_zero(double* p, int64_t words, hipStream_t s) {
+  if (words <= 0) return hipSuccess;
+  const int64_t blocks = std::min<int64_t>((words + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, s, p, words);
   return hipGetLastError();
 }
 

@@ -63,6 +63,7 @@ int large_group_size();  // LARGE_GROUP_DEFAULT, or PM7_AMD_LARGE_GROUP (2..64)
 
 hipError_t launch_integrals(const BatchDev& b, int64_t n_pairs, int n_items, int n_mols, hipStream_t s);
 hipError_t launch_scf(int variant, const BatchDev& b, int list_off, int count, hipStream_t s);
+hipError_t launch_zero(double* p, int64_t words, hipStream_t s);
 hipError_t launch_gather(const BatchDev& b, int n_mols, hipStream_t s);
 // Nuclear gradient at the converged densities of a finished batch: dE/dd per pair into gpair
 // (3 per pair), then per listed atom (items: molecule, batch-global atom) into gatom (3 per atom).

@@ -195,6 +195,17 @@ pm7_status pm7_core_hamiltonian(pm7_ctx* ctx, const pm7_molecule_v1* mol, double
 pm7_status pm7_fock_spin(pm7_ctx* ctx, const pm7_molecule_v1* mol, const double* p_tot,
                          const double* p_spin, double* f_out);
 
+/* No counterpart in the reference: pm7_fock_spin for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). Same inputs,
+ * output and result bits as pm7_fock_spin; the molecule's device layout (atoms, pairs, pair
+ * integrals, matrix slab) lives in a static per-device scratch of 32 MiB, so the call makes no
+ * hipMalloc/hipFree and calls no allocating library routine. A molecule whose layout exceeds the
+ * scratch gets PM7_ERR_INSUFFICIENT_MEMORY. Every call on a device shares that device's scratch:
+ * calls on one context are serialised, but calls that overlap on one GPU from different contexts
+ * race. */
+pm7_status pm7_fock_spin_noalloc(pm7_ctx* ctx, const pm7_molecule_v1* mol, const double* p_tot,
+                                 const double* p_spin, double* f_out);
+
 /* Replaces the CPHF response kernel of cphf_ov (src/hessian.rs:1074-1081), which calls
  * build_fock_x (src/fock.rs:665-684) once per response density and subtracts H_core:
  * g_out[k] = build_fock_x(dp[k]) - H = J(dp[k]) - K(dp[k] / 2), for n_dens symmetric densities of one

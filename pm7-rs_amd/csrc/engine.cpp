@@ -33,6 +33,13 @@
 
 using namespace pm7;
 
+// Device scratch of pm7_fock_spin_noalloc: the batch layout of one molecule (atoms, atom pairs,
+// pair integrals, matrix slab, outputs) for callers that cannot allocate device memory. One copy
+// per GPU, held until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device (pm7_ctx_create), never through the name.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)32 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 // Grow-only pinned host buffer (device-to-host copies at full link speed).
@@ -58,10 +65,26 @@ struct PinnedBuf {
   }
 };
 
+// Bump allocator over a fixed device region (the allocation-free entry point's static scratch):
+// reset at the start of a call, each buffer carved at the next 256-byte boundary.
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0, off = 0;
+};
+
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
+  Arena* arena = nullptr;  // set: carve from the arena, never hipMalloc / hipFree
   hipError_t ensure(size_t bytes) {
+    if (arena) {
+      const size_t at = (arena->off + 255) & ~(size_t)255, want = std::max<size_t>(bytes, 256);
+      if (!arena->base || at + want > arena->cap) return hipErrorOutOfMemory;
+      p = arena->base + at;
+      cap = want;
+      arena->off = at + want;
+      return hipSuccess;
+    }
     if (bytes <= cap && p) return hipSuccess;
     if (p) {
       hipError_t e = hipFree(p);
@@ -75,7 +98,7 @@ struct DevBuf {
     return e;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && !arena) (void)hipFree(p);
     p = nullptr;
     cap = 0;
   }
@@ -193,6 +216,13 @@ struct Lane {
     for (auto& e : ev_var) e = nullptr;
   }
   size_t held() const { return d_w.cap + d_slab.cap + d_gath.cap + d_pairs.cap + d_ecore.cap; }
+  // Every device buffer of this lane is carved from `a` (the allocation-free lane).
+  void use_arena(Arena* a) {
+    DevBuf* bufs[] = {&d_mols, &d_list, &d_atom_z, &d_atom_ao, &d_atom_pp, &d_xyz, &d_pairs, &d_items, &d_gath_off,
+                      &d_w, &d_slab, &d_vecs, &d_out, &d_ecore, &d_ecore_mol, &d_gath, &d_num, &d_snum,
+                      &d_dpairs, &d_gpair, &d_gatom, &d_gitems, &d_snumg, &d_rx, &d_rg, &d_rpp, &d_rstage};
+    for (DevBuf* b : bufs) b->arena = a;
+  }
 };
 
 }  // namespace
@@ -221,6 +251,11 @@ struct pm7_ctx {
   // 1.3e-7); against 1e-26 it saves 0.25 sweeps per solve (FP64 organic batch 7.40k -> 7.57k
   // solves/s, S/P/Cl batch 415 -> 437/s).
   int eig_tol_exp = 22;
+  // Allocation-free entry point: its lane's buffers live in the device's static scratch array.
+  Arena arena;
+  Lane na_lane;
+  bool na_ready = false;
+  std::string na_err;
   std::mutex mu;
 };
 
@@ -1548,6 +1583,22 @@ pm7_status pm7_ctx_create(int32_t device, const char* data_dir, pm7_ctx** out) {
     delete ctx;
     return PM7_ERR_DEVICE;
   }
+  {
+    // The allocation-free lane: this device's copy of the scratch array, and one stream.
+    void* sp = nullptr;
+    const hipError_t se = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+    if (se != hipSuccess || !sp) {
+      ctx->na_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(se);
+    } else if (!ctx->na_lane.create(true)) {
+      ctx->na_err = "stream creation for the allocation-free lane failed";
+      ctx->na_lane.destroy();
+    } else {
+      ctx->arena.base = static_cast<char*>(sp);
+      ctx->arena.cap = NOALLOC_SCRATCH_BYTES;
+      ctx->na_lane.use_arena(&ctx->arena);
+      ctx->na_ready = true;
+    }
+  }
   pm7_status st = ensure_params(ctx, PM7_METHOD_PM7);
   if (st != PM7_OK) {
     pm7_ctx_destroy(ctx);
@@ -1561,6 +1612,7 @@ void pm7_ctx_destroy(pm7_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   ctx->lane.destroy();
+  if (ctx->na_ready) ctx->na_lane.destroy();
   ctx->d_elem.release();
   ctx->d_pairpar.release();
   ctx->d_gl.release();
@@ -1729,6 +1781,51 @@ pm7_status pm7_fock_spin(pm7_ctx* ctx, const pm7_molecule_v1* mol, const double*
   HIPCHK(hipMemcpy2DAsync(f_out, J.nao * sizeof(double), base + 1 * sq, J.ld * sizeof(double), J.nao * sizeof(double),
                           J.nao, hipMemcpyDeviceToHost, ctx->lane.st[0]));
   HIPCHK(hipStreamSynchronize(ctx->lane.st[0]));
+  return PM7_OK;
+}
+
+pm7_status pm7_fock_spin_noalloc(pm7_ctx* ctx, const pm7_molecule_v1* mol, const double* p_tot,
+                                 const double* p_spin, double* f_out) {
+  if (!ctx || !mol || !p_tot || !p_spin || !f_out) return PM7_ERR_INVALID_INPUT;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  if (!ctx->na_ready) return fail(ctx, PM7_ERR_DEVICE, "allocation-free lane unavailable: " + ctx->na_err);
+  pm7_options_v2 o;
+  pm7_default_options(&o);
+  o.method = ctx->method;
+  o.reference = PM7_REF_UHF;  // any electron count: the Fock build needs no occupation
+  Job j;
+  std::string msg;
+  pm7_status st = validate(ctx, *mol, o, j, msg);
+  if (st != PM7_OK) {
+    o.multiplicity = 2;
+    st = validate(ctx, *mol, o, j, msg);
+    if (st != PM7_OK) return fail(ctx, st, msg);
+  }
+  Lane& L = ctx->na_lane;
+  ctx->arena.off = 0;  // every call lays the molecule out from the start of the scratch
+  Prepared P;
+  P.jobs.push_back(j);
+  // The budget check of prepare() sees the scratch size; carving fails past it as well.
+  st = prepare(ctx, L, P, (int64_t)(NOALLOC_SCRATCH_BYTES >> 20));
+  if (st == PM7_ERR_DEVICE && L.err.find("out of memory") != std::string::npos) st = PM7_ERR_INSUFFICIENT_MEMORY;
+  if (st != PM7_OK) {
+    return fail(ctx, st, st == PM7_ERR_INSUFFICIENT_MEMORY
+                             ? "molecule needs more than the " + std::to_string(NOALLOC_SCRATCH_BYTES >> 20) +
+                                   " MiB device scratch of pm7_fock_spin_noalloc: " + L.err
+                             : L.err);
+  }
+  const Job& J = P.jobs[0];
+  const int64_t sq = (int64_t)J.ld * J.ld;
+  double* base = P.b.slab + J.mat0;
+  HIPCHK(hipMemcpy2DAsync(base + 3 * sq, J.ld * sizeof(double), p_tot, J.nao * sizeof(double), J.nao * sizeof(double),
+                          J.nao, hipMemcpyHostToDevice, L.st[0]));
+  HIPCHK(hipMemcpy2DAsync(base + 4 * sq, J.ld * sizeof(double), p_spin, J.nao * sizeof(double), J.nao * sizeof(double),
+                          J.nao, hipMemcpyHostToDevice, L.st[0]));
+  HIPCHK(launch_fock_once(P.b, 0, 3, 4, 1, L.st[0]));
+  HIPCHK(hipMemcpy2DAsync(f_out, J.nao * sizeof(double), base + 1 * sq, J.ld * sizeof(double), J.nao * sizeof(double),
+                          J.nao, hipMemcpyDeviceToHost, L.st[0]));
+  HIPCHK(hipStreamSynchronize(L.st[0]));
   return PM7_OK;
 }
 

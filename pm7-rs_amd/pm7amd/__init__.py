@@ -105,7 +105,7 @@ ABI_FUNCTIONS = [
     "pm7_default_options", "pm7_run", "pm7_run_batch", "pm7_result_free", "pm7_core_hamiltonian",
     "pm7_fock_spin", "pm7_core_core_energy", "pm7_last_timings", "pm7_server_default_config",
     "pm7_server_start", "pm7_server_submit", "pm7_server_poll", "pm7_server_stop", "pm7_gradient_batch",
-    "pm7_gradient", "pm7_response_fock_batch", "pm7_response_fock_spin_batch",
+    "pm7_gradient", "pm7_response_fock_batch", "pm7_response_fock_spin_batch", "pm7_fock_spin_noalloc",
 ]
 
 _lib = None
@@ -137,6 +137,8 @@ def library() -> C.CDLL:
     lib.pm7_core_hamiltonian.argtypes = [vp, C.POINTER(_Molecule), C.POINTER(C.c_double), C.POINTER(C.c_int32)]
     lib.pm7_fock_spin.argtypes = [vp, C.POINTER(_Molecule), C.POINTER(C.c_double), C.POINTER(C.c_double),
                                   C.POINTER(C.c_double)]
+    lib.pm7_fock_spin_noalloc.argtypes = [vp, C.POINTER(_Molecule), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                          C.POINTER(C.c_double)]
     lib.pm7_core_core_energy.argtypes = [vp, C.POINTER(_Molecule), C.POINTER(C.c_double)]
     lib.pm7_response_fock_batch.argtypes = [vp, C.POINTER(_Molecule), C.c_int32, C.POINTER(C.c_double),
                                             C.POINTER(C.c_double)]

@@ -30,15 +30,16 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 
 SCF_KERNELS = ["pm7::k_scf<256, 64, true, true>", "pm7::k_scf<512, 112, true, true>",
                "pm7::k_scf<512, 196, true, false>", "pm7::k_scf<512, 1024, false, false>"]
-TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_organic256_fp64.json"
+TRAFFIC_FILES = {"fp64": ROOT / "profiles" / "hbm_traffic_organic256_fp64.json",
+                 "mixed": ROOT / "profiles" / "hbm_traffic_organic256_mixed.json"}
 RESPONSE_NT = int(os.environ.get("PM7_AMD_RESPONSE_NT", "512"))  # threads per density of the response kernel
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, mode="fp64"):
     """HBM bytes per launch of `kernel` from the FETCH_SIZE / WRITE_SIZE counter runs (scaled by a
     plain copy's readings) kept beside the profiles, or None when that kernel was not profiled."""
     try:
-        per = json.loads(TRAFFIC_FILE.read_text())["per_launch"]
+        per = json.loads(TRAFFIC_FILES[mode].read_text())["per_launch"]
     except (OSError, ValueError, KeyError):
         return None
     rec = per.get(kernel)
@@ -55,6 +56,11 @@ def pct(xs, q):
     return s[lo] + (s[hi] - s[lo]) * (k - lo)
 
 
+def size_class(n):
+    """The megakernel size class of a basis of n functions (<= 64, <= 112, <= 196, larger)."""
+    return 0 if n <= 64 else (1 if n <= 112 else (2 if n <= 196 else 3))
+
+
 def scf_flops(results, classes):
     """Algorithmic FP64 flops per size class: per SCF iteration 2n^3 (commutator FP),
     2 n^2 n_occ (density), 9 n^3 (dense symmetric eigensolve with vectors, Golub & Van Loan's
@@ -64,11 +70,61 @@ def scf_flops(results, classes):
         n = r.density.shape[0]
         if n == 0:
             continue
-        c = 0 if n <= 64 else (1 if n <= 112 else (2 if n <= 196 else 3))
+        c = size_class(n)
         spins = 2 if r.unrestricted else 1
         per = 11.0 * n ** 3 + 2.0 * n * n * r.n_occ
         out[c] += spins * per * r.iterations
     return out
+
+
+N_CU = 256  # MI355X compute units; one SCF workgroup owns one molecule on one CU
+HBM_PEAK_GBS = 8000.0
+
+
+def npair_of_atoms(atoms):
+    """Packed orbital pairs per atom (1 for H, 10 for an sp atom, 45 for an spd atom)."""
+    spd = {"S", "P", "Cl", "Si", "Al", "Br", "I"}
+    return [1 if a[0] == "H" else (45 if a[0] in spd else 10) for a in atoms]
+
+
+def phase_roofline(atoms_list, results):
+    """Per-phase rates of the SCF megakernel from each molecule's own device clocks, for the slowest
+    molecule (the step's critical path) and summed over the batch, against the chip's peaks and
+    against one CU's share of them (a workgroup owns a molecule):
+      Fock build: B_fock = 8 sum_pairs npair_a npair_b (w) + 24 nao^2 (P, H read, F written) bytes
+                  per iteration (the FP32 phase reads the same FP64 words);
+      commutator: 2 nao^3 flop per iteration (E = FP - (FP)^T), timed with the energy dots;
+      density:    2 nao^2 n_occ flop per iteration, doubled for UHF;
+      eigensolve: time and sweeps only."""
+    rows = []
+    for atoms, r in zip(atoms_list, results):
+        n = r.density.shape[0]
+        npair = npair_of_atoms(atoms)
+        tot = sum(npair)
+        w_words = (tot * tot - sum(x * x for x in npair)) / 2.0
+        it = r.iterations
+        spins = 2 if r.unrestricted else 1
+        rows.append({"bytes_fock": (8.0 * w_words + 24.0 * n * n) * it,
+                     "flop_comm": spins * 2.0 * n ** 3 * it,
+                     "flop_dens": spins * 2.0 * n * n * r.n_occ * it,
+                     "ms": r.device_phase_ms, "n": n})
+    def rates(sel):
+        fock_s = sum(x["ms"][0] for x in sel) * 1e-3
+        comm_s = sum(x["ms"][1] for x in sel) * 1e-3
+        dens_s = sum(x["ms"][4] for x in sel) * 1e-3
+        gbs = sum(x["bytes_fock"] for x in sel) / fock_s / 1e9 if fock_s else 0.0
+        tf_c = sum(x["flop_comm"] for x in sel) / comm_s / 1e12 if comm_s else 0.0
+        tf_d = sum(x["flop_dens"] for x in sel) / dens_s / 1e12 if dens_s else 0.0
+        cu = len(sel)
+        return {"fock_gbs": round(gbs, 2), "fock_frac_of_hbm": round(gbs / HBM_PEAK_GBS, 6),
+                "fock_frac_of_cu_share": round(gbs / (HBM_PEAK_GBS / N_CU) / cu, 4),
+                "commutator_tflops": round(tf_c, 5), "commutator_frac_of_cu_share": round(tf_c / (FP64_PEAK_TFLOPS / N_CU) / cu, 4),
+                "density_tflops": round(tf_d, 5), "density_frac_of_cu_share": round(tf_d / (FP64_PEAK_TFLOPS / N_CU) / cu, 4),
+                "eigensolve_ms": round(sum(x["ms"][3] for x in sel), 3)}
+    slow = max(rows, key=lambda x: x["ms"][5])
+    return {"slowest_molecule": dict(nao=slow["n"], **rates([slow])), "batch": rates(rows),
+            "note": "per-CU fractions divide by the number of molecules (one CU each) times one CU's share "
+                    "(8 TB/s / 256 of HBM, 78.6 TFLOP/s / 256 of FP64 MFMA)"}
 
 
 def slowest_info(results):
@@ -168,9 +224,12 @@ def main() -> int:
         out = {"workload": f"({args.large_waters} H2O) cluster, 1.0 g/cm3, O-O >= 2.6 A, seed 4",
                "atoms": len(atoms), "nao": 6 * args.large_waters}
         ref = None
-        for precision in ("fp64", "mixed"):
+        # "mixed" hands over to FP64 at a density change of 1e-5 (the configuration's); the engine's
+        # default hand-over (1e-3) is run beside it
+        for precision, switch in (("fp64", 1e-3), ("mixed", 1e-5), ("mixed_default_switch", 1e-3)):
             s0 = time.perf_counter()
-            r = eng.run(mol, pm7amd.Pm7Options(precision=precision), raise_on_error=False)
+            r = eng.run(mol, pm7amd.Pm7Options(precision=precision.split("_")[0], mixed_switch=switch),
+                        raise_on_error=False)
             wall = time.perf_counter() - s0
             ph = dict(zip(["fock", "energy_commutator", "diis", "eigensolve", "density", "total"],
                           [round(x, 2) for x in r.device_phase_ms]))
@@ -196,6 +255,9 @@ def main() -> int:
             if precision == "fp64":
                 ref = r
             else:
+                rec["mixed_switch"] = switch
+                rec["err_vs_fp64_ev_reported"] = r.err_vs_fp64_ev
+                rec["err_vs_fp64_density_reported"] = r.err_vs_fp64_density
                 rec["abs_dHf_vs_fp64_kcal"] = abs(r.heat_of_formation_kcal - ref.heat_of_formation_kcal)
                 rec["abs_dE_vs_fp64_ev"] = abs(r.total_ev - ref.total_ev)
                 rec["max_abs_dP_vs_fp64"] = float(np.max(np.abs(r.density - ref.density)))
@@ -257,6 +319,12 @@ def main() -> int:
     dhf_mx = max(abs(a.heat_of_formation_kcal - b.heat_of_formation_kcal) for a, b in zip(resmx, res64))
     de_mx = max(abs(a.total_ev - b.total_ev) for a, b in zip(resmx, res64))
     dp_mx = max(float(np.max(np.abs(a.density - b.density))) for a, b in zip(resmx, res64))
+    # The FP64 fixed point: FP64 converged to p_tol 1e-10; a default FP64 run stops up to ~1e-6 from
+    # it, and the mixed path, which reaches it along another route, is compared with the point itself.
+    res_fix = eng.run_batch(mols, pm7amd.Pm7Options(e_tol=1e-12, p_tol=1e-10, max_scf=500))
+    dp_fix_mx = max(float(np.max(np.abs(a.density - b.density))) for a, b in zip(resmx, res_fix))
+    dp_fix_64 = max(float(np.max(np.abs(a.density - b.density))) for a, b in zip(res64, res_fix))
+    err_rep = max(r.err_vs_fp64_density for r in resmx)
     mixed_ok = dhf_mx <= 1e-4 and not badmx
 
     # Serving: the continuous-batching server's FP64 capacity is measured first (every request
@@ -403,7 +471,7 @@ def main() -> int:
                     "roofline": {"bound": "hbm", "achieved": round(alg_bytes / (kernel_ms * 1e-3) / 1e9, 1),
                                  "peak": 8000.0, "unit": "GB/s",
                                  "frac": round(alg_bytes / (kernel_ms * 1e-3) / 8.0e12, 5),
-                                 "traffic": measured_traffic(f"pm7::k_fock_response<{RESPONSE_NT}>"),
+                                 "traffic": measured_traffic(f"pm7::k_fock_response<{RESPONSE_NT}>", "fp64"),
                                  "algorithmic_bytes": int(alg_bytes),
                                  "integral_words": int(w_words)}}
         if not args.no_cpu:
@@ -442,17 +510,62 @@ def main() -> int:
     single_api_rate = len(single) / (time.perf_counter() - s)
     single_rate = len(single) / single_dev
 
-    # Roofline of the dominant kernel: the SCF megakernel size class with the largest time.
-    cls_ms = [float(np.mean([k[4 + c] for k in kern64])) for c in range(4)]
-    dom = int(np.argmax(cls_ms))
-    flops = scf_flops(res64, None)
-    achieved_tf = flops[dom] / (cls_ms[dom] * 1e-3) / 1e12 if cls_ms[dom] > 0 else 0.0
-    kname = SCF_KERNELS[dom]
-    roofline = {"bound": "mfma", "kernel": kname,
+    # Roofline of the dominant kernel: the SCF megakernel size class with the largest time, its
+    # algorithmic flops (11 n^3 + 2 n^2 n_occ per molecule-iteration: the Roothaan step of the
+    # reference, whatever route the kernel takes) over its launch time (HIP events on its stream).
+    def roofline_of(res, kern, mode):
+        cls_ms = [float(np.mean([k[4 + c] for k in kern])) for c in range(4)]
+        dom = int(np.argmax(cls_ms))
+        flops = scf_flops(res, None)
+        achieved_tf = flops[dom] / (cls_ms[dom] * 1e-3) / 1e12 if cls_ms[dom] > 0 else 0.0
+        kname = SCF_KERNELS[dom]
+        cnt = sum(1 for r in res if size_class(r.density.shape[0]) == dom)
+        busy = sum(r.device_phase_ms[5] for r in res) / (N_CU * max(cls_ms))
+        return {"bound": "mfma", "kernel": kname, "mode": mode,
                 "achieved": round(achieved_tf, 6), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved_tf / FP64_PEAK_TFLOPS, 8), "traffic": measured_traffic(kname),
+                "frac": round(achieved_tf / FP64_PEAK_TFLOPS, 8), "traffic": measured_traffic(kname, mode),
                 "avg_launch_ms": round(cls_ms[dom], 4), "class_ms": [round(x, 4) for x in cls_ms],
+                "molecules_in_launch": cnt,
+                "frac_of_cu_share": round(achieved_tf / (FP64_PEAK_TFLOPS / N_CU) / max(1, cnt), 6),
+                "cu_busy_frac": round(busy, 4),
                 "measured_dgemm_peak": dgemm_tf, "frac_of_measured_dgemm": round(achieved_tf / dgemm_tf, 8)}
+
+    roofline_mx = roofline_of(resmx, kernmx, "mixed")
+    roofline_64 = roofline_of(res64, kern64, "fp64")
+    roofline = roofline_mx if mixed_ok else roofline_64
+
+    # Configurations 1 and 2 one molecule at a time (FP64): water and the jittered all-trans C32H66
+    # (98 atoms, 194 AOs), device time and whole-call time, beside the CPU oracle on one thread and
+    # on the GPU's 16-CPU share of the host.
+    configs12 = {}
+    for name, atoms in (("water", W.water()), ("C32H66", W.alkane_chain(32, 2))):
+        m1 = pm7amd.Molecule.from_symbols_angstrom(atoms)
+        eng.run(m1)
+        reps = 20 if name == "water" else 5
+        dev, t = 0.0, time.perf_counter()
+        for _ in range(reps):
+            r1 = eng.run(m1)
+            tt = eng.last_timings_ms()
+            dev += (tt[0] + tt[1] + tt[2]) * 1e-3
+        api = (time.perf_counter() - t) / reps
+        rec = {"nao": r1.density.shape[0], "iterations": r1.iterations, "device_ms": round(dev / reps * 1e3, 4),
+               "api_ms": round(api * 1e3, 4), "solves_per_s_device": round(reps / dev, 3),
+               "total_ev": r1.total_ev}
+        if not args.no_cpu:
+            sys.path.insert(0, str(ROOT / "tests"))
+            import oracle_binding as ob
+            z1, x1 = ob.atoms_to_arrays(atoms)
+            for label, thr in (("cpu_1_thread", 1), ("cpu_16_threads", 16)):
+                ob.run(z1, x1, threads=thr)
+                t = time.perf_counter()
+                nrep = 0
+                while nrep == 0 or time.perf_counter() - t < 1.0:
+                    ref1, *_ = ob.run(z1, x1, threads=thr)
+                    nrep += 1
+                rec[label] = {"ms": round((time.perf_counter() - t) / nrep * 1e3, 3), "cores": thr}
+            rec["abs_dE_vs_oracle_ev"] = abs(ref1.total_ev - r1.total_ev)
+        configs12[name] = rec
+    print(f"[bench] configs 1-2: {configs12}", file=sys.stderr, flush=True)
 
     cpu = None
     parity = None
@@ -464,6 +577,8 @@ def main() -> int:
 
         # The CPU oracle over the whole batch, molecules in parallel (one host thread each, as
         # rayon callers of run_pm7 do), then one thread alone on a sample of it.
+        # 16 = this GPU's share of the host (the pool gives one GPU 16 CPUs, whatever os.cpu_count()
+        # reports); os.cpu_count() is printed beside it
         threads = min(16, os.cpu_count() or 1)
         arrays = [ob.atoms_to_arrays(a) for a in batch]
 
@@ -486,6 +601,7 @@ def main() -> int:
             ob.run(arrays[i][0], arrays[i][1], threads=1)
         t1 = time.perf_counter() - s
         cpu = {"value": round(len(batch) / cpu_t, 3), "unit": "solves/s", "cores": threads, "kind": "port",
+               "host_cpus_reported": os.cpu_count(),
                "sample": f"the 256-molecule batch on the CPU FP64 oracle, {threads} molecules at a time "
                          f"(one thread each), {passes} passes, {cpu_t:.2f}s per pass; one thread alone on the first {n1}: "
                          f"{n1 / t1:.3f} solves/s ({t1:.1f}s)",
@@ -528,6 +644,9 @@ def main() -> int:
                   "scf_kernel_ms": round(float(np.mean([k[1] for k in kernmx])), 4),
                   "speedup_vs_fp64": round(solvesmx / solves64, 4),
                   "max_abs_dHf_vs_fp64_kcal": dhf_mx, "max_abs_dE_vs_fp64_ev": de_mx, "max_abs_dP_vs_fp64": dp_mx,
+                  "max_abs_dP_vs_fp64_fixed_point": dp_fix_mx, "fp64_default_max_abs_dP_vs_fixed_point": dp_fix_64,
+                  "max_err_vs_fp64_density_reported": err_rep,
+                  "phase_roofline": phase_roofline(batch, resmx),
                   "within_1e-4_kcal": mixed_ok,
                   "fp32_iterations_mean": round(float(np.mean([r.fp32_iterations for r in resmx])), 2),
                   "slowest_molecule": slowest_info(resmx),
@@ -562,6 +681,9 @@ def main() -> int:
         "slowest_molecule": slowest_info(res64),
         "jacobi_sweeps_per_solve": round(float(np.mean([r.jacobi_sweeps / max(1, r.iterations) for r in res64])), 2),
         "parity_vs_oracle": parity,
+        "phase_roofline_fp64": phase_roofline(batch, res64),
+        "roofline_fp64": roofline_64,
+        "configs_1_2": configs12,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

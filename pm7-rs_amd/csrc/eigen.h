@@ -285,34 +285,40 @@ __device__ __attribute__((noinline)) int eigensolve(int n, int ld, const double*
   auto vat = [&](int i, int j) -> double& { return i < vrows ? Vl[i * n2 + j] : Vg[i * n2 + j]; };
   constexpr int NW = NT / 64;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, q = lane >> 4;
-  const int nb = (n + 15) >> 4;
+  const int nb = (n + 31) >> 5;  // 32 x 32 MFMA blocks (their 32 nb rows cover n2)
   if (Cprev) {
     // T = W Cprev (W symmetric: W(i, k) read as W(k, i), so both operands stream along rows).
     for (int t = wave; t < nb * nb; t += NW) {
-      const int I = t / nb, J = t - I * nb, i0 = 16 * I, j0 = 16 * J;
+      const int I = t / nb, J = t - I * nb, i0 = 32 * I, j0 = 32 * J;
       auto w_ik = [&](int i, int k) { return (i < n && k < n) ? W[(int64_t)k * ld + i] : 0.0; };
       auto c_kj = [&](int k, int j) { return (k < n && j < n) ? Cprev[(int64_t)k * ld + j] : 0.0; };
-      const dbl4 x = tile16_f64(n, i0, j0, w_ik, c_kj);
+      dbl4 x[4];
+      block32_f64(n, i0, j0, w_ik, c_kj, x);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int i = i0 + tile_row_f64(q, u), j = j0 + r;
-        if (i < n && j < n) T[(int64_t)i * ld + j] = x[u];
-      }
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int i = i0 + 16 * (s >> 1) + tile_row_f64(q, u), j = j0 + 16 * (s & 1) + r;
+          if (i < n && j < n) T[(int64_t)i * ld + j] = x[s][u];
+        }
     }
     __syncthreads();
     // A = Cprev^T T, upper triangle (pad rows and columns up to n2 are zero).
     for (int t = wave; t < nb * nb; t += NW) {
       const int I = t / nb, J = t - I * nb;
       if (I > J) continue;
-      const int i0 = 16 * I, j0 = 16 * J;
+      const int i0 = 32 * I, j0 = 32 * J;
       auto c_ik = [&](int i, int k) { return (i < n && k < n) ? Cprev[(int64_t)k * ld + i] : 0.0; };
       auto t_kj = [&](int k, int j) { return (k < n && j < n) ? T[(int64_t)k * ld + j] : 0.0; };
-      const dbl4 x = tile16_f64(n, i0, j0, c_ik, t_kj);
+      dbl4 x[4];
+      block32_f64(n, i0, j0, c_ik, t_kj, x);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int i = i0 + tile_row_f64(q, u), j = j0 + r;
-        if (i <= j && j < n2) A[pidx(i, j)] = (i < n && j < n) ? (TA)x[u] : (TA)0;
-      }
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int i = i0 + 16 * (s >> 1) + tile_row_f64(q, u), j = j0 + 16 * (s & 1) + r;
+          if (i <= j && j < n2) A[pidx(i, j)] = (i < n && j < n) ? (TA)x[s][u] : (TA)0;
+        }
     }
   } else {
     constexpr int NX = 32, NY = NT / NX;
@@ -378,15 +384,18 @@ __device__ __attribute__((noinline)) int eigensolve(int n, int ld, const double*
   if (Cprev) {
     // Cout = Cprev V(:, perm); pad rows and columns up to ld are written as zero.
     for (int t = wave; t < nb * nb; t += NW) {
-      const int I = t / nb, J = t - I * nb, i0 = 16 * I, j0 = 16 * J;
+      const int I = t / nb, J = t - I * nb, i0 = 32 * I, j0 = 32 * J;
       auto c_ik = [&](int i, int k) { return (i < n && k < n) ? Cprev[(int64_t)i * ld + k] : 0.0; };
       auto v_kj = [&](int k, int j) { return (k < n && j < n) ? vat(k, s_perm[j]) : 0.0; };
-      const dbl4 x = tile16_f64(n, i0, j0, c_ik, v_kj);
+      dbl4 x[4];
+      block32_f64(n, i0, j0, c_ik, v_kj, x);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int i = i0 + tile_row_f64(q, u), j = j0 + r;
-        if (i < ld && j < ld) Cout[(int64_t)i * ld + j] = (i < n && j < n) ? x[u] : 0.0;
-      }
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int i = i0 + 16 * (s >> 1) + tile_row_f64(q, u), j = j0 + 16 * (s & 1) + r;
+          if (i < ld && j < ld) Cout[(int64_t)i * ld + j] = (i < n && j < n) ? x[s][u] : 0.0;
+        }
     }
   } else {
     for (int idx = threadIdx.x; idx < n2 * n2; idx += NT) {
@@ -412,7 +421,7 @@ __device__ __attribute__((noinline)) int eigensolve(int n, int ld, const double*
 // (overwritten) and Cp (overwritten). Stationary points of the SCF are the same as with the
 // full eigensolve: at self-consistency F'_ia = 0 and the step is the identity.
 template <int NT>
-__device__ void occ_virt_rotate(int n, int ld, int nocc, double* __restrict__ W, double* __restrict__ Cp,
+__device__ __attribute__((noinline)) void occ_virt_rotate(int n, int ld, int nocc, double* __restrict__ W, double* __restrict__ Cp,
                                 double* __restrict__ Cn, double* __restrict__ Ts, double* __restrict__ Ag,
                                 double* __restrict__ Vg) {
   auto rowk = [&](const double* X) {

@@ -50,6 +50,8 @@ struct LargeCtl {
   double G[HIST * HIST], M[HIST * HIST];
   double e_old, e_cur, err, dp, best_dp, e_low;
   double e_conv, check_de, check_dp;  // mixed precision: the final FP64 Roothaan step's size
+  int tail_extra, tail_stall;          // mixed tail: iterations past the caller's convergence test
+  double tail_best;
   double dots[NDOT];  // reductions land here
   Guard guard;
   double wk[(HIST + 1) * (HIST + 3)];
@@ -393,7 +395,7 @@ __global__ void __launch_bounds__(64) k_lg_ctrl_a(LargeCtl* __restrict__ ctl, in
 // Control B (one thread): convergence, stall guard and the precision hand-over
 // (src/scf.rs:1274-1287; the megakernel's tail of an iteration).
 __global__ void k_lg_ctrl_b(LargeCtl* __restrict__ ctl, int iter, int64_t nn, double e_tol, double p_tol,
-                            double resid_tol, double mixed_switch, int uhf) {
+                            double tail, double mixed_switch, int uhf) {
   if (threadIdx.x != 0) return;
   const double e = ctl->e_cur;
   const double dp = uhf ? sqrt(ctl->dots[0] / (double)nn) + sqrt(ctl->dots[1] / (double)nn)
@@ -401,8 +403,24 @@ __global__ void k_lg_ctrl_b(LargeCtl* __restrict__ ctl, int iter, int64_t nn, do
   const double de = fabs(e - ctl->e_old);
   ctl->e_old = e;
   ctl->dp = dp;
-  const bool resid_ok = ctl->guard.shift == 0.0 && ctl->err < resid_tol;
-  const bool conv = iter > 0 && de < e_tol && (dp < p_tol || resid_ok);
+  // The caller's test; a mixed run's FP64 tail asks for `tail` times both bounds, for at most
+  // MIXED_TAIL_EXTRA iterations past the first that meets the caller's (as in the megakernel).
+  const bool resid_ok = ctl->guard.shift == 0.0 && ctl->err < 1.0e-7;
+  bool conv = iter > 0 && de < e_tol && (dp < p_tol || resid_ok);
+  if (tail < 1.0 && !ctl->low) {
+    const bool tight = iter > 0 && de < e_tol && (dp < p_tol * tail || (ctl->guard.shift == 0.0 && ctl->err < 1.0e-7 * tail));
+    if (!tight && conv) {  // past the caller's test: go on while the density change still drops
+      if (ctl->tail_extra++ == 0 || dp < 0.8 * ctl->tail_best) {
+        ctl->tail_best = dp;
+        ctl->tail_stall = 0;
+      } else {
+        ++ctl->tail_stall;
+      }
+      conv = ctl->tail_stall >= 3 || ctl->tail_extra > MIXED_TAIL_EXTRA;
+    } else {
+      conv = tight;
+    }
+  }
   if (ctl->low) {
     ctl->fp32_iters++;
     ctl->e_low = e;
@@ -439,6 +457,9 @@ __global__ void k_lg_init(LargeCtl* __restrict__ ctl, double shift_floor, int lo
   ctl->low = low;
   ctl->fp32_iters = 0;
   ctl->stall = 0;
+  ctl->tail_extra = 0;
+  ctl->tail_stall = 0;
+  ctl->tail_best = 1.0e30;
   ctl->iter = 0;
   ctl->e_old = 0.0;
   ctl->e_cur = 0.0;
@@ -820,8 +841,8 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
       LG(density(slot(cslot_c), nocc, 2.0, Pn, low != 0));
       hipLaunchKernelGGL(k_lg_diff, dim3(RB), dim3(256), 0, s, n, ld, Pn, P, part);
       hipLaunchKernelGGL(k_lg_finish, dim3(1), dim3(64), 0, s, RB, 1, part, ctl_dots);
-      hipLaunchKernelGGL(k_lg_ctrl_b, dim3(1), dim3(64), 0, s, ctl, iter, nn, M.e_tol, M.p_tol * tail, 1.0e-7 * tail,
-                         M.mixed_switch, 0);
+      hipLaunchKernelGGL(k_lg_ctrl_b, dim3(1), dim3(64), 0, s, ctl, iter, nn, M.e_tol, M.p_tol, tail, M.mixed_switch,
+                         0);
       LG(hipEventRecord(lw->ev[5], s));
       LG(hipMemcpyAsync(&host, ctl, sizeof host, hipMemcpyDeviceToHost, s));
       LG(hipStreamSynchronize(s));
@@ -934,8 +955,8 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
       hipLaunchKernelGGL(k_lg_finish, dim3(1), dim3(64), 0, s, RB, 1, part, ctl_dots);
       hipLaunchKernelGGL(k_lg_diff, dim3(RB), dim3(256), 0, s, n, ld, Pbn, Pb, part);
       hipLaunchKernelGGL(k_lg_finish, dim3(1), dim3(64), 0, s, RB, 1, part, ctl_dots + 1);
-      hipLaunchKernelGGL(k_lg_ctrl_b, dim3(1), dim3(64), 0, s, ctl, iter, nn, M.e_tol, M.p_tol * tail, 1.0e-7 * tail,
-                         M.mixed_switch, 1);
+      hipLaunchKernelGGL(k_lg_ctrl_b, dim3(1), dim3(64), 0, s, ctl, iter, nn, M.e_tol, M.p_tol, tail, M.mixed_switch,
+                         1);
       LG(hipEventRecord(lw->ev[5], s));
       LG(hipMemcpyAsync(&host, ctl, sizeof host, hipMemcpyDeviceToHost, s));
       LG(hipStreamSynchronize(s));

@@ -29,6 +29,10 @@ constexpr double ROTATE_BELOW = 1.0e-2;  // default density change below which m
 // test): it reaches the fixed point along a different path than an FP64 run, so it has to end
 // closer to it for the two densities to agree to 1e-7.
 constexpr double MIXED_TAIL = 1.0e-2;
+// ... for as long as it makes progress once the caller's own test (p_tol) is met: the tail stops,
+// converged, after three iterations in a row without a 20 % drop of the density change below
+// the tail's best (a nearly degenerate frontier settles no further), or after MIXED_TAIL_EXTRA.
+constexpr int MIXED_TAIL_EXTRA = 40;
 
 // Slab slots, RHF.
 // C0/C1 double-buffer the sorted eigenvectors so each eigensolve can start from the last one;

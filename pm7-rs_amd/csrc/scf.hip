@@ -37,38 +37,41 @@ namespace pm7 {
 // their operands along rows (X(i, k) = X(k, i)). Each wave owns the upper-triangle tiles
 // wave, wave + NW, ... and writes E(i, j) and E(j, i) = -E(i, j).
 template <int NT>
-__device__ double commutator(int n, int ld, const double* __restrict__ F, const double* __restrict__ P,
+__device__ __attribute__((noinline)) double commutator(int n, int ld, const double* __restrict__ F, const double* __restrict__ P,
                              double* __restrict__ E, double* red) {
   constexpr int NW = NT / 64;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, q = lane >> 4;
-  const int nb = (n + 15) >> 4;
+  const int nb = (n + 31) >> 5;  // 32 x 32 blocks (four 16 x 16 tiles each)
   double nrm = 0.0;
   for (int t = wave; t < nb * nb; t += NW) {
     const int I = t / nb, J = t - I * nb;
     if (I > J) continue;
-    const int i0 = 16 * I, j0 = 16 * J;
+    const int i0 = 32 * I, j0 = 32 * J;
     auto f_ik = [&](int i, int k) { return (i < n && k < n) ? F[(int64_t)k * ld + i] : 0.0; };
     auto p_kj = [&](int k, int j) { return (k < n && j < n) ? P[(int64_t)k * ld + j] : 0.0; };
     auto p_ik = [&](int i, int k) { return (i < n && k < n) ? P[(int64_t)k * ld + i] : 0.0; };
     auto f_kj = [&](int k, int j) { return (k < n && j < n) ? F[(int64_t)k * ld + j] : 0.0; };
-    const dbl4 fp = tile16_f64(n, i0, j0, f_ik, p_kj);
-    const dbl4 pf = tile16_f64(n, i0, j0, p_ik, f_kj);
+    dbl4 fp[4], pf[4];
+    block32_f64(n, i0, j0, f_ik, p_kj, fp);
+    block32_f64(n, i0, j0, p_ik, f_kj, pf);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int i = i0 + tile_row_f64(q, u), j = j0 + r;
-      if (i >= n || j >= n) continue;
-      const double e = fp[u] - pf[u];
-      if (I == J) {
-        nrm += e * e;
-        if (E) E[(int64_t)i * ld + j] = e;
-      } else {
-        nrm += 2.0 * e * e;
-        if (E) {
-          E[(int64_t)i * ld + j] = e;
-          E[(int64_t)j * ld + i] = -e;
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = i0 + 16 * (s >> 1) + tile_row_f64(q, u), j = j0 + 16 * (s & 1) + r;
+        if (i >= n || j >= n) continue;
+        const double e = fp[s][u] - pf[s][u];
+        if (I == J) {
+          nrm += e * e;
+          if (E) E[(int64_t)i * ld + j] = e;
+        } else {
+          nrm += 2.0 * e * e;
+          if (E) {
+            E[(int64_t)i * ld + j] = e;
+            E[(int64_t)j * ld + i] = -e;
+          }
         }
       }
-    }
   }
   return wg_sum<NT>(nrm, red);
 }
@@ -77,46 +80,54 @@ __device__ double commutator(int n, int ld, const double* __restrict__ F, const 
 // mirrored; returns sum (P' - P)^2 over n x n. T is the precision of the product: FP64, or FP32
 // inputs with FP32 accumulation in the reduced-precision phase.
 template <int NT, typename T>
-__device__ double density(int n, int ld, int nocc, double wgt, const double* __restrict__ C,
+__device__ __attribute__((noinline)) double density(int n, int ld, int nocc, double wgt, const double* __restrict__ C,
                           const double* __restrict__ Pold, double* __restrict__ Pn, double* red) {
   constexpr int NW = NT / 64;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, q = lane >> 4;
-  const int nb = (n + 15) >> 4;
+  const int nb = (n + 31) >> 5;  // 32 x 32 blocks
   double dsq = 0.0;
   for (int t = wave; t < nb * nb; t += NW) {
     const int I = t / nb, J = t - I * nb;
     if (I > J) continue;
-    const int i0 = 16 * I, j0 = 16 * J;
-    double v[4];
-    int rows[4];
+    const int i0 = 32 * I, j0 = 32 * J;
+    double v[4][4];
+    int rows[4][4];
     if constexpr (sizeof(T) == 8) {
       auto c_ik = [&](int i, int k) { return (i < n && k < nocc) ? C[(int64_t)i * ld + k] : 0.0; };
       auto c_kj = [&](int k, int j) { return (k < nocc && j < n) ? C[(int64_t)j * ld + k] : 0.0; };
-      const dbl4 s = tile16_f64(nocc, i0, j0, c_ik, c_kj);
+      dbl4 x[4];
+      block32_f64(nocc, i0, j0, c_ik, c_kj, x);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        v[u] = wgt * s[u];
-        rows[u] = tile_row_f64(q, u);
-      }
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          v[s][u] = wgt * x[s][u];
+          rows[s][u] = tile_row_f64(q, u);
+        }
     } else {
       auto c_ik = [&](int i, int k) { return (i < n && k < nocc) ? (float)C[(int64_t)i * ld + k] : 0.0f; };
       auto c_kj = [&](int k, int j) { return (k < nocc && j < n) ? (float)C[(int64_t)j * ld + k] : 0.0f; };
-      const flt4 s = tile16_f32(nocc, i0, j0, c_ik, c_kj);
+      flt4 x[4];
+      block32_f32(nocc, i0, j0, c_ik, c_kj, x);
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          v[s][u] = wgt * (double)x[s][u];
+          rows[s][u] = tile_row_f32(q, u);
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        v[u] = wgt * (double)s[u];
-        rows[u] = tile_row_f32(q, u);
+        const int i = i0 + 16 * (s >> 1) + rows[s][u], j = j0 + 16 * (s & 1) + r;
+        if (i >= n || j >= n) continue;
+        Pn[(int64_t)i * ld + j] = v[s][u];
+        if (I != J) Pn[(int64_t)j * ld + i] = v[s][u];
+        const double d = v[s][u] - Pold[(int64_t)i * ld + j];
+        dsq += (I == J) ? d * d : 2.0 * d * d;
       }
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int i = i0 + rows[u], j = j0 + r;
-      if (i >= n || j >= n) continue;
-      Pn[(int64_t)i * ld + j] = v[u];
-      if (I != J) Pn[(int64_t)j * ld + i] = v[u];
-      const double d = v[u] - Pold[(int64_t)i * ld + j];
-      dsq += (I == J) ? d * d : 2.0 * d * d;
-    }
   }
   return wg_sum<NT>(dsq, red);
 }
@@ -153,6 +164,63 @@ __device__ void copy_mat(int ld, const double* src, double* dst) {
   __syncthreads();
 }
 
+// DIIS history dots of the newest slot h against every live slot t (s_slot[0..cnt)):
+// <E_h, E_t>, <D_h, F_t> and <D_t, F_h> over the padded ld x ld matrices, into out[3 q + k] (LDS,
+// written by thread 0). A function of its own so its 3 x HIST partial sums do not add to the
+// registers of the rest of the megakernel.
+template <int NT>
+__device__ __attribute__((noinline)) void history_dots(int ld, int cnt, const int* s_slot, const double* __restrict__ E,
+                                                       const double* __restrict__ P, const double* __restrict__ F,
+                                                       const double* __restrict__ hE, const double* __restrict__ hF,
+                                                       const double* __restrict__ hD, double* red, double* out) {
+  double acc[3 * HIST];
+  for (int k = 0; k < 3 * HIST; ++k) acc[k] = 0.0;
+  const int64_t sq = (int64_t)ld * ld;
+  for (int idx = threadIdx.x; idx < ld * ld; idx += NT) {
+    const double eh0 = E[idx], dh = P[idx], fh = F[idx];
+#pragma unroll
+    for (int q = 0; q < HIST; ++q) {
+      if (q < cnt) {
+        const int64_t off = (int64_t)s_slot[q] * sq + idx;
+        acc[3 * q] += eh0 * hE[off];
+        acc[3 * q + 1] += dh * hF[off];
+        acc[3 * q + 2] += hD[off] * fh;
+      }
+    }
+  }
+  wg_sum_n<NT, 3 * HIST>(acc, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3 * HIST; ++k) out[k] = acc[k];
+}
+
+// The convergence test of one SCF iteration (thread 0): the caller's (|dE| < e_tol and RMS dP <
+// p_tol, or a zero shift and a commutator norm below 1e-7; src/scf.rs:1274-1287). A mixed run's
+// FP64 tail asks for MIXED_TAIL times both bounds; once the caller's test is met it goes on
+// while it progresses (tail.extra iterations so far, tail.stall of them without a 20 % drop of
+// dp below tail.best) and stops, converged, on the third stalled one or after MIXED_TAIL_EXTRA.
+struct TailState {
+  int extra, stall;
+  double best;
+};
+
+__device__ inline bool tail_converged(const MolDev& M, int iter, double de, double dp, double err, double shift,
+                                      bool low, TailState& tail) {
+  const bool resid = shift == 0.0 && err < 1.0e-7;
+  const bool plain = iter > 0 && de < M.e_tol && (dp < M.p_tol || resid);
+  if (M.precision != 1 || low) return plain;
+  const bool tight = iter > 0 && de < M.e_tol &&
+                     (dp < M.p_tol * MIXED_TAIL || (shift == 0.0 && err < 1.0e-7 * MIXED_TAIL));
+  if (tight) return true;
+  if (!plain) return false;
+  if (tail.extra++ == 0 || dp < 0.8 * tail.best) {
+    tail.best = dp;
+    tail.stall = 0;
+  } else {
+    ++tail.stall;
+  }
+  return tail.stall >= 3 || tail.extra > MIXED_TAIL_EXTRA;
+}
+
 // max |A - B| over the n x n block.
 template <int NT>
 __device__ double max_abs_diff(int n, int ld, const double* A, const double* B, double* red) {
@@ -182,10 +250,14 @@ __global__ void __launch_bounds__(NT)
   __shared__ int s_pq[MAXN / 2];  // this round's rotation pairs, p | q << 16
   __shared__ double s_G[HIST * HIST], s_M[HIST * HIST], s_coef[HIST + 1];
   __shared__ int s_slot[HIST], s_cslot[HIST];
+  __shared__ double s_dots[3 * HIST];
+  __shared__ MolOut s_res;                   // the molecule's result record (thread 0 keeps it)
+  __shared__ unsigned long long s_tk[NPHASE];  // per-phase device clocks (thread 0)
   __shared__ double s_wk[(HIST + 1) * (HIST + 3)];  // bordered DIIS system
   __shared__ int s_int[8];     // 0 head, 1 cnt, 2 ncoef, 3 done, 4 reduced-precision phase
   __shared__ double s_dbl[8];  // 0 e_old, 1 err, 2 dp
   __shared__ Guard s_guard;
+  __shared__ TailState s_tail;
 
   const int mi = mol_list[blockIdx.x];
   MolCtx c;
@@ -212,29 +284,38 @@ __global__ void __launch_bounds__(NT)
     s_int[0] = 0;
     s_int[1] = 0;
     s_int[3] = 0;
+    s_tail.extra = 0;  // mixed tail: iterations past the caller's convergence test
+    s_tail.stall = 0;
+    s_tail.best = 1.0e30;
     s_dbl[0] = 0.0;
     s_dbl[2] = 1.0e30;
   }
   __syncthreads();
   double* H = c.slot(S_H);
-  MolOut res;
-  unsigned long long tk[NPHASE] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // The result record and the phase clocks live in LDS and are written by thread 0 only, so they
+  // hold no registers through the SCF; e_el is computed by every thread (uniform) and stored last.
+  MolOut& res = s_res;
+  unsigned long long* tk = s_tk;
   unsigned long long t_mark = wall_clock64();
   const unsigned long long t_start = t_mark;
   // Attribute the time since the last mark to phase `ph` (thread 0 keeps the books).
   auto lap = [&](int ph) {
     const unsigned long long now = wall_clock64();
-    tk[ph] += now - t_mark;
+    if (threadIdx.x == 0) tk[ph] += now - t_mark;
     t_mark = now;
   };
-  res.converged = 0;
-  res.iterations = 0;
-  res.dp_err = INFINITY;
-  res.err_norm = 0.0;
-  res.fp32_iters = 0;
-  res.e_fp32_last = 0.0;
-  res.check_de = 0.0;
-  res.check_dp = 0.0;
+  double e_el = 0.0;
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < NPHASE; ++k) tk[k] = 0;
+    res.converged = 0;
+    res.iterations = 0;
+    res.dp_err = INFINITY;
+    res.err_norm = 0.0;
+    res.fp32_iters = 0;
+    res.e_fp32_last = 0.0;
+    res.check_de = 0.0;
+    res.check_dp = 0.0;
+  }
   // A mixed run starts in reduced precision only when the core Hamiltonian stays within the range
   // where FP32 rounding (6e-8 relative) is far below the 1e-5 hand-over: max |H_ij| <= 1e3 eV.
   // (Pseudo-atoms such as MOPAC's capped-bond element carry |beta| up to 1e7 eV.)
@@ -305,23 +386,8 @@ __global__ void __launch_bounds__(NT)
       const int cnt = s_int[1];
       // History dots: <E_h,E_t>, <D_h,F_t>, <D_t,F_h> for every live slot t.
       {
-        double acc[3 * HIST];
-        for (int k = 0; k < 3 * HIST; ++k) acc[k] = 0.0;
-        const double* Fh = F;
-        for (int idx = threadIdx.x; idx < ld * ld; idx += NT) {
-          const double eh0 = E[idx], dh = P[idx], fh = Fh[idx];
-#pragma unroll
-          for (int q = 0; q < HIST; ++q) {
-            if (q < cnt) {
-              const int t = s_slot[q];
-              const int64_t off = (int64_t)t * ld * ld + idx;
-              acc[3 * q] += eh0 * c.slot(S_HE)[off];
-              acc[3 * q + 1] += dh * c.slot(S_HF)[off];
-              acc[3 * q + 2] += c.slot(S_HD)[off] * fh;
-            }
-          }
-        }
-        wg_sum_n<NT, 3 * HIST>(acc, s_red);
+        history_dots<NT>(ld, cnt, s_slot, E, P, F, c.slot(S_HE), c.slot(S_HF), c.slot(S_HD), s_red, s_dots);
+        const double* acc = s_dots;  // thread 0 wrote them and reads them
         if (threadIdx.x == 0) {
           for (int q = 0; q < cnt; ++q) {
             const int t = s_slot[q];
@@ -388,9 +454,7 @@ __global__ void __launch_bounds__(NT)
         const double de = fabs(e - s_dbl[0]);
         s_dbl[0] = e;
         s_dbl[2] = dp;
-        const double tail = c.M.precision == 1 ? MIXED_TAIL : 1.0;
-        const bool resid_ok = s_guard.shift == 0.0 && err < 1.0e-7 * tail;
-        const bool conv = iter > 0 && de < c.M.e_tol && (dp < c.M.p_tol * tail || resid_ok);
+        const bool conv = tail_converged(c.M, iter, de, dp, err, s_guard.shift, low, s_tail);
         if (low) {
           // Reduced-precision phase: never converges here; hands over to FP64 below the switch.
           ++res.fp32_iters;
@@ -417,10 +481,12 @@ __global__ void __launch_bounds__(NT)
         break;
       }
     }
-    res.iterations = iter > c.M.max_scf ? c.M.max_scf : iter;
-    res.converged = s_int[3];
-    res.dp_err = s_dbl[2];
-    res.err_norm = s_dbl[1];
+    if (threadIdx.x == 0) {
+      res.iterations = iter > c.M.max_scf ? c.M.max_scf : iter;
+      res.converged = s_int[3];
+      res.dp_err = s_dbl[2];
+      res.err_norm = s_dbl[1];
+    }
     // Final energy from the converged density.
     int pslot = S_HD + s_int[0];
     double* P = c.slot(pslot);
@@ -435,7 +501,7 @@ __global__ void __launch_bounds__(NT)
         eh[1] += Pf[x] * F[x];
       }
       wg_sum_n<NT, 2>(eh, s_red);
-      res.e_el = 0.5 * (eh[0] + eh[1]);
+      e_el = 0.5 * (eh[0] + eh[1]);
     };
     final_energy(P);
     // A mixed run always ends with a full FP64 Roothaan step; its size is the result's stated
@@ -450,18 +516,23 @@ __global__ void __launch_bounds__(NT)
         pslot = S_HD + (s_int[0] + 1) % HIST;
         double* P2 = c.slot(pslot);
         (void)density<NT, double>(n, ld, nocc, 2.0, c.slot(cbuf ? S_C0 : S_C1), P, P2, s_red);
-        const double e_conv = res.e_el;
+        const double e_conv = e_el;
         final_energy(P2);
         if (check) {
-          res.check_de = fabs(res.e_el - e_conv);
-          res.check_dp = max_abs_diff<NT>(n, ld, P2, P, s_red);
+          const double dpm = max_abs_diff<NT>(n, ld, P2, P, s_red);
+          if (threadIdx.x == 0) {
+            res.check_de = fabs(e_el - e_conv);
+            res.check_dp = dpm;
+          }
         }
       }
     }
-    res.p_slot = pslot;
-    res.c_slot = cbuf ? S_C0 : S_C1;
-    res.pb_slot = -1;
-    res.cb_slot = -1;
+    if (threadIdx.x == 0) {
+      res.p_slot = pslot;
+      res.c_slot = cbuf ? S_C0 : S_C1;
+      res.pb_slot = -1;
+      res.cb_slot = -1;
+    }
   } else {
     // ------------------------------------------------------------------ UHF (no DIIS)
     const int na = c.M.n_occ, nb = c.M.n_occ_b;
@@ -528,9 +599,7 @@ __global__ void __launch_bounds__(NT)
         s_dbl[0] = e;
         s_dbl[2] = dp;
         s_dbl[1] = err;
-        const double tail = c.M.precision == 1 ? MIXED_TAIL : 1.0;
-        const bool resid_ok = s_guard.shift == 0.0 && err < 1.0e-7 * tail;
-        const bool conv = iter > 0 && de < c.M.e_tol && (dp < c.M.p_tol * tail || resid_ok);
+        const bool conv = tail_converged(c.M, iter, de, dp, err, s_guard.shift, low, s_tail);
         if (low) {
           ++res.fp32_iters;
           res.e_fp32_last = e;
@@ -562,10 +631,12 @@ __global__ void __launch_bounds__(NT)
         break;
       }
     }
-    res.iterations = iter > c.M.max_scf ? c.M.max_scf : iter;
-    res.converged = s_int[3];
-    res.dp_err = s_dbl[2];
-    res.err_norm = s_dbl[1];
+    if (threadIdx.x == 0) {
+      res.iterations = iter > c.M.max_scf ? c.M.max_scf : iter;
+      res.converged = s_int[3];
+      res.dp_err = s_dbl[2];
+      res.err_norm = s_dbl[1];
+    }
     for (int idx = threadIdx.x; idx < ld * ld; idx += NT) Pt[idx] = Pa[idx] + Pb[idx];
     __syncthreads();
     fock_build<NT, double>(c, Pt, Pa, 1.0, Pb, 1.0, Fa, Fb);
@@ -578,7 +649,7 @@ __global__ void __launch_bounds__(NT)
       eh[2] += Pb[x] * Fb[x];
     }
     wg_sum_n<NT, 3>(eh, s_red);
-    res.e_el = 0.5 * (eh[0] + eh[1] + eh[2]);
+    e_el = 0.5 * (eh[0] + eh[1] + eh[2]);
     // The last loop solve left alpha/beta vectors in buffer (cbuf ^ 1).
     int ca_last = cbuf ? U_CA0 : U_CA1, cb_last = cbuf ? U_CB0 : U_CB1;
     if (s_guard.shift > 0.0 || last_rot) {
@@ -601,9 +672,11 @@ __global__ void __launch_bounds__(NT)
       // Densities and energy of the reported orbitals (see the RHF branch).
       (void)density<NT, double>(n, ld, na, 1.0, c.slot(ca_last), Pa, Pan, s_red);
       (void)density<NT, double>(n, ld, nb, 1.0, c.slot(cb_last), Pb, Pbn, s_red);
-      if (check)
-        res.check_dp = fmax(max_abs_diff<NT>(n, ld, Pan, Pa, s_red), max_abs_diff<NT>(n, ld, Pbn, Pb, s_red));
-      const double e_conv = res.e_el;
+      if (check) {
+        const double dpm = fmax(max_abs_diff<NT>(n, ld, Pan, Pa, s_red), max_abs_diff<NT>(n, ld, Pbn, Pb, s_red));
+        if (threadIdx.x == 0) res.check_dp = dpm;
+      }
+      const double e_conv = e_el;
       double* tmp = Pa;
       Pa = Pan;
       Pan = tmp;
@@ -622,19 +695,22 @@ __global__ void __launch_bounds__(NT)
         e3[2] += Pb[x] * Fb[x];
       }
       wg_sum_n<NT, 3>(e3, s_red);
-      res.e_el = 0.5 * (e3[0] + e3[1] + e3[2]);
-      if (check) res.check_de = fabs(res.e_el - e_conv);
+      e_el = 0.5 * (e3[0] + e3[1] + e3[2]);
+      if (check && threadIdx.x == 0) res.check_de = fabs(e_el - e_conv);
     }
-    res.p_slot = (Pa == c.slot(U_PA)) ? U_PA : U_PAN;
-    res.pb_slot = (Pb == c.slot(U_PB)) ? U_PB : U_PBN;
-    res.c_slot = ca_last;
-    res.cb_slot = cb_last;
+    if (threadIdx.x == 0) {
+      res.p_slot = (Pa == c.slot(U_PA)) ? U_PA : U_PAN;
+      res.pb_slot = (Pb == c.slot(U_PB)) ? U_PB : U_PBN;
+      res.c_slot = ca_last;
+      res.cb_slot = cb_last;
+    }
   }
   if (threadIdx.x == 0) {
     tk[PH_TOTAL] = wall_clock64() - t_start;
     for (int k = 0; k < NPHASE; ++k) res.ticks[k] = tk[k];
     res.shift = s_guard.shift;
     res.sweeps = sweeps;
+    res.e_el = e_el;
     out[mi] = res;
   }
 }

@@ -251,6 +251,11 @@ struct pm7_ctx {
   // 1.3e-7); against 1e-26 it saves 0.25 sweeps per solve (FP64 organic batch 7.40k -> 7.57k
   // solves/s, S/P/Cl batch 415 -> 437/s).
   int eig_tol_exp = 22;
+  // A batch holding at most GRID_FEW molecules above grid_min_nao basis functions (in the
+  // largest megakernel class) runs them on the grid-wide path: one such molecule alone on one CU
+  // takes 210 ms at 194 AOs against 58 ms grid-wide, while many of them side by side keep the
+  // device busier as megakernels (environment PM7_AMD_GRID_MIN_NAO, 32..196, default 128).
+  int grid_min_nao = 128;
   // Allocation-free entry point: its lane's buffers live in the device's static scratch array.
   Arena arena;
   Lane na_lane;
@@ -524,6 +529,13 @@ pm7_status prepare(pm7_ctx* ctx, Lane& L, Prepared& P, int64_t max_memory_mb) {
   int atom0 = 0;
   P.mols.resize(nm);
   std::vector<int> lists[SCF_VARIANTS];
+  {
+    int few = 0;
+    for (const Job& j : P.jobs) few += (j.variant == LARGE_VARIANT - 1 && j.nao > ctx->grid_min_nao) ? 1 : 0;
+    if (few > 0 && few <= GRID_FEW)
+      for (Job& j : P.jobs)
+        if (j.variant == LARGE_VARIANT - 1 && j.nao > ctx->grid_min_nao) j.variant = LARGE_VARIANT;
+  }
   for (int mi = 0; mi < nm; ++mi) {
     Job& j = P.jobs[mi];
     const int n = (int)j.z.size();
@@ -956,6 +968,9 @@ pm7_status launch_batch(Lane& L, const Prepared& P, int* count) {
 pm7_status run_large(Lane& L, const Prepared& P) {
   const int off = P.list_off[LARGE_VARIANT], cnt = P.list_off[LARGE_VARIANT + 1] - off;
   if (cnt == 0) return PM7_OK;
+  // The class's timing events bracket all of its work, one molecule at a time and grouped.
+  hipStream_t ls = L.st[LARGE_VARIANT];
+  LANECHK(hipEventRecord(L.ev_var[2 * LARGE_VARIANT], ls));
   std::vector<std::pair<int, int>> order;  // largest first
   for (int mi = 0; mi < (int)P.jobs.size(); ++mi)
     if (P.jobs[mi].variant == LARGE_VARIANT) order.push_back({-P.jobs[mi].nao, mi});
@@ -973,8 +988,7 @@ pm7_status run_large(Lane& L, const Prepared& P) {
         return PM7_ERR_DEVICE;
       }
     }
-    hipStream_t s = L.st[LARGE_VARIANT];
-    LANECHK(hipEventRecord(L.ev_var[2 * LARGE_VARIANT], s));
+    hipStream_t s = ls;
     for (size_t k = 0; k < single; ++k) {
       const int mi = order[k].second;
       MolOut out{};
@@ -982,7 +996,6 @@ pm7_status run_large(Lane& L, const Prepared& P) {
       LANECHK(hipMemcpyAsync(P.b.out + mi, &out, sizeof out, hipMemcpyHostToDevice, s));
       LANECHK(hipStreamSynchronize(s));
     }
-    LANECHK(hipEventRecord(L.ev_var[2 * LARGE_VARIANT + 1], s));
   }
   const int group = large_group_size();
   for (size_t g0 = single; g0 < order.size(); g0 += group) {
@@ -1026,6 +1039,7 @@ pm7_status run_large(Lane& L, const Prepared& P) {
         return PM7_ERR_DEVICE;
       }
   }
+  LANECHK(hipEventRecord(L.ev_var[2 * LARGE_VARIANT + 1], ls));  // every group thread synchronised its stream
   return PM7_OK;
 }
 
@@ -1223,6 +1237,9 @@ pm7_status run_jobs(pm7_ctx* ctx, std::vector<Job>& jobs, pm7_result_v2* results
     if (count[v] > 0 && hipEventElapsedTime(&ms, L.ev_var[2 * v], L.ev_var[2 * v + 1]) == hipSuccess)
       ctx->timings[4 + v] = ms;
   }
+  // A timing query that failed (an event of a class that did not run) must not stay behind as
+  // this thread's last error, where the next call's first launch check would report it.
+  (void)hipGetLastError();
   const auto t_end = std::chrono::steady_clock::now();
   using msd = std::chrono::duration<double, std::milli>;
   ctx->timings[3] = msd(t_end - t0).count();
@@ -1574,6 +1591,10 @@ pm7_status pm7_ctx_create(int32_t device, const char* data_dir, pm7_ctx** out) {
     const double v = std::strtod(rb, nullptr);
     if (v >= 1.0e-6 && v <= 1.0e-1) ctx->rotate_below = v;
   }
+  if (const char* gm = std::getenv("PM7_AMD_GRID_MIN_NAO")) {
+    const int v = std::atoi(gm);
+    if (v >= 32 && v <= VARIANT_MAX_NAO[LARGE_VARIANT - 1]) ctx->grid_min_nao = v;
+  }
   if (const char* te = std::getenv("PM7_AMD_EIG_TOL_EXP")) {
     const int v = std::atoi(te);
     if (v >= 16 && v <= 30) ctx->eig_tol_exp = v;
@@ -1900,6 +1921,7 @@ pm7_status response_fock_impl(pm7_ctx* ctx, const pm7_molecule_v1* mol, int32_t 
   float ms = 0.0f;
   for (double& t : ctx->timings) t = 0.0;
   if (hipEventElapsedTime(&ms, L.ev[0], L.ev[1]) == hipSuccess) ctx->timings[1] = ms;
+  (void)hipGetLastError();  // as in run_jobs: a failed timing query is not the next call's error
   return PM7_OK;
 }
 

@@ -53,6 +53,7 @@ constexpr int GL_WORDS = 2 * 48 + 2 * 40;
 constexpr int SCF_VARIANTS = 4;
 constexpr int VARIANT_MAX_NAO[SCF_VARIANTS] = {64, 112, 196, 1 << 30};
 constexpr int LARGE_VARIANT = 3;  // above 196 basis functions: the grid-wide path (large.hip)
+constexpr int GRID_FEW = 8;        // at most this many large-class-2 molecules of a batch go grid-wide
 // Large-class molecules up to this basis size run in groups of up to large_group_size() SCFs whose full
 // eigensolves are batched (one strided rocSOLVER call per iteration for the whole group).
 constexpr int LARGE_GROUP_MAX_NAO = 512;

@@ -849,6 +849,7 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
       float ms;
       for (int k = 0; k < 5; ++k)
         if (hipEventElapsedTime(&ms, lw->ev[k], lw->ev[k + 1]) == hipSuccess) phase[k] += ms;
+      (void)hipGetLastError();  // a failed timing query must not become the next launch's error
       low = host.low;
       dp_prev = host.dp;
       if (host.done) {
@@ -963,6 +964,7 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
       float ms;
       for (int k = 0; k < 5; ++k)
         if (hipEventElapsedTime(&ms, lw->ev[k], lw->ev[k + 1]) == hipSuccess) phase[k] += ms;
+      (void)hipGetLastError();  // a failed timing query must not become the next launch's error
       std::swap(Pa, Pan);
       std::swap(Pb, Pbn);
       low = host.low;

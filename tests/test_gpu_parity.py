@@ -335,6 +335,22 @@ def test_large_group_matches_single_runs(eng):
     assert np.max(np.abs(group[0].density - dens)) <= 1e-7
 
 
+def test_call_after_a_grouped_large_batch_on_a_fresh_context():
+    """A context whose first call runs only grouped large-class SCFs (two molecules of 197-512 basis
+    functions) and whose next call is an ordinary batch: the timing of the first call must leave no
+    error behind for the second (the large class's timing events were once recorded only on the
+    one-at-a-time branch)."""
+    rng = W.Rng(21)
+    atoms = [W.hetero_chain(rng, nc, het) for nc, het in ((34, "Cl"), (40, "S"))]
+    say("fresh context: grouped large batch, then a small batch")
+    with pm7amd.Engine(0) as e:
+        res = e.run_batch([mol_of(a) for a in atoms], pm7amd.Pm7Options(), raise_on_error=True)
+        assert all(r.converged for r in res)
+        again = e.run_batch([mol_of(W.water()), mol_of(CASES["benzene"]["atoms"])], pm7amd.Pm7Options(),
+                            raise_on_error=True)
+        assert abs(again[0].total_ev - oracle_of(W.water())[0].total_ev) <= 1e-6
+
+
 def test_config1_water(eng):
     say("config 1 water")
     atoms = W.water()

@@ -225,6 +225,76 @@ pm7_status pm7_response_fock_spin_batch(pm7_ctx* ctx, const pm7_molecule_v1* mol
 /* Replaces core_core_energy (src/repulsion.rs:16-36), eV. */
 pm7_status pm7_core_core_energy(pm7_ctx* ctx, const pm7_molecule_v1* mol, double* e_out);
 
+/* Divide-and-conquer SCF (pm7-rs DandcOptions, src/dandc/mod.rs:57-116; defaults from
+ * pm7_dandc_default_options, src/dandc/mod.rs:77-89). */
+typedef struct {
+  double buffer_bohr;    /* buffer radius around each core, Bohr (default 15) */
+  int32_t core_size;     /* target atoms per core (default 12) */
+  int32_t max_scf;       /* default 200 */
+  double fermi_width_ev; /* Fermi smearing of the common chemical potential (default 0.1 eV) */
+  double p_tol;          /* RMS change over the stored density elements (default 1e-6) */
+  double mixing;         /* linear density mixing, in (0, 1] (default 0.4) */
+} pm7_dandc_options_v1;
+
+/* pm7-rs DandcResult (src/dandc/mod.rs:117-135). The density is stored as atom-pair blocks, CSR
+ * over atoms (pm7-rs SparseDensity, src/dandc/sparse.rs): the blocks of atom a are slots
+ * row_start[a] .. row_start[a+1]-1, slot k is the pair (a, columns[k]) and holds the
+ * norb[a] x norb[columns[k]] block row-major at density + offsets[k]. The arrays are allocated by
+ * the library (one allocation) and released by pm7_dandc_result_free. */
+typedef struct {
+  int32_t status;
+  int32_t converged;
+  int32_t iterations;
+  int32_t unrestricted;
+  int32_t subsystems;
+  int32_t largest_subsystem;      /* atoms */
+  int32_t largest_subsystem_nao;  /* basis functions */
+  int32_t n_atoms;
+  int64_t stored_pairs;
+  int64_t stored_elements;
+  double fermi_ev;                /* common chemical potential */
+  double electronic_ev;           /* sum of the subsystems' weighted energies */
+  double density_error;           /* last RMS density change */
+  double device_ms;               /* device time of the SCF steps (HIP events) */
+  double needed_mb, budget_mb;    /* PM7_ERR_INSUFFICIENT_MEMORY */
+  int64_t* row_start;             /* n_atoms + 1 */
+  int32_t* columns;               /* stored_pairs */
+  int64_t* offsets;               /* stored_pairs + 1 */
+  int32_t* norb;                  /* n_atoms */
+  double* density;                /* stored_elements */
+  double* spin_density;           /* stored_elements, UHF only (else NULL) */
+  double* charges;                /* n_atoms Mulliken net charges (mulliken_charges, src/dandc/mod.rs:823-835) */
+  char message[512];
+} pm7_dandc_result_v1;
+
+void pm7_dandc_default_options(pm7_dandc_options_v1* dopts);
+
+/* Replaces run_dandc (src/dandc/mod.rs:138-420). Every subsystem's Fock build, eigensolve and
+ * weighted density of a step run as one launch each (one workgroup per subsystem, integrals and
+ * eigenvectors resident across steps); the global density, the environment field, the common
+ * Fermi level and the mixing stay on the device. Subsystems above 196 basis functions are refused
+ * (PM7_ERR_UNSUPPORTED) with the subsystem's atoms and orbitals named, as is a partition whose
+ * device working set exceeds the budget (PM7_ERR_INSUFFICIENT_MEMORY). Like the reference, an
+ * unconverged run returns PM7_OK with converged = 0. */
+pm7_status pm7_run_dandc(pm7_ctx* ctx, const pm7_molecule_v1* mol, const pm7_options_v2* opts,
+                         const pm7_dandc_options_v1* dopts, pm7_dandc_result_v1* out);
+void pm7_dandc_result_free(pm7_dandc_result_v1* res);
+
+/* Replaces partition (src/dandc/partition.rs:97-133), host only (no context, no device): recursive
+ * median bisection into cores of at most core_size atoms, each grown by every atom within
+ * buffer_bohr of one of its atoms. Subsystem s holds members[start[s] .. start[s+1]-1], its
+ * n_core[s] core atoms first, then its buffer atoms in ascending index order. Released by
+ * pm7_partition_free. */
+typedef struct {
+  int32_t n_sub;
+  int32_t* n_core;   /* n_sub */
+  int64_t* start;    /* n_sub + 1 */
+  int32_t* members;  /* start[n_sub] */
+} pm7_partition_v1;
+pm7_status pm7_dandc_partition(const pm7_molecule_v1* mol, int32_t core_size, double buffer_bohr,
+                               pm7_partition_v1* out);
+void pm7_partition_free(pm7_partition_v1* part);
+
 /* Request server: continuous batching of SCF requests (engine extension; no pm7-rs counterpart,
  * it serves the callers that would otherwise call run_pm7 once per request). A scheduler thread
  * admits queued requests, in arrival order, into up to `lanes` batches in flight at once (each on

@@ -3191,6 +3191,433 @@ void set_msg(oracle_result* out, const std::string& s) {
   std::snprintf(out->message, sizeof out->message, "%s", s.c_str());
 }
 
+// ---------------------------------------------------------------------------------------------
+// Divide-and-conquer SCF (src/dandc/mod.rs, src/dandc/partition.rs, src/dandc/sparse.rs),
+// molecular case. Written for clarity: brute-force buffer search, dense per-subsystem algebra.
+// ---------------------------------------------------------------------------------------------
+struct DcSub {
+  std::vector<int> parent;  // core atoms (bisection order), then buffer atoms (ascending)
+  int n_core = 0;
+  int size() const { return (int)parent.size(); }
+  double weight(int i, int j) const {  // src/dandc/partition.rs:60-67
+    const bool a = i < n_core, b = j < n_core;
+    return (a && b) ? 1.0 : ((a || b) ? 0.5 : 0.0);
+  }
+};
+
+inline double axis_of(const V3& p, int a) { return a == 0 ? p.x : (a == 1 ? p.y : p.z); }
+
+// src/dandc/partition.rs:136-173: split at the median of a stable sort along the axis of largest
+// extent (Iterator::max_by keeps the last of equal maxima), lower half first.
+void dc_bisect(std::vector<int> idx, const std::vector<V3>& pos, size_t target, std::vector<std::vector<int>>& out) {
+  if (idx.size() <= target || idx.size() < 2) {
+    out.push_back(std::move(idx));
+    return;
+  }
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int i : idx)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(lo[a], axis_of(pos[i], a));
+      hi[a] = std::max(hi[a], axis_of(pos[i], a));
+    }
+  int axis = 0;
+  for (int a = 1; a < 3; ++a)
+    if (hi[a] - lo[a] >= hi[axis] - lo[axis]) axis = a;
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return axis_of(pos[a], axis) < axis_of(pos[b], axis); });
+  const size_t half = idx.size() / 2;
+  std::vector<int> right(idx.begin() + half, idx.end());
+  idx.resize(half);
+  dc_bisect(std::move(idx), pos, target, out);
+  dc_bisect(std::move(right), pos, target, out);
+}
+
+// src/dandc/partition.rs:97-133, 175-222 (molecule: the one image t = 0).
+std::vector<DcSub> dc_partition(const std::vector<V3>& pos, int core_size, double buffer) {
+  const int n = (int)pos.size();
+  std::vector<int> all(n);
+  for (int i = 0; i < n; ++i) all[i] = i;
+  std::vector<std::vector<int>> cores;
+  dc_bisect(all, pos, (size_t)std::max(1, core_size), cores);
+  std::vector<DcSub> subs;
+  const double cut2 = buffer * buffer;
+  for (auto& c : cores) {
+    DcSub s;
+    s.parent = c;
+    s.n_core = (int)c.size();
+    std::vector<char> in_core(n, 0);
+    for (int i : c) in_core[i] = 1;
+    for (int b = 0; b < n; ++b) {
+      if (in_core[b]) continue;
+      for (int i : c) {
+        V3 d = sub(pos[b], pos[i]);
+        if (dot(d, d) <= cut2) {
+          s.parent.push_back(b);
+          break;
+        }
+      }
+    }
+    subs.push_back(std::move(s));
+  }
+  return subs;
+}
+
+// Atom-pair block storage, CSR over atoms with sorted columns (src/dandc/sparse.rs).
+struct DcSparse {
+  std::vector<int> norb, cols;
+  std::vector<long> row_start, offs;
+  std::vector<double> data;
+  long slot(int a, int b) const {
+    auto lo = cols.begin() + row_start[a], hi = cols.begin() + row_start[a + 1];
+    auto it = std::lower_bound(lo, hi, b);
+    return (it != hi && *it == b) ? (long)(it - cols.begin()) : -1;
+  }
+  double* block(int a, int b) {
+    long k = slot(a, b);
+    return k < 0 ? nullptr : data.data() + offs[k];
+  }
+  const double* block(int a, int b) const {
+    long k = slot(a, b);
+    return k < 0 ? nullptr : data.data() + offs[k];
+  }
+  double population(int a) const {
+    const double* d = block(a, a);
+    double s = 0.0;
+    if (d)
+      for (int i = 0; i < norb[a]; ++i) s += d[i * norb[a] + i];
+    return s;
+  }
+};
+
+struct DcSolution {
+  std::vector<double> eps, eps_b;
+  Mat c, c_b;
+  bool beta = false;
+  double energy = 0.0;
+};
+
+struct DcOut {
+  int status = 0, converged = 0, iterations = 0, unrestricted = 0, subsystems = 0, largest = 0, largest_nao = 0;
+  long stored_pairs = 0, stored_elements = 0;
+  double fermi = 0.0, e_el = 0.0, dp = INFINITY;
+  std::string msg;
+  DcSparse density, spin;
+};
+
+inline double dc_occupation(double e, double fermi, double width) {  // src/dandc/mod.rs:649-660
+  const double x = (e - fermi) / width;
+  if (x > 40.0) return 0.0;
+  if (x < -40.0) return 2.0;
+  return 2.0 / (1.0 + std::exp(x));
+}
+
+DcOut dandc_run(const Params& P, const Mol& m, const oracle_options& o, double buffer, int core_size, double width,
+                int max_scf, double p_tol, double mixing) {
+  DcOut R;
+  const int n = m.n();
+  std::vector<DcSub> subs = dc_partition(m.pos, core_size, buffer);
+  R.subsystems = (int)subs.size();
+  for (auto& s : subs) {
+    int nao = 0;
+    for (int a : s.parent) nao += m.norb[a];
+    R.largest = std::max(R.largest, s.size());
+    R.largest_nao = std::max(R.largest_nao, nao);
+  }
+  // Pattern and normaliser (src/dandc/mod.rs:218-258).
+  std::vector<std::vector<int>> nbr(n);
+  for (auto& s : subs)
+    for (int i = 0; i < s.size(); ++i)
+      for (int j = 0; j < s.size(); ++j)
+        if (s.weight(i, j) > 0.0) nbr[s.parent[i]].push_back(s.parent[j]);
+  DcSparse pat;
+  pat.norb = m.norb;
+  pat.row_start.assign(n + 1, 0);
+  long total = 0;
+  for (int a = 0; a < n; ++a) {
+    auto l = nbr[a];
+    std::sort(l.begin(), l.end());
+    l.erase(std::unique(l.begin(), l.end()), l.end());
+    for (int b : l) {
+      pat.cols.push_back(b);
+      pat.offs.push_back(total);
+      total += (long)m.norb[a] * m.norb[b];
+    }
+    pat.row_start[a + 1] = (long)pat.cols.size();
+  }
+  pat.offs.push_back(total);
+  pat.data.assign(total, 0.0);
+  R.stored_pairs = (long)pat.cols.size();
+  R.stored_elements = total;
+  std::vector<double> wsum(pat.cols.size(), 0.0);
+  for (auto& s : subs)
+    for (int i = 0; i < s.size(); ++i)
+      for (int j = 0; j < s.size(); ++j) {
+        const double w = s.weight(i, j);
+        if (w > 0.0) wsum[pat.slot(s.parent[i], s.parent[j])] += w;
+      }
+  std::vector<double> normalizer(wsum.size());
+  for (size_t k = 0; k < wsum.size(); ++k) normalizer[k] = wsum[k] > 1.0e-12 ? 1.0 / wsum[k] : 0.0;
+  // Electron count, reference, starting densities (src/dandc/mod.rs:260-306).
+  double tore = 0.0;
+  for (int a = 0; a < n; ++a) tore += m.el[a]->core_charge;
+  const double n_elec = tore - o.charge;
+  const bool uhf = o.reference == 2 || (o.reference == 0 && o.multiplicity > 1);
+  R.unrestricted = uhf;
+  const double nbe = (o.multiplicity - 1.0) * 0.5;
+  DcSparse dens = pat, spin = pat;
+  {
+    Mat guess = sad_density(m, n_elec);
+    for (int a = 0; a < n; ++a) {
+      double* d = dens.block(a, a);
+      const int na = m.norb[a], off = m.off[a];
+      for (int i = 0; i < na; ++i)
+        for (int j = 0; j < na; ++j) d[i * na + j] += guess[(size_t)(off + i) * m.nao + off + j];
+    }
+    if (uhf) {
+      const double scale = tore > 0.0 ? 2.0 * nbe / tore : 0.0;
+      for (int a = 0; a < n; ++a) {
+        const int na = m.norb[a];
+        const double* d = dens.block(a, a);
+        double* sp = spin.block(a, a);
+        for (int i = 0; i < na; ++i) sp[i * na + i] += scale * d[i * na + i];
+      }
+    }
+  }
+  // Clusters and their core Hamiltonians (geometry only, built once); the nuclear far field.
+  std::vector<Mol> cl(subs.size());
+  std::vector<Core> cores(subs.size());
+  for (size_t s = 0; s < subs.size(); ++s) {
+    std::vector<int> zz;
+    std::vector<double> xx;
+    for (int a : subs[s].parent) {
+      zz.push_back(m.z[a]);
+      xx.push_back(m.pos[a].x);
+      xx.push_back(m.pos[a].y);
+      xx.push_back(m.pos[a].z);
+    }
+    std::string err;
+    build_mol(P, (int)zz.size(), zz.data(), xx.data(), cl[s], err);
+  }
+  std::vector<double> core_q(n);
+  for (int a = 0; a < n; ++a) core_q[a] = m.el[a]->core_charge;
+  auto far = [&](const std::vector<double>& q) {  // src/dandc/mod.rs:903-958 (molecule: direct sum)
+    std::vector<double> v(n, 0.0);
+    for (int a = 0; a < n; ++a) {
+      double t = 0.0;
+      for (int b = 0; b < n; ++b)
+        if (b != a) t += EV * q[b] / norm(sub(m.pos[b], m.pos[a]));
+      v[a] = t;
+    }
+    return v;
+  };
+  // Environment of each subsystem atom: the global potential less its own cluster
+  // (src/dandc/mod.rs:961-1011).
+  auto local = [&](const std::vector<double>& g, const std::vector<double>& q, size_t s) {
+    const DcSub& S = subs[s];
+    std::vector<double> v(S.size());
+    for (int i = 0; i < S.size(); ++i) {
+      double t = 0.0;
+      for (int j = 0; j < S.size(); ++j) {
+        if (i == j) continue;
+        const double r = norm(sub(m.pos[S.parent[j]], m.pos[S.parent[i]]));
+        if (r < 1.0e-8) continue;
+        t += EV * q[S.parent[j]] / r;
+      }
+      v[i] = g[S.parent[i]] - t;
+    }
+    return v;
+  };
+  const std::vector<double> g_nuc = far(core_q);
+  const int threads = g_threads;
+  g_threads = 1;  // the subsystem loop is the parallel one
+  auto par_subs = [&](auto&& body) {
+    const long ns = (long)subs.size();
+    const int t = std::max(1, std::min<int>(threads, (int)ns));
+    std::vector<std::thread> pool;
+    const long chunk = (ns + t - 1) / t;
+    for (int k = 0; k < t; ++k) {
+      const long lo = k * chunk, hi = std::min(ns, lo + chunk);
+      if (lo >= hi) break;
+      pool.emplace_back([lo, hi, &body]() {
+        for (long s = lo; s < hi; ++s) body((size_t)s);
+      });
+    }
+    for (auto& th : pool) th.join();
+  };
+  par_subs([&](size_t s) {
+    cores[s] = build_core(cl[s]);
+    const std::vector<double> vn = local(g_nuc, core_q, s);
+    for (int i = 0; i < subs[s].size(); ++i)
+      for (int k = 0; k < cl[s].norb[i]; ++k) {
+        const int d = cl[s].off[i] + k;
+        cores[s].h[(size_t)d * cl[s].nao + d] -= vn[i];
+      }
+  });
+  std::vector<DcSolution> sol(subs.size());
+  for (int it = 0; it < max_scf; ++it) {
+    R.iterations = it + 1;
+    std::vector<double> pops(n);
+    for (int a = 0; a < n; ++a) pops[a] = dens.population(a);
+    const std::vector<double> g_el = far(pops);
+    // solve_subsystem (src/dandc/mod.rs:520-626)
+    par_subs([&](size_t s) {
+      const DcSub& S = subs[s];
+      const Mol& cm = cl[s];
+      const int nn = cm.nao;
+      const std::vector<double> ve = local(g_el, pops, s);
+      auto local_of = [&](const DcSparse& src) {
+        Mat p((size_t)nn * nn, 0.0);
+        for (int i = 0; i < S.size(); ++i)
+          for (int j = 0; j < S.size(); ++j) {
+            const double* b = src.block(S.parent[i], S.parent[j]);
+            if (!b) continue;
+            const int ni = cm.norb[i], nj = cm.norb[j];
+            for (int mu = 0; mu < ni; ++mu)
+              for (int nu = 0; nu < nj; ++nu) p[(size_t)(cm.off[i] + mu) * nn + cm.off[j] + nu] = b[mu * nj + nu];
+          }
+        return p;
+      };
+      auto add_env = [&](Mat& f) {
+        for (int i = 0; i < S.size(); ++i)
+          for (int k = 0; k < cm.norb[i]; ++k) f[(size_t)(cm.off[i] + k) * nn + cm.off[i] + k] += ve[i];
+      };
+      const Mat pl = local_of(dens);
+      Mat f;
+      Mat fb;
+      if (!uhf) {
+        f = build_fock_rhf(cm, cores[s], pl);
+      } else {
+        const Mat sl = local_of(spin);
+        Mat pa(pl.size()), pb(pl.size());
+        for (size_t k = 0; k < pl.size(); ++k) {
+          pa[k] = 0.5 * (pl[k] + sl[k]);
+          pb[k] = 0.5 * (pl[k] - sl[k]);
+        }
+        f = build_fock_spin(cm, cores[s], pl, pa);
+        fb = build_fock_spin(cm, cores[s], pl, pb);
+        add_env(fb);
+      }
+      add_env(f);
+      DcSolution& out = sol[s];
+      eigh(nn, f, out.eps, out.c);
+      out.beta = uhf;
+      if (uhf) eigh(nn, fb, out.eps_b, out.c_b);
+      double e = 0.0;
+      for (int i = 0; i < S.size(); ++i)
+        for (int j = 0; j < S.size(); ++j) {
+          const double w = S.weight(i, j);
+          if (w == 0.0) continue;
+          for (int mu = 0; mu < cm.norb[i]; ++mu)
+            for (int nu = 0; nu < cm.norb[j]; ++nu) {
+              const size_t x = (size_t)(cm.off[i] + mu) * nn + cm.off[j] + nu;
+              e += 0.5 * w * pl[x] * (cores[s].h[x] + f[x]);
+            }
+        }
+      out.energy = e;
+    });
+    // Weighted spectrum and the common Fermi level (src/dandc/mod.rs:667-770).
+    std::vector<std::vector<std::pair<double, double>>> spec(subs.size());
+    for (size_t s = 0; s < subs.size(); ++s) {
+      const DcSub& S = subs[s];
+      const Mol& cm = cl[s];
+      const int nn = cm.nao;
+      for (int ch = 0; ch < (uhf ? 2 : 1); ++ch) {
+        const std::vector<double>& ev = ch == 0 ? sol[s].eps : sol[s].eps_b;
+        const Mat& c = ch == 0 ? sol[s].c : sol[s].c_b;
+        const double scale = uhf ? 0.5 : 1.0;
+        for (int k = 0; k < nn; ++k) {
+          double nrm = 0.0;
+          for (int i = 0; i < S.n_core; ++i)
+            for (int mu = 0; mu < cm.norb[i]; ++mu) {
+              const double v = c[(size_t)(cm.off[i] + mu) * nn + k];
+              nrm += v * v;
+            }
+          spec[s].push_back({ev[k], scale * nrm});
+        }
+      }
+    }
+    auto count = [&](double mu) {
+      double t = 0.0;
+      for (auto& st : spec) {
+        double part = 0.0;
+        for (auto& ew : st) {
+          const double f = dc_occupation(ew.first, mu, width);
+          part += f < 1.0e-14 ? 0.0 : f * ew.second;
+        }
+        t += part;
+      }
+      return t;
+    };
+    double lo = -500.0, hi = 500.0;
+    if (count(lo) > n_elec || count(hi) < n_elec) {
+      R.status = 1;
+      R.msg = "no chemical potential in [-500, 500] eV holds the electron count";
+      break;
+    }
+    for (int k = 0; k < 200; ++k) {
+      const double mid = 0.5 * (lo + hi);
+      if (count(mid) < n_elec) lo = mid;
+      else hi = mid;
+      if (hi - lo < 1.0e-12) break;
+    }
+    R.fermi = 0.5 * (lo + hi);
+    // scatter_density, normalisation, RMS change, mixing (src/dandc/mod.rs:310-407, 772-821).
+    DcSparse next = pat, next_s = pat;
+    for (size_t s = 0; s < subs.size(); ++s) {
+      const DcSub& S = subs[s];
+      const Mol& cm = cl[s];
+      const int nn = cm.nao;
+      for (int spin_pass = 0; spin_pass < (uhf ? 2 : 1); ++spin_pass) {
+        DcSparse& dst = spin_pass ? next_s : next;
+        for (int i = 0; i < S.size(); ++i)
+          for (int j = 0; j < S.size(); ++j) {
+            const double w = S.weight(i, j);
+            if (w == 0.0) continue;
+            const int na = cm.norb[i], nb = cm.norb[j], oi = cm.off[i], oj = cm.off[j];
+            std::vector<double> blk((size_t)na * nb, 0.0);
+            for (int ch = 0; ch < (uhf ? 2 : 1); ++ch) {
+              const double scale = !uhf ? 1.0 : (ch == 0 ? 0.5 : (spin_pass ? -0.5 : 0.5));
+              const std::vector<double>& ev = ch == 0 ? sol[s].eps : sol[s].eps_b;
+              const Mat& c = ch == 0 ? sol[s].c : sol[s].c_b;
+              for (int k = 0; k < nn; ++k) {
+                const double f = dc_occupation(ev[k], R.fermi, width) * scale;
+                if (std::fabs(f) < 1.0e-14) continue;
+                for (int mu = 0; mu < na; ++mu) {
+                  const double cm_ = c[(size_t)(oi + mu) * nn + k];
+                  for (int nu = 0; nu < nb; ++nu) blk[(size_t)mu * nb + nu] += f * cm_ * c[(size_t)(oj + nu) * nn + k];
+                }
+              }
+            }
+            double* d = dst.block(S.parent[i], S.parent[j]);
+            for (size_t x = 0; x < blk.size(); ++x) d[x] += w * blk[x];
+          }
+      }
+    }
+    for (size_t k = 0; k + 1 < pat.offs.size(); ++k)
+      for (long x = pat.offs[k]; x < pat.offs[k + 1]; ++x) {
+        next.data[x] *= normalizer[k];
+        next_s.data[x] *= normalizer[k];
+      }
+    double sq = 0.0;
+    for (size_t x = 0; x < dens.data.size(); ++x) sq += (dens.data[x] - next.data[x]) * (dens.data[x] - next.data[x]);
+    R.dp = std::sqrt(sq / (double)std::max<size_t>(dens.data.size(), 1));
+    for (size_t x = 0; x < dens.data.size(); ++x) dens.data[x] = (1.0 - mixing) * dens.data[x] + mixing * next.data[x];
+    if (uhf)
+      for (size_t x = 0; x < spin.data.size(); ++x)
+        spin.data[x] = (1.0 - mixing) * spin.data[x] + mixing * next_s.data[x];
+    double e = 0.0;
+    for (auto& so : sol) e += so.energy;
+    R.e_el = e;
+    if (R.dp < p_tol) {
+      R.converged = 1;
+      break;
+    }
+  }
+  g_threads = threads;
+  R.density = std::move(dens);
+  if (uhf) R.spin = std::move(spin);
+  return R;
+}
+
 }  // namespace
 
 // =============================================================================================
@@ -3509,6 +3936,99 @@ int oracle_run(const char* data_dir, int n_atoms, const int* z, const double* xy
     return out->status = 5;
   }
   return out->status = 0;
+}
+
+int oracle_dandc_partition(int n_atoms, const double* xyz, int core_size, double buffer, int* n_sub,
+                           int* n_core, int* start, int* members, int cap) {
+  std::vector<V3> pos(n_atoms);
+  for (int i = 0; i < n_atoms; ++i) pos[i] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+  std::vector<DcSub> subs = dc_partition(pos, core_size, buffer);
+  *n_sub = (int)subs.size();
+  long total = 0;
+  for (auto& s : subs) total += s.size();
+  if (!n_core || !start || !members) return (int)total;
+  if (total > cap) return -1;
+  int at = 0;
+  for (size_t k = 0; k < subs.size(); ++k) {
+    n_core[k] = subs[k].n_core;
+    start[k] = at;
+    for (int a : subs[k].parent) members[at++] = a;
+  }
+  start[subs.size()] = at;
+  return (int)total;
+}
+
+void oracle_default_dandc_options(oracle_dandc_options* d) {  // src/dandc/mod.rs:77-89
+  d->buffer_bohr = 15.0;
+  d->core_size = 12;
+  d->max_scf = 200;
+  d->fermi_width_ev = 0.1;
+  d->p_tol = 1.0e-6;
+  d->mixing = 0.4;
+}
+
+int oracle_run_dandc(const char* data_dir, int n_atoms, const int* z, const double* xyz, const oracle_options* opts,
+                     const oracle_dandc_options* dopts, oracle_dandc_result* out, double* dense_density,
+                     double* dense_spin, double* charges) {
+  oracle_options o;
+  if (opts) o = *opts;
+  else oracle_default_options(&o);
+  oracle_dandc_options d;
+  if (dopts) d = *dopts;
+  else oracle_default_dandc_options(&d);
+  std::memset(out, 0, sizeof *out);
+  g_threads = o.threads > 0 ? o.threads : (int)std::max(1u, std::thread::hardware_concurrency());
+  auto bad = [&](const char* m) {
+    std::snprintf(out->message, sizeof out->message, "%s", m);
+    return out->status = 1;
+  };
+  if (n_atoms <= 0) return bad("empty molecule");
+  if (!(d.buffer_bohr > 0.0) || !std::isfinite(d.buffer_bohr) || d.core_size < 1 || !(d.fermi_width_ev > 0.0) ||
+      !(d.mixing > 0.0 && d.mixing <= 1.0) || d.max_scf < 1)
+    return bad("invalid divide-and-conquer options");
+  const Params* P = load_params(data_dir, o.method == 1 ? 1 : 0);
+  if (!P) return bad("parameter tables not found");
+  Mol m;
+  std::string err;
+  int st = build_mol(*P, n_atoms, z, xyz, m, err);
+  if (st) {
+    std::snprintf(out->message, sizeof out->message, "%s", err.c_str());
+    return out->status = st;
+  }
+  DcOut R = dandc_run(*P, m, o, d.buffer_bohr, d.core_size, d.fermi_width_ev, d.max_scf, d.p_tol, d.mixing);
+  out->status = R.status;
+  out->converged = R.converged;
+  out->iterations = R.iterations;
+  out->unrestricted = R.unrestricted;
+  out->subsystems = R.subsystems;
+  out->largest_subsystem = R.largest;
+  out->largest_subsystem_nao = R.largest_nao;
+  out->stored_pairs = R.stored_pairs;
+  out->stored_elements = R.stored_elements;
+  out->fermi_ev = R.fermi;
+  out->electronic_ev = R.e_el;
+  out->density_error = R.dp;
+  out->nao = m.nao;
+  std::snprintf(out->message, sizeof out->message, "%s", R.msg.c_str());
+  if (R.status) return R.status;
+  auto dense = [&](const DcSparse& sp, double* dst) {
+    std::memset(dst, 0, sizeof(double) * (size_t)m.nao * m.nao);
+    for (int a = 0; a < n_atoms; ++a)
+      for (long k = sp.row_start[a]; k < sp.row_start[a + 1]; ++k) {
+        const int b = sp.cols[k], na = m.norb[a], nb = m.norb[b];
+        for (int i = 0; i < na; ++i)
+          for (int j = 0; j < nb; ++j)
+            dst[(size_t)(m.off[a] + i) * m.nao + m.off[b] + j] = sp.data[sp.offs[k] + i * nb + j];
+      }
+  };
+  if (dense_density) dense(R.density, dense_density);
+  if (dense_spin) {
+    if (R.unrestricted) dense(R.spin, dense_spin);
+    else std::memset(dense_spin, 0, sizeof(double) * (size_t)m.nao * m.nao);
+  }
+  if (charges)
+    for (int a = 0; a < n_atoms; ++a) charges[a] = m.el[a]->core_charge - R.density.population(a);
+  return 0;
 }
 
 }  // extern "C"

@@ -92,6 +92,43 @@ int oracle_overlap_locals(int na, double zsa, double zpa, int nb, double zsb, do
 /* Parameter derived quantities for tests: [n_orb, dd, qq, rho0, rho1, rho2, e_isol, eheat_ev, po9]. */
 int oracle_element(const char* data_dir, int method, int z, double* out9);
 
+/* Divide-and-conquer SCF (src/dandc/mod.rs:57-420), molecular case. */
+typedef struct {
+  double buffer_bohr;    /* default 15 */
+  int core_size;         /* default 12 */
+  int max_scf;           /* default 200 */
+  double fermi_width_ev; /* default 0.1 */
+  double p_tol;          /* default 1e-6 */
+  double mixing;         /* default 0.4 */
+} oracle_dandc_options;
+
+typedef struct {
+  int status;            /* 0 ok (also when not converged), 1 invalid input / no Fermi level, 2 missing element */
+  int converged;
+  int iterations;
+  int unrestricted;
+  int subsystems;
+  int largest_subsystem;
+  int largest_subsystem_nao;
+  int nao;
+  long long stored_pairs;
+  long long stored_elements;
+  double fermi_ev;
+  double electronic_ev;
+  double density_error;
+  char message[256];
+} oracle_dandc_result;
+
+void oracle_default_dandc_options(oracle_dandc_options* d);
+/* dense_density / dense_spin: nao*nao (the stored blocks, zero elsewhere) or NULL; charges[n] or NULL. */
+int oracle_run_dandc(const char* data_dir, int n_atoms, const int* z, const double* xyz_bohr,
+                     const oracle_options* opts, const oracle_dandc_options* dopts, oracle_dandc_result* out,
+                     double* dense_density, double* dense_spin, double* charges);
+/* Partition (src/dandc/partition.rs:97-133): returns the total member count; with the arrays NULL
+ * only counts (n_sub and the return value), else fills n_core[n_sub], start[n_sub+1], members[cap]. */
+int oracle_dandc_partition(int n_atoms, const double* xyz_bohr, int core_size, double buffer, int* n_sub,
+                           int* n_core, int* start, int* members, int cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,6 +29,7 @@
 
 #include "../../include/pm7_amd.h"
 #include "host.h"
+#include "dandc.h"
 #include "kernels.h"
 
 using namespace pm7;
@@ -469,6 +470,8 @@ struct Prepared {
   std::vector<Job> jobs;
   std::list<std::vector<double>> staging;  // host copies of seeded densities (stable addresses)
   double need_mb = 0.0, budget_mb = 0.0;   // INSUFFICIENT_MEMORY payload
+  bool fixed_classes = false;  // keep every job in its size class (no grid-wide routing)
+  size_t extra_bytes = 0;      // device memory the caller adds beside the batch (budget check)
   std::vector<MolDev> mols;
   int64_t n_pairs = 0, w_words = 0, slab_words = 0, vec_words = 0, gath_words = 0, n_dpairs = 0;
   int n_items = 0;
@@ -529,7 +532,7 @@ pm7_status prepare(pm7_ctx* ctx, Lane& L, Prepared& P, int64_t max_memory_mb) {
   int atom0 = 0;
   P.mols.resize(nm);
   std::vector<int> lists[SCF_VARIANTS];
-  {
+  if (!P.fixed_classes) {
     int few = 0;
     for (const Job& j : P.jobs) few += (j.variant == LARGE_VARIANT - 1 && j.nao > ctx->grid_min_nao) ? 1 : 0;
     if (few > 0 && few <= GRID_FEW)
@@ -637,7 +640,7 @@ pm7_status prepare(pm7_ctx* ctx, Lane& L, Prepared& P, int64_t max_memory_mb) {
   P.n_items = (int)items.size();
   P.n_dpairs = (int64_t)d_pairs.size();
   // HBM budget (pm7-rs memory::guard, src/memory.rs:95-118, but for device memory).
-  size_t need = bytes_needed(P);
+  size_t need = bytes_needed(P) + P.extra_bytes;
   size_t free_b = 0, total_b = 0;
   LANECHK(hipMemGetInfo(&free_b, &total_b));
   size_t budget = max_memory_mb > 0 ? (size_t)max_memory_mb * 1024 * 1024 : (size_t)(0.8 * (double)(free_b + L.held()));
@@ -1964,6 +1967,518 @@ pm7_status pm7_core_core_energy(pm7_ctx* ctx, const pm7_molecule_v1* mol, double
   HIPCHK(hipMemcpyAsync(e_out, P.b.e_core_mol, sizeof(double), hipMemcpyDeviceToHost, ctx->lane.st[0]));
   HIPCHK(hipStreamSynchronize(ctx->lane.st[0]));
   return PM7_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Divide-and-conquer SCF (pm7-rs run_dandc, src/dandc/mod.rs:138-420)
+// ---------------------------------------------------------------------------------------------
+void pm7_dandc_default_options(pm7_dandc_options_v1* d) {
+  if (!d) return;
+  std::memset(d, 0, sizeof *d);
+  d->buffer_bohr = 15.0;
+  d->core_size = 12;
+  d->max_scf = 200;
+  d->fermi_width_ev = 0.1;
+  d->p_tol = 1.0e-6;
+  d->mixing = 0.4;
+}
+
+namespace {
+
+// DandcOptions::validate (src/dandc/mod.rs:91-115), plus the step and tolerance limits.
+bool dandc_options_ok(const pm7_dandc_options_v1& d, std::string& msg) {
+  if (!std::isfinite(d.buffer_bohr) || d.buffer_bohr <= 0.0)
+    return msg = "the divide-and-conquer buffer must be finite and positive", false;
+  if (d.core_size < 1) return msg = "the divide-and-conquer core size must be at least 1", false;
+  if (!std::isfinite(d.fermi_width_ev) || d.fermi_width_ev <= 0.0)
+    return msg = "the divide-and-conquer Fermi width must be finite and positive", false;
+  if (!(d.mixing > 0.0 && d.mixing <= 1.0)) return msg = "the divide-and-conquer mixing factor must lie in (0, 1]", false;
+  if (d.max_scf < 1) return msg = "the divide-and-conquer max_scf must be at least 1", false;
+  if (!(std::isfinite(d.p_tol) && d.p_tol > 0.0)) return msg = "the divide-and-conquer p_tol must be finite and positive", false;
+  return true;
+}
+
+// Device memory of one run beside its subsystem batch.
+struct DcBufs {
+  DevBuf diag_off, norb, xyz, core, parent, atom_mol, nc_ao, aocode, ao0, amap, amap0, slot_off, slot_dims, cstart,
+      contrib, normalizer, P, S, pop, g, v_nuc, v_el, wnorm, e_part, partial, scal;
+  PinnedBuf h_scal;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  ~DcBufs() {
+    DevBuf* all[] = {&diag_off, &norb, &xyz, &core, &parent, &atom_mol, &nc_ao, &aocode, &ao0, &amap, &amap0,
+                     &slot_off, &slot_dims, &cstart, &contrib, &normalizer, &P, &S, &pop, &g, &v_nuc, &v_el,
+                     &wnorm, &e_part, &partial, &scal};
+    for (DevBuf* b : all) b->release();
+    h_scal.release();
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+
+pm7_status dandc_impl(pm7_ctx* ctx, const pm7_molecule_v1& mol, const pm7_options_v2& o0,
+                      const pm7_dandc_options_v1& d, pm7_dandc_result_v1* out) {
+  auto refuse = [&](pm7_status st, const std::string& m) {
+    out->status = st;
+    std::snprintf(out->message, sizeof out->message, "%s", m.c_str());
+    ctx->err = m;
+    return st;
+  };
+  std::string msg;
+  if (!dandc_options_ok(d, msg)) return refuse(PM7_ERR_INVALID_INPUT, msg);
+  // The reference starts from the atomic guess whatever the options hold.
+  pm7_options_v2 o = o0;
+  o.initial_density = nullptr;
+  o.initial_spin_density = nullptr;
+  o.initial_density_nao = 0;
+  Job whole;
+  pm7_status st = validate(ctx, mol, o, whole, msg);
+  if (st != PM7_OK) {
+    out->n_atoms = mol.n_atoms;
+    return refuse(st, msg);
+  }
+  const int n = mol.n_atoms;
+  const double* X = mol.xyz_bohr;
+  const bool uhf = whole.uhf != 0;
+  std::vector<int> norb(n);
+  std::vector<double> core(n);
+  double tore = 0.0;
+  for (int a = 0; a < n; ++a) {
+    norb[a] = ctx->params.el[mol.z[a]].n_orb;
+    core[a] = ctx->params.el[mol.z[a]].core_charge;
+    tore += core[a];
+  }
+  DcPartition part;
+  dc_partition(n, X, d.core_size, d.buffer_bohr, part);
+  const int ns = part.n_sub();
+  int largest = 0, largest_nao = 0;
+  int64_t amap_words = 0, n_contrib = 0;
+  for (int s = 0; s < ns; ++s) {
+    const int len = (int)(part.start[s + 1] - part.start[s]), nc = part.n_core[s];
+    int nao = 0;
+    for (int i = 0; i < len; ++i) nao += norb[part.members[part.start[s] + i]];
+    largest = std::max(largest, len);
+    largest_nao = std::max(largest_nao, nao);
+    amap_words += (int64_t)len * len;
+    n_contrib += (int64_t)len * len - (int64_t)(len - nc) * (len - nc);
+  }
+  out->n_atoms = n;
+  out->subsystems = ns;
+  out->largest_subsystem = largest;
+  out->largest_subsystem_nao = largest_nao;
+  out->unrestricted = uhf;
+  const double buffer_ang = d.buffer_bohr * A0;
+  auto knob = [&](const std::string& what) {
+    char b[512];
+    std::snprintf(b, sizeof b,
+                  "%s\nThis is a divide-and-conquer subsystem, not the whole system: a %.2f A buffer around a "
+                  "%d-atom core gives a largest subsystem of %d atoms and %d orbitals. Reduce the buffer, or drop "
+                  "divide and conquer and use the exact SCF (pm7_run): below roughly 350 atoms the exact SCF is both "
+                  "faster and exact.",
+                  what.c_str(), buffer_ang, d.core_size, largest, largest_nao);
+    return std::string(b);
+  };
+  if (largest_nao > VARIANT_MAX_NAO[LARGE_VARIANT - 1])
+    return refuse(PM7_ERR_UNSUPPORTED, knob("one subsystem workgroup holds at most " +
+                                            std::to_string(VARIANT_MAX_NAO[LARGE_VARIANT - 1]) +
+                                            " basis functions"));
+
+  // Stored pattern: the union of the subsystems' weighted atom pairs, CSR over atoms with sorted
+  // columns (src/dandc/mod.rs:218-230, src/dandc/sparse.rs with_pattern).
+  std::vector<int64_t> row_start(n + 1, 0);
+  std::vector<int> cols;
+  std::vector<int64_t> offs;
+  {
+    std::vector<std::vector<int>> nbr(n);
+    for (int s = 0; s < ns; ++s) {
+      const int* m = &part.members[part.start[s]];
+      const int len = (int)(part.start[s + 1] - part.start[s]), nc = part.n_core[s];
+      for (int i = 0; i < len; ++i)
+        for (int j = 0; j < len; ++j)
+          if (i < nc || j < nc) nbr[m[i]].push_back(m[j]);
+    }
+    int64_t total = 0;
+    for (int a = 0; a < n; ++a) {
+      std::vector<int>& l = nbr[a];
+      std::sort(l.begin(), l.end());
+      l.erase(std::unique(l.begin(), l.end()), l.end());
+      for (int b : l) {
+        cols.push_back(b);
+        offs.push_back(total);
+        total += (int64_t)norb[a] * norb[b];
+      }
+      row_start[a + 1] = (int64_t)cols.size();
+      std::vector<int>().swap(l);
+    }
+    offs.push_back(total);
+  }
+  const int64_t n_slots = (int64_t)cols.size(), n_el = offs.back();
+  auto slot_of = [&](int a, int b) -> int64_t {
+    auto lo = cols.begin() + row_start[a], hi = cols.begin() + row_start[a + 1];
+    auto it = std::lower_bound(lo, hi, b);
+    return (it != hi && *it == b) ? (int64_t)(it - cols.begin()) : -1;
+  };
+  out->stored_pairs = n_slots;
+  out->stored_elements = n_el;
+  // Partition weights normalised by their realised totals (src/dandc/mod.rs:232-258).
+  std::vector<double> normalizer(n_slots, 0.0);
+  for (int s = 0; s < ns; ++s) {
+    const int* m = &part.members[part.start[s]];
+    const int len = (int)(part.start[s + 1] - part.start[s]), nc = part.n_core[s];
+    for (int i = 0; i < len; ++i)
+      for (int j = 0; j < len; ++j) {
+        const double w = 0.5 * ((i < nc ? 1.0 : 0.0) + (j < nc ? 1.0 : 0.0));
+        if (w > 0.0) normalizer[slot_of(m[i], m[j])] += w;
+      }
+  }
+  for (double& w : normalizer) w = w > 1.0e-12 ? 1.0 / w : 0.0;
+  // Starting densities: MOPAC's atomic guess of the whole system on the diagonal blocks, and for
+  // an open shell a spin density scaled from its diagonal (src/dandc/mod.rs:261-306).
+  const double n_elec = tore - o.charge;
+  const double yy = whole.nao > 0 ? (tore - n_elec) / whole.nao : 0.0;
+  const double spin_scale = tore > 0.0 ? 2.0 * (0.5 * (o.multiplicity - 1)) / tore : 0.0;
+  std::vector<double> P0(n_el, 0.0), S0(uhf ? n_el : 0, 0.0);
+  std::vector<int64_t> diag_off(n);
+  for (int a = 0; a < n; ++a) {
+    const int64_t k = slot_of(a, a);
+    diag_off[a] = offs[k];
+    const int na = norb[a];
+    sad_atom(ctx->elem_host[mol.z[a]], yy, 1.0, P0.data() + offs[k], 0, na);
+    if (uhf)
+      for (int i = 0; i < na; ++i) S0[offs[k] + i * na + i] = spin_scale * P0[offs[k] + i * na + i];
+  }
+
+  // The subsystems as the molecules of one batch: integrals and core Hamiltonians once.
+  Lane& L = ctx->lane;
+  Prepared P;
+  P.fixed_classes = true;
+  P.jobs.resize(ns);
+  for (int s = 0; s < ns; ++s) {
+    Job& j = P.jobs[s];
+    j.idx = s;
+    j.o = o;
+    const int len = (int)(part.start[s + 1] - part.start[s]);
+    int nao = 0, npp = 0;
+    for (int i = 0; i < len; ++i) {
+      const int a = part.members[part.start[s] + i];
+      j.z.push_back(mol.z[a]);
+      for (int k = 0; k < 3; ++k) j.xyz.push_back(X[3 * a + k]);
+      nao += norb[a];
+      npp += npair_of(norb[a]);
+      if (norb[a] == 9) j.dpath = 1;  // the cluster's own atoms decide (build_core of the cluster)
+    }
+    j.nao = nao;
+    j.ld = nao + (nao & 1);
+    j.n_pp = npp;
+    j.uhf = uhf;
+    j.variant = -1;
+    for (int v = 0; v < LARGE_VARIANT; ++v)
+      if (nao <= VARIANT_MAX_NAO[v]) {
+        j.variant = v;
+        break;
+      }
+  }
+  P.extra_bytes = (size_t)n_el * 8 * (uhf ? 3 : 2) + (size_t)n_slots * (8 + 4 + 8 + 8 + 8) +
+                  (size_t)n_contrib * sizeof(DcContrib) + (size_t)amap_words * 8 + (size_t)n * 64 +
+                  (size_t)part.members.size() * 40;
+  st = prepare(ctx, L, P, o.max_memory_mb);
+  if (st != PM7_OK) {
+    out->needed_mb = P.need_mb;
+    out->budget_mb = P.budget_mb;
+    return refuse(st, st == PM7_ERR_INSUFFICIENT_MEMORY ? knob(L.err) : L.err);
+  }
+
+  // Host tables of the device side.
+  const int nb_atoms = (int)part.members.size();
+  std::vector<int> atom_mol(nb_atoms), nc_ao(ns), aocode;
+  std::vector<int64_t> ao0(ns), amap, amap0(ns), cstart(n_slots + 1, 0);
+  std::vector<int> slot_dims(n_slots);
+  for (int a = 0; a < n; ++a)
+    for (int64_t k = row_start[a]; k < row_start[a + 1]; ++k) slot_dims[k] = norb[a] | (norb[cols[k]] << 8);
+  amap.reserve(amap_words);
+  for (int s = 0; s < ns; ++s) {
+    const int* m = &part.members[part.start[s]];
+    const int len = (int)(part.start[s + 1] - part.start[s]), nc = part.n_core[s];
+    ao0[s] = (int64_t)aocode.size();
+    amap0[s] = (int64_t)amap.size();
+    int core_ao = 0;
+    for (int i = 0; i < len; ++i) {
+      atom_mol[part.start[s] + i] = s;
+      for (int k = 0; k < norb[m[i]]; ++k) aocode.push_back(dc_ao_code(i, k, norb[m[i]]));
+      if (i < nc) core_ao += norb[m[i]];
+    }
+    nc_ao[s] = core_ao;
+    for (int i = 0; i < len; ++i)
+      for (int j = 0; j < len; ++j) {
+        const int64_t k = slot_of(m[i], m[j]);
+        amap.push_back(k < 0 ? -1 : offs[k]);
+        if (k >= 0 && (i < nc || j < nc)) ++cstart[k + 1];
+      }
+  }
+  for (int64_t k = 0; k < n_slots; ++k) cstart[k + 1] += cstart[k];
+  std::vector<DcContrib> contrib(n_contrib);
+  {
+    std::vector<int64_t> fill(cstart.begin(), cstart.end() - 1);
+    for (int s = 0; s < ns; ++s) {
+      const Job& j = P.jobs[s];
+      const int* m = &part.members[part.start[s]];
+      const int len = (int)(part.start[s + 1] - part.start[s]), nc = part.n_core[s];
+      const int64_t sq = (int64_t)j.ld * j.ld;
+      int oi = 0;
+      for (int i = 0; i < len; ++i) {
+        int oj = 0;
+        for (int jj = 0; jj < len; ++jj) {
+          if (i < nc || jj < nc) {
+            DcContrib c;
+            c.base = j.mat0 + (uhf ? (int64_t)DU_DA : (int64_t)DR_D) * sq + (int64_t)oi * j.ld + oj;
+            c.dstep = (int64_t)(DU_DB - DU_DA) * sq;
+            c.ld = j.ld;
+            c.pad = 0;
+            c.w = 0.5 * ((i < nc ? 1.0 : 0.0) + (jj < nc ? 1.0 : 0.0));
+            contrib[fill[slot_of(m[i], m[jj])]++] = c;
+          }
+          oj += norb[m[jj]];
+        }
+        oi += norb[m[i]];
+      }
+    }
+  }
+  std::vector<double> xyz(X, X + 3 * n);
+
+  DcBufs B;
+  hipStream_t s0 = L.st[0];
+#define DCCHK(expr)                                                                        \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess) return refuse(PM7_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+  auto up = [&](DevBuf& buf, const void* src, size_t bytes) -> hipError_t {
+    hipError_t e = buf.ensure(std::max<size_t>(bytes, 8));
+    if (e != hipSuccess || !bytes) return e;
+    return hipMemcpyAsync(buf.p, src, bytes, hipMemcpyHostToDevice, s0);
+  };
+  DCCHK(up(B.diag_off, diag_off.data(), diag_off.size() * 8));
+  DCCHK(up(B.norb, norb.data(), norb.size() * 4));
+  DCCHK(up(B.xyz, xyz.data(), xyz.size() * 8));
+  DCCHK(up(B.core, core.data(), core.size() * 8));
+  DCCHK(up(B.parent, part.members.data(), part.members.size() * 4));
+  DCCHK(up(B.atom_mol, atom_mol.data(), atom_mol.size() * 4));
+  DCCHK(up(B.nc_ao, nc_ao.data(), nc_ao.size() * 4));
+  DCCHK(up(B.aocode, aocode.data(), aocode.size() * 4));
+  DCCHK(up(B.ao0, ao0.data(), ao0.size() * 8));
+  DCCHK(up(B.amap, amap.data(), amap.size() * 8));
+  DCCHK(up(B.amap0, amap0.data(), amap0.size() * 8));
+  DCCHK(up(B.slot_off, offs.data(), offs.size() * 8));
+  DCCHK(up(B.slot_dims, slot_dims.data(), slot_dims.size() * 4));
+  DCCHK(up(B.cstart, cstart.data(), cstart.size() * 8));
+  DCCHK(up(B.contrib, contrib.data(), contrib.size() * sizeof(DcContrib)));
+  DCCHK(up(B.normalizer, normalizer.data(), normalizer.size() * 8));
+  DCCHK(up(B.P, P0.data(), P0.size() * 8));
+  DCCHK(up(B.S, S0.data(), S0.size() * 8));
+  DCCHK(B.pop.ensure((size_t)n * 8));
+  DCCHK(B.g.ensure((size_t)n * 8));
+  DCCHK(B.v_nuc.ensure((size_t)nb_atoms * 8));
+  DCCHK(B.v_el.ensure((size_t)nb_atoms * 8));
+  DCCHK(B.wnorm.ensure((size_t)P.vec_words * 8));
+  DCCHK(B.e_part.ensure((size_t)ns * 8));
+  DCCHK(B.partial.ensure((size_t)std::max<int64_t>(n_slots, 1) * 8));
+  DCCHK(B.scal.ensure(8 * 8));
+  DCCHK(B.h_scal.ensure(8 * 8));
+  DCCHK(hipEventCreate(&B.ev[0]));
+  DCCHK(hipEventCreate(&B.ev[1]));
+  DcDev dd;
+  dd.n_atoms = n;
+  dd.n_sub = ns;
+  dd.uhf = uhf;
+  dd.n_batch_atoms = nb_atoms;
+  dd.n_slots = n_slots;
+  dd.n_el = n_el;
+  dd.n_states = P.vec_words;
+  dd.diag_off = B.diag_off.as<int64_t>();
+  dd.norb = B.norb.as<int>();
+  dd.xyz = B.xyz.as<double>();
+  dd.core = B.core.as<double>();
+  dd.parent = B.parent.as<int>();
+  dd.atom_mol = B.atom_mol.as<int>();
+  dd.nc_ao = B.nc_ao.as<int>();
+  dd.aocode = B.aocode.as<int>();
+  dd.ao0 = B.ao0.as<int64_t>();
+  dd.amap = B.amap.as<int64_t>();
+  dd.amap0 = B.amap0.as<int64_t>();
+  dd.slot_off = B.slot_off.as<int64_t>();
+  dd.slot_dims = B.slot_dims.as<int>();
+  dd.cstart = B.cstart.as<int64_t>();
+  dd.contrib = B.contrib.as<DcContrib>();
+  dd.normalizer = B.normalizer.as<double>();
+  dd.P = B.P.as<double>();
+  dd.S = uhf ? B.S.as<double>() : nullptr;
+  dd.pop = B.pop.as<double>();
+  dd.g = B.g.as<double>();
+  dd.v_nuc = B.v_nuc.as<double>();
+  dd.v_el = B.v_el.as<double>();
+  dd.wnorm = B.wnorm.as<double>();
+  dd.e_part = B.e_part.as<double>();
+  dd.partial = B.partial.as<double>();
+  dd.scal = B.scal.as<double>();
+  const BatchDev& b = P.b;
+  // The nuclear environment, once per geometry, into the subsystems' core Hamiltonians.
+  DCCHK(launch_dc_field(b, dd, dd.core, dd.g, dd.v_nuc, s0));
+  DCCHK(launch_dc_shift_h(b, dd, s0));
+  DCCHK(hipEventRecord(B.ev[0], s0));
+  const double* hs = static_cast<const double*>(B.h_scal.p);
+  int it = 0;
+  bool converged = false;
+  double rms = INFINITY, e_el = 0.0, fermi = 0.0;
+  for (; it < d.max_scf;) {
+    DCCHK(launch_dc_pop(dd, s0));
+    DCCHK(launch_dc_field(b, dd, dd.pop, dd.g, dd.v_el, s0));
+    DCCHK(launch_dc_gather(b, dd, s0));
+    for (int v = 0; v < LARGE_VARIANT; ++v)
+      DCCHK(launch_dc_solve(v, b, P.list_off[v], P.list_off[v + 1] - P.list_off[v], dd.nc_ao, dd.v_el, it, dd.wnorm,
+                            dd.e_part, s0));
+    DCCHK(launch_dc_fermi(b, dd, n_elec, d.fermi_width_ev, s0));
+    DCCHK(launch_dc_density(b, ns, dd.scal, d.fermi_width_ev, it, s0));
+    DCCHK(launch_dc_assemble(b, dd, d.mixing, s0));
+    DCCHK(launch_dc_finish(dd, s0));
+    DCCHK(hipMemcpyAsync(B.h_scal.p, dd.scal, 4 * 8, hipMemcpyDeviceToHost, s0));
+    DCCHK(hipStreamSynchronize(s0));
+    ++it;
+    if (hs[3] != 0.0) {
+      // No chemical potential brackets the electron count (src/dandc/mod.rs:744-758).
+      std::vector<double> ev(P.vec_words);
+      DCCHK(hipMemcpy(ev.data(), b.vecs, ev.size() * 8, hipMemcpyDeviceToHost));
+      double lo = INFINITY, hi = -INFINITY;
+      for (const Job& j : P.jobs) {
+        lo = std::min(lo, ev[j.vec0]);
+        hi = std::max(hi, ev[j.vec0 + j.nao - 1]);
+      }
+      char bmsg[256];
+      std::snprintf(bmsg, sizeof bmsg,
+                    "no chemical potential in [-500, 500] eV holds %g electrons; the subsystem spectra span %.3f "
+                    "to %.3f eV",
+                    n_elec, lo, hi);
+      out->iterations = it;
+      return refuse(PM7_ERR_INVALID_INPUT, bmsg);
+    }
+    rms = hs[0];
+    e_el = hs[1];
+    fermi = hs[2];
+    if (rms < d.p_tol) {
+      converged = true;
+      break;
+    }
+  }
+  DCCHK(hipEventRecord(B.ev[1], s0));
+  DCCHK(hipEventSynchronize(B.ev[1]));
+  float ms = 0.0f;
+  DCCHK(hipEventElapsedTime(&ms, B.ev[0], B.ev[1]));
+  out->device_ms = ms;
+  out->iterations = it;
+  out->converged = converged;
+  out->density_error = rms;
+  out->electronic_ev = e_el;
+  out->fermi_ev = fermi;
+
+  // Results: one allocation based at row_start.
+  const size_t bytes = (size_t)(n + 1) * 8 + (size_t)(n_slots + 1) * 8 + (size_t)n_el * 8 * (uhf ? 2 : 1) +
+                       (size_t)n * 8 + (size_t)n_slots * 4 + (size_t)n * 4;
+  char* blk = static_cast<char*>(std::malloc(std::max<size_t>(bytes, 8)));
+  if (!blk) return refuse(PM7_ERR_INSUFFICIENT_MEMORY, "host allocation of the divide-and-conquer result failed");
+  char* q = blk;
+  auto take = [&](size_t nbytes) {
+    char* r = q;
+    q += nbytes;
+    return r;
+  };
+  out->row_start = reinterpret_cast<int64_t*>(take((size_t)(n + 1) * 8));
+  out->offsets = reinterpret_cast<int64_t*>(take((size_t)(n_slots + 1) * 8));
+  out->density = reinterpret_cast<double*>(take((size_t)n_el * 8));
+  out->spin_density = uhf ? reinterpret_cast<double*>(take((size_t)n_el * 8)) : nullptr;
+  out->charges = reinterpret_cast<double*>(take((size_t)n * 8));
+  out->columns = reinterpret_cast<int32_t*>(take((size_t)n_slots * 4));
+  out->norb = reinterpret_cast<int32_t*>(take((size_t)n * 4));
+  std::copy(row_start.begin(), row_start.end(), out->row_start);
+  std::copy(offs.begin(), offs.end(), out->offsets);
+  std::copy(cols.begin(), cols.end(), out->columns);
+  std::copy(norb.begin(), norb.end(), out->norb);
+  DCCHK(hipMemcpy(out->density, dd.P, (size_t)n_el * 8, hipMemcpyDeviceToHost));
+  if (uhf) DCCHK(hipMemcpy(out->spin_density, dd.S, (size_t)n_el * 8, hipMemcpyDeviceToHost));
+  for (int a = 0; a < n; ++a) {
+    double pop = 0.0;
+    for (int i = 0; i < norb[a]; ++i) pop += out->density[diag_off[a] + i * norb[a] + i];
+    out->charges[a] = core[a] - pop;
+  }
+#undef DCCHK
+  out->status = PM7_OK;
+  return PM7_OK;
+}
+
+}  // namespace
+
+pm7_status pm7_run_dandc(pm7_ctx* ctx, const pm7_molecule_v1* mol, const pm7_options_v2* opts,
+                         const pm7_dandc_options_v1* dopts, pm7_dandc_result_v1* out) {
+  if (!ctx || !mol || !out) return PM7_ERR_INVALID_INPUT;
+  std::memset(out, 0, sizeof *out);
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  pm7_options_v2 o;
+  if (opts) o = *opts;
+  else pm7_default_options(&o);
+  pm7_dandc_options_v1 d;
+  if (dopts) d = *dopts;
+  else pm7_dandc_default_options(&d);
+  if (!opts) o.method = ctx->method;
+  const pm7_status st = dandc_impl(ctx, *mol, o, d, out);
+  if (st != PM7_OK) {
+    pm7_dandc_result_free(out);
+    out->status = st;
+  }
+  return st;
+}
+
+void pm7_dandc_result_free(pm7_dandc_result_v1* r) {
+  if (!r) return;
+  std::free(r->row_start);  // the base of the result's single allocation
+  r->row_start = nullptr;
+  r->offsets = nullptr;
+  r->columns = nullptr;
+  r->norb = nullptr;
+  r->density = nullptr;
+  r->spin_density = nullptr;
+  r->charges = nullptr;
+}
+
+pm7_status pm7_dandc_partition(const pm7_molecule_v1* mol, int32_t core_size, double buffer_bohr,
+                               pm7_partition_v1* out) {
+  if (!out) return PM7_ERR_INVALID_INPUT;
+  std::memset(out, 0, sizeof *out);
+  if (!mol || mol->n_atoms <= 0 || !mol->xyz_bohr || core_size < 1 || !std::isfinite(buffer_bohr) ||
+      buffer_bohr <= 0.0)
+    return PM7_ERR_INVALID_INPUT;
+  for (int i = 0; i < 3 * mol->n_atoms; ++i)
+    if (!std::isfinite(mol->xyz_bohr[i])) return PM7_ERR_INVALID_INPUT;
+  DcPartition part;
+  dc_partition(mol->n_atoms, mol->xyz_bohr, core_size, buffer_bohr, part);
+  const int ns = part.n_sub();
+  const size_t bytes = (size_t)(ns + 1) * 8 + (size_t)ns * 4 + part.members.size() * 4;
+  char* blk = static_cast<char*>(std::malloc(bytes));
+  if (!blk) return PM7_ERR_INSUFFICIENT_MEMORY;
+  out->n_sub = ns;
+  out->start = reinterpret_cast<int64_t*>(blk);
+  out->n_core = reinterpret_cast<int32_t*>(blk + (size_t)(ns + 1) * 8);
+  out->members = out->n_core + ns;
+  std::copy(part.start.begin(), part.start.end(), out->start);
+  std::copy(part.n_core.begin(), part.n_core.end(), out->n_core);
+  std::copy(part.members.begin(), part.members.end(), out->members);
+  return PM7_OK;
+}
+
+void pm7_partition_free(pm7_partition_v1* p) {
+  if (!p) return;
+  std::free(p->start);  // the base of the single allocation
+  p->start = nullptr;
+  p->n_core = nullptr;
+  p->members = nullptr;
+  p->n_sub = 0;
 }
 
 void pm7_last_timings(const pm7_ctx* ctx, double* ms, int32_t n) {

@@ -66,37 +66,6 @@ __device__ inline double oc_val(const MolCtx& c, const ElemDev& e, int mu, int n
   return oc_int(mu, nu, la, si, e);
 }
 
-// MOPAC diagonal guess for one atom (pm7-rs src/scf.rs:1050-1114): the core charge spread over
-// the s/p orbitals (a main-group d shell gets only the charge correction), a transition metal's
-// filled s, then d, then p; yy spreads the molecular charge. Writes the n_orb diagonal entries.
-__device__ inline void sad_atom(const ElemDev& e, double yy, double scale, double* P, int o, int ld) {
-  double occ[9];
-  const int n = e.n_orb;
-  const double tore = e.core_charge;
-  for (int k = 0; k < 9; ++k) occ[k] = 0.0;
-  if (n == 1) {
-    occ[0] = tore - yy;
-  } else if (n == 4) {
-    for (int k = 0; k < 4; ++k) occ[k] = tore * 0.25 - yy;
-  } else if (n == 9) {
-    const int z = e.z;
-    if (z < 21 || (z > 30 && z < 39) || (z > 48 && z < 57)) {
-      for (int k = 0; k < 4; ++k) occ[k] = tore * 0.25 - yy;
-      for (int k = 4; k < 9; ++k) occ[k] = -yy;
-    } else {
-      double sum = tore - 9.0 * yy;
-      occ[0] = fmin(fmax(sum, 0.0), 2.0);
-      sum -= 2.0;
-      if (sum > 0.0) {
-        for (int k = 4; k < 9; ++k) occ[k] = fmin(fmax(sum * 0.2, 0.0), 2.0);
-        sum -= 10.0;
-        if (sum > 0.0)
-          for (int k = 1; k < 4; ++k) occ[k] = sum / 3.0;
-      }
-    }
-  }
-  for (int k = 0; k < n; ++k) P[(int64_t)(o + k) * ld + o + k] = occ[k] * scale;
-}
 
 __device__ inline int find_atom(const int* atom_pp, int a0, int na, int t) {
   int lo = 0, hi = na - 1;

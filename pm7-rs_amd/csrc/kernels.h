@@ -84,6 +84,53 @@ hipError_t launch_fock_response(const BatchDev& b, int mol, const double* xs, do
 hipError_t launch_fock_response_spin(const BatchDev& b, int mol, const double* xs, double* ts, double* gs,
                                      double* pps, int64_t pp_stride, int n_dens, hipStream_t s);
 
+// Divide and conquer: device arrays of one system's partition (the subsystems are the molecules of
+// a prepared batch, in subsystem order; "batch atoms" are the subsystem atoms, core atoms first).
+struct DcDev {
+  int n_atoms = 0, n_sub = 0, uhf = 0, n_batch_atoms = 0;
+  int64_t n_slots = 0, n_el = 0, n_states = 0;
+  const int64_t* diag_off = nullptr;  // per system atom: element offset of its diagonal block
+  const int* norb = nullptr;          // per system atom
+  const double* xyz = nullptr;        // system positions (Bohr)
+  const double* core = nullptr;       // system core charges
+  const int* parent = nullptr;        // per batch atom: its system atom
+  const int* atom_mol = nullptr;      // per batch atom: its subsystem
+  const int* nc_ao = nullptr;         // per subsystem: AOs of its core atoms
+  const int* aocode = nullptr;        // per subsystem AO: dc_ao_code(local atom, orbital, atom's AOs)
+  const int64_t* ao0 = nullptr;       // per subsystem: first entry in aocode
+  const int64_t* amap = nullptr;      // per subsystem local atom pair (row-major): global block offset or -1
+  const int64_t* amap0 = nullptr;     // per subsystem: first entry in amap
+  const int64_t* slot_off = nullptr;  // per stored block: element offset
+  const int* slot_dims = nullptr;     // per stored block: rows | cols << 8
+  const int64_t* cstart = nullptr;    // per stored block: first contribution (n_slots + 1 entries)
+  const DcContrib* contrib = nullptr;
+  const double* normalizer = nullptr;  // per stored block: 1 / total partition weight
+  double* P = nullptr;                 // total density (stored elements)
+  double* S = nullptr;                 // spin density (UHF)
+  double* pop = nullptr;               // per system atom
+  double* g = nullptr;                 // per system atom: global monopole potential
+  double* v_nuc = nullptr;             // per batch atom: nuclear environment
+  double* v_el = nullptr;              // per batch atom: electronic environment
+  double* wnorm = nullptr;             // like vecs: weighted norm per state (0 on padding)
+  double* e_part = nullptr;            // per subsystem: weighted energy
+  double* partial = nullptr;           // per stored block: squared change
+  double* scal = nullptr;              // [rms, e_el, fermi, fermi failed]
+};
+
+hipError_t launch_dc_solve(int variant, const BatchDev& b, int list_off, int count, const int* nc_ao,
+                           const double* v_el, int step, double* wnorm, double* e_part, hipStream_t s);
+hipError_t launch_dc_density(const BatchDev& b, int n_sub, const double* scal, double width, int step,
+                             hipStream_t s);
+hipError_t launch_dc_pop(const DcDev& d, hipStream_t s);
+// g = global monopole potential of charges q, v = each subsystem atom's environment from it.
+hipError_t launch_dc_field(const BatchDev& b, const DcDev& d, const double* q, double* g, double* v,
+                           hipStream_t s);
+hipError_t launch_dc_shift_h(const BatchDev& b, const DcDev& d, hipStream_t s);
+hipError_t launch_dc_gather(const BatchDev& b, const DcDev& d, hipStream_t s);
+hipError_t launch_dc_fermi(const BatchDev& b, const DcDev& d, double n_elec, double width, hipStream_t s);
+hipError_t launch_dc_assemble(const BatchDev& b, const DcDev& d, double mixing, hipStream_t s);
+hipError_t launch_dc_finish(const DcDev& d, hipStream_t s);
+
 // Large-system SCF of molecule `mi` (host copy M) of a prepared batch: grid-wide kernels and
 // rocBLAS/rocSOLVER on stream s; fills `out` as the megakernel would. Blocks the calling thread.
 struct LargeWork;

@@ -44,6 +44,31 @@ enum UhfSlot {
   U_AG = 9, U_T = 10, U_CA1 = 11, U_CB1 = 12, U_PT = 13, U_VG = 14
 };
 
+// Slab slots of a divide-and-conquer subsystem (no DIIS history: the SCF's ring is free).
+// RHF: gathered density, Fock, eigenvectors double-buffered across steps, eigensolver scratch,
+// the occupation-weighted local density and the occupation-scaled eigenvectors.
+enum DcRhfSlot { DR_P = 24, DR_F = 1, DR_C0 = 3, DR_C1 = 4, DR_VG = 5, DR_AG = 6, DR_T = 7, DR_D = 9, DR_CS = 10 };
+// UHF: total, alpha and beta densities, the two Fock matrices, eigenvectors per spin (double
+// buffered), scratch, the two spin channels' weighted local densities and the scaled vectors.
+enum DcUhfSlot {
+  DU_PT = 13, DU_PA = 3, DU_PB = 4, DU_FA = 1, DU_FB = 2, DU_CA0 = 7, DU_CA1 = 11, DU_CB0 = 8, DU_CB1 = 12,
+  DU_AG = 9, DU_T = 10, DU_VG = 14, DU_DA = 15, DU_DB = 16, DU_CS = 17
+};
+
+// One subsystem-to-global contribution of the divide-and-conquer density assembly: the element
+// (mu, nu) of the atom-pair block it feeds reads slab[base + mu * ld + nu] (the alpha, or only,
+// channel; a UHF subsystem's beta channel sits dstep doubles further) with partition weight w.
+struct DcContrib {
+  int64_t base;
+  int64_t dstep;
+  int ld;
+  int pad;
+  double w;
+};
+
+// Packed local-AO descriptor of a subsystem: local atom | orbital in atom << 16 | atom's AOs << 24.
+__host__ __device__ inline int dc_ao_code(int atom, int mu, int norb) { return atom | (mu << 16) | (norb << 24); }
+
 struct ElemDev {
   double u_ss, u_pp, beta_s, beta_p, core_charge;
   double g_ss, g_sp, g_pp, g_p2, h_sp;
@@ -99,6 +124,38 @@ struct MolOut {
   int iterations, converged, sweeps, p_slot, c_slot, cb_slot, pb_slot, fp32_iters;
   unsigned long long ticks[NPHASE];
 };
+
+// MOPAC diagonal guess for one atom (pm7-rs src/scf.rs:1050-1114): the core charge spread over
+// the s/p orbitals (a main-group d shell gets only the charge correction), a transition metal's
+// filled s, then d, then p; yy spreads the molecular charge. Writes the n_orb diagonal entries.
+__host__ __device__ inline void sad_atom(const ElemDev& e, double yy, double scale, double* P, int o, int ld) {
+  double occ[9];
+  const int n = e.n_orb;
+  const double tore = e.core_charge;
+  for (int k = 0; k < 9; ++k) occ[k] = 0.0;
+  if (n == 1) {
+    occ[0] = tore - yy;
+  } else if (n == 4) {
+    for (int k = 0; k < 4; ++k) occ[k] = tore * 0.25 - yy;
+  } else if (n == 9) {
+    const int z = e.z;
+    if (z < 21 || (z > 30 && z < 39) || (z > 48 && z < 57)) {
+      for (int k = 0; k < 4; ++k) occ[k] = tore * 0.25 - yy;
+      for (int k = 4; k < 9; ++k) occ[k] = -yy;
+    } else {
+      double sum = tore - 9.0 * yy;
+      occ[0] = sum < 0.0 ? 0.0 : (sum > 2.0 ? 2.0 : sum);
+      sum -= 2.0;
+      if (sum > 0.0) {
+        for (int k = 4; k < 9; ++k) occ[k] = sum * 0.2 < 0.0 ? 0.0 : (sum * 0.2 > 2.0 ? 2.0 : sum * 0.2);
+        sum -= 10.0;
+        if (sum > 0.0)
+          for (int k = 1; k < 4; ++k) occ[k] = sum / 3.0;
+      }
+    }
+  }
+  for (int k = 0; k < n; ++k) P[(int64_t)(o + k) * ld + o + k] = occ[k] * scale;
+}
 
 __host__ __device__ inline int npair_of(int norb) { return norb * (norb + 1) / 2; }
 

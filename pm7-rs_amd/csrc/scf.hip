@@ -77,7 +77,7 @@ __device__ __attribute__((noinline)) double commutator(int n, int ld, const doub
 }
 
 // P' = w C_occ C_occ^T (C columns sorted ascending) on the matrix cores, upper-triangle tiles
-// mirrored; returns sum (P' - P)^2 over n x n. T is the precision of the product: FP64, or FP32
+// mirrored; returns sum (P' - P)^2 over n x n (0 when Pold is null). T is the precision of the product: FP64, or FP32
 // inputs with FP32 accumulation in the reduced-precision phase.
 template <int NT, typename T>
 __device__ __attribute__((noinline)) double density(int n, int ld, int nocc, double wgt, const double* __restrict__ C,
@@ -125,6 +125,7 @@ __device__ __attribute__((noinline)) double density(int n, int ld, int nocc, dou
         if (i >= n || j >= n) continue;
         Pn[(int64_t)i * ld + j] = v[s][u];
         if (I != J) Pn[(int64_t)j * ld + i] = v[s][u];
+        if (!Pold) continue;
         const double d = v[s][u] - Pold[(int64_t)i * ld + j];
         dsq += (I == J) ? d * d : 2.0 * d * d;
       }
@@ -821,6 +822,220 @@ __global__ void __launch_bounds__(NT) k_fock_response_spin(int mi, const MolDev*
   }
   __syncthreads();
   fock_build<NT, double>(c, Xt, Xa, 1.0, Xb, 1.0, gs + blockIdx.x * sq, gs + (gridDim.x + blockIdx.x) * sq);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Divide and conquer (Dixon-Merz; pm7-rs src/dandc/mod.rs). One workgroup per subsystem, the
+// subsystems of a system one launch; every subsystem keeps its integrals, core Hamiltonian and
+// last eigenvectors in its slab across the SCF steps.
+// ---------------------------------------------------------------------------------------------
+
+// One subsystem step (solve_subsystem, src/dandc/mod.rs:520-626): F = H' + G(P) + diag(v_el)
+// over the subsystem's own basis, where H' is the core Hamiltonian less the environment's nuclear
+// potential (subtracted once, at set-up) and v_el the electronic environment of each atom
+// (src/dandc/mod.rs:545-564); the partition-weighted energy 1/2 sum w_{mu nu} P (H' + F_alpha)
+// with w = 1 core-core, 1/2 core-buffer, 0 buffer-buffer (src/dandc/mod.rs:593-613, weights
+// src/dandc/partition.rs:60-67); (eps, C) = eig(F), warm-started from the previous step's vectors
+// (cold at step 0); and per state the core-weighted norm sum_{mu in core} C_{mu k}^2 times the
+// channel's spin scale (1 RHF, 1/2 per UHF channel), the weights of the common Fermi level
+// (weighted_spectrum, src/dandc/mod.rs:667-712). Core atoms come first, so the core AOs are
+// 0 .. nc_ao[mi]. UHF: F_s = H' + J(P) - K(P_s) for s = alpha, beta (src/dandc/mod.rs:566-591).
+template <int NT, int MAXN, bool A_LDS, bool V_LDS>
+__global__ void __launch_bounds__(NT)
+    k_dc_solve(const int* __restrict__ mol_list, const MolDev* __restrict__ mols, const int* __restrict__ atom_z,
+               const int* __restrict__ atom_ao, const int* __restrict__ atom_pp, const PairDev* __restrict__ pairs,
+               const ElemDev* __restrict__ elem, const double* __restrict__ ocd, const double* __restrict__ w,
+               double* __restrict__ slab, double* __restrict__ vecs, const int* __restrict__ nc_ao,
+               const double* __restrict__ v_el, int step, double* __restrict__ wnorm,
+               double* __restrict__ e_part) {
+  __shared__ double s_a[A_LDS ? MAXN * (MAXN + 1) / 2 : 1];
+  __shared__ double s_v[V_LDS ? MAXN * MAXN : 1];
+  __shared__ double s_red[(Wg<NT>::NW + 1) * 3 * HIST];
+  __shared__ double2 s_cs[MAXN / 2];
+  __shared__ double s_t[MAXN / 2];
+  __shared__ int s_perm[MAXN];
+  __shared__ int s_pq[MAXN / 2];
+  __shared__ unsigned long long s_tk[3];
+  const int mi = mol_list[blockIdx.x];
+  MolCtx c;
+  c.M = mols[mi];
+  c.atom_z = atom_z;
+  c.atom_ao = atom_ao;
+  c.atom_pp = atom_pp;
+  c.pairs = pairs;
+  c.elem = elem;
+  c.ocd = ocd;
+  c.w = w;
+  c.ld = c.M.ld;
+  c.base = slab + c.M.mat0;
+  c.pp = slab + c.M.pp0;
+  const int n = c.M.nao, ld = c.ld, nc = nc_ao[mi];
+  const bool uhf = c.M.uhf;
+  double* eps = vecs + c.M.vec0;
+  double* wn = wnorm + c.M.vec0;
+  const double* H = c.slot(S_H);
+  double* Fa = c.slot(uhf ? (int)DU_FA : (int)DR_F);
+  double* Fb = c.slot(DU_FB);
+  const double* Pt = c.slot(uhf ? (int)DU_PT : (int)DR_P);
+  if (uhf) fock_build<NT, double>(c, Pt, c.slot(DU_PA), 1.0, c.slot(DU_PB), 1.0, Fa, Fb);
+  else fock_build<NT, double>(c, Pt, Pt, 0.5, nullptr, 0.0, Fa, nullptr);
+  // Electronic environment on the Fock diagonal (both spins).
+  for (int la = threadIdx.x; la < c.M.n_atoms; la += NT) {
+    const int a = c.M.atom0 + la;
+    const int o = atom_ao[a], no = elem[atom_z[a]].n_orb;
+    const double v = v_el[a];
+    for (int k = 0; k < no; ++k) {
+      const int64_t d = (int64_t)(o + k) * ld + o + k;
+      Fa[d] += v;
+      if (uhf) Fb[d] += v;
+    }
+  }
+  __syncthreads();
+  double e = 0.0;
+  for (int idx = threadIdx.x; idx < n * n; idx += NT) {
+    const int i = idx / n, j = idx - i * n;
+    const double wgt = 0.5 * ((i < nc ? 1.0 : 0.0) + (j < nc ? 1.0 : 0.0));
+    if (wgt == 0.0) continue;
+    const int64_t x = (int64_t)i * ld + j;
+    e += 0.5 * wgt * Pt[x] * (H[x] + Fa[x]);
+  }
+  e = wg_sum<NT>(e, s_red);
+  if (threadIdx.x == 0) {
+    e_part[mi] = e;
+    s_tk[0] = s_tk[1] = s_tk[2] = 0;
+  }
+  const double tol = exp10((double)-c.M.eig_tol_exp);
+  const int nch = uhf ? 2 : 1;
+  for (int ch = 0; ch < nch; ++ch) {
+    const double* W = ch == 0 ? Fa : Fb;
+    int c_new, c_old, ag, vg, tg;
+    if (!uhf) {
+      c_new = (step & 1) ? DR_C1 : DR_C0;
+      c_old = (step & 1) ? DR_C0 : DR_C1;
+      ag = DR_AG, vg = DR_VG, tg = DR_T;
+    } else {
+      c_new = ch == 0 ? ((step & 1) ? DU_CA1 : DU_CA0) : ((step & 1) ? DU_CB1 : DU_CB0);
+      c_old = ch == 0 ? ((step & 1) ? DU_CA0 : DU_CA1) : ((step & 1) ? DU_CB0 : DU_CB1);
+      ag = DU_AG, vg = DU_VG, tg = DU_T;
+    }
+    double* ev = eps + ch * ld;
+    eigensolve<NT, MAXN, A_LDS, V_LDS, double>(n, ld, W, step > 0 ? c.slot(c_old) : nullptr, c.slot(c_new), ev,
+                                               c.slot(ag), c.slot(vg), c.slot(tg), s_a, s_v, s_cs, s_t, s_perm,
+                                               s_pq, s_red, s_tk, tol, 40, 0);
+    __syncthreads();
+    const double* C = c.slot(c_new);
+    const double scale = uhf ? 0.5 : 1.0;
+    for (int k = threadIdx.x; k < ld; k += NT) {
+      double s = 0.0;
+      if (k < n)
+        for (int mu = 0; mu < nc; ++mu) {
+          const double v = C[(int64_t)mu * ld + k];
+          s += v * v;
+        }
+      wn[ch * ld + k] = scale * s;
+      if (k >= n) ev[k] = 0.0;
+    }
+    __syncthreads();
+  }
+  if (!uhf)
+    for (int k = threadIdx.x; k < ld; k += NT) {
+      wn[ld + k] = 0.0;
+      eps[ld + k] = 0.0;
+    }
+}
+
+// Fermi-Dirac occupation of a spin-degenerate orbital (src/dandc/mod.rs:649-660).
+__device__ inline double dc_occupation(double e, double fermi, double width) {
+  const double x = (e - fermi) / width;
+  if (x > 40.0) return 0.0;
+  if (x < -40.0) return 2.0;
+  return 2.0 / (1.0 + exp(x));
+}
+
+// The subsystem's occupation-weighted density at the common Fermi level (scatter_density,
+// src/dandc/mod.rs:772-821): D_s = sum_k f_k s C_k C_k^T with f the Fermi-Dirac occupation
+// (dropped below 1e-14) and s the channel's spin scale (1 RHF, 1/2 per UHF channel), as
+// (C sqrt(f s)) (C sqrt(f s))^T on the matrix cores. The states are sorted ascending, so the
+// occupied ones are a prefix. Written to DR_D (RHF) or DU_DA / DU_DB.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_dc_density(const MolDev* __restrict__ mols, double* __restrict__ slab,
+                                                     const double* __restrict__ vecs,
+                                                     const double* __restrict__ scal, double width, int step) {
+  __shared__ double s_red[(Wg<NT>::NW + 1) * 3 * HIST];
+  __shared__ int s_cnt;
+  const MolDev M = mols[blockIdx.x];
+  const int n = M.nao, ld = M.ld;
+  double* base = slab + M.mat0;
+  const int64_t sq = (int64_t)ld * ld;
+  const double fermi = scal[2];
+  const int nch = M.uhf ? 2 : 1;
+  for (int ch = 0; ch < nch; ++ch) {
+    int cs;
+    double* D;
+    const double* C;
+    if (!M.uhf) {
+      C = base + sq * ((step & 1) ? DR_C1 : DR_C0);
+      D = base + sq * DR_D;
+      cs = DR_CS;
+    } else {
+      C = base + sq * (ch == 0 ? ((step & 1) ? DU_CA1 : DU_CA0) : ((step & 1) ? DU_CB1 : DU_CB0));
+      D = base + sq * (ch == 0 ? DU_DA : DU_DB);
+      cs = DU_CS;
+    }
+    double* Cs = base + sq * cs;
+    const double* ev = vecs + M.vec0 + ch * ld;
+    const double scale = M.uhf ? 0.5 : 1.0;
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
+    // Occupied prefix length, then the scaled columns.
+    for (int k = threadIdx.x; k < n; k += NT) {
+      const double f = dc_occupation(ev[k], fermi, width) * scale;
+      if (f >= 1.0e-14) atomicMax(&s_cnt, k + 1);
+    }
+    __syncthreads();
+    const int nocc = s_cnt;
+    for (int idx = threadIdx.x; idx < n * nocc; idx += NT) {
+      const int mu = idx / nocc, k = idx - mu * nocc;
+      const double f = dc_occupation(ev[k], fermi, width) * scale;
+      Cs[(int64_t)mu * ld + k] = f >= 1.0e-14 ? C[(int64_t)mu * ld + k] * sqrt(f) : 0.0;
+    }
+    __syncthreads();
+    density<NT, double>(n, ld, nocc, 1.0, Cs, nullptr, D, s_red);
+    __syncthreads();
+  }
+}
+
+#define PM7_DC_INSTANCE(NT, MAXN, AL, VL)                                                                          \
+  template __global__ void k_dc_solve<NT, MAXN, AL, VL>(const int*, const MolDev*, const int*, const int*, const int*, \
+                                                        const PairDev*, const ElemDev*, const double*, const double*, \
+                                                        double*, double*, const int*, const double*, int, double*,   \
+                                                        double*);
+PM7_DC_INSTANCE(256, 64, true, true)
+PM7_DC_INSTANCE(512, 112, true, true)
+PM7_DC_INSTANCE(512, 196, true, false)
+
+hipError_t launch_dc_solve(int variant, const BatchDev& b, int list_off, int count, const int* nc_ao,
+                           const double* v_el, int step, double* wnorm, double* e_part, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  const int* list = b.mol_list + list_off;
+#define PM7_DC_LAUNCH(NT, MAXN, AL, VL)                                                                            \
+  hipLaunchKernelGGL((k_dc_solve<NT, MAXN, AL, VL>), dim3(count), dim3(NT), 0, s, list, b.mols, b.atom_z, b.atom_ao, \
+                     b.atom_pp, b.pairs, b.elem, b.ocd, b.w, b.slab, b.vecs, nc_ao, v_el, step, wnorm, e_part)
+  switch (variant) {
+    case 0: PM7_DC_LAUNCH(256, 64, true, true); break;
+    case 1: PM7_DC_LAUNCH(512, 112, true, true); break;
+    case 2: PM7_DC_LAUNCH(512, 196, true, false); break;
+    default: return hipErrorInvalidValue;  // subsystems above 196 basis functions are refused by the host
+  }
+#undef PM7_DC_LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t launch_dc_density(const BatchDev& b, int n_sub, const double* scal, double width, int step,
+                             hipStream_t s) {
+  if (n_sub <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dc_density<256>, dim3(n_sub), dim3(256), 0, s, b.mols, b.slab, b.vecs, scal, width, step);
+  return hipGetLastError();
 }
 
 hipError_t launch_scf(int variant, const BatchDev& b, int list_off, int count, hipStream_t s) {

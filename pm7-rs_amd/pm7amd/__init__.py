@@ -106,6 +106,8 @@ ABI_FUNCTIONS = [
     "pm7_fock_spin", "pm7_core_core_energy", "pm7_last_timings", "pm7_server_default_config",
     "pm7_server_start", "pm7_server_submit", "pm7_server_poll", "pm7_server_stop", "pm7_gradient_batch",
     "pm7_gradient", "pm7_response_fock_batch", "pm7_response_fock_spin_batch", "pm7_fock_spin_noalloc",
+    "pm7_dandc_default_options", "pm7_run_dandc", "pm7_dandc_result_free", "pm7_dandc_partition",
+    "pm7_partition_free",
 ]
 
 _lib = None

@@ -135,6 +135,22 @@ XH = {"C": 1.09, "N": 1.01, "O": 0.96}
 VALENCE = {"C": 4, "N": 3, "O": 2}
 
 
+def flat_alkane(n_carbon: int):
+    """Linear alkane C_nH_2n+2 in the flat all-anti zig-zag of pm7-rs's divide-and-conquer tests
+    and scaling record (tests/dandc.rs:95-127, tests/dandc_scaling.rs:24-44): C at (1.26 i, 0 or
+    0.44, 0) Angstrom, two H per carbon at z = +-0.89, one cap H on each end."""
+    dx, dy = 1.26, 0.44
+    atoms = []
+    for i in range(n_carbon):
+        x = i * dx
+        y = 0.0 if i % 2 == 0 else dy
+        hy = y + (-0.5 if i % 2 == 0 else 0.5)
+        atoms += [["C", x, y, 0.0], ["H", x, hy, 0.89], ["H", x, hy, -0.89]]
+    atoms.append(["H", -1.09, 0.0, 0.0])
+    atoms.append(["H", (n_carbon - 1) * dx + 1.09, 0.0 if (n_carbon - 1) % 2 == 0 else dy, 0.0])
+    return atoms
+
+
 def _random_rotation(rng: Rng):
     q = _unit([rng.gauss() for _ in range(4)])
     w, x, y, z = q

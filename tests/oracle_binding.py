@@ -51,6 +51,19 @@ class OracleResult(C.Structure):
                 ("dipole_pd", C.c_double * 3), ("message", C.c_char * 256)]
 
 
+class OracleDandcOptions(C.Structure):
+    _fields_ = [("buffer_bohr", C.c_double), ("core_size", C.c_int), ("max_scf", C.c_int),
+                ("fermi_width_ev", C.c_double), ("p_tol", C.c_double), ("mixing", C.c_double)]
+
+
+class OracleDandcResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("converged", C.c_int), ("iterations", C.c_int), ("unrestricted", C.c_int),
+                ("subsystems", C.c_int), ("largest_subsystem", C.c_int), ("largest_subsystem_nao", C.c_int),
+                ("nao", C.c_int), ("stored_pairs", C.c_longlong), ("stored_elements", C.c_longlong),
+                ("fermi_ev", C.c_double), ("electronic_ev", C.c_double), ("density_error", C.c_double),
+                ("message", C.c_char * 256)]
+
+
 _lib = None
 
 
@@ -77,6 +90,10 @@ def lib():
         _lib.oracle_default_options.argtypes = [C.POINTER(OracleOptions)]
         _lib.oracle_overlap_locals.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
                                                C.c_double, C.c_int, dp]
+        _lib.oracle_default_dandc_options.argtypes = [C.POINTER(OracleDandcOptions)]
+        _lib.oracle_run_dandc.argtypes = [C.c_char_p, C.c_int, ip, dp, C.POINTER(OracleOptions),
+                                          C.POINTER(OracleDandcOptions), C.POINTER(OracleDandcResult), dp, dp, dp]
+        _lib.oracle_dandc_partition.argtypes = [C.c_int, dp, C.c_int, C.c_double, ip, ip, ip, ip, C.c_int]
     return _lib
 
 
@@ -214,3 +231,36 @@ def overlap_locals(na, zsa, zpa, nb, zsb, zpb, r, numeric=False):
     if st:
         raise RuntimeError("no closed form for these principal quantum numbers")
     return out
+
+
+def dandc_partition(xyz_bohr, core_size, buffer):
+    """[(members, n_core)] per subsystem (src/dandc/partition.rs:97-133)."""
+    xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
+    n = xyz.shape[0]
+    ns = C.c_int(0)
+    total = lib().oracle_dandc_partition(n, _d(xyz), int(core_size), float(buffer), C.byref(ns), None, None, None, 0)
+    ncore = np.zeros(ns.value, np.int32)
+    start = np.zeros(ns.value + 1, np.int32)
+    members = np.zeros(max(total, 1), np.int32)
+    lib().oracle_dandc_partition(n, _d(xyz), int(core_size), float(buffer), C.byref(ns), _i(ncore), _i(start),
+                                 _i(members), len(members))
+    return [(list(members[start[k]:start[k + 1]]), int(ncore[k])) for k in range(ns.value)]
+
+
+def run_dandc(z, xyz_bohr, charge=0.0, multiplicity=1, buffer=15.0, core_size=12, max_scf=200, fermi_width_ev=0.1,
+              p_tol=1.0e-6, mixing=0.4, **kw):
+    """Divide-and-conquer SCF: (result, dense density, dense spin density, charges)."""
+    z = np.ascontiguousarray(z, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64)
+    n = len(z)
+    m = max(nao(z), 1)
+    o = default_options(charge=float(charge), multiplicity=int(multiplicity), **kw)
+    d = OracleDandcOptions(float(buffer), int(core_size), int(max_scf), float(fermi_width_ev), float(p_tol),
+                           float(mixing))
+    res = OracleDandcResult()
+    dens = np.zeros((m, m))
+    spin = np.zeros((m, m))
+    q = np.zeros(n)
+    lib().oracle_run_dandc(DATA_DIR.encode(), n, _i(z), _d(xyz), C.byref(o), C.byref(d), C.byref(res), _d(dens),
+                           _d(spin), _d(q))
+    return res, dens, spin, q

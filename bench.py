@@ -154,6 +154,7 @@ def main() -> int:
                     help="S/P/Cl molecules (MNDO/d pairs, <= 196 AOs) batched in FP64 and mixed (0: skip)")
     ap.add_argument("--gradient", type=int, default=1, help="time batched nuclear gradients (0: skip)")
     ap.add_argument("--response", type=int, default=1, help="time the CPHF response Fock (0: skip)")
+    ap.add_argument("--dandc", type=int, default=1, help="divide-and-conquer scaling series (0: skip)")
     ap.add_argument("--serve-lanes", type=int, default=4)
     ap.add_argument("--serve-batch", type=int, default=64)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -497,6 +498,68 @@ def main() -> int:
                                                   "each call includes one integral build (integral_build_ms)"}
         print(f"[bench] response fock: {response}", file=sys.stderr, flush=True)
 
+    # Divide and conquer: the reference's scaling record (docs/divide_and_conquer.md:85-92: flat
+    # alkanes, buffer 15 Bohr, core_size 8, D&C p_tol 1e-7; exact SCF e_tol 1e-10, p_tol 1e-8;
+    # 16 cores, single wall-clock runs) on the GPU, the CPU oracle's D&C beside it on the small sizes.
+    dandc = None
+    if args.dandc:
+        from pm7amd.dandc import DandcOptions, run_dandc
+
+        ref_s = {40: 2.08, 80: 5.07, 160: 9.53, 320: 19.32, 640: 39.51}
+        ref_exact_s = {40: 0.54, 80: 2.48, 160: 15.42}
+        ref_de = {40: 0.0019, 80: 0.0043, 160: 0.0089}
+        dopt = DandcOptions(buffer=15.0, core_size=8, p_tol=1e-7, max_scf=400)
+        xopt = pm7amd.Pm7Options(e_tol=1e-10, p_tol=1e-8, max_scf=400)
+        gpu_e = {}
+        rows = []
+        for nc in (40, 80, 160, 320, 640):
+            atoms = W.flat_alkane(nc)
+            dmol = pm7amd.Molecule.from_symbols_angstrom(atoms)
+            if nc == 40:
+                run_dandc(dmol, dandc=dopt, engine=eng)
+            t = time.perf_counter()
+            r = run_dandc(dmol, dandc=dopt, engine=eng)
+            wall = time.perf_counter() - t
+            gpu_e[nc] = r.electronic_ev
+            row = {"carbons": nc, "atoms": len(atoms), "wall_s": round(wall, 4), "device_ms": round(r.device_ms, 2),
+                   "iterations": r.iterations, "ms_per_iteration": round(r.device_ms / r.iterations, 3),
+                   "subsystems": r.subsystems, "largest_subsystem": r.largest_subsystem,
+                   "largest_subsystem_nao": r.largest_subsystem_nao, "converged": r.converged,
+                   "reference_s_16_cores": ref_s[nc], "speedup_vs_reference": round(ref_s[nc] / wall, 1)}
+            if nc <= 160:
+                t = time.perf_counter()
+                ex = eng.run(dmol, xopt)
+                row["exact_wall_s"] = round(time.perf_counter() - t, 4)
+                row["reference_exact_s_16_cores"] = ref_exact_s[nc]
+                row["dE_vs_exact_ev"] = round(abs(r.electronic_ev - ex.electronic_ev), 6)
+                row["reference_dE_vs_exact_ev"] = ref_de[nc]
+                row["stored_fraction"] = round(r.density.stored_elements() / ex.density.shape[0] ** 2, 4)
+            rows.append(row)
+        xs = np.log([r_["atoms"] for r_ in rows])
+        dandc = {"workload": "flat all-anti alkanes C40-C640 (122-1922 atoms), buffer 15 Bohr, core_size 8, "
+                             "p_tol 1e-7 (the reference's scaling record)",
+                 "series": rows,
+                 "log_log_slope_wall": round(float(np.polyfit(xs, np.log([r_["wall_s"] for r_ in rows]), 1)[0]), 3),
+                 "log_log_slope_per_iteration": round(float(np.polyfit(
+                     xs, np.log([r_["ms_per_iteration"] for r_ in rows]), 1)[0]), 3)}
+        if not args.no_cpu:
+            sys.path.insert(0, str(ROOT / "tests"))
+            import oracle_binding as ob
+
+            cpu_rows = []
+            for nc in (40, 80):
+                z1, x1 = ob.atoms_to_arrays(W.flat_alkane(nc))
+                t = time.perf_counter()
+                rr, *_ = ob.run_dandc(z1, x1, buffer=15.0, core_size=8, p_tol=1e-7, max_scf=400,
+                                      threads=min(16, os.cpu_count() or 1))
+                cpu_rows.append({"carbons": nc, "wall_s": round(time.perf_counter() - t, 3),
+                                 "iterations": rr.iterations,
+                                 "abs_dE_vs_gpu_ev": abs(rr.electronic_ev - gpu_e[nc])})
+            dandc["cpu_baseline"] = {"kind": "port", "cores": min(16, os.cpu_count() or 1), "series": cpu_rows,
+                                     "sample": "the CPU oracle's divide and conquer on C40 and C80, subsystems in "
+                                               "parallel (one thread each)"}
+        print(f"[bench] dandc: {dandc}", file=sys.stderr, flush=True)
+
     # One molecule at a time (the plain FP64 GPU path the batched path is compared with).
     single = mols[: args.single]
     for m in single[:2]:
@@ -660,6 +723,7 @@ def main() -> int:
         "d_batch": d_batch,
         "gradient": gradient,
         "response_fock": response,
+        "dandc": dandc,
         "single_molecule_fp64": {"solves_per_s": round(single_rate, 3), "molecules": len(single),
                                  "batched_speedup": round((solvesmx if head_mixed else solves64) / single_rate, 3)},
         "kernel_ms": {"integrals": round(float(np.mean([k[0] for k in kern64])), 4),

@@ -171,6 +171,7 @@ struct Lane {
   DevBuf d_gpair, d_gatom, d_gitems, d_snumg;  // nuclear gradient (pm7_gradient_batch only)
   DevBuf d_rx, d_rg, d_rpp, d_rstage;          // response densities, their G, packed scratch, staging
   PinnedBuf h_out;  // staging for the packed results
+  std::vector<int2> h_gitems;  // host source of d_gitems
   LargeWork* large = nullptr;  // rocBLAS handle and buffers of the large-system path (lazily made)
   // Several large-system SCFs in lockstep: one work set and one stream each (lazily made).
   std::vector<LargeWork*> group_work;
@@ -466,7 +467,17 @@ pm7_status validate(pm7_ctx* ctx, const pm7_molecule_v1& m, const pm7_options_v2
   return PM7_OK;
 }
 
+// Host copies of a batch's device layout (the sources of its uploads).
+struct HostLayout {
+  std::vector<int> atom_z, atom_ao, atom_pp, list;
+  std::vector<double> xyz;
+  std::vector<PairDev> pairs;
+  std::vector<int2> items;
+  std::vector<int64_t> gath_off, num_pairs, d_pairs;
+};
+
 struct Prepared {
+  HostLayout host;
   std::vector<Job> jobs;
   std::list<std::vector<double>> staging;  // host copies of seeded densities (stable addresses)
   double need_mb = 0.0, budget_mb = 0.0;   // INSUFFICIENT_MEMORY payload
@@ -521,13 +532,20 @@ size_t job_bytes(const pm7_ctx* ctx, const Job& j) {
 // Lay out, upload and run the once-per-geometry kernels.
 pm7_status prepare(pm7_ctx* ctx, Lane& L, Prepared& P, int64_t max_memory_mb) {
   const int nm = (int)P.jobs.size();
-  std::vector<int> atom_z, atom_ao, atom_pp;
-  std::vector<double> xyz;
-  std::vector<PairDev> pairs;
-  std::vector<int2> items;
-  std::vector<int64_t> gath_off(nm);
-  std::vector<int64_t> num_pairs;  // pairs whose overlaps need the quadrature (a shell with n >= 4)
-  std::vector<int64_t> d_pairs;    // pairs of molecules with a d-bearing atom
+  // The host arrays of the uploads live in P: hipMemcpyAsync from pageable memory may read the
+  // source after it returns, so a source must outlive the stream's use of it (the call's end).
+  HostLayout& hl = P.host;
+  std::vector<int>& atom_z = hl.atom_z;
+  std::vector<int>& atom_ao = hl.atom_ao;
+  std::vector<int>& atom_pp = hl.atom_pp;
+  std::vector<double>& xyz = hl.xyz;
+  std::vector<PairDev>& pairs = hl.pairs;
+  std::vector<int2>& items = hl.items;
+  std::vector<int64_t>& gath_off = hl.gath_off;
+  std::vector<int64_t>& num_pairs = hl.num_pairs;  // pairs whose overlaps need the quadrature (a shell with n >= 4)
+  std::vector<int64_t>& d_pairs = hl.d_pairs;      // pairs of molecules with a d-bearing atom
+  hl = HostLayout();
+  gath_off.assign(nm, 0);
   int64_t w_off = 0, mat = 0, vec = 0, gath = 0, pair0 = 0;
   int atom0 = 0;
   P.mols.resize(nm);
@@ -623,7 +641,7 @@ pm7_status prepare(pm7_ctx* ctx, Lane& L, Prepared& P, int64_t max_memory_mb) {
     atom0 += n;
     pair0 += (int64_t)n * (n - 1) / 2;
   }
-  std::vector<int> list;
+  std::vector<int>& list = hl.list;
   for (int v = 0; v < SCF_VARIANTS; ++v) {
     P.list_off[v] = (int)list.size();
     // Largest molecules first so the longest SCFs start first.
@@ -1138,7 +1156,8 @@ void corrections_grad_kcal(const Job& j, double* g) {
 // array. A molecule with a pair that has no device gradient (its SCF left no density) gets NaN and
 // PM7_ERR_UNSUPPORTED.
 pm7_status launch_gradient_batch(Lane& L, const Prepared& P, int* n_items_out) {
-  std::vector<int2> items;
+  std::vector<int2>& items = L.h_gitems;  // the upload's source outlives the call (see HostLayout)
+  items.clear();
   int n_atoms = 0;
   for (size_t mi = 0; mi < P.jobs.size(); ++mi) {
     const Job& j = P.jobs[mi];

@@ -28,11 +28,13 @@
 #include <cstdio>
 #include <mutex>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "diis.h"
 #include "fock.h"
 #include "kernels.h"
+#include "lockstep.h"
 #include "pm7_device.h"
 
 namespace pm7 {
@@ -549,12 +551,7 @@ struct EigBatcher {
     double* eps;
     double* Ct;
   };
-  std::mutex mu;
-  std::condition_variable cv;
-  int active = 0, arrived = 0;
-  uint64_t round = 0;
-  hipError_t status = hipSuccess;
-  std::vector<Item> items;
+  std::unique_ptr<Lockstep<Item, hipError_t>> sync;
   std::vector<hipEvent_t> ready;
   rocblas_handle blas = nullptr;
   hipStream_t st = nullptr;
@@ -562,70 +559,52 @@ struct EigBatcher {
   Buf work, D, E, info;
   int nmax = 0, cap = 0;
 
-  // Called with the lock held by the last participant of a round.
-  void run_round() {
-    if (!items.empty() && status == hipSuccess) {
-      const int cnt = (int)items.size();
-      const int64_t nn = (int64_t)nmax * nmax;
-      double* wk = static_cast<double*>(work.p);
-      for (int k = 0; k < cnt && status == hipSuccess; ++k) {
-        status = hipStreamWaitEvent(st, ready[k], 0);
-        hipLaunchKernelGGL(k_lg_pad, dim3(grid_for(nn)), dim3(256), 0, st, items[k].n, items[k].ld, nmax, items[k].W,
-                           wk + k * nn);
-      }
-      if (status == hipSuccess &&
-          rocsolver_dsyevd_strided_batched(blas, rocblas_evect_original, rocblas_fill_upper, nmax, wk, nmax, nn,
-                                           static_cast<double*>(D.p), nmax, static_cast<double*>(E.p), nmax,
-                                           static_cast<rocblas_int*>(info.p), cnt) != rocblas_status_success)
-        status = hipErrorUnknown;
-      for (int k = 0; k < cnt && status == hipSuccess; ++k) {
-        const int64_t sq = (int64_t)items[k].ld * items[k].ld;
-        hipLaunchKernelGGL(k_lg_unpad, dim3(grid_for(sq)), dim3(256), 0, st, items[k].n, items[k].ld, nmax,
-                           wk + k * nn, static_cast<double*>(D.p) + (int64_t)k * nmax, items[k].Ct, items[k].eps);
-      }
-      if (status == hipSuccess) status = hipGetLastError();
-      if (status == hipSuccess) status = hipEventRecord(done, st);
+  // One round's batched solve (run by the last participant to arrive, lock held): each matrix
+  // padded into the strided batch once its owner's stream reached it, one dsyevd for all, the
+  // vectors and values copied back; `done` marks the end for the owners' streams.
+  hipError_t run_round(std::vector<Item>& items) {
+    const int cnt = (int)items.size();
+    const int64_t nn = (int64_t)nmax * nmax;
+    double* wk = static_cast<double*>(work.p);
+    for (int k = 0; k < cnt; ++k) {
+      const hipError_t e = hipStreamWaitEvent(st, ready[k], 0);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL(k_lg_pad, dim3(grid_for(nn)), dim3(256), 0, st, items[k].n, items[k].ld, nmax, items[k].W,
+                         wk + k * nn);
     }
-    items.clear();
-    arrived = 0;
-    ++round;
-    cv.notify_all();
+    if (rocsolver_dsyevd_strided_batched(blas, rocblas_evect_original, rocblas_fill_upper, nmax, wk, nmax, nn,
+                                         static_cast<double*>(D.p), nmax, static_cast<double*>(E.p), nmax,
+                                         static_cast<rocblas_int*>(info.p), cnt) != rocblas_status_success)
+      return hipErrorUnknown;
+    for (int k = 0; k < cnt; ++k) {
+      const int64_t sq = (int64_t)items[k].ld * items[k].ld;
+      hipLaunchKernelGGL(k_lg_unpad, dim3(grid_for(sq)), dim3(256), 0, st, items[k].n, items[k].ld, nmax,
+                         wk + k * nn, static_cast<double*>(D.p) + (int64_t)k * nmax, items[k].Ct, items[k].eps);
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(done, st);
+    return e;
   }
 
   // This participant's matrices for the round (k = 0: none), then wait for the round; the
   // caller's stream is ordered after the batched solve.
   hipError_t submit(const Item* its, int k, hipStream_t s) {
-    std::unique_lock<std::mutex> lk(mu);
-    for (int q = 0; q < k; ++q) {
-      if ((int)items.size() >= cap) {
-        status = hipErrorInvalidValue;
-        break;
-      }
-      const int slot = (int)items.size();
-      items.push_back(its[q]);
-      if (hipEventRecord(ready[slot], s) != hipSuccess) status = hipErrorUnknown;
-    }
-    const uint64_t mine = round;
-    if (++arrived == active) run_round();
-    else cv.wait(lk, [&] { return round != mine; });
-    const hipError_t st_ = status;
-    lk.unlock();
+    const hipError_t st_ = sync->submit(its, k, s);
     if (k > 0 && st_ == hipSuccess) return hipStreamWaitEvent(s, done, 0);
     return st_;
   }
 
-  void leave() {
-    std::lock_guard<std::mutex> lk(mu);
-    --active;
-    if (active > 0 && arrived == active) run_round();
-  }
+  void leave() { sync->leave(); }
 };
 
 EigBatcher* eig_batcher_create(int participants, int nmax, std::string& err) {
   auto* b = new EigBatcher();
-  b->active = participants;
   b->nmax = nmax;
   b->cap = 2 * participants;  // UHF hands in alpha and beta
+  b->sync = std::make_unique<Lockstep<EigBatcher::Item, hipError_t>>(
+      participants, b->cap, hipSuccess, [b](std::vector<EigBatcher::Item>& items) { return b->run_round(items); },
+      [b](int slot, void* caller) { return hipEventRecord(b->ready[slot], static_cast<hipStream_t>(caller)); });
+  b->sync->set_overflow_status(hipErrorInvalidValue);
   bool ok = rocblas_create_handle(&b->blas) == rocblas_status_success && hipStreamCreate(&b->st) == hipSuccess &&
             rocblas_set_stream(b->blas, b->st) == rocblas_status_success && hipEventCreate(&b->done) == hipSuccess;
   b->ready.assign(b->cap, nullptr);

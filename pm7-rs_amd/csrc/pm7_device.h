@@ -14,6 +14,12 @@
 
 #include <stdint.h>
 
+// The host-only sanitizer builds compile the host sources with g++: the HIP qualifiers are empty.
+#ifndef __HIPCC__
+#define __host__
+#define __device__
+#endif
+
 namespace pm7 {
 
 constexpr double EV = 27.211386245988;        // eV per Hartree (CODATA 2018, MOPAC v23.2.5)

@@ -9,7 +9,7 @@
 //   k_dc_global_field EV sum_{b != a} q_b / r_ab for every atom (nuclear once, electronic per step)
 //   k_dc_local_field  each subsystem atom's environment: the global potential less its own cluster
 //   k_dc_gather       the global blocks of every subsystem's atom pairs into its dense local matrix
-//   k_dc_fermi        the common chemical potential by bisection over every weighted state
+//   k_dc_fermi        the common chemical potential, bracketed over every weighted state
 //   k_dc_assemble     one wave per stored block: weighted sum of the subsystem densities that hold
 //                     it, normalised, the RMS change and the linear mixing fused
 //   k_dc_finish       RMS change and electronic energy
@@ -148,46 +148,114 @@ __global__ void __launch_bounds__(256) k_dc_gather(const MolDev* __restrict__ mo
 }
 
 // Electrons at chemical potential mu over every weighted state (occupation dropped below 1e-14).
-__device__ inline double dc_count(int64_t n_states, const double* __restrict__ eps, const double* __restrict__ wn,
-                                  double mu, double width, double* red) {
-  double s = 0.0;
-  for (int64_t k = threadIdx.x; k < n_states; k += 1024) {
-    const double w = wn[k];
-    if (w == 0.0) continue;
-    const double x = (eps[k] - mu) / width;
-    const double f = x > 40.0 ? 0.0 : (x < -40.0 ? 2.0 : 2.0 / (1.0 + exp(x)));
-    if (f >= 1.0e-14) s += f * w;
-  }
-  return wg_sum<1024>(s, red);
+__device__ inline double dc_occ_weighted(double e, double w, double mu, double width) {
+  const double x = (e - mu) / width;
+  const double f = x > 40.0 ? 0.0 : (x < -40.0 ? 2.0 : 2.0 / (1.0 + exp(x)));
+  return f >= 1.0e-14 ? f * w : 0.0;
 }
 
-// The common chemical potential (find_fermi, src/dandc/mod.rs:715-770): bisection on [-500, 500]
-// eV until the bracket is below 1e-12 eV (at most 200 halvings); scal[2] = E_F. If the bracket
-// does not hold n_elec electrons, scal[3] = 1 and the host refuses the step.
-__global__ void __launch_bounds__(1024) k_dc_fermi(int64_t n_states, const double* __restrict__ eps,
-                                                   const double* __restrict__ wn, double n_elec, double width,
-                                                   double* __restrict__ scal) {
-  __shared__ double red[(Wg<1024>::NW + 1) * 1];
+constexpr int FERMI_SPLIT = 15;   // interior points per narrowing step (the bracket shrinks 16x)
+constexpr int FERMI_BLOCKS = 128;  // workgroups of one count pass (the states split between them)
+constexpr int FERMI_BT = 256;
+constexpr int FERMI_STEPS = 14;    // 16^13 > 1e15: the 1000 eV bracket is below 1e-12 eV after 13 passes
+static_assert(DC_FERMI_PART_WORDS == FERMI_BLOCKS * 16, "Fermi partial-count buffer");
+
+// The common chemical potential (find_fermi, src/dandc/mod.rs:715-770). The electron count is
+// monotone in mu, and the reference brackets its root in [-500, 500] eV until the bracket is below
+// 1e-12 eV. Here each pass evaluates the count at 15 interior points of the bracket at once and
+// the next pass keeps the sub-interval where it crosses n_elec: 13 passes instead of 50 halvings,
+// the same root to within the 1e-12 eV bracket. A pass is one launch of FERMI_BLOCKS workgroups
+// over the states (partial sums per workgroup, fixed order); the next launch's workgroups each
+// reduce those partials in workgroup order and take the same decision. Pass 0 counts at -500 and
+// 500 eV: if that bracket does not hold n_elec electrons, scal[3] = 1 and the host refuses the
+// step. The last launch writes scal[2] = E_F. fst[2][4]: bracket, failed, done (ping-pong);
+// fpart[2][FERMI_BLOCKS][16]: the partial counts (ping-pong).
+__global__ void __launch_bounds__(FERMI_BT) k_dc_fermi_step(int step, int64_t n_states, const double* __restrict__ eps,
+                                                            const double* __restrict__ wn, double n_elec,
+                                                            double width, double* __restrict__ fpart,
+                                                            double* __restrict__ fst, double* __restrict__ scal) {
+  __shared__ double red[(Wg<FERMI_BT>::NW + 1) * FERMI_SPLIT];
   double lo = -500.0, hi = 500.0;
-  const double c_lo = dc_count(n_states, eps, wn, lo, width, red);
-  const double c_hi = dc_count(n_states, eps, wn, hi, width, red);
-  if (c_lo > n_elec || c_hi < n_elec) {
-    if (threadIdx.x == 0) {
-      scal[2] = 0.0;
-      scal[3] = 1.0;
+  bool failed = false, done = false;
+  if (step > 0) {
+    const double* ps = fst + 4 * ((step - 1) & 1);
+    lo = ps[0];
+    hi = ps[1];
+    failed = ps[2] != 0.0;
+    done = ps[3] != 0.0;
+    if (!failed && !done) {
+      const double* pp = fpart + (int64_t)((step - 1) & 1) * FERMI_BLOCKS * 16;
+      double c[FERMI_SPLIT];
+#pragma unroll
+      for (int j = 0; j < FERMI_SPLIT; ++j) {
+        double t = 0.0;
+        if (step > 1 || j < 2)
+          for (int b = 0; b < FERMI_BLOCKS; ++b) t += pp[b * 16 + j];
+        c[j] = t;
+      }
+      if (step == 1) {
+        failed = c[0] > n_elec || c[1] < n_elec;
+      } else {
+        // The first interior point holding n_elec electrons closes the new bracket.
+        const double stp = (hi - lo) / (FERMI_SPLIT + 1);
+        double nlo = lo, nhi = hi;
+        bool found = false;
+#pragma unroll
+        for (int j = 0; j < FERMI_SPLIT; ++j) {
+          const double mu = lo + (j + 1) * stp;
+          if (!found) {
+            if (c[j] < n_elec) {
+              nlo = mu;
+            } else {
+              nhi = mu;
+              found = true;
+            }
+          }
+        }
+        lo = nlo;
+        hi = nhi;
+        done = hi - lo < 1.0e-12;
+      }
     }
-    return;
   }
-  for (int it = 0; it < 200; ++it) {
-    const double mid = 0.5 * (lo + hi);
-    if (dc_count(n_states, eps, wn, mid, width, red) < n_elec) lo = mid;
-    else hi = mid;
-    if (hi - lo < 1.0e-12) break;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double* ws = fst + 4 * (step & 1);
+    ws[0] = lo;
+    ws[1] = hi;
+    ws[2] = failed ? 1.0 : 0.0;
+    ws[3] = done ? 1.0 : 0.0;
+    if (step == FERMI_STEPS || failed || done) {
+      scal[2] = failed ? 0.0 : 0.5 * (lo + hi);
+      scal[3] = failed ? 1.0 : 0.0;
+    }
   }
-  if (threadIdx.x == 0) {
-    scal[2] = 0.5 * (lo + hi);
-    scal[3] = 0.0;
+  if (failed || done || step == FERMI_STEPS) return;  // the same decision in every workgroup
+  // This pass's partial counts: at lo and hi (pass 0) or at the 15 interior points.
+  const double stp = (hi - lo) / (FERMI_SPLIT + 1);
+  double v[FERMI_SPLIT];
+#pragma unroll
+  for (int j = 0; j < FERMI_SPLIT; ++j) v[j] = 0.0;
+  for (int64_t k = (int64_t)blockIdx.x * FERMI_BT + threadIdx.x; k < n_states; k += (int64_t)FERMI_BLOCKS * FERMI_BT) {
+    const double w = wn[k];
+    if (w == 0.0) continue;
+    const double e = eps[k];
+    if (step == 0) {
+      v[0] += dc_occ_weighted(e, w, lo, width);
+      v[1] += dc_occ_weighted(e, w, hi, width);
+    } else if ((e - lo) / width < -40.0) {
+      // saturated over the whole bracket (rounding is monotone, so at every interior point too)
+#pragma unroll
+      for (int j = 0; j < FERMI_SPLIT; ++j) v[j] += 2.0 * w;
+    } else if (!((e - hi) / width > 40.0)) {
+#pragma unroll
+      for (int j = 0; j < FERMI_SPLIT; ++j) v[j] += dc_occ_weighted(e, w, lo + (j + 1) * stp, width);
+    }
   }
+  wg_sum_n<FERMI_BT, FERMI_SPLIT>(v, red);
+  double* pw = fpart + (int64_t)(step & 1) * FERMI_BLOCKS * 16 + blockIdx.x * 16;
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int j = 0; j < FERMI_SPLIT; ++j) pw[j] = v[j];
 }
 
 // One wave per stored atom-pair block: the next density is the partition-weighted sum of the
@@ -279,7 +347,9 @@ hipError_t launch_dc_gather(const BatchDev& b, const DcDev& d, hipStream_t s) {
 }
 
 hipError_t launch_dc_fermi(const BatchDev& b, const DcDev& d, double n_elec, double width, hipStream_t s) {
-  hipLaunchKernelGGL(k_dc_fermi, dim3(1), dim3(1024), 0, s, d.n_states, b.vecs, d.wnorm, n_elec, width, d.scal);
+  for (int step = 0; step <= FERMI_STEPS; ++step)
+    hipLaunchKernelGGL(k_dc_fermi_step, dim3(step == FERMI_STEPS ? 1 : FERMI_BLOCKS), dim3(FERMI_BT), 0, s, step,
+                       d.n_states, b.vecs, d.wnorm, n_elec, width, d.fermi_part, d.fermi_state, d.scal);
   return hipGetLastError();
 }
 

@@ -2020,13 +2020,13 @@ bool dandc_options_ok(const pm7_dandc_options_v1& d, std::string& msg) {
 // Device memory of one run beside its subsystem batch.
 struct DcBufs {
   DevBuf diag_off, norb, xyz, core, parent, atom_mol, nc_ao, aocode, ao0, amap, amap0, slot_off, slot_dims, cstart,
-      contrib, normalizer, P, S, pop, g, v_nuc, v_el, wnorm, e_part, partial, scal;
+      contrib, normalizer, P, S, pop, g, v_nuc, v_el, wnorm, e_part, partial, scal, fermi_part, fermi_state;
   PinnedBuf h_scal;
-  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipEvent_t ev[6] = {};  // timing (0, 1), class fan-out (2) and fan-in (3 + class)
   ~DcBufs() {
     DevBuf* all[] = {&diag_off, &norb, &xyz, &core, &parent, &atom_mol, &nc_ao, &aocode, &ao0, &amap, &amap0,
                      &slot_off, &slot_dims, &cstart, &contrib, &normalizer, &P, &S, &pop, &g, &v_nuc, &v_el,
-                     &wnorm, &e_part, &partial, &scal};
+                     &wnorm, &e_part, &partial, &scal, &fermi_part, &fermi_state};
     for (DevBuf* b : all) b->release();
     h_scal.release();
     for (auto& e : ev)
@@ -2301,9 +2301,10 @@ pm7_status dandc_impl(pm7_ctx* ctx, const pm7_molecule_v1& mol, const pm7_option
   DCCHK(B.e_part.ensure((size_t)ns * 8));
   DCCHK(B.partial.ensure((size_t)std::max<int64_t>(n_slots, 1) * 8));
   DCCHK(B.scal.ensure(8 * 8));
+  DCCHK(B.fermi_part.ensure((size_t)2 * DC_FERMI_PART_WORDS * 8));
+  DCCHK(B.fermi_state.ensure(8 * 8));
   DCCHK(B.h_scal.ensure(8 * 8));
-  DCCHK(hipEventCreate(&B.ev[0]));
-  DCCHK(hipEventCreate(&B.ev[1]));
+  for (int k = 0; k < 6; ++k) DCCHK(hipEventCreateWithFlags(&B.ev[k], k < 2 ? hipEventDefault : hipEventDisableTiming));
   DcDev dd;
   dd.n_atoms = n;
   dd.n_sub = ns;
@@ -2338,6 +2339,8 @@ pm7_status dandc_impl(pm7_ctx* ctx, const pm7_molecule_v1& mol, const pm7_option
   dd.e_part = B.e_part.as<double>();
   dd.partial = B.partial.as<double>();
   dd.scal = B.scal.as<double>();
+  dd.fermi_part = B.fermi_part.as<double>();
+  dd.fermi_state = B.fermi_state.as<double>();
   const BatchDev& b = P.b;
   // The nuclear environment, once per geometry, into the subsystems' core Hamiltonians.
   DCCHK(launch_dc_field(b, dd, dd.core, dd.g, dd.v_nuc, s0));
@@ -2351,9 +2354,19 @@ pm7_status dandc_impl(pm7_ctx* ctx, const pm7_molecule_v1& mol, const pm7_option
     DCCHK(launch_dc_pop(dd, s0));
     DCCHK(launch_dc_field(b, dd, dd.pop, dd.g, dd.v_el, s0));
     DCCHK(launch_dc_gather(b, dd, s0));
-    for (int v = 0; v < LARGE_VARIANT; ++v)
-      DCCHK(launch_dc_solve(v, b, P.list_off[v], P.list_off[v + 1] - P.list_off[v], dd.nc_ao, dd.v_el, it, dd.wnorm,
-                            dd.e_part, s0));
+    // The size classes' subsystem solves side by side, each on its own stream.
+    DCCHK(hipEventRecord(B.ev[2], s0));
+    for (int v = 0; v < LARGE_VARIANT; ++v) {
+      const int cnt = P.list_off[v + 1] - P.list_off[v];
+      if (cnt == 0) continue;
+      hipStream_t sv = v == 0 ? s0 : L.st[v];
+      if (sv != s0) DCCHK(hipStreamWaitEvent(sv, B.ev[2], 0));
+      DCCHK(launch_dc_solve(v, b, P.list_off[v], cnt, dd.nc_ao, dd.v_el, it, dd.wnorm, dd.e_part, sv));
+      if (sv != s0) {
+        DCCHK(hipEventRecord(B.ev[3 + v], sv));
+        DCCHK(hipStreamWaitEvent(s0, B.ev[3 + v], 0));
+      }
+    }
     DCCHK(launch_dc_fermi(b, dd, n_elec, d.fermi_width_ev, s0));
     DCCHK(launch_dc_density(b, ns, dd.scal, d.fermi_width_ev, it, s0));
     DCCHK(launch_dc_assemble(b, dd, d.mixing, s0));

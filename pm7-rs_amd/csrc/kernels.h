@@ -115,7 +115,11 @@ struct DcDev {
   double* e_part = nullptr;            // per subsystem: weighted energy
   double* partial = nullptr;           // per stored block: squared change
   double* scal = nullptr;              // [rms, e_el, fermi, fermi failed]
+  double* fermi_part = nullptr;        // 2 x DC_FERMI_PART_WORDS: partial counts of the Fermi passes
+  double* fermi_state = nullptr;       // 2 x 4: bracket and flags of the Fermi passes
 };
+
+constexpr int DC_FERMI_PART_WORDS = 128 * 16;  // workgroups of a Fermi pass x 16 partial counts
 
 hipError_t launch_dc_solve(int variant, const BatchDev& b, int list_off, int count, const int* nc_ao,
                            const double* v_el, int step, double* wnorm, double* e_part, hipStream_t s);

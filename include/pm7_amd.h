@@ -173,6 +173,19 @@ pm7_status pm7_run_batch(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols,
 
 void pm7_result_free(pm7_result_v2* res);
 
+/* pm7_run_batch split in two, so that consecutive batches overlap on the device (no pm7-rs
+ * counterpart; it serves callers that issue batch after batch, as rayon callers of run_pm7 do).
+ * pm7_batch_submit validates, lays out, uploads and launches the batch on one of the context's
+ * asynchronous lanes (own streams and device buffers; PM7_AMD_ASYNC_LANES, default 2, batches in
+ * flight) and returns with a ticket; a batch with molecules above 196 AOs runs those before it
+ * returns. results[] must stay valid until pm7_batch_wait(ticket) has returned: the wait fills
+ * them, exactly as pm7_run_batch would, and returns what pm7_run_batch would have returned. A
+ * submit while every lane is busy first completes the oldest batch in flight (its wait then returns
+ * at once). Per-molecule outcomes are in results[i].status. */
+pm7_status pm7_batch_submit(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols, const pm7_options_v2* opts,
+                            int32_t n_opts, pm7_result_v2* results, int64_t* ticket);
+pm7_status pm7_batch_wait(pm7_ctx* ctx, int64_t ticket);
+
 /* Replaces closed_form_gradient (src/gradient.rs:500-555) for many molecules: the SCF of each
  * (results[i] exactly as pm7_run_batch fills it), then dE_total/dR at the converged density on the
  * device, in eV/Bohr, plus the gradient of the geometric corrections (host). Molecule i's atom a,
@@ -331,8 +344,12 @@ void pm7_server_stop(pm7_server* srv);
  * [8] validation, layout and upload, [9] kernel launches, host-side corrections and the copy of
  * the packed results to the host (which waits for the kernels), [10] remaining device wait,
  * [11] result assembly. After pm7_response_fock_batch: [1] its response kernel, the rest 0.
+ * After pm7_batch_wait: [0]-[2] and [4]-[7] of that batch, [3] submit to wait, and [12], [13] the
+ * device times at which its integrals kernel started and its output gather ended, in ms from the
+ * context's first asynchronous submit (so the device span of several batches in flight is
+ * max [13] - min [12]); 0 after the synchronous calls.
  * Writes min(n, PM7_TIMING_SLOTS) values. */
-#define PM7_TIMING_SLOTS 12
+#define PM7_TIMING_SLOTS 14
 void pm7_last_timings(const pm7_ctx* ctx, double* ms, int32_t n);
 
 #ifdef __cplusplus

@@ -229,6 +229,10 @@ struct Lane {
 
 }  // namespace
 
+namespace {
+struct AsyncBatch;
+}
+
 struct pm7_ctx {
   int device = 0;
   std::string data_dir;
@@ -263,6 +267,18 @@ struct pm7_ctx {
   Lane na_lane;
   bool na_ready = false;
   std::string na_err;
+  // Asynchronous batches (pm7_batch_submit / pm7_batch_wait): up to async_max batches in flight,
+  // each on its own lane; lanes are made on first use and kept.
+  std::vector<Lane*> async_lanes;
+  std::deque<AsyncBatch*> inflight;               // submit order
+  std::unordered_map<int64_t, pm7_status> async_done;  // collected early (a submit needed the lane)
+  int64_t next_ticket = 1;
+  // 2: on the 256-molecule organic batch, two batches in flight raised the device throughput
+  // 8.0k -> 10.2k solves/s (FP64) and 12.3k -> 15.6k (mixed) with step latency p99 47 / 32 ms;
+  // 4-6 in flight reached 11-14k / 17-21k at twice the latency (16 hardware queues).
+  int async_max = 2;
+  bool async_one_stream = false;
+  hipEvent_t epoch = nullptr;  // recorded at the first asynchronous submit: origin of timings [12], [13]
   std::mutex mu;
 };
 
@@ -1265,11 +1281,68 @@ pm7_status run_jobs(pm7_ctx* ctx, std::vector<Job>& jobs, pm7_result_v2* results
   const auto t_end = std::chrono::steady_clock::now();
   using msd = std::chrono::duration<double, std::milli>;
   ctx->timings[3] = msd(t_end - t0).count();
+  ctx->timings[12] = ctx->timings[13] = 0.0;
   ctx->timings[8] = msd(t_prep - t0).count();
   ctx->timings[9] = msd(t_launched - t_prep).count();
   ctx->timings[10] = msd(t_synced - t_launched).count();
   ctx->timings[11] = msd(t_end - t_synced).count();
   return PM7_OK;
+}
+
+// ---- asynchronous batches -----------------------------------------------------------------
+
+// One batch in flight: its lane (streams and device buffers), its layout and the caller's result
+// array, which pm7_batch_wait (or a submit that needs the lane) fills.
+struct AsyncBatch {
+  int64_t ticket = 0;
+  Lane* lane = nullptr;
+  Prepared P;
+  pm7_result_v2* results = nullptr;
+  int count[SCF_VARIANTS] = {};
+  std::chrono::steady_clock::time_point t0;
+};
+
+// Wait for batch b, fill its results, record its timings and free its lane. Lock held.
+pm7_status async_collect(pm7_ctx* ctx, AsyncBatch* b) {
+  Lane& L = *b->lane;
+  pm7_status st = PM7_OK;
+  if (hipEventSynchronize(L.ev[5]) != hipSuccess) st = fail(ctx, PM7_ERR_DEVICE, "batch wait failed");
+  if (st == PM7_OK) {
+    st = collect_batch(ctx, L, b->P, b->results);
+    if (st != PM7_OK) ctx->err = L.err;
+  }
+  if (st == PM7_OK) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, L.ev[0], L.ev[1]) == hipSuccess) ctx->timings[0] = ms;
+    if (hipEventElapsedTime(&ms, L.ev[1], L.ev[2]) == hipSuccess) ctx->timings[1] = ms;
+    if (hipEventElapsedTime(&ms, L.ev[2], L.ev[5]) == hipSuccess) ctx->timings[2] = ms;
+    for (int v = 0; v < SCF_VARIANTS; ++v) {
+      ctx->timings[4 + v] = 0.0;
+      if (b->count[v] > 0 && hipEventElapsedTime(&ms, L.ev_var[2 * v], L.ev_var[2 * v + 1]) == hipSuccess)
+        ctx->timings[4 + v] = ms;
+    }
+    ctx->timings[12] = ctx->timings[13] = 0.0;
+    if (ctx->epoch) {
+      if (hipEventElapsedTime(&ms, ctx->epoch, L.ev[0]) == hipSuccess) ctx->timings[12] = ms;
+      if (hipEventElapsedTime(&ms, ctx->epoch, L.ev[5]) == hipSuccess) ctx->timings[13] = ms;
+    }
+    (void)hipGetLastError();
+    ctx->timings[3] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - b->t0).count();
+  }
+  if (st != PM7_OK)
+    for (const Job& j : b->P.jobs) {
+      b->results[j.idx].status = st;
+      set_msg(&b->results[j.idx], ctx->err);
+    }
+  return st;
+}
+
+// Collect and retire the oldest batch in flight; its status is kept for its own wait.
+void async_retire_oldest(pm7_ctx* ctx) {
+  AsyncBatch* b = ctx->inflight.front();
+  ctx->inflight.pop_front();
+  ctx->async_done[b->ticket] = async_collect(ctx, b);
+  delete b;
 }
 
 // ---- request server (continuous batching) ------------------------------------------------
@@ -1617,6 +1690,11 @@ pm7_status pm7_ctx_create(int32_t device, const char* data_dir, pm7_ctx** out) {
     const int v = std::atoi(gm);
     if (v >= 32 && v <= VARIANT_MAX_NAO[LARGE_VARIANT - 1]) ctx->grid_min_nao = v;
   }
+  if (const char* al = std::getenv("PM7_AMD_ASYNC_LANES")) {
+    const int v = std::atoi(al);
+    if (v >= 1 && v <= 8) ctx->async_max = v;
+  }
+  if (const char* ao = std::getenv("PM7_AMD_ASYNC_ONE_STREAM")) ctx->async_one_stream = std::atoi(ao) != 0;
   if (const char* te = std::getenv("PM7_AMD_EIG_TOL_EXP")) {
     const int v = std::atoi(te);
     if (v >= 16 && v <= 30) ctx->eig_tol_exp = v;
@@ -1654,6 +1732,17 @@ pm7_status pm7_ctx_create(int32_t device, const char* data_dir, pm7_ctx** out) {
 void pm7_ctx_destroy(pm7_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
+  for (AsyncBatch* b : ctx->inflight) {  // batches never waited for: finish, drop their results
+    (void)hipEventSynchronize(b->lane->ev[5]);
+    delete b;
+  }
+  ctx->inflight.clear();
+  for (Lane* l : ctx->async_lanes) {
+    l->destroy();
+    delete l;
+  }
+  ctx->async_lanes.clear();
+  if (ctx->epoch) (void)hipEventDestroy(ctx->epoch);
   ctx->lane.destroy();
   if (ctx->na_ready) ctx->na_lane.destroy();
   ctx->d_elem.release();
@@ -1743,6 +1832,112 @@ pm7_status run_batch_impl(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols, 
   return st == PM7_ERR_INSUFFICIENT_MEMORY ? PM7_OK : st;
 }
 }  // namespace
+
+pm7_status pm7_batch_submit(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols, const pm7_options_v2* opts,
+                            int32_t n_opts, pm7_result_v2* results, int64_t* ticket) {
+  if (!ctx || n < 0 || !ticket || (n > 0 && (!mols || !results))) return PM7_ERR_INVALID_INPUT;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  const auto t0 = std::chrono::steady_clock::now();
+  *ticket = ctx->next_ticket++;
+  pm7_options_v2 def;
+  pm7_default_options(&def);
+  std::vector<Job> jobs;
+  int64_t budget = 0;
+  for (int i = 0; i < n; ++i) {
+    std::memset(&results[i], 0, sizeof results[i]);
+    const pm7_options_v2& o = (opts && n_opts > 0) ? opts[n_opts == 1 ? 0 : i] : def;
+    Job j;
+    j.idx = i;
+    std::string msg;
+    const pm7_status st = validate(ctx, mols[i], o, j, msg);
+    if (st != PM7_OK) {
+      results[i].status = st;
+      results[i].n_atoms = mols[i].n_atoms;
+      results[i].missing_element = j.missing_z;
+      set_msg(&results[i], msg);
+      continue;
+    }
+    if (o.max_memory_mb > 0) budget = budget > 0 ? std::min(budget, o.max_memory_mb) : o.max_memory_mb;
+    jobs.push_back(std::move(j));
+  }
+  if (jobs.empty()) {
+    ctx->async_done[*ticket] = PM7_OK;
+    return PM7_OK;
+  }
+  // A lane: a free one, a new one while fewer than async_max exist, else the oldest batch's.
+  if ((int)ctx->inflight.size() >= ctx->async_max) async_retire_oldest(ctx);
+  Lane* lane = nullptr;
+  for (Lane* l : ctx->async_lanes) {
+    bool busy = false;
+    for (const AsyncBatch* b : ctx->inflight) busy = busy || b->lane == l;
+    if (!busy) {
+      lane = l;
+      break;
+    }
+  }
+  if (!lane) {
+    lane = new Lane();
+    if (!lane->create(ctx->async_one_stream)) {
+      lane->destroy();
+      delete lane;
+      return fail(ctx, PM7_ERR_DEVICE, "stream creation for an asynchronous batch failed");
+    }
+    ctx->async_lanes.push_back(lane);
+  }
+  if (!ctx->epoch) {
+    if (hipEventCreate(&ctx->epoch) != hipSuccess || hipEventRecord(ctx->epoch, lane->st[0]) != hipSuccess)
+      return fail(ctx, PM7_ERR_DEVICE, "timing event of the asynchronous batches failed");
+  }
+  auto* b = new AsyncBatch();
+  b->ticket = *ticket;
+  b->lane = lane;
+  b->results = results;
+  b->t0 = t0;
+  b->P.jobs = std::move(jobs);
+  Lane& L = *lane;
+  pm7_status st = prepare(ctx, L, b->P, budget);
+  if (st == PM7_OK) st = launch_batch(L, b->P, b->count);
+  if (st == PM7_OK) st = run_large(L, b->P);  // molecules above 196 AOs: blocks until they finish
+  if (st == PM7_OK) st = finish_batch(L, b->P);
+  if (st != PM7_OK) {
+    ctx->err = L.err;
+    for (const Job& j : b->P.jobs) {
+      results[j.idx].status = st;
+      results[j.idx].needed_mb = b->P.need_mb;
+      results[j.idx].budget_mb = b->P.budget_mb;
+      set_msg(&results[j.idx], L.err);
+    }
+    if (st != PM7_ERR_INSUFFICIENT_MEMORY) (void)hipStreamSynchronize(L.st[0]);
+    delete b;
+    ctx->async_done[*ticket] = st == PM7_ERR_INSUFFICIENT_MEMORY ? PM7_OK : st;
+    return st == PM7_ERR_INSUFFICIENT_MEMORY ? PM7_OK : st;
+  }
+  host_corrections(b->P.jobs);  // host work of this batch while the device runs it
+  ctx->inflight.push_back(b);
+  return PM7_OK;
+}
+
+pm7_status pm7_batch_wait(pm7_ctx* ctx, int64_t ticket) {
+  if (!ctx) return PM7_ERR_INVALID_INPUT;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  auto it = ctx->async_done.find(ticket);
+  if (it != ctx->async_done.end()) {
+    const pm7_status st = it->second;
+    ctx->async_done.erase(it);
+    return st;
+  }
+  for (auto q = ctx->inflight.begin(); q != ctx->inflight.end(); ++q) {
+    if ((*q)->ticket != ticket) continue;
+    AsyncBatch* b = *q;
+    ctx->inflight.erase(q);
+    const pm7_status st = async_collect(ctx, b);
+    delete b;
+    return st;
+  }
+  return fail(ctx, PM7_ERR_INVALID_INPUT, "unknown batch ticket " + std::to_string(ticket));
+}
 
 pm7_status pm7_run(pm7_ctx* ctx, const pm7_molecule_v1* mol, const pm7_options_v2* opts, pm7_result_v2* out) {
   if (!ctx || !mol || !out) return PM7_ERR_INVALID_INPUT;

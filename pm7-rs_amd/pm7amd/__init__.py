@@ -98,7 +98,7 @@ class _Result(C.Structure):
                 ("charges", C.POINTER(C.c_double)), ("message", C.c_char * 256)]
 
 
-TIMING_SLOTS = 12
+TIMING_SLOTS = 14
 
 ABI_FUNCTIONS = [
     "pm7_ctx_create", "pm7_ctx_destroy", "pm7_last_error", "pm7_abi_version", "pm7_params_load",
@@ -107,7 +107,7 @@ ABI_FUNCTIONS = [
     "pm7_server_start", "pm7_server_submit", "pm7_server_poll", "pm7_server_stop", "pm7_gradient_batch",
     "pm7_gradient", "pm7_response_fock_batch", "pm7_response_fock_spin_batch", "pm7_fock_spin_noalloc",
     "pm7_dandc_default_options", "pm7_run_dandc", "pm7_dandc_result_free", "pm7_dandc_partition",
-    "pm7_partition_free",
+    "pm7_partition_free", "pm7_batch_submit", "pm7_batch_wait",
 ]
 
 _lib = None
@@ -153,6 +153,9 @@ def library() -> C.CDLL:
     lib.pm7_server_submit.argtypes = [vp, C.POINTER(_Molecule), C.POINTER(_Options), C.POINTER(C.c_int64)]
     lib.pm7_server_poll.argtypes = [vp, C.c_int64, C.c_int32, C.POINTER(C.c_int64), C.POINTER(_Result),
                                     C.POINTER(C.c_double)]
+    lib.pm7_batch_submit.argtypes = [vp, C.c_int32, C.POINTER(_Molecule), C.POINTER(_Options), C.c_int32,
+                                     C.POINTER(_Result), C.POINTER(C.c_int64)]
+    lib.pm7_batch_wait.argtypes = [vp, C.c_int64]
     lib.pm7_server_stop.argtypes = [vp]
     lib.pm7_server_stop.restype = None
     _lib = lib
@@ -545,6 +548,47 @@ class Engine:
         else:
             st = self._lib.pm7_gradient_batch(self._h, n, c_mols, c_opts, len(opts_list), c_res,
                                               grad_flat.ctypes.data_as(C.POINTER(C.c_double)))
+        if st != 0:
+            raise Pm7Error(st, self.last_error())
+        out = _convert_all(c_res, n, self._lib.pm7_result_free)
+        if raise_on_error:
+            for r in out:
+                if r.status != 0:
+                    raise Pm7Error(r.status, r.message, r)
+        return out
+
+    def submit_batch(self, molecules: Sequence[Molecule], options=None) -> int:
+        """pm7_batch_submit: launch a batch and return its ticket at once; wait_batch(ticket) returns
+        what run_batch would. Consecutive batches overlap on the device."""
+        n = len(molecules)
+        if options is None:
+            options = Pm7Options()
+        opts_list = options if isinstance(options, (list, tuple)) else [options]
+        methods = {METHODS[o.method.lower()] for o in opts_list}
+        if len(methods) == 1 and next(iter(methods)) != self.method and next(iter(methods)) in (0, 1, 3, 4):
+            self.load(opts_list[0].method)
+        c_mols = (_Molecule * max(n, 1))()
+        keep = []
+        for i, m in enumerate(molecules):
+            cm, k = self._mol(m)
+            c_mols[i] = cm
+            keep.append(k)
+        c_list = [o.to_c() for o in opts_list]
+        c_opts = (_Options * len(opts_list))(*c_list)
+        c_res = (_Result * max(n, 1))()
+        ticket = C.c_int64(0)
+        st = self._lib.pm7_batch_submit(self._h, n, c_mols, c_opts, len(opts_list), c_res, C.byref(ticket))
+        if st != 0:
+            raise Pm7Error(st, self.last_error())
+        # The result array is written at the wait; the molecule and option arrays only at submit.
+        if not hasattr(self, "_pending"):
+            self._pending = {}
+        self._pending[ticket.value] = (c_res, n)
+        return ticket.value
+
+    def wait_batch(self, ticket: int, raise_on_error: bool = False):
+        c_res, n = self._pending.pop(ticket)
+        st = self._lib.pm7_batch_wait(self._h, int(ticket))
         if st != 0:
             raise Pm7Error(st, self.last_error())
         out = _convert_all(c_res, n, self._lib.pm7_result_free)

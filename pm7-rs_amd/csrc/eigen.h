@@ -17,6 +17,13 @@ __device__ inline int pidx(int i, int j) {
   return b * (b + 1) / 2 + a;
 }
 
+// Scale of the Jacobi stopping test off^2 <= tol2 * scale: the matrix's total square norm, capped
+// at n^2 (100 eV)^2. The cap leaves ordinary Fock matrices (elements of tens of eV) untouched; a
+// matrix with a few huge diagonal entries (pseudo-atoms carry 1e4-1e7 eV) would otherwise let the
+// off-diagonal residual grow with those entries and leave the ordinary orbitals mixed far beyond
+// the density's tolerance.
+__device__ inline double jacobi_scale(double total, int n) { return fmin(total, (double)n * (double)n * 1.0e4); }
+
 // Parallel cyclic Jacobi on the packed symmetric n2 x n2 matrix A, accumulating the rotations
 // into V (row-major, stride n2). Round-robin pairing gives m = n2/2 disjoint rotations per round;
 // the 2x2 block (pair k rows, pair l columns) takes both its row and column rotation in one
@@ -34,7 +41,7 @@ __device__ inline int pidx(int i, int j) {
 // TA is the storage type of A: double, or float for the reduced-precision phase of a mixed run.
 // The angles are always formed in FP64 (from the stored values), and V always accumulates in
 // FP64, so the rotations stay orthogonal to FP64 rounding whatever TA is. The sweeps stop when
-// off-diagonal^2 <= tol2 * total^2, or after max_sweeps.
+// off-diagonal^2 <= tol2 * jacobi_scale(total^2), or after max_sweeps.
 // V is split by rows: rows [0, vrows) at Vl (LDS), the rest at Vg (L2/HBM), stride n2 in both.
 template <int NT, int MAXN, bool V_LDS, typename TA>
 __device__ int jacobi_core(int n2, TA* A, double* Vl, double* Vg, int vrows, double2* s_cs, double* s_t, int* s_pq,
@@ -60,7 +67,7 @@ __device__ int jacobi_core(int n2, TA* A, double* Vl, double* Vg, int vrows, dou
       }
     }
     wg_sum_n<NT, 2>(sums, red);
-    if (sums[0] <= tol2 * (sums[0] + sums[1])) break;
+    if (sums[0] <= tol2 * jacobi_scale(sums[0] + sums[1], n2)) break;
     for (int r = 0; r < n2 - 1; ++r) {
       if (threadIdx.x < m) {
         // Round-robin pairing: position 0 stays, positions 1..n2-1 turn by one each round.
@@ -243,6 +250,347 @@ __device__ int jacobi_core(int n2, TA* A, double* Vl, double* Vg, int vrows, dou
   return sweep;
 }
 
+// LDS words of a size class's s_a array: the packed matrix when it lives in LDS, plus, for the
+// block solver (size classes whose limit is a multiple of 16 and whose rotation accumulator is in
+// LDS), one 16 x 16 subproblem and its rotation (512 doubles) per block pair.
+template <int MAXN, bool A_LDS, bool V_LDS>
+struct EigLds {
+  static constexpr bool kBlock = V_LDS && MAXN % 16 == 0;
+  static constexpr int kA = A_LDS ? MAXN * (MAXN + 1) / 2 : 0;
+  static constexpr int kBQ = kBlock ? (MAXN / 16) * 512 : 0;
+  static constexpr int kWords = kA + kBQ > 0 ? kA + kBQ : 1;
+};
+
+// LDS stores of this wave are visible to its own later LDS loads (a wave's LDS operations complete
+// in order; the clobber keeps the compiler from moving memory accesses across).
+__device__ inline void wave_lds_sync() {
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// A pointer into LDS typed as such, so accesses compile to LDS instructions with 32-bit addresses.
+typedef __attribute__((address_space(3))) double lds_f64;
+__device__ inline lds_f64* as_lds(double* p) { return (lds_f64*)p; }
+
+// Blocks (I, J) of block pair k in round r of a round-robin over nb blocks (the pairing of
+// jacobi_core, one level up).
+__device__ inline void rr_pair(int nb, int r, int k, int& I, int& J) {
+  int a = k - 1 + r;
+  a -= (a >= nb - 1) ? nb - 1 : 0;
+  I = k == 0 ? 0 : 1 + a;
+  int b = nb - 2 - k + r;
+  b -= (b >= nb - 1) ? nb - 1 : 0;
+  J = 1 + b;
+}
+
+// The 2 x 2 rotation of jacobi_core that annihilates a_pq (c = 1, s = t = 0 when a_pq is 0).
+__device__ inline void jacobi_angle(double app, double aqq, double apq, double& c, double& s, double& t) {
+  c = 1.0;
+  s = 0.0;
+  t = 0.0;
+  if (apq != 0.0) {
+    const double d = aqq - app;
+    const double x = 2.0 * apq;
+    const double r = sqrt(d * d + x * x);
+    const double den = fabs(d) + r;
+    t = (d >= 0.0 ? x : -x) / den;
+    c = sqrt(den / (2.0 * r));
+    s = t * c;
+  }
+}
+
+// The same rotation from the hardware reciprocal square root and reciprocal, each refined by two
+// Newton steps (about 27 FP64 operations instead of the ~60 of the correctly rounded square roots
+// and divisions): t is within a few ulps of jacobi_angle's, and c = (1 + t^2)^-1/2, s = t c keep
+// c^2 + s^2 = 1 to rounding whatever t is, so the rotation stays orthogonal. Arguments outside the
+// range where the squares neither overflow nor lose precision take the exact formula.
+__device__ inline void jacobi_angle_fast(double app, double aqq, double apq, double& c, double& s, double& t) {
+  c = 1.0;
+  s = 0.0;
+  t = 0.0;
+  if (apq == 0.0) return;
+  const double d = aqq - app;
+  const double x = 2.0 * apq;
+  const double h = fma(d, d, x * x);
+  if (!(h > 1.0e-280 && h < 1.0e280)) {
+    jacobi_angle(app, aqq, apq, c, s, t);
+    return;
+  }
+  // sqrt(h): g ~ h / sqrt(h), hh ~ 1 / (2 sqrt(h)), two coupled Newton steps.
+  const double y = __builtin_amdgcn_rsq(h);
+  double g = h * y, hh = 0.5 * y;
+  const double e0 = fma(-g, hh, 0.5);
+  g = fma(g, e0, g);
+  hh = fma(hh, e0, hh);
+  const double d0 = fma(-g, g, h);
+  g = fma(d0, hh, g);
+  const double den = fabs(d) + g;
+  double z = __builtin_amdgcn_rcp(den);
+  z = fma(z, fma(-den, z, 1.0), z);
+  z = fma(z, fma(-den, z, 1.0), z);
+  t = (d >= 0.0 ? x : -x) * z;
+  const double u = fma(t, t, 1.0);  // in [1, 2]
+  double w = __builtin_amdgcn_rsq(u);
+  const double hu = 0.5 * u;
+  w = w * fma(-hu, w * w, 1.5);
+  w = w * fma(-hu, w * w, 1.5);
+  c = w;
+  s = t * c;
+}
+
+// Block Jacobi on the packed symmetric N x N matrix A (N = 16 P; indices past the matrix are
+// decoupled zero rows and stay exactly zero), accumulating into V (N x N, row-major, in LDS).
+// The indices form 2P blocks of 8; a round pairs the blocks round-robin, and block pair k owns
+// the 16 indices S_k (its two blocks). Per round:
+//  1. wave k < P rotates its 16 x 16 diagonal block in LDS (the 2 x 2 update of jacobi_core),
+//     accumulating the rotations into Q_k, and writes the rotated block back to A: the first round
+//     of a sweep pairs blocks 2k, 2k+1 and turns only the pairs inside each 8-block (7 inner rounds
+//     of a round-robin over 8), every later round turns only the 64 pairs across its two blocks
+//     (8 inner rounds, rotation j = (j, 8 + (j + rr) mod 8)), so a sweep turns every index pair
+//     once, in 7 + 8 (2P - 1) inner rounds of 8 disjoint rotations. Lane (jr, jc) = (lane / 8, lane % 8) owns
+//     the 2 x 2 block of rotation pairs jr, jc (jr <= jc) and rows jr, jr + 8 of Q in pair jc's
+//     columns, and forms the angles of both of its pairs itself (every lane gets the same bits),
+//     so an inner round is one LDS read pass, the angles, one write pass and one wave-level wait;
+//  2. every off-diagonal block becomes A[S_k, S_l] <- Q_k^T A[S_k, S_l] Q_l and every 16-row tile
+//     of the accumulator V[R, S_l] <- V[R, S_l] Q_l, FP64 matrix-core products (8 and 4 per item)
+//     dealt to the waves; a pair whose sweep rotated nothing has Q = I and its items are skipped.
+//     The off-diagonal blocks are not touched by step 1, so each wave loads its blocks' operands
+//     before step 1 and the loads complete while the inner sweeps run.
+// A round is two workgroup barriers instead of one per index pair, and the bulk of its work is
+// dense products on the matrix cores. As in jacobi_core, only exact zeros skip their rotation: a
+// rounding-level element inside an exactly degenerate set still turns the set (by 45 degrees), and
+// that decides which members an occupied count that splits the set fills.
+// bq: P x 512 doubles of LDS (subproblem, then Q); s_flag: P ints. Returns the number of sweeps.
+template <int NT, int MAXN, bool A_LDS>
+__device__ int bjacobi_core(int N, double* A, double* V, double* bq, int* s_flag, double* red, double tol2,
+                            int max_sweeps) {
+  constexpr int NW = NT / 64;
+  constexpr int NX = 32, NY = NT / NX;
+  constexpr int PMAX = MAXN / 16;
+  constexpr int MAXIA = (PMAX * (PMAX - 1) / 2 + NW - 1) / NW;  // off-diagonal blocks per wave
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r16 = lane & 15, q4 = lane >> 4;
+  const int P = N / 16, NB = N / 8;
+  const int nA = P * (P - 1) / 2, nItems = nA + (N / 16) * P;
+  lds_f64* Vs = as_lds(V);
+  lds_f64* bqs = as_lds(bq);
+  auto aload = [&](int idx) -> double {
+    if constexpr (A_LDS) return as_lds(A)[idx];
+    else return A[idx];
+  };
+  auto astore = [&](int idx, double v) {
+    if constexpr (A_LDS) as_lds(A)[idx] = v;
+    else A[idx] = v;
+  };
+  // Off-diagonal block item -> (k, l), k < l.
+  auto item_kl = [&](int it, int& k, int& l) {
+    k = 0;
+    int t = it;
+    while (t >= P - 1 - k) {
+      t -= P - 1 - k;
+      ++k;
+    }
+    l = k + 1 + t;
+  };
+  const int jr = lane >> 3, jc = lane & 7;
+  int sweep = 0;
+  for (; sweep < max_sweeps; ++sweep) {
+    double sums[2] = {0.0, 0.0};
+    for (int j = threadIdx.x / NX; j < N; j += NY) {
+      const int rowj = j * (j + 1) / 2;
+      for (int i = threadIdx.x % NX; i <= j; i += NX) {
+        const double v = aload(rowj + i);
+        if (i == j) sums[1] += v * v;
+        else sums[0] += 2.0 * v * v;
+      }
+    }
+    wg_sum_n<NT, 2>(sums, red);
+    const double total = sums[0] + sums[1];
+    if (sums[0] <= tol2 * jacobi_scale(total, N)) break;
+    for (int r = 0; r < NB; ++r) {
+      // Round 0 pairs blocks 2k, 2k+1 (pairs inside each block); rounds 1..NB-1 the round-robin.
+      auto blocks = [&](int k, int& I, int& J) {
+        if (r == 0) {
+          I = 2 * k;
+          J = 2 * k + 1;
+        } else {
+          rr_pair(NB, r - 1, k, I, J);
+        }
+      };
+      // Operands of this wave's off-diagonal blocks (untouched by the inner sweeps).
+      double pav[MAXIA][4];
+#pragma unroll
+      for (int m = 0; m < MAXIA; ++m) {
+        const int it = wave + m * NW;
+        if (it < nA) {
+          int k, l, Ik, Jk, Il, Jl;
+          item_kl(it, k, l);
+          blocks(k, Ik, Jk);
+          blocks(l, Il, Jl);
+          const int gi = r16 < 8 ? 8 * Ik + r16 : 8 * Jk + r16 - 8;
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            const int b = 4 * s + q4;
+            pav[m][s] = aload(pidx(gi, b < 8 ? 8 * Il + b : 8 * Jl + b - 8));
+          }
+        }
+      }
+      if (wave < P) {
+        const int k = wave;
+        int I, J;
+        blocks(k, I, J);
+        auto g = [&](int a) { return a < 8 ? 8 * I + a : 8 * J + a - 8; };
+        lds_f64* B = bqs + k * 512;
+        lds_f64* Q = B + 256;
+        double ld4[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int e = lane + 64 * u;
+          ld4[u] = aload(pidx(g(e >> 4), g(e & 15)));
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int e = lane + 64 * u;
+          B[e] = ld4[u];
+          Q[e] = (e >> 4) == (e & 15) ? 1.0 : 0.0;
+        }
+        wave_lds_sync();
+        int any = 0;
+        const int n_inner = r == 0 ? 7 : 8;
+        for (int rr = 0; rr < n_inner; ++rr) {
+          // Rotation pair j = (p_j, q_j) of inner round rr: round 0 runs a round-robin over the 8
+          // indices of each block (j < 4 in the first, j >= 4 in the second), later rounds pair
+          // index j of the first block with index (j + rr) % 8 of the second.
+          auto pq16 = [&](int j, int& p, int& q) {
+            if (r == 0) {
+              const int h = j & 3, o = j & 4 ? 8 : 0;
+              int a = h - 1 + rr;
+              a -= (a >= 7) ? 7 : 0;
+              p = o + (h == 0 ? 0 : 1 + a);
+              int b = 6 - h + rr;
+              b -= (b >= 7) ? 7 : 0;
+              q = o + 1 + b;
+            } else {
+              p = j;
+              q = 8 + ((j + rr) & 7);
+            }
+          };
+          int pr, qr, pc, qc;
+          pq16(jr, pr, qr);
+          pq16(jc, pc, qc);
+          const double rpp = B[pr * 17], rqq = B[qr * 17], rpq = B[pr * 16 + qr];
+          const double cpp = B[pc * 17], cqq = B[qc * 17], cpq = B[pc * 16 + qc];
+          double x0 = 0.0, x1 = 0.0, x2 = 0.0, x3 = 0.0;
+          if (jr < jc) {
+            x0 = B[pr * 16 + pc];
+            x1 = B[pr * 16 + qc];
+            x2 = B[qr * 16 + pc];
+            x3 = B[qr * 16 + qc];
+          }
+          const double vp0 = Q[jr * 16 + pc], vq0 = Q[jr * 16 + qc];
+          const double vp1 = Q[(jr + 8) * 16 + pc], vq1 = Q[(jr + 8) * 16 + qc];
+          asm volatile("" ::: "memory");  // every read of the round is issued before any write
+          double cr, sr, tr, cc, sc, tc;
+          jacobi_angle_fast(rpp, rqq, rpq, cr, sr, tr);
+          jacobi_angle_fast(cpp, cqq, cpq, cc, sc, tc);
+          any |= __any(cpq != 0.0);
+          if (jr == jc) {
+            if (tr != 0.0) {
+              B[pr * 17] = rpp - tr * rpq;
+              B[qr * 17] = rqq + tr * rpq;
+              B[pr * 16 + qr] = 0.0;
+              B[qr * 16 + pr] = 0.0;
+            }
+          } else if (jr < jc) {
+            if (sr != 0.0 || sc != 0.0) {
+              const double y00 = cr * x0 - sr * x2, y01 = cr * x1 - sr * x3;
+              const double y10 = sr * x0 + cr * x2, y11 = sr * x1 + cr * x3;
+              const double z0 = cc * y00 - sc * y01, z1 = sc * y00 + cc * y01;
+              const double z2 = cc * y10 - sc * y11, z3 = sc * y10 + cc * y11;
+              B[pr * 16 + pc] = z0;
+              B[pc * 16 + pr] = z0;
+              B[pr * 16 + qc] = z1;
+              B[qc * 16 + pr] = z1;
+              B[qr * 16 + pc] = z2;
+              B[pc * 16 + qr] = z2;
+              B[qr * 16 + qc] = z3;
+              B[qc * 16 + qr] = z3;
+            }
+          }
+          if (sc != 0.0) {
+            Q[jr * 16 + pc] = cc * vp0 - sc * vq0;
+            Q[jr * 16 + qc] = sc * vp0 + cc * vq0;
+            Q[(jr + 8) * 16 + pc] = cc * vp1 - sc * vq1;
+            Q[(jr + 8) * 16 + qc] = sc * vp1 + cc * vq1;
+          }
+          wave_lds_sync();
+        }
+        if (any) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int e = lane + 64 * u, a = e >> 4, b = e & 15;
+            if (a <= b) astore(pidx(g(a), g(b)), B[e]);
+          }
+        }
+        if (lane == 0) s_flag[k] = any;
+      }
+      __syncthreads();
+      for (int it = wave, m = 0; it < nItems; it += NW, ++m) {
+        if (it < nA) {
+          int k, l;
+          item_kl(it, k, l);
+          if (!s_flag[k] && !s_flag[l]) continue;
+          int Ik, Jk, Il, Jl;
+          blocks(k, Ik, Jk);
+          blocks(l, Il, Jl);
+          auto gk = [&](int a) { return a < 8 ? 8 * Ik + a : 8 * Jk + a - 8; };
+          auto gl = [&](int a) { return a < 8 ? 8 * Il + a : 8 * Jl + a - 8; };
+          const lds_f64* Qk = bqs + k * 512 + 256;
+          const lds_f64* Ql = bqs + l * 512 + 256;
+          double av[4], qlv[4], qkv[4];
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            // m < MAXIA here: item it = wave + m NW < nA <= MAXIA NW.
+#pragma unroll
+            for (int mm = 0; mm < MAXIA; ++mm)
+              if (mm == m) av[s] = pav[mm][s];
+            qlv[s] = Ql[(4 * s + q4) * 16 + r16];
+            qkv[s] = Qk[(4 * s + q4) * 16 + r16];
+          }
+          // X = A[S_k, S_l] Q_l, then Y = Q_k^T X (X's accumulator is the second product's B operand).
+          dbl4 x = {0.0, 0.0, 0.0, 0.0}, y = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int s = 0; s < 4; ++s) x = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s], qlv[s], x, 0, 0, 0);
+#pragma unroll
+          for (int s = 0; s < 4; ++s) y = __builtin_amdgcn_mfma_f64_16x16x4f64(qkv[s], x[s], y, 0, 0, 0);
+#pragma unroll
+          for (int t4 = 0; t4 < 4; ++t4) astore(pidx(gk(tile_row_f64(q4, t4)), gl(r16)), y[t4]);
+        } else {
+          const int v = it - nA, rt = v / P, l = v - rt * P;
+          if (!s_flag[l]) continue;
+          int Il, Jl;
+          blocks(l, Il, Jl);
+          auto gl = [&](int a) { return a < 8 ? 8 * Il + a : 8 * Jl + a - 8; };
+          const lds_f64* Ql = bqs + l * 512 + 256;
+          const int i0 = 16 * rt;
+          double av[4], qlv[4];
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            av[s] = Vs[(i0 + r16) * N + gl(4 * s + q4)];
+            qlv[s] = Ql[(4 * s + q4) * 16 + r16];
+          }
+          dbl4 x = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int s = 0; s < 4; ++s) x = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s], qlv[s], x, 0, 0, 0);
+#pragma unroll
+          for (int t4 = 0; t4 < 4; ++t4) Vs[(i0 + tile_row_f64(q4, t4)) * N + gl(r16)] = x[t4];
+        }
+      }
+      __syncthreads();
+    }
+  }
+  return sweep;
+}
+
 // Relative width of an eigenvalue cluster: eigenvalues whose neighbouring gaps are all below
 // TIE_REL * max(1, |eps|) eV count as one degenerate set.
 constexpr double TIE_REL = 1.0e-10;
@@ -270,10 +618,17 @@ __device__ __attribute__((noinline)) int eigensolve(int n, int ld, const double*
                           double* __restrict__ Cout, double* __restrict__ eps, double* Ag, double* Vg,
                           double* __restrict__ T, double* s_a, double* s_v, double2* s_cs, double* s_t,
                           int* s_perm, int* s_pq, double* red, unsigned long long* tsub, double tol2,
-                          int max_sweeps, int tie_mode) {
+                          int max_sweeps, int tie_mode, bool block) {
   const int n2 = ld;
   unsigned long long t0 = wall_clock64();
   TA* A = reinterpret_cast<TA*>(A_LDS ? s_a : Ag);
+  // Block Jacobi where the size class offers it (FP64 matrix; an L2-resident packed matrix of the
+  // padded size must fit the molecule's ld x ld slot): the Jacobi dimension is ld rounded up to 16.
+  using Lds = EigLds<MAXN, A_LDS, V_LDS>;
+  const int n16 = (n2 + 15) & ~15;
+  const bool blk = Lds::kBlock && sizeof(TA) == 8 && block && n16 <= MAXN &&
+                   (A_LDS || (int64_t)n16 * (n16 + 1) / 2 <= (int64_t)ld * ld);
+  const int nj = blk ? n16 : n2;  // Jacobi dimension (pad rows and columns are zero)
   // Rotation accumulator: all in LDS (V_LDS), or as many leading rows as fit in the LDS the packed
   // matrix leaves free (A_LDS: the buffer holds MAXN(MAXN+1)/2 doubles), the rest in Vg.
   double* Vl = V_LDS ? s_v : (A_LDS ? s_a + (n2 * (n2 + 1) / 2 + 1) / 2 * 2 : nullptr);
@@ -282,7 +637,8 @@ __device__ __attribute__((noinline)) int eigensolve(int n, int ld, const double*
     const int free_words = A_LDS ? MAXN * (MAXN + 1) / 2 - (n2 * (n2 + 1) / 2 + 1) / 2 * 2 : 0;
     vrows = free_words > 0 ? min(n2, free_words / n2) : 0;
   }
-  auto vat = [&](int i, int j) -> double& { return i < vrows ? Vl[i * n2 + j] : Vg[i * n2 + j]; };
+  if (blk) vrows = nj;  // V_LDS: the whole nj x nj accumulator is in LDS
+  auto vat = [&](int i, int j) -> double& { return i < vrows ? Vl[i * nj + j] : Vg[i * nj + j]; };
   constexpr int NW = NT / 64;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, q = lane >> 4;
   const int nb = (n + 31) >> 5;  // 32 x 32 MFMA blocks (their 32 nb rows cover n2)
@@ -317,22 +673,31 @@ __device__ __attribute__((noinline)) int eigensolve(int n, int ld, const double*
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const int i = i0 + 16 * (s >> 1) + tile_row_f64(q, u), j = j0 + 16 * (s & 1) + r;
-          if (i <= j && j < n2) A[pidx(i, j)] = (i < n && j < n) ? (TA)x[s][u] : (TA)0;
+          if (i <= j && j < nj) A[pidx(i, j)] = (i < n && j < n) ? (TA)x[s][u] : (TA)0;
         }
     }
   } else {
     constexpr int NX = 32, NY = NT / NX;
-    for (int j = threadIdx.x / NX; j < n2; j += NY)
-      for (int i = threadIdx.x % NX; i <= j; i += NX) A[j * (j + 1) / 2 + i] = (TA)W[(int64_t)i * ld + j];
+    for (int j = threadIdx.x / NX; j < nj; j += NY)
+      for (int i = threadIdx.x % NX; i <= j; i += NX)
+        A[j * (j + 1) / 2 + i] = j < n2 ? (TA)W[(int64_t)i * ld + j] : (TA)0;
   }
-  for (int idx = threadIdx.x; idx < n2 * n2; idx += NT) {
-    const int i = idx / n2, j = idx - i * n2;
+  for (int idx = threadIdx.x; idx < nj * nj; idx += NT) {
+    const int i = idx / nj, j = idx - i * nj;
     vat(i, j) = (i == j) ? 1.0 : 0.0;
   }
   __syncthreads();
   unsigned long long t1 = wall_clock64();
-  const int sweeps =
-      jacobi_core<NT, MAXN, V_LDS, TA>(n2, A, Vl, Vg, vrows, s_cs, s_t, s_pq, red, tol2, max_sweeps);
+  int sweeps;
+  if constexpr (Lds::kBlock && sizeof(TA) == 8) {
+    if (blk)
+      sweeps = bjacobi_core<NT, MAXN, A_LDS>(nj, reinterpret_cast<double*>(A), Vl, s_a + Lds::kA, s_perm, red, tol2,
+                                             max_sweeps);
+    else
+      sweeps = jacobi_core<NT, MAXN, V_LDS, TA>(n2, A, Vl, Vg, vrows, s_cs, s_t, s_pq, red, tol2, max_sweeps);
+  } else {
+    sweeps = jacobi_core<NT, MAXN, V_LDS, TA>(n2, A, Vl, Vg, vrows, s_cs, s_t, s_pq, red, tol2, max_sweeps);
+  }
   unsigned long long t2 = wall_clock64();
   for (int i = threadIdx.x; i < n; i += NT) {
     const double di = (double)A[pidx(i, i)];

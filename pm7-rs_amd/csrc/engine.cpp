@@ -257,6 +257,9 @@ struct pm7_ctx {
   // 1.3e-7); against 1e-26 it saves 0.25 sweeps per solve (FP64 organic batch 7.40k -> 7.57k
   // solves/s, S/P/Cl batch 415 -> 437/s).
   int eig_tol_exp = 22;
+  // Block Jacobi (16 x 16 subproblems, rotations applied as matrix-core products) in the size
+  // classes that offer it; environment PM7_AMD_EIG_BLOCK=0 selects the scalar rounds everywhere.
+  int eig_block = 1;
   // A batch holding at most GRID_FEW molecules above grid_min_nao basis functions (in the
   // largest megakernel class) runs them on the grid-wide path: one such molecule alone on one CU
   // takes 210 ms at 194 AOs against 58 ms grid-wide, while many of them side by side keep the
@@ -652,6 +655,7 @@ pm7_status prepare(pm7_ctx* ctx, Lane& L, Prepared& P, int64_t max_memory_mb) {
     M.dpath = j.dpath;
     M.rotate_below = ctx->rotate_below;
     M.eig_tol_exp = ctx->eig_tol_exp;
+    M.eig_block = ctx->eig_block;
     M.init_p = j.p0.empty() ? 0 : 1;
     lists[j.variant].push_back(mi);
     atom0 += n;
@@ -1699,6 +1703,7 @@ pm7_status pm7_ctx_create(int32_t device, const char* data_dir, pm7_ctx** out) {
     const int v = std::atoi(te);
     if (v >= 16 && v <= 30) ctx->eig_tol_exp = v;
   }
+  if (const char* eb = std::getenv("PM7_AMD_EIG_BLOCK")) ctx->eig_block = std::atoi(eb) != 0 ? 1 : 0;
   if (!ctx->lane.create()) {
     ctx->lane.destroy();
     delete ctx;

@@ -112,7 +112,7 @@ struct MolDev {
   double e_tol, p_tol, adiis_switch, shift_floor, sad_yy, mixed_switch;
   double rotate_below;  // mixed precision: density change below which orbitals are rotated
   int init_p;           // 1: the starting density was uploaded into the first density slot(s)
-  int pad_;
+  int eig_block;        // 1: block Jacobi in the size classes that offer it (0: the scalar rounds)
 };
 
 // Phases of an SCF iteration timed inside the megakernel (device wall clock, 100 MHz ticks).

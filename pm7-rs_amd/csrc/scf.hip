@@ -242,7 +242,7 @@ __global__ void __launch_bounds__(NT)
           const ElemDev* __restrict__ elem, const double* __restrict__ ocd, const double* __restrict__ w,
           double* __restrict__ slab,
           double* __restrict__ vecs, MolOut* __restrict__ out) {
-  __shared__ double s_a[A_LDS ? MAXN * (MAXN + 1) / 2 : 1];
+  __shared__ double s_a[EigLds<MAXN, A_LDS, V_LDS>::kWords];
   __shared__ double s_v[V_LDS ? MAXN * MAXN : 1];
   __shared__ double s_red[(Wg<NT>::NW + 1) * 3 * HIST];
   __shared__ double2 s_cs[MAXN / 2];  // this round's (cos, sin) per rotation pair
@@ -343,13 +343,18 @@ __global__ void __launch_bounds__(NT)
   // against virtual orbitals only), to the accuracy the FP32 density supports (off-diagonal norm
   // 1e-7 of the total) in at most mixed_sweeps sweeps.
   const double tol_full = exp10((double)-c.M.eig_tol_exp);
-  // tie: 0 RHF, 1 UHF alpha, 2 UHF beta; it sets the order inside degenerate clusters.
+  // tie: 0 RHF, 1 UHF alpha, 2 UHF beta; it sets the order inside degenerate clusters. The
+  // reduced-precision cold solve (the first iteration of a mixed run, a few capped sweeps) keeps
+  // the scalar rounds: where a degenerate set is split by the occupation, the basis inside the set
+  // that its capped sweeps leave is the one the run keeps, and the scalar rounds' is the one that
+  // settles the reference's state on the degenerate golden cases (CfF3).
   auto eig = [&](bool low, const double* Wm, const double* Cp, double* Co, double* ev, int ag, int vg, int tg,
                  int tie) -> int {
     return eigensolve<NT, MAXN, A_LDS, V_LDS, double>(n, ld, Wm, Cp, Co, ev, c.slot(ag), c.slot(vg), c.slot(tg), s_a,
                                                       s_v, s_cs, s_t, s_perm, s_pq, s_red,
                                                       tk + PH_EIG_IN, low ? 1.0e-14 : tol_full,
-                                                      low ? c.M.mixed_sweeps : 40, tie);
+                                                      low ? c.M.mixed_sweeps : 40, tie,
+                                                      c.M.eig_block != 0 && !(low && Cp == nullptr));
   };
 
   if (!c.M.uhf) {
@@ -724,7 +729,7 @@ __global__ void __launch_bounds__(NT)
                                                    double*, double*, \
                                                    MolOut*);
 PM7_SCF_INSTANCE(256, 64, true, true)
-PM7_SCF_INSTANCE(512, 112, true, true)
+PM7_SCF_INSTANCE(512, 112, false, true)
 PM7_SCF_INSTANCE(512, 196, true, false)
 
 // One Fock build on a molecule's slab (the build_fock_spin entry point): F = H + J(Pt) - K(Ps).
@@ -848,7 +853,7 @@ __global__ void __launch_bounds__(NT)
                double* __restrict__ slab, double* __restrict__ vecs, const int* __restrict__ nc_ao,
                const double* __restrict__ v_el, int step, double* __restrict__ wnorm,
                double* __restrict__ e_part) {
-  __shared__ double s_a[A_LDS ? MAXN * (MAXN + 1) / 2 : 1];
+  __shared__ double s_a[EigLds<MAXN, A_LDS, V_LDS>::kWords];
   __shared__ double s_v[V_LDS ? MAXN * MAXN : 1];
   __shared__ double s_red[(Wg<NT>::NW + 1) * 3 * HIST];
   __shared__ double2 s_cs[MAXN / 2];
@@ -921,7 +926,7 @@ __global__ void __launch_bounds__(NT)
     double* ev = eps + ch * ld;
     eigensolve<NT, MAXN, A_LDS, V_LDS, double>(n, ld, W, step > 0 ? c.slot(c_old) : nullptr, c.slot(c_new), ev,
                                                c.slot(ag), c.slot(vg), c.slot(tg), s_a, s_v, s_cs, s_t, s_perm,
-                                               s_pq, s_red, s_tk, tol, 40, 0);
+                                               s_pq, s_red, s_tk, tol, 40, 0, c.M.eig_block != 0);
     __syncthreads();
     const double* C = c.slot(c_new);
     const double scale = uhf ? 0.5 : 1.0;
@@ -1011,7 +1016,7 @@ __global__ void __launch_bounds__(NT) k_dc_density(const MolDev* __restrict__ mo
                                                         double*, double*, const int*, const double*, int, double*,   \
                                                         double*);
 PM7_DC_INSTANCE(256, 64, true, true)
-PM7_DC_INSTANCE(512, 112, true, true)
+PM7_DC_INSTANCE(512, 112, false, true)
 PM7_DC_INSTANCE(512, 196, true, false)
 
 hipError_t launch_dc_solve(int variant, const BatchDev& b, int list_off, int count, const int* nc_ao,
@@ -1023,7 +1028,7 @@ hipError_t launch_dc_solve(int variant, const BatchDev& b, int list_off, int cou
                      b.atom_pp, b.pairs, b.elem, b.ocd, b.w, b.slab, b.vecs, nc_ao, v_el, step, wnorm, e_part)
   switch (variant) {
     case 0: PM7_DC_LAUNCH(256, 64, true, true); break;
-    case 1: PM7_DC_LAUNCH(512, 112, true, true); break;
+    case 1: PM7_DC_LAUNCH(512, 112, false, true); break;
     case 2: PM7_DC_LAUNCH(512, 196, true, false); break;
     default: return hipErrorInvalidValue;  // subsystems above 196 basis functions are refused by the host
   }
@@ -1047,7 +1052,7 @@ hipError_t launch_scf(int variant, const BatchDev& b, int list_off, int count, h
                          b.atom_ao, b.atom_pp, b.pairs, b.elem, b.ocd, b.w, b.slab, b.vecs, b.out);
       break;
     case 1:
-      hipLaunchKernelGGL((k_scf<512, 112, true, true>), dim3(count), dim3(512), 0, s, list, b.mols, b.atom_z,
+      hipLaunchKernelGGL((k_scf<512, 112, false, true>), dim3(count), dim3(512), 0, s, list, b.mols, b.atom_z,
                          b.atom_ao, b.atom_pp, b.pairs, b.elem, b.ocd, b.w, b.slab, b.vecs, b.out);
       break;
     case 2:

@@ -31,7 +31,7 @@ os.environ["GPU_MAX_HW_QUEUES"] = "16"
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 
 
-SCF_KERNELS = ["pm7::k_scf<256, 64, true, true>", "pm7::k_scf<512, 112, false, true>",
+SCF_KERNELS = ["pm7::k_scf<256, 64, true, true>", "pm7::k_scf<512, 120, false, true>",
                "pm7::k_scf<512, 196, true, false>", "pm7::k_scf<512, 1024, false, false>"]
 TRAFFIC_FILES = {"fp64": ROOT / "profiles" / "hbm_traffic_organic256_fp64.json",
                  "mixed": ROOT / "profiles" / "hbm_traffic_organic256_mixed.json"}
@@ -60,8 +60,8 @@ def pct(xs, q):
 
 
 def size_class(n):
-    """The megakernel size class of a basis of n functions (<= 64, <= 112, <= 196, larger)."""
-    return 0 if n <= 64 else (1 if n <= 112 else (2 if n <= 196 else 3))
+    """The megakernel size class of a basis of n functions (<= 64, <= 120, <= 196, larger)."""
+    return 0 if n <= 64 else (1 if n <= 120 else (2 if n <= 196 else 3))
 
 
 def scf_flops(results, classes):

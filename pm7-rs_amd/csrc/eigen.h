@@ -250,15 +250,23 @@ __device__ int jacobi_core(int n2, TA* A, double* Vl, double* Vg, int vrows, dou
   return sweep;
 }
 
-// LDS words of a size class's s_a array: the packed matrix when it lives in LDS, plus, for the
-// block solver (size classes whose limit is a multiple of 16 and whose rotation accumulator is in
-// LDS), one 16 x 16 subproblem and its rotation (512 doubles) per block pair.
+// LDS words of a size class's s_a array, and where the block solver runs. The block solver's
+// dimension is ld rounded up to 16; the rows and columns past ld are decoupled zeros whose part of
+// the rotation accumulator stays the identity, so the accumulator is stored ld x ld either way.
+// With the accumulator in LDS the block solver covers the whole class and s_a adds one 16 x 16
+// subproblem and its rotation (512 doubles) per block pair after the packed matrix (if that is in
+// LDS). With the packed matrix in LDS and the accumulator in L2 it covers dimensions up to 128, its
+// subproblems carved from the packed array's unused tail (128 * 129 / 2 + 8 * 512 doubles fit the
+// 196 class's array).
 template <int MAXN, bool A_LDS, bool V_LDS>
 struct EigLds {
-  static constexpr bool kBlock = V_LDS && MAXN % 16 == 0;
+  static constexpr int kBMax = V_LDS ? (MAXN + 15) / 16 * 16 : ((A_LDS && MAXN >= 128) ? 128 : 0);
+  static constexpr bool kBlock = kBMax > 0;
   static constexpr int kA = A_LDS ? MAXN * (MAXN + 1) / 2 : 0;
-  static constexpr int kBQ = kBlock ? (MAXN / 16) * 512 : 0;
+  static constexpr int kBQ = (kBlock && V_LDS) ? (kBMax / 16) * 512 : 0;
   static constexpr int kWords = kA + kBQ > 0 ? kA + kBQ : 1;
+  static_assert(!(A_LDS && V_LDS) || kBMax == MAXN, "an LDS-resident packed matrix must hold the padded size");
+  static_assert(V_LDS || !kBlock || 128 * 129 / 2 + 8 * 512 <= kA, "block subproblems must fit the packed array");
 };
 
 // LDS stores of this wave are visible to its own later LDS loads (a wave's LDS operations complete
@@ -361,18 +369,34 @@ __device__ inline void jacobi_angle_fast(double app, double aqq, double apq, dou
 // rounding-level element inside an exactly degenerate set still turns the set (by 45 degrees), and
 // that decides which members an occupied count that splits the set fills.
 // bq: P x 512 doubles of LDS (subproblem, then Q); s_flag: P ints. Returns the number of sweeps.
-template <int NT, int MAXN, bool A_LDS>
-__device__ int bjacobi_core(int N, double* A, double* V, double* bq, int* s_flag, double* red, double tol2,
+// A and V each sit in LDS or in L2 (A_LDS, V_LDS); operands that live in L2 are loaded before the
+// inner sweeps. BMAX: the largest N. V is stored nv x nv (nv <= N): its rows and columns past nv
+// belong to the decoupled pad indices and are the identity, read as such and never written.
+template <int NT, int BMAX, bool A_LDS, bool V_LDS>
+__device__ int bjacobi_core(int N, int nv, double* A, double* V, double* bq, int* s_flag, double* red, double tol2,
                             int max_sweeps) {
   constexpr int NW = NT / 64;
   constexpr int NX = 32, NY = NT / NX;
-  constexpr int PMAX = MAXN / 16;
-  constexpr int MAXIA = (PMAX * (PMAX - 1) / 2 + NW - 1) / NW;  // off-diagonal blocks per wave
+  constexpr int PMAX = BMAX / 16;
+  constexpr int MAXIA = A_LDS ? 1 : (PMAX * (PMAX - 1) / 2 + NW - 1) / NW;  // prefetched off-diagonal blocks
+  constexpr int MAXIV = V_LDS ? 1 : (PMAX * PMAX + NW - 1) / NW;            // prefetched accumulator tiles
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r16 = lane & 15, q4 = lane >> 4;
   const int P = N / 16, NB = N / 8;
-  const int nA = P * (P - 1) / 2, nItems = nA + (N / 16) * P;
-  lds_f64* Vs = as_lds(V);
+  const int nA = P * (P - 1) / 2, nV = (N / 16) * P;
   lds_f64* bqs = as_lds(bq);
+  auto vload = [&](int i, int j) -> double {
+    if (i < nv && j < nv) {
+      if constexpr (V_LDS) return as_lds(V)[i * nv + j];
+      else return V[i * nv + j];
+    }
+    return i == j ? 1.0 : 0.0;
+  };
+  auto vstore = [&](int i, int j, double v) {
+    if (i < nv && j < nv) {
+      if constexpr (V_LDS) as_lds(V)[i * nv + j] = v;
+      else V[i * nv + j] = v;
+    }
+  };
   auto aload = [&](int idx) -> double {
     if constexpr (A_LDS) return as_lds(A)[idx];
     else return A[idx];
@@ -416,12 +440,13 @@ __device__ int bjacobi_core(int N, double* A, double* V, double* bq, int* s_flag
           rr_pair(NB, r - 1, k, I, J);
         }
       };
-      // Operands of this wave's off-diagonal blocks (untouched by the inner sweeps).
-      double pav[MAXIA][4];
+      // Operands in L2 of this wave's off-diagonal blocks and accumulator tiles (untouched by the
+      // inner sweeps).
+      double pav[MAXIA][4], pv[MAXIV][4];
 #pragma unroll
       for (int m = 0; m < MAXIA; ++m) {
         const int it = wave + m * NW;
-        if (it < nA) {
+        if (!A_LDS && it < nA) {
           int k, l, Ik, Jk, Il, Jl;
           item_kl(it, k, l);
           blocks(k, Ik, Jk);
@@ -431,6 +456,20 @@ __device__ int bjacobi_core(int N, double* A, double* V, double* bq, int* s_flag
           for (int s = 0; s < 4; ++s) {
             const int b = 4 * s + q4;
             pav[m][s] = aload(pidx(gi, b < 8 ? 8 * Il + b : 8 * Jl + b - 8));
+          }
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < MAXIV; ++m) {
+        const int iv = wave + m * NW;
+        if (!V_LDS && iv < nV) {
+          const int rt = iv / P, l = iv - rt * P;
+          int Il, Jl;
+          blocks(l, Il, Jl);
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            const int b = 4 * s + q4;
+            pv[m][s] = vload(16 * rt + r16, b < 8 ? 8 * Il + b : 8 * Jl + b - 8);
           }
         }
       }
@@ -534,56 +573,66 @@ __device__ int bjacobi_core(int N, double* A, double* V, double* bq, int* s_flag
         if (lane == 0) s_flag[k] = any;
       }
       __syncthreads();
-      for (int it = wave, m = 0; it < nItems; it += NW, ++m) {
-        if (it < nA) {
-          int k, l;
-          item_kl(it, k, l);
-          if (!s_flag[k] && !s_flag[l]) continue;
-          int Ik, Jk, Il, Jl;
-          blocks(k, Ik, Jk);
-          blocks(l, Il, Jl);
-          auto gk = [&](int a) { return a < 8 ? 8 * Ik + a : 8 * Jk + a - 8; };
-          auto gl = [&](int a) { return a < 8 ? 8 * Il + a : 8 * Jl + a - 8; };
-          const lds_f64* Qk = bqs + k * 512 + 256;
-          const lds_f64* Ql = bqs + l * 512 + 256;
-          double av[4], qlv[4], qkv[4];
+      for (int it = wave, m = 0; it < nA; it += NW, ++m) {
+        int k, l;
+        item_kl(it, k, l);
+        if (!s_flag[k] && !s_flag[l]) continue;
+        int Ik, Jk, Il, Jl;
+        blocks(k, Ik, Jk);
+        blocks(l, Il, Jl);
+        auto gk = [&](int a) { return a < 8 ? 8 * Ik + a : 8 * Jk + a - 8; };
+        auto gl = [&](int a) { return a < 8 ? 8 * Il + a : 8 * Jl + a - 8; };
+        const lds_f64* Qk = bqs + k * 512 + 256;
+        const lds_f64* Ql = bqs + l * 512 + 256;
+        double av[4], qlv[4], qkv[4];
 #pragma unroll
-          for (int s = 0; s < 4; ++s) {
-            // m < MAXIA here: item it = wave + m NW < nA <= MAXIA NW.
+        for (int s = 0; s < 4; ++s) {
+          if constexpr (A_LDS) {
+            av[s] = aload(pidx(gk(r16), gl(4 * s + q4)));
+          } else {
+            // m < MAXIA: item it = wave + m NW < nA <= MAXIA NW.
 #pragma unroll
             for (int mm = 0; mm < MAXIA; ++mm)
               if (mm == m) av[s] = pav[mm][s];
-            qlv[s] = Ql[(4 * s + q4) * 16 + r16];
-            qkv[s] = Qk[(4 * s + q4) * 16 + r16];
           }
-          // X = A[S_k, S_l] Q_l, then Y = Q_k^T X (X's accumulator is the second product's B operand).
-          dbl4 x = {0.0, 0.0, 0.0, 0.0}, y = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-          for (int s = 0; s < 4; ++s) x = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s], qlv[s], x, 0, 0, 0);
-#pragma unroll
-          for (int s = 0; s < 4; ++s) y = __builtin_amdgcn_mfma_f64_16x16x4f64(qkv[s], x[s], y, 0, 0, 0);
-#pragma unroll
-          for (int t4 = 0; t4 < 4; ++t4) astore(pidx(gk(tile_row_f64(q4, t4)), gl(r16)), y[t4]);
-        } else {
-          const int v = it - nA, rt = v / P, l = v - rt * P;
-          if (!s_flag[l]) continue;
-          int Il, Jl;
-          blocks(l, Il, Jl);
-          auto gl = [&](int a) { return a < 8 ? 8 * Il + a : 8 * Jl + a - 8; };
-          const lds_f64* Ql = bqs + l * 512 + 256;
-          const int i0 = 16 * rt;
-          double av[4], qlv[4];
-#pragma unroll
-          for (int s = 0; s < 4; ++s) {
-            av[s] = Vs[(i0 + r16) * N + gl(4 * s + q4)];
-            qlv[s] = Ql[(4 * s + q4) * 16 + r16];
-          }
-          dbl4 x = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-          for (int s = 0; s < 4; ++s) x = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s], qlv[s], x, 0, 0, 0);
-#pragma unroll
-          for (int t4 = 0; t4 < 4; ++t4) Vs[(i0 + tile_row_f64(q4, t4)) * N + gl(r16)] = x[t4];
+          qlv[s] = Ql[(4 * s + q4) * 16 + r16];
+          qkv[s] = Qk[(4 * s + q4) * 16 + r16];
         }
+        // X = A[S_k, S_l] Q_l, then Y = Q_k^T X (X's accumulator is the second product's B operand).
+        dbl4 x = {0.0, 0.0, 0.0, 0.0}, y = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) x = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s], qlv[s], x, 0, 0, 0);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) y = __builtin_amdgcn_mfma_f64_16x16x4f64(qkv[s], x[s], y, 0, 0, 0);
+#pragma unroll
+        for (int t4 = 0; t4 < 4; ++t4) astore(pidx(gk(tile_row_f64(q4, t4)), gl(r16)), y[t4]);
+      }
+      for (int iv = wave, m = 0; iv < nV; iv += NW, ++m) {
+        const int rt = iv / P, l = iv - rt * P;
+        if (!s_flag[l]) continue;
+        int Il, Jl;
+        blocks(l, Il, Jl);
+        auto gl = [&](int a) { return a < 8 ? 8 * Il + a : 8 * Jl + a - 8; };
+        const lds_f64* Ql = bqs + l * 512 + 256;
+        const int i0 = 16 * rt;
+        double av[4], qlv[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          if constexpr (V_LDS) {
+            av[s] = vload(i0 + r16, gl(4 * s + q4));
+          } else {
+            // m < MAXIV: tile iv = wave + m NW < nV <= MAXIV NW.
+#pragma unroll
+            for (int mm = 0; mm < MAXIV; ++mm)
+              if (mm == m) av[s] = pv[mm][s];
+          }
+          qlv[s] = Ql[(4 * s + q4) * 16 + r16];
+        }
+        dbl4 x = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) x = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s], qlv[s], x, 0, 0, 0);
+#pragma unroll
+        for (int t4 = 0; t4 < 4; ++t4) vstore(i0 + tile_row_f64(q4, t4), gl(r16), x[t4]);
       }
       __syncthreads();
     }
@@ -626,9 +675,12 @@ __device__ __attribute__((noinline)) int eigensolve(int n, int ld, const double*
   // padded size must fit the molecule's ld x ld slot): the Jacobi dimension is ld rounded up to 16.
   using Lds = EigLds<MAXN, A_LDS, V_LDS>;
   const int n16 = (n2 + 15) & ~15;
-  const bool blk = Lds::kBlock && sizeof(TA) == 8 && block && n16 <= MAXN &&
+  const bool blk = Lds::kBlock && sizeof(TA) == 8 && block && n16 <= Lds::kBMax &&
                    (A_LDS || (int64_t)n16 * (n16 + 1) / 2 <= (int64_t)ld * ld);
   const int nj = blk ? n16 : n2;  // Jacobi dimension (pad rows and columns are zero)
+  // The block solver's subproblems: after the packed matrix in s_a (V in LDS: after the class's
+  // whole packed array; V in L2: after this matrix's).
+  double* bq = V_LDS ? s_a + Lds::kA : s_a + ((n16 * (n16 + 1) / 2 + 1) & ~1);
   // Rotation accumulator: all in LDS (V_LDS), or as many leading rows as fit in the LDS the packed
   // matrix leaves free (A_LDS: the buffer holds MAXN(MAXN+1)/2 doubles), the rest in Vg.
   double* Vl = V_LDS ? s_v : (A_LDS ? s_a + (n2 * (n2 + 1) / 2 + 1) / 2 * 2 : nullptr);
@@ -637,8 +689,8 @@ __device__ __attribute__((noinline)) int eigensolve(int n, int ld, const double*
     const int free_words = A_LDS ? MAXN * (MAXN + 1) / 2 - (n2 * (n2 + 1) / 2 + 1) / 2 * 2 : 0;
     vrows = free_words > 0 ? min(n2, free_words / n2) : 0;
   }
-  if (blk) vrows = nj;  // V_LDS: the whole nj x nj accumulator is in LDS
-  auto vat = [&](int i, int j) -> double& { return i < vrows ? Vl[i * nj + j] : Vg[i * nj + j]; };
+  if (blk) vrows = V_LDS ? n2 : 0;  // the whole n2 x n2 accumulator in LDS, or all of it at Vg
+  auto vat = [&](int i, int j) -> double& { return i < vrows ? Vl[i * n2 + j] : Vg[i * n2 + j]; };
   constexpr int NW = NT / 64;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, q = lane >> 4;
   const int nb = (n + 31) >> 5;  // 32 x 32 MFMA blocks (their 32 nb rows cover n2)
@@ -682,8 +734,8 @@ __device__ __attribute__((noinline)) int eigensolve(int n, int ld, const double*
       for (int i = threadIdx.x % NX; i <= j; i += NX)
         A[j * (j + 1) / 2 + i] = j < n2 ? (TA)W[(int64_t)i * ld + j] : (TA)0;
   }
-  for (int idx = threadIdx.x; idx < nj * nj; idx += NT) {
-    const int i = idx / nj, j = idx - i * nj;
+  for (int idx = threadIdx.x; idx < n2 * n2; idx += NT) {
+    const int i = idx / n2, j = idx - i * n2;
     vat(i, j) = (i == j) ? 1.0 : 0.0;
   }
   __syncthreads();
@@ -691,8 +743,8 @@ __device__ __attribute__((noinline)) int eigensolve(int n, int ld, const double*
   int sweeps;
   if constexpr (Lds::kBlock && sizeof(TA) == 8) {
     if (blk)
-      sweeps = bjacobi_core<NT, MAXN, A_LDS>(nj, reinterpret_cast<double*>(A), Vl, s_a + Lds::kA, s_perm, red, tol2,
-                                             max_sweeps);
+      sweeps = bjacobi_core<NT, Lds::kBMax, A_LDS, V_LDS>(nj, n2, reinterpret_cast<double*>(A), V_LDS ? Vl : Vg,
+                                                          bq, s_perm, red, tol2, max_sweeps);
     else
       sweeps = jacobi_core<NT, MAXN, V_LDS, TA>(n2, A, Vl, Vg, vrows, s_cs, s_t, s_pq, red, tol2, max_sweeps);
   } else {

@@ -48,10 +48,11 @@ struct BatchDev {
 constexpr int GL_WORDS = 2 * 48 + 2 * 40;
 
 // SCF kernel size classes (padded basis size): 0 <= 64 (256 threads, eigensolver fully in LDS),
-// 1 <= 112 (512 threads, fully in LDS), 2 <= 196 (packed matrix in LDS, rotations in L2),
-// 3 <= 1024 (eigensolver in L2/HBM).
+// 1 <= 120 (512 threads, packed matrix in L2, rotation accumulator in LDS), 2 <= 196 (packed
+// matrix in LDS, rotations in L2), 3: larger (the grid-wide path). Classes 0 and 1, and class 2 up
+// to 128 basis functions, diagonalise with the block Jacobi solver.
 constexpr int SCF_VARIANTS = 4;
-constexpr int VARIANT_MAX_NAO[SCF_VARIANTS] = {64, 112, 196, 1 << 30};
+constexpr int VARIANT_MAX_NAO[SCF_VARIANTS] = {64, 120, 196, 1 << 30};
 constexpr int LARGE_VARIANT = 3;  // above 196 basis functions: the grid-wide path (large.hip)
 constexpr int GRID_FEW = 8;        // at most this many large-class-2 molecules of a batch go grid-wide
 // Large-class molecules up to this basis size run in groups of up to large_group_size() SCFs whose full

@@ -721,15 +721,16 @@ __global__ void __launch_bounds__(NT)
   }
 }
 
-// Size classes: <= 64 and <= 112 basis functions keep the packed matrix and the rotation
-// accumulator in LDS; <= 196 keeps the packed matrix in LDS; larger molecules run from L2/HBM.
+// Size classes: <= 64 basis functions keep the packed matrix and the rotation accumulator in LDS,
+// <= 120 the accumulator (the packed matrix in L2), <= 196 the packed matrix; larger molecules run
+// on the grid-wide path.
 #define PM7_SCF_INSTANCE(NT, MAXN, AL, VL)                                                                        \
   template __global__ void k_scf<NT, MAXN, AL, VL>(const int*, const MolDev*, const int*, const int*, const int*, \
                                                    const PairDev*, const ElemDev*, const double*, const double*, \
                                                    double*, double*, \
                                                    MolOut*);
 PM7_SCF_INSTANCE(256, 64, true, true)
-PM7_SCF_INSTANCE(512, 112, false, true)
+PM7_SCF_INSTANCE(512, 120, false, true)
 PM7_SCF_INSTANCE(512, 196, true, false)
 
 // One Fock build on a molecule's slab (the build_fock_spin entry point): F = H + J(Pt) - K(Ps).
@@ -1016,7 +1017,7 @@ __global__ void __launch_bounds__(NT) k_dc_density(const MolDev* __restrict__ mo
                                                         double*, double*, const int*, const double*, int, double*,   \
                                                         double*);
 PM7_DC_INSTANCE(256, 64, true, true)
-PM7_DC_INSTANCE(512, 112, false, true)
+PM7_DC_INSTANCE(512, 120, false, true)
 PM7_DC_INSTANCE(512, 196, true, false)
 
 hipError_t launch_dc_solve(int variant, const BatchDev& b, int list_off, int count, const int* nc_ao,
@@ -1028,7 +1029,7 @@ hipError_t launch_dc_solve(int variant, const BatchDev& b, int list_off, int cou
                      b.atom_pp, b.pairs, b.elem, b.ocd, b.w, b.slab, b.vecs, nc_ao, v_el, step, wnorm, e_part)
   switch (variant) {
     case 0: PM7_DC_LAUNCH(256, 64, true, true); break;
-    case 1: PM7_DC_LAUNCH(512, 112, false, true); break;
+    case 1: PM7_DC_LAUNCH(512, 120, false, true); break;
     case 2: PM7_DC_LAUNCH(512, 196, true, false); break;
     default: return hipErrorInvalidValue;  // subsystems above 196 basis functions are refused by the host
   }
@@ -1052,7 +1053,7 @@ hipError_t launch_scf(int variant, const BatchDev& b, int list_off, int count, h
                          b.atom_ao, b.atom_pp, b.pairs, b.elem, b.ocd, b.w, b.slab, b.vecs, b.out);
       break;
     case 1:
-      hipLaunchKernelGGL((k_scf<512, 112, false, true>), dim3(count), dim3(512), 0, s, list, b.mols, b.atom_z,
+      hipLaunchKernelGGL((k_scf<512, 120, false, true>), dim3(count), dim3(512), 0, s, list, b.mols, b.atom_z,
                          b.atom_ao, b.atom_pp, b.pairs, b.elem, b.ocd, b.w, b.slab, b.vecs, b.out);
       break;
     case 2:

@@ -343,18 +343,20 @@ __global__ void __launch_bounds__(NT)
   // against virtual orbitals only), to the accuracy the FP32 density supports (off-diagonal norm
   // 1e-7 of the total) in at most mixed_sweeps sweeps.
   const double tol_full = exp10((double)-c.M.eig_tol_exp);
-  // tie: 0 RHF, 1 UHF alpha, 2 UHF beta; it sets the order inside degenerate clusters. The
-  // reduced-precision cold solve (the first iteration of a mixed run, a few capped sweeps) keeps
-  // the scalar rounds: where a degenerate set is split by the occupation, the basis inside the set
-  // that its capped sweeps leave is the one the run keeps, and the scalar rounds' is the one that
-  // settles the reference's state on the degenerate golden cases (CfF3).
+  // tie: 0 RHF, 1 UHF alpha, 2 UHF beta; it sets the order inside degenerate clusters. In the
+  // smallest class the reduced-precision cold solve (the first iteration of a mixed run, a few
+  // capped sweeps) keeps the scalar rounds: where the occupation splits a degenerate set, the basis
+  // inside the set that its capped sweeps leave is the one the run keeps, and the scalar rounds' is
+  // the one that settles the reference's state on the small symmetric molecules of the golden set
+  // (CfF3). There the packed matrix is in LDS and the scalar rounds cost little; in the larger
+  // classes (packed matrix in L2) they would cost as much as the rest of the eigensolves.
   auto eig = [&](bool low, const double* Wm, const double* Cp, double* Co, double* ev, int ag, int vg, int tg,
                  int tie) -> int {
     return eigensolve<NT, MAXN, A_LDS, V_LDS, double>(n, ld, Wm, Cp, Co, ev, c.slot(ag), c.slot(vg), c.slot(tg), s_a,
                                                       s_v, s_cs, s_t, s_perm, s_pq, s_red,
                                                       tk + PH_EIG_IN, low ? 1.0e-14 : tol_full,
                                                       low ? c.M.mixed_sweeps : 40, tie,
-                                                      c.M.eig_block != 0 && !(low && Cp == nullptr));
+                                                      c.M.eig_block != 0 && !(low && Cp == nullptr && MAXN <= 64));
   };
 
   if (!c.M.uhf) {

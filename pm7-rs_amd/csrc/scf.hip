@@ -166,27 +166,38 @@ __device__ void copy_mat(int ld, const double* src, double* dst) {
 }
 
 // DIIS history dots of the newest slot h against every live slot t (s_slot[0..cnt)):
-// <E_h, E_t>, <D_h, F_t> and <D_t, F_h> over the padded ld x ld matrices, into out[3 q + k] (LDS,
-// written by thread 0). A function of its own so its 3 x HIST partial sums do not add to the
-// registers of the rest of the megakernel.
+// <E_h, E_t>, and with want_m also <D_h, F_t> and <D_t, F_h>, over the padded ld x ld matrices,
+// into out[3 q + k] (LDS, written by thread 0). The <D, F> dots feed only A-DIIS, so iterations
+// that will not run it read the error matrices alone (a third of the history bytes). A function of
+// its own so its 3 x HIST partial sums do not add to the registers of the rest of the megakernel.
 template <int NT>
 __device__ __attribute__((noinline)) void history_dots(int ld, int cnt, const int* s_slot, const double* __restrict__ E,
                                                        const double* __restrict__ P, const double* __restrict__ F,
                                                        const double* __restrict__ hE, const double* __restrict__ hF,
-                                                       const double* __restrict__ hD, double* red, double* out) {
+                                                       const double* __restrict__ hD, bool want_m, double* red,
+                                                       double* out) {
   double acc[3 * HIST];
   for (int k = 0; k < 3 * HIST; ++k) acc[k] = 0.0;
   const int64_t sq = (int64_t)ld * ld;
-  for (int idx = threadIdx.x; idx < ld * ld; idx += NT) {
-    const double eh0 = E[idx], dh = P[idx], fh = F[idx];
+  if (want_m) {
+    for (int idx = threadIdx.x; idx < ld * ld; idx += NT) {
+      const double eh0 = E[idx], dh = P[idx], fh = F[idx];
 #pragma unroll
-    for (int q = 0; q < HIST; ++q) {
-      if (q < cnt) {
-        const int64_t off = (int64_t)s_slot[q] * sq + idx;
-        acc[3 * q] += eh0 * hE[off];
-        acc[3 * q + 1] += dh * hF[off];
-        acc[3 * q + 2] += hD[off] * fh;
+      for (int q = 0; q < HIST; ++q) {
+        if (q < cnt) {
+          const int64_t off = (int64_t)s_slot[q] * sq + idx;
+          acc[3 * q] += eh0 * hE[off];
+          acc[3 * q + 1] += dh * hF[off];
+          acc[3 * q + 2] += hD[off] * fh;
+        }
       }
+    }
+  } else {
+    for (int idx = threadIdx.x; idx < ld * ld; idx += NT) {
+      const double eh0 = E[idx];
+#pragma unroll
+      for (int q = 0; q < HIST; ++q)
+        if (q < cnt) acc[3 * q] += eh0 * hE[(int64_t)s_slot[q] * sq + idx];
     }
   }
   wg_sum_n<NT, 3 * HIST>(acc, red);
@@ -251,6 +262,7 @@ __global__ void __launch_bounds__(NT)
   __shared__ int s_pq[MAXN / 2];  // this round's rotation pairs, p | q << 16
   __shared__ double s_G[HIST * HIST], s_M[HIST * HIST], s_coef[HIST + 1];
   __shared__ int s_slot[HIST], s_cslot[HIST];
+  __shared__ int s_mok[HIST];  // slot's <D, F> dots against the slots before it are in s_M
   __shared__ double s_dots[3 * HIST];
   __shared__ MolOut s_res;                   // the molecule's result record (thread 0 keeps it)
   __shared__ unsigned long long s_tk[NPHASE];  // per-phase device clocks (thread 0)
@@ -290,6 +302,7 @@ __global__ void __launch_bounds__(NT)
     s_tail.best = 1.0e30;
     s_dbl[0] = 0.0;
     s_dbl[2] = 1.0e30;
+    for (int k = 0; k < HIST; ++k) s_mok[k] = 0;
   }
   __syncthreads();
   double* H = c.slot(S_H);
@@ -392,18 +405,26 @@ __global__ void __launch_bounds__(NT)
       }
       __syncthreads();
       const int cnt = s_int[1];
-      // History dots: <E_h,E_t>, <D_h,F_t>, <D_t,F_h> for every live slot t.
+      // History dots: <E_h,E_t> for every live slot t, and <D_h,F_t>, <D_t,F_h> when this iteration
+      // runs A-DIIS. A slot added without them is marked; if A-DIIS runs again while it is live,
+      // its dots against every live slot are taken then (the same products in the same order, so
+      // the same values as if they had been taken on time).
+      const bool use_adiis = c.M.use_diis && c.M.accel == 2 && err > c.M.adiis_switch && cnt >= 2;
       {
-        history_dots<NT>(ld, cnt, s_slot, E, P, F, c.slot(S_HE), c.slot(S_HF), c.slot(S_HD), s_red, s_dots);
+        history_dots<NT>(ld, cnt, s_slot, E, P, F, c.slot(S_HE), c.slot(S_HF), c.slot(S_HD), use_adiis, s_red,
+                         s_dots);
         const double* acc = s_dots;  // thread 0 wrote them and reads them
         if (threadIdx.x == 0) {
           for (int q = 0; q < cnt; ++q) {
             const int t = s_slot[q];
             s_G[h * HIST + t] = acc[3 * q];
             s_G[t * HIST + h] = acc[3 * q];
-            s_M[h * HIST + t] = acc[3 * q + 1];
-            s_M[t * HIST + h] = acc[3 * q + 2];
+            if (use_adiis) {
+              s_M[h * HIST + t] = acc[3 * q + 1];
+              s_M[t * HIST + h] = acc[3 * q + 2];
+            }
           }
+          s_mok[h] = use_adiis ? 1 : 0;
           int ncoef = 0, skip = 0;
           if (c.M.use_diis && c.M.accel != 0 && (c.M.accel == 1 || err <= c.M.adiis_switch))
             ncoef = cdiis(cnt, s_slot, s_G, s_coef, &skip, s_wk);
@@ -412,7 +433,23 @@ __global__ void __launch_bounds__(NT)
           s_dbl[1] = err;
         }
         __syncthreads();
-        const bool use_adiis = c.M.use_diis && c.M.accel == 2 && err > c.M.adiis_switch && cnt >= 2;
+        if (use_adiis) {
+          for (int q = 0; q < cnt; ++q) {
+            const int t = s_slot[q];
+            if (s_mok[t]) continue;  // uniform: thread 0 wrote it before the barrier
+            history_dots<NT>(ld, cnt, s_slot, c.slot(S_HE + t), c.slot(S_HD + t), c.slot(S_HF + t), c.slot(S_HE),
+                             c.slot(S_HF), c.slot(S_HD), true, s_red, s_dots);
+            if (threadIdx.x == 0) {
+              for (int u = 0; u < cnt; ++u) {
+                const int v = s_slot[u];
+                s_M[t * HIST + v] = s_dots[3 * u + 1];
+                s_M[v * HIST + t] = s_dots[3 * u + 2];
+              }
+              s_mok[t] = 1;
+            }
+            __syncthreads();
+          }
+        }
         if (use_adiis && threadIdx.x < 64) {
           adiis_wave(cnt, s_slot, s_M, s_coef);
           if (threadIdx.x == 0) {

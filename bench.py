@@ -149,6 +149,7 @@ def main() -> int:
     ap.add_argument("--single", type=int, default=32, help="molecules timed one at a time")
     ap.add_argument("--cpu-sample", type=int, default=256, help="molecules timed on one CPU thread")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (engine defaults)")
     ap.add_argument("--serving", type=int, default=2000, help="requests in the serving stream (0: skip)")
     ap.add_argument("--large-waters", type=int, default=500,
                     help="water cluster of the large-system run (0: skip); 500 waters = 3000 AOs")
@@ -281,6 +282,16 @@ def main() -> int:
     mols = [pm7amd.Molecule.from_symbols_angstrom(a) for a in batch]
     print(f"[bench] workload built: {len(mols)} molecules in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     eng = pm7amd.Engine(0)
+    # The engine's autotuner picks the launch shapes for this workload before anything is timed:
+    # per size class block Jacobi or scalar rounds, the reduced-precision sweep cap, and the number
+    # of batches in flight (highest throughput whose median step latency stays within 1.5x of one
+    # batch at a time). Every measurement below runs with its choice.
+    tune = None
+    if not args.no_tune:
+        tune = eng.autotune(mols, pm7amd.Pm7Options(precision="mixed"), steps=3, latency_budget=1.5)
+        tune = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in tune.items()}
+        print(f"[bench] autotune: {tune}", file=sys.stderr, flush=True)
+    in_flight = tune["batches_in_flight"] if tune else 2
 
     def run_mode(precision):
         opts = pm7amd.Pm7Options(precision=precision)
@@ -299,7 +310,7 @@ def main() -> int:
         total = time.perf_counter() - start
         return res, lat, kern, total
 
-    def run_pipelined(precision, in_flight=2):
+    def run_pipelined(precision, in_flight=in_flight):
         """The same steps issued back to back through pm7_batch_submit / pm7_batch_wait, `in_flight`
         batches on the device at once (PM7_AMD_ASYNC_LANES): each step is still one batched solve of
         the 256 molecules, its latency is submit -> results on the host; the device span runs from
@@ -739,8 +750,10 @@ def main() -> int:
         "dtype": "f32+f64" if head_mixed else "f64",
         "data": "synthetic",
         "config": {"workload": "256 closed-shell H/C/N/O molecules, 3-60 atoms, seeded random walk (seed 3), "
-                               "one batched solve per step, steps pipelined 2 in flight", "n_molecules": args.molecules,
+                               f"one batched solve per step, steps pipelined {head['in_flight']} in flight",
+                   "n_molecules": args.molecules,
                    "precision": "mixed" if head_mixed else "fp64", "steps_in_flight": head["in_flight"]},
+        "autotune": tune,
         "latency_ms": head["latency_ms"],
         "value_note": "device span from the first timed step's integrals kernel to the last step's output gather "
                       "(HIP events), steps pipelined through pm7_batch_submit/wait; latency = submit -> results on "

@@ -337,10 +337,35 @@ pm7_status pm7_server_poll(pm7_server* srv, int64_t request_id, int32_t timeout_
 /* Finish the queued and in-flight requests, release unclaimed results, stop the thread. */
 void pm7_server_stop(pm7_server* srv);
 
+/* Autotuner (no pm7-rs counterpart; it serves callers that run many batches of one kind of
+ * workload). Times candidate launch shapes on the caller's batch and keeps the fastest in the
+ * context: per size class (<= 64, <= 120, <= 196 basis functions) the block Jacobi solver or the
+ * scalar rounds, the Jacobi sweep cap of the reduced-precision phase (mixed runs only), and the
+ * number of batches pm7_batch_submit keeps in flight, the one with the highest device throughput
+ * whose median step latency stays within latency_budget (>= 1, 0: 2) times the one-in-flight
+ * latency. Each candidate runs `steps` (1..64, 0: 4) batches after one untimed batch. Every
+ * candidate gives results within the parity bars of the defaults; the number in flight does not
+ * change results at all. cache_path (may be NULL): a text file of earlier choices keyed by device
+ * and workload signature; a matching line is applied without timing (candidates = 0), a new choice
+ * is written back. No batch may be in flight on the context during the call. */
+typedef struct {
+  int32_t eig_block_mask;       /* bit v: size class v uses the block Jacobi solver */
+  int32_t mixed_sweeps;         /* Jacobi sweep cap of the reduced-precision phase */
+  int32_t batches_in_flight;    /* pm7_batch_submit depth */
+  int32_t candidates;           /* settings timed (0: taken from the cache file) */
+  double solves_per_s;          /* the chosen setting: device throughput of `steps` batches */
+  double step_ms;               /* its median submit-to-result latency per batch */
+  double default_solves_per_s;  /* the context's settings before the call, measured the same way */
+  double tune_ms;               /* wall time of the call */
+} pm7_tune_result_v1;
+pm7_status pm7_autotune(pm7_ctx* ctx, int32_t n, const pm7_molecule_v1* mols, const pm7_options_v2* opts,
+                        int32_t n_opts, int32_t steps, double latency_budget, const char* cache_path,
+                        pm7_tune_result_v1* out);
+
 /* Timing of the last pm7_run / pm7_run_batch call in milliseconds, measured with HIP events on
  * the stream each kernel runs on: [0] integrals + H_core, [1] all SCF megakernels, [2] output
  * gather, [3] whole call (host wall clock), [4..7] SCF megakernel of size class 0/1/2/3 (<= 64,
- * <= 112, <= 196, <= 1024 basis functions; 0 when the class was empty), then host wall clock:
+ * <= 120, <= 196, larger basis functions; 0 when the class was empty), then host wall clock:
  * [8] validation, layout and upload, [9] kernel launches, host-side corrections and the copy of
  * the packed results to the host (which waits for the kernels), [10] remaining device wait,
  * [11] result assembly. After pm7_response_fock_batch: [1] its response kernel, the rest 0.

@@ -257,9 +257,10 @@ struct pm7_ctx {
   // 1.3e-7); against 1e-26 it saves 0.25 sweeps per solve (FP64 organic batch 7.40k -> 7.57k
   // solves/s, S/P/Cl batch 415 -> 437/s).
   int eig_tol_exp = 22;
-  // Block Jacobi (16 x 16 subproblems, rotations applied as matrix-core products) in the size
-  // classes that offer it; environment PM7_AMD_EIG_BLOCK=0 selects the scalar rounds everywhere.
-  int eig_block = 1;
+  // Block Jacobi (16 x 16 subproblems, rotations applied as matrix-core products): bit v on means
+  // size class v uses it where the class offers it (environment PM7_AMD_EIG_BLOCK: 0 the scalar
+  // rounds everywhere, 1 block everywhere, larger values a mask; pm7_autotune sets it).
+  int eig_block_mask = 0xF;
   // A batch holding at most GRID_FEW molecules above grid_min_nao basis functions (in the
   // largest megakernel class) runs them on the grid-wide path: one such molecule alone on one CU
   // takes 210 ms at 194 AOs against 58 ms grid-wide, while many of them side by side keep the
@@ -655,7 +656,7 @@ pm7_status prepare(pm7_ctx* ctx, Lane& L, Prepared& P, int64_t max_memory_mb) {
     M.dpath = j.dpath;
     M.rotate_below = ctx->rotate_below;
     M.eig_tol_exp = ctx->eig_tol_exp;
-    M.eig_block = ctx->eig_block;
+    M.eig_block = (ctx->eig_block_mask >> j.variant) & 1;
     M.init_p = j.p0.empty() ? 0 : 1;
     lists[j.variant].push_back(mi);
     atom0 += n;
@@ -1703,7 +1704,10 @@ pm7_status pm7_ctx_create(int32_t device, const char* data_dir, pm7_ctx** out) {
     const int v = std::atoi(te);
     if (v >= 16 && v <= 30) ctx->eig_tol_exp = v;
   }
-  if (const char* eb = std::getenv("PM7_AMD_EIG_BLOCK")) ctx->eig_block = std::atoi(eb) != 0 ? 1 : 0;
+  if (const char* eb = std::getenv("PM7_AMD_EIG_BLOCK")) {
+    const int v = std::atoi(eb);
+    ctx->eig_block_mask = v == 0 ? 0 : (v == 1 ? 0xF : (v & 0xF));
+  }
   if (!ctx->lane.create()) {
     ctx->lane.destroy();
     delete ctx;
@@ -2719,3 +2723,30 @@ void pm7_last_timings(const pm7_ctx* ctx, double* ms, int32_t n) {
 }
 
 }  // extern "C"
+
+namespace pm7 {
+
+TuneKnobs tune_knobs(const pm7_ctx* ctx) {
+  return TuneKnobs{ctx->eig_block_mask, ctx->mixed_sweeps, ctx->async_max};
+}
+
+void set_tune_knobs(pm7_ctx* ctx, const TuneKnobs& k) {
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->eig_block_mask = k.eig_block_mask & 0xF;
+  if (k.mixed_sweeps >= 1 && k.mixed_sweeps <= 40) ctx->mixed_sweeps = k.mixed_sweeps;
+  if (k.batches_in_flight >= 1 && k.batches_in_flight <= 8) ctx->async_max = k.batches_in_flight;
+}
+
+int ctx_device(const pm7_ctx* ctx) { return ctx->device; }
+
+int ctx_batches_in_flight(pm7_ctx* ctx) {
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return (int)ctx->inflight.size();
+}
+
+void ctx_set_error(pm7_ctx* ctx, const std::string& msg) {
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->err = msg;
+}
+
+}  // namespace pm7

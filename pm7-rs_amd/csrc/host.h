@@ -10,7 +10,21 @@
 
 #include "pm7_device.h"
 
+struct pm7_ctx;
+
 namespace pm7 {
+
+// Launch-shape settings of a context that pm7_autotune chooses (engine.cpp keeps them).
+struct TuneKnobs {
+  int eig_block_mask;     // bit v: size class v diagonalises with the block Jacobi solver
+  int mixed_sweeps;       // Jacobi sweep cap of the reduced-precision phase
+  int batches_in_flight;  // pm7_batch_submit depth
+};
+TuneKnobs tune_knobs(const pm7_ctx* ctx);
+void set_tune_knobs(pm7_ctx* ctx, const TuneKnobs& k);
+int ctx_device(const pm7_ctx* ctx);
+int ctx_batches_in_flight(pm7_ctx* ctx);
+void ctx_set_error(pm7_ctx* ctx, const std::string& msg);
 
 struct HostElement {
   bool present = false;

@@ -107,10 +107,16 @@ ABI_FUNCTIONS = [
     "pm7_server_start", "pm7_server_submit", "pm7_server_poll", "pm7_server_stop", "pm7_gradient_batch",
     "pm7_gradient", "pm7_response_fock_batch", "pm7_response_fock_spin_batch", "pm7_fock_spin_noalloc",
     "pm7_dandc_default_options", "pm7_run_dandc", "pm7_dandc_result_free", "pm7_dandc_partition",
-    "pm7_partition_free", "pm7_batch_submit", "pm7_batch_wait",
+    "pm7_partition_free", "pm7_batch_submit", "pm7_batch_wait", "pm7_autotune",
 ]
 
 _lib = None
+
+
+class _TuneResult(C.Structure):
+    _fields_ = [("eig_block_mask", C.c_int32), ("mixed_sweeps", C.c_int32), ("batches_in_flight", C.c_int32),
+                ("candidates", C.c_int32), ("solves_per_s", C.c_double), ("step_ms", C.c_double),
+                ("default_solves_per_s", C.c_double), ("tune_ms", C.c_double)]
 
 
 def library() -> C.CDLL:
@@ -137,6 +143,8 @@ def library() -> C.CDLL:
     lib.pm7_gradient.argtypes = [vp, C.POINTER(_Molecule), C.POINTER(_Options), C.POINTER(_Result),
                                  C.POINTER(C.c_double)]
     lib.pm7_core_hamiltonian.argtypes = [vp, C.POINTER(_Molecule), C.POINTER(C.c_double), C.POINTER(C.c_int32)]
+    lib.pm7_autotune.argtypes = [vp, C.c_int32, C.POINTER(_Molecule), C.POINTER(_Options), C.c_int32, C.c_int32,
+                                 C.c_double, C.c_char_p, C.POINTER(_TuneResult)]
     lib.pm7_fock_spin.argtypes = [vp, C.POINTER(_Molecule), C.POINTER(C.c_double), C.POINTER(C.c_double),
                                   C.POINTER(C.c_double)]
     lib.pm7_fock_spin_noalloc.argtypes = [vp, C.POINTER(_Molecule), C.POINTER(C.c_double), C.POINTER(C.c_double),
@@ -521,6 +529,30 @@ class Engine:
             grads.append(flat[off:off + 3 * len(m.z)].reshape(-1, 3))
             off += 3 * len(m.z)
         return res, grads
+
+    def autotune(self, molecules: Sequence[Molecule], options=None, steps: int = 4, latency_budget: float = 2.0,
+                 cache_path: Optional[str] = None) -> dict:
+        """pm7_autotune: time this context's launch shapes on the given batch (eigensolver per size
+        class, reduced-precision sweep cap, batches in flight) and keep the fastest; returns the
+        choice and its measured throughput."""
+        n = len(molecules)
+        if options is None:
+            options = Pm7Options()
+        opts_list = options if isinstance(options, (list, tuple)) else [options]
+        c_mols = (_Molecule * n)()
+        keep = []
+        for i, m in enumerate(molecules):
+            cm, k = self._mol(m)
+            c_mols[i] = cm
+            keep.append(k)
+        c_list = [o.to_c() for o in opts_list]
+        c_opts = (_Options * len(opts_list))(*c_list)
+        out = _TuneResult()
+        st = self._lib.pm7_autotune(self._h, n, c_mols, c_opts, len(opts_list), int(steps), float(latency_budget),
+                                    cache_path.encode() if cache_path else None, C.byref(out))
+        if st != 0:
+            raise Pm7Error(st, self.last_error())
+        return {f: getattr(out, f) for f, _ in _TuneResult._fields_}
 
     def _batch(self, molecules, options, raise_on_error, grad_flat):
         n = len(molecules)

@@ -69,7 +69,8 @@ def test_struct_layout_matches_the_compiled_header(tmp_path):
         pytest.skip("no C compiler")
     lines = ["#include <stddef.h>", "#include <stdio.h>", f'#include "{HEADER}"', "int main(void) {"]
     for struct, cls in (("pm7_options_v2", pm7amd._Options), ("pm7_result_v2", pm7amd._Result),
-                        ("pm7_molecule_v1", pm7amd._Molecule), ("pm7_server_config_v1", pm7amd._ServerConfig)):
+                        ("pm7_molecule_v1", pm7amd._Molecule), ("pm7_server_config_v1", pm7amd._ServerConfig),
+                        ("pm7_tune_result_v1", pm7amd._TuneResult)):
         lines.append(f'  printf("{struct} size %zu\\n", sizeof({struct}));')
         for name, _ in cls._fields_:
             lines.append(f'  printf("{struct} {name} %zu\\n", offsetof({struct}, {name}));')
@@ -83,7 +84,8 @@ def test_struct_layout_matches_the_compiled_header(tmp_path):
         struct, name, value = line.split()
         got[(struct, name)] = int(value)
     for struct, cls in (("pm7_options_v2", pm7amd._Options), ("pm7_result_v2", pm7amd._Result),
-                        ("pm7_molecule_v1", pm7amd._Molecule), ("pm7_server_config_v1", pm7amd._ServerConfig)):
+                        ("pm7_molecule_v1", pm7amd._Molecule), ("pm7_server_config_v1", pm7amd._ServerConfig),
+                        ("pm7_tune_result_v1", pm7amd._TuneResult)):
         assert got[(struct, "size")] == C.sizeof(cls), struct
         for name, _ in cls._fields_:
             assert got[(struct, name)] == getattr(cls, name).offset, (struct, name)

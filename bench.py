@@ -632,8 +632,8 @@ def main() -> int:
         eng.run(m)
         t = eng.last_timings_ms()
         single_dev += (t[0] + t[1] + t[2]) * 1e-3
-    single_api_rate = len(single) / (time.perf_counter() - s)
-    single_rate = len(single) / single_dev
+    single_api_rate = len(single) / (time.perf_counter() - s) if single else 0.0
+    single_rate = len(single) / single_dev if single_dev > 0 else 0.0
 
     # Roofline of the dominant kernel: the SCF megakernel size class with the largest time, its
     # algorithmic flops (11 n^3 + 2 n^2 n_occ per molecule-iteration: the Roothaan step of the
@@ -795,7 +795,7 @@ def main() -> int:
         "response_fock": response,
         "dandc": dandc,
         "single_molecule_fp64": {"solves_per_s": round(single_rate, 3), "molecules": len(single),
-                                 "batched_speedup": round(head["solves_per_s"] / single_rate, 3)},
+                                 "batched_speedup": round(head["solves_per_s"] / single_rate, 3) if single_rate else None},
         "kernel_ms": {"integrals": round(float(np.mean([k[0] for k in kern64])), 4),
                       "scf": round(float(np.mean([k[1] for k in kern64])), 4),
                       "gather": round(float(np.mean([k[2] for k in kern64])), 4),

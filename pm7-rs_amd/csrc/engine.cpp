@@ -266,6 +266,9 @@ struct pm7_ctx {
   // takes 210 ms at 194 AOs against 58 ms grid-wide, while many of them side by side keep the
   // device busier as megakernels (environment PM7_AMD_GRID_MIN_NAO, 32..196, default 128).
   int grid_min_nao = 128;
+  // Divide and conquer replays each step after the first as a captured HIP graph (environment
+  // PM7_AMD_DC_GRAPH=0: every step's launches issued one by one).
+  bool dc_graph = true;
   // Allocation-free entry point: its lane's buffers live in the device's static scratch array.
   Arena arena;
   Lane na_lane;
@@ -1704,6 +1707,7 @@ pm7_status pm7_ctx_create(int32_t device, const char* data_dir, pm7_ctx** out) {
     const int v = std::atoi(te);
     if (v >= 16 && v <= 30) ctx->eig_tol_exp = v;
   }
+  if (const char* dg = std::getenv("PM7_AMD_DC_GRAPH")) ctx->dc_graph = std::atoi(dg) != 0;
   if (const char* eb = std::getenv("PM7_AMD_EIG_BLOCK")) {
     const int v = std::atoi(eb);
     ctx->eig_block_mask = v == 0 ? 0 : (v == 1 ? 0xF : (v & 0xF));
@@ -2554,28 +2558,66 @@ pm7_status dandc_impl(pm7_ctx* ctx, const pm7_molecule_v1& mol, const pm7_option
   int it = 0;
   bool converged = false;
   double rms = INFINITY, e_el = 0.0, fermi = 0.0;
-  for (; it < d.max_scf;) {
-    DCCHK(launch_dc_pop(dd, s0));
-    DCCHK(launch_dc_field(b, dd, dd.pop, dd.g, dd.v_el, s0));
-    DCCHK(launch_dc_gather(b, dd, s0));
-    // The size classes' subsystem solves side by side, each on its own stream.
-    DCCHK(hipEventRecord(B.ev[2], s0));
+  // One step's launches (about 22: the environment, the gather, the size classes' solves forked
+  // onto their streams and joined by events, the Fermi passes, the densities, the assembly, the
+  // finish and the 32-byte read-back). They are the same every step but for the parity of the
+  // eigenvector buffers (and step 0's cold solve), so steps >= 1 replay one of two HIP graphs, each
+  // captured the first time its parity comes up (no memset nodes: every buffer the step clears is
+  // written by a kernel). PM7_AMD_DC_GRAPH=0 issues every step eagerly.
+  auto issue_step = [&](int step) -> hipError_t {
+    hipError_t e;
+    if ((e = launch_dc_pop(dd, s0)) != hipSuccess) return e;
+    if ((e = launch_dc_field(b, dd, dd.pop, dd.g, dd.v_el, s0)) != hipSuccess) return e;
+    if ((e = launch_dc_gather(b, dd, s0)) != hipSuccess) return e;
+    if ((e = hipEventRecord(B.ev[2], s0)) != hipSuccess) return e;
     for (int v = 0; v < LARGE_VARIANT; ++v) {
       const int cnt = P.list_off[v + 1] - P.list_off[v];
       if (cnt == 0) continue;
       hipStream_t sv = v == 0 ? s0 : L.st[v];
-      if (sv != s0) DCCHK(hipStreamWaitEvent(sv, B.ev[2], 0));
-      DCCHK(launch_dc_solve(v, b, P.list_off[v], cnt, dd.nc_ao, dd.v_el, it, dd.wnorm, dd.e_part, sv));
+      if (sv != s0 && (e = hipStreamWaitEvent(sv, B.ev[2], 0)) != hipSuccess) return e;
+      if ((e = launch_dc_solve(v, b, P.list_off[v], cnt, dd.nc_ao, dd.v_el, step, dd.wnorm, dd.e_part, sv)) !=
+          hipSuccess)
+        return e;
       if (sv != s0) {
-        DCCHK(hipEventRecord(B.ev[3 + v], sv));
-        DCCHK(hipStreamWaitEvent(s0, B.ev[3 + v], 0));
+        if ((e = hipEventRecord(B.ev[3 + v], sv)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(s0, B.ev[3 + v], 0)) != hipSuccess) return e;
       }
     }
-    DCCHK(launch_dc_fermi(b, dd, n_elec, d.fermi_width_ev, s0));
-    DCCHK(launch_dc_density(b, ns, dd.scal, d.fermi_width_ev, it, s0));
-    DCCHK(launch_dc_assemble(b, dd, d.mixing, s0));
-    DCCHK(launch_dc_finish(dd, s0));
-    DCCHK(hipMemcpyAsync(B.h_scal.p, dd.scal, 4 * 8, hipMemcpyDeviceToHost, s0));
+    if ((e = launch_dc_fermi(b, dd, n_elec, d.fermi_width_ev, s0)) != hipSuccess) return e;
+    if ((e = launch_dc_density(b, ns, dd.scal, d.fermi_width_ev, step, s0)) != hipSuccess) return e;
+    if ((e = launch_dc_assemble(b, dd, d.mixing, s0)) != hipSuccess) return e;
+    if ((e = launch_dc_finish(dd, s0)) != hipSuccess) return e;
+    return hipMemcpyAsync(B.h_scal.p, dd.scal, 4 * 8, hipMemcpyDeviceToHost, s0);
+  };
+  struct StepGraphs {
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    ~StepGraphs() {
+      for (hipGraphExec_t& g : exec)
+        if (g) (void)hipGraphExecDestroy(g);
+    }
+  } graphs;
+  const bool use_graph = ctx->dc_graph;
+  for (; it < d.max_scf;) {
+    if (use_graph && it >= 1) {
+      hipGraphExec_t& gx = graphs.exec[it & 1];
+      if (!gx) {
+        hipGraph_t g = nullptr;
+        DCCHK(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+        const hipError_t e_issue = issue_step(it);
+        const hipError_t e_end = hipStreamEndCapture(s0, &g);
+        if (e_issue != hipSuccess || e_end != hipSuccess) {
+          if (g) (void)hipGraphDestroy(g);
+          return refuse(PM7_ERR_DEVICE, std::string("capturing a divide-and-conquer step: ") +
+                                            hipGetErrorString(e_issue != hipSuccess ? e_issue : e_end));
+        }
+        const hipError_t e_inst = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        DCCHK(e_inst);
+      }
+      DCCHK(hipGraphLaunch(gx, s0));
+    } else {
+      DCCHK(issue_step(it));
+    }
     DCCHK(hipStreamSynchronize(s0));
     ++it;
     if (hs[3] != 0.0) {

@@ -154,3 +154,34 @@ def test_invalid_options_are_refused(eng, bad):
     with pytest.raises(pm7amd.Pm7Error) as e:
         run_dandc(mol, dandc=DandcOptions(**bad), engine=eng)
     assert e.value.kind == "InvalidInput"
+
+
+@pytest.mark.parametrize("mult", [1, 2])
+def test_graph_replayed_steps_equal_eager_steps(mult):
+    """Steps after the first replay two captured HIP graphs (one per eigenvector-buffer parity);
+    the same run with every launch issued eagerly (PM7_AMD_DC_GRAPH=0) gives the same bits. Run
+    twice on one context, so graphs captured in the first call are rebuilt, not reused stale."""
+    import os
+    atoms = W.flat_alkane(40)
+    if mult == 2:
+        atoms = W.remove_one_h(atoms)
+    mol = pm7amd.Molecule.from_symbols_angstrom(atoms, 0.0, mult)
+    opts = pm7amd.Pm7Options(multiplicity=mult)
+    dopt = DandcOptions(buffer=15.0, core_size=8)
+    say(f"dandc graph vs eager, multiplicity {mult}")
+    old = os.environ.get("PM7_AMD_DC_GRAPH")
+    os.environ["PM7_AMD_DC_GRAPH"] = "0"
+    try:
+        with pm7amd.Engine(0) as e0:
+            eager = run_dandc(mol, options=opts, dandc=dopt, engine=e0)
+    finally:
+        if old is None:
+            os.environ.pop("PM7_AMD_DC_GRAPH", None)
+        else:
+            os.environ["PM7_AMD_DC_GRAPH"] = old
+    with pm7amd.Engine(0) as e1:
+        runs = [run_dandc(mol, options=opts, dandc=dopt, engine=e1) for _ in range(2)]
+    for r in runs:
+        assert r.converged and r.iterations == eager.iterations and r.iterations > 3
+        assert r.electronic_ev == eager.electronic_ev and r.fermi_ev == eager.fermi_ev
+        assert np.array_equal(r.density.to_dense(), eager.density.to_dense())

@@ -388,6 +388,10 @@ def main() -> int:
     presmx, pipemx = run_pipelined("mixed")
     pipe64["bitwise_equal_to_one_at_a_time"] = identical(pres64, res64)
     pipemx["bitwise_equal_to_one_at_a_time"] = identical(presmx, resmx)
+    # CU-busy fraction with steps in flight: the molecules' own device time per step (each owns one
+    # CU while its SCF runs) over the CU-time of a step of the pipelined span.
+    for pr, pp in ((pres64, pipe64), (presmx, pipemx)):
+        pp["cu_busy_frac"] = round(sum(r.device_phase_ms[5] for r in pr) / (N_CU * pp["ms_per_step"]), 4)
     print(f"[bench] pipelined: fp64 {pipe64}, mixed {pipemx}", file=sys.stderr, flush=True)
 
     # Serving: the continuous-batching server's FP64 capacity is measured first (every request

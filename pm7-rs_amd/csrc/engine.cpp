@@ -271,9 +271,9 @@ struct pm7_ctx {
   // Divide and conquer replays each step after the first as a captured HIP graph (environment
   // PM7_AMD_DC_GRAPH=0: every step's launches issued one by one).
   bool dc_graph = true;
-  // Closed-shell response Fock as density-index contractions (PM7_AMD_RESPONSE_ROWS=0: one
+  // Closed-shell response Fock as density-index contractions (PM7_AMD_RESPONSE_CONTRACT=0: one
   // workgroup per density, the Fock-build kernel of the SCF).
-  bool response_rows = true;
+  bool response_contract = true;
   // Allocation-free entry point: its lane's buffers live in the device's static scratch array.
   Arena arena;
   Lane na_lane;
@@ -1713,7 +1713,7 @@ pm7_status pm7_ctx_create(int32_t device, const char* data_dir, pm7_ctx** out) {
     if (v >= 16 && v <= 30) ctx->eig_tol_exp = v;
   }
   if (const char* dg = std::getenv("PM7_AMD_DC_GRAPH")) ctx->dc_graph = std::atoi(dg) != 0;
-  if (const char* rr = std::getenv("PM7_AMD_RESPONSE_ROWS")) ctx->response_rows = std::atoi(rr) != 0;
+  if (const char* rc = std::getenv("PM7_AMD_RESPONSE_CONTRACT")) ctx->response_contract = std::atoi(rc) != 0;
   if (const char* eb = std::getenv("PM7_AMD_EIG_BLOCK")) {
     const int v = std::atoi(eb);
     ctx->eig_block_mask = v == 0 ? 0 : (v == 1 ? 0xF : (v & 0xF));
@@ -2121,23 +2121,20 @@ pm7_status response_fock_impl(pm7_ctx* ctx, const pm7_molecule_v1* mol, int32_t 
   Lane& L = ctx->lane;
   const int64_t slice = (int64_t)J.nao * J.ld;  // densities are stored with row stride ld, nao rows each
   const int64_t pp_stride = ((int64_t)J.n_pp + 31) / 32 * 32;
-  int nfa_max = 1;
-  bool sp = true;  // every atom with 1 or 4 orbitals
-  for (int z : J.z) {
-    const int nf = ctx->params.el[z].n_orb;
-    nfa_max = std::max(nfa_max, nf);
-    sp = sp && (nf == 1 || nf == 4);
-  }
-  // Closed shell: the density-index contractions (one integral read per group of densities) where
-  // the molecule fits them; the open-shell pairs and larger molecules take one workgroup per density.
-  const bool rows = !spin && ctx->response_rows && response_rows_fit(J.n_pp, nfa_max, J.nao, J.ld, (int)J.z.size());
+  bool spd = false;  // some atom with d orbitals
+  for (int z : J.z) spd = spd || ctx->params.el[z].n_orb > 4;
+  // Closed shell: the density-index contractions (one integral read per 256 densities) where the
+  // molecule fits them; the open-shell pairs and larger molecules take one workgroup per density.
+  const bool contract = !spin && ctx->response_contract && response_pairs_fit(J.n_pp);
+  const int dp = response_dp(n_dens);
   // alpha and beta slices back to back (plus the alpha + beta sums for the open-shell kernel)
-  const size_t mat_bytes = (size_t)n_sp * n_dens * (size_t)slice * sizeof(double);
+  const size_t mat_bytes = contract ? (size_t)J.nao * J.nao * dp * sizeof(double)
+                                : (size_t)n_sp * n_dens * (size_t)slice * sizeof(double);
   const size_t host_bytes = (size_t)n_dens * (size_t)J.nao * J.nao * sizeof(double);
-  const size_t pp_bytes = (size_t)n_dens * (size_t)(rows ? J.n_pp : pp_stride) * sizeof(double);
+  const size_t pp_bytes = (size_t)n_dens * (size_t)(contract ? J.n_pp : pp_stride) * sizeof(double);
   if (L.d_rx.ensure(mat_bytes + (spin ? mat_bytes / 2 : 0)) != hipSuccess || L.d_rg.ensure(mat_bytes) != hipSuccess ||
       L.d_rstage.ensure(n_sp * host_bytes) != hipSuccess || L.d_rpp.ensure(pp_bytes) != hipSuccess ||
-      (rows && (L.d_rwall.ensure((size_t)J.n_pp * J.n_pp * sizeof(double)) != hipSuccess ||
+      (contract && (L.d_rwall.ensure((size_t)J.n_pp * J.n_pp * sizeof(double)) != hipSuccess ||
                 L.d_rfd.ensure(pp_bytes) != hipSuccess))) {
     (void)hipGetLastError();
     return fail(ctx, PM7_ERR_INSUFFICIENT_MEMORY,
@@ -2147,13 +2144,14 @@ pm7_status response_fock_impl(pm7_ctx* ctx, const pm7_molecule_v1* mol, int32_t 
   double* xs = static_cast<double*>(L.d_rx.p);
   double* gs = static_cast<double*>(L.d_rg.p);
   // One contiguous copy each way over PCIe (a pitched copy from pageable memory goes row by row);
-  // the change of row stride nao <-> ld is a device-to-device copy.
+  // the change of layout (row stride ld for the per-density kernels, density-fastest for the
+  // contractions) is a device-to-device pass.
   double* stage = static_cast<double*>(L.d_rstage.p);
   const int64_t host_words = (int64_t)n_dens * J.nao * J.nao;
   for (int k = 0; k < n_sp; ++k)
     HIPCHK(hipMemcpyAsync(stage + k * host_words, in[k], host_bytes, hipMemcpyHostToDevice, L.st[0]));
-  if (rows)
-    HIPCHK(launch_response_relayout(stage, xs, J.nao, J.ld, n_dens, true, L.st[0]));
+  if (contract)
+    HIPCHK(launch_response_relayout(stage, xs, J.nao, n_dens, dp, true, L.st[0]));
   else
     HIPCHK(hipMemcpy2DAsync(xs, J.ld * sizeof(double), stage, J.nao * sizeof(double), J.nao * sizeof(double),
                             (size_t)n_sp * n_dens * J.nao, hipMemcpyDeviceToDevice, L.st[0]));
@@ -2161,15 +2159,15 @@ pm7_status response_fock_impl(pm7_ctx* ctx, const pm7_molecule_v1* mol, int32_t 
   if (spin)
     HIPCHK(launch_fock_response_spin(P.b, 0, xs, xs + 2 * n_dens * slice, gs, static_cast<double*>(L.d_rpp.p),
                                      pp_stride, n_dens, L.st[0]));
-  else if (rows)
-    HIPCHK(launch_response_rows(P.b, 0, J.n_pp, (int)J.z.size(), J.nao, J.ld, nfa_max, sp, xs, gs, n_dens,
-                                static_cast<double*>(L.d_rwall.p), static_cast<double*>(L.d_rpp.p),
-                                static_cast<double*>(L.d_rfd.p), L.st[0]));
+  else if (contract)
+    HIPCHK(launch_response_pairs(P.b, 0, J.n_pp, (int)J.z.size(), spd, xs, gs, n_dens, dp,
+                                 static_cast<double*>(L.d_rwall.p), static_cast<double*>(L.d_rpp.p),
+                                 static_cast<double*>(L.d_rfd.p), L.st[0]));
   else
     HIPCHK(launch_fock_response(P.b, 0, xs, gs, static_cast<double*>(L.d_rpp.p), pp_stride, n_dens, L.st[0]));
   HIPCHK(hipEventRecord(L.ev[1], L.st[0]));
-  if (rows)
-    HIPCHK(launch_response_relayout(gs, stage, J.nao, J.ld, n_dens, false, L.st[0]));
+  if (contract)
+    HIPCHK(launch_response_relayout(gs, stage, J.nao, n_dens, dp, false, L.st[0]));
   else
     HIPCHK(hipMemcpy2DAsync(stage, J.nao * sizeof(double), gs, J.ld * sizeof(double), J.nao * sizeof(double),
                             (size_t)n_sp * n_dens * J.nao, hipMemcpyDeviceToDevice, L.st[0]));

@@ -75,30 +75,18 @@ hipError_t launch_gradient(const BatchDev& b, int64_t n_pairs, const int2* items
 hipError_t launch_fock_once(const BatchDev& b, int mol, int pt_slot, int ps_slot, int f_slot, hipStream_t s);
 // Closed-shell response Fock of n_dens densities of molecule `mol` as contractions over the
 // density index (response.hip): wall (n_pp^2 doubles), pp and fd (n_pp x n_dens each) are scratch;
-// X and G in the row layout, element (d, i, j) at (i * n_dens + d) * ld + j.
-// sp: every atom has 1 or 4 orbitals (the specialised row kernels).
-hipError_t launch_response_rows(const BatchDev& b, int mol, int n_pp, int n_atoms, int nao, int ld, int nfa_max,
-                                bool sp, const double* xs, double* gs, int n_dens, double* wall, double* pp,
-                                double* fd, hipStream_t s);
-// Densities between the caller's layout ((d * nao + i) * nao + j) and the row layout (to_rows:
-// caller's to row layout).
-hipError_t launch_response_relayout(const double* src, double* dst, int nao, int ld, int n_dens, bool to_rows,
+// X and G density-fastest, element (d, i, j) at (i * nao + j) * dp + d. spd: some atom has d
+// orbitals (else every atom has 1 or 4).
+hipError_t launch_response_pairs(const BatchDev& b, int mol, int n_pp, int n_atoms, bool spd, const double* xs,
+                                 double* gs, int n_dens, int dp, double* wall, double* pp, double* fd, hipStream_t s);
+// Densities between the caller's layout ((d * nao + i) * nao + j) and the density-fastest one
+// (to_pairs: caller's to density-fastest).
+hipError_t launch_response_relayout(const double* src, double* dst, int nao, int n_dens, int dp, bool to_pairs,
                                     hipStream_t s);
-// The row kernel's LDS (64 KB = 8192 words): the X rows of a group of densities (nf x ld words
-// each), per atom an integral offset and a packed int, and a trash word per thread.
-constexpr int RESP_DC = 8;  // densities per group of the atoms with several orbitals (16 at most)
-constexpr int RESP_LDS_WORDS = 8192;
-constexpr int RESP_TRASH = 256;  // one word per thread for the results that fall outside a block
-inline int response_table_words(int n_atoms) { return n_atoms + (n_atoms + 1) / 2; }
-inline int response_group(int nf, int ld, int n_atoms, int cap) {
-  const int g = (RESP_LDS_WORDS - response_table_words(n_atoms) - RESP_TRASH) / (nf * ld);
-  return g < 0 ? 0 : (g > cap ? cap : g);
-}
-// The contraction path's limits: the n_pp^2 integral matrix (<= 512 MB), AO offsets below 2^16 in
-// the row kernel's packed ints and one density's rows of an atom in its LDS.
-inline bool response_rows_fit(int n_pp, int nfa_max, int nao, int ld, int n_atoms) {
-  return n_pp <= 8192 && nao < 65536 && response_group(nfa_max, ld, n_atoms, RESP_DC) >= 1;
-}
+// Row stride of the density-fastest layout: whole 128-byte lines per (i, j).
+inline int response_dp(int n_dens) { return (n_dens + 15) / 16 * 16; }
+// The contraction path's limit: the n_pp^2 integral matrix (<= 512 MB).
+inline bool response_pairs_fit(int n_pp) { return n_pp <= 8192; }
 // G_k = J(X_k) - K(X_k / 2) for n_dens densities of molecule `mol` (slices of nao*ld at xs / gs;
 // pp_stride doubles of packed-density scratch per density at pps).
 // 512 threads per density: 0.41 ms for 294 densities of a 194-AO molecule (256: 0.58, 1024: 0.55),

@@ -34,8 +34,10 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 SCF_KERNELS = ["pm7::k_scf<256, 64, true, true>", "pm7::k_scf<512, 120, false, true>",
                "pm7::k_scf<512, 196, true, false>", "pm7::k_scf<512, 1024, false, false>"]
 TRAFFIC_FILES = {"fp64": ROOT / "profiles" / "hbm_traffic_organic256_fp64.json",
-                 "mixed": ROOT / "profiles" / "hbm_traffic_organic256_mixed.json"}
-RESPONSE_NT = int(os.environ.get("PM7_AMD_RESPONSE_NT", "512"))  # threads per density of the response kernel
+                 "mixed": ROOT / "profiles" / "hbm_traffic_organic256_mixed.json",
+                 "response": ROOT / "profiles" / "hbm_traffic_response_c32h66.json"}
+# The closed-shell response path of an s/p molecule, in launch order.
+RESPONSE_KERNELS = ["pm7::k_resp_wall", "pm7::k_resp_pack", "pm7::k_resp_gemm", "pm7::k_resp_pairs<false>"]
 
 
 def measured_traffic(kernel, mode="fp64"):
@@ -154,6 +156,7 @@ def main() -> int:
     ap.add_argument("--large-waters", type=int, default=500,
                     help="water cluster of the large-system run (0: skip); 500 waters = 3000 AOs")
     ap.add_argument("--large-only", action="store_true", help="only the large-system run")
+    ap.add_argument("--response-only", action="store_true", help="only the response-Fock run")
     ap.add_argument("--d-batch", type=int, default=64,
                     help="S/P/Cl molecules (MNDO/d pairs, <= 196 AOs) batched in FP64 and mixed (0: skip)")
     ap.add_argument("--gradient", type=int, default=1, help="time batched nuclear gradients (0: skip)")
@@ -270,6 +273,74 @@ def main() -> int:
             print(f"[bench] large {precision}: {rec['wall_s']} s, {rec['iterations']} iterations",
                   file=sys.stderr, flush=True)
         return out
+
+    def response_fock(eng):
+        """Response Fock (the CPHF kernel of the Hessian): 3N symmetric densities of the C32 alkane
+        (98 atoms, 194 AOs, the size of the 102-atom Hessian molecule pm7-rs profiles) against its
+        one integral set, one call. Algorithmic HBM bytes per call: the pair integrals and H once,
+        each density read and its G written once. kernel_ms spans the response kernels (HIP events
+        on their stream); api_ms_per_call adds the integrals, the copies over PCIe and the change of
+        layout."""
+        ratoms = W.alkane_chain(32)
+        rmol = pm7amd.Molecule.from_symbols_angstrom(ratoms)
+        rn = 4 * 32 + 66
+        rk = 3 * len(ratoms)
+        rng = np.random.default_rng(17)
+        ra = rng.normal(size=(rk, rn, rn)) * 0.05
+        rxs = ra + ra.transpose(0, 2, 1)
+        eng.response_fock(rmol, rxs)
+        reps = max(2, min(args.steps, 5))
+        kms, t = [], time.perf_counter()
+        for _ in range(reps):
+            eng.response_fock(rmol, rxs)
+            kms.append(eng.last_timings_ms()[1])
+        dt = (time.perf_counter() - t) / reps
+        kernel_ms = float(np.mean(kms))
+        n_orb = [4 if a[0] != "H" else 1 for a in ratoms]
+        npair = {1: 1, 4: 10}
+        w_words = sum(npair[n_orb[u]] * npair[n_orb[v]] for u in range(len(n_orb)) for v in range(u + 1, len(n_orb)))
+        alg_bytes = 8.0 * (w_words + rn * rn + 2 * rk * rn * rn)
+        traffic = [measured_traffic(k, "response") for k in RESPONSE_KERNELS]
+        out = {"workload": f"C32H66 alkane (98 atoms, {rn} AOs), {rk} symmetric response densities (3N), one call",
+               "densities_per_s": round(rk / (kernel_ms * 1e-3), 1), "kernel_ms": round(kernel_ms, 4),
+               "api_ms_per_call": round(dt * 1e3, 3),
+               "kernels": RESPONSE_KERNELS,
+               "roofline": {"bound": "hbm", "achieved": round(alg_bytes / (kernel_ms * 1e-3) / 1e9, 1),
+                            "peak": 8000.0, "unit": "GB/s",
+                            "frac": round(alg_bytes / (kernel_ms * 1e-3) / 8.0e12, 5),
+                            "traffic": None if None in traffic else int(sum(traffic)),
+                            "algorithmic_bytes": int(alg_bytes),
+                            "integral_words": int(w_words)}}
+        if not args.no_cpu:
+            # the oracle's Fock build minus H, one thread, on the first 24 densities
+            sys.path.insert(0, str(ROOT / "tests"))
+            import oracle_binding as ob
+
+            ob.hcore(rmol.z, rmol.xyz)
+            t = time.perf_counter()
+            for _ in range(8):
+                ob.hcore(rmol.z, rmol.xyz)
+            t_int = (time.perf_counter() - t) / 8
+            h = ob.hcore(rmol.z, rmol.xyz)
+            t = time.perf_counter()
+            for x in rxs[:24]:
+                ob.fock(rmol.z, rmol.xyz, x, 0.5 * x) - h
+            per = (time.perf_counter() - t) / 24
+            # each oracle call also rebuilds the pair integrals (one H_core build, reported beside
+            # the rate); the reference keeps them between calls
+            out["cpu_baseline"] = {"densities_per_s": round(1.0 / per, 1), "cores": 1, "kind": "port",
+                                   "ms_per_density": round(per * 1e3, 3),
+                                   "integral_build_ms": round(t_int * 1e3, 3),
+                                   "sample": "24 of the densities through the CPU oracle's Fock build minus H; "
+                                             "each call includes one integral build (integral_build_ms)"}
+        print(f"[bench] response fock: {out}", file=sys.stderr, flush=True)
+        return out
+
+    if args.response_only:
+        eng = pm7amd.Engine(0)
+        print(json.dumps({"response_fock": response_fock(eng)}), flush=True)
+        eng.close()
+        return 0
 
     if args.large_only:
         eng = pm7amd.Engine(0)
@@ -507,62 +578,7 @@ def main() -> int:
                 "max_abs_gradient_ev_bohr": round(max(float(np.max(np.abs(g))) for g in g_vals), 6)}
         print(f"[bench] gradient: {gradient}", file=sys.stderr, flush=True)
 
-    # Response Fock (the CPHF kernel of the Hessian): 3N symmetric densities of the C32 alkane
-    # (98 atoms, 194 AOs, the size of the 102-atom Hessian molecule pm7-rs profiles) against its one
-    # integral set, one launch. Algorithmic HBM bytes per launch: the pair integrals and H once,
-    # each density read and its G written once.
-    response = None
-    if args.response:
-        ratoms = W.alkane_chain(32)
-        rmol = pm7amd.Molecule.from_symbols_angstrom(ratoms)
-        rn = 4 * 32 + 66
-        rk = 3 * len(ratoms)
-        rng = np.random.default_rng(17)
-        ra = rng.normal(size=(rk, rn, rn)) * 0.05
-        rxs = ra + ra.transpose(0, 2, 1)
-        eng.response_fock(rmol, rxs)
-        reps = max(2, min(args.steps, 5))
-        kms, t = [], time.perf_counter()
-        for _ in range(reps):
-            eng.response_fock(rmol, rxs)
-            kms.append(eng.last_timings_ms()[1])
-        dt = (time.perf_counter() - t) / reps
-        kernel_ms = float(np.mean(kms))
-        n_orb = [4 if a[0] != "H" else 1 for a in ratoms]
-        npair = {1: 1, 4: 10}
-        w_words = sum(npair[n_orb[u]] * npair[n_orb[v]] for u in range(len(n_orb)) for v in range(u + 1, len(n_orb)))
-        alg_bytes = 8.0 * (w_words + rn * rn + 2 * rk * rn * rn)
-        response = {"workload": f"C32H66 alkane (98 atoms, {rn} AOs), {rk} symmetric response densities (3N), one call",
-                    "densities_per_s": round(rk / (kernel_ms * 1e-3), 1), "kernel_ms": round(kernel_ms, 4),
-                    "api_ms_per_call": round(dt * 1e3, 3),
-                    "roofline": {"bound": "hbm", "achieved": round(alg_bytes / (kernel_ms * 1e-3) / 1e9, 1),
-                                 "peak": 8000.0, "unit": "GB/s",
-                                 "frac": round(alg_bytes / (kernel_ms * 1e-3) / 8.0e12, 5),
-                                 "traffic": measured_traffic(f"pm7::k_fock_response<{RESPONSE_NT}>", "fp64"),
-                                 "algorithmic_bytes": int(alg_bytes),
-                                 "integral_words": int(w_words)}}
-        if not args.no_cpu:
-            # the oracle's Fock build minus H, one thread, on the first 24 densities
-            sys.path.insert(0, str(ROOT / "tests"))
-            import oracle_binding as ob
-
-            h = ob.hcore(rmol.z, rmol.xyz)
-            t = time.perf_counter()
-            for _ in range(8):
-                ob.hcore(rmol.z, rmol.xyz)
-            t_int = (time.perf_counter() - t) / 8
-            t = time.perf_counter()
-            for x in rxs[:24]:
-                ob.fock(rmol.z, rmol.xyz, x, 0.5 * x) - h
-            per = (time.perf_counter() - t) / 24
-            # each oracle call also rebuilds the pair integrals (one H_core build, reported beside
-            # the rate); the reference keeps them between calls
-            response["cpu_baseline"] = {"densities_per_s": round(1.0 / per, 1), "cores": 1, "kind": "port",
-                                        "ms_per_density": round(per * 1e3, 3),
-                                        "integral_build_ms": round(t_int * 1e3, 3),
-                                        "sample": "24 of the densities through the CPU oracle's Fock build minus H; "
-                                                  "each call includes one integral build (integral_build_ms)"}
-        print(f"[bench] response fock: {response}", file=sys.stderr, flush=True)
+    response = response_fock(eng) if args.response else None
 
     # Divide and conquer: the reference's scaling record (docs/divide_and_conquer.md:85-92: flat
     # alkanes, buffer 15 Bohr, core_size 8, D&C p_tol 1e-7; exact SCF e_tol 1e-10, p_tol 1e-8;

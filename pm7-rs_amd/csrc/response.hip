@@ -98,16 +98,18 @@ __global__ void __launch_bounds__(256) k_resp_pack(int mi, const MolDev* __restr
   pp[idx] = xs[((int64_t)(o + mu) * M.nao + o + nu) * dp + d] * (mu == nu ? 1.0 : 2.0);
 }
 
-// Fd (n x D) = Wall (n x n) PP (n x D) on the matrix cores: each workgroup's four wavefronts take
-// one 16 x 16 output tile over four quarters of k, summed (w0 + w1) + (w2 + w3) through LDS. Each
-// element's sum runs over k in the same order whatever D is.
-__global__ void __launch_bounds__(256) k_resp_gemm(int n, int n_dens, const double* __restrict__ wall,
-                                                   const double* __restrict__ pp, double* __restrict__ fd) {
-  __shared__ dbl4 part[3][64];
+// Fd (n x D) = Wall (n x n) PP (n x D) on the matrix cores: each workgroup's eight wavefronts take
+// one 16 x 16 output tile over eight slices of k, summed in slice order through LDS. Each element's
+// sum runs over k in the same order whatever D is.
+constexpr int RESP_KSPLIT = 8;
+__global__ void __launch_bounds__(64 * RESP_KSPLIT) k_resp_gemm(int n, int n_dens, const double* __restrict__ wall,
+                                                                const double* __restrict__ pp,
+                                                                double* __restrict__ fd) {
+  __shared__ dbl4 part[RESP_KSPLIT - 1][64];
   const int td = (n_dens + 15) >> 4;
   const int I = blockIdx.x / td, J = blockIdx.x - I * td;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
-  const int kq = ((n + 15) >> 4) << 2;  // a quarter of k, a multiple of 4
+  const int kq = ((n + 4 * RESP_KSPLIT - 1) / (4 * RESP_KSPLIT)) * 4;  // a slice of k, a multiple of 4
   const int kb = wv * kq, kl = min(kq, n - kb);
   auto fa = [&](int i, int k) { return (i < n && k < kl) ? wall[(int64_t)i * n + kb + k] : 0.0; };
   auto fb = [&](int k, int j) { return (k < kl && j < n_dens) ? pp[(int64_t)(kb + k) * n_dens + j] : 0.0; };
@@ -116,11 +118,16 @@ __global__ void __launch_bounds__(256) k_resp_gemm(int n, int n_dens, const doub
   if (wv > 0) part[wv - 1][lane] = acc;
   __syncthreads();
   if (wv != 0) return;
-  const dbl4 p1 = part[0][lane], p2 = part[1][lane], p3 = part[2][lane];
+#pragma unroll 1
+  for (int p = 0; p < RESP_KSPLIT - 1; ++p) {
+    const dbl4 v = part[p][lane];
+#pragma unroll
+    for (int t4 = 0; t4 < 4; ++t4) acc[t4] += v[t4];
+  }
 #pragma unroll
   for (int t4 = 0; t4 < 4; ++t4) {
     const int i = 16 * I + tile_row_f64(q, t4), j = 16 * J + r;
-    if (i < n && j < n_dens) fd[(int64_t)i * n_dens + j] = (acc[t4] + p1[t4]) + (p2[t4] + p3[t4]);
+    if (i < n && j < n_dens) fd[(int64_t)i * n_dens + j] = acc[t4];
   }
 }
 
@@ -265,7 +272,7 @@ hipError_t launch_response_pairs(const BatchDev& b, int mol, int n_pp, int n_ato
   hipLaunchKernelGGL(k_resp_pack, dim3((unsigned)((npd + 255) / 256)), dim3(256), 0, s, mol, b.mols, b.atom_ao,
                      b.atom_pp, xs, n_dens, dp, pp);
   const int64_t tiles = (int64_t)((n_pp + 15) / 16) * ((n_dens + 15) / 16);
-  hipLaunchKernelGGL(k_resp_gemm, dim3((unsigned)tiles), dim3(256), 0, s, n_pp, n_dens, wall, pp, fd);
+  hipLaunchKernelGGL(k_resp_gemm, dim3((unsigned)tiles), dim3(64 * RESP_KSPLIT), 0, s, n_pp, n_dens, wall, pp, fd);
   const int64_t items = (int64_t)n_atoms * (n_atoms - 1) / 2 + n_atoms;
   const dim3 grid((unsigned)items, (unsigned)((n_dens + 255) / 256));
   if (spd)

@@ -22,13 +22,16 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <algorithm>
 #include <condition_variable>
 #include <cstddef>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <mutex>
 #include <string>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "diis.h"
@@ -474,6 +477,190 @@ __global__ void k_lg_init(LargeCtl* __restrict__ ctl, double shift_floor, int lo
   ctl->guard.n = 0;
 }
 
+// ---- s/p closed-shell Fock from per-atom integral streams ---------------------------------------
+//
+// For molecules whose atoms all carry 1 or 4 orbitals, each atom a gets its own stream of the pair
+// integral blocks (a, b) over every partner b, built once per SCF: partners are grouped by class
+// (4-orbital atoms, then 1-orbital atoms, each list in atom order and padded to a multiple of 64
+// with -1) and a chunk of 64 partners is stored element-major, element k of lane t at
+// [chunk][k][t], each block in the pair's own order w[e_F][kl_S] (F the pair's first atom), zero
+// for a's own slot and the padding. One workgroup per atom then reads its stream once, coalesced,
+// and writes its own rows of F: the Coulomb sum of the diagonal block over all partners (lanes,
+// then a fixed-order reduction) and each partner's exchange block, accumulated element by element
+// in the pair's own order and oriented by F and S, so the blocks (a, b) and (b, a) from two
+// workgroups are the same sums bit for bit and F stays exactly symmetric.
+
+constexpr int LGS_CHUNK = 64;
+constexpr int LGS_THREADS = 256;  // threads of a k_lgs_fock workgroup
+
+// The larger orbital index of packed pair e (pack_idx's h), at compile time.
+constexpr int ce_hi(int e) {
+  return e >= 36 ? 8 : e >= 28 ? 7 : e >= 21 ? 6 : e >= 15 ? 5 : e >= 10 ? 4 : e >= 6 ? 3 : e >= 3 ? 2 : e >= 1 ? 1 : 0;
+}
+
+// f(std::integral_constant<int, I>) for I in [B, E), each I a compile-time constant.
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// Orbital count of every atom of molecule mi, in molecule order.
+__global__ void k_lgs_norb(int mi, BatchDev b, int* __restrict__ norb) {
+  const MolDev M = b.mols[mi];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M.n_atoms; i += gridDim.x * blockDim.x)
+    norb[i] = b.elem[b.atom_z[M.atom0 + i]].n_orb;
+}
+
+// Workgroups (x, atom a on y): a's stream (class lists cls[0, pA) and cls[pA, pA + pB), stream of
+// atom a at off[a]).
+template <typename T>
+__global__ void __launch_bounds__(256) k_lgs_build(int mi, BatchDev b, const int* __restrict__ cls, int pA, int pB,
+                                                   const int64_t* __restrict__ off, T* __restrict__ st) {
+  const MolDev M = b.mols[mi];
+  const int la = blockIdx.y, na = M.n_atoms;
+  const int npa = npair_of(b.elem[b.atom_z[M.atom0 + la]].n_orb);
+  const int64_t segA = (int64_t)pA * npa * 10, len = segA + (int64_t)pB * npa;
+  T* dst = st + off[la];
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < len; idx += (int64_t)gridDim.x * 256) {
+    const bool inA = idx < segA;
+    const int64_t loc = inA ? idx : idx - segA;
+    const int K = npa * (inA ? 10 : 1);
+    const int t = (int)(loc & (LGS_CHUNK - 1));
+    const int64_t kk = loc / LGS_CHUNK;
+    const int chunk = (int)(kk / K), k = (int)(kk - (int64_t)chunk * K);
+    const int lb = cls[(inA ? 0 : pA) + chunk * LGS_CHUNK + t];
+    double v = 0.0;
+    if (lb >= 0 && lb != la) v = b.w[b.pairs[M.pair0 + tri_index(min(la, lb), max(la, lb), na)].w_off + k];
+    dst[idx] = (T)v;
+  }
+}
+
+// One chunk of partners (class with NFB orbitals) of a workgroup whose atom has NFA orbitals: lane
+// `lane` takes partner cls[lane]; J gathers the Coulomb sums of a's packed pairs.
+template <typename T, int NFA, int NFB>
+__device__ __forceinline__ void lgs_chunk(const MolDev& M, const BatchDev& b, int la, int oa,
+                                          const T* __restrict__ sp, const int* __restrict__ cls, int lane,
+                                          const double* __restrict__ P, double xa, double* __restrict__ F,
+                                          T (&J)[10]) {
+  constexpr int NF = NFA > NFB ? NFA : NFB, NS = NFA > NFB ? NFB : NFA;
+  constexpr int NPF = NF * (NF + 1) / 2, NPS = NS * (NS + 1) / 2;
+  const int ld = M.ld;
+  const int lb = cls[lane];
+  const bool live = lb >= 0 && lb != la;
+  const int bb = M.atom0 + (live ? lb : la);  // a valid atom for the loads of idle lanes
+  const int ob = b.atom_ao[bb];
+  const double* ppb = b.slab + M.pp0 + b.atom_pp[bb];
+  // a is the pair's first atom: the larger orbital count, on a tie the lower index
+  const bool af = NFA > NFB ? true : (NFA < NFB ? false : la < lb);
+  const int oF = af ? oa : ob, oS = af ? ob : oa;
+  T pfs[NF][NS], acc[NF][NS];
+#pragma unroll
+  for (int p = 0; p < NF; ++p)
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      pfs[p][q] = (T)P[(int64_t)(oF + p) * ld + oS + q];
+      acc[p][q] = (T)0;
+    }
+  T ppv[NPF > NPS ? NPF : NPS];
+  const int npb = NFB == 4 ? 10 : 1;
+#pragma unroll
+  for (int k = 0; k < (NPF > NPS ? NPF : NPS); ++k) ppv[k] = k < npb ? (T)ppb[k] : (T)0;
+  // Every index below is a compile-time constant (static_for), so the block lives in registers.
+  static_for<0, NPF>([&](auto ec) {
+    constexpr int e = decltype(ec)::value, mu = ce_hi(e), p = e - mu * (mu + 1) / 2;
+    static_for<0, NPS>([&](auto kc) {
+      constexpr int kl = decltype(kc)::value, l = ce_hi(kl), q = kl - l * (l + 1) / 2;
+      const T w = sp[(e * NPS + kl) * LGS_CHUNK];
+      // Coulomb: a's packed pair is e when a is F, kl when a is S
+      if constexpr (NFA > NFB) {
+        J[e] += w * ppv[kl];
+      } else if constexpr (NFA < NFB) {
+        J[kl] += w * ppv[e];
+      } else {
+        const T c = w * (af ? ppv[kl] : ppv[e]);
+        J[e] += af ? c : (T)0;
+        J[kl] += af ? (T)0 : c;
+      }
+      // exchange: every (mu' nu') x (la' sg') term this packed element stands for
+      acc[mu][l] += pfs[p][q] * w;
+      if constexpr (mu != p) acc[p][l] += pfs[mu][q] * w;
+      if constexpr (l != q) acc[mu][q] += pfs[p][l] * w;
+      if constexpr (mu != p && l != q) acc[p][q] += pfs[mu][l] * w;
+    });
+  });
+  if (!live) return;
+  const double* H = b.slab + M.mat0 + (int64_t)S_H * ld * ld;
+#pragma unroll
+  for (int p = 0; p < NF; ++p)
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      const int64_t fs = (int64_t)(oF + p) * ld + oS + q;  // the element's place in F's rows
+      const double v = H[fs] - (double)((T)xa * acc[p][q]);
+      F[af ? fs : (int64_t)(oS + q) * ld + oF + p] = v;
+    }
+}
+
+template <typename T, int NFA>
+__device__ __forceinline__ void lgs_atom(const MolDev& M, const BatchDev& b, int la, int oa, const T* __restrict__ sa,
+                                         const int* __restrict__ cls, int pA, int pB, int wv, int lane,
+                                         const double* __restrict__ P, double xa, double* __restrict__ F, T (&J)[10]) {
+  constexpr int NPA = NFA == 4 ? 10 : 1;
+  constexpr int NW = LGS_THREADS / 64;
+  for (int c = wv; c < pA / LGS_CHUNK; c += NW)
+    lgs_chunk<T, NFA, 4>(M, b, la, oa, sa + (int64_t)c * NPA * 10 * LGS_CHUNK + lane, cls + c * LGS_CHUNK, lane, P,
+                         xa, F, J);
+  const T* sb = sa + (int64_t)pA * NPA * 10;
+  for (int c = wv; c < pB / LGS_CHUNK; c += NW)
+    lgs_chunk<T, NFA, 1>(M, b, la, oa, sb + (int64_t)c * NPA * LGS_CHUNK + lane, cls + pA + c * LGS_CHUNK, lane, P,
+                         xa, F, J);
+}
+
+// Workgroup per atom a: its rows of F = H + J(P) - xa K(P) (closed shell), the Coulomb sums from
+// the packed density of every atom (k_lg_pack<T>), one-centre terms as fock_diag_item.
+template <typename T>
+__global__ void __launch_bounds__(LGS_THREADS) k_lgs_fock(int mi, BatchDev b, const T* __restrict__ st,
+                                                  const int* __restrict__ cls, int pA, int pB,
+                                                  const int64_t* __restrict__ off, const double* __restrict__ P,
+                                                  double xa, double* __restrict__ F) {
+  __shared__ T red[10][LGS_THREADS];
+  const MolDev M = b.mols[mi];
+  const int la = blockIdx.x, a = M.atom0 + la, ld = M.ld;
+  const ElemDev ea = b.elem[b.atom_z[a]];
+  const int nfa = ea.n_orb, oa = b.atom_ao[a];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  T J[10];
+#pragma unroll
+  for (int e = 0; e < 10; ++e) J[e] = (T)0;
+  const T* sa = st + off[la];
+  if (nfa == 4) lgs_atom<T, 4>(M, b, la, oa, sa, cls, pA, pB, wv, lane, P, xa, F, J);
+  else lgs_atom<T, 1>(M, b, la, oa, sa, cls, pA, pB, wv, lane, P, xa, F, J);
+#pragma unroll
+  for (int e = 0; e < 10; ++e) red[e][threadIdx.x] = J[e];  // every thread, in thread order
+  __syncthreads();
+  const int npa = npair_of(nfa);
+  if ((int)threadIdx.x >= npa) return;
+  const int e = threadIdx.x;
+  T acc = (T)0;
+  for (int i = 0; i < LGS_THREADS; ++i) acc += red[e][i];
+  int mu, nu;
+  unpack_e(e, mu, nu);
+  T ja = (T)0, ka = (T)0;
+  for (int l = 0; l < nfa; ++l)
+    for (int s2 = 0; s2 < nfa; ++s2) {
+      const int64_t ix = (int64_t)(oa + l) * ld + oa + s2;
+      ja += (T)P[ix] * (T)oc_int(mu, nu, l, s2, ea);
+      ka += (T)P[ix] * (T)oc_int(mu, l, nu, s2, ea);
+    }
+  const double* H = b.slab + M.mat0 + (int64_t)S_H * ld * ld;
+  const int64_t i1 = (int64_t)(oa + mu) * ld + oa + nu, i2 = (int64_t)(oa + nu) * ld + oa + mu;
+  const double f = H[i1] + (double)(acc + ja - (T)xa * ka);
+  F[i1] = f;
+  F[i2] = f;
+}
+
 // SAD guess of the large path (pm7-rs src/scf.rs:1050-1114): diagonal tore/4 per sp AO, minus the
 // charge spread (sad_yy), scaled; pads and off-diagonal stay zero (the slab is zeroed).
 __global__ void k_lg_sad(int mi, BatchDev b, double* __restrict__ P, double scale) {
@@ -490,6 +677,7 @@ __global__ void k_lg_sad(int mi, BatchDev b, double* __restrict__ P, double scal
 struct LargeWork {
   rocblas_handle blas = nullptr;
   Buf ctl, part, dots, info, f32a, f32b;
+  Buf lgs_st, lgs_st32, lgs_cls, lgs_off, lgs_norb;  // per-atom integral streams (s/p closed shell)
   std::vector<hipEvent_t> ev;
 };
 
@@ -759,6 +947,97 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
   const bool mixed_ok = mixed;  // the FP32 range guard of the megakernel is applied below
   hipLaunchKernelGGL(k_lg_init, dim3(1), dim3(64), 0, s, ctl, M.shift_floor, mixed_ok ? 1 : 0);
   std::vector<float> phase(5, 0.0f);
+  // Closed-shell s/p molecules: the per-atom integral streams (k_lgs_fock), built once here, their
+  // device time counted in the Fock phase. Other molecules, and PM7_AMD_LARGE_STREAMS=0, keep the
+  // element and pair kernels (k_lg_fock_diag, k_lg_fock_pairs).
+  int lgs_pA = 0, lgs_pB = 0;
+  bool lgs_on = false;
+  auto lgs_setup = [&]() -> hipError_t {
+    const char* env = std::getenv("PM7_AMD_LARGE_STREAMS");  // read per SCF
+    const bool enabled = !(env && std::atoi(env) == 0);
+    if (!enabled || M.dpath || M.uhf) return hipSuccess;
+    const int na = M.n_atoms;
+    hipError_t e;
+    if ((e = lw->lgs_norb.ensure((size_t)na * sizeof(int))) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_lgs_norb, dim3(grid_for(na)), dim3(256), 0, s, mi, b, static_cast<int*>(lw->lgs_norb.p));
+    std::vector<int> norb(na);
+    if ((e = hipMemcpyAsync(norb.data(), lw->lgs_norb.p, (size_t)na * sizeof(int), hipMemcpyDeviceToHost, s)) !=
+            hipSuccess ||
+        (e = hipStreamSynchronize(s)) != hipSuccess)
+      return e;
+    std::vector<int> heavy, light;
+    for (int i = 0; i < na; ++i) {
+      if (norb[i] == 4) heavy.push_back(i);
+      else if (norb[i] == 1) light.push_back(i);
+      else return hipSuccess;  // an atom with another shell count: the element and pair kernels
+    }
+    const int pA = (int)((heavy.size() + LGS_CHUNK - 1) / LGS_CHUNK * LGS_CHUNK);
+    const int pB = (int)((light.size() + LGS_CHUNK - 1) / LGS_CHUNK * LGS_CHUNK);
+    std::vector<int> cls((size_t)pA + pB, -1);
+    std::copy(heavy.begin(), heavy.end(), cls.begin());
+    std::copy(light.begin(), light.end(), cls.begin() + pA);
+    std::vector<int64_t> off(na);
+    int64_t total = 0;
+    for (int i = 0; i < na; ++i) {
+      off[i] = total;
+      const int64_t npa = norb[i] == 4 ? 10 : 1;
+      total += npa * ((int64_t)pA * 10 + pB);
+    }
+    if (lw->lgs_st.ensure((size_t)total * sizeof(double)) != hipSuccess ||
+        (mixed && lw->lgs_st32.ensure((size_t)total * sizeof(float)) != hipSuccess) ||
+        lw->lgs_cls.ensure(cls.size() * sizeof(int)) != hipSuccess ||
+        lw->lgs_off.ensure(off.size() * sizeof(int64_t)) != hipSuccess) {
+      (void)hipGetLastError();  // no room for the streams: the element and pair kernels
+      return hipSuccess;
+    }
+    if ((e = hipMemcpyAsync(lw->lgs_cls.p, cls.data(), cls.size() * sizeof(int), hipMemcpyHostToDevice, s)) !=
+            hipSuccess ||
+        (e = hipMemcpyAsync(lw->lgs_off.p, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice, s)) !=
+            hipSuccess ||
+        (e = hipEventRecord(lw->ev[0], s)) != hipSuccess)
+      return e;
+    const int* dcls = static_cast<const int*>(lw->lgs_cls.p);
+    const int64_t* doff = static_cast<const int64_t*>(lw->lgs_off.p);
+    hipLaunchKernelGGL(k_lgs_build<double>, dim3(64, na), dim3(256), 0, s, mi, b, dcls, pA, pB, doff,
+                       static_cast<double*>(lw->lgs_st.p));
+    if (mixed)
+      hipLaunchKernelGGL(k_lgs_build<float>, dim3(64, na), dim3(256), 0, s, mi, b, dcls, pA, pB, doff,
+                         static_cast<float*>(lw->lgs_st32.p));
+    if ((e = hipGetLastError()) != hipSuccess || (e = hipEventRecord(lw->ev[1], s)) != hipSuccess ||
+        (e = hipEventSynchronize(lw->ev[1])) != hipSuccess)
+      return e;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, lw->ev[0], lw->ev[1]) == hipSuccess) phase[0] += ms;
+    lgs_pA = pA;
+    lgs_pB = pB;
+    lgs_on = true;
+    return hipSuccess;
+  };
+  // F = H + G(P) of a closed shell (G with K(P/2)); low: reduced-precision accumulation.
+  auto fock_rhf = [&](const double* P, double* F, bool lowp) -> hipError_t {
+    if (lowp) hipLaunchKernelGGL(k_lg_pack<float>, dim3(grid_for(M.n_pp)), dim3(256), 0, s, mi, b, P);
+    else hipLaunchKernelGGL(k_lg_pack<double>, dim3(grid_for(M.n_pp)), dim3(256), 0, s, mi, b, P);
+    const int* dcls = static_cast<const int*>(lw->lgs_cls.p);
+    const int64_t* doff = static_cast<const int64_t*>(lw->lgs_off.p);
+    if (lgs_on && lowp)
+      hipLaunchKernelGGL(k_lgs_fock<float>, dim3(M.n_atoms), dim3(LGS_THREADS), 0, s, mi, b,
+                         static_cast<const float*>(lw->lgs_st32.p), dcls, lgs_pA, lgs_pB, doff, P, 0.5, F);
+    else if (lgs_on)
+      hipLaunchKernelGGL(k_lgs_fock<double>, dim3(M.n_atoms), dim3(LGS_THREADS), 0, s, mi, b,
+                         static_cast<const double*>(lw->lgs_st.p), dcls, lgs_pA, lgs_pB, doff, P, 0.5, F);
+    else if (lowp) {
+      hipLaunchKernelGGL(k_lg_fock_diag<float>, dim3(grid_for((int64_t)M.n_pp * 64)), dim3(256), 0, s, mi, b, P, P,
+                         0.5, nullptr, 0.0, F, nullptr);
+      hipLaunchKernelGGL(k_lg_fock_pairs<float>, dim3(grid_for((int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256), 0, s,
+                         mi, b, P, 0.5, nullptr, 0.0, F, nullptr);
+    } else {
+      hipLaunchKernelGGL(k_lg_fock_diag<double>, dim3(grid_for((int64_t)M.n_pp * 64)), dim3(256), 0, s, mi, b, P, P,
+                         0.5, nullptr, 0.0, F, nullptr);
+      hipLaunchKernelGGL(k_lg_fock_pairs<double>, dim3(grid_for((int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256), 0,
+                         s, mi, b, P, 0.5, nullptr, 0.0, F, nullptr);
+    }
+    return hipGetLastError();
+  };
   LargeCtl host{};
   int iter = 0;
   int low = mixed_ok ? 1 : 0;
@@ -767,6 +1046,7 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
   double dp_prev = 1.0e30;
   if (!M.uhf) {
     const int nocc = M.n_occ;
+    LG(lgs_setup());
     if (!M.init_p) hipLaunchKernelGGL(k_lg_sad, dim3(grid_for(M.n_atoms)), dim3(256), 0, s, mi, b, slot(S_HD), 1.0);
     for (; iter < M.max_scf; ++iter) {
       const int h = iter % HIST;
@@ -775,16 +1055,7 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
       double* F = slot(S_HF + h);
       double* E = slot(S_HE + h);
       LG(hipEventRecord(lw->ev[0], s));
-      if (low) {
-        hipLaunchKernelGGL(k_lg_pack<float>, dim3(grid_for(M.n_pp)), dim3(256), 0, s, mi, b, P);
-        hipLaunchKernelGGL(k_lg_fock_diag<float>, dim3(grid_for((int64_t)M.n_pp * 64)), dim3(256), 0, s, mi, b, P, P, 0.5, nullptr, 0.0, F, nullptr);
-        hipLaunchKernelGGL(k_lg_fock_pairs<float>, dim3(grid_for((int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256), 0, s, mi, b, P, 0.5, nullptr, 0.0, F, nullptr);
-      } else {
-        hipLaunchKernelGGL(k_lg_pack<double>, dim3(grid_for(M.n_pp)), dim3(256), 0, s, mi, b, P);
-        hipLaunchKernelGGL(k_lg_fock_diag<double>, dim3(grid_for((int64_t)M.n_pp * 64)), dim3(256), 0, s, mi, b, P, P, 0.5, nullptr, 0.0, F, nullptr);
-        hipLaunchKernelGGL(k_lg_fock_pairs<double>, dim3(grid_for((int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256), 0, s, mi, b, P, 0.5, nullptr, 0.0, F, nullptr);
-      }
-      LG(hipGetLastError());
+      LG(fock_rhf(P, F, low != 0));
       LG(hipEventRecord(lw->ev[1], s));
       // energy and commutator
       LG(dots(2, P, slot(S_H), P, F, nullptr, nullptr, d3));
@@ -841,9 +1112,7 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
     const int hp = iter % HIST;
     double* P = slot(S_HD + hp);
     double* F = slot(S_F);
-    hipLaunchKernelGGL(k_lg_pack<double>, dim3(grid_for(M.n_pp)), dim3(256), 0, s, mi, b, P);
-    hipLaunchKernelGGL(k_lg_fock_diag<double>, dim3(grid_for((int64_t)M.n_pp * 64)), dim3(256), 0, s, mi, b, P, P, 0.5, nullptr, 0.0, F, nullptr);
-        hipLaunchKernelGGL(k_lg_fock_pairs<double>, dim3(grid_for((int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256), 0, s, mi, b, P, 0.5, nullptr, 0.0, F, nullptr);
+    LG(fock_rhf(P, F, false));
     LG(dots(2, P, slot(S_H), P, F, nullptr, nullptr, d3));
     hipLaunchKernelGGL(k_lg_energy, dim3(1), dim3(64), 0, s, ctl, d3, 0);
     int pslot = S_HD + hp;
@@ -860,9 +1129,7 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
         pslot = S_HD + (hp + 1) % HIST;
         double* P2 = slot(pslot);
         LG(density(slot(cslot_c), M.n_occ, 2.0, P2, false));
-        hipLaunchKernelGGL(k_lg_pack<double>, dim3(grid_for(M.n_pp)), dim3(256), 0, s, mi, b, P2);
-        hipLaunchKernelGGL(k_lg_fock_diag<double>, dim3(grid_for((int64_t)M.n_pp * 64)), dim3(256), 0, s, mi, b, P2, P2, 0.5, nullptr, 0.0, F, nullptr);
-        hipLaunchKernelGGL(k_lg_fock_pairs<double>, dim3(grid_for((int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256), 0, s, mi, b, P2, 0.5, nullptr, 0.0, F, nullptr);
+        LG(fock_rhf(P2, F, false));
         LG(dots(2, P2, slot(S_H), P2, F, nullptr, nullptr, d3));
         hipLaunchKernelGGL(k_lg_energy, dim3(1), dim3(64), 0, s, ctl, d3, 0);
         if (check) {

@@ -9,6 +9,8 @@ import json
 import sys
 from pathlib import Path
 
+import os
+
 import numpy as np
 import pytest
 
@@ -312,6 +314,31 @@ def test_large_path_water_cluster_matches_oracle(eng, n_water, precision):
     assert np.max(np.abs(c.T @ c - np.eye(c.shape[0]))) <= 1e-9
     occ = c[:, : r.n_occ]
     assert np.max(np.abs(2.0 * occ @ occ.T - r.density)) <= 1e-9
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_large_path_integral_streams_equal_pair_kernels(eng, precision):
+    """The per-atom integral streams of the closed-shell s/p large path (one workgroup per atom) and
+    the element and pair kernels (PM7_AMD_LARGE_STREAMS=0) are two orders of the same sums: same
+    SCF to well inside the parity bars, and the streamed run is reproducible bit for bit."""
+    atoms = W.water_cluster(40, 4)
+    say(f"large path streams vs pair kernels, {precision}")
+    opts = pm7amd.Pm7Options(precision=precision)
+    a = eng.run(mol_of(atoms), opts)
+    b = eng.run(mol_of(atoms), opts)
+    old = os.environ.get("PM7_AMD_LARGE_STREAMS")
+    os.environ["PM7_AMD_LARGE_STREAMS"] = "0"
+    try:
+        c = eng.run(mol_of(atoms), opts)
+    finally:
+        if old is None:
+            os.environ.pop("PM7_AMD_LARGE_STREAMS", None)
+        else:
+            os.environ["PM7_AMD_LARGE_STREAMS"] = old
+    assert a.converged and c.converged
+    assert a.total_ev == b.total_ev and np.array_equal(a.density, b.density)
+    assert abs(a.total_ev - c.total_ev) <= 1e-8
+    assert np.max(np.abs(a.density - c.density)) <= 1e-8
 
 
 def test_large_group_matches_single_runs(eng):

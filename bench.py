@@ -157,12 +157,13 @@ def main() -> int:
                     help="water cluster of the large-system run (0: skip); 500 waters = 3000 AOs")
     ap.add_argument("--large-only", action="store_true", help="only the large-system run")
     ap.add_argument("--response-only", action="store_true", help="only the response-Fock run")
+    ap.add_argument("--serving-only", action="store_true", help="only the serving run")
     ap.add_argument("--d-batch", type=int, default=64,
                     help="S/P/Cl molecules (MNDO/d pairs, <= 196 AOs) batched in FP64 and mixed (0: skip)")
     ap.add_argument("--gradient", type=int, default=1, help="time batched nuclear gradients (0: skip)")
     ap.add_argument("--response", type=int, default=1, help="time the CPHF response Fock (0: skip)")
     ap.add_argument("--dandc", type=int, default=1, help="divide-and-conquer scaling series (0: skip)")
-    ap.add_argument("--serve-lanes", type=int, default=4)
+    ap.add_argument("--serve-lanes", type=int, default=8)
     ap.add_argument("--serve-batch", type=int, default=64)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (calibrates HBM byte counters)")
@@ -336,6 +337,69 @@ def main() -> int:
         print(f"[bench] response fock: {out}", file=sys.stderr, flush=True)
         return out
 
+    # Serving: the continuous-batching server's FP64 capacity is measured first (every request
+    # submitted at once), then Poisson arrivals at 75 % of it for FP64 and for mixed precision;
+    # latency is submit -> completion per request.
+    def serving_run(eng):
+        import threading
+        stream = W.serving_stream(args.serving, seed=5, rate_per_s=1.0, frac_sp=0.7, frac_d=0.2)
+        mols_s = [(pm7amd.Molecule.from_symbols_angstrom(a, c, m), c, m) for _, a, c, m, _ in stream]
+        kinds = [k for *_, k in stream]
+        gaps = [t for t, *_ in stream]  # arrival times at rate 1/s; scaled below
+        reqs = [(0.0, mol, c, m) for mol, c, m in mols_s]
+
+        def serve(precision):
+            lat, bad = [], 0
+            by_kind, kind_of, got = {}, {}, []
+            with pm7amd.Server(eng, lanes=args.serve_lanes, max_batch=args.serve_batch) as srv:
+                t0 = time.perf_counter()
+
+                def feed():
+                    for k, (t, mol, c, m) in enumerate(reqs):
+                        dt = t - (time.perf_counter() - t0)
+                        if dt > 0:
+                            time.sleep(dt)
+                        kind_of[srv.submit(mol, pm7amd.Pm7Options(charge=c, multiplicity=m, precision=precision))] = kinds[k]
+
+                th = threading.Thread(target=feed)
+                th.start()
+                done = 0
+                while done < len(reqs):
+                    out = srv.poll(-1, timeout_ms=60000)
+                    if out is None:
+                        raise SystemExit("serving stalled")
+                    done += 1
+                    lat.append(out[2])
+                    got.append((out[0], out[2]))
+                    bad += out[1].status != 0
+                elapsed = time.perf_counter() - t0
+                th.join()
+            for rid, ms in got:
+                by_kind.setdefault(kind_of.get(rid, "?"), []).append(ms)
+            return {"solves_per_s": round(len(reqs) / elapsed, 3), "failed": bad,
+                    "latency_ms": {"p50": round(pct(lat, 0.5), 3), "p99": round(pct(lat, 0.99), 3)},
+                    "latency_ms_by_kind": {k: {"n": len(v), "p50": round(pct(v, 0.5), 3), "p99": round(pct(v, 0.99), 3)}
+                                           for k, v in sorted(by_kind.items())}}
+
+        serve("fp64")  # warm the server lanes' buffers
+        capacity = serve("fp64")  # all requests at t = 0
+        rate = 0.75 * capacity["solves_per_s"]
+        reqs = [(t / rate, mol, c, m) for t, (mol, c, m) in zip(gaps, mols_s)]
+        serving = {"requests": len(reqs), "capacity_fp64_solves_per_s": capacity["solves_per_s"],
+                   "rate_per_s": round(rate, 1), "lanes": args.serve_lanes,
+                   "max_batch": args.serve_batch,
+                   "mix": "70% closed-shell H/C/N/O (3-60 atoms), 20% S/P/Cl molecules of 12-200 atoms "
+                          "(MNDO/d pairs), 10% UHF doublets (one H removed)",
+                   "fp64": serve("fp64"), "mixed": serve("mixed")}
+        print(f"[bench] serving: {serving['fp64']} / {serving['mixed']}", file=sys.stderr, flush=True)
+        return serving
+
+    if args.serving_only:
+        eng = pm7amd.Engine(0)
+        print(json.dumps({"serving": serving_run(eng)}), flush=True)
+        eng.close()
+        return 0
+
     if args.response_only:
         eng = pm7amd.Engine(0)
         print(json.dumps({"response_fock": response_fock(eng)}), flush=True)
@@ -465,62 +529,7 @@ def main() -> int:
         pp["cu_busy_frac"] = round(sum(r.device_phase_ms[5] for r in pr) / (N_CU * pp["ms_per_step"]), 4)
     print(f"[bench] pipelined: fp64 {pipe64}, mixed {pipemx}", file=sys.stderr, flush=True)
 
-    # Serving: the continuous-batching server's FP64 capacity is measured first (every request
-    # submitted at once), then Poisson arrivals at 75 % of it for FP64 and for mixed precision;
-    # latency is submit -> completion per request.
-    serving = None
-    if args.serving > 0:
-        import threading
-        stream = W.serving_stream(args.serving, seed=5, rate_per_s=1.0, frac_sp=0.7, frac_d=0.2)
-        mols_s = [(pm7amd.Molecule.from_symbols_angstrom(a, c, m), c, m) for _, a, c, m, _ in stream]
-        kinds = [k for *_, k in stream]
-        gaps = [t for t, *_ in stream]  # arrival times at rate 1/s; scaled below
-        reqs = [(0.0, mol, c, m) for mol, c, m in mols_s]
-
-        def serve(precision):
-            lat, bad = [], 0
-            by_kind, kind_of, got = {}, {}, []
-            with pm7amd.Server(eng, lanes=args.serve_lanes, max_batch=args.serve_batch) as srv:
-                t0 = time.perf_counter()
-
-                def feed():
-                    for k, (t, mol, c, m) in enumerate(reqs):
-                        dt = t - (time.perf_counter() - t0)
-                        if dt > 0:
-                            time.sleep(dt)
-                        kind_of[srv.submit(mol, pm7amd.Pm7Options(charge=c, multiplicity=m, precision=precision))] = kinds[k]
-
-                th = threading.Thread(target=feed)
-                th.start()
-                done = 0
-                while done < len(reqs):
-                    out = srv.poll(-1, timeout_ms=60000)
-                    if out is None:
-                        raise SystemExit("serving stalled")
-                    done += 1
-                    lat.append(out[2])
-                    got.append((out[0], out[2]))
-                    bad += out[1].status != 0
-                elapsed = time.perf_counter() - t0
-                th.join()
-            for rid, ms in got:
-                by_kind.setdefault(kind_of.get(rid, "?"), []).append(ms)
-            return {"solves_per_s": round(len(reqs) / elapsed, 3), "failed": bad,
-                    "latency_ms": {"p50": round(pct(lat, 0.5), 3), "p99": round(pct(lat, 0.99), 3)},
-                    "latency_ms_by_kind": {k: {"n": len(v), "p50": round(pct(v, 0.5), 3), "p99": round(pct(v, 0.99), 3)}
-                                           for k, v in sorted(by_kind.items())}}
-
-        serve("fp64")  # warm the server lanes' buffers
-        capacity = serve("fp64")  # all requests at t = 0
-        rate = 0.75 * capacity["solves_per_s"]
-        reqs = [(t / rate, mol, c, m) for t, (mol, c, m) in zip(gaps, mols_s)]
-        serving = {"requests": len(reqs), "capacity_fp64_solves_per_s": capacity["solves_per_s"],
-                   "rate_per_s": round(rate, 1), "lanes": args.serve_lanes,
-                   "max_batch": args.serve_batch,
-                   "mix": "70% closed-shell H/C/N/O (3-60 atoms), 20% S/P/Cl molecules of 12-200 atoms "
-                          "(MNDO/d pairs), 10% UHF doublets (one H removed)",
-                   "fp64": serve("fp64"), "mixed": serve("mixed")}
-        print(f"[bench] serving: {serving['fp64']} / {serving['mixed']}", file=sys.stderr, flush=True)
+    serving = serving_run(eng) if args.serving > 0 else None
 
     large = large_system(eng) if args.large_waters > 0 else None
     copy_gbs = copy_bandwidth_gbs()

@@ -555,13 +555,14 @@ __device__ __forceinline__ void lgs_chunk(const MolDev& M, const BatchDev& b, in
   const double* ppb = b.slab + M.pp0 + b.atom_pp[bb];
   // a is the pair's first atom: the larger orbital count, on a tie the lower index
   const bool af = NFA > NFB ? true : (NFA < NFB ? false : la < lb);
-  const int oF = af ? oa : ob, oS = af ? ob : oa;
+  // P[F_p][S_q] read at a's rows (P is symmetric): in the class-sorted basis the partners of a
+  // chunk are contiguous in AO order, so these loads and the stores below are coalesced.
   T pfs[NF][NS], acc[NF][NS];
 #pragma unroll
   for (int p = 0; p < NF; ++p)
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
-      pfs[p][q] = (T)P[(int64_t)(oF + p) * ld + oS + q];
+      pfs[p][q] = (T)P[af ? (int64_t)(oa + p) * ld + ob + q : (int64_t)(oa + q) * ld + ob + p];
       acc[p][q] = (T)0;
     }
   T ppv[NPF > NPS ? NPF : NPS];
@@ -597,9 +598,8 @@ __device__ __forceinline__ void lgs_chunk(const MolDev& M, const BatchDev& b, in
   for (int p = 0; p < NF; ++p)
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
-      const int64_t fs = (int64_t)(oF + p) * ld + oS + q;  // the element's place in F's rows
-      const double v = H[fs] - (double)((T)xa * acc[p][q]);
-      F[af ? fs : (int64_t)(oS + q) * ld + oF + p] = v;
+      const int64_t at = af ? (int64_t)(oa + p) * ld + ob + q : (int64_t)(oa + q) * ld + ob + p;  // a's row
+      F[at] = H[at] - (double)((T)xa * acc[p][q]);
     }
 }
 
@@ -661,6 +661,25 @@ __global__ void __launch_bounds__(LGS_THREADS) k_lgs_fock(int mi, BatchDev b, co
   F[i2] = f;
 }
 
+// Basis permutations of n x n matrices (row stride ld): dst[i][j] = src[pi[i]][pi[j]], and for the
+// orbitals (rows are orbitals, columns AOs) dst[k][j] = src[k][pi[j]].
+__global__ void __launch_bounds__(256) k_lgs_perm_sym(int n, int ld, const int* __restrict__ pi,
+                                                      const double* __restrict__ src, double* __restrict__ dst) {
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < nn; idx += (int64_t)gridDim.x * 256) {
+    const int i = (int)(idx / n), j = (int)(idx - (int64_t)i * n);
+    dst[(int64_t)i * ld + j] = src[(int64_t)pi[i] * ld + pi[j]];
+  }
+}
+__global__ void __launch_bounds__(256) k_lgs_perm_cols(int n, int ld, const int* __restrict__ pi,
+                                                       const double* __restrict__ src, double* __restrict__ dst) {
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < nn; idx += (int64_t)gridDim.x * 256) {
+    const int k = (int)(idx / n), j = (int)(idx - (int64_t)k * n);
+    dst[(int64_t)k * ld + j] = src[(int64_t)k * ld + pi[j]];
+  }
+}
+
 // SAD guess of the large path (pm7-rs src/scf.rs:1050-1114): diagonal tore/4 per sp AO, minus the
 // charge spread (sad_yy), scaled; pads and off-diagonal stay zero (the slab is zeroed).
 __global__ void k_lg_sad(int mi, BatchDev b, double* __restrict__ P, double scale) {
@@ -678,6 +697,7 @@ struct LargeWork {
   rocblas_handle blas = nullptr;
   Buf ctl, part, dots, info, f32a, f32b;
   Buf lgs_st, lgs_st32, lgs_cls, lgs_off, lgs_norb;  // per-atom integral streams (s/p closed shell)
+  Buf lgs_perm, lgs_ao;                               // class-sorted basis: permutations, AO offsets
   std::vector<hipEvent_t> ev;
 };
 
@@ -952,6 +972,20 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
   // element and pair kernels (k_lg_fock_diag, k_lg_fock_pairs).
   int lgs_pA = 0, lgs_pB = 0;
   bool lgs_on = false;
+  // With the streams the SCF runs in a class-sorted basis (the 4-orbital atoms' AOs first, then the
+  // 1-orbital atoms', each in atom order): bp carries that basis's AO offsets, lgs_perm its
+  // permutation (new -> old, then old -> new). H and a seeded density are permuted at the start, the
+  // density, the orbitals and H back at the end; everything between is basis-independent.
+  BatchDev bp = b;
+  const int* lgs_n2o = nullptr;
+  const int* lgs_o2n = nullptr;
+  auto permute = [&](double* m, const int* pi, bool cols) -> hipError_t {
+    double* tmp = slot(S_W);
+    hipLaunchKernelGGL(k_lg_copy, dim3(grid_for(sq)), dim3(256), 0, s, sq, m, tmp);
+    if (cols) hipLaunchKernelGGL(k_lgs_perm_cols, dim3(grid_for(nn)), dim3(256), 0, s, n, ld, pi, tmp, m);
+    else hipLaunchKernelGGL(k_lgs_perm_sym, dim3(grid_for(nn)), dim3(256), 0, s, n, ld, pi, tmp, m);
+    return hipGetLastError();
+  };
   auto lgs_setup = [&]() -> hipError_t {
     const char* env = std::getenv("PM7_AMD_LARGE_STREAMS");  // read per SCF
     const bool enabled = !(env && std::atoi(env) == 0);
@@ -976,6 +1010,24 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
     std::vector<int> cls((size_t)pA + pB, -1);
     std::copy(heavy.begin(), heavy.end(), cls.begin());
     std::copy(light.begin(), light.end(), cls.begin() + pA);
+    // class-sorted basis: new AO offsets and the permutation (new -> old, old -> new)
+    std::vector<int> ao_new(na), n2o(n), o2n(n);
+    {
+      int ao_old = 0;
+      std::vector<int> ao_old_of(na);
+      for (int i = 0; i < na; ++i) {
+        ao_old_of[i] = ao_old;
+        ao_old += norb[i];
+      }
+      const int nh = (int)heavy.size();
+      for (int r = 0; r < nh; ++r) ao_new[heavy[r]] = 4 * r;
+      for (int r = 0; r < (int)light.size(); ++r) ao_new[light[r]] = 4 * nh + r;
+      for (int i = 0; i < na; ++i)
+        for (int k = 0; k < norb[i]; ++k) {
+          n2o[ao_new[i] + k] = ao_old_of[i] + k;
+          o2n[ao_old_of[i] + k] = ao_new[i] + k;
+        }
+    }
     std::vector<int64_t> off(na);
     int64_t total = 0;
     for (int i = 0; i < na; ++i) {
@@ -986,7 +1038,9 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
     if (lw->lgs_st.ensure((size_t)total * sizeof(double)) != hipSuccess ||
         (mixed && lw->lgs_st32.ensure((size_t)total * sizeof(float)) != hipSuccess) ||
         lw->lgs_cls.ensure(cls.size() * sizeof(int)) != hipSuccess ||
-        lw->lgs_off.ensure(off.size() * sizeof(int64_t)) != hipSuccess) {
+        lw->lgs_off.ensure(off.size() * sizeof(int64_t)) != hipSuccess ||
+        lw->lgs_perm.ensure((size_t)2 * n * sizeof(int)) != hipSuccess ||
+        lw->lgs_ao.ensure((size_t)(M.atom0 + na) * sizeof(int)) != hipSuccess) {
       (void)hipGetLastError();  // no room for the streams: the element and pair kernels
       return hipSuccess;
     }
@@ -994,6 +1048,12 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
             hipSuccess ||
         (e = hipMemcpyAsync(lw->lgs_off.p, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice, s)) !=
             hipSuccess ||
+        (e = hipMemcpyAsync(lw->lgs_perm.p, n2o.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, s)) !=
+            hipSuccess ||
+        (e = hipMemcpyAsync(static_cast<int*>(lw->lgs_perm.p) + n, o2n.data(), (size_t)n * sizeof(int),
+                            hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (e = hipMemcpyAsync(static_cast<int*>(lw->lgs_ao.p) + M.atom0, ao_new.data(), (size_t)na * sizeof(int),
+                            hipMemcpyHostToDevice, s)) != hipSuccess ||
         (e = hipEventRecord(lw->ev[0], s)) != hipSuccess)
       return e;
     const int* dcls = static_cast<const int*>(lw->lgs_cls.p);
@@ -1011,10 +1071,16 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
     lgs_pA = pA;
     lgs_pB = pB;
     lgs_on = true;
+    bp.atom_ao = static_cast<const int*>(lw->lgs_ao.p);
+    lgs_n2o = static_cast<const int*>(lw->lgs_perm.p);
+    lgs_o2n = lgs_n2o + n;
+    if ((e = permute(slot(S_H), lgs_n2o, false)) != hipSuccess) return e;
+    if (M.init_p && (e = permute(slot(S_HD), lgs_n2o, false)) != hipSuccess) return e;  // the seeded density
     return hipSuccess;
   };
   // F = H + G(P) of a closed shell (G with K(P/2)); low: reduced-precision accumulation.
   auto fock_rhf = [&](const double* P, double* F, bool lowp) -> hipError_t {
+    const BatchDev& b = bp;  // the basis the SCF runs in
     if (lowp) hipLaunchKernelGGL(k_lg_pack<float>, dim3(grid_for(M.n_pp)), dim3(256), 0, s, mi, b, P);
     else hipLaunchKernelGGL(k_lg_pack<double>, dim3(grid_for(M.n_pp)), dim3(256), 0, s, mi, b, P);
     const int* dcls = static_cast<const int*>(lw->lgs_cls.p);
@@ -1047,7 +1113,7 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
   if (!M.uhf) {
     const int nocc = M.n_occ;
     LG(lgs_setup());
-    if (!M.init_p) hipLaunchKernelGGL(k_lg_sad, dim3(grid_for(M.n_atoms)), dim3(256), 0, s, mi, b, slot(S_HD), 1.0);
+    if (!M.init_p) hipLaunchKernelGGL(k_lg_sad, dim3(grid_for(M.n_atoms)), dim3(256), 0, s, mi, bp, slot(S_HD), 1.0);
     for (; iter < M.max_scf; ++iter) {
       const int h = iter % HIST;
       const int cnt = std::min(iter + 1, HIST);
@@ -1137,6 +1203,11 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
           hipLaunchKernelGGL(k_lg_check, dim3(1), dim3(64), 0, s, ctl, 1, RB, part);
         }
       }
+    }
+    if (lgs_on) {  // back to the caller's basis: the density, the orbitals (AO columns of Ct), H
+      LG(permute(slot(pslot), lgs_o2n, false));
+      LG(permute(slot(cslot_c), lgs_o2n, true));
+      LG(permute(slot(S_H), lgs_o2n, false));
     }
     // C = Ct^T into the C1 slot for the output gather.
     hipLaunchKernelGGL(k_lg_transpose, dim3(grid_for(sq)), dim3(256), 0, s, ld, slot(cslot_c), slot(S_C1));

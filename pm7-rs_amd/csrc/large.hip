@@ -514,27 +514,40 @@ __global__ void k_lgs_norb(int mi, BatchDev b, int* __restrict__ norb) {
     norb[i] = b.elem[b.atom_z[M.atom0 + i]].n_orb;
 }
 
-// Workgroups (x, atom a on y): a's stream (class lists cls[0, pA) and cls[pA, pA + pB), stream of
-// atom a at off[a]).
-template <typename T>
+// Workgroup per (chunk c of atom a's stream): the 64 partners' blocks read one after another
+// (each block contiguous), transposed through LDS, written element-major; the FP64 stream always,
+// the FP32 one when st32 is set. Chunks of a: the 4-orbital partners' (cls[0, pA)), then the
+// 1-orbital partners' (cls[pA, pA + pB)); a's stream starts at off[a].
 __global__ void __launch_bounds__(256) k_lgs_build(int mi, BatchDev b, const int* __restrict__ cls, int pA, int pB,
-                                                   const int64_t* __restrict__ off, T* __restrict__ st) {
+                                                   const int64_t* __restrict__ off, double* __restrict__ st,
+                                                   float* __restrict__ st32) {
+  __shared__ double tile[100 * LGS_CHUNK];
+  __shared__ int64_t s_woff[LGS_CHUNK];
   const MolDev M = b.mols[mi];
   const int la = blockIdx.y, na = M.n_atoms;
   const int npa = npair_of(b.elem[b.atom_z[M.atom0 + la]].n_orb);
-  const int64_t segA = (int64_t)pA * npa * 10, len = segA + (int64_t)pB * npa;
-  T* dst = st + off[la];
-  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < len; idx += (int64_t)gridDim.x * 256) {
-    const bool inA = idx < segA;
-    const int64_t loc = inA ? idx : idx - segA;
-    const int K = npa * (inA ? 10 : 1);
-    const int t = (int)(loc & (LGS_CHUNK - 1));
-    const int64_t kk = loc / LGS_CHUNK;
-    const int chunk = (int)(kk / K), k = (int)(kk - (int64_t)chunk * K);
-    const int lb = cls[(inA ? 0 : pA) + chunk * LGS_CHUNK + t];
-    double v = 0.0;
-    if (lb >= 0 && lb != la) v = b.w[b.pairs[M.pair0 + tri_index(min(la, lb), max(la, lb), na)].w_off + k];
-    dst[idx] = (T)v;
+  const int ca = pA / LGS_CHUNK, c = blockIdx.x;
+  if (c >= ca + pB / LGS_CHUNK) return;  // the whole workgroup
+  const bool inA = c < ca;
+  const int K = npa * (inA ? 10 : 1);
+  const int* list = inA ? cls + c * LGS_CHUNK : cls + pA + (c - ca) * LGS_CHUNK;
+  const int64_t base = off[la] + (inA ? (int64_t)c * K * LGS_CHUNK
+                                      : (int64_t)pA * npa * 10 + (int64_t)(c - ca) * K * LGS_CHUNK);
+  if (threadIdx.x < LGS_CHUNK) {
+    const int lb = list[threadIdx.x];
+    s_woff[threadIdx.x] =
+        (lb >= 0 && lb != la) ? b.pairs[M.pair0 + tri_index(min(la, lb), max(la, lb), na)].w_off : (int64_t)-1;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < K * LGS_CHUNK; i += 256) {  // k fastest: each block read contiguously
+    const int t = i / K, k = i - t * K;
+    const int64_t wo = s_woff[t];
+    tile[k * LGS_CHUNK + t] = wo >= 0 ? b.w[wo + k] : 0.0;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < K * LGS_CHUNK; i += 256) {  // element-major, coalesced
+    st[base + i] = tile[i];
+    if (st32) st32[base + i] = (float)tile[i];
   }
 }
 
@@ -618,25 +631,28 @@ __device__ __forceinline__ void lgs_atom(const MolDev& M, const BatchDev& b, int
                          xa, F, J);
 }
 
-// Workgroup per atom a: its rows of F = H + J(P) - xa K(P) (closed shell), the Coulomb sums from
-// the packed density of every atom (k_lg_pack<T>), one-centre terms as fock_diag_item.
-template <typename T>
+// Workgroup per atom a with NFA orbitals (a = atoms[blockIdx.x]; one launch per orbital count, so
+// the 1-orbital atoms run at the occupancy of their own small blocks): its rows of
+// F = H + J(P) - xa K(P) (closed shell), the Coulomb sums from the packed density of every atom
+// (k_lg_pack<T>), one-centre terms as fock_diag_item.
+template <typename T, int NFA>
 __global__ void __launch_bounds__(LGS_THREADS) k_lgs_fock(int mi, BatchDev b, const T* __restrict__ st,
                                                   const int* __restrict__ cls, int pA, int pB,
-                                                  const int64_t* __restrict__ off, const double* __restrict__ P,
-                                                  double xa, double* __restrict__ F) {
+                                                  const int64_t* __restrict__ off, const int* __restrict__ atoms,
+                                                  const double* __restrict__ P, double xa,
+                                                  double* __restrict__ F) {
   __shared__ T red[10][LGS_THREADS];
   const MolDev M = b.mols[mi];
-  const int la = blockIdx.x, a = M.atom0 + la, ld = M.ld;
+  const int la = atoms[blockIdx.x], a = M.atom0 + la, ld = M.ld;
   const ElemDev ea = b.elem[b.atom_z[a]];
-  const int nfa = ea.n_orb, oa = b.atom_ao[a];
+  constexpr int nfa = NFA;
+  const int oa = b.atom_ao[a];
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   T J[10];
 #pragma unroll
   for (int e = 0; e < 10; ++e) J[e] = (T)0;
   const T* sa = st + off[la];
-  if (nfa == 4) lgs_atom<T, 4>(M, b, la, oa, sa, cls, pA, pB, wv, lane, P, xa, F, J);
-  else lgs_atom<T, 1>(M, b, la, oa, sa, cls, pA, pB, wv, lane, P, xa, F, J);
+  lgs_atom<T, NFA>(M, b, la, oa, sa, cls, pA, pB, wv, lane, P, xa, F, J);
 #pragma unroll
   for (int e = 0; e < 10; ++e) red[e][threadIdx.x] = J[e];  // every thread, in thread order
   __syncthreads();
@@ -970,7 +986,7 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
   // Closed-shell s/p molecules: the per-atom integral streams (k_lgs_fock), built once here, their
   // device time counted in the Fock phase. Other molecules, and PM7_AMD_LARGE_STREAMS=0, keep the
   // element and pair kernels (k_lg_fock_diag, k_lg_fock_pairs).
-  int lgs_pA = 0, lgs_pB = 0;
+  int lgs_pA = 0, lgs_pB = 0, lgs_nA = 0, lgs_nB = 0;
   bool lgs_on = false;
   // With the streams the SCF runs in a class-sorted basis (the 4-orbital atoms' AOs first, then the
   // 1-orbital atoms', each in atom order): bp carries that basis's AO offsets, lgs_perm its
@@ -1058,11 +1074,8 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
       return e;
     const int* dcls = static_cast<const int*>(lw->lgs_cls.p);
     const int64_t* doff = static_cast<const int64_t*>(lw->lgs_off.p);
-    hipLaunchKernelGGL(k_lgs_build<double>, dim3(64, na), dim3(256), 0, s, mi, b, dcls, pA, pB, doff,
-                       static_cast<double*>(lw->lgs_st.p));
-    if (mixed)
-      hipLaunchKernelGGL(k_lgs_build<float>, dim3(64, na), dim3(256), 0, s, mi, b, dcls, pA, pB, doff,
-                         static_cast<float*>(lw->lgs_st32.p));
+    hipLaunchKernelGGL(k_lgs_build, dim3((pA + pB) / LGS_CHUNK, na), dim3(256), 0, s, mi, b, dcls, pA, pB, doff,
+                       static_cast<double*>(lw->lgs_st.p), mixed ? static_cast<float*>(lw->lgs_st32.p) : nullptr);
     if ((e = hipGetLastError()) != hipSuccess || (e = hipEventRecord(lw->ev[1], s)) != hipSuccess ||
         (e = hipEventSynchronize(lw->ev[1])) != hipSuccess)
       return e;
@@ -1070,6 +1083,8 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
     if (hipEventElapsedTime(&ms, lw->ev[0], lw->ev[1]) == hipSuccess) phase[0] += ms;
     lgs_pA = pA;
     lgs_pB = pB;
+    lgs_nA = (int)heavy.size();
+    lgs_nB = (int)light.size();
     lgs_on = true;
     bp.atom_ao = static_cast<const int*>(lw->lgs_ao.p);
     lgs_n2o = static_cast<const int*>(lw->lgs_perm.p);
@@ -1085,12 +1100,19 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
     else hipLaunchKernelGGL(k_lg_pack<double>, dim3(grid_for(M.n_pp)), dim3(256), 0, s, mi, b, P);
     const int* dcls = static_cast<const int*>(lw->lgs_cls.p);
     const int64_t* doff = static_cast<const int64_t*>(lw->lgs_off.p);
+    auto streamed = [&](auto* st) {
+      using T = std::remove_const_t<std::remove_pointer_t<decltype(st)>>;
+      if (lgs_nA > 0)
+        hipLaunchKernelGGL((k_lgs_fock<T, 4>), dim3(lgs_nA), dim3(LGS_THREADS), 0, s, mi, b, st, dcls, lgs_pA, lgs_pB,
+                           doff, dcls, P, 0.5, F);
+      if (lgs_nB > 0)
+        hipLaunchKernelGGL((k_lgs_fock<T, 1>), dim3(lgs_nB), dim3(LGS_THREADS), 0, s, mi, b, st, dcls, lgs_pA, lgs_pB,
+                           doff, dcls + lgs_pA, P, 0.5, F);
+    };
     if (lgs_on && lowp)
-      hipLaunchKernelGGL(k_lgs_fock<float>, dim3(M.n_atoms), dim3(LGS_THREADS), 0, s, mi, b,
-                         static_cast<const float*>(lw->lgs_st32.p), dcls, lgs_pA, lgs_pB, doff, P, 0.5, F);
+      streamed(static_cast<const float*>(lw->lgs_st32.p));
     else if (lgs_on)
-      hipLaunchKernelGGL(k_lgs_fock<double>, dim3(M.n_atoms), dim3(LGS_THREADS), 0, s, mi, b,
-                         static_cast<const double*>(lw->lgs_st.p), dcls, lgs_pA, lgs_pB, doff, P, 0.5, F);
+      streamed(static_cast<const double*>(lw->lgs_st.p));
     else if (lowp) {
       hipLaunchKernelGGL(k_lg_fock_diag<float>, dim3(grid_for((int64_t)M.n_pp * 64)), dim3(256), 0, s, mi, b, P, P,
                          0.5, nullptr, 0.0, F, nullptr);

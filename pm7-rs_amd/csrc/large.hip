@@ -616,34 +616,36 @@ __device__ __forceinline__ void lgs_chunk(const MolDev& M, const BatchDev& b, in
     }
 }
 
-template <typename T, int NFA>
+// The partner chunks of segment SEG (0: both, 1: the 4-orbital partners, 2: the 1-orbital ones).
+template <typename T, int NFA, int SEG>
 __device__ __forceinline__ void lgs_atom(const MolDev& M, const BatchDev& b, int la, int oa, const T* __restrict__ sa,
                                          const int* __restrict__ cls, int pA, int pB, int wv, int lane,
                                          const double* __restrict__ P, double xa, double* __restrict__ F, T (&J)[10]) {
   constexpr int NPA = NFA == 4 ? 10 : 1;
   constexpr int NW = LGS_THREADS / 64;
-  for (int c = wv; c < pA / LGS_CHUNK; c += NW)
-    lgs_chunk<T, NFA, 4>(M, b, la, oa, sa + (int64_t)c * NPA * 10 * LGS_CHUNK + lane, cls + c * LGS_CHUNK, lane, P,
-                         xa, F, J);
+  if constexpr (SEG != 2)
+    for (int c = wv; c < pA / LGS_CHUNK; c += NW)
+      lgs_chunk<T, NFA, 4>(M, b, la, oa, sa + (int64_t)c * NPA * 10 * LGS_CHUNK + lane, cls + c * LGS_CHUNK, lane,
+                           P, xa, F, J);
   const T* sb = sa + (int64_t)pA * NPA * 10;
-  for (int c = wv; c < pB / LGS_CHUNK; c += NW)
-    lgs_chunk<T, NFA, 1>(M, b, la, oa, sb + (int64_t)c * NPA * LGS_CHUNK + lane, cls + pA + c * LGS_CHUNK, lane, P,
-                         xa, F, J);
+  if constexpr (SEG != 1)
+    for (int c = wv; c < pB / LGS_CHUNK; c += NW)
+      lgs_chunk<T, NFA, 1>(M, b, la, oa, sb + (int64_t)c * NPA * LGS_CHUNK + lane, cls + pA + c * LGS_CHUNK, lane,
+                           P, xa, F, J);
 }
 
-// Workgroup per atom a with NFA orbitals (a = atoms[blockIdx.x]; one launch per orbital count, so
-// the 1-orbital atoms run at the occupancy of their own small blocks): its rows of
-// F = H + J(P) - xa K(P) (closed shell), the Coulomb sums from the packed density of every atom
-// (k_lg_pack<T>), one-centre terms as fock_diag_item.
-template <typename T, int NFA>
-__global__ void __launch_bounds__(LGS_THREADS) k_lgs_fock(int mi, BatchDev b, const T* __restrict__ st,
-                                                  const int* __restrict__ cls, int pA, int pB,
-                                                  const int64_t* __restrict__ off, const int* __restrict__ atoms,
-                                                  const double* __restrict__ P, double xa,
-                                                  double* __restrict__ F) {
-  __shared__ T red[10][LGS_THREADS];
-  const MolDev M = b.mols[mi];
-  const int la = atoms[blockIdx.x], a = M.atom0 + la, ld = M.ld;
+// Workgroup per atom a with NFA orbitals: its rows of F = H + J(P) - xa K(P) (closed shell), the
+// Coulomb sums from the packed density of every atom (k_lg_pack<T>), one-centre terms as
+// fock_diag_item. SEG 0 takes all partners. A 4-orbital atom takes two workgroups in two launches:
+// SEG 2 its 1-orbital partners, Coulomb sums into jpart[slot]; SEG 1 its 4-orbital partners, then
+// jpart[slot] + its own sums and the diagonal block.
+template <typename T, int NFA, int SEG>
+__device__ __forceinline__ void lgs_fock_atom(const MolDev& M, const BatchDev& b, const T* __restrict__ st,
+                                              const int* __restrict__ cls, int pA, int pB,
+                                              const int64_t* __restrict__ off, int la, int slot,
+                                              const double* __restrict__ P, double xa, double* __restrict__ F,
+                                              T* __restrict__ jpart, T (&red)[10][LGS_THREADS]) {
+  const int a = M.atom0 + la, ld = M.ld;
   const ElemDev ea = b.elem[b.atom_z[a]];
   constexpr int nfa = NFA;
   const int oa = b.atom_ao[a];
@@ -651,16 +653,19 @@ __global__ void __launch_bounds__(LGS_THREADS) k_lgs_fock(int mi, BatchDev b, co
   T J[10];
 #pragma unroll
   for (int e = 0; e < 10; ++e) J[e] = (T)0;
-  const T* sa = st + off[la];
-  lgs_atom<T, NFA>(M, b, la, oa, sa, cls, pA, pB, wv, lane, P, xa, F, J);
+  lgs_atom<T, NFA, SEG>(M, b, la, oa, st + off[la], cls, pA, pB, wv, lane, P, xa, F, J);
 #pragma unroll
   for (int e = 0; e < 10; ++e) red[e][threadIdx.x] = J[e];  // every thread, in thread order
   __syncthreads();
   const int npa = npair_of(nfa);
   if ((int)threadIdx.x >= npa) return;
   const int e = threadIdx.x;
-  T acc = (T)0;
+  T acc = SEG == 1 ? jpart[slot * 10 + e] : (T)0;
   for (int i = 0; i < LGS_THREADS; ++i) acc += red[e][i];
+  if constexpr (SEG == 2) {
+    jpart[slot * 10 + e] = acc;
+    return;
+  }
   int mu, nu;
   unpack_e(e, mu, nu);
   T ja = (T)0, ka = (T)0;
@@ -675,6 +680,37 @@ __global__ void __launch_bounds__(LGS_THREADS) k_lgs_fock(int mi, BatchDev b, co
   const double f = H[i1] + (double)(acc + ja - (T)xa * ka);
   F[i1] = f;
   F[i2] = f;
+}
+
+// The 4-orbital atoms' 4-orbital partners (SEG 1, after k_lgs_fock_light): workgroup i takes
+// atom cls[i].
+template <typename T>
+__global__ void __launch_bounds__(LGS_THREADS) k_lgs_fock_heavy(int mi, BatchDev b, const T* __restrict__ st,
+                                                        const int* __restrict__ cls, int pA, int pB,
+                                                        const int64_t* __restrict__ off,
+                                                        const double* __restrict__ P, double xa,
+                                                        double* __restrict__ F, T* __restrict__ jpart) {
+  __shared__ T red[10][LGS_THREADS];
+  const MolDev M = b.mols[mi];
+  lgs_fock_atom<T, 4, 1>(M, b, st, cls, pA, pB, off, cls[blockIdx.x], blockIdx.x, P, xa, F, jpart, red);
+}
+
+// The small blocks in one launch: workgroups [0, nA) take the 4-orbital atoms' 1-orbital partners
+// (SEG 2), the next nB the 1-orbital atoms (all partners); both need few registers and run at the
+// same occupancy.
+template <typename T>
+__global__ void __launch_bounds__(LGS_THREADS) k_lgs_fock_light(int mi, BatchDev b, const T* __restrict__ st,
+                                                        const int* __restrict__ cls, int pA, int pB, int nA,
+                                                        const int64_t* __restrict__ off,
+                                                        const double* __restrict__ P, double xa,
+                                                        double* __restrict__ F, T* __restrict__ jpart) {
+  __shared__ T red[10][LGS_THREADS];
+  const MolDev M = b.mols[mi];
+  const int i = blockIdx.x;
+  if (i < nA)
+    lgs_fock_atom<T, 4, 2>(M, b, st, cls, pA, pB, off, cls[i], i, P, xa, F, jpart, red);
+  else
+    lgs_fock_atom<T, 1, 0>(M, b, st, cls, pA, pB, off, cls[pA + i - nA], 0, P, xa, F, jpart, red);
 }
 
 // Basis permutations of n x n matrices (row stride ld): dst[i][j] = src[pi[i]][pi[j]], and for the
@@ -714,6 +750,7 @@ struct LargeWork {
   Buf ctl, part, dots, info, f32a, f32b;
   Buf lgs_st, lgs_st32, lgs_cls, lgs_off, lgs_norb;  // per-atom integral streams (s/p closed shell)
   Buf lgs_perm, lgs_ao;                               // class-sorted basis: permutations, AO offsets
+  Buf lgs_jpart;                                      // partial Coulomb sums of the 4-orbital atoms
   std::vector<hipEvent_t> ev;
 };
 
@@ -1056,6 +1093,7 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
         lw->lgs_cls.ensure(cls.size() * sizeof(int)) != hipSuccess ||
         lw->lgs_off.ensure(off.size() * sizeof(int64_t)) != hipSuccess ||
         lw->lgs_perm.ensure((size_t)2 * n * sizeof(int)) != hipSuccess ||
+        lw->lgs_jpart.ensure((size_t)10 * std::max<size_t>(heavy.size(), 1) * sizeof(double)) != hipSuccess ||
         lw->lgs_ao.ensure((size_t)(M.atom0 + na) * sizeof(int)) != hipSuccess) {
       (void)hipGetLastError();  // no room for the streams: the element and pair kernels
       return hipSuccess;
@@ -1102,12 +1140,13 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
     const int64_t* doff = static_cast<const int64_t*>(lw->lgs_off.p);
     auto streamed = [&](auto* st) {
       using T = std::remove_const_t<std::remove_pointer_t<decltype(st)>>;
+      T* jp = static_cast<T*>(lw->lgs_jpart.p);
+      if (lgs_nA + lgs_nB > 0)
+        hipLaunchKernelGGL((k_lgs_fock_light<T>), dim3(lgs_nA + lgs_nB), dim3(LGS_THREADS), 0, s, mi, b, st, dcls,
+                           lgs_pA, lgs_pB, lgs_nA, doff, P, 0.5, F, jp);
       if (lgs_nA > 0)
-        hipLaunchKernelGGL((k_lgs_fock<T, 4>), dim3(lgs_nA), dim3(LGS_THREADS), 0, s, mi, b, st, dcls, lgs_pA, lgs_pB,
-                           doff, dcls, P, 0.5, F);
-      if (lgs_nB > 0)
-        hipLaunchKernelGGL((k_lgs_fock<T, 1>), dim3(lgs_nB), dim3(LGS_THREADS), 0, s, mi, b, st, dcls, lgs_pA, lgs_pB,
-                           doff, dcls + lgs_pA, P, 0.5, F);
+        hipLaunchKernelGGL((k_lgs_fock_heavy<T>), dim3(lgs_nA), dim3(LGS_THREADS), 0, s, mi, b, st, dcls, lgs_pA,
+                           lgs_pB, doff, P, 0.5, F, jp);
     };
     if (lgs_on && lowp)
       streamed(static_cast<const float*>(lw->lgs_st32.p));

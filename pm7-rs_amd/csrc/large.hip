@@ -479,19 +479,25 @@ __global__ void k_lg_init(LargeCtl* __restrict__ ctl, double shift_floor, int lo
 
 // ---- s/p closed-shell Fock from per-atom integral streams ---------------------------------------
 //
-// For molecules whose atoms all carry 1 or 4 orbitals, each atom a gets its own stream of the pair
-// integral blocks (a, b) over every partner b, built once per SCF: partners are grouped by class
-// (4-orbital atoms, then 1-orbital atoms, each list in atom order and padded to a multiple of 64
-// with -1) and a chunk of 64 partners is stored element-major, element k of lane t at
-// [chunk][k][t], each block in the pair's own order w[e_F][kl_S] (F the pair's first atom), zero
-// for a's own slot and the padding. One workgroup per atom then reads its stream once, coalesced,
-// and writes its own rows of F: the Coulomb sum of the diagonal block over all partners (lanes,
-// then a fixed-order reduction) and each partner's exchange block, accumulated element by element
-// in the pair's own order and oriented by F and S, so the blocks (a, b) and (b, a) from two
-// workgroups are the same sums bit for bit and F stays exactly symmetric.
+// For molecules whose atoms all carry 1 or 4 orbitals. The atoms form two classes, A (4 orbitals)
+// and B (1 orbital), each in atom order; an atom's position there is its rank, and its sorted index
+// u is its rank (A) or nA + rank (B); the SCF runs in the AO basis of that order. Every atom pair
+// is taken once, by the atom of lower sorted index, a, which is also the pair's first atom in the
+// pair's own order w[e_F][kl_S] (A before B; within a class the lower index). Partners are grouped
+// in chunks of 64 of one class aligned on rank (chunk c holds ranks 64c..64c+63), so partner b sits
+// at lane rank_b % 64 of chunk rank_b / 64 in every stream. a's stream, built once per SCF, holds
+// the chunks from the one holding a's first later partner: its class-A chunks (A atoms only), then
+// its class-B chunks; element k of lane t at [chunk][k][t], zero for lanes that are not a later
+// partner. Each pair's integrals are read once per Fock build.
+//
+// One wavefront per (a, chunk): the exchange block F(a, b) = H(a, b) - xa K_ab, stored at (a, b) and
+// mirrored at (b, a), so F is exactly symmetric; a's Coulomb sum over the chunk (a fixed shuffle
+// tree) into jrow; each partner's Coulomb sum from a's packed density into jcol. k_lgs_colsum adds
+// the jcol entries of every partner chunk in fixed parts, and k_lgs_diag adds the parts and the jrow
+// entries of each atom in a fixed order and writes its diagonal block with the one-centre terms.
 
 constexpr int LGS_CHUNK = 64;
-constexpr int LGS_THREADS = 256;  // threads of a k_lgs_fock workgroup
+constexpr int LGS_NPART = 32;  // fixed parts of k_lgs_colsum's sums per partner chunk
 
 // The larger orbital index of packed pair e (pack_idx's h), at compile time.
 constexpr int ce_hi(int e) {
@@ -507,6 +513,16 @@ __device__ __forceinline__ void static_for(F&& f) {
   }
 }
 
+struct LgsDims {
+  int nA, nB, nchA, nchB;  // atoms and chunks per class
+};
+struct LgsAtom {           // per sorted index
+  int64_t soff, joff;      // offsets of its stream and of its jcol entries
+  int la, rank;            // molecule-local atom, rank in its class
+};
+// The chunk holding the first later partner of rank r in its own class.
+__host__ __device__ inline int lgs_c0(int r) { return (r + 1) / LGS_CHUNK; }
+
 // Orbital count of every atom of molecule mi, in molecule order.
 __global__ void k_lgs_norb(int mi, BatchDev b, int* __restrict__ norb) {
   const MolDev M = b.mols[mi];
@@ -514,29 +530,35 @@ __global__ void k_lgs_norb(int mi, BatchDev b, int* __restrict__ norb) {
     norb[i] = b.elem[b.atom_z[M.atom0 + i]].n_orb;
 }
 
-// Workgroup per (chunk c of atom a's stream): the 64 partners' blocks read one after another
-// (each block contiguous), transposed through LDS, written element-major; the FP64 stream always,
-// the FP32 one when st32 is set. Chunks of a: the 4-orbital partners' (cls[0, pA)), then the
-// 1-orbital partners' (cls[pA, pA + pB)); a's stream starts at off[a].
-__global__ void __launch_bounds__(256) k_lgs_build(int mi, BatchDev b, const int* __restrict__ cls, int pA, int pB,
-                                                   const int64_t* __restrict__ off, double* __restrict__ st,
+// Workgroup per (local chunk j, atom u): the 64 partners' blocks read one after another (each
+// contiguous), transposed through LDS, written element-major; FP64 always, FP32 when st32 is set.
+__global__ void __launch_bounds__(256) k_lgs_build(int mi, BatchDev b, LgsDims d, const LgsAtom* __restrict__ at,
+                                                   const int* __restrict__ srt, double* __restrict__ st,
                                                    float* __restrict__ st32) {
   __shared__ double tile[100 * LGS_CHUNK];
   __shared__ int64_t s_woff[LGS_CHUNK];
   const MolDev M = b.mols[mi];
-  const int la = blockIdx.y, na = M.n_atoms;
-  const int npa = npair_of(b.elem[b.atom_z[M.atom0 + la]].n_orb);
-  const int ca = pA / LGS_CHUNK, c = blockIdx.x;
-  if (c >= ca + pB / LGS_CHUNK) return;  // the whole workgroup
-  const bool inA = c < ca;
+  const int u = blockIdx.y, j = blockIdx.x, na = M.n_atoms;
+  const LgsAtom A = at[u];
+  const bool aA = u < d.nA;
+  const int npa = aA ? 10 : 1;
+  const int nsegA = aA ? d.nchA - lgs_c0(A.rank) : 0;
+  const int c0B = aA ? 0 : lgs_c0(A.rank);
+  if (j >= nsegA + d.nchB - c0B) return;  // the whole workgroup
+  const bool inA = j < nsegA;
+  const int c = inA ? lgs_c0(A.rank) + j : c0B + (j - nsegA);
   const int K = npa * (inA ? 10 : 1);
-  const int* list = inA ? cls + c * LGS_CHUNK : cls + pA + (c - ca) * LGS_CHUNK;
-  const int64_t base = off[la] + (inA ? (int64_t)c * K * LGS_CHUNK
-                                      : (int64_t)pA * npa * 10 + (int64_t)(c - ca) * K * LGS_CHUNK);
+  const int64_t base = A.soff + (inA ? (int64_t)j * K * LGS_CHUNK
+                                     : (int64_t)nsegA * npa * 10 * LGS_CHUNK + (int64_t)(j - nsegA) * K * LGS_CHUNK);
   if (threadIdx.x < LGS_CHUNK) {
-    const int lb = list[threadIdx.x];
-    s_woff[threadIdx.x] =
-        (lb >= 0 && lb != la) ? b.pairs[M.pair0 + tri_index(min(la, lb), max(la, lb), na)].w_off : (int64_t)-1;
+    const int rb = c * LGS_CHUNK + threadIdx.x;
+    const bool later = rb < (inA ? d.nA : d.nB) && (aA != inA || rb > A.rank);
+    int64_t wo = -1;
+    if (later) {
+      const int lb = srt[(inA ? 0 : d.nA) + rb];
+      wo = b.pairs[M.pair0 + tri_index(min(A.la, lb), max(A.la, lb), na)].w_off;
+    }
+    s_woff[threadIdx.x] = wo;
   }
   __syncthreads();
   for (int i = threadIdx.x; i < K * LGS_CHUNK; i += 256) {  // k fastest: each block read contiguously
@@ -551,166 +573,292 @@ __global__ void __launch_bounds__(256) k_lgs_build(int mi, BatchDev b, const int
   }
 }
 
-// One chunk of partners (class with NFB orbitals) of a workgroup whose atom has NFA orbitals: lane
-// `lane` takes partner cls[lane]; J gathers the Coulomb sums of a's packed pairs.
+// One (a, chunk) item: a has NFA orbitals, the chunk's partners NFB; lane `lane` takes partner lb
+// (live: a later partner). sp, jc: this item's stream and jcol entries at the lane; jr: a's jrow
+// entry of this chunk.
 template <typename T, int NFA, int NFB>
-__device__ __forceinline__ void lgs_chunk(const MolDev& M, const BatchDev& b, int la, int oa,
-                                          const T* __restrict__ sp, const int* __restrict__ cls, int lane,
-                                          const double* __restrict__ P, double xa, double* __restrict__ F,
-                                          T (&J)[10]) {
-  constexpr int NF = NFA > NFB ? NFA : NFB, NS = NFA > NFB ? NFB : NFA;
-  constexpr int NPF = NF * (NF + 1) / 2, NPS = NS * (NS + 1) / 2;
+__device__ __forceinline__ void lgs_item(const MolDev& M, const BatchDev& b, int la, const T* __restrict__ sp,
+                                         int lb, bool live, const double* __restrict__ P, double xa,
+                                         double* __restrict__ F, T* __restrict__ jc, T* __restrict__ jr) {
+  constexpr int NPA = NFA * (NFA + 1) / 2, NPB = NFB * (NFB + 1) / 2;
   const int ld = M.ld;
-  const int lb = cls[lane];
-  const bool live = lb >= 0 && lb != la;
+  const int a = M.atom0 + la;
+  const int oa = b.atom_ao[a];
   const int bb = M.atom0 + (live ? lb : la);  // a valid atom for the loads of idle lanes
   const int ob = b.atom_ao[bb];
+  const double* ppa = b.slab + M.pp0 + b.atom_pp[a];
   const double* ppb = b.slab + M.pp0 + b.atom_pp[bb];
-  // a is the pair's first atom: the larger orbital count, on a tie the lower index
-  const bool af = NFA > NFB ? true : (NFA < NFB ? false : la < lb);
-  // P[F_p][S_q] read at a's rows (P is symmetric): in the class-sorted basis the partners of a
-  // chunk are contiguous in AO order, so these loads and the stores below are coalesced.
-  T pfs[NF][NS], acc[NF][NS];
+  // P[a_p][b_q] at a's rows: in the class-sorted basis the partners of a chunk are contiguous in
+  // AO order, so these loads and the row stores below are coalesced.
+  T pab[NFA][NFB], acc[NFA][NFB];
 #pragma unroll
-  for (int p = 0; p < NF; ++p)
+  for (int p = 0; p < NFA; ++p)
 #pragma unroll
-    for (int q = 0; q < NS; ++q) {
-      pfs[p][q] = (T)P[af ? (int64_t)(oa + p) * ld + ob + q : (int64_t)(oa + q) * ld + ob + p];
+    for (int q = 0; q < NFB; ++q) {
+      pab[p][q] = (T)P[(int64_t)(oa + p) * ld + ob + q];
       acc[p][q] = (T)0;
     }
-  T ppv[NPF > NPS ? NPF : NPS];
-  const int npb = NFB == 4 ? 10 : 1;
+  T pa[NPA], pb[NPB], ja[NPA], jb[NPB];
 #pragma unroll
-  for (int k = 0; k < (NPF > NPS ? NPF : NPS); ++k) ppv[k] = k < npb ? (T)ppb[k] : (T)0;
+  for (int k = 0; k < NPA; ++k) {
+    pa[k] = (T)ppa[k];
+    ja[k] = (T)0;
+  }
+#pragma unroll
+  for (int k = 0; k < NPB; ++k) {
+    pb[k] = (T)ppb[k];
+    jb[k] = (T)0;
+  }
   // Every index below is a compile-time constant (static_for), so the block lives in registers.
-  static_for<0, NPF>([&](auto ec) {
+  static_for<0, NPA>([&](auto ec) {
     constexpr int e = decltype(ec)::value, mu = ce_hi(e), p = e - mu * (mu + 1) / 2;
-    static_for<0, NPS>([&](auto kc) {
+    static_for<0, NPB>([&](auto kc) {
       constexpr int kl = decltype(kc)::value, l = ce_hi(kl), q = kl - l * (l + 1) / 2;
-      const T w = sp[(e * NPS + kl) * LGS_CHUNK];
-      // Coulomb: a's packed pair is e when a is F, kl when a is S
-      if constexpr (NFA > NFB) {
-        J[e] += w * ppv[kl];
-      } else if constexpr (NFA < NFB) {
-        J[kl] += w * ppv[e];
-      } else {
-        const T c = w * (af ? ppv[kl] : ppv[e]);
-        J[e] += af ? c : (T)0;
-        J[kl] += af ? (T)0 : c;
-      }
+      const T w = sp[(e * NPB + kl) * LGS_CHUNK];
+      ja[e] += w * pb[kl];
+      jb[kl] += w * pa[e];
       // exchange: every (mu' nu') x (la' sg') term this packed element stands for
-      acc[mu][l] += pfs[p][q] * w;
-      if constexpr (mu != p) acc[p][l] += pfs[mu][q] * w;
-      if constexpr (l != q) acc[mu][q] += pfs[p][l] * w;
-      if constexpr (mu != p && l != q) acc[p][q] += pfs[mu][l] * w;
+      acc[mu][l] += pab[p][q] * w;
+      if constexpr (mu != p) acc[p][l] += pab[mu][q] * w;
+      if constexpr (l != q) acc[mu][q] += pab[p][l] * w;
+      if constexpr (mu != p && l != q) acc[p][q] += pab[mu][l] * w;
     });
   });
+#pragma unroll
+  for (int k = 0; k < NPB; ++k) jc[k * LGS_CHUNK] = jb[k];  // zero on idle lanes (zero integrals)
+#pragma unroll
+  for (int k = 0; k < NPA; ++k) {
+    T v = ja[k];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);  // the same tree on every lane
+    ja[k] = v;
+  }
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < NPA; ++k) jr[k] = ja[k];
+  }
   if (!live) return;
   const double* H = b.slab + M.mat0 + (int64_t)S_H * ld * ld;
 #pragma unroll
-  for (int p = 0; p < NF; ++p)
+  for (int p = 0; p < NFA; ++p)
 #pragma unroll
-    for (int q = 0; q < NS; ++q) {
-      const int64_t at = af ? (int64_t)(oa + p) * ld + ob + q : (int64_t)(oa + q) * ld + ob + p;  // a's row
-      F[at] = H[at] - (double)((T)xa * acc[p][q]);
+    for (int q = 0; q < NFB; ++q) {
+      const int64_t at = (int64_t)(oa + p) * ld + ob + q;
+      const double f = H[at] - (double)((T)xa * acc[p][q]);
+      F[at] = f;
+      F[(int64_t)(ob + q) * ld + oa + p] = f;
     }
 }
 
-// The partner chunks of segment SEG (0: both, 1: the 4-orbital partners, 2: the 1-orbital ones).
-template <typename T, int NFA, int SEG>
-__device__ __forceinline__ void lgs_atom(const MolDev& M, const BatchDev& b, int la, int oa, const T* __restrict__ sa,
-                                         const int* __restrict__ cls, int pA, int pB, int wv, int lane,
-                                         const double* __restrict__ P, double xa, double* __restrict__ F, T (&J)[10]) {
-  constexpr int NPA = NFA == 4 ? 10 : 1;
-  constexpr int NW = LGS_THREADS / 64;
-  if constexpr (SEG != 2)
-    for (int c = wv; c < pA / LGS_CHUNK; c += NW)
-      lgs_chunk<T, NFA, 4>(M, b, la, oa, sa + (int64_t)c * NPA * 10 * LGS_CHUNK + lane, cls + c * LGS_CHUNK, lane,
-                           P, xa, F, J);
-  const T* sb = sa + (int64_t)pA * NPA * 10;
-  if constexpr (SEG != 1)
-    for (int c = wv; c < pB / LGS_CHUNK; c += NW)
-      lgs_chunk<T, NFA, 1>(M, b, la, oa, sb + (int64_t)c * NPA * LGS_CHUNK + lane, cls + pA + c * LGS_CHUNK, lane,
-                           P, xa, F, J);
+// Class-A atoms against class-A chunks: workgroup i is (group g of four A atoms, chunk c) = wg[i];
+// wavefront w takes atom u = 4g + w, so the mirrored stores of one workgroup fill 128-byte rows.
+template <typename T>
+__global__ void __launch_bounds__(256) k_lgs_fock_aa(int mi, BatchDev b, LgsDims d, const LgsAtom* __restrict__ at,
+                                                     const int* __restrict__ srt, const int2* __restrict__ wg,
+                                                     const T* __restrict__ st, const double* __restrict__ P,
+                                                     double xa, double* __restrict__ F, T* __restrict__ jcol,
+                                                     T* __restrict__ jrow) {
+  const MolDev M = b.mols[mi];
+  const int2 g = wg[blockIdx.x];
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int u = 4 * g.x + w, c = g.y;
+  if (u >= d.nA) return;
+  const LgsAtom A = at[u];
+  const int c0 = lgs_c0(A.rank);
+  if (c < c0) return;
+  const int rb = c * LGS_CHUNK + lane;
+  const bool live = rb < d.nA && rb > A.rank;
+  const int lb = live ? srt[rb] : -1;
+  lgs_item<T, 4, 4>(M, b, A.la, st + A.soff + (int64_t)(c - c0) * 100 * LGS_CHUNK + lane, lb, live, P, xa, F,
+                    jcol + A.joff + (int64_t)(c - c0) * 10 * LGS_CHUNK + lane,
+                    jrow + ((int64_t)u * d.nchA + c) * 10);
 }
 
-// Workgroup per atom a with NFA orbitals: its rows of F = H + J(P) - xa K(P) (closed shell), the
-// Coulomb sums from the packed density of every atom (k_lg_pack<T>), one-centre terms as
-// fock_diag_item. SEG 0 takes all partners. A 4-orbital atom takes two workgroups in two launches:
-// SEG 2 its 1-orbital partners, Coulomb sums into jpart[slot]; SEG 1 its 4-orbital partners, then
-// jpart[slot] + its own sums and the diagonal block.
-template <typename T, int NFA, int SEG>
-__device__ __forceinline__ void lgs_fock_atom(const MolDev& M, const BatchDev& b, const T* __restrict__ st,
-                                              const int* __restrict__ cls, int pA, int pB,
-                                              const int64_t* __restrict__ off, int la, int slot,
-                                              const double* __restrict__ P, double xa, double* __restrict__ F,
-                                              T* __restrict__ jpart, T (&red)[10][LGS_THREADS]) {
-  const int a = M.atom0 + la, ld = M.ld;
-  const ElemDev ea = b.elem[b.atom_z[a]];
-  constexpr int nfa = NFA;
-  const int oa = b.atom_ao[a];
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  T J[10];
-#pragma unroll
-  for (int e = 0; e < 10; ++e) J[e] = (T)0;
-  lgs_atom<T, NFA, SEG>(M, b, la, oa, st + off[la], cls, pA, pB, wv, lane, P, xa, F, J);
-#pragma unroll
-  for (int e = 0; e < 10; ++e) red[e][threadIdx.x] = J[e];  // every thread, in thread order
-  __syncthreads();
-  const int npa = npair_of(nfa);
-  if ((int)threadIdx.x >= npa) return;
-  const int e = threadIdx.x;
-  T acc = SEG == 1 ? jpart[slot * 10 + e] : (T)0;
-  for (int i = 0; i < LGS_THREADS; ++i) acc += red[e][i];
-  if constexpr (SEG == 2) {
-    jpart[slot * 10 + e] = acc;
-    return;
+// Class-B chunks: wg[i] = (g | kind << 24, chunk c); kind 0: the four A atoms 4g.., kind 1: the four
+// B atoms of ranks 4g... jrow: per (u, class-B chunk), 10 words.
+template <typename T>
+__global__ void __launch_bounds__(256) k_lgs_fock_xb(int mi, BatchDev b, LgsDims d, const LgsAtom* __restrict__ at,
+                                                     const int* __restrict__ srt, const int2* __restrict__ wg,
+                                                     const T* __restrict__ st, const double* __restrict__ P,
+                                                     double xa, double* __restrict__ F, T* __restrict__ jcol,
+                                                     T* __restrict__ jrow) {
+  const MolDev M = b.mols[mi];
+  const int2 g = wg[blockIdx.x];
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int kind = g.x >> 24, r = 4 * (g.x & 0xffffff) + w, c = g.y;
+  const int rb = c * LGS_CHUNK + lane;
+  if (kind == 0) {
+    if (r >= d.nA) return;
+    const LgsAtom A = at[r];
+    const int nsegA = d.nchA - lgs_c0(A.rank);
+    const bool live = rb < d.nB;
+    const int lb = live ? srt[d.nA + rb] : -1;
+    lgs_item<T, 4, 1>(M, b, A.la,
+                      st + A.soff + (int64_t)nsegA * 100 * LGS_CHUNK + (int64_t)c * 10 * LGS_CHUNK + lane, lb,
+                      live, P, xa, F, jcol + A.joff + (int64_t)nsegA * 10 * LGS_CHUNK + (int64_t)c * LGS_CHUNK + lane,
+                      jrow + ((int64_t)r * d.nchB + c) * 10);
+  } else {
+    if (r >= d.nB) return;
+    const int u = d.nA + r;
+    const LgsAtom A = at[u];
+    const int c0 = lgs_c0(r);
+    if (c < c0) return;
+    const bool live = rb < d.nB && rb > r;
+    const int lb = live ? srt[d.nA + rb] : -1;
+    lgs_item<T, 1, 1>(M, b, A.la, st + A.soff + (int64_t)(c - c0) * LGS_CHUNK + lane, lb, live, P, xa, F,
+                      jcol + A.joff + (int64_t)(c - c0) * LGS_CHUNK + lane, jrow + ((int64_t)u * d.nchB + c) * 10);
   }
+}
+
+// acc[k] += jcol entries k of the contributors i0 + w, i0 + w + 4, ... below i1 at lane `lane`;
+// off(i) is contributor i's entry offset for the chunk.
+template <int NPB, typename T, class Off>
+__device__ __forceinline__ void lgs_colsum_range(T (&acc)[10], const T* __restrict__ jcol, int i0, int i1, int w,
+                                                 int lane, Off off) {
+#pragma unroll 4
+  for (int i = i0 + w; i < i1; i += 4) {  // unrolled: several entries' offsets and loads in flight
+    const int64_t o = off(i) + lane;
+#pragma unroll
+    for (int k = 0; k < NPB; ++k) acc[k] += jcol[o + k * LGS_CHUNK];
+  }
+}
+
+// Workgroup per (partner chunk cg: class A chunks, then class B; part p): the jcol entries of the
+// chunk from the atoms of sorted index in part p of [0, nc), wavefront w taking every fourth, then
+// the four wavefronts in order: part[cg][p][k][lane].
+template <typename T>
+__global__ void __launch_bounds__(256) k_lgs_colsum(LgsDims d, const LgsAtom* __restrict__ at,
+                                                    const T* __restrict__ jcol, T* __restrict__ part) {
+  __shared__ T red[4][10][LGS_CHUNK];
+  const int cg = blockIdx.x, p = blockIdx.y;
+  const bool xa = cg < d.nchA;
+  const int C = xa ? cg : cg - d.nchA;
+  const int npb = xa ? 10 : 1;
+  // earlier atoms whose streams hold chunk C: ranks up to 64C + 62 of the chunk's class
+  const int lim = C * LGS_CHUNK + LGS_CHUNK - 1;
+  const int nc = xa ? min(d.nA, lim) : d.nA + min(d.nB, lim);
+  const int i0 = (int)((int64_t)nc * p / LGS_NPART), i1 = (int)((int64_t)nc * (p + 1) / LGS_NPART);
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  T acc[10];
+#pragma unroll
+  for (int k = 0; k < 10; ++k) acc[k] = (T)0;
+  if (xa) {
+    lgs_colsum_range<10>(acc, jcol, i0, i1, w, lane, [&](int i) {
+      return at[i].joff + (int64_t)(C - lgs_c0(i)) * 10 * LGS_CHUNK;
+    });
+  } else {
+    // the A atoms' class-B entries, then the earlier B atoms' (each range strided from its start)
+    const int mA = min(i1, d.nA);
+    lgs_colsum_range<1>(acc, jcol, i0, mA, w, lane, [&](int i) {
+      return at[i].joff + (int64_t)(d.nchA - lgs_c0(i)) * 10 * LGS_CHUNK + (int64_t)C * LGS_CHUNK;
+    });
+    lgs_colsum_range<1>(acc, jcol, max(i0, d.nA), i1, w, lane, [&](int i) {
+      return at[i].joff + (int64_t)(C - lgs_c0(i - d.nA)) * LGS_CHUNK;
+    });
+  }
+#pragma unroll
+  for (int k = 0; k < 10; ++k) red[w][k][lane] = acc[k];
+  __syncthreads();
+  if (w != 0) return;
+  for (int k = 0; k < npb; ++k)
+    part[(((int64_t)cg * LGS_NPART + p) * 10 + k) * LGS_CHUNK + lane] =
+        ((red[0][k][lane] + red[1][k][lane]) + red[2][k][lane]) + red[3][k][lane];
+}
+
+// Diagonal element e of an atom with NFA orbitals at AO oa (one-centre parameters g_ss, g_sp, g_pp,
+// g_p2, h_sp at ocp): the one-centre terms as fock_diag_item (returns xa ko, sets jo) and H, at
+// element i1 (and its mirror i2) of the block.
+template <typename T, int NFA>
+__device__ __forceinline__ T lgs_diag_oc(const MolDev& M, const BatchDev& b, int oa, const double* __restrict__ ocp,
+                                         int e, const double* __restrict__ P, double xa, double& h, int64_t& i1,
+                                         int64_t& i2, T& jo) {
+  const int ld = M.ld;
+  ElemDev ea;  // only the one-centre fields oc_int reads
+  ea.g_ss = ocp[0];
+  ea.g_sp = ocp[1];
+  ea.g_pp = ocp[2];
+  ea.g_p2 = ocp[3];
+  ea.h_sp = ocp[4];
   int mu, nu;
   unpack_e(e, mu, nu);
-  T ja = (T)0, ka = (T)0;
-  for (int l = 0; l < nfa; ++l)
-    for (int s2 = 0; s2 < nfa; ++s2) {
-      const int64_t ix = (int64_t)(oa + l) * ld + oa + s2;
-      ja += (T)P[ix] * (T)oc_int(mu, nu, l, s2, ea);
-      ka += (T)P[ix] * (T)oc_int(mu, l, nu, s2, ea);
+  double pd[NFA][NFA];
+#pragma unroll
+  for (int l = 0; l < NFA; ++l)
+#pragma unroll
+    for (int s2 = 0; s2 < NFA; ++s2) pd[l][s2] = P[(int64_t)(oa + l) * ld + oa + s2];  // all in flight
+  i1 = (int64_t)(oa + mu) * ld + oa + nu;
+  i2 = (int64_t)(oa + nu) * ld + oa + mu;
+  h = b.slab[M.mat0 + (int64_t)S_H * ld * ld + i1];
+  T ko = (T)0;
+  jo = (T)0;
+#pragma unroll
+  for (int l = 0; l < NFA; ++l)
+#pragma unroll
+    for (int s2 = 0; s2 < NFA; ++s2) {
+      jo += (T)pd[l][s2] * (T)oc_int(mu, nu, l, s2, ea);
+      ko += (T)pd[l][s2] * (T)oc_int(mu, l, nu, s2, ea);
     }
-  const double* H = b.slab + M.mat0 + (int64_t)S_H * ld * ld;
-  const int64_t i1 = (int64_t)(oa + mu) * ld + oa + nu, i2 = (int64_t)(oa + nu) * ld + oa + mu;
-  const double f = H[i1] + (double)(acc + ja - (T)xa * ka);
+  return (T)xa * ko;
+}
+
+// One-centre parameters (g_ss, g_sp, g_pp, g_p2, h_sp) of every atom by sorted index, once per SCF.
+__global__ void k_lgs_ocpar(int mi, BatchDev b, int na, const LgsAtom* __restrict__ at, double* __restrict__ ocp) {
+  const MolDev M = b.mols[mi];
+  for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < na; u += gridDim.x * blockDim.x) {
+    const ElemDev& ea = b.elem[b.atom_z[M.atom0 + at[u].la]];
+    ocp[5 * u] = ea.g_ss;
+    ocp[5 * u + 1] = ea.g_sp;
+    ocp[5 * u + 2] = ea.g_pp;
+    ocp[5 * u + 3] = ea.g_p2;
+    ocp[5 * u + 4] = ea.h_sp;
+  }
+}
+
+// Sum of the entries v[idx(c)] for c in [c_begin, c_end), in order, eight loads in flight.
+template <typename T, class Idx>
+__device__ __forceinline__ T lgs_sum_chunks(const T* __restrict__ v, int c_begin, int c_end, Idx idx) {
+  T sum = (T)0;
+  for (int c = c_begin; c < c_end; c += 8) {
+    T x[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x[j] = c + j < c_end ? v[idx(c + j)] : (T)0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) sum += x[j];
+  }
+  return sum;
+}
+
+// Wavefront per (chunk cg as in k_lgs_colsum, packed pair k), lane per atom of the chunk: the
+// atom's Coulomb sum of pair k (the parts, then its jrow entries by chunk) and its diagonal
+// element k with the one-centre terms. Workgroups of one wavefront: the element and jrow accesses
+// are one line per lane, so they are spread over as many CUs as possible.
+template <typename T>
+__global__ void __launch_bounds__(64) k_lgs_diag(int mi, BatchDev b, LgsDims d, const double* __restrict__ ocp,
+                                                 const T* __restrict__ part, const T* __restrict__ jrowA,
+                                                 const T* __restrict__ jrowB, const double* __restrict__ P,
+                                                 double xa, double* __restrict__ F) {
+  const MolDev M = b.mols[mi];
+  const int cg = blockIdx.x, k = blockIdx.y, lane = threadIdx.x;
+  const bool inA = cg < d.nchA;
+  const int r = (inA ? cg : cg - d.nchA) * LGS_CHUNK + lane;
+  if (r >= (inA ? d.nA : d.nB) || k >= (inA ? 10 : 1)) return;
+  const int u = inA ? r : d.nA + r;
+  double h;
+  int64_t i1, i2;
+  T jo;
+  // in the class-sorted basis the atom's AOs start at 4 r (class A) or 4 nA + r (class B)
+  const double* op = ocp + 5 * u;
+  const T xko = inA ? lgs_diag_oc<T, 4>(M, b, 4 * r, op, k, P, xa, h, i1, i2, jo)
+                    : lgs_diag_oc<T, 1>(M, b, 4 * d.nA + r, op, k, P, xa, h, i1, i2, jo);
+  T J = (T)0;
+  const T* pp = part + ((int64_t)cg * LGS_NPART * 10 + k) * LGS_CHUNK + lane;
+#pragma unroll
+  for (int p = 0; p < LGS_NPART; ++p) J += pp[(int64_t)p * 10 * LGS_CHUNK];
+  if (inA) J += lgs_sum_chunks(jrowA, lgs_c0(r), d.nchA, [&](int c) { return ((int64_t)u * d.nchA + c) * 10 + k; });
+  J += lgs_sum_chunks(jrowB, inA ? 0 : lgs_c0(r), d.nchB, [&](int c) { return ((int64_t)u * d.nchB + c) * 10 + k; });
+  const double f = h + (double)(J + jo - xko);
   F[i1] = f;
   F[i2] = f;
-}
-
-// The 4-orbital atoms' 4-orbital partners (SEG 1, after k_lgs_fock_light): workgroup i takes
-// atom cls[i].
-template <typename T>
-__global__ void __launch_bounds__(LGS_THREADS) k_lgs_fock_heavy(int mi, BatchDev b, const T* __restrict__ st,
-                                                        const int* __restrict__ cls, int pA, int pB,
-                                                        const int64_t* __restrict__ off,
-                                                        const double* __restrict__ P, double xa,
-                                                        double* __restrict__ F, T* __restrict__ jpart) {
-  __shared__ T red[10][LGS_THREADS];
-  const MolDev M = b.mols[mi];
-  lgs_fock_atom<T, 4, 1>(M, b, st, cls, pA, pB, off, cls[blockIdx.x], blockIdx.x, P, xa, F, jpart, red);
-}
-
-// The small blocks in one launch: workgroups [0, nA) take the 4-orbital atoms' 1-orbital partners
-// (SEG 2), the next nB the 1-orbital atoms (all partners); both need few registers and run at the
-// same occupancy.
-template <typename T>
-__global__ void __launch_bounds__(LGS_THREADS) k_lgs_fock_light(int mi, BatchDev b, const T* __restrict__ st,
-                                                        const int* __restrict__ cls, int pA, int pB, int nA,
-                                                        const int64_t* __restrict__ off,
-                                                        const double* __restrict__ P, double xa,
-                                                        double* __restrict__ F, T* __restrict__ jpart) {
-  __shared__ T red[10][LGS_THREADS];
-  const MolDev M = b.mols[mi];
-  const int i = blockIdx.x;
-  if (i < nA)
-    lgs_fock_atom<T, 4, 2>(M, b, st, cls, pA, pB, off, cls[i], i, P, xa, F, jpart, red);
-  else
-    lgs_fock_atom<T, 1, 0>(M, b, st, cls, pA, pB, off, cls[pA + i - nA], 0, P, xa, F, jpart, red);
 }
 
 // Basis permutations of n x n matrices (row stride ld): dst[i][j] = src[pi[i]][pi[j]], and for the
@@ -748,9 +896,10 @@ __global__ void k_lg_sad(int mi, BatchDev b, double* __restrict__ P, double scal
 struct LargeWork {
   rocblas_handle blas = nullptr;
   Buf ctl, part, dots, info, f32a, f32b;
-  Buf lgs_st, lgs_st32, lgs_cls, lgs_off, lgs_norb;  // per-atom integral streams (s/p closed shell)
-  Buf lgs_perm, lgs_ao;                               // class-sorted basis: permutations, AO offsets
-  Buf lgs_jpart;                                      // partial Coulomb sums of the 4-orbital atoms
+  Buf lgs_st, lgs_st32, lgs_at, lgs_srt, lgs_wg, lgs_norb;  // per-atom integral streams (s/p closed shell)
+  Buf lgs_jcol, lgs_jrow, lgs_part;                          // partial Coulomb sums
+  Buf lgs_ocp;                                               // one-centre parameters by sorted index
+  Buf lgs_perm, lgs_ao;                                      // class-sorted basis: permutations, AO offsets
   std::vector<hipEvent_t> ev;
 };
 
@@ -1023,7 +1172,8 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
   // Closed-shell s/p molecules: the per-atom integral streams (k_lgs_fock), built once here, their
   // device time counted in the Fock phase. Other molecules, and PM7_AMD_LARGE_STREAMS=0, keep the
   // element and pair kernels (k_lg_fock_diag, k_lg_fock_pairs).
-  int lgs_pA = 0, lgs_pB = 0, lgs_nA = 0, lgs_nB = 0;
+  LgsDims lgs_d{0, 0, 0, 0};
+  int lgs_n_aa = 0, lgs_n_xb = 0;  // workgroups of the class-A and class-B chunk launches
   bool lgs_on = false;
   // With the streams the SCF runs in a class-sorted basis (the 4-orbital atoms' AOs first, then the
   // 1-orbital atoms', each in atom order): bp carries that basis's AO offsets, lgs_perm its
@@ -1058,11 +1208,10 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
       else if (norb[i] == 1) light.push_back(i);
       else return hipSuccess;  // an atom with another shell count: the element and pair kernels
     }
-    const int pA = (int)((heavy.size() + LGS_CHUNK - 1) / LGS_CHUNK * LGS_CHUNK);
-    const int pB = (int)((light.size() + LGS_CHUNK - 1) / LGS_CHUNK * LGS_CHUNK);
-    std::vector<int> cls((size_t)pA + pB, -1);
-    std::copy(heavy.begin(), heavy.end(), cls.begin());
-    std::copy(light.begin(), light.end(), cls.begin() + pA);
+    const int nA = (int)heavy.size(), nB = (int)light.size();
+    const LgsDims d{nA, nB, (nA + LGS_CHUNK - 1) / LGS_CHUNK, (nB + LGS_CHUNK - 1) / LGS_CHUNK};
+    std::vector<int> srt(heavy);
+    srt.insert(srt.end(), light.begin(), light.end());
     // class-sorted basis: new AO offsets and the permutation (new -> old, old -> new)
     std::vector<int> ao_new(na), n2o(n), o2n(n);
     {
@@ -1072,57 +1221,80 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
         ao_old_of[i] = ao_old;
         ao_old += norb[i];
       }
-      const int nh = (int)heavy.size();
-      for (int r = 0; r < nh; ++r) ao_new[heavy[r]] = 4 * r;
-      for (int r = 0; r < (int)light.size(); ++r) ao_new[light[r]] = 4 * nh + r;
+      for (int r = 0; r < nA; ++r) ao_new[heavy[r]] = 4 * r;
+      for (int r = 0; r < nB; ++r) ao_new[light[r]] = 4 * nA + r;
       for (int i = 0; i < na; ++i)
         for (int k = 0; k < norb[i]; ++k) {
           n2o[ao_new[i] + k] = ao_old_of[i] + k;
           o2n[ao_old_of[i] + k] = ao_new[i] + k;
         }
     }
-    std::vector<int64_t> off(na);
-    int64_t total = 0;
-    for (int i = 0; i < na; ++i) {
-      off[i] = total;
-      const int64_t npa = norb[i] == 4 ? 10 : 1;
-      total += npa * ((int64_t)pA * 10 + pB);
+    // per sorted index: stream and jcol offsets (A: its class-A chunks of 100 words and class-B
+    // chunks of 10 per lane; B: its class-B chunks of 1)
+    std::vector<LgsAtom> at(na);
+    int64_t total = 0, jtotal = 0;
+    for (int u = 0; u < na; ++u) {
+      const bool aA = u < nA;
+      const int r = aA ? u : u - nA;
+      const int64_t segA = aA ? d.nchA - lgs_c0(r) : 0, segB = aA ? d.nchB : d.nchB - lgs_c0(r);
+      at[u] = LgsAtom{total, jtotal, srt[u], r};
+      total += (aA ? segA * 100 + segB * 10 : segB) * LGS_CHUNK;
+      jtotal += (aA ? segA * 10 + segB : segB) * LGS_CHUNK;
     }
+    // workgroups: (group of four A atoms, class-A chunk), then (group | kind << 24, class-B chunk)
+    std::vector<int2> wg;
+    for (int g = 0; 4 * g < nA; ++g)
+      for (int c = lgs_c0(4 * g); c < d.nchA; ++c) wg.push_back(make_int2(g, c));
+    const int n_aa = (int)wg.size();
+    for (int g = 0; 4 * g < nA; ++g)
+      for (int c = 0; c < d.nchB; ++c) wg.push_back(make_int2(g, c));
+    for (int g = 0; 4 * g < nB; ++g)
+      for (int c = lgs_c0(4 * g); c < d.nchB; ++c) wg.push_back(make_int2(g | 1 << 24, c));
+    const size_t jrow_words = (size_t)nA * d.nchA * 10 + (size_t)na * d.nchB * 10;
     if (lw->lgs_st.ensure((size_t)total * sizeof(double)) != hipSuccess ||
         (mixed && lw->lgs_st32.ensure((size_t)total * sizeof(float)) != hipSuccess) ||
-        lw->lgs_cls.ensure(cls.size() * sizeof(int)) != hipSuccess ||
-        lw->lgs_off.ensure(off.size() * sizeof(int64_t)) != hipSuccess ||
+        lw->lgs_at.ensure(at.size() * sizeof(LgsAtom)) != hipSuccess ||
+        lw->lgs_srt.ensure(srt.size() * sizeof(int)) != hipSuccess ||
+        lw->lgs_wg.ensure(std::max<size_t>(wg.size(), 1) * sizeof(int2)) != hipSuccess ||
+        lw->lgs_jcol.ensure((size_t)jtotal * sizeof(double)) != hipSuccess ||
+        lw->lgs_jrow.ensure(jrow_words * sizeof(double)) != hipSuccess ||
+        lw->lgs_part.ensure((size_t)(d.nchA + d.nchB) * LGS_NPART * 10 * LGS_CHUNK * sizeof(double)) !=
+            hipSuccess ||
         lw->lgs_perm.ensure((size_t)2 * n * sizeof(int)) != hipSuccess ||
-        lw->lgs_jpart.ensure((size_t)10 * std::max<size_t>(heavy.size(), 1) * sizeof(double)) != hipSuccess ||
+        lw->lgs_ocp.ensure((size_t)5 * na * sizeof(double)) != hipSuccess ||
         lw->lgs_ao.ensure((size_t)(M.atom0 + na) * sizeof(int)) != hipSuccess) {
       (void)hipGetLastError();  // no room for the streams: the element and pair kernels
       return hipSuccess;
     }
-    if ((e = hipMemcpyAsync(lw->lgs_cls.p, cls.data(), cls.size() * sizeof(int), hipMemcpyHostToDevice, s)) !=
+    if ((e = hipMemcpyAsync(lw->lgs_at.p, at.data(), at.size() * sizeof(LgsAtom), hipMemcpyHostToDevice, s)) !=
             hipSuccess ||
-        (e = hipMemcpyAsync(lw->lgs_off.p, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice, s)) !=
+        (e = hipMemcpyAsync(lw->lgs_srt.p, srt.data(), srt.size() * sizeof(int), hipMemcpyHostToDevice, s)) !=
             hipSuccess ||
+        (!wg.empty() &&
+         (e = hipMemcpyAsync(lw->lgs_wg.p, wg.data(), wg.size() * sizeof(int2), hipMemcpyHostToDevice, s)) !=
+             hipSuccess) ||
         (e = hipMemcpyAsync(lw->lgs_perm.p, n2o.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, s)) !=
             hipSuccess ||
         (e = hipMemcpyAsync(static_cast<int*>(lw->lgs_perm.p) + n, o2n.data(), (size_t)n * sizeof(int),
                             hipMemcpyHostToDevice, s)) != hipSuccess ||
         (e = hipMemcpyAsync(static_cast<int*>(lw->lgs_ao.p) + M.atom0, ao_new.data(), (size_t)na * sizeof(int),
                             hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (e = hipStreamSynchronize(s)) != hipSuccess ||  // the host vectors end with this lambda
         (e = hipEventRecord(lw->ev[0], s)) != hipSuccess)
       return e;
-    const int* dcls = static_cast<const int*>(lw->lgs_cls.p);
-    const int64_t* doff = static_cast<const int64_t*>(lw->lgs_off.p);
-    hipLaunchKernelGGL(k_lgs_build, dim3((pA + pB) / LGS_CHUNK, na), dim3(256), 0, s, mi, b, dcls, pA, pB, doff,
+    hipLaunchKernelGGL(k_lgs_build, dim3(d.nchA + d.nchB, na), dim3(256), 0, s, mi, b, d,
+                       static_cast<const LgsAtom*>(lw->lgs_at.p), static_cast<const int*>(lw->lgs_srt.p),
                        static_cast<double*>(lw->lgs_st.p), mixed ? static_cast<float*>(lw->lgs_st32.p) : nullptr);
+    hipLaunchKernelGGL(k_lgs_ocpar, dim3(grid_for(na)), dim3(256), 0, s, mi, b, na,
+                       static_cast<const LgsAtom*>(lw->lgs_at.p), static_cast<double*>(lw->lgs_ocp.p));
     if ((e = hipGetLastError()) != hipSuccess || (e = hipEventRecord(lw->ev[1], s)) != hipSuccess ||
         (e = hipEventSynchronize(lw->ev[1])) != hipSuccess)
       return e;
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, lw->ev[0], lw->ev[1]) == hipSuccess) phase[0] += ms;
-    lgs_pA = pA;
-    lgs_pB = pB;
-    lgs_nA = (int)heavy.size();
-    lgs_nB = (int)light.size();
+    lgs_d = d;
+    lgs_n_aa = n_aa;
+    lgs_n_xb = (int)wg.size() - n_aa;
     lgs_on = true;
     bp.atom_ao = static_cast<const int*>(lw->lgs_ao.p);
     lgs_n2o = static_cast<const int*>(lw->lgs_perm.p);
@@ -1136,23 +1308,33 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
     const BatchDev& b = bp;  // the basis the SCF runs in
     if (lowp) hipLaunchKernelGGL(k_lg_pack<float>, dim3(grid_for(M.n_pp)), dim3(256), 0, s, mi, b, P);
     else hipLaunchKernelGGL(k_lg_pack<double>, dim3(grid_for(M.n_pp)), dim3(256), 0, s, mi, b, P);
-    const int* dcls = static_cast<const int*>(lw->lgs_cls.p);
-    const int64_t* doff = static_cast<const int64_t*>(lw->lgs_off.p);
     auto streamed = [&](auto* st) {
       using T = std::remove_const_t<std::remove_pointer_t<decltype(st)>>;
-      T* jp = static_cast<T*>(lw->lgs_jpart.p);
-      if (lgs_nA + lgs_nB > 0)
-        hipLaunchKernelGGL((k_lgs_fock_light<T>), dim3(lgs_nA + lgs_nB), dim3(LGS_THREADS), 0, s, mi, b, st, dcls,
-                           lgs_pA, lgs_pB, lgs_nA, doff, P, 0.5, F, jp);
-      if (lgs_nA > 0)
-        hipLaunchKernelGGL((k_lgs_fock_heavy<T>), dim3(lgs_nA), dim3(LGS_THREADS), 0, s, mi, b, st, dcls, lgs_pA,
-                           lgs_pB, doff, P, 0.5, F, jp);
+      const LgsDims& d = lgs_d;
+      const auto* at = static_cast<const LgsAtom*>(lw->lgs_at.p);
+      const int* srt = static_cast<const int*>(lw->lgs_srt.p);
+      const int2* wg = static_cast<const int2*>(lw->lgs_wg.p);
+      T* jcol = static_cast<T*>(lw->lgs_jcol.p);
+      T* jrowA = static_cast<T*>(lw->lgs_jrow.p);
+      T* jrowB = jrowA + (size_t)d.nA * d.nchA * 10;
+      T* part = static_cast<T*>(lw->lgs_part.p);
+      const double* ocp = static_cast<const double*>(lw->lgs_ocp.p);
+      if (lgs_n_aa > 0)
+        hipLaunchKernelGGL((k_lgs_fock_aa<T>), dim3(lgs_n_aa), dim3(256), 0, s, mi, b, d, at, srt, wg, st, P, 0.5, F,
+                           jcol, jrowA);
+      if (lgs_n_xb > 0)
+        hipLaunchKernelGGL((k_lgs_fock_xb<T>), dim3(lgs_n_xb), dim3(256), 0, s, mi, b, d, at, srt, wg + lgs_n_aa, st,
+                           P, 0.5, F, jcol, jrowB);
+      hipLaunchKernelGGL((k_lgs_colsum<T>), dim3(d.nchA + d.nchB, LGS_NPART), dim3(256), 0, s, d, at, jcol, part);
+      hipLaunchKernelGGL((k_lgs_diag<T>), dim3(d.nchA + d.nchB, 10), dim3(64), 0, s, mi, b, d, ocp, part, jrowA, jrowB, P,
+                         0.5, F);
+      return hipSuccess;
     };
-    if (lgs_on && lowp)
-      streamed(static_cast<const float*>(lw->lgs_st32.p));
-    else if (lgs_on)
-      streamed(static_cast<const double*>(lw->lgs_st.p));
-    else if (lowp) {
+    if (lgs_on) {
+      const hipError_t e = lowp ? streamed(static_cast<const float*>(lw->lgs_st32.p))
+                                : streamed(static_cast<const double*>(lw->lgs_st.p));
+      if (e != hipSuccess) return e;
+    } else if (lowp) {
       hipLaunchKernelGGL(k_lg_fock_diag<float>, dim3(grid_for((int64_t)M.n_pp * 64)), dim3(256), 0, s, mi, b, P, P,
                          0.5, nullptr, 0.0, F, nullptr);
       hipLaunchKernelGGL(k_lg_fock_pairs<float>, dim3(grid_for((int64_t)M.n_atoms * M.n_atoms / 2)), dim3(256), 0, s,

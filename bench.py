@@ -35,7 +35,8 @@ SCF_KERNELS = ["pm7::k_scf<256, 64, true, true>", "pm7::k_scf<512, 120, false, t
                "pm7::k_scf<512, 196, true, false>", "pm7::k_scf<512, 1024, false, false>"]
 TRAFFIC_FILES = {"fp64": ROOT / "profiles" / "hbm_traffic_organic256_fp64.json",
                  "mixed": ROOT / "profiles" / "hbm_traffic_organic256_mixed.json",
-                 "response": ROOT / "profiles" / "hbm_traffic_response_c32h66.json"}
+                 "response": ROOT / "profiles" / "hbm_traffic_response_c32h66.json",
+                 "large": ROOT / "profiles" / "hbm_traffic_large_water500.json"}
 # The closed-shell response path of an s/p molecule, in launch order.
 RESPONSE_KERNELS = ["pm7::k_resp_wall", "pm7::k_resp_pack", "pm7::k_resp_gemm", "pm7::k_resp_pairs<false>"]
 
@@ -156,6 +157,9 @@ def main() -> int:
     ap.add_argument("--large-waters", type=int, default=500,
                     help="water cluster of the large-system run (0: skip); 500 waters = 3000 AOs")
     ap.add_argument("--large-only", action="store_true", help="only the large-system run")
+    ap.add_argument("--large-warmup", type=int, default=1, help="untimed smaller large-path runs first (0: none)")
+    ap.add_argument("--large-max-scf", type=int, default=200,
+                    help="iteration cap of the large-system runs (a few for counter profiles)")
     ap.add_argument("--response-only", action="store_true", help="only the response-Fock run")
     ap.add_argument("--serving-only", action="store_true", help="only the serving run")
     ap.add_argument("--d-batch", type=int, default=64,
@@ -233,12 +237,18 @@ def main() -> int:
         out = {"workload": f"({args.large_waters} H2O) cluster, 1.0 g/cm3, O-O >= 2.6 A, seed 4",
                "atoms": len(atoms), "nao": 6 * args.large_waters}
         ref = None
+        # warm-up: a 40-water cluster (240 AOs, the same path) in each precision loads every kernel
+        # and library code object the timed runs use (lazy loading would otherwise land in the
+        # first run's phases)
+        warm = pm7amd.Molecule.from_symbols_angstrom(W.water_cluster(40, 4))
+        for precision in ("fp64", "mixed") if args.large_warmup else ():
+            eng.run(warm, pm7amd.Pm7Options(precision=precision), raise_on_error=False)
         # "mixed" hands over to FP64 at a density change of 1e-5 (the configuration's); the engine's
         # default hand-over (1e-3) is run beside it
         for precision, switch in (("fp64", 1e-3), ("mixed", 1e-5), ("mixed_default_switch", 1e-3)):
             s0 = time.perf_counter()
-            r = eng.run(mol, pm7amd.Pm7Options(precision=precision.split("_")[0], mixed_switch=switch),
-                        raise_on_error=False)
+            r = eng.run(mol, pm7amd.Pm7Options(precision=precision.split("_")[0], mixed_switch=switch,
+                                               max_scf=args.large_max_scf), raise_on_error=False)
             wall = time.perf_counter() - s0
             ph = dict(zip(["fock", "energy_commutator", "diis", "eigensolve", "density", "total"],
                           [round(x, 2) for x in r.device_phase_ms]))
@@ -261,6 +271,16 @@ def main() -> int:
             gbs = b_iter * r.iterations / fock_s / 1e9 if fock_s > 0 else 0.0
             rec["fock_gp"] = {"bytes_per_iteration": b_iter, "ms_per_iteration": round(r.device_phase_ms[0] / max(1, r.iterations), 3),
                               "achieved_gbs": round(gbs, 1), "peak_gbs": 8000.0, "frac": round(gbs / 8000.0, 4)}
+            # measured HBM bytes of one FP64 and one FP32 Fock build (rocprofv3 FETCH_SIZE/WRITE_SIZE
+            # of the streamed kernels, committed profile of this workload)
+            try:
+                fb = json.loads(TRAFFIC_FILES["large"].read_text())["fock_build"]
+                if args.large_waters == 500:
+                    rec["fock_gp"]["traffic_fp64"] = fb["double"]["bytes"]
+                    rec["fock_gp"]["traffic_fp32"] = fb["float"]["bytes"]
+                    rec["fock_gp"]["traffic_ratio_fp64"] = round(fb["double"]["bytes"] / b_iter, 3)
+            except (OSError, KeyError, ValueError):
+                pass
             if precision == "fp64":
                 ref = r
             else:

@@ -535,7 +535,8 @@ __global__ void k_lgs_norb(int mi, BatchDev b, int* __restrict__ norb) {
 __global__ void __launch_bounds__(256) k_lgs_build(int mi, BatchDev b, LgsDims d, const LgsAtom* __restrict__ at,
                                                    const int* __restrict__ srt, double* __restrict__ st,
                                                    float* __restrict__ st32) {
-  __shared__ double tile[100 * LGS_CHUNK];
+  constexpr int TS = LGS_CHUNK + 1;  // padded row: the transposing writes spread over the LDS banks
+  __shared__ double tile[100 * TS];
   __shared__ int64_t s_woff[LGS_CHUNK];
   const MolDev M = b.mols[mi];
   const int u = blockIdx.y, j = blockIdx.x, na = M.n_atoms;
@@ -564,12 +565,13 @@ __global__ void __launch_bounds__(256) k_lgs_build(int mi, BatchDev b, LgsDims d
   for (int i = threadIdx.x; i < K * LGS_CHUNK; i += 256) {  // k fastest: each block read contiguously
     const int t = i / K, k = i - t * K;
     const int64_t wo = s_woff[t];
-    tile[k * LGS_CHUNK + t] = wo >= 0 ? b.w[wo + k] : 0.0;
+    tile[k * TS + t] = wo >= 0 ? b.w[wo + k] : 0.0;
   }
   __syncthreads();
   for (int i = threadIdx.x; i < K * LGS_CHUNK; i += 256) {  // element-major, coalesced
-    st[base + i] = tile[i];
-    if (st32) st32[base + i] = (float)tile[i];
+    const double v = tile[(i / LGS_CHUNK) * TS + i % LGS_CHUNK];
+    st[base + i] = v;
+    if (st32) st32[base + i] = (float)v;
   }
 }
 

@@ -901,6 +901,7 @@ struct LargeWork {
   Buf lgs_st, lgs_st32, lgs_at, lgs_srt, lgs_wg, lgs_norb;  // per-atom integral streams (s/p closed shell)
   Buf lgs_jcol, lgs_jrow, lgs_part;                          // partial Coulomb sums
   Buf lgs_ocp;                                               // one-centre parameters by sorted index
+  LargeCtl* flag_host = nullptr;  // pinned: the stop flag's copy stays asynchronous
   Buf lgs_perm, lgs_ao;                                      // class-sorted basis: permutations, AO offsets
   std::vector<hipEvent_t> ev;
 };
@@ -911,12 +912,17 @@ LargeWork* large_work_create() {
     delete w;
     return nullptr;
   }
-  w->ev.resize(8);
+  w->ev.resize(10);
   for (auto& e : w->ev)
     if (hipEventCreate(&e) != hipSuccess) {
       large_work_destroy(w);
       return nullptr;
     }
+  if (hipHostMalloc(reinterpret_cast<void**>(&w->flag_host), sizeof(LargeCtl), hipHostMallocDefault) != hipSuccess) {
+    w->flag_host = nullptr;
+    large_work_destroy(w);
+    return nullptr;
+  }
   return w;
 }
 
@@ -925,6 +931,7 @@ void large_work_destroy(LargeWork* w) {
   if (w->blas) rocblas_destroy_handle(w->blas);
   for (auto& e : w->ev)
     if (e) (void)hipEventDestroy(e);
+  if (w->flag_host) (void)hipHostFree(w->flag_host);
   delete w;
 }
 
@@ -1359,15 +1366,26 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
     const int nocc = M.n_occ;
     LG(lgs_setup());
     if (!M.init_p) hipLaunchKernelGGL(k_lg_sad, dim3(grid_for(M.n_atoms)), dim3(256), 0, s, mi, bp, slot(S_HD), 1.0);
+    // Each iteration's Fock build is enqueued at the end of the previous one, before the host waits
+    // for the stop flag, so the device is not idle while the host wakes up: it is discarded when the
+    // loop stops (it writes the next DIIS slot only) and redone when the precision switches.
+    // Its timing events alternate between two pairs (ev 0/1 and 6/7).
+    bool pre = false;   // this iteration's build is already enqueued
+    int pre_low = 0;    // in that precision
     for (; iter < M.max_scf; ++iter) {
       const int h = iter % HIST;
       const int cnt = std::min(iter + 1, HIST);
       double* P = slot(S_HD + h);
       double* F = slot(S_HF + h);
       double* E = slot(S_HE + h);
-      LG(hipEventRecord(lw->ev[0], s));
-      LG(fock_rhf(P, F, low != 0));
-      LG(hipEventRecord(lw->ev[1], s));
+      hipEvent_t fs = (iter & 1) ? lw->ev[6] : lw->ev[0], fe = (iter & 1) ? lw->ev[7] : lw->ev[1];
+      if (!pre || pre_low != (low != 0)) {
+        LG(hipEventRecord(fs, s));
+        LG(fock_rhf(P, F, low != 0));
+        LG(hipEventRecord(fe, s));
+      }
+      pre = false;
+      LG(hipEventRecord(lw->ev[9], s));
       // energy and commutator
       LG(dots(2, P, slot(S_H), P, F, nullptr, nullptr, d3));
       hipLaunchKernelGGL(k_lg_energy, dim3(1), dim3(64), 0, s, ctl, d3, 0);
@@ -1405,10 +1423,23 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
       hipLaunchKernelGGL(k_lg_ctrl_b, dim3(1), dim3(64), 0, s, ctl, iter, nn, M.e_tol, M.p_tol, tail, M.mixed_switch,
                          0);
       LG(hipEventRecord(lw->ev[5], s));
-      LG(hipMemcpyAsync(&host, ctl, sizeof host, hipMemcpyDeviceToHost, s));
-      LG(hipStreamSynchronize(s));
+      LG(hipMemcpyAsync(lw->flag_host, ctl, sizeof host, hipMemcpyDeviceToHost, s));
+      LG(hipEventRecord(lw->ev[8], s));
+      if (iter + 1 < M.max_scf) {  // the next iteration's Fock build, in the current precision
+        const int hn = (iter + 1) % HIST;
+        hipEvent_t fsn = (iter & 1) ? lw->ev[0] : lw->ev[6], fen = (iter & 1) ? lw->ev[1] : lw->ev[7];
+        LG(hipEventRecord(fsn, s));
+        LG(fock_rhf(slot(S_HD + hn), slot(S_HF + hn), low != 0));
+        LG(hipEventRecord(fen, s));
+        pre = true;
+        pre_low = low != 0;
+      }
+      LG(hipEventSynchronize(lw->ev[8]));
+      host = *lw->flag_host;
       float ms;
-      for (int k = 0; k < 5; ++k)
+      if (hipEventElapsedTime(&ms, fs, fe) == hipSuccess) phase[0] += ms;
+      if (hipEventElapsedTime(&ms, lw->ev[9], lw->ev[2]) == hipSuccess) phase[1] += ms;
+      for (int k = 2; k < 5; ++k)
         if (hipEventElapsedTime(&ms, lw->ev[k], lw->ev[k + 1]) == hipSuccess) phase[k] += ms;
       (void)hipGetLastError();  // a failed timing query must not become the next launch's error
       low = host.low;

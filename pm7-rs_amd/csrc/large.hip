@@ -1369,7 +1369,8 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
     // Each iteration's Fock build is enqueued at the end of the previous one, before the host waits
     // for the stop flag, so the device is not idle while the host wakes up: it is discarded when the
     // loop stops (it writes the next DIIS slot only) and redone when the precision switches.
-    // Its timing events alternate between two pairs (ev 0/1 and 6/7).
+    // Its timing events alternate between two pairs (ev 0/1 and 6/7). SCFs of a lockstep group
+    // (eb set) keep the plain order.
     bool pre = false;   // this iteration's build is already enqueued
     int pre_low = 0;    // in that precision
     for (; iter < M.max_scf; ++iter) {
@@ -1425,7 +1426,7 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
       LG(hipEventRecord(lw->ev[5], s));
       LG(hipMemcpyAsync(lw->flag_host, ctl, sizeof host, hipMemcpyDeviceToHost, s));
       LG(hipEventRecord(lw->ev[8], s));
-      if (iter + 1 < M.max_scf) {  // the next iteration's Fock build, in the current precision
+      if (!eb && iter + 1 < M.max_scf) {  // the next iteration's Fock build, in the current precision
         const int hn = (iter + 1) % HIST;
         hipEvent_t fsn = (iter & 1) ? lw->ev[0] : lw->ev[6], fen = (iter & 1) ? lw->ev[1] : lw->ev[7];
         LG(hipEventRecord(fsn, s));

@@ -1178,7 +1178,8 @@ hipError_t large_scf(LargeWork* lw, const BatchDev& b, const MolDev& M, int mi, 
   const bool mixed_ok = mixed;  // the FP32 range guard of the megakernel is applied below
   hipLaunchKernelGGL(k_lg_init, dim3(1), dim3(64), 0, s, ctl, M.shift_floor, mixed_ok ? 1 : 0);
   std::vector<float> phase(5, 0.0f);
-  // Closed-shell s/p molecules: the per-atom integral streams (k_lgs_fock), built once here, their
+  // Closed-shell s/p molecules: the per-atom integral streams (k_lgs_fock_aa, k_lgs_fock_xb,
+  // k_lgs_colsum, k_lgs_diag), built once here, their
   // device time counted in the Fock phase. Other molecules, and PM7_AMD_LARGE_STREAMS=0, keep the
   // element and pair kernels (k_lg_fock_diag, k_lg_fock_pairs).
   LgsDims lgs_d{0, 0, 0, 0};
